@@ -1,0 +1,149 @@
+/*
+ * spk.h -- C-ABI of libspk.so, the MI355X (gfx950) spectral-packing library.
+ *
+ * Plain C types only: caller-owned host buffers, int status codes, an opaque
+ * context per device.  Every entry point returns SPK_OK (0) or an error code;
+ * spk_last_error() gives the message of the last failure on the calling
+ * thread.  Grids are int32, row-major, index (i*ny + j)*nz + k, exactly the
+ * layout of the reference FlatVoxelGrid (spectral_packing/types.h:24-62).
+ *
+ * Reference interfaces replaced (Vrroom/psacking, spectral_packing/):
+ *   spk_tray_set          GPUTrayContext::initialize + gpu_tray_context_init
+ *                         (fft3.cu:1106-1175, bindings.cpp:400-405) and
+ *                         calculate_distance (fft3.cu:951-1004)
+ *   spk_search            fft_search_placement (packing.cpp:590-642),
+ *                         fft_search_with_gpu_context (fft3.cu:1506-1538),
+ *                         fft_search_batch[_interlocking_free]
+ *                         (fft3.cu:1549-1632)
+ *   spk_search_oriented   the Python orientation loop packer.py:487-505 with
+ *                         rotations.py:159-268 done as index maps on device
+ *   spk_place             place_in_tray (packing.cpp:773-788)
+ *   spk_pack              BinPacker.pack_voxels greedy loop (packer.py:447-529)
+ *   spk_distance          calculate_distance (fft3.cu:944-1004)
+ *   spk_correlate         dft_conv3 / dft_corr3 (fft3.cu:643-846,
+ *                         packing.cpp:225-239)
+ *   spk_collision_grid    collision_grid (packing.cpp:241-251)
+ *   spk_interlock_mask    gpu_interlocking_free_positions (fft3.cu:348-402)
+ *                         behind interlocking_free_positions (bindings.cpp:561-648)
+ *   spk_bounds            get_voxel_grid_bounds_flat (voxelGrid.cpp:279-299)
+ *   spk_save_vox          saveAsVox (packing.cpp:275-305)
+ */
+#ifndef SPK_H
+#define SPK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct spk_ctx spk_ctx;
+
+enum {
+  SPK_OK = 0,
+  SPK_EINVAL = 1,       /* bad argument (shape, pointer, flag)            */
+  SPK_EDEVICE = 2,      /* HIP runtime / kernel failure                   */
+  SPK_ENOMEM = 3,       /* device allocation failed                       */
+  SPK_EOVERLAP = 4,     /* place: item intersects occupied tray voxels     */
+  SPK_ESTATE = 5,       /* context has no tray                            */
+  SPK_EUNSUPPORTED = 6  /* size or value range outside what is built       */
+};
+
+enum { SPK_METRIC_L1 = 0, SPK_METRIC_EDT = 1 };          /* distance metric  */
+enum { SPK_SOURCES_EQ1 = 0, SPK_SOURCES_NONZERO = 1 };   /* distance sources */
+enum { SPK_PENALTY_POW = 0, SPK_PENALTY_CUBE = 1 };      /* height penalty   */
+
+/* An axis-aligned orientation as an index map: output axis a runs along base
+ * axis perm[a], reversed when flip[a] != 0. */
+typedef struct {
+  int32_t perm[3];
+  int32_t flip[3];
+} spk_orient;
+
+typedef struct {
+  int32_t pos[3];      /* (x,y,z) offset, (-1,-1,-1) when not found         */
+  int32_t found;       /* 1 if a feasible offset exists                     */
+  int32_t orient;      /* index into the orientation list, -1 if none      */
+  int32_t path;        /* 0 = FFT path, 1 = exact direct-sum path          */
+  double score;        /* proximity + height penalty, 0.0 when not found   */
+  int64_t collision;   /* exact overlap count at pos (0 when found)         */
+  int64_t proximity;   /* exact proximity sum at pos                        */
+  float residual;      /* max distance of an FFT output to its integer      */
+  int32_t reserved;
+} spk_result;
+
+const char* spk_last_error(void);
+int spk_version(int* major, int* minor, int* patch);
+
+int spk_ctx_create(int device, spk_ctx** out);
+int spk_ctx_destroy(spk_ctx* ctx);
+
+/* Upload a tray, compute (or take, when dist != NULL) its distance field and
+ * build the cached spectra.  generation is stored for cache keying. */
+int spk_tray_set(spk_ctx* ctx, const int32_t* tray, int nx, int ny, int nz,
+                 const int32_t* dist, int metric, int sources, uint64_t generation);
+int spk_tray_get(spk_ctx* ctx, int32_t* tray_out, int32_t* dist_out);
+int spk_tray_info(spk_ctx* ctx, int* nx, int* ny, int* nz, uint64_t* generation,
+                  int* valid);
+int spk_tray_clear(spk_ctx* ctx);
+
+/* Search n explicit item grids (one per orientation) against the cached tray.
+ * items[o] has dims[3*o .. 3*o+2].  best gets the winner (strictly smaller
+ * score wins, list order), per_orient (nullable) every orientation's best. */
+int spk_search(spk_ctx* ctx, const int32_t* const* items, const int32_t* dims, int n,
+               int penalty, int interlocking_free, spk_result* best,
+               spk_result* per_orient);
+
+/* Same, for one base item and n orientation index maps applied on device. */
+int spk_search_oriented(spk_ctx* ctx, const int32_t* item, int ix, int iy, int iz,
+                        const spk_orient* orients, int n, int penalty,
+                        int interlocking_free, spk_result* best, spk_result* per_orient);
+
+/* Write id into the cached tray where the (oriented) item is > 0 and the tray
+ * is 0, then refresh the distance field and spectra.  SPK_EOVERLAP (tray left
+ * unchanged) if an item voxel lands on an occupied voxel. */
+int spk_place(spk_ctx* ctx, const int32_t* item, int ix, int iy, int iz,
+              const spk_orient* orient, int x, int y, int z, int32_t id);
+
+/* Greedy loop over n_items items (concatenated in items, dims n_items*3), in
+ * the given order, each trying n_orient orientations; results[t] is the
+ * outcome of the t-th processed item.  The tray must have been set. */
+int spk_pack(spk_ctx* ctx, const int32_t* items, const int32_t* dims, int n_items,
+             const int32_t* order, const spk_orient* orients, int n_orient,
+             int penalty, int interlocking_free, int32_t id_offset,
+             spk_result* results);
+
+/* Standalone full-grid operations (host in, host out). */
+int spk_distance(spk_ctx* ctx, const int32_t* grid, int nx, int ny, int nz, int metric,
+                 int sources, int32_t* out);
+int spk_correlate(spk_ctx* ctx, const int32_t* a, const int32_t* b, int nx, int ny,
+                  int nz, int convolve, int32_t* out);
+int spk_collision_grid(spk_ctx* ctx, const int32_t* tray, int nx, int ny, int nz,
+                       const int32_t* item, int ix, int iy, int iz, int32_t* out);
+int spk_interlock_mask(spk_ctx* ctx, const int32_t* tray, int nx, int ny, int nz,
+                       const int32_t* obj, int ox, int oy, int oz, int32_t* out);
+
+/* place_in_tray on caller grids (packing.cpp:773-788 via bindings.cpp:259-276):
+ * out = tray with id written where item > 0 and tray == 0; SPK_EOVERLAP (out
+ * untouched) if an item > 0 voxel meets a tray > 0 voxel.  Item parts outside
+ * the tray are ignored. */
+int spk_place_grid(spk_ctx* ctx, const int32_t* tray, int nx, int ny, int nz,
+                   const int32_t* item, int ix, int iy, int iz, int x, int y, int z,
+                   int32_t id, int32_t* out);
+
+/* Host grid utilities. */
+int spk_bounds(const int32_t* grid, int nx, int ny, int nz, int32_t lo[3], int32_t hi[3]);
+int spk_save_vox(const int32_t* grid, int nx, int ny, int nz, const char* path);
+
+/* Per-kernel HIP-event timing on the context's stream (off by default). */
+int spk_timing_enable(spk_ctx* ctx, int on);
+int spk_timing_get(spk_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
+int spk_timing_reset(spk_ctx* ctx);
+/* Plan actually used for the cached tray: FFT sizes and digit base. */
+int spk_plan_info(spk_ctx* ctx, int* px, int* py, int* pz, int* digit_base);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPK_H */

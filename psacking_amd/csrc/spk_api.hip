@@ -1,0 +1,1166 @@
+// spk_api.hip -- host side of libspk.so: context, FFT planning, launches,
+// exact fallback, and the extern "C" entry points declared in include/spk.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/spk.h"
+#include "spk_kernels.hpp"
+
+using namespace spk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+struct HipError {
+  int code;
+  std::string msg;
+};
+
+#define HIP_OK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      throw HipError{e_ == hipErrorOutOfMemory ? SPK_ENOMEM : SPK_EDEVICE,        \
+                     std::string(#expr) + ": " + hipGetErrorString(e_)};          \
+  } while (0)
+
+struct SpkError {
+  int code;
+  std::string msg;
+};
+
+// smooth sizes whose factors are all supported radices
+bool smooth(int n) {
+  for (int r : {2, 3, 5, 7})
+    while (n % r == 0) n /= r;
+  return n == 1;
+}
+
+AxisPlan make_plan(int n) {
+  AxisPlan p{};
+  p.n = n;
+  p.nst = 0;
+  int m = n;
+  int a = 0;
+  while (m % 2 == 0) { m /= 2; ++a; }
+  std::vector<int> rs;
+  int q = a / 3, r = a % 3;
+  if (r == 1 && q >= 1) { q -= 1; rs.push_back(4); rs.push_back(4); }
+  else if (r == 1) rs.push_back(2);
+  else if (r == 2) rs.push_back(4);
+  for (int i = 0; i < q; ++i) rs.push_back(8);
+  while (m % 5 == 0) { m /= 5; rs.push_back(5); }
+  while (m % 3 == 0) { m /= 3; rs.push_back(3); }
+  while (m % 7 == 0) { m /= 7; rs.push_back(7); }
+  // larger radices first
+  std::sort(rs.begin(), rs.end(), [](int x, int y) { return x > y; });
+  p.rad = 0;
+  for (int v : rs) p.rad |= (unsigned long long)v << (4 * p.nst++);
+  return p;
+}
+
+int next_smooth(int n) {
+  int m = std::max(n, 1);
+  while (!smooth(m)) ++m;
+  return m;
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  void ensure(size_t n) {
+    if (n <= cap) return;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    HIP_OK(hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)));
+    cap = n;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct Timed {
+  std::string name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct spk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // tray
+  bool valid = false;
+  int nx = 0, ny = 0, nz = 0;
+  uint64_t generation = 0;
+  int metric = SPK_METRIC_L1, sources = SPK_SOURCES_EQ1;
+  AxisPlan px{}, py{}, pz{};
+  int T = 16;
+  bool lds_attrs = false;
+  DevBuf<int> tray, phi, tmpi;
+  DevBuf<FieldStats> stats;
+  DevBuf<float2> tw, t1, t2, g1, g2;
+  DevBuf<double> pen;          // [2][nz]: pow, cube
+  // search workspace
+  DevBuf<int> items;
+  DevBuf<OrientDesc> od;
+  DevBuf<float2> ispec, y1, y2;
+  DevBuf<WgBest> rec, per;
+  DevBuf<SearchOut> sout;
+  DevBuf<long long> ver;
+  DevBuf<int> flag;
+  DevBuf<Tap> taps;
+  DevBuf<long long> big;       // standalone full-grid outputs
+  DevBuf<unsigned char> ffree, freach;   // IF: feasibility / reached grids [m][V]
+  DevBuf<long long> pgrid;               // IF: proximity grids [m][V]
+  DevBuf<int> exits;                     // IF: high seed faces [m][3]
+  // timing
+  bool timing = false;
+  std::vector<Timed> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long long>> acc;
+  // host-side copy of item buffer bookkeeping
+  std::vector<int> h_items;
+
+  hipEvent_t ev() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    return e;
+  }
+  void flush_timing() {
+    for (auto& t : pending) {
+      HIP_OK(hipEventSynchronize(t.b));
+      float ms = 0.f;
+      HIP_OK(hipEventElapsedTime(&ms, t.a, t.b));
+      auto& a = acc[t.name];
+      a.first += ms;
+      a.second += 1;
+      pool.push_back(t.a);
+      pool.push_back(t.b);
+    }
+    pending.clear();
+  }
+};
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    hipGetDevice(&prev);
+    if (prev != dev) hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    hipGetDevice(&cur);
+    if (prev >= 0 && cur != prev) hipSetDevice(prev);
+  }
+};
+
+template <typename F>
+int guarded(spk_ctx* ctx, F&& f) {
+  if (!ctx) return fail(SPK_EINVAL, "null context");
+  try {
+    DeviceGuard g(ctx->device);
+    return f();
+  } catch (const HipError& e) {
+    return fail(e.code, e.msg);
+  } catch (const SpkError& e) {
+    return fail(e.code, e.msg);
+  } catch (const std::bad_alloc&) {
+    return fail(SPK_ENOMEM, "host allocation failed");
+  }
+}
+
+// launch with optional event timing
+template <typename K, typename... Args>
+void launch(spk_ctx* c, const char* name, K kernel, dim3 grid, dim3 block, size_t shmem,
+            Args... args) {
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    a = c->ev();
+    b = c->ev();
+    HIP_OK(hipEventRecord(a, c->stream));
+  }
+  hipLaunchKernelGGL(kernel, grid, block, shmem, c->stream, args...);
+  HIP_OK(hipGetLastError());
+  if (c->timing) {
+    HIP_OK(hipEventRecord(b, c->stream));
+    c->pending.push_back({name, a, b});
+  }
+}
+
+template <typename K>
+void allow_shmem(K kernel, size_t bytes) {
+  if (bytes > 65536)
+    HIP_OK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes));
+}
+
+void sync(spk_ctx* c) { HIP_OK(hipStreamSynchronize(c->stream)); }
+
+// Host sources are often temporaries (pageable memory), so an upload returns
+// only once the copy has been consumed.
+void upload(spk_ctx* c, void* dst, const void* src, size_t bytes) {
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+}
+void download(spk_ctx* c, void* dst, const void* src, size_t bytes) {
+  HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+}
+
+void check_dims(int nx, int ny, int nz) {
+  if (nx <= 0 || ny <= 0 || nz <= 0)
+    throw SpkError{SPK_EINVAL, "grid dimensions must be positive"};
+}
+
+size_t slab_shmem(const spk_ctx* c) {
+  return (size_t)(c->py.n + c->pz.n + c->py.n * c->pz.n) * sizeof(float2);
+}
+size_t pencil_shmem(const spk_ctx* c) {
+  return (size_t)(c->px.n + c->px.n * c->T) * sizeof(float2);
+}
+
+// ---- FFT planning for a tray --------------------------------------------
+void plan_tray(spk_ctx* c) {
+  const int px = next_smooth(c->nx), pz = next_smooth(c->nz);
+  int py = next_smooth(c->ny);
+  if (px > kMaxAxis || py > kMaxAxis || pz > kMaxAxis)
+    throw SpkError{SPK_EUNSUPPORTED, "tray axis longer than 1024 is not supported"};
+  if ((long long)py * pz > kSlabMax)
+    throw SpkError{SPK_EUNSUPPORTED,
+                   "tray y*z plane larger than 16384 FFT points is not supported by the "
+                   "two-pass plan"};
+  if ((long long)c->ny * c->nz > 32768)
+    throw SpkError{SPK_EUNSUPPORTED, "tray y*z plane larger than 32768 voxels"};
+  c->px = make_plan(px);
+  c->py = make_plan(py);
+  c->pz = make_plan(pz);
+  int T = 16;
+  while (T > 1 && px * T > kPencilMax) T >>= 1;
+  c->T = T;
+  // twiddle tables exp(-2 pi i q / n), double -> float
+  std::vector<float2> tw(px + py + pz);
+  auto fill = [&](int off, int n) {
+    for (int q = 0; q < n; ++q) {
+      const double a = -2.0 * M_PI * (double)q / (double)n;
+      tw[off + q] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+  };
+  fill(0, px);
+  fill(px, py);
+  fill(px + py, pz);
+  c->tw.ensure(tw.size());
+  upload(c, c->tw.p, tw.data(), tw.size() * sizeof(float2));
+  // penalty tables: packing.cpp:630-631 (pow) and fft3.cu:173 (cube)
+  std::vector<double> pen(2 * (size_t)c->nz);
+  for (int k = 0; k < c->nz; ++k) {
+    const double qz = (k + 0.0) / (c->nz + 0.0);
+    pen[k] = 100000000.0 * std::pow(qz, 3.0);
+    pen[c->nz + k] = ((100000000.0 * qz) * qz) * qz;
+  }
+  c->pen.ensure(pen.size());
+  upload(c, c->pen.p, pen.data(), pen.size() * sizeof(double));
+  int lds_max = 0;
+  HIP_OK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+  const size_t need = slab_shmem(c) + 512;
+  if (need > (size_t)lds_max || (size_t)c->ny * c->nz * sizeof(int) > (size_t)lds_max)
+    throw SpkError{SPK_EUNSUPPORTED, "tray y*z slab does not fit the device's LDS (" +
+                                         std::to_string(need) + " > " + std::to_string(lds_max) + ")"};
+  if (!c->lds_attrs) {
+    const size_t dyn = (size_t)lds_max - 1024;   // leave room for static shared arrays
+    allow_shmem(k_tray_slab_fwd, dyn);
+    allow_shmem(k_item_slab_fwd, dyn);
+    allow_shmem(k_score_yz, dyn);
+    allow_shmem(k_dist_zy, dyn);
+    allow_shmem(k_corr_x, dyn);
+    allow_shmem(k_tray_x_fwd, dyn);
+    c->lds_attrs = true;
+  }
+}
+
+// ---- distance field --------------------------------------------------------
+void run_distance(spk_ctx* c, const int* d_grid, int* d_out, int* d_tmp, FieldStats* d_st,
+                  int nx, int ny, int nz, int metric, int sources) {
+  if ((long long)ny * nz > 32768)
+    throw SpkError{SPK_EUNSUPPORTED, "distance: y*z plane larger than 32768 voxels"};
+  const int fill = nx + ny + nz + 10;
+  const int edt = metric == SPK_METRIC_EDT ? 1 : 0;
+  const int big = edt ? fill * fill : fill;
+  FieldStats init{0ULL, INT_MAX, INT_MIN};
+  upload(c, d_st, &init, sizeof(init));
+  launch(c, "dist_zy", k_dist_zy, dim3(nx), dim3(kDistThreads), (size_t)ny * nz * sizeof(int),
+         (const int*)d_grid, d_tmp, nx, ny, nz, edt, sources, big);
+  constexpr int TK = 16;
+  launch(c, "dist_x", k_dist_x<TK>, dim3((nz + TK - 1) / TK, ny), dim3(kDistThreads),
+         (size_t)nx * TK * sizeof(int), (const int*)d_tmp, d_out, d_st, nx, ny, nz, edt, fill);
+}
+
+void build_spectra(spk_ctx* c) {
+  const int py = c->py.n, pz = c->pz.n, hz = pz / 2 + 1;
+  c->t1.ensure((size_t)c->nx * py * pz);
+  c->t2.ensure((size_t)c->nx * py * hz);
+  c->g1.ensure((size_t)c->px.n * py * pz);
+  c->g2.ensure((size_t)c->px.n * py * hz);
+  const float2* tw = c->tw.p;
+  launch(c, "tray_slab_fwd", k_tray_slab_fwd, dim3(c->nx, 2), dim3(kSlabThreads), slab_shmem(c),
+         (const int*)c->tray.p, (const int*)c->phi.p, (const FieldStats*)c->stats.p, c->t1.p,
+         c->t2.p, c->nx, c->ny, c->nz, c->py, c->pz, tw + c->px.n, tw + c->px.n + py);
+  launch(c, "tray_x_fwd", k_tray_x_fwd, dim3((pz + c->T - 1) / c->T, py, 2),
+         dim3(kPencilThreads), pencil_shmem(c), (const float2*)c->t1.p, (const float2*)c->t2.p,
+         c->g1.p, c->g2.p, c->nx, c->px, py, pz, c->T, tw);
+}
+
+void refresh_field_and_spectra(spk_ctx* c, const int* dist_host) {
+  const size_t V = (size_t)c->nx * c->ny * c->nz;
+  if (dist_host) {
+    upload(c, c->phi.p, dist_host, V * sizeof(int));
+    FieldStats init{0ULL, INT_MAX, INT_MIN};
+    upload(c, c->stats.p, &init, sizeof(init));
+    const int nb = (int)std::min<size_t>(1024, (V + 255) / 256);
+    launch(c, "field_stats", k_field_stats, dim3(nb), dim3(256), 0, (const int*)c->phi.p,
+           (long long)V, c->stats.p);
+  } else {
+    run_distance(c, c->tray.p, c->phi.p, c->tmpi.p, c->stats.p, c->nx, c->ny, c->nz, c->metric,
+                 c->sources);
+  }
+  build_spectra(c);
+}
+
+// ---- orientation descriptors ----------------------------------------------
+struct HostItem {
+  const int* data;
+  int dims[3];
+  int lo[3], hi[3];   // bounds of voxels > 0, hi = -1 when empty
+  long long sum;
+};
+
+HostItem scan_item(const int* data, int ix, int iy, int iz) {
+  if (!data && (long long)ix * iy * iz > 0) throw SpkError{SPK_EINVAL, "null item"};
+  check_dims(ix, iy, iz);
+  HostItem h{data, {ix, iy, iz}, {INT_MAX, INT_MAX, INT_MAX}, {-1, -1, -1}, 0};
+  for (int i = 0; i < ix; ++i)
+    for (int j = 0; j < iy; ++j)
+      for (int k = 0; k < iz; ++k) {
+        const int v = data[((size_t)i * iy + j) * iz + k];
+        if (v < 0)
+          throw SpkError{SPK_EINVAL, "negative item voxel values are not supported"};
+        if (v > 0) {
+          h.sum += v;
+          h.lo[0] = std::min(h.lo[0], i); h.hi[0] = std::max(h.hi[0], i);
+          h.lo[1] = std::min(h.lo[1], j); h.hi[1] = std::max(h.hi[1], j);
+          h.lo[2] = std::min(h.lo[2], k); h.hi[2] = std::max(h.hi[2], k);
+        }
+      }
+  return h;
+}
+
+OrientDesc make_desc(const spk_ctx* c, const HostItem& it, int item_off, const spk_orient* o) {
+  OrientDesc d{};
+  d.item_off = item_off;
+  d.bx = it.dims[0]; d.by = it.dims[1]; d.bz = it.dims[2];
+  int perm[3] = {0, 1, 2}, flip[3] = {0, 0, 0};
+  if (o) {
+    bool seen[3] = {false, false, false};
+    for (int a = 0; a < 3; ++a) {
+      perm[a] = o->perm[a];
+      flip[a] = o->flip[a] ? 1 : 0;
+      if (perm[a] < 0 || perm[a] > 2 || seen[perm[a]])
+        throw SpkError{SPK_EINVAL, "orientation perm must be a permutation of 0,1,2"};
+      seen[perm[a]] = true;
+    }
+  }
+  const bool empty = it.hi[0] < 0;
+  const int N[3] = {c->nx, c->ny, c->nz};
+  int ext[3], hi[3], nv[3];
+  for (int a = 0; a < 3; ++a) {
+    d.perm[a] = perm[a];
+    d.flip[a] = flip[a];
+    const int b = perm[a];
+    ext[a] = it.dims[b];
+    if (empty) hi[a] = -1;
+    else hi[a] = flip[a] ? (it.dims[b] - 1 - it.lo[b]) : it.hi[b];
+    nv[a] = std::min(N[a], N[a] - hi[a]);
+  }
+  d.ex = ext[0]; d.ey = ext[1]; d.ez = ext[2];
+  d.hx = hi[0]; d.hy = hi[1]; d.hz = hi[2];
+  d.nvx = nv[0]; d.nvy = nv[1]; d.nvz = nv[2];
+  d.isum = it.sum;
+  return d;
+}
+
+bool fits(const spk_ctx* c, const OrientDesc& d) {
+  return d.ex <= c->nx && d.ey <= c->ny && d.ez <= c->nz && d.nvx > 0 && d.nvy > 0 && d.nvz > 0;
+}
+
+void to_result(const spk_ctx* c, bool found, int idx, double score, int orient, long long coll,
+               long long prox, float resid, int path, spk_result* r) {
+  std::memset(r, 0, sizeof(*r));
+  if (found) {
+    r->pos[2] = idx % c->nz;
+    r->pos[1] = (idx / c->nz) % c->ny;
+    r->pos[0] = idx / (c->ny * c->nz);
+    r->found = 1;
+    r->orient = orient;
+    r->score = score;
+    r->collision = coll;
+    r->proximity = prox;
+  } else {
+    r->pos[0] = r->pos[1] = r->pos[2] = -1;
+    r->found = 0;
+    r->orient = -1;
+    r->score = 0.0;
+  }
+  r->residual = resid;
+  r->path = path;
+}
+
+// non-zero voxels of an oriented item as (offset, value) taps
+std::vector<Tap> oriented_taps(const HostItem& it, const OrientDesc& d) {
+  std::vector<Tap> taps;
+  for (int w0 = 0; w0 < d.ex; ++w0)
+    for (int w1 = 0; w1 < d.ey; ++w1)
+      for (int w2 = 0; w2 < d.ez; ++w2) {
+        int w[3] = {w0, w1, w2}, u[3];
+        const int bd[3] = {d.bx, d.by, d.bz};
+        for (int a = 0; a < 3; ++a) {
+          const int ax = d.perm[a];
+          u[ax] = d.flip[a] ? (bd[ax] - 1 - w[a]) : w[a];
+        }
+        const int v = it.data[((size_t)u[0] * d.by + u[1]) * d.bz + u[2]];
+        if (v > 0) taps.push_back(Tap{w0, w1, w2, v});
+      }
+  return taps;
+}
+
+// exact direct-sum search of one orientation (fallback path)
+WgBest exact_search_one(spk_ctx* c, const HostItem& it, const OrientDesc& d, int penalty) {
+  const std::vector<Tap> taps = oriented_taps(it, d);
+  const int nb = 1024;
+  c->taps.ensure(std::max<size_t>(taps.size(), 1));
+  if (!taps.empty()) upload(c, c->taps.p, taps.data(), taps.size() * sizeof(Tap));
+  c->rec.ensure(nb);
+  launch(c, "direct_search", k_direct_search, dim3(nb), dim3(256), 0, (const int*)c->tray.p,
+         (const int*)c->phi.p, (const Tap*)c->taps.p, (int)taps.size(), d.nvx, d.nvy, d.nvz,
+         c->ny, c->nz, (const double*)(c->pen.p + (penalty == SPK_PENALTY_CUBE ? c->nz : 0)),
+         c->rec.p);
+  std::vector<WgBest> rec(nb);
+  download(c, rec.data(), c->rec.p, nb * sizeof(WgBest));
+  sync(c);
+  WgBest best{1e300, INT_MAX, 0, 0, 0.f, 0};
+  for (const auto& r : rec)
+    if (r.score < best.score || (r.score == best.score && r.idx < best.idx)) best = r;
+  return best;
+}
+
+// Flood fill m feasibility grids (c->ffree, m*V bytes) from their seed faces to
+// the fixpoint into c->freach.  exits[3o..] are the high seed faces.
+void flood_fill(spk_ctx* c, int m, const std::vector<int>& exits, int nx, int ny, int nz) {
+  const size_t V = (size_t)nx * ny * nz;
+  c->freach.ensure((size_t)m * V);
+  c->flag.ensure(1);
+  c->exits.ensure((size_t)3 * m);
+  upload(c, c->exits.p, exits.data(), exits.size() * sizeof(int));
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>(2048, (V + 255) / 256));
+  launch(c, "ff_seed", k_ff_seed, dim3(nb, m), dim3(256), 0, (const unsigned char*)c->ffree.p,
+         c->freach.p, (const int*)c->exits.p, nx, ny, nz);
+  const int xt = (nx + FFX - 1) / FFX;
+  const dim3 grid((nz + FFZ - 1) / FFZ, (ny + FFY - 1) / FFY, xt * m);
+  // a round that reports a change reached at least one new cell, so at most
+  // V rounds report one and the loop always ends
+  for (size_t it = 0; it <= (size_t)m * V; ++it) {
+    HIP_OK(hipMemsetAsync(c->flag.p, 0, sizeof(int), c->stream));
+    launch(c, "ff_tile", k_ff_tile, grid, dim3(FFX * FFY * FFZ), 0,
+           (const unsigned char*)c->ffree.p, c->freach.p, nx, ny, nz, xt, c->flag.p);
+    int ch = 0;
+    download(c, &ch, c->flag.p, sizeof(int));
+    sync(c);
+    if (!ch) break;
+  }
+}
+
+// IF mode: feasibility grids -> reachable set -> masked argmin records
+void if_masked_select(spk_ctx* c, const std::vector<OrientDesc>& ods, const double* pen) {
+  const int m = (int)ods.size();
+  std::vector<int> exits(3 * (size_t)m);
+  for (int o = 0; o < m; ++o) {
+    exits[3 * o] = c->nx - 1 - ods[o].hx;
+    exits[3 * o + 1] = c->ny - 1 - ods[o].hy;
+    exits[3 * o + 2] = c->nz - 1 - ods[o].hz;
+  }
+  flood_fill(c, m, exits, c->nx, c->ny, c->nz);
+  launch(c, "masked_argmin", k_masked_argmin, dim3(c->nx, m), dim3(256), 0,
+         (const unsigned char*)c->freach.p, (const long long*)c->pgrid.p,
+         (const OrientDesc*)c->od.p, pen, c->rec.p, c->nx, c->ny, c->nz);
+}
+
+// ---- the search --------------------------------------------------------------
+// items: per entry base item + orientation; all on host.
+struct Entry {
+  HostItem item;
+  const spk_orient* orient;
+  int item_index;   // which distinct base item
+};
+
+void do_search(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector<int>& base_off,
+               bool bases_uploaded, const std::vector<Entry>& entries, int penalty,
+               int interlocking_free, spk_result* best, spk_result* per_orient) {
+  if (!c->valid) throw SpkError{SPK_ESTATE, "no tray set in this context"};
+  const int n = (int)entries.size();
+  if (n <= 0) throw SpkError{SPK_EINVAL, "no orientations"};
+  // upload base items
+  if (!bases_uploaded) {
+    size_t tot = 0;
+    for (const auto& b : bases) tot += (size_t)b.dims[0] * b.dims[1] * b.dims[2];
+    c->items.ensure(std::max<size_t>(tot, 1));
+    for (size_t i = 0; i < bases.size(); ++i) {
+      const auto& b = bases[i];
+      const size_t sz = (size_t)b.dims[0] * b.dims[1] * b.dims[2];
+      if (sz) upload(c, c->items.p + base_off[i], b.data, sz * sizeof(int));
+    }
+  }
+  // descriptors of the orientations that fit
+  std::vector<OrientDesc> ods;
+  std::vector<int> map;   // ods index -> entry index
+  int max_ex = 1;
+  for (int e = 0; e < n; ++e) {
+    const auto& en = entries[e];
+    OrientDesc d = make_desc(c, en.item, base_off[en.item_index], en.orient);
+    if (!fits(c, d)) continue;
+    ods.push_back(d);
+    map.push_back(e);
+    max_ex = std::max(max_ex, d.ex);
+  }
+  if (per_orient)
+    for (int e = 0; e < n; ++e) to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, &per_orient[e]);
+  if (ods.empty()) {
+    to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, best);
+    return;
+  }
+  const int m = (int)ods.size();
+  const int py = c->py.n, pz = c->pz.n, hz = pz / 2 + 1;
+  c->od.ensure(m);
+  upload(c, c->od.p, ods.data(), m * sizeof(OrientDesc));
+  c->ispec.ensure((size_t)m * max_ex * py * pz);
+  c->y1.ensure((size_t)m * c->nx * py * pz);
+  c->y2.ensure((size_t)m * c->nx * py * hz);
+  c->rec.ensure((size_t)m * c->nx);
+  c->per.ensure(m);
+  c->sout.ensure(1);
+  c->ver.ensure(3);
+  const float2* tw = c->tw.p;
+  launch(c, "item_slab_fwd", k_item_slab_fwd, dim3(max_ex, m), dim3(kSlabThreads),
+         slab_shmem(c), (const int*)c->items.p, (const OrientDesc*)c->od.p, c->ispec.p, max_ex,
+         c->py, c->pz, tw + c->px.n, tw + c->px.n + py);
+  const int tiles = (pz + c->T - 1) / c->T;
+  int nchunk = std::max(1, std::min(m, (2048 + tiles * py - 1) / (tiles * py)));
+  const int chunk = (m + nchunk - 1) / nchunk;
+  nchunk = (m + chunk - 1) / chunk;
+  const float scale = (float)(1.0 / ((double)c->px.n * py * pz));
+  launch(c, "corr_x", k_corr_x, dim3(tiles, py, nchunk), dim3(kPencilThreads), pencil_shmem(c),
+         (const float2*)c->g1.p, (const float2*)c->g2.p, (const float2*)c->ispec.p,
+         (const OrientDesc*)c->od.p, c->y1.p, c->y2.p, m, chunk, max_ex, c->nx, c->px, py, pz,
+         c->T, scale, tw);
+  const double* pen = c->pen.p + (penalty == SPK_PENALTY_CUBE ? c->nz : 0);
+  const size_t V = (size_t)c->nx * c->ny * c->nz;
+  if (interlocking_free) {
+    c->ffree.ensure((size_t)m * V);
+    c->pgrid.ensure((size_t)m * V);
+    HIP_OK(hipMemsetAsync(c->ffree.p, 0, (size_t)m * V, c->stream));
+  }
+  launch(c, "score_yz", k_score_yz, dim3(c->nx, m), dim3(kSlabThreads), slab_shmem(c),
+         (const float2*)c->y1.p, (const float2*)c->y2.p, (const OrientDesc*)c->od.p,
+         (const FieldStats*)c->stats.p, pen, c->rec.p, c->nx, c->ny, c->nz, c->py, c->pz,
+         tw + c->px.n, tw + c->px.n + py, interlocking_free ? c->ffree.p : (unsigned char*)nullptr,
+         interlocking_free ? c->pgrid.p : (long long*)nullptr);
+  if (interlocking_free) if_masked_select(c, ods, pen);
+  launch(c, "select", k_select, dim3(1), dim3(256), 0, (const WgBest*)c->rec.p,
+         (const OrientDesc*)c->od.p, m, c->nx, c->per.p, c->sout.p);
+  launch(c, "verify", k_verify, dim3(1), dim3(256), 0, (const int*)c->tray.p,
+         (const int*)c->phi.p, (const int*)c->items.p, (const OrientDesc*)c->od.p,
+         (const SearchOut*)c->sout.p, c->ver.p, c->ny, c->nz);
+  SearchOut so;
+  std::vector<WgBest> per(m);
+  long long ver[3];
+  download(c, &so, c->sout.p, sizeof(so));
+  download(c, per.data(), c->per.p, m * sizeof(WgBest));
+  download(c, ver, c->ver.p, sizeof(ver));
+  sync(c);
+  if (c->timing) c->flush_timing();
+  const float kResidLimit = 0.25f;
+  const bool bad = so.unsupported || so.resid > kResidLimit ||
+                   (so.found && (ver[1] != 0 || ver[2] != so.prox));
+  if (!bad) {
+    for (int i = 0; i < m; ++i) {
+      const WgBest& r = per[i];
+      const bool f = r.idx != INT_MAX;
+      const long long p = ((long long)r.p_hi << 32) | (unsigned int)r.p_lo;
+      if (per_orient)
+        to_result(c, f, r.idx, r.score, map[i], 0, p, r.resid, 0, &per_orient[map[i]]);
+    }
+    to_result(c, so.found, so.idx, so.score, so.found ? map[so.orient] : -1, ver[0], so.prox,
+              so.resid, 0, best);
+    return;
+  }
+  // exact fallback: direct sums for every orientation
+  if (interlocking_free) {
+    HIP_OK(hipMemsetAsync(c->ffree.p, 0, (size_t)m * V, c->stream));
+    for (int i = 0; i < m; ++i) {
+      const std::vector<Tap> taps = oriented_taps(entries[map[i]].item, ods[i]);
+      c->taps.ensure(std::max<size_t>(taps.size(), 1));
+      if (!taps.empty()) upload(c, c->taps.p, taps.data(), taps.size() * sizeof(Tap));
+      const int nb = 1024;
+      launch(c, "direct_grid", k_direct_grid, dim3(nb), dim3(256), 0, (const int*)c->tray.p,
+             (const int*)c->phi.p, (const Tap*)c->taps.p, (int)taps.size(), ods[i].nvx,
+             ods[i].nvy, ods[i].nvz, c->ny, c->nz, c->ffree.p + (size_t)i * V,
+             c->pgrid.p + (size_t)i * V);
+      sync(c);
+    }
+    if_masked_select(c, ods, pen);
+    launch(c, "select", k_select, dim3(1), dim3(256), 0, (const WgBest*)c->rec.p,
+           (const OrientDesc*)c->od.p, m, c->nx, c->per.p, c->sout.p);
+    SearchOut s2;
+    download(c, &s2, c->sout.p, sizeof(s2));
+    download(c, per.data(), c->per.p, m * sizeof(WgBest));
+    sync(c);
+    for (int i = 0; i < m && per_orient; ++i) {
+      const WgBest& r = per[i];
+      const long long p = ((long long)r.p_hi << 32) | (unsigned int)r.p_lo;
+      to_result(c, r.idx != INT_MAX, r.idx, r.score, map[i], 0, p, so.resid, 1, &per_orient[map[i]]);
+    }
+    to_result(c, s2.found, s2.idx, s2.score, s2.found ? map[s2.orient] : -1, 0, s2.prox,
+              so.resid, 1, best);
+    return;
+  }
+  double bs = 1e30;
+  int bi = -1, bo = -1;
+  long long bp = 0;
+  for (int i = 0; i < m; ++i) {
+    const Entry& en = entries[map[i]];
+    WgBest r = exact_search_one(c, en.item, ods[i], penalty);
+    const bool f = r.idx != INT_MAX;
+    const long long p = ((long long)r.p_hi << 32) | (unsigned int)r.p_lo;
+    if (per_orient) to_result(c, f, r.idx, r.score, map[i], 0, p, so.resid, 1, &per_orient[map[i]]);
+    if (f && r.score < bs) { bs = r.score; bi = r.idx; bo = i; bp = p; }
+  }
+  if (bo < 0) {
+    to_result(c, false, 0, 0.0, -1, 0, 0, so.resid, 1, best);
+    return;
+  }
+  to_result(c, true, bi, bs, map[bo], 0, bp, so.resid, 1, best);
+}
+
+void do_place(spk_ctx* c, const OrientDesc& d, int x, int y, int z, int id, bool refresh) {
+  c->flag.ensure(1);
+  HIP_OK(hipMemsetAsync(c->flag.p, 0, sizeof(int), c->stream));
+  c->od.ensure(1);
+  upload(c, c->od.p, &d, sizeof(d));
+  const int E = d.ex * d.ey * d.ez;
+  const int nb = std::max(1, std::min(1024, (E + 255) / 256));
+  launch(c, "place", k_place, dim3(nb), dim3(256), 0, c->tray.p, (const int*)c->items.p,
+         (const OrientDesc*)c->od.p, (const SearchOut*)nullptr, 0, x, y, z, id, c->nx, c->ny,
+         c->nz, 0, c->flag.p);
+  int flag = 0;
+  download(c, &flag, c->flag.p, sizeof(int));
+  sync(c);
+  if (flag) throw SpkError{SPK_EOVERLAP, "(place_in_tray): Item to be placed intersects with contents of tray"};
+  launch(c, "place", k_place, dim3(nb), dim3(256), 0, c->tray.p, (const int*)c->items.p,
+         (const OrientDesc*)c->od.p, (const SearchOut*)nullptr, 0, x, y, z, id, c->nx, c->ny,
+         c->nz, 1, c->flag.p);
+  if (refresh) refresh_field_and_spectra(c, nullptr);
+}
+
+}  // namespace
+
+// =============================================================================
+// C-ABI
+// =============================================================================
+extern "C" {
+
+const char* spk_last_error(void) { return g_err.c_str(); }
+
+int spk_version(int* major, int* minor, int* patch) {
+  if (major) *major = 0;
+  if (minor) *minor = 1;
+  if (patch) *patch = 0;
+  return SPK_OK;
+}
+
+int spk_ctx_create(int device, spk_ctx** out) {
+  if (!out) return fail(SPK_EINVAL, "null out pointer");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+    return fail(SPK_EDEVICE, "no HIP device available");
+  if (device < 0 || device >= n) return fail(SPK_EINVAL, "device index out of range");
+  spk_ctx* c = new (std::nothrow) spk_ctx();
+  if (!c) return fail(SPK_ENOMEM, "host allocation failed");
+  c->device = device;
+  int rc = guarded(c, [&]() {
+    HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    return SPK_OK;
+  });
+  if (rc != SPK_OK) {
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return SPK_OK;
+}
+
+int spk_ctx_destroy(spk_ctx* c) {
+  if (!c) return SPK_OK;
+  {
+    DeviceGuard g(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& t : c->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
+    for (auto e : c->pool) hipEventDestroy(e);
+    c->tray.release(); c->phi.release(); c->tmpi.release(); c->stats.release();
+    c->tw.release(); c->t1.release(); c->t2.release(); c->g1.release(); c->g2.release();
+    c->pen.release(); c->items.release(); c->od.release(); c->ispec.release();
+    c->y1.release(); c->y2.release(); c->rec.release(); c->per.release(); c->sout.release();
+    c->ver.release(); c->flag.release(); c->taps.release(); c->big.release();
+    c->ffree.release(); c->freach.release(); c->pgrid.release(); c->exits.release();
+    if (c->stream) hipStreamDestroy(c->stream);
+  }
+  delete c;
+  return SPK_OK;
+}
+
+int spk_tray_set(spk_ctx* c, const int32_t* tray, int nx, int ny, int nz, const int32_t* dist,
+                 int metric, int sources, uint64_t generation) {
+  return guarded(c, [&]() {
+    if (!tray) throw SpkError{SPK_EINVAL, "null tray"};
+    check_dims(nx, ny, nz);
+    if (metric != SPK_METRIC_L1 && metric != SPK_METRIC_EDT)
+      throw SpkError{SPK_EINVAL, "metric must be SPK_METRIC_L1 or SPK_METRIC_EDT"};
+    if (sources != SPK_SOURCES_EQ1 && sources != SPK_SOURCES_NONZERO)
+      throw SpkError{SPK_EINVAL, "sources must be SPK_SOURCES_EQ1 or SPK_SOURCES_NONZERO"};
+    c->valid = false;
+    const bool reshape = !(c->nx == nx && c->ny == ny && c->nz == nz) || c->tw.p == nullptr;
+    c->nx = nx; c->ny = ny; c->nz = nz;
+    c->metric = metric; c->sources = sources;
+    if (reshape) plan_tray(c);
+    const size_t V = (size_t)nx * ny * nz;
+    c->tray.ensure(V);
+    c->phi.ensure(V);
+    c->tmpi.ensure(V);
+    c->stats.ensure(1);
+    upload(c, c->tray.p, tray, V * sizeof(int));
+    refresh_field_and_spectra(c, dist);
+    sync(c);
+    if (c->timing) c->flush_timing();
+    c->generation = generation;
+    c->valid = true;
+    return SPK_OK;
+  });
+}
+
+int spk_tray_get(spk_ctx* c, int32_t* tray_out, int32_t* dist_out) {
+  return guarded(c, [&]() {
+    if (!c->valid) throw SpkError{SPK_ESTATE, "no tray set in this context"};
+    const size_t V = (size_t)c->nx * c->ny * c->nz;
+    if (tray_out) download(c, tray_out, c->tray.p, V * sizeof(int));
+    if (dist_out) download(c, dist_out, c->phi.p, V * sizeof(int));
+    sync(c);
+    return SPK_OK;
+  });
+}
+
+int spk_tray_info(spk_ctx* c, int* nx, int* ny, int* nz, uint64_t* generation, int* valid) {
+  if (!c) return fail(SPK_EINVAL, "null context");
+  if (nx) *nx = c->nx;
+  if (ny) *ny = c->ny;
+  if (nz) *nz = c->nz;
+  if (generation) *generation = c->generation;
+  if (valid) *valid = c->valid ? 1 : 0;
+  return SPK_OK;
+}
+
+int spk_tray_clear(spk_ctx* c) {
+  if (!c) return fail(SPK_EINVAL, "null context");
+  c->valid = false;
+  return SPK_OK;
+}
+
+int spk_search(spk_ctx* c, const int32_t* const* items, const int32_t* dims, int n, int penalty,
+               int interlocking_free, spk_result* best, spk_result* per_orient) {
+  return guarded(c, [&]() {
+    if (!items || !dims || !best || n <= 0) throw SpkError{SPK_EINVAL, "bad search arguments"};
+    std::vector<HostItem> bases;
+    std::vector<int> off;
+    std::vector<Entry> entries;
+    int tot = 0;
+    for (int o = 0; o < n; ++o) {
+      bases.push_back(scan_item(items[o], dims[3 * o], dims[3 * o + 1], dims[3 * o + 2]));
+      off.push_back(tot);
+      tot += dims[3 * o] * dims[3 * o + 1] * dims[3 * o + 2];
+    }
+    for (int o = 0; o < n; ++o) entries.push_back(Entry{bases[o], nullptr, o});
+    do_search(c, bases, off, false, entries, penalty, interlocking_free, best, per_orient);
+    return SPK_OK;
+  });
+}
+
+int spk_search_oriented(spk_ctx* c, const int32_t* item, int ix, int iy, int iz,
+                        const spk_orient* orients, int n, int penalty, int interlocking_free,
+                        spk_result* best, spk_result* per_orient) {
+  return guarded(c, [&]() {
+    if (!item || !best || n <= 0) throw SpkError{SPK_EINVAL, "bad search arguments"};
+    std::vector<HostItem> bases{scan_item(item, ix, iy, iz)};
+    std::vector<int> off{0};
+    std::vector<Entry> entries;
+    for (int o = 0; o < n; ++o) entries.push_back(Entry{bases[0], orients ? &orients[o] : nullptr, 0});
+    do_search(c, bases, off, false, entries, penalty, interlocking_free, best, per_orient);
+    return SPK_OK;
+  });
+}
+
+int spk_place(spk_ctx* c, const int32_t* item, int ix, int iy, int iz, const spk_orient* orient,
+              int x, int y, int z, int32_t id) {
+  return guarded(c, [&]() {
+    if (!c->valid) throw SpkError{SPK_ESTATE, "no tray set in this context"};
+    HostItem it = scan_item(item, ix, iy, iz);
+    const size_t sz = (size_t)ix * iy * iz;
+    c->items.ensure(std::max<size_t>(sz, 1));
+    upload(c, c->items.p, item, sz * sizeof(int));
+    OrientDesc d = make_desc(c, it, 0, orient);
+    do_place(c, d, x, y, z, id, true);
+    sync(c);
+    if (c->timing) c->flush_timing();
+    return SPK_OK;
+  });
+}
+
+int spk_pack(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_items,
+             const int32_t* order, const spk_orient* orients, int n_orient, int penalty,
+             int interlocking_free, int32_t id_offset, spk_result* results) {
+  return guarded(c, [&]() {
+    if (!c->valid) throw SpkError{SPK_ESTATE, "no tray set in this context"};
+    if (!items || !dims || n_items <= 0 || !results || n_orient <= 0)
+      throw SpkError{SPK_EINVAL, "bad pack arguments"};
+    std::vector<HostItem> bases;
+    std::vector<int> off;
+    size_t tot = 0;
+    for (int t = 0; t < n_items; ++t) {
+      const int* p = items + tot;
+      bases.push_back(scan_item(p, dims[3 * t], dims[3 * t + 1], dims[3 * t + 2]));
+      off.push_back((int)tot);
+      tot += (size_t)dims[3 * t] * dims[3 * t + 1] * dims[3 * t + 2];
+    }
+    c->items.ensure(std::max<size_t>(tot, 1));
+    upload(c, c->items.p, items, tot * sizeof(int));
+    int placed = 0;
+    for (int t = 0; t < n_items; ++t) {
+      const int it = order ? order[t] : t;
+      if (it < 0 || it >= n_items) throw SpkError{SPK_EINVAL, "order index out of range"};
+      std::vector<Entry> entries;
+      for (int o = 0; o < n_orient; ++o)
+        entries.push_back(Entry{bases[it], orients ? &orients[o] : nullptr, it});
+      do_search(c, bases, off, true, entries, penalty, interlocking_free, &results[t], nullptr);
+      if (results[t].found) {
+        const int o = results[t].orient;
+        OrientDesc d = make_desc(c, bases[it], off[it], orients ? &orients[o] : nullptr);
+        do_place(c, d, results[t].pos[0], results[t].pos[1], results[t].pos[2],
+                 id_offset + placed + 1, true);
+        ++placed;
+      }
+    }
+    sync(c);
+    if (c->timing) c->flush_timing();
+    c->generation += (uint64_t)placed;
+    return SPK_OK;
+  });
+}
+
+int spk_distance(spk_ctx* c, const int32_t* grid, int nx, int ny, int nz, int metric, int sources,
+                 int32_t* out) {
+  return guarded(c, [&]() {
+    if (!grid || !out) throw SpkError{SPK_EINVAL, "null grid"};
+    check_dims(nx, ny, nz);
+    const size_t V = (size_t)nx * ny * nz;
+    DevBuf<int> g, o, t;
+    DevBuf<FieldStats> st;
+    g.ensure(V); o.ensure(V); t.ensure(V); st.ensure(1);
+    upload(c, g.p, grid, V * sizeof(int));
+    run_distance(c, g.p, o.p, t.p, st.p, nx, ny, nz, metric, sources);
+    download(c, out, o.p, V * sizeof(int));
+    sync(c);
+    if (c->timing) c->flush_timing();
+    g.release(); o.release(); t.release(); st.release();
+    return SPK_OK;
+  });
+}
+
+namespace {
+std::vector<Tap> taps_of(const int* b, int nx, int ny, int nz) {
+  std::vector<Tap> taps;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k) {
+        const int v = b[((size_t)i * ny + j) * nz + k];
+        if (v != 0) taps.push_back(Tap{i, j, k, v});
+      }
+  return taps;
+}
+
+// direct correlation of grid a (device) with taps, int64 result downloaded
+void direct_full(spk_ctx* c, const int* d_a, const std::vector<Tap>& taps, int nx, int ny, int nz,
+                 int conv, std::vector<long long>& out) {
+  const size_t V = (size_t)nx * ny * nz;
+  c->taps.ensure(std::max<size_t>(taps.size(), 1));
+  if (!taps.empty()) upload(c, c->taps.p, taps.data(), taps.size() * sizeof(Tap));
+  c->big.ensure(V);
+  const int nb = (int)std::max<size_t>(1, std::min<size_t>(4096, (V + 255) / 256));
+  launch(c, "direct_corr", k_direct_corr, dim3(nb), dim3(256), 0, d_a, (const Tap*)c->taps.p,
+         (int)taps.size(), c->big.p, nx, ny, nz, conv);
+  out.resize(V);
+  download(c, out.data(), c->big.p, V * sizeof(long long));
+  sync(c);
+}
+}  // namespace
+
+int spk_correlate(spk_ctx* c, const int32_t* a, const int32_t* b, int nx, int ny, int nz,
+                  int convolve, int32_t* out) {
+  return guarded(c, [&]() {
+    if (!a || !b || !out) throw SpkError{SPK_EINVAL, "null grid"};
+    check_dims(nx, ny, nz);
+    const size_t V = (size_t)nx * ny * nz;
+    DevBuf<int> g;
+    g.ensure(V);
+    upload(c, g.p, a, V * sizeof(int));
+    std::vector<long long> r;
+    direct_full(c, g.p, taps_of(b, nx, ny, nz), nx, ny, nz, convolve ? 1 : 0, r);
+    for (size_t q = 0; q < V; ++q) out[q] = (int32_t)r[q];
+    g.release();
+    if (c->timing) c->flush_timing();
+    return SPK_OK;
+  });
+}
+
+int spk_collision_grid(spk_ctx* c, const int32_t* tray, int nx, int ny, int nz, const int32_t* item,
+                       int ix, int iy, int iz, int32_t* out) {
+  return guarded(c, [&]() {
+    if (!tray || !item || !out) throw SpkError{SPK_EINVAL, "null grid"};
+    check_dims(nx, ny, nz);
+    check_dims(ix, iy, iz);
+    const size_t V = (size_t)nx * ny * nz;
+    // item padded/truncated to the tray box (bindings.cpp:366-368)
+    std::vector<Tap> taps;
+    const int kNegInf = -1000000000;   // constants.h:21 (-INF for an empty item)
+    int hi[3] = {kNegInf, kNegInf, kNegInf};
+    for (int i = 0; i < std::min(ix, nx); ++i)
+      for (int j = 0; j < std::min(iy, ny); ++j)
+        for (int k = 0; k < std::min(iz, nz); ++k) {
+          const int v = item[((size_t)i * iy + j) * iz + k];
+          if (v != 0) taps.push_back(Tap{i, j, k, v});
+          if (v > 0) { hi[0] = std::max(hi[0], i); hi[1] = std::max(hi[1], j); hi[2] = std::max(hi[2], k); }
+        }
+    DevBuf<int> g;
+    g.ensure(V);
+    upload(c, g.p, tray, V * sizeof(int));
+    std::vector<long long> r;
+    direct_full(c, g.p, taps, nx, ny, nz, 0, r);
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) {
+          const size_t q = ((size_t)i * ny + j) * nz + k;
+          int v = (int32_t)r[q];
+          const bool ok = ((long long)i + hi[0] <= nx - 1) && ((long long)j + hi[1] <= ny - 1) &&
+                          ((long long)k + hi[2] <= nz - 1);
+          if (!ok) v = std::max(v, 1);
+          out[q] = v;
+        }
+    g.release();
+    if (c->timing) c->flush_timing();
+    return SPK_OK;
+  });
+}
+
+int spk_place_grid(spk_ctx* c, const int32_t* tray, int nx, int ny, int nz, const int32_t* item,
+                   int ix, int iy, int iz, int x, int y, int z, int32_t id, int32_t* out) {
+  return guarded(c, [&]() {
+    if (!tray || !item || !out) throw SpkError{SPK_EINVAL, "null grid"};
+    check_dims(nx, ny, nz);
+    check_dims(ix, iy, iz);
+    const size_t V = (size_t)nx * ny * nz;
+    const size_t I = (size_t)ix * iy * iz;
+    DevBuf<int> g, it;
+    DevBuf<OrientDesc> od;
+    DevBuf<int> flag;
+    g.ensure(V); it.ensure(I); od.ensure(1); flag.ensure(1);
+    upload(c, g.p, tray, V * sizeof(int));
+    upload(c, it.p, item, I * sizeof(int));
+    OrientDesc d{};
+    d.bx = ix; d.by = iy; d.bz = iz;
+    d.perm[0] = 0; d.perm[1] = 1; d.perm[2] = 2;
+    d.ex = ix; d.ey = iy; d.ez = iz;
+    upload(c, od.p, &d, sizeof(d));
+    HIP_OK(hipMemsetAsync(flag.p, 0, sizeof(int), c->stream));
+    const int nb = (int)std::max<size_t>(1, std::min<size_t>(1024, (I + 255) / 256));
+    for (int phase = 0; phase < 2; ++phase)
+      launch(c, "place", k_place, dim3(nb), dim3(256), 0, g.p, (const int*)it.p,
+             (const OrientDesc*)od.p, (const SearchOut*)nullptr, 0, x, y, z, (int)id, nx, ny, nz,
+             phase, flag.p);
+    int f = 0;
+    download(c, &f, flag.p, sizeof(int));
+    download(c, out, g.p, V * sizeof(int));
+    sync(c);
+    g.release(); it.release(); od.release(); flag.release();
+    if (c->timing) c->flush_timing();
+    if (f) {
+      std::memcpy(out, tray, V * sizeof(int));
+      throw SpkError{SPK_EOVERLAP,
+                     "(place_in_tray): Item to be placed intersects with contents of tray"};
+    }
+    return SPK_OK;
+  });
+}
+
+int spk_interlock_mask(spk_ctx* c, const int32_t* tray, int nx, int ny, int nz, const int32_t* obj,
+                       int ox, int oy, int oz, int32_t* out) {
+  return guarded(c, [&]() {
+    if (!tray || !obj || !out) throw SpkError{SPK_EINVAL, "null grid"};
+    check_dims(nx, ny, nz);
+    check_dims(ox, oy, oz);
+    const size_t V = (size_t)nx * ny * nz;
+    int om[3] = {0, 0, 0};   // bindings.cpp:597-609: max over voxels != 0, from 0
+    std::vector<Tap> taps;
+    for (int i = 0; i < ox; ++i)
+      for (int j = 0; j < oy; ++j)
+        for (int k = 0; k < oz; ++k) {
+          const int v = obj[((size_t)i * oy + j) * oz + k];
+          if (v != 0) {
+            om[0] = std::max(om[0], i); om[1] = std::max(om[1], j); om[2] = std::max(om[2], k);
+            if (i < nx && j < ny && k < nz) taps.push_back(Tap{i, j, k, v});
+          }
+        }
+    DevBuf<int> g;
+    g.ensure(V);
+    upload(c, g.p, tray, V * sizeof(int));
+    std::vector<long long> r;
+    direct_full(c, g.p, taps, nx, ny, nz, 0, r);
+    std::vector<unsigned char> fr(V);
+    for (int i = 0; i < nx; ++i)
+      for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) {
+          const size_t q = ((size_t)i * ny + j) * nz + k;
+          const bool oob = i + om[0] >= nx || j + om[1] >= ny || k + om[2] >= nz;
+          fr[q] = (!oob && r[q] == 0) ? 1 : 0;
+        }
+    c->ffree.ensure(V);
+    upload(c, c->ffree.p, fr.data(), V);
+    // exit faces max(0, N-1-objmax) (fft3.cu:276-278)
+    const std::vector<int> exits{std::max(0, nx - 1 - om[0]), std::max(0, ny - 1 - om[1]),
+                                 std::max(0, nz - 1 - om[2])};
+    flood_fill(c, 1, exits, nx, ny, nz);
+    std::vector<unsigned char> re(V);
+    download(c, re.data(), c->freach.p, V);
+    sync(c);
+    for (size_t q = 0; q < V; ++q) out[q] = re[q] ? 1 : 0;
+    g.release();
+    if (c->timing) c->flush_timing();
+    return SPK_OK;
+  });
+}
+
+int spk_bounds(const int32_t* grid, int nx, int ny, int nz, int32_t lo[3], int32_t hi[3]) {
+  if (!grid || !lo || !hi || nx < 0 || ny < 0 || nz < 0) return fail(SPK_EINVAL, "bad arguments");
+  const int INF = 1000000000;   // constants.h:21
+  lo[0] = lo[1] = lo[2] = INF;
+  hi[0] = hi[1] = hi[2] = -INF;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = 0; k < nz; ++k)
+        if (grid[((size_t)i * ny + j) * nz + k] > 0) {
+          lo[0] = std::min(lo[0], i); lo[1] = std::min(lo[1], j); lo[2] = std::min(lo[2], k);
+          hi[0] = std::max(hi[0], i); hi[1] = std::max(hi[1], j); hi[2] = std::max(hi[2], k);
+        }
+  return SPK_OK;
+}
+
+int spk_save_vox(const int32_t* grid, int nx, int ny, int nz, const char* path) {
+  if (!grid || !path || nx < 0 || ny < 0 || nz < 0) return fail(SPK_EINVAL, "bad arguments");
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(SPK_EINVAL, std::string("cannot open ") + path);
+  // slab .vox layout of saveAsVox (packing.cpp:275-305): three 4-byte sizes,
+  // voxels with x and z descending, palette byte 123 + v%5 (255 empty, 123
+  // for the INSIDE flag 8), then 256 RGB palette entries.
+  const int32_t s[3] = {nx, ny, nz};
+  std::fwrite(s, 4, 3, f);
+  for (int i = nx - 1; i >= 0; --i)
+    for (int j = 0; j < ny; ++j)
+      for (int k = nz - 1; k >= 0; --k) {
+        const unsigned char v = (unsigned char)grid[((size_t)i * ny + j) * nz + k];
+        unsigned char pal = v != 0 ? (unsigned char)(123 + (v % 5)) : 255;
+        if (v == 8) pal = 123;
+        std::fwrite(&pal, 1, 1, f);
+      }
+  unsigned char palette[256][3];
+  std::memset(palette, 0, sizeof(palette));
+  const unsigned char colors[5][3] = {{127, 0, 127}, {255, 0, 0}, {0, 255, 0}, {0, 0, 255}, {255, 255, 255}};
+  for (int q = 0; q < 5; ++q) std::memcpy(palette[123 + q], colors[q], 3);
+  std::fwrite(palette, 3, 256, f);
+  std::fclose(f);
+  return SPK_OK;
+}
+
+int spk_timing_enable(spk_ctx* c, int on) {
+  if (!c) return fail(SPK_EINVAL, "null context");
+  c->timing = on != 0;
+  return SPK_OK;
+}
+
+int spk_timing_get(spk_ctx* c, const char* kernel, double* total_ms, int64_t* launches) {
+  return guarded(c, [&]() {
+    c->flush_timing();
+    auto it = c->acc.find(kernel ? kernel : "");
+    if (total_ms) *total_ms = it == c->acc.end() ? 0.0 : it->second.first;
+    if (launches) *launches = it == c->acc.end() ? 0 : it->second.second;
+    return SPK_OK;
+  });
+}
+
+int spk_timing_reset(spk_ctx* c) {
+  return guarded(c, [&]() {
+    c->flush_timing();
+    c->acc.clear();
+    return SPK_OK;
+  });
+}
+
+int spk_plan_info(spk_ctx* c, int* px, int* py, int* pz, int* digit_base) {
+  if (!c) return fail(SPK_EINVAL, "null context");
+  if (px) *px = c->px.n;
+  if (py) *py = c->py.n;
+  if (pz) *pz = c->pz.n;
+  if (digit_base) *digit_base = 0;
+  return SPK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,227 @@
+// spk_fft.hpp -- workgroup-cooperative mixed-radix Stockham FFT in LDS (gfx950).
+//
+// One buffer in LDS holds `batch` transforms of length n, element e of
+// transform b at buf[b*bs + e*es].  Each stage reads every butterfly's R
+// inputs into registers, barriers, and writes the R outputs in Stockham
+// (self-sorting) order, so a single LDS buffer suffices and results come out
+// in natural order.  Radices 2,3,4,5,7,8; twiddles come from a table of the n
+// roots exp(-2*pi*i*q/n), computed on the host in double and rounded once to
+// float.  Stage s with radix R and span Ns (product of earlier radices):
+//     j in [0, n/R), k = j mod Ns,
+//     v_r = x[j + r*n/R] * w_n^{r*k*n/(Ns*R)},  V = DFT_R(v),
+//     y[(j/Ns)*Ns*R + k + m*Ns] = V_m.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spk {
+
+constexpr int kMaxStages = 12;
+
+struct AxisPlan {
+  int n;
+  int nst;
+  unsigned long long rad;   // 4 bits per stage, stage 0 in the low bits
+};
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// i*sgn*z with sgn = -1 (forward) or +1 (inverse)
+template <bool FWD>
+__device__ __forceinline__ float2 isg(float2 z) {
+  return FWD ? make_float2(z.y, -z.x) : make_float2(-z.y, z.x);
+}
+
+// cos/sin(2*pi*q/R) for the odd radices
+template <int R> struct OddTab;
+template <> struct OddTab<3> {
+  static constexpr float c[3] = {1.0f, -0.5f, -0.5f};
+  static constexpr float s[3] = {0.0f, 0.86602540378443865f, -0.86602540378443865f};
+};
+template <> struct OddTab<5> {
+  static constexpr float c[5] = {1.0f, 0.30901699437494742f, -0.80901699437494742f,
+                                 -0.80901699437494742f, 0.30901699437494742f};
+  static constexpr float s[5] = {0.0f, 0.95105651629515357f, 0.58778525229247313f,
+                                 -0.58778525229247313f, -0.95105651629515357f};
+};
+template <> struct OddTab<7> {
+  static constexpr float c[7] = {1.0f, 0.62348980185873353f, -0.22252093395631440f,
+                                 -0.90096886790241913f, -0.90096886790241913f,
+                                 -0.22252093395631440f, 0.62348980185873353f};
+  static constexpr float s[7] = {0.0f, 0.78183148246802981f, 0.97492791218182361f,
+                                 0.43388373911755812f, -0.43388373911755812f,
+                                 -0.97492791218182361f, -0.78183148246802981f};
+};
+
+template <int R, bool FWD> struct Bfly;
+
+template <bool FWD> struct Bfly<2, FWD> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <bool FWD> struct Bfly<4, FWD> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    float2 s13 = cadd(v[1], v[3]), d13 = isg<FWD>(csub(v[1], v[3]));
+    v[0] = cadd(s02, s13);
+    v[2] = csub(s02, s13);
+    v[1] = cadd(d02, d13);
+    v[3] = csub(d02, d13);
+  }
+};
+
+template <bool FWD> struct Bfly<8, FWD> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    Bfly<4, FWD>::run(e);
+    Bfly<4, FWD>::run(o);
+    const float h = 0.70710678118654752f;
+    // o[m] *= w8^m, w8 = exp(sgn*i*pi/4)
+    float2 o1 = o[1], o3 = o[3];
+    if (FWD) {
+      o[1] = make_float2(h * (o1.x + o1.y), h * (o1.y - o1.x));
+      o[3] = make_float2(h * (o3.y - o3.x), -h * (o3.x + o3.y));
+    } else {
+      o[1] = make_float2(h * (o1.x - o1.y), h * (o1.x + o1.y));
+      o[3] = make_float2(-h * (o3.x + o3.y), h * (o3.x - o3.y));
+    }
+    o[2] = isg<FWD>(o[2]);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      v[m] = cadd(e[m], o[m]);
+      v[m + 4] = csub(e[m], o[m]);
+    }
+  }
+};
+
+// odd radix: pair r with R-r
+template <int R, bool FWD> struct BflyOdd {
+  __device__ __forceinline__ static void run(float2* v) {
+    constexpr int H = (R - 1) / 2;
+    float2 sp[H], dm[H];
+    float2 v0 = v[0];
+    float2 tot = v0;
+#pragma unroll
+    for (int r = 1; r <= H; ++r) {
+      sp[r - 1] = cadd(v[r], v[R - r]);
+      dm[r - 1] = csub(v[r], v[R - r]);
+      tot = cadd(tot, sp[r - 1]);
+    }
+#pragma unroll
+    for (int m = 1; m <= H; ++m) {
+      float2 re = v0, im = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int r = 1; r <= H; ++r) {
+        const int q = (r * m) % R;
+        re = cadd(re, cscale(sp[r - 1], OddTab<R>::c[q]));
+        im = cadd(im, cscale(dm[r - 1], OddTab<R>::s[q]));
+      }
+      float2 t = isg<FWD>(im);
+      v[m] = cadd(re, t);
+      v[R - m] = csub(re, t);
+    }
+    v[0] = tot;
+  }
+};
+template <bool FWD> struct Bfly<3, FWD> : BflyOdd<3, FWD> {};
+template <bool FWD> struct Bfly<5, FWD> : BflyOdd<5, FWD> {};
+template <bool FWD> struct Bfly<7, FWD> : BflyOdd<7, FWD> {};
+
+// n / d for 0 <= n < 2^22, d >= 1, with inv = 1/d: the float quotient is off
+// by at most one, fixed by the remainder test.
+__device__ __forceinline__ int fdiv(int n, int d, float inv) {
+  int q = (int)((float)n * inv);
+  const int r = n - q * d;
+  q += (r >= d) ? 1 : 0;
+  q -= (r < 0) ? 1 : 0;
+  return q;
+}
+
+// One Stockham stage over `batch` transforms.  NT threads, at most MAXEL
+// elements per thread (batch*n <= NT*MAXEL, checked by the host planner).
+// jfast: consecutive lanes take consecutive j (use when es == 1), otherwise
+// consecutive lanes take consecutive transforms (strided transforms).
+// Not inlined: one register allocation per stage keeps the 1024-thread
+// kernels inside 128 VGPRs (inlined, the six radix bodies spill).
+template <int R, bool FWD, int NT, int MAXEL>
+__device__ __attribute__((noinline)) void fft_stage(float2* buf, int n, int Ns, int es, int batch, int bs,
+                                          bool jfast, const float2* tw, int tid) {
+  constexpr int MS = (MAXEL + R - 1) / R;
+  const int m = n / R;
+  const int total = batch * m;
+  const int tws = n / (Ns * R);
+  const int dv = jfast ? m : batch;
+  const float inv_dv = 1.0f / (float)dv;
+  const float inv_ns = 1.0f / (float)Ns;
+  float2 v[MS][R];
+  int ob[MS];
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    const int b = tid + s * NT;
+    ob[s] = -1;
+    if (b < total) {
+      const int qd = fdiv(b, dv, inv_dv);
+      const int rd = b - qd * dv;
+      const int bt = jfast ? qd : rd;
+      const int j = jfast ? rd : qd;
+      const int off = bt * bs;
+      const int k = j - fdiv(j, Ns, inv_ns) * Ns;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[s][r] = buf[off + (j + r * m) * es];
+      if (k != 0) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * k * tws];
+          if (!FWD) w.y = -w.y;
+          v[s][r] = cmul(v[s][r], w);
+        }
+      }
+      Bfly<R, FWD>::run(v[s]);
+      ob[s] = off + ((j - k) * R + k) * es;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    if (ob[s] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[ob[s] + r * Ns * es] = v[s][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full transform of length p.n (all stages).  Caller has synchronised the
+// buffer; returns synchronised.
+template <bool FWD, int NT, int MAXEL>
+__device__ void lds_fft(float2* buf, const AxisPlan& p, int es, int batch, int bs, bool jfast,
+                        const float2* tw, int tid) {
+  int Ns = 1;
+  for (int s = 0; s < p.nst; ++s) {
+    const int R = (int)((p.rad >> (4 * s)) & 15ULL);
+    switch (R) {
+      case 2: fft_stage<2, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, tid); break;
+      case 3: fft_stage<3, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, tid); break;
+      case 4: fft_stage<4, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, tid); break;
+      case 5: fft_stage<5, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, tid); break;
+      case 7: fft_stage<7, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, tid); break;
+      default: fft_stage<8, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, tid); break;
+    }
+    Ns *= R;
+  }
+}
+
+}  // namespace spk

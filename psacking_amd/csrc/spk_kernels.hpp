@@ -1,0 +1,939 @@
+// spk_kernels.hpp -- HIP kernels of the spectral-packing hot path (gfx950).
+//
+// Correlation scheme (replaces the reference's padded C2C cuFFT pipeline,
+// fft3.cu:1248-1294 and :1299-1373):
+//   tray side, once per tray state, on an FFT grid P >= N (circular
+//   correlation is exact at every in-bounds offset because q + y <= N-1):
+//     B   = tray != 0                      (collision field)
+//     Phi'= Phi - mu = b*h + l + b/2       (two-digit split, base b)
+//     G1  = FFT3(B + i*h)                  (full complex spectrum)
+//     G2  = FFT3(l)  kz in [0, Pz/2]       (real field -> half spectrum)
+//   item side, per orientation: I~ = FFT_yz(item) for the item's x-slices.
+//   pass A  (x pencils): I^ = FFT_x(I~); Y1 = G1*conj(I^)/P, Y2 = G2*conj(I^)/P;
+//           inverse FFT_x; store only the x rows that are valid offsets.
+//   pass B  (yz slabs):  inverse FFT_yz(Y2) -> l-sum, FFT_yz(Y1) -> (c, h-sum);
+//           round each to an integer, p = b*h + l + (b/2+mu)*sum(I),
+//           score = double(p) + penalty[k], workgroup argmin (score, index).
+// Each rounded part has an error far below 0.5 (its field has a small
+// range); the maximum distance to the nearest integer is reported so the
+// host can route a search to the exact direct-sum path if it ever grows.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <climits>
+
+#include "spk_fft.hpp"
+
+namespace spk {
+
+constexpr int kSlabThreads = 1024;   // yz-slab kernels
+constexpr int kSlabMaxEl = 16;       // elements per thread -> slab <= 16384 complex
+constexpr int kSlabMax = kSlabThreads * kSlabMaxEl;
+constexpr int kPencilThreads = 256;  // x-pencil kernels
+constexpr int kPencilMaxEl = 16;     // px * T <= 4096
+constexpr int kPencilMax = kPencilThreads * kPencilMaxEl;
+constexpr int kMaxAxis = 1024;       // largest FFT length along one axis
+constexpr int kDistThreads = 1024;
+
+// ---------------------------------------------------------------------------
+// small shared structs (host and device)
+// ---------------------------------------------------------------------------
+
+struct FieldStats {          // reduction of the distance field
+  unsigned long long sum;    // two's complement int64 sum
+  int mn, mx;
+};
+
+struct FieldParams {         // derived on device identically by every block
+  int mu;                    // integer centre
+  int lb;                    // log2 of the digit base
+  int ok;                    // 0: range too large for the split
+};
+
+__device__ __forceinline__ FieldParams field_params(const FieldStats* st, long long vol) {
+  FieldParams f;
+  const long long s = (long long)st->sum;
+  f.mu = (int)llrint((double)s / (double)vol);
+  const long long r1 = (long long)st->mx - f.mu, r2 = (long long)f.mu - st->mn;
+  const long long R = r1 > r2 ? r1 : r2;
+  f.ok = 1;
+  if (R <= 512) f.lb = 4;
+  else if (R <= 1024) f.lb = 5;
+  else if (R <= 4096) f.lb = 6;
+  else { f.lb = 6; f.ok = 0; }
+  return f;
+}
+
+struct OrientDesc {          // one searched orientation
+  int item_off;              // offset of the base item in the item buffer
+  int bx, by, bz;            // base item dims
+  int perm[3], flip[3];      // output axis a reads base axis perm[a]
+  int ex, ey, ez;            // oriented dims
+  int hx, hy, hz;            // oriented max index of voxels > 0 (-1 if empty)
+  int nvx, nvy, nvz;         // valid offsets per axis: N - 1 - h  (+1), clamped
+  long long isum;            // sum of item values
+};
+
+struct WgBest {              // per (orientation, x-slice) argmin record
+  double score;
+  int idx;                   // linear tray index, INT_MAX when nothing feasible
+  int p_lo;                  // low 32 bits of the proximity at idx
+  int p_hi;
+  float resid;               // max |v - rint(v)| over the slice
+  int unsupported;
+};
+
+struct SearchOut {           // written by k_select, read by verify/place
+  int found;
+  int orient;
+  int idx;
+  int pad;
+  double score;
+  long long prox;
+  float resid;
+  int unsupported;
+};
+
+__device__ __forceinline__ int oriented_index(const OrientDesc& d, int w0, int w1, int w2) {
+  int w[3] = {w0, w1, w2};
+  int ext[3] = {d.ex, d.ey, d.ez};
+  int u[3];
+  int bd[3] = {d.bx, d.by, d.bz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int ax = d.perm[a];
+    u[ax] = d.flip[a] ? (bd[ax] - 1 - w[a]) : w[a];
+    (void)ext;
+  }
+  return (u[0] * d.by + u[1]) * d.bz + u[2];
+}
+
+__device__ __forceinline__ void load_table(float2* dst, const float2* src, int n, int tid, int nt) {
+  for (int q = tid; q < n; q += nt) dst[q] = src[q];
+}
+
+// lexicographic (score, idx) minimum
+__device__ __forceinline__ bool better(double s1, int i1, double s2, int i2) {
+  return (s1 < s2) || (s1 == s2 && i1 < i2);
+}
+
+// ---------------------------------------------------------------------------
+// distance field: separable exact min-plus transforms
+//   L1 : g(k) = min_k' f(k') + |k-k'|      (reference sweeps, fft3.cu:887-942)
+//   EDT: g(k) = min_k' f(k') + (k-k')^2    (squared distances, exact ints)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ int dcost(int d, int edt) { return edt ? d * d : (d < 0 ? -d : d); }
+
+// one x-slice: init (sources) + z pass + y pass, in LDS; writes tmp int grid.
+__global__ void __launch_bounds__(kDistThreads)
+k_dist_zy(const int* __restrict__ tray, int* __restrict__ out, int nx, int ny, int nz, int edt,
+          int sources, int big) {
+  extern __shared__ int sh[];
+  const int i = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int S = ny * nz;
+  const int* src = tray + (size_t)i * S;
+  for (int e = tid; e < S; e += kDistThreads) {
+    const int v = src[e];
+    const bool s = sources ? (v != 0) : (v == 1);
+    sh[e] = s ? 0 : big;
+  }
+  __syncthreads();
+  constexpr int MAXR = 32;   // elements per thread: ny*nz <= 32768
+  int res[MAXR];
+  // z pass (rows)
+#pragma unroll
+  for (int t = 0; t < MAXR; ++t) {
+    const int e = tid + t * kDistThreads;
+    if (e < S) {
+      const int j = e / nz, k = e - j * nz;
+      const int* row = sh + j * nz;
+      int best = row[k];
+      for (int kk = 0; kk < nz; ++kk) {
+        const int c = row[kk] + dcost(k - kk, edt);
+        best = c < best ? c : best;
+      }
+      res[t] = best;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < MAXR; ++t) {
+    const int e = tid + t * kDistThreads;
+    if (e < S) sh[e] = res[t];
+  }
+  __syncthreads();
+  // y pass (columns)
+#pragma unroll
+  for (int t = 0; t < MAXR; ++t) {
+    const int e = tid + t * kDistThreads;
+    if (e < S) {
+      const int j = e / nz, k = e - j * nz;
+      int best = sh[e];
+      for (int jj = 0; jj < ny; ++jj) {
+        const int c = sh[jj * nz + k] + dcost(j - jj, edt);
+        best = c < best ? c : best;
+      }
+      res[t] = best;
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < MAXR; ++t) {
+    const int e = tid + t * kDistThreads;
+    if (e < S) out[(size_t)i * S + e] = res[t];
+  }
+}
+
+// x pass for a tile of TK consecutive k at fixed j; finalize and reduce stats.
+template <int TK>
+__global__ void __launch_bounds__(kDistThreads)
+k_dist_x(const int* __restrict__ in, int* __restrict__ phi, FieldStats* __restrict__ st, int nx,
+         int ny, int nz, int edt, int fill) {
+  extern __shared__ int sh[];
+  const int j = blockIdx.y;
+  const int k0 = blockIdx.x * TK;
+  const int tid = threadIdx.x;
+  const int tk = min(TK, nz - k0);
+  const int E = nx * TK;
+  for (int e = tid; e < E; e += kDistThreads) {
+    const int x = e / TK, t = e - x * TK;
+    if (t < tk) sh[e] = in[((size_t)x * ny + j) * nz + k0 + t];
+  }
+  __syncthreads();
+  long long lsum = 0;
+  int lmn = INT_MAX, lmx = INT_MIN;
+  for (int e = tid; e < E; e += kDistThreads) {
+    const int x = e / TK, t = e - x * TK;
+    if (t >= tk) continue;
+    int best = sh[e];
+    for (int xx = 0; xx < nx; ++xx) {
+      const int c = sh[xx * TK + t] + dcost(x - xx, edt);
+      best = c < best ? c : best;
+    }
+    int v;
+    if (edt) {
+      v = (best >= fill * fill) ? fill : (int)rint(sqrt((double)best));
+      v = v > fill ? fill : v;
+    } else {
+      v = best > fill ? fill : best;
+    }
+    phi[((size_t)x * ny + j) * nz + k0 + t] = v;
+    lsum += v;
+    lmn = v < lmn ? v : lmn;
+    lmx = v > lmx ? v : lmx;
+  }
+  // block reduction of stats
+  __shared__ long long rs[kDistThreads / 64];
+  __shared__ int rmn[kDistThreads / 64], rmx[kDistThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    lsum += __shfl_xor(lsum, o);
+    const int a = __shfl_xor(lmn, o), b = __shfl_xor(lmx, o);
+    lmn = a < lmn ? a : lmn;
+    lmx = b > lmx ? b : lmx;
+  }
+  __syncthreads();
+  if ((tid & 63) == 0) {
+    rs[tid >> 6] = lsum;
+    rmn[tid >> 6] = lmn;
+    rmx[tid >> 6] = lmx;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    long long s = 0;
+    int a = INT_MAX, b = INT_MIN;
+    for (int w = 0; w < kDistThreads / 64; ++w) {
+      s += rs[w];
+      a = rmn[w] < a ? rmn[w] : a;
+      b = rmx[w] > b ? rmx[w] : b;
+    }
+    atomicAdd(&st->sum, (unsigned long long)s);
+    atomicMin(&st->mn, a);
+    atomicMax(&st->mx, b);
+  }
+}
+
+// stats of a caller-provided distance field
+__global__ void __launch_bounds__(256)
+k_field_stats(const int* __restrict__ phi, long long n, FieldStats* __restrict__ st) {
+  long long lsum = 0;
+  int lmn = INT_MAX, lmx = INT_MIN;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
+    const int v = phi[e];
+    lsum += v;
+    lmn = v < lmn ? v : lmn;
+    lmx = v > lmx ? v : lmx;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    lsum += __shfl_xor(lsum, o);
+    const int a = __shfl_xor(lmn, o), b = __shfl_xor(lmx, o);
+    lmn = a < lmn ? a : lmn;
+    lmx = b > lmx ? b : lmx;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&st->sum, (unsigned long long)lsum);
+    atomicMin(&st->mn, lmn);
+    atomicMax(&st->mx, lmx);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// tray spectra
+// ---------------------------------------------------------------------------
+
+// grid (nx, 2): field 0 -> slab of B + i*h, full kz; field 1 -> slab of l,
+// only kz < hz kept.  Forward 2D FFT (z rows < ny, then y columns).
+__global__ void __launch_bounds__(kSlabThreads)
+k_tray_slab_fwd(const int* __restrict__ tray, const int* __restrict__ phi,
+                const FieldStats* __restrict__ st, float2* __restrict__ t1,
+                float2* __restrict__ t2, int nx, int ny, int nz, AxisPlan py_, AxisPlan pz_,
+                const float2* __restrict__ tw_y, const float2* __restrict__ tw_z) {
+  extern __shared__ float2 smem[];
+  const int py = py_.n, pz = pz_.n, hz = pz / 2 + 1;
+  float2* twy = smem;
+  float2* twz = twy + py;
+  float2* buf = twz + pz;
+  const int x = blockIdx.x, field = blockIdx.y, tid = threadIdx.x;
+  load_table(twy, tw_y, py, tid, kSlabThreads);
+  load_table(twz, tw_z, pz, tid, kSlabThreads);
+  const FieldParams fp = field_params(st, (long long)nx * ny * nz);
+  const int b = 1 << fp.lb, half = b >> 1;
+  const size_t so = (size_t)x * ny * nz;
+  for (int e = tid; e < py * pz; e += kSlabThreads) {
+    const int j = e / pz, k = e - j * pz;
+    float2 v = make_float2(0.f, 0.f);
+    if (j < ny && k < nz) {
+      const int d = phi[so + j * nz + k] - fp.mu;
+      const int h = d >> fp.lb;            // floor division
+      if (field == 0) {
+        v.x = tray[so + j * nz + k] != 0 ? 1.f : 0.f;
+        v.y = (float)h;
+      } else {
+        v.x = (float)(d - (h << fp.lb) - half);
+      }
+    }
+    buf[e] = v;
+  }
+  __syncthreads();
+  lds_fft<true, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, ny, pz, true, twz, tid);
+  const int ncol = field == 0 ? pz : hz;
+  lds_fft<true, kSlabThreads, kSlabMaxEl>(buf, py_, pz, ncol, 1, false, twy, tid);
+  if (field == 0) {
+    float2* dst = t1 + (size_t)x * py * pz;
+    for (int e = tid; e < py * pz; e += kSlabThreads) dst[e] = buf[e];
+  } else {
+    float2* dst = t2 + (size_t)x * py * hz;
+    for (int e = tid; e < py * hz; e += kSlabThreads) {
+      const int j = e / hz, k = e - j * hz;
+      dst[e] = buf[j * pz + k];
+    }
+  }
+}
+
+// grid (ceil(ncols/T), py, 2): forward FFT along x of the slab spectra
+// (rows x >= nx are zero), writes G[kx][ky][kz].
+__global__ void __launch_bounds__(kPencilThreads)
+k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
+             float2* __restrict__ g1, float2* __restrict__ g2, int nx, AxisPlan px_, int py,
+             int pz, int T, const float2* __restrict__ tw_x) {
+  extern __shared__ float2 smem[];
+  const int px = px_.n, hz = pz / 2 + 1;
+  float2* twx = smem;
+  float2* buf = twx + px;
+  const int field = blockIdx.z, ky = blockIdx.y, tid = threadIdx.x;
+  const int ncol = field == 0 ? pz : hz;
+  const int kz0 = blockIdx.x * T;
+  if (kz0 >= ncol) return;
+  load_table(twx, tw_x, px, tid, kPencilThreads);
+  const float2* src = field == 0 ? t1 : t2;
+  float2* dst = field == 0 ? g1 : g2;
+  const size_t rs = (size_t)py * ncol;   // x stride
+  for (int e = tid; e < px * T; e += kPencilThreads) {
+    const int x = e / T, t = e - x * T;
+    const int kz = kz0 + t;
+    float2 v = make_float2(0.f, 0.f);
+    if (x < nx && kz < ncol) v = src[x * rs + (size_t)ky * ncol + kz];
+    buf[e] = v;
+  }
+  __syncthreads();
+  lds_fft<true, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
+  for (int e = tid; e < px * T; e += kPencilThreads) {
+    const int x = e / T, t = e - x * T;
+    const int kz = kz0 + t;
+    if (kz < ncol) dst[x * rs + (size_t)ky * ncol + kz] = buf[e];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// item spectra: grid (max_ex, n_orient).  I~[o][x][ky][kz] = FFT_yz(item_o[x])
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kSlabThreads)
+k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od,
+                float2* __restrict__ ispec, int max_ex, AxisPlan py_, AxisPlan pz_,
+                const float2* __restrict__ tw_y, const float2* __restrict__ tw_z) {
+  extern __shared__ float2 smem[];
+  const int py = py_.n, pz = pz_.n;
+  const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
+  const OrientDesc d = od[o];
+  if (x >= d.ex) return;
+  float2* twy = smem;
+  float2* twz = twy + py;
+  float2* buf = twz + pz;
+  load_table(twy, tw_y, py, tid, kSlabThreads);
+  load_table(twz, tw_z, pz, tid, kSlabThreads);
+  const int* base = items + d.item_off;
+  for (int e = tid; e < py * pz; e += kSlabThreads) {
+    const int j = e / pz, k = e - j * pz;
+    float2 v = make_float2(0.f, 0.f);
+    if (j < d.ey && k < d.ez) v.x = (float)base[oriented_index(d, x, j, k)];
+    buf[e] = v;
+  }
+  __syncthreads();
+  lds_fft<true, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, d.ey, pz, true, twz, tid);
+  lds_fft<true, kSlabThreads, kSlabMaxEl>(buf, py_, pz, pz, 1, false, twy, tid);
+  float2* dst = ispec + ((size_t)o * max_ex + x) * py * pz;
+  for (int e = tid; e < py * pz; e += kSlabThreads) dst[e] = buf[e];
+}
+
+// ---------------------------------------------------------------------------
+// pass A: grid (ceil(pz/T), py, n_chunks).  x pencils of T consecutive kz.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kPencilThreads)
+k_corr_x(const float2* __restrict__ g1, const float2* __restrict__ g2,
+         const float2* __restrict__ ispec, const OrientDesc* __restrict__ od,
+         float2* __restrict__ y1, float2* __restrict__ y2, int n_orient, int chunk, int max_ex,
+         int nx, AxisPlan px_, int py, int pz, int T, float scale, const float2* __restrict__ tw_x) {
+  extern __shared__ float2 smem[];
+  const int px = px_.n, hz = pz / 2 + 1;
+  float2* twx = smem;
+  float2* buf = twx + px;
+  const int ky = blockIdx.y, tid = threadIdx.x;
+  const int kz0 = blockIdx.x * T;
+  const int o0 = blockIdx.z * chunk;
+  const int o1 = min(n_orient, o0 + chunk);
+  load_table(twx, tw_x, px, tid, kPencilThreads);
+  const bool has2 = kz0 < hz;
+  float2 G1[kPencilMaxEl], G2[kPencilMaxEl];
+#pragma unroll
+  for (int s = 0; s < kPencilMaxEl; ++s) {
+    const int e = tid + s * kPencilThreads;
+    const int x = e / T, t = e - x * T;
+    const int kz = kz0 + t;
+    G1[s] = make_float2(0.f, 0.f);
+    G2[s] = make_float2(0.f, 0.f);
+    if (e < px * T && kz < pz) {
+      G1[s] = g1[((size_t)x * py + ky) * pz + kz];
+      if (kz < hz) G2[s] = g2[((size_t)x * py + ky) * hz + kz];
+    }
+  }
+  for (int o = o0; o < o1; ++o) {
+    const OrientDesc d = od[o];
+    const float2* is = ispec + (size_t)o * max_ex * py * pz;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kPencilMaxEl; ++s) {
+      const int e = tid + s * kPencilThreads;
+      if (e < px * T) {
+        const int x = e / T, t = e - x * T;
+        const int kz = kz0 + t;
+        float2 v = make_float2(0.f, 0.f);
+        if (x < d.ex && kz < pz) v = is[((size_t)x * py + ky) * pz + kz];
+        buf[e] = v;
+      }
+    }
+    __syncthreads();
+    lds_fft<true, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
+    float2 I[kPencilMaxEl];
+#pragma unroll
+    for (int s = 0; s < kPencilMaxEl; ++s) {
+      const int e = tid + s * kPencilThreads;
+      I[s] = e < px * T ? buf[e] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kPencilMaxEl; ++s) {
+      const int e = tid + s * kPencilThreads;
+      if (e < px * T) buf[e] = cscale(cmulc(G1[s], I[s]), scale);
+    }
+    __syncthreads();
+    lds_fft<false, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
+    float2* d1 = y1 + (size_t)o * nx * py * pz;
+    for (int e = tid; e < d.nvx * T; e += kPencilThreads) {
+      const int x = e / T, t = e - x * T;
+      const int kz = kz0 + t;
+      if (kz < pz) d1[((size_t)x * py + ky) * pz + kz] = buf[e];
+    }
+    if (has2) {
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < kPencilMaxEl; ++s) {
+        const int e = tid + s * kPencilThreads;
+        if (e < px * T) buf[e] = cscale(cmulc(G2[s], I[s]), scale);
+      }
+      __syncthreads();
+      lds_fft<false, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
+      float2* d2 = y2 + (size_t)o * nx * py * hz;
+      for (int e = tid; e < d.nvx * T; e += kPencilThreads) {
+        const int x = e / T, t = e - x * T;
+        const int kz = kz0 + t;
+        if (kz < hz) d2[((size_t)x * py + ky) * hz + kz] = buf[e];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pass B: grid (nx, n_orient).  Inverse yz, round, score, argmin.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void wg_argmin_write(double sc, int idx, long long p, float res,
+                                                int unsup, WgBest* out, int tid, int nt) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const double s2 = __shfl_xor(sc, o);
+    const int i2 = __shfl_xor(idx, o);
+    const long long p2 = __shfl_xor(p, o);
+    const float r2 = __shfl_xor(res, o);
+    if (better(s2, i2, sc, idx)) { sc = s2; idx = i2; p = p2; }
+    res = r2 > res ? r2 : res;
+  }
+  __shared__ double ws[16];
+  __shared__ int wi[16];
+  __shared__ long long wp[16];
+  __shared__ float wr[16];
+  __syncthreads();
+  if ((tid & 63) == 0) {
+    ws[tid >> 6] = sc; wi[tid >> 6] = idx; wp[tid >> 6] = p; wr[tid >> 6] = res;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < nt / 64; ++w) {
+      if (better(ws[w], wi[w], sc, idx)) { sc = ws[w]; idx = wi[w]; p = wp[w]; }
+      res = wr[w] > res ? wr[w] : res;
+    }
+    WgBest b;
+    b.score = sc;
+    b.idx = idx;
+    b.p_lo = (int)(p & 0xffffffffLL);
+    b.p_hi = (int)(p >> 32);
+    b.resid = res;
+    b.unsupported = unsup;
+    *out = b;
+  }
+}
+
+__global__ void __launch_bounds__(kSlabThreads)
+k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
+           const OrientDesc* __restrict__ od, const FieldStats* __restrict__ st,
+           const double* __restrict__ pen, WgBest* __restrict__ rec, int nx, int ny, int nz,
+           AxisPlan py_, AxisPlan pz_, const float2* __restrict__ tw_y,
+           const float2* __restrict__ tw_z, unsigned char* __restrict__ fgrid,
+           long long* __restrict__ pgrid) {
+  extern __shared__ float2 smem[];
+  const int py = py_.n, pz = pz_.n, hz = pz / 2 + 1;
+  const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
+  const OrientDesc d = od[o];
+  WgBest* out = rec + (size_t)o * nx + x;
+  if (x >= d.nvx) return;
+  float2* twy = smem;
+  float2* twz = twy + py;
+  float2* buf = twz + pz;
+  load_table(twy, tw_y, py, tid, kSlabThreads);
+  load_table(twz, tw_z, pz, tid, kSlabThreads);
+  const FieldParams fp = field_params(st, (long long)nx * ny * nz);
+  const long long b = 1LL << fp.lb;
+  const long long addback = (b / 2 + fp.mu) * d.isum;
+  // --- low digit (real field, half spectrum) ---
+  const float2* s2 = y2 + ((size_t)o * nx + x) * py * hz;
+  for (int e = tid; e < py * hz; e += kSlabThreads) {
+    const int j = e / hz, k = e - j * hz;
+    buf[j * pz + k] = s2[e];
+  }
+  __syncthreads();
+  lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, py_, pz, hz, 1, false, twy, tid);
+  const int nvy = d.nvy, nvz = d.nvz;
+  for (int e = tid; e < nvy * (pz - hz); e += kSlabThreads) {
+    const int j = e / (pz - hz), k = hz + (e - j * (pz - hz));
+    const float2 v = buf[j * pz + (pz - k)];
+    buf[j * pz + k] = make_float2(v.x, -v.y);
+  }
+  __syncthreads();
+  lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, nvy, pz, true, twz, tid);
+  float res = 0.f;
+  int lo[kSlabMaxEl];
+#pragma unroll
+  for (int s = 0; s < kSlabMaxEl; ++s) {
+    const int e = tid + s * kSlabThreads;
+    const int j = e / pz, k = e - j * pz;
+    lo[s] = 0;
+    if (j < nvy && k < nvz) {
+      const float v = buf[e].x;
+      const float r = rintf(v);
+      res = fmaxf(res, fabsf(v - r));
+      lo[s] = (int)r;
+    }
+  }
+  __syncthreads();
+  // --- collision + high digit (complex field, full spectrum) ---
+  const float2* s1 = y1 + ((size_t)o * nx + x) * py * pz;
+  for (int e = tid; e < py * pz; e += kSlabThreads) buf[e] = s1[e];
+  __syncthreads();
+  lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, py_, pz, pz, 1, false, twy, tid);
+  lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, nvy, pz, true, twz, tid);
+  double best = 1e300;
+  int bidx = INT_MAX;
+  long long bp = 0;
+#pragma unroll
+  for (int s = 0; s < kSlabMaxEl; ++s) {
+    const int e = tid + s * kSlabThreads;
+    const int j = e / pz, k = e - j * pz;
+    if (j < nvy && k < nvz) {
+      const float2 v = buf[e];
+      const float c = rintf(v.x), h = rintf(v.y);
+      res = fmaxf(res, fmaxf(fabsf(v.x - c), fabsf(v.y - h)));
+      const long long p = b * (long long)h + lo[s] + addback;
+      const int idx = (x * ny + j) * nz + k;
+      if (fgrid) {
+        const size_t g = (size_t)o * nx * ny * nz + idx;
+        fgrid[g] = c == 0.f ? 1 : 0;
+        pgrid[g] = p;
+      }
+      if (c == 0.f) {
+        const double sc = (double)p + pen[k];
+        if (better(sc, idx, best, bidx)) { best = sc; bidx = idx; bp = p; }
+      }
+    }
+  }
+  wg_argmin_write(best, bidx, bp, res, fp.ok ? 0 : 1, out, tid, kSlabThreads);
+}
+
+// ---------------------------------------------------------------------------
+// selection: one block.  Per orientation min over its x records, then the
+// first strictly smaller score across orientations.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_select(const WgBest* __restrict__ rec, const OrientDesc* __restrict__ od, int n_orient,
+         int nx, WgBest* __restrict__ per_orient, SearchOut* __restrict__ out) {
+  const int tid = threadIdx.x;
+  for (int o = 0; o < n_orient; ++o) {
+    const int nvx = od[o].nvx;
+    double sc = 1e300;
+    int idx = INT_MAX;
+    long long p = 0;
+    float res = 0.f;
+    int uns = 0;
+    for (int x = tid; x < nvx; x += 256) {
+      const WgBest r = rec[(size_t)o * nx + x];
+      if (better(r.score, r.idx, sc, idx)) {
+        sc = r.score; idx = r.idx;
+        p = ((long long)r.p_hi << 32) | (unsigned int)r.p_lo;
+      }
+      res = fmaxf(res, r.resid);
+      uns |= r.unsupported;
+    }
+    for (int s = 32; s > 0; s >>= 1) {
+      const double s2 = __shfl_xor(sc, s);
+      const int i2 = __shfl_xor(idx, s);
+      const long long p2 = __shfl_xor(p, s);
+      const float r2 = __shfl_xor(res, s);
+      const int u2 = __shfl_xor(uns, s);
+      if (better(s2, i2, sc, idx)) { sc = s2; idx = i2; p = p2; }
+      res = fmaxf(res, r2);
+      uns |= u2;
+    }
+    __shared__ double ws[4];
+    __shared__ int wi[4], wu[4];
+    __shared__ long long wp[4];
+    __shared__ float wr[4];
+    __syncthreads();
+    if ((tid & 63) == 0) { ws[tid >> 6] = sc; wi[tid >> 6] = idx; wp[tid >> 6] = p; wr[tid >> 6] = res; wu[tid >> 6] = uns; }
+    __syncthreads();
+    if (tid == 0) {
+      for (int w = 1; w < 4; ++w) {
+        if (better(ws[w], wi[w], sc, idx)) { sc = ws[w]; idx = wi[w]; p = wp[w]; }
+        res = fmaxf(res, wr[w]);
+        uns |= wu[w];
+      }
+      WgBest b;
+      b.score = sc; b.idx = idx;
+      b.p_lo = (int)(p & 0xffffffffLL); b.p_hi = (int)(p >> 32);
+      b.resid = res; b.unsupported = uns;
+      per_orient[o] = b;
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    SearchOut r;
+    r.found = 0; r.orient = -1; r.idx = -1; r.pad = 0; r.score = 1e30; r.prox = 0;
+    r.resid = 0.f; r.unsupported = 0;
+    for (int o = 0; o < n_orient; ++o) {
+      const WgBest b = per_orient[o];
+      r.resid = fmaxf(r.resid, b.resid);
+      r.unsupported |= b.unsupported;
+      if (b.idx != INT_MAX && b.score < r.score) {
+        r.found = 1; r.orient = o; r.idx = b.idx; r.score = b.score;
+        r.prox = ((long long)b.p_hi << 32) | (unsigned int)b.p_lo;
+      }
+    }
+    if (!r.found) r.score = 0.0;
+    *out = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exact verification of the selected offset: one block, direct sums.
+// ver[0] = raw collision sum (tray * item), ver[1] = overlap count
+// (tray != 0 & item > 0... item value weighted), ver[2] = proximity.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_verify(const int* __restrict__ tray, const int* __restrict__ phi, const int* __restrict__ items,
+         const OrientDesc* __restrict__ od, const SearchOut* __restrict__ so,
+         long long* __restrict__ ver, int ny, int nz) {
+  const SearchOut s = *so;
+  const int tid = threadIdx.x;
+  long long c = 0, ov = 0, p = 0;
+  if (s.found) {
+    const OrientDesc d = od[s.orient];
+    const int qz = s.idx % nz, qy = (s.idx / nz) % ny, qx = s.idx / (ny * nz);
+    const int* base = items + d.item_off;
+    const int E = d.ex * d.ey * d.ez;
+    for (int e = tid; e < E; e += 256) {
+      const int w2 = e % d.ez, w1 = (e / d.ez) % d.ey, w0 = e / (d.ey * d.ez);
+      const int v = base[oriented_index(d, w0, w1, w2)];
+      if (v == 0) continue;
+      const size_t ti = ((size_t)(qx + w0) * ny + (qy + w1)) * nz + (qz + w2);
+      const int t = tray[ti];
+      c += (long long)t * v;
+      ov += (t != 0) ? (long long)v : 0;
+      p += (long long)phi[ti] * v;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    c += __shfl_xor(c, o);
+    ov += __shfl_xor(ov, o);
+    p += __shfl_xor(p, o);
+  }
+  __shared__ long long r[3][4];
+  if ((tid & 63) == 0) { r[0][tid >> 6] = c; r[1][tid >> 6] = ov; r[2][tid >> 6] = p; }
+  __syncthreads();
+  if (tid == 0) {
+    ver[0] = r[0][0] + r[0][1] + r[0][2] + r[0][3];
+    ver[1] = r[1][0] + r[1][1] + r[1][2] + r[1][3];
+    ver[2] = r[2][0] + r[2][1] + r[2][2] + r[2][3];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// placement of an oriented item at an offset (from host args or SearchOut)
+// phase 0: detect overlap (flag), phase 1: write id where tray == 0.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+k_place(int* __restrict__ tray, const int* __restrict__ items, const OrientDesc* __restrict__ od,
+        const SearchOut* __restrict__ so, int orient, int qx, int qy, int qz, int id, int nx,
+        int ny, int nz, int phase, int* __restrict__ flag) {
+  OrientDesc d;
+  if (so) {
+    const SearchOut s = *so;
+    if (!s.found) return;
+    d = od[s.orient];
+    qz = s.idx % nz; qy = (s.idx / nz) % ny; qx = s.idx / (ny * nz);
+  } else {
+    d = od[orient];
+  }
+  if (phase == 1 && *flag) return;
+  const int* base = items + d.item_off;
+  const int E = d.ex * d.ey * d.ez;
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < E; e += gridDim.x * 256) {
+    const int w2 = e % d.ez, w1 = (e / d.ez) % d.ey, w0 = e / (d.ey * d.ez);
+    const int X = qx + w0, Y = qy + w1, Z = qz + w2;
+    if (X < 0 || Y < 0 || Z < 0 || X >= nx || Y >= ny || Z >= nz) continue;
+    const int v = base[oriented_index(d, w0, w1, w2)];
+    if (v <= 0) continue;
+    int* t = tray + ((size_t)X * ny + Y) * nz + Z;
+    if (phase == 0) {
+      if (*t > 0) atomicOr(flag, 1);
+    } else if (*t == 0) {
+      *t = id;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exact direct-sum kernels (fallback search, standalone full-grid ops)
+// item given as a list of (dx, dy, dz, value) non-zero voxels.
+// ---------------------------------------------------------------------------
+struct Tap { int dx, dy, dz, v; };
+
+// mode 0: out[q] = sum a[q+y] v ; mode 1 (conv): out[q] = sum a[q-y] v
+__global__ void __launch_bounds__(256)
+k_direct_corr(const int* __restrict__ a, const Tap* __restrict__ taps, int ntap,
+              long long* __restrict__ out, int nx, int ny, int nz, int conv) {
+  const long long V = (long long)nx * ny * nz;
+  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < V; q += (long long)gridDim.x * 256) {
+    const int k = (int)(q % nz), j = (int)((q / nz) % ny), i = (int)(q / ((long long)ny * nz));
+    long long s = 0;
+    for (int t = 0; t < ntap; ++t) {
+      const Tap tp = taps[t];
+      const int X = conv ? i - tp.dx : i + tp.dx;
+      const int Y = conv ? j - tp.dy : j + tp.dy;
+      const int Z = conv ? k - tp.dz : k + tp.dz;
+      if (X < 0 || Y < 0 || Z < 0 || X >= nx || Y >= ny || Z >= nz) continue;
+      s += (long long)a[((size_t)X * ny + Y) * nz + Z] * tp.v;
+    }
+    out[q] = s;
+  }
+}
+
+// exact search over all in-bounds offsets of one orientation; per-block record
+__global__ void __launch_bounds__(256)
+k_direct_search(const int* __restrict__ tray, const int* __restrict__ phi,
+                const Tap* __restrict__ taps, int ntap, int nvx, int nvy, int nvz, int ny,
+                int nz, const double* __restrict__ pen, WgBest* __restrict__ rec) {
+  const long long V = (long long)nvx * nvy * nvz;
+  double best = 1e300;
+  int bidx = INT_MAX;
+  long long bp = 0;
+  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < V; q += (long long)gridDim.x * 256) {
+    const int k = (int)(q % nvz), j = (int)((q / nvz) % nvy), i = (int)(q / ((long long)nvy * nvz));
+    long long ov = 0, p = 0;
+    for (int t = 0; t < ntap; ++t) {
+      const Tap tp = taps[t];
+      const size_t ti = ((size_t)(i + tp.dx) * ny + (j + tp.dy)) * nz + (k + tp.dz);
+      const int tv = tray[ti];
+      ov += (tv != 0) ? 1 : 0;
+      p += (long long)phi[ti] * tp.v;
+    }
+    if (ov == 0) {
+      const double sc = (double)p + pen[k];
+      const int idx = (i * ny + j) * nz + k;
+      if (better(sc, idx, best, bidx)) { best = sc; bidx = idx; bp = p; }
+    }
+  }
+  wg_argmin_write(best, bidx, bp, 0.f, 0, rec + blockIdx.x, threadIdx.x, 256);
+}
+
+// exact feasibility / proximity grids of one orientation over its valid
+// offsets (interlocking-free fallback); cells outside stay as cleared.
+__global__ void __launch_bounds__(256)
+k_direct_grid(const int* __restrict__ tray, const int* __restrict__ phi,
+              const Tap* __restrict__ taps, int ntap, int nvx, int nvy, int nvz, int ny, int nz,
+              unsigned char* __restrict__ fgrid, long long* __restrict__ pgrid) {
+  const long long V = (long long)nvx * nvy * nvz;
+  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < V; q += (long long)gridDim.x * 256) {
+    const int k = (int)(q % nvz), j = (int)((q / nvz) % nvy), i = (int)(q / ((long long)nvy * nvz));
+    long long ov = 0, p = 0;
+    for (int t = 0; t < ntap; ++t) {
+      const Tap tp = taps[t];
+      const size_t ti = ((size_t)(i + tp.dx) * ny + (j + tp.dy)) * nz + (k + tp.dz);
+      ov += tray[ti] != 0 ? 1 : 0;
+      p += (long long)phi[ti] * tp.v;
+    }
+    const size_t g = ((size_t)i * ny + j) * nz + k;
+    fgrid[g] = ov == 0 ? 1 : 0;
+    pgrid[g] = p;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// interlocking-free flood fill on a feasibility grid
+// free[q] = 1 for feasible offsets; reached grows from seed faces to fixpoint
+// ---------------------------------------------------------------------------
+// grid (blocks, m): seeds of grid o are its free cells on i|j|k == 0 or on the
+// exit faces exits[3o..3o+2] (N-1-h, fft3.cu:276-287).
+__global__ void __launch_bounds__(256)
+k_ff_seed(const unsigned char* __restrict__ fr, unsigned char* __restrict__ reached,
+          const int* __restrict__ exits, int nx, int ny, int nz) {
+  const long long V = (long long)nx * ny * nz;
+  const int o = blockIdx.y;
+  const int ex = exits[3 * o], ey = exits[3 * o + 1], ez = exits[3 * o + 2];
+  const unsigned char* f = fr + (size_t)o * V;
+  unsigned char* r = reached + (size_t)o * V;
+  for (long long q = blockIdx.x * 256LL + threadIdx.x; q < V; q += (long long)gridDim.x * 256) {
+    const int k = (int)(q % nz), j = (int)((q / nz) % ny), i = (int)(q / ((long long)ny * nz));
+    const bool face = i == 0 || j == 0 || k == 0 || i == ex || j == ey || k == ez;
+    r[q] = (f[q] && face) ? 1 : 0;
+  }
+}
+
+// tile 4 x 8 x 32 with a one-cell halo; local fixpoint in LDS.
+// grid (ceil(nz/32), ceil(ny/8), ceil(nx/4) * m): the z index also selects the grid o.
+constexpr int FFX = 4, FFY = 8, FFZ = 32;
+__global__ void __launch_bounds__(FFX * FFY * FFZ)
+k_ff_tile(const unsigned char* __restrict__ fr_all, unsigned char* __restrict__ reached_all,
+          int nx, int ny, int nz, int xtiles, int* __restrict__ changed) {
+  constexpr int HX = FFX + 2, HY = FFY + 2, HZ = FFZ + 2;
+  __shared__ unsigned char sf[HX * HY * HZ], sr[HX * HY * HZ];
+  __shared__ int gch;
+  const int tid = threadIdx.x;
+  const int tz = tid % FFZ, ty = (tid / FFZ) % FFY, tx = tid / (FFZ * FFY);
+  const int o = blockIdx.z / xtiles;
+  const size_t V = (size_t)nx * ny * nz;
+  const unsigned char* fr = fr_all + o * V;
+  unsigned char* reached = reached_all + o * V;
+  const int bx = (blockIdx.z - o * xtiles) * FFX, by = blockIdx.y * FFY, bz = blockIdx.x * FFZ;
+  for (int e = tid; e < HX * HY * HZ; e += FFX * FFY * FFZ) {
+    const int hz_ = e % HZ, hy_ = (e / HZ) % HY, hx_ = e / (HZ * HY);
+    const int X = bx + hx_ - 1, Y = by + hy_ - 1, Z = bz + hz_ - 1;
+    unsigned char f = 0, r = 0;
+    if (X >= 0 && Y >= 0 && Z >= 0 && X < nx && Y < ny && Z < nz) {
+      const size_t q = ((size_t)X * ny + Y) * nz + Z;
+      f = fr[q];
+      r = reached[q];
+    }
+    sf[e] = f;
+    sr[e] = r;
+  }
+  if (tid == 0) gch = 0;
+  __syncthreads();
+  const int c = ((tx + 1) * HY + (ty + 1)) * HZ + (tz + 1);
+  const bool inside = (bx + tx < nx) && (by + ty < ny) && (bz + tz < nz);
+  bool mine = false;
+  for (int it = 0; it < 4096; ++it) {
+    int upd = 0;
+    if (inside && sf[c] && !sr[c]) {
+      if (sr[c - 1] | sr[c + 1] | sr[c - HZ] | sr[c + HZ] | sr[c - HY * HZ] | sr[c + HY * HZ]) {
+        sr[c] = 1;
+        mine = true;
+        upd = 1;
+      }
+    }
+    if (!__syncthreads_or(upd)) break;
+  }
+  if (inside && mine) {
+    reached[((size_t)(bx + tx) * ny + (by + ty)) * nz + (bz + tz)] = 1;
+    gch = 1;
+  }
+  __syncthreads();
+  if (tid == 0 && gch) atomicOr(changed, 1);
+}
+
+// grid (nx, m): argmin of double(p) + penalty[k] over reached offsets of slice x;
+// keeps the residual/unsupported fields the score kernel wrote in the record.
+__global__ void __launch_bounds__(256)
+k_masked_argmin(const unsigned char* __restrict__ reached, const long long* __restrict__ pgrid,
+                const OrientDesc* __restrict__ od, const double* __restrict__ pen,
+                WgBest* __restrict__ rec, int nx, int ny, int nz) {
+  const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
+  const OrientDesc d = od[o];
+  if (x >= d.nvx) return;
+  const size_t base = (size_t)o * nx * ny * nz + (size_t)x * ny * nz;
+  double best = 1e300;
+  int bidx = INT_MAX;
+  long long bp = 0;
+  const int S = d.nvy * d.nvz;
+  for (int e = tid; e < S; e += 256) {
+    const int j = e / d.nvz, k = e - j * d.nvz;
+    const int q = j * nz + k;
+    if (!reached[base + q]) continue;
+    const long long p = pgrid[base + q];
+    const double sc = (double)p + pen[k];
+    const int idx = x * ny * nz + q;
+    if (better(sc, idx, best, bidx)) { best = sc; bidx = idx; bp = p; }
+  }
+  WgBest* out = rec + (size_t)o * nx + x;
+  __shared__ float r0;
+  __shared__ int u0;
+  if (tid == 0) { r0 = out->resid; u0 = out->unsupported; }
+  __syncthreads();
+  wg_argmin_write(best, bidx, bp, r0, u0, out, tid, 256);
+}
+
+}  // namespace spk

@@ -1,0 +1,266 @@
+"""GPU parity: every _core entry point and the packer against the CPU oracle.
+
+Integer outputs must be bit-exact; scores are doubles computed from exact
+integers and must be equal.  Each test prints the case it runs first, so a
+failing run names it.
+"""
+import numpy as np
+import pytest
+
+from oracle import spk_oracle as O
+from tests.conftest import grid_from_rows, piece
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def P():
+    import psacking_amd
+    return psacking_amd
+
+
+def rand_tray(rng, N, p=0.3, ids=3):
+    return (rng.integers(1, ids + 1, N) * (rng.random(N) < p)).astype(np.int32)
+
+
+def test_distance_kats(P, kats):
+    say("distance kats")
+    for case in kats["calculate_distance"]:
+        g = np.zeros(case["shape"], dtype=np.int32)
+        for p in case["ones"]:
+            g[tuple(p)] = 1
+        np.testing.assert_array_equal(P.calculate_distance(g), np.array(case["expected"]))
+
+
+@pytest.mark.parametrize("metric,sources", [("l1", "eq1"), ("l1", "nonzero"), ("edt", "eq1"),
+                                            ("edt", "nonzero")])
+def test_distance_random(P, metric, sources):
+    rng = np.random.default_rng(1)
+    for N in [(1, 1, 1), (5, 1, 9), (17, 13, 11), (64, 64, 64), (100, 37, 53), (240, 123, 100)]:
+        say("distance", metric, sources, N)
+        g = rand_tray(rng, N, p=0.02)
+        np.testing.assert_array_equal(P.distance_field(g, metric, sources),
+                                      O.distance_field(g, metric, sources))
+    g = np.zeros((7, 8, 9), np.int32)
+    np.testing.assert_array_equal(P.distance_field(g, metric, sources), O.distance_field(g, metric, sources))
+
+
+def test_correlation_kats(P, kats):
+    say("correlation kats")
+    for case in kats["dft_conv3"]:
+        np.testing.assert_array_equal(P.dft_conv3(np.array(case["a"]), np.array(case["b"])),
+                                      np.array(case["expected"]))
+    for case in kats["dft_corr3"]:
+        np.testing.assert_array_equal(P.dft_corr3(np.array(case["a"]), np.array(case["b"])),
+                                      np.array(case["expected"]))
+    with pytest.raises(RuntimeError):
+        P.dft_corr3(np.ones((2, 2, 2)), np.ones((2, 2, 3)))
+
+
+def test_correlation_random(P):
+    rng = np.random.default_rng(2)
+    for N in [(3, 4, 5), (16, 9, 21)]:
+        say("correlation", N)
+        a = rng.integers(-5, 9, N).astype(np.int32)
+        b = rng.integers(-3, 4, N).astype(np.int32) * (rng.random(N) < 0.3)
+        np.testing.assert_array_equal(P.dft_corr3(a, b), O.dft_corr3(a, b))
+        np.testing.assert_array_equal(P.dft_conv3(a, b), O.dft_conv3(a, b))
+
+
+def test_collision_grid(P):
+    rng = np.random.default_rng(3)
+    for N, I in [((10, 10, 10), (3, 3, 3)), ((20, 15, 12), (4, 6, 2)), ((8, 8, 8), (2, 9, 1))]:
+        say("collision_grid", N, I)
+        tray = rand_tray(rng, N, 0.2, 5)
+        item = (rng.random(I) < 0.7).astype(np.int32)
+        np.testing.assert_array_equal(P.collision_grid(tray, item), O.collision_grid(tray, item))
+
+
+def test_interlocking_mask(P):
+    rng = np.random.default_rng(4)
+    for N, I in [((12, 11, 10), (2, 2, 2)), ((20, 20, 20), (3, 1, 2)), ((9, 9, 9), (1, 1, 1))]:
+        say("interlock", N, I)
+        tray = (rng.random(N) < 0.3).astype(np.int32)
+        obj = (rng.random(I) < 0.8).astype(np.int32)
+        np.testing.assert_array_equal(P.interlocking_free_positions(tray, obj),
+                                      O.interlocking_free_positions(tray, obj))
+    cage = np.zeros((15, 15, 15), np.int32)
+    cage[3:12, 3:12, 3:12] = 1
+    cage[4:11, 4:11, 4:11] = 0
+    say("interlock cage")
+    m = P.interlocking_free_positions(cage, np.ones((1, 1, 1), np.int32))
+    np.testing.assert_array_equal(m, O.interlocking_free_positions(cage, np.ones((1, 1, 1))))
+    assert m[7, 7, 7] == 0 and m[0, 0, 0] == 1
+
+
+def test_place_in_tray(P):
+    say("place_in_tray")
+    rng = np.random.default_rng(5)
+    tray = rand_tray(rng, (10, 11, 12), 0.1)
+    item = (rng.random((3, 4, 2)) < 0.6).astype(np.int32)
+    for pos in [(0, 0, 0), (8, 9, 11), (-1, 2, 3), (5, 5, 5)]:
+        try:
+            exp = O.place_in_tray(item, tray, pos, 9)
+        except RuntimeError:
+            with pytest.raises(RuntimeError):
+                P.place_in_tray(item, tray, pos, 9)
+            continue
+        np.testing.assert_array_equal(P.place_in_tray(item, tray, pos, 9), exp)
+    np.testing.assert_array_equal(P.place_in_tray(item, np.zeros_like(tray), (1, 1, 1)),
+                                  O.place_in_tray(item, np.zeros_like(tray), (1, 1, 1), 1))
+
+
+def test_search_single_random(P):
+    rng = np.random.default_rng(6)
+    for t in range(12):
+        N = tuple(int(v) for v in rng.integers(4, 30, 3))
+        I = tuple(int(min(v, n)) for v, n in zip(rng.integers(1, 8, 3), N))
+        tray = rand_tray(rng, N, 0.25, 3)
+        item = (rng.random(I) < 0.7).astype(np.int32)
+        say("fft_search_placement", t, N, I)
+        r = O.search(item, tray, penalty="pow")
+        got = P.fft_search_placement(item, tray)
+        assert got == (r.position, r.found, r.score)
+
+
+def test_search_empty_and_oversize(P):
+    say("search edge cases")
+    tray = np.zeros((6, 7, 8), np.int32)
+    assert P.fft_search_placement(np.ones((7, 1, 1), np.int32), tray) == ((-1, -1, -1), False, 0.0)
+    assert P.fft_search_placement(np.zeros((2, 2, 2), np.int32), tray) == ((0, 0, 0), True, 0.0)
+    assert P.fft_search_placement(np.ones((1, 1, 1), np.int32), np.ones((3, 3, 3), np.int32)) == \
+        ((-1, -1, -1), False, 0.0)
+    r = O.search(np.ones((6, 7, 8), np.int32), tray, penalty="pow")
+    assert P.fft_search_placement(np.ones((6, 7, 8), np.int32), tray) == (r.position, r.found, r.score)
+
+
+def test_search_batch_matches_oracle(P):
+    rng = np.random.default_rng(7)
+    from psacking_amd.rotations import get_orientations
+    for t, (N, I) in enumerate([((30, 25, 20), (6, 4, 3)), ((41, 37, 33), (9, 7, 11)),
+                                ((64, 50, 40), (12, 10, 8))]):
+        tray = rand_tray(rng, N, 0.15, 4)
+        item = (rng.random(I) < 0.6).astype(np.int32)
+        phi = O.calculate_distance(tray)
+        orients = [np.ascontiguousarray(g) for g in get_orientations(item, 24)]
+        say("fft_search_batch", N, I)
+        exp = O.search_batch(orients, tray, phi, penalty="cube")
+        got = P.fft_search_batch(orients, tray, phi, 1000 + t)
+        assert got == (exp[0], exp[1], exp[2])
+        for o in (0, 5, 17):
+            r = O.search(orients[o], tray, phi, penalty="cube")
+            assert P.fft_search_placement_with_cache(orients[o], tray, phi, 1000 + t) == \
+                (r.position, r.found, r.score)
+
+
+def test_search_batch_interlocking_free(P):
+    rng = np.random.default_rng(8)
+    from psacking_amd.rotations import get_orientations
+    N = (20, 18, 16)
+    tray = (rng.random(N) < 0.25).astype(np.int32)
+    tray[5:15, 5:15, 4:12] = 1
+    tray[7:13, 7:13, 6:10] = 0
+    item = np.ones((2, 3, 2), np.int32)
+    phi = O.calculate_distance(tray)
+    orients = [np.ascontiguousarray(g) for g in get_orientations(item, 24)]
+    say("fft_search_batch_interlocking_free", N)
+    exp = O.search_batch(orients, tray, phi, penalty="cube", interlocking_free=True)
+    got = P.fft_search_batch_interlocking_free(orients, tray, phi, 77)
+    assert got == exp
+
+
+def test_gpu_tray_context(P):
+    say("gpu_tray context")
+    rng = np.random.default_rng(9)
+    tray = rand_tray(rng, (24, 20, 16), 0.2)
+    phi = O.calculate_distance(tray)
+    P.gpu_tray_cleanup()
+    assert not P.gpu_tray_is_initialized()
+    with pytest.raises(RuntimeError):
+        P.gpu_tray_search(np.ones((2, 2, 2), np.int32))
+    P.gpu_tray_init(tray, phi)
+    assert P.gpu_tray_is_initialized()
+    item = np.ones((3, 2, 4), np.int32)
+    r = O.search(item, tray, phi, penalty="cube")
+    assert P.gpu_tray_search(item) == (r.position, r.found, r.score)
+    P.gpu_tray_cleanup()
+    assert not P.gpu_tray_is_initialized()
+
+
+@pytest.mark.parametrize("engine", ["fused", "percall"])
+def test_tetromino_fixture(P, kats, engine):
+    shapes = kats["tetrominoes"]
+    for run in kats["tetromino_runs"]:
+        say("tetromino", engine, run["num_orientations"])
+        items = [piece(shapes[c]) for c in run["order"]]
+        bp = P.BinPacker(tuple(run["tray"]), num_orientations=run["num_orientations"],
+                         engine=engine)
+        r = bp.pack_voxels(items, sort_by_volume=False)
+        assert r.num_placed == 7
+        assert abs(r.density - run["density"]) < 1e-12
+        assert [tuple(p.position) for p in r.placements] == [tuple(p) for p in run["positions"]]
+        assert [p.orientation_index for p in r.placements] == run["orientations"]
+        np.testing.assert_array_equal(r.tray[:, :9, :], grid_from_rows(run["grid_cols_0_8"], (8, 9, 1)))
+
+
+def test_tetromino_reference_sort(P, kats):
+    # the reference test passes the dict order with sort_by_volume=True
+    shapes = kats["tetrominoes"]
+    items = [piece(shapes[c]) for c in "IOTSZJL"]
+    say("tetromino sorted")
+    r = P.BinPacker((8, 16, 1), num_orientations=4).pack_voxels(items, sort_by_volume=True)
+    run = kats["tetromino_runs"][1]
+    np.testing.assert_array_equal(r.tray[:, :9, :], grid_from_rows(run["grid_cols_0_8"], (8, 9, 1)))
+
+
+def _compare_pack(P, w, n_orient, interlocking_free=False, metric="l1", sources="eq1", n=None):
+    items = w.items if n is None else w.items[:n]
+    vols = [int((it > 0).sum()) for it in items]
+    order = [int(i) for i in np.argsort(vols)[::-1]]
+    say("pack", w.name, len(items), n_orient, interlocking_free, metric, sources)
+    exp = O.pack_voxels(items, w.tray_size, num_orientations=n_orient,
+                        interlocking_free=interlocking_free, order=order, metric=metric,
+                        sources=sources)
+    got = P.BinPacker(w.tray_size, num_orientations=n_orient, interlocking_free=interlocking_free,
+                      metric=metric, sources=sources).pack_voxels(items)
+    assert got.num_placed == exp.num_placed
+    for a, b in zip(got.placements, exp.placements):
+        assert (a.item_index, a.success, a.position, a.orientation_index, a.score) == \
+            (b.item_index, b.success, b.position, b.orientation_index, b.score)
+    np.testing.assert_array_equal(got.tray, exp.tray)
+    assert got.density == exp.density
+
+
+def test_pack_boxes_64(P):
+    from psacking_amd import workloads as W
+    _compare_pack(P, W.boxes(), 1)
+
+
+def test_pack_blobs_100(P):
+    from psacking_amd import workloads as W
+    _compare_pack(P, W.blobs(), 1, n=25)
+
+
+def test_pack_blobs_24_orient(P):
+    from psacking_amd import workloads as W
+    w = W.blobs(seed=12, n=6, tray=(60, 50, 40))
+    _compare_pack(P, w, 24)
+    _compare_pack(P, w, 24, metric="edt", sources="nonzero")
+
+
+def test_pack_interlocking_free(P):
+    from psacking_amd import workloads as W
+    w = W.blobs(seed=13, n=5, tray=(40, 36, 30))
+    _compare_pack(P, w, 4, interlocking_free=True)
+
+
+def test_orientation_count_case(P, kats):
+    for case in kats["orientation_counts"]:
+        say("orientation counts")
+        item = np.ones(case["item_shape"], np.int32)
+        assert P.BinPacker(tuple(case["tray"]), num_orientations=1).pack_voxels([item]).num_placed == case["placed_1"]
+        assert P.BinPacker(tuple(case["tray"]), num_orientations=24).pack_voxels([item]).num_placed == case["placed_24"]
