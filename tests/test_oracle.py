@@ -241,3 +241,17 @@ def test_orientations_are_distinct_rotations():
         assert np.array_equal(a, b)
     for a, b in zip(O.get_orientations(g, 6), outs[::4]):
         assert np.array_equal(a, b)
+
+
+def test_reference_cpu_path_as_written_matches_oracle():
+    from oracle import ref_cpu_path as R
+    rng = np.random.default_rng(31)
+    for _ in range(4):
+        N = tuple(int(v) for v in rng.integers(4, 14, 3))
+        tray = (rng.integers(1, 3, N) * (rng.random(N) < 0.3)).astype(np.int32)
+        item = (rng.random(tuple(int(v) for v in rng.integers(1, 4, 3))) < 0.7).astype(np.int32)
+        exp = O.search(item, tray, penalty="pow")
+        got = R.search_as_written(item, tray)
+        assert got == (exp.position, exp.found, exp.score)
+        np.testing.assert_array_equal(R.corr3_as_written(tray, O.pad_to(item, N)),
+                                      O.corr_exact(tray, O.pad_to(item, N)))
