@@ -64,7 +64,21 @@ def main(argv=None) -> int:
     ap.add_argument("--sources", default="nonzero", choices=["nonzero", "eq1"])
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU work for the baseline sample (0 = skip)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args(argv)
+
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch_read": n, "copy_bytes_per_launch_write": n,
+                          "launches": 5}), flush=True)
+        return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
