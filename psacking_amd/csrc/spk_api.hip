@@ -309,6 +309,7 @@ void run_distance(spk_ctx* c, const int* d_grid, int* d_out, int* d_tmp, FieldSt
                   int nx, int ny, int nz, int metric, int sources) {
   if ((long long)ny * nz > 32768)
     throw SpkError{SPK_EUNSUPPORTED, "distance: y*z plane larger than 32768 voxels"};
+  if (nx > 1024) throw SpkError{SPK_EUNSUPPORTED, "distance: x axis longer than 1024 voxels"};
   const int fill = nx + ny + nz + 10;
   const int edt = metric == SPK_METRIC_EDT ? 1 : 0;
   const int big = edt ? fill * fill : fill;
@@ -600,10 +601,12 @@ void do_search(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector
          interlocking_free ? c->pgrid.p : (long long*)nullptr);
   if (interlocking_free) if_masked_select(c, ods, pen);
   launch(c, "select", k_select, dim3(1), dim3(256), 0, (const WgBest*)c->rec.p,
-         (const OrientDesc*)c->od.p, m, c->nx, c->per.p, c->sout.p);
-  launch(c, "verify", k_verify, dim3(1), dim3(256), 0, (const int*)c->tray.p,
-         (const int*)c->phi.p, (const int*)c->items.p, (const OrientDesc*)c->od.p,
-         (const SearchOut*)c->sout.p, c->ver.p, c->ny, c->nz);
+         (const OrientDesc*)c->od.p, m, c->nx, c->per.p, c->sout.p, c->ver.p);
+  int max_e = 1;
+  for (const auto& d : ods) max_e = std::max(max_e, d.ex * d.ey * d.ez);
+  launch(c, "verify", k_verify, dim3(std::min(512, (max_e + 255) / 256)), dim3(256), 0,
+         (const int*)c->tray.p, (const int*)c->phi.p, (const int*)c->items.p,
+         (const OrientDesc*)c->od.p, (const SearchOut*)c->sout.p, c->ver.p, c->ny, c->nz);
   SearchOut so;
   std::vector<WgBest> per(m);
   long long ver[3];
@@ -643,7 +646,7 @@ void do_search(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector
     }
     if_masked_select(c, ods, pen);
     launch(c, "select", k_select, dim3(1), dim3(256), 0, (const WgBest*)c->rec.p,
-           (const OrientDesc*)c->od.p, m, c->nx, c->per.p, c->sout.p);
+           (const OrientDesc*)c->od.p, m, c->nx, c->per.p, c->sout.p, c->ver.p);
     SearchOut s2;
     download(c, &s2, c->sout.p, sizeof(s2));
     download(c, per.data(), c->per.p, m * sizeof(WgBest));

@@ -154,10 +154,8 @@ __device__ __forceinline__ int fdiv(int n, int d, float inv) {
 // elements per thread (batch*n <= NT*MAXEL, checked by the host planner).
 // jfast: consecutive lanes take consecutive j (use when es == 1), otherwise
 // consecutive lanes take consecutive transforms (strided transforms).
-// Not inlined: one register allocation per stage keeps the 1024-thread
-// kernels inside 128 VGPRs (inlined, the six radix bodies spill).
 template <int R, bool FWD, int NT, int MAXEL>
-__device__ __attribute__((noinline)) void fft_stage(float2* buf, int n, int Ns, int es, int batch, int bs,
+__device__ __forceinline__ void fft_stage(float2* buf, int n, int Ns, int es, int batch, int bs,
                                           bool jfast, const float2* tw, int tid) {
   constexpr int MS = (MAXEL + R - 1) / R;
   const int m = n / R;
@@ -212,13 +210,18 @@ __device__ void lds_fft(float2* buf, const AxisPlan& p, int es, int batch, int b
   int Ns = 1;
   for (int s = 0; s < p.nst; ++s) {
     const int R = (int)((p.rad >> (4 * s)) & 15ULL);
+    // An opaque per-iteration copy of tid: without it the compiler hoists
+    // every radix case's index set-up out of the loop and keeps all of them
+    // live at once (the six cases then spill at 128 VGPRs).
+    int t = tid;
+    asm volatile("" : "+v"(t));
     switch (R) {
-      case 2: fft_stage<2, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, tid); break;
-      case 3: fft_stage<3, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, tid); break;
-      case 4: fft_stage<4, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, tid); break;
-      case 5: fft_stage<5, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, tid); break;
-      case 7: fft_stage<7, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, tid); break;
-      default: fft_stage<8, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, tid); break;
+      case 2: fft_stage<2, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
+      case 3: fft_stage<3, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
+      case 4: fft_stage<4, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
+      case 5: fft_stage<5, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
+      case 7: fft_stage<7, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
+      default: fft_stage<8, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
     }
     Ns *= R;
   }

@@ -112,6 +112,29 @@ __device__ __forceinline__ void load_table(float2* dst, const float2* src, int n
   for (int q = tid; q < n; q += nt) dst[q] = src[q];
 }
 
+// Global -> LDS copy of n contiguous complex values with every load issued
+// before the first LDS store (one round trip of HBM latency per thread, not
+// one per element).  Element e lands at lds[(e / w) * ld + e % w].
+template <int NT, int MAXEL>
+__device__ __forceinline__ void stage_rows(float2* lds, const float2* __restrict__ src, int n,
+                                           int w, int ld, int tid) {
+  float2 v[MAXEL];
+  const float inv_w = 1.0f / (float)w;
+#pragma unroll
+  for (int s = 0; s < MAXEL; ++s) {
+    const int e = tid + s * NT;
+    if (e < n) v[s] = src[e];
+  }
+#pragma unroll
+  for (int s = 0; s < MAXEL; ++s) {
+    const int e = tid + s * NT;
+    if (e < n) {
+      const int r = fdiv(e, w, inv_w);
+      lds[r * ld + (e - r * w)] = v[s];
+    }
+  }
+}
+
 // lexicographic (score, idx) minimum
 __device__ __forceinline__ bool better(double s1, int i1, double s2, int i2) {
   return (s1 < s2) || (s1 == s2 && i1 < i2);
@@ -134,10 +157,21 @@ k_dist_zy(const int* __restrict__ tray, int* __restrict__ out, int nx, int ny, i
   const int tid = threadIdx.x;
   const int S = ny * nz;
   const int* src = tray + (size_t)i * S;
-  for (int e = tid; e < S; e += kDistThreads) {
-    const int v = src[e];
-    const bool s = sources ? (v != 0) : (v == 1);
-    sh[e] = s ? 0 : big;
+  {
+    int v[32];
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+      const int e = tid + t * kDistThreads;
+      v[t] = e < S ? src[e] : 0;
+    }
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+      const int e = tid + t * kDistThreads;
+      if (e < S) {
+        const bool s = sources ? (v[t] != 0) : (v[t] == 1);
+        sh[e] = s ? 0 : big;
+      }
+    }
   }
   __syncthreads();
   constexpr int MAXR = 32;   // elements per thread: ny*nz <= 32768
@@ -196,9 +230,19 @@ k_dist_x(const int* __restrict__ in, int* __restrict__ phi, FieldStats* __restri
   const int tid = threadIdx.x;
   const int tk = min(TK, nz - k0);
   const int E = nx * TK;
-  for (int e = tid; e < E; e += kDistThreads) {
-    const int x = e / TK, t = e - x * TK;
-    if (t < tk) sh[e] = in[((size_t)x * ny + j) * nz + k0 + t];
+  {
+    int v[16];   // E = nx * TK <= 1024 * 16
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int e = tid + s * kDistThreads;
+      const int x = e / TK, t = e - x * TK;
+      v[s] = (e < E && t < tk) ? in[((size_t)x * ny + j) * nz + k0 + t] : 0;
+    }
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      const int e = tid + s * kDistThreads;
+      if (e < E) sh[e] = v[s];
+    }
   }
   __syncthreads();
   long long lsum = 0;
@@ -299,20 +343,39 @@ k_tray_slab_fwd(const int* __restrict__ tray, const int* __restrict__ phi,
   const FieldParams fp = field_params(st, (long long)nx * ny * nz);
   const int b = 1 << fp.lb, half = b >> 1;
   const size_t so = (size_t)x * ny * nz;
-  for (int e = tid; e < py * pz; e += kSlabThreads) {
-    const int j = e / pz, k = e - j * pz;
-    float2 v = make_float2(0.f, 0.f);
-    if (j < ny && k < nz) {
-      const int d = phi[so + j * nz + k] - fp.mu;
-      const int h = d >> fp.lb;            // floor division
-      if (field == 0) {
-        v.x = tray[so + j * nz + k] != 0 ? 1.f : 0.f;
-        v.y = (float)h;
-      } else {
-        v.x = (float)(d - (h << fp.lb) - half);
+  {
+    // all global loads of the slab first, then the LDS stores
+    int dv[kSlabMaxEl], tv[kSlabMaxEl];
+    const float inv_pz = 1.0f / (float)pz;
+#pragma unroll
+    for (int s = 0; s < kSlabMaxEl; ++s) {
+      const int e = tid + s * kSlabThreads;
+      const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
+      dv[s] = 0;
+      tv[s] = 0;
+      if (e < py * pz && j < ny && k < nz) {
+        dv[s] = phi[so + j * nz + k];
+        if (field == 0) tv[s] = tray[so + j * nz + k];
       }
     }
-    buf[e] = v;
+#pragma unroll
+    for (int s = 0; s < kSlabMaxEl; ++s) {
+      const int e = tid + s * kSlabThreads;
+      const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
+      if (e >= py * pz) continue;
+      float2 v = make_float2(0.f, 0.f);
+      if (j < ny && k < nz) {
+        const int d = dv[s] - fp.mu;
+        const int h = d >> fp.lb;            // floor division
+        if (field == 0) {
+          v.x = tv[s] != 0 ? 1.f : 0.f;
+          v.y = (float)h;
+        } else {
+          v.x = (float)(d - (h << fp.lb) - half);
+        }
+      }
+      buf[e] = v;
+    }
   }
   __syncthreads();
   lds_fft<true, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, ny, pz, true, twz, tid);
@@ -348,12 +411,21 @@ k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
   const float2* src = field == 0 ? t1 : t2;
   float2* dst = field == 0 ? g1 : g2;
   const size_t rs = (size_t)py * ncol;   // x stride
-  for (int e = tid; e < px * T; e += kPencilThreads) {
-    const int x = e / T, t = e - x * T;
-    const int kz = kz0 + t;
-    float2 v = make_float2(0.f, 0.f);
-    if (x < nx && kz < ncol) v = src[x * rs + (size_t)ky * ncol + kz];
-    buf[e] = v;
+  {
+    float2 v[kPencilMaxEl];
+#pragma unroll
+    for (int s = 0; s < kPencilMaxEl; ++s) {
+      const int e = tid + s * kPencilThreads;
+      const int x = e / T, t = e - x * T;
+      const int kz = kz0 + t;
+      v[s] = make_float2(0.f, 0.f);
+      if (e < px * T && x < nx && kz < ncol) v[s] = src[x * rs + (size_t)ky * ncol + kz];
+    }
+#pragma unroll
+    for (int s = 0; s < kPencilMaxEl; ++s) {
+      const int e = tid + s * kPencilThreads;
+      if (e < px * T) buf[e] = v[s];
+    }
   }
   __syncthreads();
   lds_fft<true, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
@@ -382,11 +454,20 @@ k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od
   load_table(twy, tw_y, py, tid, kSlabThreads);
   load_table(twz, tw_z, pz, tid, kSlabThreads);
   const int* base = items + d.item_off;
-  for (int e = tid; e < py * pz; e += kSlabThreads) {
-    const int j = e / pz, k = e - j * pz;
-    float2 v = make_float2(0.f, 0.f);
-    if (j < d.ey && k < d.ez) v.x = (float)base[oriented_index(d, x, j, k)];
-    buf[e] = v;
+  {
+    int iv[kSlabMaxEl];
+    const float inv_pz = 1.0f / (float)pz;
+#pragma unroll
+    for (int s = 0; s < kSlabMaxEl; ++s) {
+      const int e = tid + s * kSlabThreads;
+      const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
+      iv[s] = (e < py * pz && j < d.ey && k < d.ez) ? base[oriented_index(d, x, j, k)] : 0;
+    }
+#pragma unroll
+    for (int s = 0; s < kSlabMaxEl; ++s) {
+      const int e = tid + s * kSlabThreads;
+      if (e < py * pz) buf[e] = make_float2((float)iv[s], 0.f);
+    }
   }
   __syncthreads();
   lds_fft<true, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, d.ey, pz, true, twz, tid);
@@ -543,10 +624,7 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
   const long long addback = (b / 2 + fp.mu) * d.isum;
   // --- low digit (real field, half spectrum) ---
   const float2* s2 = y2 + ((size_t)o * nx + x) * py * hz;
-  for (int e = tid; e < py * hz; e += kSlabThreads) {
-    const int j = e / hz, k = e - j * hz;
-    buf[j * pz + k] = s2[e];
-  }
+  stage_rows<kSlabThreads, kSlabMaxEl>(buf, s2, py * hz, hz, pz, tid);
   __syncthreads();
   lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, py_, pz, hz, 1, false, twy, tid);
   const int nvy = d.nvy, nvz = d.nvz;
@@ -574,7 +652,7 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
   __syncthreads();
   // --- collision + high digit (complex field, full spectrum) ---
   const float2* s1 = y1 + ((size_t)o * nx + x) * py * pz;
-  for (int e = tid; e < py * pz; e += kSlabThreads) buf[e] = s1[e];
+  stage_rows<kSlabThreads, kSlabMaxEl>(buf, s1, py * pz, pz, pz, tid);
   __syncthreads();
   lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, py_, pz, pz, 1, false, twy, tid);
   lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, nvy, pz, true, twz, tid);
@@ -611,8 +689,10 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
 k_select(const WgBest* __restrict__ rec, const OrientDesc* __restrict__ od, int n_orient,
-         int nx, WgBest* __restrict__ per_orient, SearchOut* __restrict__ out) {
+         int nx, WgBest* __restrict__ per_orient, SearchOut* __restrict__ out,
+         long long* __restrict__ ver) {
   const int tid = threadIdx.x;
+  if (tid < 3) ver[tid] = 0;   // accumulator of the following k_verify
   for (int o = 0; o < n_orient; ++o) {
     const int nvx = od[o].nvx;
     double sc = 1e300;
@@ -683,6 +763,8 @@ k_select(const WgBest* __restrict__ rec, const OrientDesc* __restrict__ od, int 
 // ver[0] = raw collision sum (tray * item), ver[1] = overlap count
 // (tray != 0 & item > 0... item value weighted), ver[2] = proximity.
 // ---------------------------------------------------------------------------
+// grid-stride over the item voxels; ver[] must be zero on entry (k_select
+// clears it) and is accumulated with 64-bit atomics.
 __global__ void __launch_bounds__(256)
 k_verify(const int* __restrict__ tray, const int* __restrict__ phi, const int* __restrict__ items,
          const OrientDesc* __restrict__ od, const SearchOut* __restrict__ so,
@@ -695,7 +777,7 @@ k_verify(const int* __restrict__ tray, const int* __restrict__ phi, const int* _
     const int qz = s.idx % nz, qy = (s.idx / nz) % ny, qx = s.idx / (ny * nz);
     const int* base = items + d.item_off;
     const int E = d.ex * d.ey * d.ez;
-    for (int e = tid; e < E; e += 256) {
+    for (int e = blockIdx.x * 256 + tid; e < E; e += gridDim.x * 256) {
       const int w2 = e % d.ez, w1 = (e / d.ez) % d.ey, w0 = e / (d.ey * d.ez);
       const int v = base[oriented_index(d, w0, w1, w2)];
       if (v == 0) continue;
@@ -714,10 +796,9 @@ k_verify(const int* __restrict__ tray, const int* __restrict__ phi, const int* _
   __shared__ long long r[3][4];
   if ((tid & 63) == 0) { r[0][tid >> 6] = c; r[1][tid >> 6] = ov; r[2][tid >> 6] = p; }
   __syncthreads();
-  if (tid == 0) {
-    ver[0] = r[0][0] + r[0][1] + r[0][2] + r[0][3];
-    ver[1] = r[1][0] + r[1][1] + r[1][2] + r[1][3];
-    ver[2] = r[2][0] + r[2][1] + r[2][2] + r[2][3];
+  if (tid < 3) {
+    const long long v = r[tid][0] + r[tid][1] + r[tid][2] + r[tid][3];
+    if (v) atomicAdd((unsigned long long*)&ver[tid], (unsigned long long)v);
   }
 }
 
