@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -50,26 +51,43 @@ bool smooth(int n) {
   return n == 1;
 }
 
+// Radices with a register butterfly (spk_fft.hpp).
+const int kRadices[] = {16, 15, 14, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
+
+// Fewest stages first (each stage is one LDS round trip and two barriers),
+// then the most even split; larger radices run first.
 AxisPlan make_plan(int n) {
+  std::map<int, std::vector<int>> memo;
+  std::function<std::vector<int>(int)> best = [&](int m) -> std::vector<int> {
+    if (m == 1) return {};
+    auto it = memo.find(m);
+    if (it != memo.end()) return it->second;
+    std::vector<int> b;
+    bool have = false;
+    for (int r : kRadices) {
+      if (m % r) continue;
+      std::vector<int> c = best(m / r);
+      if (!c.empty() && c[0] == -1) continue;
+      c.push_back(r);
+      auto score = [](const std::vector<int>& v) {
+        int mx = 0;
+        for (int x : v) mx = std::max(mx, x);
+        return std::make_pair((int)v.size(), mx);
+      };
+      if (!have || score(c) < score(b)) { b = c; have = true; }
+    }
+    if (!have) b = {-1};
+    memo[m] = b;
+    return b;
+  };
+  std::vector<int> rs = best(n);
+  if (!rs.empty() && rs[0] == -1) throw SpkError{SPK_EUNSUPPORTED, "FFT length has no plan"};
+  std::sort(rs.begin(), rs.end(), [](int x, int y) { return x > y; });
   AxisPlan p{};
   p.n = n;
   p.nst = 0;
-  int m = n;
-  int a = 0;
-  while (m % 2 == 0) { m /= 2; ++a; }
-  std::vector<int> rs;
-  int q = a / 3, r = a % 3;
-  if (r == 1 && q >= 1) { q -= 1; rs.push_back(4); rs.push_back(4); }
-  else if (r == 1) rs.push_back(2);
-  else if (r == 2) rs.push_back(4);
-  for (int i = 0; i < q; ++i) rs.push_back(8);
-  while (m % 5 == 0) { m /= 5; rs.push_back(5); }
-  while (m % 3 == 0) { m /= 3; rs.push_back(3); }
-  while (m % 7 == 0) { m /= 7; rs.push_back(7); }
-  // larger radices first
-  std::sort(rs.begin(), rs.end(), [](int x, int y) { return x > y; });
   p.rad = 0;
-  for (int v : rs) p.rad |= (unsigned long long)v << (4 * p.nst++);
+  for (int v : rs) p.rad |= (unsigned long long)v << (kRadixBits * p.nst++);
   return p;
 }
 
@@ -242,6 +260,10 @@ void check_dims(int nx, int ny, int nz) {
 size_t slab_shmem(const spk_ctx* c) {
   return (size_t)(c->py.n + c->pz.n + c->py.n * c->pz.n) * sizeof(float2);
 }
+// pass B: slab + tables + the [ny][nz] low-digit sums
+size_t score_shmem(const spk_ctx* c) {
+  return slab_shmem(c) + (size_t)c->ny * c->nz * sizeof(int);
+}
 size_t pencil_shmem(const spk_ctx* c) {
   return (size_t)(c->px.n + c->px.n * c->T) * sizeof(float2);
 }
@@ -288,7 +310,7 @@ void plan_tray(spk_ctx* c) {
   upload(c, c->pen.p, pen.data(), pen.size() * sizeof(double));
   int lds_max = 0;
   HIP_OK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-  const size_t need = slab_shmem(c) + 512;
+  const size_t need = score_shmem(c) + 512;
   if (need > (size_t)lds_max || (size_t)c->ny * c->nz * sizeof(int) > (size_t)lds_max)
     throw SpkError{SPK_EUNSUPPORTED, "tray y*z slab does not fit the device's LDS (" +
                                          std::to_string(need) + " > " + std::to_string(lds_max) + ")"};
@@ -594,7 +616,7 @@ void do_search(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector
     c->pgrid.ensure((size_t)m * V);
     HIP_OK(hipMemsetAsync(c->ffree.p, 0, (size_t)m * V, c->stream));
   }
-  launch(c, "score_yz", k_score_yz, dim3(c->nx, m), dim3(kSlabThreads), slab_shmem(c),
+  launch(c, "score_yz", k_score_yz, dim3(c->nx, m), dim3(kSlabThreads), score_shmem(c),
          (const float2*)c->y1.p, (const float2*)c->y2.p, (const OrientDesc*)c->od.p,
          (const FieldStats*)c->stats.p, pen, c->rec.p, c->nx, c->ny, c->nz, c->py, c->pz,
          tw + c->px.n, tw + c->px.n + py, interlocking_free ? c->ffree.p : (unsigned char*)nullptr,

@@ -18,10 +18,13 @@ namespace spk {
 
 constexpr int kMaxStages = 12;
 
+constexpr int kRadixBits = 5;
+constexpr unsigned long long kRadixMask = 31ULL;
+
 struct AxisPlan {
   int n;
   int nst;
-  unsigned long long rad;   // 4 bits per stage, stage 0 in the low bits
+  unsigned long long rad;   // kRadixBits per stage, stage 0 in the low bits
 };
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
@@ -140,6 +143,95 @@ template <bool FWD> struct Bfly<3, FWD> : BflyOdd<3, FWD> {};
 template <bool FWD> struct Bfly<5, FWD> : BflyOdd<5, FWD> {};
 template <bool FWD> struct Bfly<7, FWD> : BflyOdd<7, FWD> {};
 
+// ---- composite radices in registers -------------------------------------
+// cos/sin(2*pi*m/R) evaluated at compile time (range-reduced Taylor series in
+// double, rounded once to float).
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double taylor_sin(double x) {
+  double term = x, sum = x;
+  for (int n = 1; n < 14; ++n) {
+    term *= -x * x / ((2.0 * n) * (2.0 * n + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double taylor_cos(double x) {
+  double term = 1.0, sum = 1.0;
+  for (int n = 1; n < 14; ++n) {
+    term *= -x * x / ((2.0 * n - 1.0) * (2.0 * n));
+    sum += term;
+  }
+  return sum;
+}
+// exp(i*2*pi*m/R) for 0 <= m < R, via the octant of the angle
+constexpr double root_cos(int m, int R) {
+  m %= R;
+  const double a = kTwoPi * m / R;
+  return a <= kTwoPi / 2 ? (a <= kTwoPi / 4 ? taylor_cos(a) : -taylor_cos(kTwoPi / 2 - a))
+                         : (a <= 3 * kTwoPi / 4 ? -taylor_cos(a - kTwoPi / 2) : taylor_cos(kTwoPi - a));
+}
+constexpr double root_sin(int m, int R) {
+  m %= R;
+  const double a = kTwoPi * m / R;
+  return a <= kTwoPi / 2 ? (a <= kTwoPi / 4 ? taylor_sin(a) : taylor_sin(kTwoPi / 2 - a))
+                         : (a <= 3 * kTwoPi / 4 ? -taylor_sin(a - kTwoPi / 2) : -taylor_sin(kTwoPi - a));
+}
+
+template <int R>
+struct Roots {
+  float c[R], s[R];
+  constexpr Roots() : c(), s() {
+    for (int m = 0; m < R; ++m) {
+      c[m] = (float)root_cos(m, R);
+      s[m] = (float)root_sin(m, R);
+    }
+  }
+};
+template <int R>
+struct RootTab {
+  static constexpr Roots<R> v{};
+};
+
+// X[k1 + A*k2] = sum_{n2} w_B^{n2 k2} w_R^{n2 k1} sum_{n1} x[B n1 + n2] w_A^{n1 k1},
+// R = A*B, every index a compile-time constant after unrolling.
+template <int A, int B, bool FWD>
+struct BflyCT {
+  static constexpr int R = A * B;
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 y[B][A];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+#pragma unroll
+      for (int n1 = 0; n1 < A; ++n1) y[n2][n1] = v[B * n1 + n2];
+      Bfly<A, FWD>::run(y[n2]);
+#pragma unroll
+      for (int k1 = 1; k1 < A; ++k1) {
+        if (n2 == 0) continue;
+        const int q = (n2 * k1) % R;
+        const float c = RootTab<R>::v.c[q];
+        const float s = FWD ? -RootTab<R>::v.s[q] : RootTab<R>::v.s[q];
+        y[n2][k1] = cmul(y[n2][k1], make_float2(c, s));
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+      float2 t[B];
+#pragma unroll
+      for (int n2 = 0; n2 < B; ++n2) t[n2] = y[n2][k1];
+      Bfly<B, FWD>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) v[k1 + A * k2] = t[k2];
+    }
+  }
+};
+template <bool FWD> struct Bfly<6, FWD> : BflyCT<2, 3, FWD> {};
+template <bool FWD> struct Bfly<9, FWD> : BflyCT<3, 3, FWD> {};
+template <bool FWD> struct Bfly<10, FWD> : BflyCT<2, 5, FWD> {};
+template <bool FWD> struct Bfly<12, FWD> : BflyCT<4, 3, FWD> {};
+template <bool FWD> struct Bfly<14, FWD> : BflyCT<2, 7, FWD> {};
+template <bool FWD> struct Bfly<15, FWD> : BflyCT<3, 5, FWD> {};
+template <bool FWD> struct Bfly<16, FWD> : BflyCT<4, 4, FWD> {};
+
 // n / d for 0 <= n < 2^22, d >= 1, with inv = 1/d: the float quotient is off
 // by at most one, fixed by the remainder test.
 __device__ __forceinline__ int fdiv(int n, int d, float inv) {
@@ -164,6 +256,11 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n, int Ns, int es, in
   const int dv = jfast ? m : batch;
   const float inv_dv = 1.0f / (float)dv;
   const float inv_ns = 1.0f / (float)Ns;
+  // Two layouts only: rows (es == 1, transforms bs apart, lanes along j) and
+  // columns (bs == 1, elements es apart, lanes along the transforms).  Every
+  // per-r offset is then a uniform step times a compile-time r.
+  const int in_step = jfast ? m : m * es;
+  const int out_step = jfast ? Ns : Ns * es;
   float2 v[MS][R];
   int ob[MS];
 #pragma unroll
@@ -173,22 +270,33 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n, int Ns, int es, in
     if (b < total) {
       const int qd = fdiv(b, dv, inv_dv);
       const int rd = b - qd * dv;
-      const int bt = jfast ? qd : rd;
       const int j = jfast ? rd : qd;
-      const int off = bt * bs;
       const int k = j - fdiv(j, Ns, inv_ns) * Ns;
+      const int jo = (j - k) * R + k;
+      int ib, obase;
+      if (jfast) {
+        const int off = qd * bs;
+        ib = off + j;
+        obase = off + jo;
+      } else {
+        ib = rd + j * es;
+        obase = rd + jo * es;
+      }
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[s][r] = buf[off + (j + r * m) * es];
+      for (int r = 0; r < R; ++r) v[s][r] = buf[ib + r * in_step];
       if (k != 0) {
+        const int kt = k * tws;
+        int ti = kt;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 w = tw[r * k * tws];
+          float2 w = tw[ti];
+          ti += kt;
           if (!FWD) w.y = -w.y;
           v[s][r] = cmul(v[s][r], w);
         }
       }
       Bfly<R, FWD>::run(v[s]);
-      ob[s] = off + ((j - k) * R + k) * es;
+      ob[s] = obase;
     }
   }
   __syncthreads();
@@ -196,7 +304,7 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n, int Ns, int es, in
   for (int s = 0; s < MS; ++s) {
     if (ob[s] >= 0) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[ob[s] + r * Ns * es] = v[s][r];
+      for (int r = 0; r < R; ++r) buf[ob[s] + r * out_step] = v[s][r];
     }
   }
   __syncthreads();
@@ -205,24 +313,25 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n, int Ns, int es, in
 // Full transform of length p.n (all stages).  Caller has synchronised the
 // buffer; returns synchronised.
 template <bool FWD, int NT, int MAXEL>
-__device__ void lds_fft(float2* buf, const AxisPlan& p, int es, int batch, int bs, bool jfast,
+__device__ __forceinline__ void lds_fft(float2* buf, const AxisPlan& p, int es, int batch, int bs,
+                                        bool jfast,
                         const float2* tw, int tid) {
   int Ns = 1;
   for (int s = 0; s < p.nst; ++s) {
-    const int R = (int)((p.rad >> (4 * s)) & 15ULL);
+    const int R = (int)((p.rad >> (kRadixBits * s)) & kRadixMask);
     // An opaque per-iteration copy of tid: without it the compiler hoists
     // every radix case's index set-up out of the loop and keeps all of them
-    // live at once (the six cases then spill at 128 VGPRs).
+    // live at once (the radix cases then spill at 128 VGPRs).
     int t = tid;
     asm volatile("" : "+v"(t));
+#define SPK_STAGE(RR) \
+  case RR: fft_stage<RR, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
     switch (R) {
-      case 2: fft_stage<2, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
-      case 3: fft_stage<3, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
-      case 4: fft_stage<4, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
-      case 5: fft_stage<5, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
-      case 7: fft_stage<7, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
+      SPK_STAGE(2) SPK_STAGE(3) SPK_STAGE(4) SPK_STAGE(5) SPK_STAGE(6) SPK_STAGE(7)
+      SPK_STAGE(9) SPK_STAGE(10) SPK_STAGE(12) SPK_STAGE(14) SPK_STAGE(15) SPK_STAGE(16)
       default: fft_stage<8, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
     }
+#undef SPK_STAGE
     Ns *= R;
   }
 }

@@ -635,18 +635,20 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
   }
   __syncthreads();
   lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, nvy, pz, true, twz, tid);
+  // the rounded low-digit sums wait in LDS behind the slab ([ny][nz] ints),
+  // not in registers: the second slab FFT needs the whole register budget
+  int* lo = reinterpret_cast<int*>(buf + py * pz);
   float res = 0.f;
-  int lo[kSlabMaxEl];
+  const float inv_pz = 1.0f / (float)pz;
 #pragma unroll
   for (int s = 0; s < kSlabMaxEl; ++s) {
     const int e = tid + s * kSlabThreads;
-    const int j = e / pz, k = e - j * pz;
-    lo[s] = 0;
+    const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
     if (j < nvy && k < nvz) {
       const float v = buf[e].x;
       const float r = rintf(v);
       res = fmaxf(res, fabsf(v - r));
-      lo[s] = (int)r;
+      lo[j * nz + k] = (int)r;
     }
   }
   __syncthreads();
@@ -662,12 +664,12 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
 #pragma unroll
   for (int s = 0; s < kSlabMaxEl; ++s) {
     const int e = tid + s * kSlabThreads;
-    const int j = e / pz, k = e - j * pz;
+    const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
     if (j < nvy && k < nvz) {
       const float2 v = buf[e];
       const float c = rintf(v.x), h = rintf(v.y);
       res = fmaxf(res, fmaxf(fabsf(v.x - c), fabsf(v.y - h)));
-      const long long p = b * (long long)h + lo[s] + addback;
+      const long long p = b * (long long)h + lo[j * nz + k] + addback;
       const int idx = (x * ny + j) * nz + k;
       if (fgrid) {
         const size_t g = (size_t)o * nx * ny * nz + idx;
