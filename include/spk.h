@@ -139,6 +139,13 @@ int spk_save_vox(const int32_t* grid, int nx, int ny, int nz, const char* path);
 int spk_timing_enable(spk_ctx* ctx, int on);
 int spk_timing_get(spk_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
 int spk_timing_reset(spk_ctx* ctx);
+/* Tuning/diagnostic options of a context (no reference counterpart):
+ *   "resid_limit"  largest accepted distance of an FFT output from its
+ *                  integer before a search is redone on the exact direct-sum
+ *                  path (default 0.25; a negative value forces the exact path
+ *                  for every search, which the tests use to check that path). */
+int spk_set_option(spk_ctx* ctx, const char* name, double value);
+
 /* Plan actually used for the cached tray: FFT sizes and digit base. */
 int spk_plan_info(spk_ctx* ctx, int* px, int* py, int* pz, int* digit_base);
 

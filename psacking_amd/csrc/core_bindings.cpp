@@ -433,6 +433,9 @@ class Session {
     for (auto& r : res) out.append(result_dict(r));
     return out;
   }
+  void set_option(const std::string& name, double value) {
+    check(spk_set_option(ctx_, name.c_str(), value));
+  }
   void timing(bool on) { check(spk_timing_enable(ctx_, on ? 1 : 0)); }
   py::tuple timing_get(const std::string& name) {
     double ms = 0;
@@ -498,6 +501,7 @@ PYBIND11_MODULE(_core, m) {
       .def("pack", &Session::pack, py::arg("items"), py::arg("order"), py::arg("perms"),
            py::arg("flips"), py::arg("penalty") = "cube", py::arg("interlocking_free") = false,
            py::arg("id_offset") = 0)
+      .def("set_option", &Session::set_option, py::arg("name"), py::arg("value"))
       .def("timing", &Session::timing, py::arg("on"))
       .def("timing_get", &Session::timing_get, py::arg("name"))
       .def("timing_reset", &Session::timing_reset)

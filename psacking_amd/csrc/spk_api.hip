@@ -151,6 +151,12 @@ struct spk_ctx {
   DevBuf<unsigned char> ffree, freach;   // IF: feasibility / reached grids [m][V]
   DevBuf<long long> pgrid;               // IF: proximity grids [m][V]
   DevBuf<int> exits;                     // IF: high seed faces [m][3]
+  // greedy loop
+  DevBuf<OrientDesc> od_all;             // descriptors of every item's fitting orientations
+  DevBuf<PackOut> pout;                  // per-item outcome decided on the device
+  DevBuf<int> ctl;                       // CTL_* control words
+  DevBuf<int> zero;                      // an int that stays 0 (overlap flag of pipelined places)
+  float resid_limit = 0.25f;             // largest accepted FFT rounding residual
   // timing
   bool timing = false;
   std::vector<Timed> pending;
@@ -328,23 +334,23 @@ void plan_tray(spk_ctx* c) {
 
 // ---- distance field --------------------------------------------------------
 void run_distance(spk_ctx* c, const int* d_grid, int* d_out, int* d_tmp, FieldStats* d_st,
-                  int nx, int ny, int nz, int metric, int sources) {
+                  int nx, int ny, int nz, int metric, int sources, const int* ctl = nullptr) {
   if ((long long)ny * nz > 32768)
     throw SpkError{SPK_EUNSUPPORTED, "distance: y*z plane larger than 32768 voxels"};
   if (nx > 1024) throw SpkError{SPK_EUNSUPPORTED, "distance: x axis longer than 1024 voxels"};
   const int fill = nx + ny + nz + 10;
   const int edt = metric == SPK_METRIC_EDT ? 1 : 0;
   const int big = edt ? fill * fill : fill;
-  FieldStats init{0ULL, INT_MAX, INT_MIN};
-  upload(c, d_st, &init, sizeof(init));
+  launch(c, "stats_init", k_stats_init, dim3(1), dim3(64), 0, d_st, ctl);
   launch(c, "dist_zy", k_dist_zy, dim3(nx), dim3(kDistThreads), (size_t)ny * nz * sizeof(int),
-         (const int*)d_grid, d_tmp, nx, ny, nz, edt, sources, big);
+         (const int*)d_grid, d_tmp, nx, ny, nz, edt, sources, big, ctl);
   constexpr int TK = 16;
   launch(c, "dist_x", k_dist_x<TK>, dim3((nz + TK - 1) / TK, ny), dim3(kDistThreads),
-         (size_t)nx * TK * sizeof(int), (const int*)d_tmp, d_out, d_st, nx, ny, nz, edt, fill);
+         (size_t)nx * TK * sizeof(int), (const int*)d_tmp, d_out, d_st, nx, ny, nz, edt, fill,
+         ctl);
 }
 
-void build_spectra(spk_ctx* c) {
+void build_spectra(spk_ctx* c, const int* ctl = nullptr) {
   const int py = c->py.n, pz = c->pz.n, hz = pz / 2 + 1;
   c->t1.ensure((size_t)c->nx * py * pz);
   c->t2.ensure((size_t)c->nx * py * hz);
@@ -353,26 +359,28 @@ void build_spectra(spk_ctx* c) {
   const float2* tw = c->tw.p;
   launch(c, "tray_slab_fwd", k_tray_slab_fwd, dim3(c->nx, 2), dim3(kSlabThreads), slab_shmem(c),
          (const int*)c->tray.p, (const int*)c->phi.p, (const FieldStats*)c->stats.p, c->t1.p,
-         c->t2.p, c->nx, c->ny, c->nz, c->py, c->pz, tw + c->px.n, tw + c->px.n + py);
+         c->t2.p, c->nx, c->ny, c->nz, c->py, c->pz, tw + c->px.n, tw + c->px.n + py, ctl);
   launch(c, "tray_x_fwd", k_tray_x_fwd, dim3((pz + c->T - 1) / c->T, py, 2),
          dim3(kPencilThreads), pencil_shmem(c), (const float2*)c->t1.p, (const float2*)c->t2.p,
-         c->g1.p, c->g2.p, c->nx, c->px, py, pz, c->T, tw);
+         c->g1.p, c->g2.p, c->nx, c->px, py, pz, c->T, tw, ctl);
 }
 
-void refresh_field_and_spectra(spk_ctx* c, const int* dist_host) {
+// Distance field (computed, or the caller's) + statistics + cached spectra.
+// With ctl (greedy loop) every kernel is skipped on the device when the last
+// item placed nothing or the loop was handed back to the host.
+void refresh_field_and_spectra(spk_ctx* c, const int* dist_host, const int* ctl = nullptr) {
   const size_t V = (size_t)c->nx * c->ny * c->nz;
   if (dist_host) {
     upload(c, c->phi.p, dist_host, V * sizeof(int));
-    FieldStats init{0ULL, INT_MAX, INT_MIN};
-    upload(c, c->stats.p, &init, sizeof(init));
+    launch(c, "stats_init", k_stats_init, dim3(1), dim3(64), 0, c->stats.p, (const int*)nullptr);
     const int nb = (int)std::min<size_t>(1024, (V + 255) / 256);
     launch(c, "field_stats", k_field_stats, dim3(nb), dim3(256), 0, (const int*)c->phi.p,
            (long long)V, c->stats.p);
   } else {
     run_distance(c, c->tray.p, c->phi.p, c->tmpi.p, c->stats.p, c->nx, c->ny, c->nz, c->metric,
-                 c->sources);
+                 c->sources, ctl);
   }
-  build_spectra(c);
+  build_spectra(c, ctl);
 }
 
 // ---- orientation descriptors ----------------------------------------------
@@ -543,6 +551,51 @@ void if_masked_select(spk_ctx* c, const std::vector<OrientDesc>& ods, const doub
 }
 
 // ---- the search --------------------------------------------------------------
+void ensure_search_ws(spk_ctx* c, int m, int max_ex) {
+  const int py = c->py.n, pz = c->pz.n, hz = pz / 2 + 1;
+  c->ispec.ensure((size_t)m * max_ex * py * pz);
+  c->y1.ensure((size_t)m * c->nx * py * pz);
+  c->y2.ensure((size_t)m * c->nx * py * hz);
+  c->rec.ensure((size_t)m * c->nx);
+  c->per.ensure(m);
+  c->sout.ensure(1);
+  c->ver.ensure(3);
+}
+
+// Enqueue one batched search of m orientation descriptors (device array od):
+// item spectra, pass A, pass B, selection and the exact re-check of the
+// winner.  Results land in c->per, c->sout and c->ver.  With fgrid/pgrid
+// (interlocking-free) the feasibility and proximity grids are written and the
+// flood fill + masked argmin run before the selection.
+void enqueue_search(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
+                    const int* ctl, unsigned char* fgrid, long long* pgrid,
+                    const std::vector<OrientDesc>* host_ods) {
+  const int py = c->py.n, pz = c->pz.n;
+  const float2* tw = c->tw.p;
+  launch(c, "item_slab_fwd", k_item_slab_fwd, dim3(max_ex, m), dim3(kSlabThreads),
+         slab_shmem(c), (const int*)c->items.p, od, c->ispec.p, max_ex, c->py, c->pz,
+         tw + c->px.n, tw + c->px.n + py, ctl);
+  const int tiles = (pz + c->T - 1) / c->T;
+  int nchunk = std::max(1, std::min(m, (2048 + tiles * py - 1) / (tiles * py)));
+  const int chunk = (m + nchunk - 1) / nchunk;
+  nchunk = (m + chunk - 1) / chunk;
+  const float scale = (float)(1.0 / ((double)c->px.n * py * pz));
+  launch(c, "corr_x", k_corr_x, dim3(tiles, py, nchunk), dim3(kPencilThreads), pencil_shmem(c),
+         (const float2*)c->g1.p, (const float2*)c->g2.p, (const float2*)c->ispec.p, od, c->y1.p,
+         c->y2.p, m, chunk, max_ex, c->nx, c->px, py, pz, c->T, scale, tw, ctl);
+  const double* pen = c->pen.p + (penalty == SPK_PENALTY_CUBE ? c->nz : 0);
+  launch(c, "score_yz", k_score_yz, dim3(c->nx, m), dim3(kSlabThreads), score_shmem(c),
+         (const float2*)c->y1.p, (const float2*)c->y2.p, od, (const FieldStats*)c->stats.p, pen,
+         c->rec.p, c->nx, c->ny, c->nz, c->py, c->pz, tw + c->px.n, tw + c->px.n + py, fgrid,
+         pgrid, ctl);
+  if (fgrid) if_masked_select(c, *host_ods, pen);
+  launch(c, "select", k_select, dim3(1), dim3(kSelectThreads), 0, (const WgBest*)c->rec.p, od, m,
+         c->nx, c->per.p, c->sout.p, c->ver.p, ctl);
+  launch(c, "verify", k_verify, dim3(std::min(512, (max_e + 255) / 256)), dim3(256), 0,
+         (const int*)c->tray.p, (const int*)c->phi.p, (const int*)c->items.p, od,
+         (const SearchOut*)c->sout.p, c->ver.p, c->ny, c->nz, ctl);
+}
+
 // items: per entry base item + orientation; all on host.
 struct Entry {
   HostItem item;
@@ -586,29 +639,11 @@ void do_search(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector
     return;
   }
   const int m = (int)ods.size();
-  const int py = c->py.n, pz = c->pz.n, hz = pz / 2 + 1;
+  int max_e = 1;
+  for (const auto& d : ods) max_e = std::max(max_e, d.ex * d.ey * d.ez);
   c->od.ensure(m);
   upload(c, c->od.p, ods.data(), m * sizeof(OrientDesc));
-  c->ispec.ensure((size_t)m * max_ex * py * pz);
-  c->y1.ensure((size_t)m * c->nx * py * pz);
-  c->y2.ensure((size_t)m * c->nx * py * hz);
-  c->rec.ensure((size_t)m * c->nx);
-  c->per.ensure(m);
-  c->sout.ensure(1);
-  c->ver.ensure(3);
-  const float2* tw = c->tw.p;
-  launch(c, "item_slab_fwd", k_item_slab_fwd, dim3(max_ex, m), dim3(kSlabThreads),
-         slab_shmem(c), (const int*)c->items.p, (const OrientDesc*)c->od.p, c->ispec.p, max_ex,
-         c->py, c->pz, tw + c->px.n, tw + c->px.n + py);
-  const int tiles = (pz + c->T - 1) / c->T;
-  int nchunk = std::max(1, std::min(m, (2048 + tiles * py - 1) / (tiles * py)));
-  const int chunk = (m + nchunk - 1) / nchunk;
-  nchunk = (m + chunk - 1) / chunk;
-  const float scale = (float)(1.0 / ((double)c->px.n * py * pz));
-  launch(c, "corr_x", k_corr_x, dim3(tiles, py, nchunk), dim3(kPencilThreads), pencil_shmem(c),
-         (const float2*)c->g1.p, (const float2*)c->g2.p, (const float2*)c->ispec.p,
-         (const OrientDesc*)c->od.p, c->y1.p, c->y2.p, m, chunk, max_ex, c->nx, c->px, py, pz,
-         c->T, scale, tw);
+  ensure_search_ws(c, m, max_ex);
   const double* pen = c->pen.p + (penalty == SPK_PENALTY_CUBE ? c->nz : 0);
   const size_t V = (size_t)c->nx * c->ny * c->nz;
   if (interlocking_free) {
@@ -616,19 +651,9 @@ void do_search(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector
     c->pgrid.ensure((size_t)m * V);
     HIP_OK(hipMemsetAsync(c->ffree.p, 0, (size_t)m * V, c->stream));
   }
-  launch(c, "score_yz", k_score_yz, dim3(c->nx, m), dim3(kSlabThreads), score_shmem(c),
-         (const float2*)c->y1.p, (const float2*)c->y2.p, (const OrientDesc*)c->od.p,
-         (const FieldStats*)c->stats.p, pen, c->rec.p, c->nx, c->ny, c->nz, c->py, c->pz,
-         tw + c->px.n, tw + c->px.n + py, interlocking_free ? c->ffree.p : (unsigned char*)nullptr,
-         interlocking_free ? c->pgrid.p : (long long*)nullptr);
-  if (interlocking_free) if_masked_select(c, ods, pen);
-  launch(c, "select", k_select, dim3(1), dim3(256), 0, (const WgBest*)c->rec.p,
-         (const OrientDesc*)c->od.p, m, c->nx, c->per.p, c->sout.p, c->ver.p);
-  int max_e = 1;
-  for (const auto& d : ods) max_e = std::max(max_e, d.ex * d.ey * d.ez);
-  launch(c, "verify", k_verify, dim3(std::min(512, (max_e + 255) / 256)), dim3(256), 0,
-         (const int*)c->tray.p, (const int*)c->phi.p, (const int*)c->items.p,
-         (const OrientDesc*)c->od.p, (const SearchOut*)c->sout.p, c->ver.p, c->ny, c->nz);
+  enqueue_search(c, c->od.p, m, max_ex, max_e, penalty, nullptr,
+                 interlocking_free ? c->ffree.p : nullptr, interlocking_free ? c->pgrid.p : nullptr,
+                 interlocking_free ? &ods : nullptr);
   SearchOut so;
   std::vector<WgBest> per(m);
   long long ver[3];
@@ -637,8 +662,7 @@ void do_search(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector
   download(c, ver, c->ver.p, sizeof(ver));
   sync(c);
   if (c->timing) c->flush_timing();
-  const float kResidLimit = 0.25f;
-  const bool bad = so.unsupported || so.resid > kResidLimit ||
+  const bool bad = so.unsupported || !(so.resid <= c->resid_limit) ||
                    (so.found && (ver[1] != 0 || ver[2] != so.prox));
   if (!bad) {
     for (int i = 0; i < m; ++i) {
@@ -667,8 +691,9 @@ void do_search(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector
       sync(c);
     }
     if_masked_select(c, ods, pen);
-    launch(c, "select", k_select, dim3(1), dim3(256), 0, (const WgBest*)c->rec.p,
-           (const OrientDesc*)c->od.p, m, c->nx, c->per.p, c->sout.p, c->ver.p);
+    launch(c, "select", k_select, dim3(1), dim3(kSelectThreads), 0, (const WgBest*)c->rec.p,
+           (const OrientDesc*)c->od.p, m, c->nx, c->per.p, c->sout.p, c->ver.p,
+           (const int*)nullptr);
     SearchOut s2;
     download(c, &s2, c->sout.p, sizeof(s2));
     download(c, per.data(), c->per.p, m * sizeof(WgBest));
@@ -709,15 +734,127 @@ void do_place(spk_ctx* c, const OrientDesc& d, int x, int y, int z, int id, bool
   const int nb = std::max(1, std::min(1024, (E + 255) / 256));
   launch(c, "place", k_place, dim3(nb), dim3(256), 0, c->tray.p, (const int*)c->items.p,
          (const OrientDesc*)c->od.p, (const SearchOut*)nullptr, 0, x, y, z, id, c->nx, c->ny,
-         c->nz, 0, c->flag.p);
+         c->nz, 0, c->flag.p, (const int*)nullptr);
   int flag = 0;
   download(c, &flag, c->flag.p, sizeof(int));
   sync(c);
   if (flag) throw SpkError{SPK_EOVERLAP, "(place_in_tray): Item to be placed intersects with contents of tray"};
   launch(c, "place", k_place, dim3(nb), dim3(256), 0, c->tray.p, (const int*)c->items.p,
          (const OrientDesc*)c->od.p, (const SearchOut*)nullptr, 0, x, y, z, id, c->nx, c->ny,
-         c->nz, 1, c->flag.p);
+         c->nz, 1, c->flag.p, (const int*)nullptr);
   if (refresh) refresh_field_and_spectra(c, nullptr);
+}
+
+// The greedy loop with every item enqueued back to back: search, k_decide,
+// placement and refresh of item t+1 follow item t on the stream with no host
+// round trip.  k_decide hands a search back to the host (CTL_HALT) when its
+// FFT result cannot be trusted; the kernels of the remaining items then exit
+// at once, the host redoes that item on the exact direct-sum path, places it
+// and resumes the pipeline after it.  Returns the number of items placed.
+int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector<int>& off,
+                   const int32_t* order, const spk_orient* orients, int n_orient, int penalty,
+                   int32_t id_offset, spk_result* results) {
+  const int n_items = (int)bases.size();
+  struct Plan {
+    int od_off = 0, m = 0, max_ex = 1, max_e = 1;
+    std::vector<int> map;   // fitting index -> orientation index
+  };
+  std::vector<Plan> plans(n_items);
+  std::vector<OrientDesc> all;
+  int ws_m = 1, ws_ex = 1;
+  for (int t = 0; t < n_items; ++t) {
+    const int it = order ? order[t] : t;
+    Plan& pl = plans[t];
+    pl.od_off = (int)all.size();
+    for (int o = 0; o < n_orient; ++o) {
+      OrientDesc d = make_desc(c, bases[it], off[it], orients ? &orients[o] : nullptr);
+      if (!fits(c, d)) continue;
+      all.push_back(d);
+      pl.map.push_back(o);
+      pl.max_ex = std::max(pl.max_ex, d.ex);
+      pl.max_e = std::max(pl.max_e, d.ex * d.ey * d.ez);
+    }
+    pl.m = (int)pl.map.size();
+    ws_m = std::max(ws_m, pl.m);
+    ws_ex = std::max(ws_ex, pl.max_ex);
+  }
+  c->od_all.ensure(std::max<size_t>(all.size(), 1));
+  if (!all.empty()) upload(c, c->od_all.p, all.data(), all.size() * sizeof(OrientDesc));
+  ensure_search_ws(c, ws_m, ws_ex);
+  c->pout.ensure(n_items);
+  c->ctl.ensure(CTL_WORDS);
+  c->zero.ensure(1);
+  HIP_OK(hipMemsetAsync(c->zero.p, 0, sizeof(int), c->stream));
+
+  int placed = 0;
+  int t0 = 0;
+  while (t0 < n_items) {
+    const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
+    upload(c, c->ctl.p, ctl0, sizeof(ctl0));
+    for (int t = t0; t < n_items; ++t) {
+      const Plan& pl = plans[t];
+      if (pl.m == 0) continue;   // nothing fits: no search, tray unchanged
+      const OrientDesc* od = c->od_all.p + pl.od_off;
+      enqueue_search(c, od, pl.m, pl.max_ex, pl.max_e, penalty, c->ctl.p, nullptr, nullptr,
+                     nullptr);
+      launch(c, "decide", k_decide, dim3(1), dim3(64), 0, (const SearchOut*)c->sout.p,
+             (const long long*)c->ver.p, c->pout.p, t, c->resid_limit, c->ctl.p);
+      const int nb = std::max(1, std::min(1024, (pl.max_e + 255) / 256));
+      launch(c, "place", k_place, dim3(nb), dim3(256), 0, c->tray.p, (const int*)c->items.p, od,
+             (const SearchOut*)c->sout.p, 0, 0, 0, 0, (int)id_offset, c->nx, c->ny, c->nz, 1,
+             c->zero.p, (const int*)c->ctl.p);
+      refresh_field_and_spectra(c, nullptr, c->ctl.p);
+    }
+    int ctl[CTL_WORDS];
+    std::vector<PackOut> po(n_items - t0);
+    download(c, ctl, c->ctl.p, sizeof(ctl));
+    download(c, po.data(), c->pout.p + t0, po.size() * sizeof(PackOut));
+    sync(c);
+    if (c->timing) c->flush_timing();
+    const int stop = ctl[CTL_HALT] ? ctl[CTL_HALT_ITEM] : n_items;
+    for (int t = t0; t < stop; ++t) {
+      const Plan& pl = plans[t];
+      if (pl.m == 0) {
+        to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, &results[t]);
+        continue;
+      }
+      const SearchOut& s = po[t - t0].so;
+      to_result(c, s.found, s.idx, s.score, s.found ? pl.map[s.orient] : -1,
+                po[t - t0].collision, s.prox, s.resid, 0, &results[t]);
+      if (s.found) ++placed;
+    }
+    if (!ctl[CTL_HALT]) break;
+    // item `stop` goes through the exact path on the host's orders
+    const int h = stop;
+    const Plan& pl = plans[h];
+    const int it = order ? order[h] : h;
+    const float resid = po[h - t0].so.resid;
+    double bs = 1e30;
+    int bi = -1, bo = -1;
+    long long bp = 0;
+    for (int i = 0; i < pl.m; ++i) {
+      const WgBest r = exact_search_one(c, bases[it], all[pl.od_off + i], penalty);
+      if (r.idx != INT_MAX && r.score < bs) {
+        bs = r.score;
+        bi = r.idx;
+        bo = i;
+        bp = ((long long)r.p_hi << 32) | (unsigned int)r.p_lo;
+      }
+    }
+    if (bo < 0) {
+      to_result(c, false, 0, 0.0, -1, 0, 0, resid, 1, &results[h]);
+    } else {
+      to_result(c, true, bi, bs, pl.map[bo], 0, bp, resid, 1, &results[h]);
+      const spk_result& r = results[h];
+      // do_place takes its descriptor from c->od
+      c->od.ensure(1);
+      do_place(c, all[pl.od_off + bo], r.pos[0], r.pos[1], r.pos[2], id_offset + placed + 1,
+               true);
+      ++placed;
+    }
+    t0 = h + 1;
+  }
+  return placed;
 }
 
 }  // namespace
@@ -771,6 +908,7 @@ int spk_ctx_destroy(spk_ctx* c) {
     c->y1.release(); c->y2.release(); c->rec.release(); c->per.release(); c->sout.release();
     c->ver.release(); c->flag.release(); c->taps.release(); c->big.release();
     c->ffree.release(); c->freach.release(); c->pgrid.release(); c->exits.release();
+    c->od_all.release(); c->pout.release(); c->ctl.release(); c->zero.release();
     if (c->stream) hipStreamDestroy(c->stream);
   }
   delete c;
@@ -900,21 +1038,31 @@ int spk_pack(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_items,
     }
     c->items.ensure(std::max<size_t>(tot, 1));
     upload(c, c->items.p, items, tot * sizeof(int));
-    int placed = 0;
     for (int t = 0; t < n_items; ++t) {
       const int it = order ? order[t] : t;
       if (it < 0 || it >= n_items) throw SpkError{SPK_EINVAL, "order index out of range"};
-      std::vector<Entry> entries;
-      for (int o = 0; o < n_orient; ++o)
-        entries.push_back(Entry{bases[it], orients ? &orients[o] : nullptr, it});
-      do_search(c, bases, off, true, entries, penalty, interlocking_free, &results[t], nullptr);
-      if (results[t].found) {
-        const int o = results[t].orient;
-        OrientDesc d = make_desc(c, bases[it], off[it], orients ? &orients[o] : nullptr);
-        do_place(c, d, results[t].pos[0], results[t].pos[1], results[t].pos[2],
-                 id_offset + placed + 1, true);
-        ++placed;
+    }
+    int placed = 0;
+    if (interlocking_free) {
+      // the flood fill iterates to its fixpoint under host control: one
+      // synchronous search per item
+      for (int t = 0; t < n_items; ++t) {
+        const int it = order ? order[t] : t;
+        std::vector<Entry> entries;
+        for (int o = 0; o < n_orient; ++o)
+          entries.push_back(Entry{bases[it], orients ? &orients[o] : nullptr, it});
+        do_search(c, bases, off, true, entries, penalty, 1, &results[t], nullptr);
+        if (results[t].found) {
+          const int o = results[t].orient;
+          OrientDesc d = make_desc(c, bases[it], off[it], orients ? &orients[o] : nullptr);
+          do_place(c, d, results[t].pos[0], results[t].pos[1], results[t].pos[2],
+                   id_offset + placed + 1, true);
+          ++placed;
+        }
       }
+    } else {
+      placed = pack_pipelined(c, bases, off, order, orients, n_orient, penalty, id_offset,
+                              results);
     }
     sync(c);
     if (c->timing) c->flush_timing();
@@ -1051,7 +1199,7 @@ int spk_place_grid(spk_ctx* c, const int32_t* tray, int nx, int ny, int nz, cons
     for (int phase = 0; phase < 2; ++phase)
       launch(c, "place", k_place, dim3(nb), dim3(256), 0, g.p, (const int*)it.p,
              (const OrientDesc*)od.p, (const SearchOut*)nullptr, 0, x, y, z, (int)id, nx, ny, nz,
-             phase, flag.p);
+             phase, flag.p, (const int*)nullptr);
     int f = 0;
     download(c, &f, flag.p, sizeof(int));
     download(c, out, g.p, V * sizeof(int));
@@ -1177,6 +1325,15 @@ int spk_timing_reset(spk_ctx* c) {
     c->acc.clear();
     return SPK_OK;
   });
+}
+
+int spk_set_option(spk_ctx* c, const char* name, double value) {
+  if (!c || !name) return fail(SPK_EINVAL, "null argument");
+  if (std::string(name) == "resid_limit") {
+    c->resid_limit = (float)value;
+    return SPK_OK;
+  }
+  return fail(SPK_EINVAL, std::string("unknown option ") + name);
 }
 
 int spk_plan_info(spk_ctx* c, int* px, int* py, int* pz, int* digit_base) {

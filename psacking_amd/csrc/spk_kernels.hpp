@@ -94,6 +94,21 @@ struct SearchOut {           // written by k_select, read by verify/place
   int unsupported;
 };
 
+// Control words of the device-driven greedy loop (spk_pack).  Kernels take
+// a nullable pointer: outside that loop they always run.
+enum { CTL_HALT = 0, CTL_HALT_ITEM = 1, CTL_SKIP_REFRESH = 2, CTL_PLACED = 3, CTL_WORDS = 4 };
+// search kernels stop once a search has been handed back to the host
+__device__ __forceinline__ bool search_off(const int* ctl) { return ctl && ctl[CTL_HALT]; }
+// refresh kernels also stop when the last item placed nothing
+__device__ __forceinline__ bool refresh_off(const int* ctl) {
+  return ctl && (ctl[CTL_HALT] | ctl[CTL_SKIP_REFRESH]);
+}
+
+struct PackOut {             // one item of the greedy loop, as the device decided it
+  SearchOut so;
+  long long collision;
+};
+
 __device__ __forceinline__ int oriented_index(const OrientDesc& d, int w0, int w1, int w2) {
   int w[3] = {w0, w1, w2};
   int ext[3] = {d.ex, d.ey, d.ez};
@@ -151,7 +166,8 @@ __device__ __forceinline__ int dcost(int d, int edt) { return edt ? d * d : (d <
 // one x-slice: init (sources) + z pass + y pass, in LDS; writes tmp int grid.
 __global__ void __launch_bounds__(kDistThreads)
 k_dist_zy(const int* __restrict__ tray, int* __restrict__ out, int nx, int ny, int nz, int edt,
-          int sources, int big) {
+          int sources, int big, const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
   extern __shared__ int sh[];
   const int i = blockIdx.x;
   const int tid = threadIdx.x;
@@ -223,7 +239,8 @@ k_dist_zy(const int* __restrict__ tray, int* __restrict__ out, int nx, int ny, i
 template <int TK>
 __global__ void __launch_bounds__(kDistThreads)
 k_dist_x(const int* __restrict__ in, int* __restrict__ phi, FieldStats* __restrict__ st, int nx,
-         int ny, int nz, int edt, int fill) {
+         int ny, int nz, int edt, int fill, const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
   extern __shared__ int sh[];
   const int j = blockIdx.y;
   const int k0 = blockIdx.x * TK;
@@ -331,7 +348,9 @@ __global__ void __launch_bounds__(kSlabThreads)
 k_tray_slab_fwd(const int* __restrict__ tray, const int* __restrict__ phi,
                 const FieldStats* __restrict__ st, float2* __restrict__ t1,
                 float2* __restrict__ t2, int nx, int ny, int nz, AxisPlan py_, AxisPlan pz_,
-                const float2* __restrict__ tw_y, const float2* __restrict__ tw_z) {
+                const float2* __restrict__ tw_y, const float2* __restrict__ tw_z,
+                const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
   extern __shared__ float2 smem[];
   const int py = py_.n, pz = pz_.n, hz = pz / 2 + 1;
   float2* twy = smem;
@@ -398,7 +417,8 @@ k_tray_slab_fwd(const int* __restrict__ tray, const int* __restrict__ phi,
 __global__ void __launch_bounds__(kPencilThreads)
 k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
              float2* __restrict__ g1, float2* __restrict__ g2, int nx, AxisPlan px_, int py,
-             int pz, int T, const float2* __restrict__ tw_x) {
+             int pz, int T, const float2* __restrict__ tw_x, const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
   extern __shared__ float2 smem[];
   const int px = px_.n, hz = pz / 2 + 1;
   float2* twx = smem;
@@ -442,7 +462,9 @@ k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
 __global__ void __launch_bounds__(kSlabThreads)
 k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od,
                 float2* __restrict__ ispec, int max_ex, AxisPlan py_, AxisPlan pz_,
-                const float2* __restrict__ tw_y, const float2* __restrict__ tw_z) {
+                const float2* __restrict__ tw_y, const float2* __restrict__ tw_z,
+                const int* __restrict__ ctl) {
+  if (search_off(ctl)) return;
   extern __shared__ float2 smem[];
   const int py = py_.n, pz = pz_.n;
   const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
@@ -483,7 +505,9 @@ __global__ void __launch_bounds__(kPencilThreads)
 k_corr_x(const float2* __restrict__ g1, const float2* __restrict__ g2,
          const float2* __restrict__ ispec, const OrientDesc* __restrict__ od,
          float2* __restrict__ y1, float2* __restrict__ y2, int n_orient, int chunk, int max_ex,
-         int nx, AxisPlan px_, int py, int pz, int T, float scale, const float2* __restrict__ tw_x) {
+         int nx, AxisPlan px_, int py, int pz, int T, float scale, const float2* __restrict__ tw_x,
+         const int* __restrict__ ctl) {
+  if (search_off(ctl)) return;
   extern __shared__ float2 smem[];
   const int px = px_.n, hz = pz / 2 + 1;
   float2* twx = smem;
@@ -607,7 +631,8 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
            const double* __restrict__ pen, WgBest* __restrict__ rec, int nx, int ny, int nz,
            AxisPlan py_, AxisPlan pz_, const float2* __restrict__ tw_y,
            const float2* __restrict__ tw_z, unsigned char* __restrict__ fgrid,
-           long long* __restrict__ pgrid) {
+           long long* __restrict__ pgrid, const int* __restrict__ ctl) {
+  if (search_off(ctl)) return;
   extern __shared__ float2 smem[];
   const int py = py_.n, pz = pz_.n, hz = pz / 2 + 1;
   const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
@@ -686,23 +711,26 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
 }
 
 // ---------------------------------------------------------------------------
-// selection: one block.  Per orientation min over its x records, then the
-// first strictly smaller score across orientations.
+// selection: one block of kSelectThreads; wave w reduces orientations w,
+// w+waves, ... over their x records, then thread 0 takes the first strictly
+// smaller score across orientations (list order).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
+constexpr int kSelectThreads = 1024;
+__global__ void __launch_bounds__(kSelectThreads)
 k_select(const WgBest* __restrict__ rec, const OrientDesc* __restrict__ od, int n_orient,
          int nx, WgBest* __restrict__ per_orient, SearchOut* __restrict__ out,
-         long long* __restrict__ ver) {
-  const int tid = threadIdx.x;
+         long long* __restrict__ ver, const int* __restrict__ ctl) {
+  if (search_off(ctl)) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid < 3) ver[tid] = 0;   // accumulator of the following k_verify
-  for (int o = 0; o < n_orient; ++o) {
+  for (int o = wave; o < n_orient; o += kSelectThreads / 64) {
     const int nvx = od[o].nvx;
     double sc = 1e300;
     int idx = INT_MAX;
     long long p = 0;
     float res = 0.f;
     int uns = 0;
-    for (int x = tid; x < nvx; x += 256) {
+    for (int x = lane; x < nvx; x += 64) {
       const WgBest r = rec[(size_t)o * nx + x];
       if (better(r.score, r.idx, sc, idx)) {
         sc = r.score; idx = r.idx;
@@ -721,19 +749,7 @@ k_select(const WgBest* __restrict__ rec, const OrientDesc* __restrict__ od, int 
       res = fmaxf(res, r2);
       uns |= u2;
     }
-    __shared__ double ws[4];
-    __shared__ int wi[4], wu[4];
-    __shared__ long long wp[4];
-    __shared__ float wr[4];
-    __syncthreads();
-    if ((tid & 63) == 0) { ws[tid >> 6] = sc; wi[tid >> 6] = idx; wp[tid >> 6] = p; wr[tid >> 6] = res; wu[tid >> 6] = uns; }
-    __syncthreads();
-    if (tid == 0) {
-      for (int w = 1; w < 4; ++w) {
-        if (better(ws[w], wi[w], sc, idx)) { sc = ws[w]; idx = wi[w]; p = wp[w]; }
-        res = fmaxf(res, wr[w]);
-        uns |= wu[w];
-      }
+    if (lane == 0) {
       WgBest b;
       b.score = sc; b.idx = idx;
       b.p_lo = (int)(p & 0xffffffffLL); b.p_hi = (int)(p >> 32);
@@ -770,7 +786,8 @@ k_select(const WgBest* __restrict__ rec, const OrientDesc* __restrict__ od, int 
 __global__ void __launch_bounds__(256)
 k_verify(const int* __restrict__ tray, const int* __restrict__ phi, const int* __restrict__ items,
          const OrientDesc* __restrict__ od, const SearchOut* __restrict__ so,
-         long long* __restrict__ ver, int ny, int nz) {
+         long long* __restrict__ ver, int ny, int nz, const int* __restrict__ ctl) {
+  if (search_off(ctl)) return;
   const SearchOut s = *so;
   const int tid = threadIdx.x;
   long long c = 0, ov = 0, p = 0;
@@ -808,10 +825,14 @@ k_verify(const int* __restrict__ tray, const int* __restrict__ phi, const int* _
 // placement of an oriented item at an offset (from host args or SearchOut)
 // phase 0: detect overlap (flag), phase 1: write id where tray == 0.
 // ---------------------------------------------------------------------------
+// In the greedy loop (ctl != nullptr) the offset comes from the SearchOut the
+// device accepted and the id is id + the loop's placed count.
 __global__ void __launch_bounds__(256)
 k_place(int* __restrict__ tray, const int* __restrict__ items, const OrientDesc* __restrict__ od,
         const SearchOut* __restrict__ so, int orient, int qx, int qy, int qz, int id, int nx,
-        int ny, int nz, int phase, int* __restrict__ flag) {
+        int ny, int nz, int phase, int* __restrict__ flag, const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
+  if (ctl) id += ctl[CTL_PLACED];
   OrientDesc d;
   if (so) {
     const SearchOut s = *so;
@@ -837,6 +858,40 @@ k_place(int* __restrict__ tray, const int* __restrict__ items, const OrientDesc*
       *t = id;
     }
   }
+}
+
+// Greedy loop, one thread: accept the search of item t or hand it back to the
+// host.  A search is accepted when every rounded FFT output was within
+// resid_limit of its integer, the digit base covered the distance range and
+// the exact re-check of the winner (k_verify) agrees; then the item is placed
+// by the following kernels (when found) and CTL_PLACED counts it.
+__global__ void k_decide(const SearchOut* __restrict__ so, const long long* __restrict__ ver,
+                         PackOut* __restrict__ out, int t, float resid_limit,
+                         int* __restrict__ ctl) {
+  if (threadIdx.x != 0 || ctl[CTL_HALT]) return;
+  const SearchOut s = *so;
+  const bool bad = s.unsupported || !(s.resid <= resid_limit) ||
+                   (s.found && (ver[1] != 0 || ver[2] != s.prox));
+  PackOut r;
+  r.so = s;
+  r.collision = ver[0];
+  out[t] = r;
+  if (bad) {
+    ctl[CTL_HALT] = 1;
+    ctl[CTL_HALT_ITEM] = t;
+    ctl[CTL_SKIP_REFRESH] = 1;
+    return;
+  }
+  ctl[CTL_SKIP_REFRESH] = s.found ? 0 : 1;
+  if (s.found) ctl[CTL_PLACED] += 1;
+}
+
+// reset the distance-field statistics before a refresh
+__global__ void k_stats_init(FieldStats* __restrict__ st, const int* __restrict__ ctl) {
+  if (threadIdx.x != 0 || refresh_off(ctl)) return;
+  st->sum = 0ULL;
+  st->mn = INT_MAX;
+  st->mx = INT_MIN;
 }
 
 // ---------------------------------------------------------------------------

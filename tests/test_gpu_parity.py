@@ -264,3 +264,69 @@ def test_orientation_count_case(P, kats):
         item = np.ones(case["item_shape"], np.int32)
         assert P.BinPacker(tuple(case["tray"]), num_orientations=1).pack_voxels([item]).num_placed == case["placed_1"]
         assert P.BinPacker(tuple(case["tray"]), num_orientations=24).pack_voxels([item]).num_placed == case["placed_24"]
+
+
+def _session_pack(P, w, n_orient, resid_limit=None, metric="l1", sources="eq1", n=None,
+                  interlocking_free=False):
+    from psacking_amd.rotations import orientation_maps
+    items = w.items if n is None else w.items[:n]
+    vols = [int((it > 0).sum()) for it in items]
+    order = [int(i) for i in np.argsort(vols)[::-1]]
+    perms, flips = orientation_maps(n_orient)
+    s = P.Session(0)
+    if resid_limit is not None:
+        s.set_option("resid_limit", resid_limit)
+    s.set_tray(np.zeros(w.tray_size, np.int32), metric, sources, 0)
+    res = s.pack(items, np.array(order, np.int32), perms, flips,
+                 "pow" if n_orient == 1 and not interlocking_free else "cube", interlocking_free, 0)
+    exp = O.pack_voxels(items, w.tray_size, num_orientations=n_orient, order=order,
+                        metric=metric, sources=sources, interlocking_free=interlocking_free)
+    return s, res, exp, order
+
+
+def test_pack_forced_exact_path(P):
+    """resid_limit < 0 hands every search back to the host's exact path."""
+    from psacking_amd import workloads as W
+    w = W.blobs(seed=14, n=6, tray=(40, 36, 30))
+    say("pack forced exact path")
+    s, res, exp, order = _session_pack(P, w, 24, resid_limit=-1.0)
+    by_item = {p.item_index: p for p in exp.placements}
+    for t, r in enumerate(res):
+        e = by_item[order[t]]
+        assert r["path"] == 1
+        assert r["found"] == e.success
+        if e.success:
+            assert (tuple(r["position"]), r["orientation"], r["score"]) == \
+                (e.position, e.orientation_index, e.score)
+    np.testing.assert_array_equal(s.tray(), exp.tray)
+
+
+def test_pack_pipelined_matches_oracle(P):
+    from psacking_amd import workloads as W
+    w = W.blobs(seed=15, n=10, tray=(64, 48, 40))
+    say("pack pipelined")
+    s, res, exp, order = _session_pack(P, w, 24, metric="edt", sources="nonzero")
+    by_item = {p.item_index: p for p in exp.placements}
+    for t, r in enumerate(res):
+        e = by_item[order[t]]
+        assert r["path"] == 0 and r["found"] == e.success
+        if e.success:
+            assert (tuple(r["position"]), r["orientation"], r["score"]) == \
+                (e.position, e.orientation_index, e.score)
+            assert r["collision"] == 0
+    np.testing.assert_array_equal(s.tray(), exp.tray)
+
+
+def test_search_forced_exact_interlocking(P):
+    from psacking_amd import workloads as W
+    w = W.blobs(seed=16, n=3, tray=(30, 28, 24))
+    say("interlocking forced exact path")
+    s, res, exp, order = _session_pack(P, w, 4, resid_limit=-1.0, interlocking_free=True)
+    by_item = {p.item_index: p for p in exp.placements}
+    for t, r in enumerate(res):
+        e = by_item[order[t]]
+        assert r["found"] == e.success
+        if e.success:
+            assert (tuple(r["position"]), r["orientation"], r["score"]) == \
+                (e.position, e.orientation_index, e.score)
+    np.testing.assert_array_equal(s.tray(), exp.tray)
