@@ -293,8 +293,9 @@ def test_pack_forced_exact_path(P):
     by_item = {p.item_index: p for p in exp.placements}
     for t, r in enumerate(res):
         e = by_item[order[t]]
-        assert r["path"] == 1
         assert r["found"] == e.success
+        # items with no orientation inside the tray are never searched
+        assert r["path"] == 1 or not r["found"]
         if e.success:
             assert (tuple(r["position"]), r["orientation"], r["score"]) == \
                 (e.position, e.orientation_index, e.score)
