@@ -64,6 +64,8 @@ def main(argv=None) -> int:
     ap.add_argument("--sources", default="nonzero", choices=["nonzero", "eq1"])
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU work for the baseline sample (0 = skip)")
+    ap.add_argument("--plan", type=int, default=0, choices=[0, 1, 2],
+                    help="FFT plan: 0 auto, 1 yz slabs, 2 z rows + y pencils")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -100,6 +102,7 @@ def main(argv=None) -> int:
     penalty = "cube" if n_orient > 1 else "pow"
 
     s = P.Session(0)
+    s.set_option("plan", args.plan)
     tray0 = np.zeros(w.tray_size, np.int32)
     t0 = time.perf_counter()
     s.set_tray(tray0, args.metric, args.sources, 0)
@@ -125,8 +128,10 @@ def main(argv=None) -> int:
     max_resid = max(float(r["residual"]) for r in res)
 
     # per-kernel event timing over the timed region
-    names = ["dist_zy", "dist_x", "tray_slab_fwd", "tray_x_fwd", "item_slab_fwd", "corr_x",
-             "score_yz", "select", "verify", "place", "direct_search"]
+    names = ["dist_zy", "dist_x", "tray_slab_fwd", "tray_rows_fwd", "tray_x_fwd",
+             "item_slab_fwd", "item_rows_fwd", "pencil_y_fwd", "corr_x", "score_yz",
+             "pencil_y_inv", "rows_score", "select", "verify", "decide", "place",
+             "direct_search"]
     kt = {n: s.timing_get(n) for n in names}
     busy = sum(v[0] for v in kt.values())
     dom = max(("score_yz", "corr_x"), key=lambda n: kt[n][0])

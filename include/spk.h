@@ -113,6 +113,27 @@ int spk_pack(spk_ctx* ctx, const int32_t* items, const int32_t* dims, int n_item
              int penalty, int interlocking_free, int32_t id_offset,
              spk_result* results);
 
+/* Arguments of one spk_pack call, for spk_pack_many. */
+typedef struct {
+  const int32_t* items;
+  const int32_t* dims;
+  int n_items;
+  const int32_t* order;
+  const spk_orient* orients;
+  int n_orient;
+  int penalty;
+  int interlocking_free;
+  int32_t id_offset;
+  spk_result* results;
+} spk_pack_args;
+
+/* Many independent packing problems at once (no reference counterpart: the
+ * multi-instance batch of uneven trays).  ctxs[i] must hold instance i's tray
+ * (spk_tray_set); every context's work runs on its own stream, concurrently.
+ * Returns the first failing instance's status (its message in
+ * spk_last_error()); the other instances still complete. */
+int spk_pack_many(spk_ctx* const* ctxs, int n, const spk_pack_args* args);
+
 /* Standalone full-grid operations (host in, host out). */
 int spk_distance(spk_ctx* ctx, const int32_t* grid, int nx, int ny, int nz, int metric,
                  int sources, int32_t* out);

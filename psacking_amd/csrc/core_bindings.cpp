@@ -450,9 +450,68 @@ class Session {
     return py::make_tuple(px, py_, pz);
   }
 
+  spk_ctx* ctx() const { return ctx_; }
+
  private:
   spk_ctx* ctx_ = nullptr;
 };
+
+// Items of one instance flattened for the C-ABI.
+struct PackJob {
+  std::vector<int32_t> flat, dims, order;
+  std::vector<spk_orient> orients;
+  std::vector<spk_result> res;
+};
+
+py::list py_pack_many(py::list sessions, py::list item_lists, py::list orders, py::object perms,
+                      py::object flips, const std::string& penalty, bool interlocking_free) {
+  const size_t n = py::len(sessions);
+  if (py::len(item_lists) != n || py::len(orders) != n)
+    throw py::value_error("sessions, item_lists and orders must have the same length");
+  std::vector<spk_ctx*> ctxs(n);
+  std::vector<PackJob> jobs(n);
+  std::vector<spk_pack_args> args(n);
+  auto os = orients_of(perms, flips);
+  for (size_t i = 0; i < n; ++i) {
+    ctxs[i] = sessions[i].cast<Session&>().ctx();
+    std::vector<Grid> gs = grids_of(item_lists[i].cast<py::list>());
+    PackJob& j = jobs[i];
+    for (auto& g : gs) {
+      j.dims.push_back(g.nx);
+      j.dims.push_back(g.ny);
+      j.dims.push_back(g.nz);
+      j.flat.insert(j.flat.end(), g.ptr(), g.ptr() + g.size());
+    }
+    auto ord = py::array_t<int32_t, py::array::c_style | py::array::forcecast>::ensure(orders[i]);
+    if (!ord || ord.ndim() != 1 || (size_t)ord.shape(0) != gs.size())
+      throw py::value_error("order must list every item once");
+    j.order.assign(ord.data(), ord.data() + gs.size());
+    j.orients = os;
+    j.res.resize(gs.size());
+    spk_pack_args& a = args[i];
+    a.items = j.flat.data();
+    a.dims = j.dims.data();
+    a.n_items = (int)gs.size();
+    a.order = j.order.data();
+    a.orients = j.orients.data();
+    a.n_orient = (int)j.orients.size();
+    a.penalty = penalty_of(penalty);
+    a.interlocking_free = interlocking_free ? 1 : 0;
+    a.id_offset = 0;
+    a.results = j.res.data();
+  }
+  {
+    py::gil_scoped_release nogil;
+    check(spk_pack_many(ctxs.data(), (int)n, args.data()));
+  }
+  py::list out;
+  for (auto& j : jobs) {
+    py::list l;
+    for (auto& r : j.res) l.append(result_dict(r));
+    out.append(l);
+  }
+  return out;
+}
 
 }  // namespace
 
@@ -506,6 +565,10 @@ PYBIND11_MODULE(_core, m) {
       .def("timing_get", &Session::timing_get, py::arg("name"))
       .def("timing_reset", &Session::timing_reset)
       .def("plan", &Session::plan);
+
+  m.def("pack_many", &py_pack_many, py::arg("sessions"), py::arg("item_lists"), py::arg("orders"),
+        py::arg("perms"), py::arg("flips"), py::arg("penalty") = "cube",
+        py::arg("interlocking_free") = false);
 
   m.attr("VOXEL_RESOLUTION") = 128;
   m.attr("HEIGHT_PENALTY") = 100000000.0;

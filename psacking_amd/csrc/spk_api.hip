@@ -10,6 +10,7 @@
 #include <functional>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/spk.h"
@@ -132,7 +133,11 @@ struct spk_ctx {
   uint64_t generation = 0;
   int metric = SPK_METRIC_L1, sources = SPK_SOURCES_EQ1;
   AxisPlan px{}, py{}, pz{};
-  int T = 16;
+  int T = 16;                  // x-pencil tile
+  int Ty = 16;                 // y-pencil tile (split plan)
+  int Rf = 1, Rs = 1;          // rows per block: forward rows, score rows (split plan)
+  bool split = false;          // split plan in use
+  int plan_mode = 0;           // 0 auto, 1 slab, 2 split
   bool lds_attrs = false;
   DevBuf<int> tray, phi, tmpi;
   DevBuf<FieldStats> stats;
@@ -142,7 +147,7 @@ struct spk_ctx {
   DevBuf<int> items;
   DevBuf<OrientDesc> od;
   DevBuf<float2> ispec, y1, y2;
-  DevBuf<WgBest> rec, per;
+  DevBuf<WgBest> rec, per, rec2;   // rec2: interlocking-free masked records
   DevBuf<SearchOut> sout;
   DevBuf<long long> ver;
   DevBuf<int> flag;
@@ -280,18 +285,20 @@ void plan_tray(spk_ctx* c) {
   int py = next_smooth(c->ny);
   if (px > kMaxAxis || py > kMaxAxis || pz > kMaxAxis)
     throw SpkError{SPK_EUNSUPPORTED, "tray axis longer than 1024 is not supported"};
-  if ((long long)py * pz > kSlabMax)
-    throw SpkError{SPK_EUNSUPPORTED,
-                   "tray y*z plane larger than 16384 FFT points is not supported by the "
-                   "two-pass plan"};
   if ((long long)c->ny * c->nz > 32768)
     throw SpkError{SPK_EUNSUPPORTED, "tray y*z plane larger than 32768 voxels"};
   c->px = make_plan(px);
   c->py = make_plan(py);
   c->pz = make_plan(pz);
-  int T = 16;
-  while (T > 1 && px * T > kPencilMax) T >>= 1;
-  c->T = T;
+  auto tile = [](int n) {
+    int T = 16;
+    while (T > 1 && n * T > kPencilMax) T >>= 1;
+    return T;
+  };
+  c->T = tile(px);
+  c->Ty = tile(py);
+  c->Rf = std::max(1, kRowMax / pz);        // forward rows per block
+  c->Rs = std::max(1, kRowMax / (2 * pz));  // score rows per block (two row halves)
   // twiddle tables exp(-2 pi i q / n), double -> float
   std::vector<float2> tw(px + py + pz);
   auto fill = [&](int off, int n) {
@@ -316,10 +323,14 @@ void plan_tray(spk_ctx* c) {
   upload(c, c->pen.p, pen.data(), pen.size() * sizeof(double));
   int lds_max = 0;
   HIP_OK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
-  const size_t need = score_shmem(c) + 512;
-  if (need > (size_t)lds_max || (size_t)c->ny * c->nz * sizeof(int) > (size_t)lds_max)
-    throw SpkError{SPK_EUNSUPPORTED, "tray y*z slab does not fit the device's LDS (" +
-                                         std::to_string(need) + " > " + std::to_string(lds_max) + ")"};
+  if ((size_t)c->ny * c->nz * sizeof(int) > (size_t)lds_max)
+    throw SpkError{SPK_EUNSUPPORTED, "tray y*z plane does not fit the device's LDS"};
+  // slab plan when a whole yz slab (and its low-digit sums) fits one block's
+  // LDS, else (or when asked) the split plan of z rows and y pencils
+  const bool slab_fits = (long long)py * pz <= kSlabMax && score_shmem(c) + 512 <= (size_t)lds_max;
+  c->split = c->plan_mode == 2 || (c->plan_mode == 0 && !slab_fits);
+  if (!c->split && !slab_fits)
+    throw SpkError{SPK_EUNSUPPORTED, "the slab plan does not fit this tray's yz plane"};
   if (!c->lds_attrs) {
     const size_t dyn = (size_t)lds_max - 1024;   // leave room for static shared arrays
     allow_shmem(k_tray_slab_fwd, dyn);
@@ -357,9 +368,25 @@ void build_spectra(spk_ctx* c, const int* ctl = nullptr) {
   c->g1.ensure((size_t)c->px.n * py * pz);
   c->g2.ensure((size_t)c->px.n * py * hz);
   const float2* tw = c->tw.p;
-  launch(c, "tray_slab_fwd", k_tray_slab_fwd, dim3(c->nx, 2), dim3(kSlabThreads), slab_shmem(c),
-         (const int*)c->tray.p, (const int*)c->phi.p, (const FieldStats*)c->stats.p, c->t1.p,
-         c->t2.p, c->nx, c->ny, c->nz, c->py, c->pz, tw + c->px.n, tw + c->px.n + py, ctl);
+  if (c->split) {
+    const int jb = (c->ny + c->Rf - 1) / c->Rf;
+    launch(c, "tray_rows_fwd", k_tray_rows_fwd, dim3(jb * c->nx, 2), dim3(kRowThreads),
+           (size_t)(pz + kRowMax) * sizeof(float2), (const int*)c->tray.p, (const int*)c->phi.p,
+           (const FieldStats*)c->stats.p, c->t1.p, c->t2.p, c->nx, c->ny, c->nz, py, c->pz, c->Rf,
+           tw + c->px.n + py, ctl);
+    const size_t sh = (size_t)(py + py * c->Ty) * sizeof(float2);
+    launch(c, "pencil_y_fwd", k_pencil<true>, dim3((pz + c->Ty - 1) / c->Ty, c->nx),
+           dim3(kPencilThreads), sh, c->t1.p, c->py, pz, c->Ty, c->ny, py, tw + c->px.n,
+           (const OrientDesc*)nullptr, c->nx, (int)PENCIL_PLAIN, ctl, 1);
+    launch(c, "pencil_y_fwd", k_pencil<true>, dim3((hz + c->Ty - 1) / c->Ty, c->nx),
+           dim3(kPencilThreads), sh, c->t2.p, c->py, hz, c->Ty, c->ny, py, tw + c->px.n,
+           (const OrientDesc*)nullptr, c->nx, (int)PENCIL_PLAIN, ctl, 1);
+  } else {
+    launch(c, "tray_slab_fwd", k_tray_slab_fwd, dim3(c->nx, 2), dim3(kSlabThreads),
+           slab_shmem(c), (const int*)c->tray.p, (const int*)c->phi.p,
+           (const FieldStats*)c->stats.p, c->t1.p, c->t2.p, c->nx, c->ny, c->nz, c->py, c->pz,
+           tw + c->px.n, tw + c->px.n + py, ctl);
+  }
   launch(c, "tray_x_fwd", k_tray_x_fwd, dim3((pz + c->T - 1) / c->T, py, 2),
          dim3(kPencilThreads), pencil_shmem(c), (const float2*)c->t1.p, (const float2*)c->t2.p,
          c->g1.p, c->g2.p, c->nx, c->px, py, pz, c->T, tw, ctl);
@@ -545,9 +572,18 @@ void if_masked_select(spk_ctx* c, const std::vector<OrientDesc>& ods, const doub
     exits[3 * o + 2] = c->nz - 1 - ods[o].hz;
   }
   flood_fill(c, m, exits, c->nx, c->ny, c->nz);
+  // c->per holds each orientation's unmasked summary (residual, range flag)
+  c->rec2.ensure((size_t)m * c->nx);
   launch(c, "masked_argmin", k_masked_argmin, dim3(c->nx, m), dim3(256), 0,
          (const unsigned char*)c->freach.p, (const long long*)c->pgrid.p,
-         (const OrientDesc*)c->od.p, pen, c->rec.p, c->nx, c->ny, c->nz);
+         (const OrientDesc*)c->od.p, pen, (const WgBest*)c->per.p, c->rec2.p, c->nx, c->ny,
+         c->nz);
+}
+
+// records per orientation of the score pass: slab plan one per x slice,
+// split plan one per block of Rs valid rows
+int rec_stride(const spk_ctx* c) {
+  return c->split ? (c->nx * c->ny + c->Rs - 1) / c->Rs : c->nx;
 }
 
 // ---- the search --------------------------------------------------------------
@@ -556,7 +592,7 @@ void ensure_search_ws(spk_ctx* c, int m, int max_ex) {
   c->ispec.ensure((size_t)m * max_ex * py * pz);
   c->y1.ensure((size_t)m * c->nx * py * pz);
   c->y2.ensure((size_t)m * c->nx * py * hz);
-  c->rec.ensure((size_t)m * c->nx);
+  c->rec.ensure((size_t)m * rec_stride(c));
   c->per.ensure(m);
   c->sout.ensure(1);
   c->ver.ensure(3);
@@ -570,11 +606,23 @@ void ensure_search_ws(spk_ctx* c, int m, int max_ex) {
 void enqueue_search(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
                     const int* ctl, unsigned char* fgrid, long long* pgrid,
                     const std::vector<OrientDesc>* host_ods) {
-  const int py = c->py.n, pz = c->pz.n;
+  const int py = c->py.n, pz = c->pz.n, hz = pz / 2 + 1;
   const float2* tw = c->tw.p;
-  launch(c, "item_slab_fwd", k_item_slab_fwd, dim3(max_ex, m), dim3(kSlabThreads),
-         slab_shmem(c), (const int*)c->items.p, od, c->ispec.p, max_ex, c->py, c->pz,
-         tw + c->px.n, tw + c->px.n + py, ctl);
+  const size_t ysh = (size_t)(py + py * c->Ty) * sizeof(float2);
+  if (c->split) {
+    const int max_ey = std::min(c->ny, py);
+    const int jb = (max_ey + c->Rf - 1) / c->Rf;
+    launch(c, "item_rows_fwd", k_item_rows_fwd, dim3(jb * max_ex, m), dim3(kRowThreads),
+           (size_t)(pz + kRowMax) * sizeof(float2), (const int*)c->items.p, od, c->ispec.p,
+           max_ex, max_ey, py, c->pz, c->Rf, tw + c->px.n + py, ctl);
+    launch(c, "pencil_y_fwd", k_pencil<true>, dim3((pz + c->Ty - 1) / c->Ty, m * max_ex),
+           dim3(kPencilThreads), ysh, c->ispec.p, c->py, pz, c->Ty, py, py, tw + c->px.n, od,
+           max_ex, (int)PENCIL_ITEM, ctl, 0);
+  } else {
+    launch(c, "item_slab_fwd", k_item_slab_fwd, dim3(max_ex, m), dim3(kSlabThreads),
+           slab_shmem(c), (const int*)c->items.p, od, c->ispec.p, max_ex, c->py, c->pz,
+           tw + c->px.n, tw + c->px.n + py, ctl);
+  }
   const int tiles = (pz + c->T - 1) / c->T;
   int nchunk = std::max(1, std::min(m, (2048 + tiles * py - 1) / (tiles * py)));
   const int chunk = (m + nchunk - 1) / nchunk;
@@ -584,13 +632,33 @@ void enqueue_search(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
          (const float2*)c->g1.p, (const float2*)c->g2.p, (const float2*)c->ispec.p, od, c->y1.p,
          c->y2.p, m, chunk, max_ex, c->nx, c->px, py, pz, c->T, scale, tw, ctl);
   const double* pen = c->pen.p + (penalty == SPK_PENALTY_CUBE ? c->nz : 0);
-  launch(c, "score_yz", k_score_yz, dim3(c->nx, m), dim3(kSlabThreads), score_shmem(c),
-         (const float2*)c->y1.p, (const float2*)c->y2.p, od, (const FieldStats*)c->stats.p, pen,
-         c->rec.p, c->nx, c->ny, c->nz, c->py, c->pz, tw + c->px.n, tw + c->px.n + py, fgrid,
-         pgrid, ctl);
-  if (fgrid) if_masked_select(c, *host_ods, pen);
+  const int rs = rec_stride(c);
+  if (c->split) {
+    launch(c, "pencil_y_inv", k_pencil<false>, dim3((pz + c->Ty - 1) / c->Ty, m * c->nx),
+           dim3(kPencilThreads), ysh, c->y1.p, c->py, pz, c->Ty, py, py, tw + c->px.n, od, c->nx,
+           (int)PENCIL_SCORE, ctl, 0);
+    launch(c, "pencil_y_inv", k_pencil<false>, dim3((hz + c->Ty - 1) / c->Ty, m * c->nx),
+           dim3(kPencilThreads), ysh, c->y2.p, c->py, hz, c->Ty, py, py, tw + c->px.n, od, c->nx,
+           (int)PENCIL_SCORE, ctl, 0);
+    launch(c, "rows_score", k_rows_score, dim3(rs, m), dim3(kRowThreads),
+           (size_t)(pz + 2 * c->Rs * pz) * sizeof(float2), (const float2*)c->y1.p,
+           (const float2*)c->y2.p, od, (const FieldStats*)c->stats.p, pen, c->rec.p, rs, c->nx,
+           c->ny, c->nz, py, c->pz, c->Rs, tw + c->px.n + py, fgrid, pgrid, ctl);
+  } else {
+    launch(c, "score_yz", k_score_yz, dim3(c->nx, m), dim3(kSlabThreads), score_shmem(c),
+           (const float2*)c->y1.p, (const float2*)c->y2.p, od, (const FieldStats*)c->stats.p,
+           pen, c->rec.p, c->nx, c->ny, c->nz, c->py, c->pz, tw + c->px.n, tw + c->px.n + py,
+           fgrid, pgrid, ctl);
+  }
   launch(c, "select", k_select, dim3(1), dim3(kSelectThreads), 0, (const WgBest*)c->rec.p, od, m,
-         c->nx, c->per.p, c->sout.p, c->ver.p, ctl);
+         rs, c->split ? c->Rs : 0, c->per.p, c->sout.p, c->ver.p, ctl);
+  if (fgrid) {
+    // interlocking-free: the summary above feeds the masked argmin, whose
+    // per-slice records are selected again
+    if_masked_select(c, *host_ods, pen);
+    launch(c, "select", k_select, dim3(1), dim3(kSelectThreads), 0, (const WgBest*)c->rec2.p, od,
+           m, c->nx, 0, c->per.p, c->sout.p, c->ver.p, ctl);
+  }
   launch(c, "verify", k_verify, dim3(std::min(512, (max_e + 255) / 256)), dim3(256), 0,
          (const int*)c->tray.p, (const int*)c->phi.p, (const int*)c->items.p, od,
          (const SearchOut*)c->sout.p, c->ver.p, c->ny, c->nz, ctl);
@@ -691,8 +759,8 @@ void do_search(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector
       sync(c);
     }
     if_masked_select(c, ods, pen);
-    launch(c, "select", k_select, dim3(1), dim3(kSelectThreads), 0, (const WgBest*)c->rec.p,
-           (const OrientDesc*)c->od.p, m, c->nx, c->per.p, c->sout.p, c->ver.p,
+    launch(c, "select", k_select, dim3(1), dim3(kSelectThreads), 0, (const WgBest*)c->rec2.p,
+           (const OrientDesc*)c->od.p, m, c->nx, 0, c->per.p, c->sout.p, c->ver.p,
            (const int*)nullptr);
     SearchOut s2;
     download(c, &s2, c->sout.p, sizeof(s2));
@@ -906,6 +974,7 @@ int spk_ctx_destroy(spk_ctx* c) {
     c->tw.release(); c->t1.release(); c->t2.release(); c->g1.release(); c->g2.release();
     c->pen.release(); c->items.release(); c->od.release(); c->ispec.release();
     c->y1.release(); c->y2.release(); c->rec.release(); c->per.release(); c->sout.release();
+    c->rec2.release();
     c->ver.release(); c->flag.release(); c->taps.release(); c->big.release();
     c->ffree.release(); c->freach.release(); c->pgrid.release(); c->exits.release();
     c->od_all.release(); c->pout.release(); c->ctl.release(); c->zero.release();
@@ -1069,6 +1138,31 @@ int spk_pack(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_items,
     c->generation += (uint64_t)placed;
     return SPK_OK;
   });
+}
+
+int spk_pack_many(spk_ctx* const* ctxs, int n, const spk_pack_args* args) {
+  if (!ctxs || !args || n <= 0) return fail(SPK_EINVAL, "bad pack_many arguments");
+  for (int i = 0; i < n; ++i)
+    if (!ctxs[i]) return fail(SPK_EINVAL, "null context in pack_many");
+  std::vector<int> rc(n, SPK_OK);
+  std::vector<std::string> msg(n);
+  // one host thread per instance: each drives its own context and stream,
+  // so the instances' kernels (and their host-controlled flood fills)
+  // overlap on the device
+  std::vector<std::thread> th;
+  th.reserve(n);
+  for (int i = 0; i < n; ++i) {
+    th.emplace_back([&, i]() {
+      const spk_pack_args& a = args[i];
+      rc[i] = spk_pack(ctxs[i], a.items, a.dims, a.n_items, a.order, a.orients, a.n_orient,
+                       a.penalty, a.interlocking_free, a.id_offset, a.results);
+      if (rc[i] != SPK_OK) msg[i] = spk_last_error();
+    });
+  }
+  for (auto& t : th) t.join();
+  for (int i = 0; i < n; ++i)
+    if (rc[i] != SPK_OK) return fail(rc[i], "instance " + std::to_string(i) + ": " + msg[i]);
+  return SPK_OK;
 }
 
 int spk_distance(spk_ctx* c, const int32_t* grid, int nx, int ny, int nz, int metric, int sources,
@@ -1331,6 +1425,14 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
   if (!c || !name) return fail(SPK_EINVAL, "null argument");
   if (std::string(name) == "resid_limit") {
     c->resid_limit = (float)value;
+    return SPK_OK;
+  }
+  if (std::string(name) == "plan") {
+    const int v = (int)value;
+    if (v < 0 || v > 2) return fail(SPK_EINVAL, "plan must be 0 (auto), 1 (slab) or 2 (split)");
+    c->plan_mode = v;
+    c->tw.release();   // re-plan at the next spk_tray_set
+    c->valid = false;
     return SPK_OK;
   }
   return fail(SPK_EINVAL, std::string("unknown option ") + name);

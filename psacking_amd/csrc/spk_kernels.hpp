@@ -718,20 +718,23 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
 constexpr int kSelectThreads = 1024;
 __global__ void __launch_bounds__(kSelectThreads)
 k_select(const WgBest* __restrict__ rec, const OrientDesc* __restrict__ od, int n_orient,
-         int nx, WgBest* __restrict__ per_orient, SearchOut* __restrict__ out,
-         long long* __restrict__ ver, const int* __restrict__ ctl) {
+         int rec_stride, int rows_per_rec, WgBest* __restrict__ per_orient,
+         SearchOut* __restrict__ out, long long* __restrict__ ver, const int* __restrict__ ctl) {
   if (search_off(ctl)) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid < 3) ver[tid] = 0;   // accumulator of the following k_verify
   for (int o = wave; o < n_orient; o += kSelectThreads / 64) {
-    const int nvx = od[o].nvx;
+    const OrientDesc d = od[o];
+    // slab plan: one record per valid x slice; split plan: one per block of
+    // rows_per_rec valid (x, y) rows
+    const int nrec = rows_per_rec ? (d.nvx * d.nvy + rows_per_rec - 1) / rows_per_rec : d.nvx;
     double sc = 1e300;
     int idx = INT_MAX;
     long long p = 0;
     float res = 0.f;
     int uns = 0;
-    for (int x = lane; x < nvx; x += 64) {
-      const WgBest r = rec[(size_t)o * nx + x];
+    for (int x = lane; x < nrec; x += 64) {
+      const WgBest r = rec[(size_t)o * rec_stride + x];
       if (better(r.score, r.idx, sc, idx)) {
         sc = r.score; idx = r.idx;
         p = ((long long)r.p_hi << 32) | (unsigned int)r.p_lo;
@@ -895,6 +898,234 @@ __global__ void k_stats_init(FieldStats* __restrict__ st, const int* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// split plan: the yz transforms as z-row kernels plus y-pencil kernels, each
+// with a small LDS tile (used when a whole yz slab does not fit LDS, or when
+// the autotuner finds it faster).
+// ---------------------------------------------------------------------------
+constexpr int kRowThreads = 256;
+constexpr int kRowMaxEl = 16;
+constexpr int kRowMax = kRowThreads * kRowMaxEl;   // complex elements per row block
+
+enum { PENCIL_PLAIN = 0, PENCIL_SCORE = 1, PENCIL_ITEM = 2 };
+
+// In-place FFT along n of complex data laid out [A][n][B]; block (b-tile, a)
+// takes the T pencils b0..b0+T-1 of slab a.  Inputs at index >= n_in read as
+// zero, outputs at index >= n_out are not stored.  PENCIL_SCORE: slab
+// a = o*nx + x is skipped for x >= nvx(o) and stores only j < nvy(o);
+// PENCIL_ITEM: slab a = o*nx + x is skipped for x >= ex(o) and reads j < ey(o).
+template <bool FWD>
+__global__ void __launch_bounds__(kPencilThreads)
+k_pencil(float2* __restrict__ data, AxisPlan p_, int B, int T, int n_in, int n_out,
+         const float2* __restrict__ tw_g, const OrientDesc* __restrict__ od, int nx, int mode,
+         const int* __restrict__ ctl, int refresh_gate) {
+  if (refresh_gate ? refresh_off(ctl) : search_off(ctl)) return;
+  extern __shared__ float2 smem[];
+  const int n = p_.n;
+  const int a = blockIdx.y, b0 = blockIdx.x * T, tid = threadIdx.x;
+  if (b0 >= B) return;
+  if (mode != PENCIL_PLAIN) {
+    const int o = a / nx, x = a - o * nx;
+    const OrientDesc d = od[o];
+    if (mode == PENCIL_SCORE) {
+      if (x >= d.nvx) return;
+      n_out = min(n_out, d.nvy);
+    } else {
+      if (x >= d.ex) return;
+      n_in = min(n_in, d.ey);
+    }
+  }
+  float2* tw = smem;
+  float2* buf = tw + n;
+  load_table(tw, tw_g, n, tid, kPencilThreads);
+  float2* base = data + (size_t)a * n * B + b0;
+  {
+    float2 v[kPencilMaxEl];
+#pragma unroll
+    for (int s = 0; s < kPencilMaxEl; ++s) {
+      const int e = tid + s * kPencilThreads;
+      const int y = e / T, t = e - y * T;
+      v[s] = make_float2(0.f, 0.f);
+      if (e < n * T && y < n_in && b0 + t < B) v[s] = base[(size_t)y * B + t];
+    }
+#pragma unroll
+    for (int s = 0; s < kPencilMaxEl; ++s) {
+      const int e = tid + s * kPencilThreads;
+      if (e < n * T) buf[e] = v[s];
+    }
+  }
+  __syncthreads();
+  lds_fft<FWD, kPencilThreads, kPencilMaxEl>(buf, p_, T, T, 1, false, tw, tid);
+  for (int e = tid; e < n_out * T; e += kPencilThreads) {
+    const int y = e / T, t = e - y * T;
+    if (b0 + t < B) base[(size_t)y * B + t] = buf[e];
+  }
+}
+
+// forward z FFT of tray rows: grid (ceil(ny/R) * nx, 2 fields).  Field 0 rows
+// are B + i*h (full pz), field 1 rows are l (first hz stored).
+__global__ void __launch_bounds__(kRowThreads)
+k_tray_rows_fwd(const int* __restrict__ tray, const int* __restrict__ phi,
+                const FieldStats* __restrict__ st, float2* __restrict__ t1,
+                float2* __restrict__ t2, int nx, int ny, int nz, int py, AxisPlan pz_, int R,
+                const float2* __restrict__ tw_z, const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
+  extern __shared__ float2 smem[];
+  const int pz = pz_.n, hz = pz / 2 + 1, tid = threadIdx.x, field = blockIdx.y;
+  const int jb = (ny + R - 1) / R;
+  const int x = blockIdx.x / jb, j0 = (blockIdx.x - x * jb) * R;
+  const int rows = min(R, ny - j0);
+  float2* tw = smem;
+  float2* buf = tw + pz;
+  load_table(tw, tw_z, pz, tid, kRowThreads);
+  const FieldParams fp = field_params(st, (long long)nx * ny * nz);
+  const int half = (1 << fp.lb) >> 1;
+  const float inv_pz = 1.0f / (float)pz;
+  int dv[kRowMaxEl], tv[kRowMaxEl];
+#pragma unroll
+  for (int s = 0; s < kRowMaxEl; ++s) {
+    const int e = tid + s * kRowThreads;
+    const int r = fdiv(e, pz, inv_pz), k = e - r * pz;
+    dv[s] = 0;
+    tv[s] = 0;
+    if (r < rows && k < nz) {
+      const size_t g = ((size_t)x * ny + j0 + r) * nz + k;
+      dv[s] = phi[g];
+      if (field == 0) tv[s] = tray[g];
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < kRowMaxEl; ++s) {
+    const int e = tid + s * kRowThreads;
+    const int r = fdiv(e, pz, inv_pz), k = e - r * pz;
+    if (r >= rows) continue;
+    float2 v = make_float2(0.f, 0.f);
+    if (k < nz) {
+      const int d = dv[s] - fp.mu;
+      const int h = d >> fp.lb;
+      if (field == 0) v = make_float2(tv[s] != 0 ? 1.f : 0.f, (float)h);
+      else v.x = (float)(d - (h << fp.lb) - half);
+    }
+    buf[e] = v;
+  }
+  __syncthreads();
+  lds_fft<true, kRowThreads, kRowMaxEl>(buf, pz_, 1, rows, pz, true, tw, tid);
+  const int w = field == 0 ? pz : hz;
+  float2* dst = field == 0 ? t1 : t2;
+  for (int e = tid; e < rows * w; e += kRowThreads) {
+    const int r = e / w, k = e - r * w;
+    dst[((size_t)x * py + j0 + r) * w + k] = buf[r * pz + k];
+  }
+}
+
+// forward z FFT of oriented item rows: grid (ceil(max_ey/R) * max_ex, m).
+__global__ void __launch_bounds__(kRowThreads)
+k_item_rows_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od,
+                float2* __restrict__ ispec, int max_ex, int max_ey, int py, AxisPlan pz_, int R,
+                const float2* __restrict__ tw_z, const int* __restrict__ ctl) {
+  if (search_off(ctl)) return;
+  extern __shared__ float2 smem[];
+  const int pz = pz_.n, tid = threadIdx.x, o = blockIdx.y;
+  const int jb = (max_ey + R - 1) / R;
+  const int x = blockIdx.x / jb, j0 = (blockIdx.x - x * jb) * R;
+  const OrientDesc d = od[o];
+  if (x >= d.ex || j0 >= d.ey) return;
+  const int rows = min(R, d.ey - j0);
+  float2* tw = smem;
+  float2* buf = tw + pz;
+  load_table(tw, tw_z, pz, tid, kRowThreads);
+  const int* base = items + d.item_off;
+  const float inv_pz = 1.0f / (float)pz;
+  int iv[kRowMaxEl];
+#pragma unroll
+  for (int s = 0; s < kRowMaxEl; ++s) {
+    const int e = tid + s * kRowThreads;
+    const int r = fdiv(e, pz, inv_pz), k = e - r * pz;
+    iv[s] = (r < rows && k < d.ez) ? base[oriented_index(d, x, j0 + r, k)] : 0;
+  }
+#pragma unroll
+  for (int s = 0; s < kRowMaxEl; ++s) {
+    const int e = tid + s * kRowThreads;
+    if (e < rows * pz) buf[e] = make_float2((float)iv[s], 0.f);
+  }
+  __syncthreads();
+  lds_fft<true, kRowThreads, kRowMaxEl>(buf, pz_, 1, rows, pz, true, tw, tid);
+  float2* dst = ispec + (((size_t)o * max_ex + x) * py + j0) * pz;
+  for (int e = tid; e < rows * pz; e += kRowThreads) dst[e] = buf[e];
+}
+
+// pass B, split plan: grid (ceil(nx*ny/R), m).  Block takes R rows r of the
+// orientation's valid rows (x = r / nvy, j = r % nvy): inverse z FFT of the
+// Hermitian low-digit row and of the complex (c, h) row, round, score, argmin.
+__global__ void __launch_bounds__(kRowThreads)
+k_rows_score(const float2* __restrict__ y1, const float2* __restrict__ y2,
+             const OrientDesc* __restrict__ od, const FieldStats* __restrict__ st,
+             const double* __restrict__ pen, WgBest* __restrict__ rec, int rec_stride, int nx,
+             int ny, int nz, int py, AxisPlan pz_, int R, const float2* __restrict__ tw_z,
+             unsigned char* __restrict__ fgrid, long long* __restrict__ pgrid,
+             const int* __restrict__ ctl) {
+  if (search_off(ctl)) return;
+  extern __shared__ float2 smem[];
+  const int pz = pz_.n, hz = pz / 2 + 1, tid = threadIdx.x, o = blockIdx.y;
+  const OrientDesc d = od[o];
+  const int nrows = d.nvx * d.nvy;
+  const int r0 = blockIdx.x * R;
+  if (r0 >= nrows) return;
+  const int rows = min(R, nrows - r0);
+  float2* tw = smem;
+  float2* buf = tw + pz;              // rows [0, R): low digit, [R, 2R): (c, h)
+  load_table(tw, tw_z, pz, tid, kRowThreads);
+  const size_t so = (size_t)o * nx;
+  for (int e = tid; e < rows * hz; e += kRowThreads) {
+    const int r = e / hz, k = e - r * hz;
+    const int rr = r0 + r, x = rr / d.nvy, j = rr - x * d.nvy;
+    buf[r * pz + k] = y2[((so + x) * py + j) * hz + k];
+  }
+  for (int e = tid; e < rows * pz; e += kRowThreads) {
+    const int r = e / pz, k = e - r * pz;
+    const int rr = r0 + r, x = rr / d.nvy, j = rr - x * d.nvy;
+    buf[(R + r) * pz + k] = y1[((so + x) * py + j) * pz + k];
+  }
+  __syncthreads();
+  for (int e = tid; e < rows * (pz - hz); e += kRowThreads) {
+    const int r = e / (pz - hz), k = hz + (e - r * (pz - hz));
+    const float2 v = buf[r * pz + (pz - k)];
+    buf[r * pz + k] = make_float2(v.x, -v.y);
+  }
+  // rows [rows, R) of the low-digit half stay unused; transform 2R rows so
+  // both halves line up (the unused ones hold stale data and are not read)
+  __syncthreads();
+  lds_fft<false, kRowThreads, kRowMaxEl>(buf, pz_, 1, 2 * R, pz, true, tw, tid);
+  const FieldParams fp = field_params(st, (long long)nx * ny * nz);
+  const long long b = 1LL << fp.lb;
+  const long long addback = (b / 2 + fp.mu) * d.isum;
+  double best = 1e300;
+  int bidx = INT_MAX;
+  long long bp = 0;
+  float res = 0.f;
+  for (int e = tid; e < rows * d.nvz; e += kRowThreads) {
+    const int r = e / d.nvz, k = e - r * d.nvz;
+    const int rr = r0 + r, x = rr / d.nvy, j = rr - x * d.nvy;
+    const float l = buf[r * pz + k].x;
+    const float2 v = buf[(R + r) * pz + k];
+    const float lr = rintf(l), c = rintf(v.x), h = rintf(v.y);
+    res = fmaxf(res, fmaxf(fabsf(l - lr), fmaxf(fabsf(v.x - c), fabsf(v.y - h))));
+    const long long p = b * (long long)h + (long long)lr + addback;
+    const int idx = (x * ny + j) * nz + k;
+    if (fgrid) {
+      const size_t g = (size_t)o * nx * ny * nz + idx;
+      fgrid[g] = c == 0.f ? 1 : 0;
+      pgrid[g] = p;
+    }
+    if (c == 0.f) {
+      const double sc = (double)p + pen[k];
+      if (better(sc, idx, best, bidx)) { best = sc; bidx = idx; bp = p; }
+    }
+  }
+  wg_argmin_write(best, bidx, bp, res, fp.ok ? 0 : 1, rec + (size_t)o * rec_stride + blockIdx.x,
+                  tid, kRowThreads);
+}
+
+// ---------------------------------------------------------------------------
 // exact direct-sum kernels (fallback search, standalone full-grid ops)
 // item given as a list of (dx, dy, dz, value) non-zero voxels.
 // ---------------------------------------------------------------------------
@@ -1043,12 +1274,14 @@ k_ff_tile(const unsigned char* __restrict__ fr_all, unsigned char* __restrict__ 
   if (tid == 0 && gch) atomicOr(changed, 1);
 }
 
-// grid (nx, m): argmin of double(p) + penalty[k] over reached offsets of slice x;
-// keeps the residual/unsupported fields the score kernel wrote in the record.
+// grid (nx, m): argmin of double(p) + penalty[k] over reached offsets of slice x
+// into rec[o*nx + x]; the residual/unsupported fields come from the
+// orientation's unmasked summary per_o[o] (a k_select over the score records).
 __global__ void __launch_bounds__(256)
 k_masked_argmin(const unsigned char* __restrict__ reached, const long long* __restrict__ pgrid,
                 const OrientDesc* __restrict__ od, const double* __restrict__ pen,
-                WgBest* __restrict__ rec, int nx, int ny, int nz) {
+                const WgBest* __restrict__ per_o, WgBest* __restrict__ rec, int nx, int ny,
+                int nz) {
   const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
   const OrientDesc d = od[o];
   if (x >= d.nvx) return;
@@ -1067,11 +1300,8 @@ k_masked_argmin(const unsigned char* __restrict__ reached, const long long* __re
     if (better(sc, idx, best, bidx)) { best = sc; bidx = idx; bp = p; }
   }
   WgBest* out = rec + (size_t)o * nx + x;
-  __shared__ float r0;
-  __shared__ int u0;
-  if (tid == 0) { r0 = out->resid; u0 = out->unsupported; }
-  __syncthreads();
-  wg_argmin_write(best, bidx, bp, r0, u0, out, tid, 256);
+  const WgBest s = per_o[o];
+  wg_argmin_write(best, bidx, bp, s.resid, s.unsupported, out, tid, 256);
 }
 
 }  // namespace spk

@@ -331,3 +331,45 @@ def test_search_forced_exact_interlocking(P):
             assert (tuple(r["position"]), r["orientation"], r["score"]) == \
                 (e.position, e.orientation_index, e.score)
     np.testing.assert_array_equal(s.tray(), exp.tray)
+
+
+def _pack_with_plan(P, w, n_orient, plan, metric="l1", sources="eq1", interlocking_free=False):
+    from psacking_amd.rotations import orientation_maps
+    items = w.items
+    vols = [int((it > 0).sum()) for it in items]
+    order = [int(i) for i in np.argsort(vols)[::-1]]
+    perms, flips = orientation_maps(n_orient)
+    s = P.Session(0)
+    s.set_option("plan", plan)
+    s.set_tray(np.zeros(w.tray_size, np.int32), metric, sources, 0)
+    res = s.pack(items, np.array(order, np.int32), perms, flips,
+                 "pow" if n_orient == 1 and not interlocking_free else "cube", interlocking_free, 0)
+    exp = O.pack_voxels(items, w.tray_size, num_orientations=n_orient, order=order,
+                        metric=metric, sources=sources, interlocking_free=interlocking_free)
+    by_item = {p.item_index: p for p in exp.placements}
+    for t, r in enumerate(res):
+        e = by_item[order[t]]
+        assert r["found"] == e.success
+        if e.success:
+            assert (tuple(r["position"]), r["orientation"], r["score"]) == \
+                (e.position, e.orientation_index, e.score)
+    np.testing.assert_array_equal(s.tray(), exp.tray)
+
+
+@pytest.mark.parametrize("plan", [1, 2])
+def test_plans_agree_with_oracle(P, plan):
+    from psacking_amd import workloads as W
+    say("plan", plan, "24 orientations edt")
+    _pack_with_plan(P, W.blobs(seed=17, n=8, tray=(50, 44, 36)), 24, plan, "edt", "nonzero")
+    say("plan", plan, "1 orientation l1")
+    _pack_with_plan(P, W.boxes(seed=18, n=10, tray=(33, 29, 27)), 1, plan)
+    say("plan", plan, "interlocking free")
+    _pack_with_plan(P, W.blobs(seed=19, n=3, tray=(30, 26, 22)), 4, plan, interlocking_free=True)
+
+
+def test_large_plane_tray_uses_split_plan(P):
+    # y*z = 150*120 exceeds one block's LDS for the slab plan
+    from psacking_amd import workloads as W
+    w = W.blobs(seed=20, n=3, tray=(60, 150, 120))
+    say("large-plane tray (auto split)")
+    _pack_with_plan(P, w, 4, 0, "edt", "nonzero")
