@@ -36,6 +36,7 @@ from ._core import (  # noqa: F401
     gpu_tray_search,
     interlocking_free_positions,
     make_tight,
+    pack_many,
     place_in_tray,
     save_vox,
     voxelize_stl,
@@ -57,7 +58,7 @@ __all__ = [
     "fft_search_placement", "fft_search_placement_with_cache", "fft_search_batch",
     "fft_search_batch_interlocking_free", "place_in_tray", "voxelize_stl", "dft_conv3",
     "dft_corr3", "calculate_distance", "distance_field", "collision_grid", "make_tight",
-    "get_bounds", "save_vox", "gpu_tray_init", "gpu_tray_cleanup", "gpu_tray_is_initialized",
+    "get_bounds", "pack_many", "save_vox", "gpu_tray_init", "gpu_tray_cleanup", "gpu_tray_is_initialized",
     "gpu_tray_search", "interlocking_free_positions", "get_orientations",
     "get_24_orientations", "orientation_maps", "make_contiguous", "VOXEL_RESOLUTION",
     "HEIGHT_PENALTY",

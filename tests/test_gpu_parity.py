@@ -373,3 +373,36 @@ def test_large_plane_tray_uses_split_plan(P):
     w = W.blobs(seed=20, n=3, tray=(60, 150, 120))
     say("large-plane tray (auto split)")
     _pack_with_plan(P, w, 4, 0, "edt", "nonzero")
+
+
+def test_pack_many_uneven_trays(P):
+    """Independent instances of different tray sizes packed concurrently,
+    interlocking-free check after each placement, each equal to its own
+    oracle run."""
+    from psacking_amd import workloads as W
+    from psacking_amd.rotations import orientation_maps
+    ws = [W.blobs(seed=21, n=4, tray=(34, 30, 26)),
+          W.blobs(seed=22, n=5, tray=(48, 28, 20)),
+          W.blobs(seed=23, n=3, tray=(26, 40, 30))]
+    perms, flips = orientation_maps(4)
+    sessions, lists, orders = [], [], []
+    for w in ws:
+        s = P.Session(0)
+        s.set_tray(np.zeros(w.tray_size, np.int32), "l1", "eq1", 0)
+        vols = [int((it > 0).sum()) for it in w.items]
+        sessions.append(s)
+        lists.append(list(w.items))
+        orders.append(np.array([int(i) for i in np.argsort(vols)[::-1]], np.int32))
+    say("pack_many 3 uneven trays, 4 orientations, interlocking free")
+    got = P.pack_many(sessions, lists, orders, perms, flips, "cube", True)
+    for w, s, res, order in zip(ws, sessions, got, orders):
+        exp = O.pack_voxels(w.items, w.tray_size, num_orientations=4, order=list(order),
+                            interlocking_free=True)
+        by_item = {p.item_index: p for p in exp.placements}
+        for t, r in enumerate(res):
+            e = by_item[int(order[t])]
+            assert r["found"] == e.success
+            if e.success:
+                assert (tuple(r["position"]), r["orientation"], r["score"]) == \
+                    (e.position, e.orientation_index, e.score)
+        np.testing.assert_array_equal(s.tray(), exp.tray)
