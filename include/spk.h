@@ -164,11 +164,85 @@ int spk_timing_reset(spk_ctx* ctx);
  *   "resid_limit"  largest accepted distance of an FFT output from its
  *                  integer before a search is redone on the exact direct-sum
  *                  path (default 0.25; a negative value forces the exact path
- *                  for every search, which the tests use to check that path). */
+ *                  for every search, which the tests use to check that path).
+ *   "plan"         FFT plan of the tray: 0 auto, 1 yz slabs in one block,
+ *                  2 z rows + y pencils (large y*z planes); applies at the
+ *                  next spk_tray_set. */
 int spk_set_option(spk_ctx* ctx, const char* name, double value);
 
 /* Plan actually used for the cached tray: FFT sizes and digit base. */
 int spk_plan_info(spk_ctx* ctx, int* px, int* py, int* pz, int* digit_base);
+
+/* ---- Device memory (no reference counterpart) -------------------------------
+ * For callers that hold no HIP runtime of their own (ctypes, other
+ * languages): device buffers for the proposer's features, targets and masks.
+ * Every call synchronises the device.  kind: 0 host->device, 1 device->host,
+ * 2 device->device.  spk_dev_gather copies rows idx[0..n) (host array, each
+ * in [0, src_rows)) of src into consecutive rows of dst; row_bytes must be a
+ * multiple of 16. */
+int spk_dev_alloc(int device, int64_t bytes, void** out);
+int spk_dev_free(int device, void* ptr);
+int spk_dev_copy(int device, void* dst, const void* src, int64_t bytes, int kind);
+int spk_dev_gather(int device, void* dst, const void* src, int64_t row_bytes, int64_t src_rows,
+                   const int64_t* idx, int n);
+
+/* ---- Learned orientation proposer (no reference counterpart) --------------
+ * The reference searches every orientation (spectral_packer/packer.py:487-505,
+ * fft3.cu:1549-1632).  A 3-D CNN (262,760 parameters, bf16 weights and
+ * activations, fp32 accumulation) maps the item's occupancy at pitch 2
+ * (32^3 cells) and the tray's occupancy and below-surface fraction on a
+ * 64x32x32 grid to 24 orientation logits; the exact search judges its top-k
+ * proposals.  Feature buffers, targets and masks are DEVICE pointers (bf16
+ * bits as uint16); logits, parameters and gradients are host buffers.  One
+ * proposer is used by one thread at a time. */
+typedef struct spk_nn spk_nn;
+#define SPK_NN_ITEM_FEATURES 32768    /* [32][32][32][1] */
+#define SPK_NN_TRAY_FEATURES 131072   /* [64][32][32][2] */
+#define SPK_NN_LOGITS 24
+
+int spk_nn_create(int device, int max_batch, spk_nn** out);
+int spk_nn_destroy(spk_nn* nn);
+int spk_nn_num_params(spk_nn* nn, int64_t* n);
+/* fp32 master parameters, layer by layer: W [cout][27*cin or cin] then b [cout];
+ * setting them clears the optimizer state. */
+int spk_nn_set_params(spk_nn* nn, const float* params, int64_t n);
+int spk_nn_get_params(spk_nn* nn, float* params, int64_t n);
+/* logits [batch][24] of batch (item, tray) feature pairs */
+int spk_nn_forward(spk_nn* nn, const uint16_t* item_feat, const uint16_t* tray_feat, int batch,
+                   float* logits);
+/* soft-target cross entropy over the orientations whose mask bit is set:
+ * mean loss and d(loss)/d(params) for one batch, parameters unchanged */
+int spk_nn_gradient(spk_nn* nn, const uint16_t* item_feat, const uint16_t* tray_feat,
+                    const float* target, const int32_t* mask, int batch, float* grads,
+                    float* loss);
+/* the same, followed by one Adam step (beta 0.9/0.999, eps 1e-8) */
+int spk_nn_train_step(spk_nn* nn, const uint16_t* item_feat, const uint16_t* tray_feat,
+                      const float* target, const int32_t* mask, int batch, float lr,
+                      float* loss);
+/* features of a host item (base orientation) into a device buffer */
+int spk_nn_item_features(spk_nn* nn, const int32_t* item, int ix, int iy, int iz,
+                         uint16_t* out);
+/* features of the context's cached tray into a device buffer */
+int spk_tray_features(spk_ctx* ctx, uint16_t* out);
+
+/* spk_pack (plain, not interlocking-free) that also records, for item t, the
+ * tray features it saw (tray_feat + t*SPK_NN_TRAY_FEATURES, device) and each
+ * orientation's best score (scores[t*n_orient + o], host; +inf when the
+ * orientation does not fit or has no feasible offset). */
+int spk_pack_records(spk_ctx* ctx, const spk_pack_args* args, uint16_t* tray_feat,
+                     double* scores);
+
+/* The greedy loop with learned proposals (plain, not interlocking-free; at
+ * most 24 orientations).  For each item with more than topk fitting
+ * orientations whose topk best-scored ones hold a softmax mass >= min_conf,
+ * only those are searched and the best placement among them is taken; every
+ * fitting orientation is searched otherwise, or when none of them is
+ * feasible.  proposals[t*topk + j] (optional) lists the searched proposals
+ * (-1 when the item was searched over all its orientations); counts[4]
+ * (optional): items decided on proposals, full searches by rule, full
+ * searches after infeasible proposals, exact host-path decisions. */
+int spk_pack_learned(spk_ctx* ctx, spk_nn* nn, const spk_pack_args* args, int topk,
+                     float min_conf, int32_t* proposals, int32_t* counts);
 
 #ifdef __cplusplus
 }

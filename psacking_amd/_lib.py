@@ -19,6 +19,10 @@ EXPORTS = (
     "spk_place", "spk_pack", "spk_pack_many", "spk_distance", "spk_correlate", "spk_collision_grid",
     "spk_interlock_mask", "spk_place_grid", "spk_bounds", "spk_save_vox", "spk_timing_enable",
     "spk_timing_get", "spk_timing_reset", "spk_set_option", "spk_plan_info",
+    "spk_dev_alloc", "spk_dev_free", "spk_dev_copy", "spk_dev_gather",
+    "spk_nn_create", "spk_nn_destroy", "spk_nn_num_params", "spk_nn_set_params",
+    "spk_nn_get_params", "spk_nn_forward", "spk_nn_gradient", "spk_nn_train_step",
+    "spk_nn_item_features", "spk_tray_features", "spk_pack_records", "spk_pack_learned",
 )
 
 
@@ -32,6 +36,14 @@ class SpkResult(C.Structure):
                 ("proximity", C.c_int64), ("residual", C.c_float), ("reserved", C.c_int32)]
 
 
+class SpkPackArgs(C.Structure):
+    _fields_ = [("items", C.POINTER(C.c_int32)), ("dims", C.POINTER(C.c_int32)),
+                ("n_items", C.c_int), ("order", C.POINTER(C.c_int32)),
+                ("orients", C.POINTER(SpkOrient)), ("n_orient", C.c_int), ("penalty", C.c_int),
+                ("interlocking_free", C.c_int), ("id_offset", C.c_int32),
+                ("results", C.POINTER(SpkResult))]
+
+
 _lib = None
 
 
@@ -43,6 +55,25 @@ def load() -> C.CDLL:
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m psacking_amd.build`")
     lib = C.CDLL(LIB_PATH)
     lib.spk_last_error.restype = C.c_char_p
+    vp, i32, i64, f32 = C.c_void_p, C.c_int, C.c_int64, C.c_float
+    fp, dp = C.POINTER(C.c_float), C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    lib.spk_dev_alloc.argtypes = [i32, i64, C.POINTER(vp)]
+    lib.spk_dev_free.argtypes = [i32, vp]
+    lib.spk_dev_copy.argtypes = [i32, vp, vp, i64, i32]
+    lib.spk_dev_gather.argtypes = [i32, vp, vp, i64, i64, C.POINTER(C.c_int64), i32]
+    lib.spk_nn_create.argtypes = [i32, i32, C.POINTER(vp)]
+    lib.spk_nn_destroy.argtypes = [vp]
+    lib.spk_nn_num_params.argtypes = [vp, C.POINTER(i64)]
+    lib.spk_nn_set_params.argtypes = [vp, fp, i64]
+    lib.spk_nn_get_params.argtypes = [vp, fp, i64]
+    lib.spk_nn_forward.argtypes = [vp, vp, vp, i32, fp]
+    lib.spk_nn_gradient.argtypes = [vp, vp, vp, vp, vp, i32, fp, fp]
+    lib.spk_nn_train_step.argtypes = [vp, vp, vp, vp, vp, i32, f32, fp]
+    lib.spk_nn_item_features.argtypes = [vp, ip, i32, i32, i32, vp]
+    lib.spk_tray_features.argtypes = [vp, vp]
+    lib.spk_pack_records.argtypes = [vp, C.POINTER(SpkPackArgs), vp, dp]
+    lib.spk_pack_learned.argtypes = [vp, vp, C.POINTER(SpkPackArgs), i32, f32, ip, ip]
     _lib = lib
     return lib
 

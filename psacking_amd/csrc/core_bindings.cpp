@@ -564,7 +564,9 @@ PYBIND11_MODULE(_core, m) {
       .def("timing", &Session::timing, py::arg("on"))
       .def("timing_get", &Session::timing_get, py::arg("name"))
       .def("timing_reset", &Session::timing_reset)
-      .def("plan", &Session::plan);
+      .def("plan", &Session::plan)
+      .def("handle", [](const Session& s) { return (uintptr_t)s.ctx(); },
+           "the spk_ctx* of this session, for C-ABI calls (ctypes)");
 
   m.def("pack_many", &py_pack_many, py::arg("sessions"), py::arg("item_lists"), py::arg("orders"),
         py::arg("perms"), py::arg("flips"), py::arg("penalty") = "cube",

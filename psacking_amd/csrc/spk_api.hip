@@ -162,6 +162,13 @@ struct spk_ctx {
   DevBuf<int> ctl;                       // CTL_* control words
   DevBuf<int> zero;                      // an int that stays 0 (overlap flag of pipelined places)
   float resid_limit = 0.25f;             // largest accepted FFT rounding residual
+  // learned proposals and training records
+  DevBuf<int> heights;                   // column heights [nx][ny]
+  DevBuf<int> map_all;                   // orientation index of each od_all entry
+  DevBuf<unsigned short> tfeat;          // tray features [64][32][32][2] (bf16)
+  DevBuf<unsigned short> ifeat, ipool;   // item features [n][32^3], pooled item rows [n][384]
+  DevBuf<double> rscores;                // per-orientation best scores [n][n_orient]
+  DevBuf<OrientDesc> od_prop;            // descriptors of the proposed orientations
   // timing
   bool timing = false;
   std::vector<Timed> pending;
@@ -267,6 +274,12 @@ void check_dims(int nx, int ny, int nz) {
   if (nx <= 0 || ny <= 0 || nz <= 0)
     throw SpkError{SPK_EINVAL, "grid dimensions must be positive"};
 }
+
+}  // namespace
+
+#include "spk_nn_host.hpp"
+
+namespace {
 
 size_t slab_shmem(const spk_ctx* c) {
   return (size_t)(c->py.n + c->pz.n + c->py.n * c->pz.n) * sizeof(float2);
@@ -813,31 +826,35 @@ void do_place(spk_ctx* c, const OrientDesc& d, int x, int y, int z, int id, bool
   if (refresh) refresh_field_and_spectra(c, nullptr);
 }
 
-// The greedy loop with every item enqueued back to back: search, k_decide,
-// placement and refresh of item t+1 follow item t on the stream with no host
-// round trip.  k_decide hands a search back to the host (CTL_HALT) when its
-// FFT result cannot be trusted; the kernels of the remaining items then exit
-// at once, the host redoes that item on the exact direct-sum path, places it
-// and resumes the pipeline after it.  Returns the number of items placed.
-int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector<int>& off,
-                   const int32_t* order, const spk_orient* orients, int n_orient, int penalty,
-                   int32_t id_offset, spk_result* results) {
+// Orientations of every item that fit the tray, in processing order.
+struct ItemPlan {
+  int od_off = 0, m = 0, max_ex = 1, max_e = 1;
+  std::vector<int> map;   // fitting index -> orientation index
+};
+
+struct PackPlan {
+  std::vector<ItemPlan> items;
+  std::vector<OrientDesc> all;    // descriptors, item after item
+};
+
+// Builds the plan, uploads the descriptors (c->od_all) and their orientation
+// indices (c->map_all) and sizes the search workspace and loop state.
+PackPlan plan_pack(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector<int>& off,
+                   const int32_t* order, const spk_orient* orients, int n_orient) {
   const int n_items = (int)bases.size();
-  struct Plan {
-    int od_off = 0, m = 0, max_ex = 1, max_e = 1;
-    std::vector<int> map;   // fitting index -> orientation index
-  };
-  std::vector<Plan> plans(n_items);
-  std::vector<OrientDesc> all;
+  PackPlan P;
+  P.items.resize(n_items);
+  std::vector<int> map_all;
   int ws_m = 1, ws_ex = 1;
   for (int t = 0; t < n_items; ++t) {
     const int it = order ? order[t] : t;
-    Plan& pl = plans[t];
-    pl.od_off = (int)all.size();
+    ItemPlan& pl = P.items[t];
+    pl.od_off = (int)P.all.size();
     for (int o = 0; o < n_orient; ++o) {
       OrientDesc d = make_desc(c, bases[it], off[it], orients ? &orients[o] : nullptr);
       if (!fits(c, d)) continue;
-      all.push_back(d);
+      P.all.push_back(d);
+      map_all.push_back(o);
       pl.map.push_back(o);
       pl.max_ex = std::max(pl.max_ex, d.ex);
       pl.max_e = std::max(pl.max_e, d.ex * d.ey * d.ez);
@@ -846,13 +863,109 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
     ws_m = std::max(ws_m, pl.m);
     ws_ex = std::max(ws_ex, pl.max_ex);
   }
-  c->od_all.ensure(std::max<size_t>(all.size(), 1));
-  if (!all.empty()) upload(c, c->od_all.p, all.data(), all.size() * sizeof(OrientDesc));
+  c->od_all.ensure(std::max<size_t>(P.all.size(), 1));
+  c->map_all.ensure(std::max<size_t>(map_all.size(), 1));
+  if (!P.all.empty()) {
+    upload(c, c->od_all.p, P.all.data(), P.all.size() * sizeof(OrientDesc));
+    upload(c, c->map_all.p, map_all.data(), map_all.size() * sizeof(int));
+  }
   ensure_search_ws(c, ws_m, ws_ex);
-  c->pout.ensure(n_items);
+  c->pout.ensure(std::max(n_items, 1));
   c->ctl.ensure(CTL_WORDS);
   c->zero.ensure(1);
   HIP_OK(hipMemsetAsync(c->zero.p, 0, sizeof(int), c->stream));
+  return P;
+}
+
+// One item of the device-driven loop: batched search over m descriptors (od),
+// k_decide, optional per-orientation score record, placement of the accepted
+// winner and refresh of the distance field and spectra.
+void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
+                  int t, int32_t id_offset, int halt_not_found, double* rec_row,
+                  const int* rec_map) {
+  enqueue_search(c, od, m, max_ex, max_e, penalty, c->ctl.p, nullptr, nullptr, nullptr);
+  launch(c, "decide", k_decide, dim3(1), dim3(64), 0, (const SearchOut*)c->sout.p,
+         (const long long*)c->ver.p, c->pout.p, t, c->resid_limit, c->ctl.p, halt_not_found);
+  if (rec_row)
+    launch(c, "record", k_record_scores, dim3(1), dim3(64), 0, (const WgBest*)c->per.p, m, rec_map,
+           rec_row, (const int*)c->ctl.p);
+  const int nb = std::max(1, std::min(1024, (max_e + 255) / 256));
+  launch(c, "place", k_place, dim3(nb), dim3(256), 0, c->tray.p, (const int*)c->items.p, od,
+         (const SearchOut*)c->sout.p, 0, 0, 0, 0, (int)id_offset, c->nx, c->ny, c->nz, 1,
+         c->zero.p, (const int*)c->ctl.p);
+  refresh_field_and_spectra(c, nullptr, c->ctl.p);
+}
+
+struct ExactBest {
+  int fit = -1;            // index into the item's fitting orientations, -1 = none
+  int idx = 0;
+  double score = 0.0;
+  long long prox = 0;
+};
+
+// Exact direct-sum search of the listed fitting orientations (ods indexed by
+// fitting index); the first strictly lower score wins.  per receives each
+// listed orientation's best score (+inf if none), in list order.
+ExactBest exact_over(spk_ctx* c, const HostItem& item, const OrientDesc* ods,
+                     const std::vector<int>& list, int penalty, std::vector<double>* per) {
+  ExactBest b;
+  double bs = 1e30;
+  if (per) per->assign(list.size(), (double)INFINITY);
+  for (size_t j = 0; j < list.size(); ++j) {
+    const int i = list[j];
+    const WgBest r = exact_search_one(c, item, ods[i], penalty);
+    if (r.idx == INT_MAX) continue;
+    if (per) (*per)[j] = r.score;
+    if (r.score < bs) {
+      bs = r.score;
+      b.fit = i;
+      b.idx = r.idx;
+      b.score = r.score;
+      b.prox = ((long long)r.p_hi << 32) | (unsigned int)r.p_lo;
+    }
+  }
+  return b;
+}
+
+// Result and placement of an item decided on the host's exact path; returns
+// 1 if it was placed.
+int place_exact(spk_ctx* c, const ExactBest& eb, const ItemPlan& pl,
+                const std::vector<OrientDesc>& all, float resid, int32_t id, spk_result* res) {
+  if (eb.fit < 0) {
+    to_result(c, false, 0, 0.0, -1, 0, 0, resid, 1, res);
+    return 0;
+  }
+  to_result(c, true, eb.idx, eb.score, pl.map[eb.fit], 0, eb.prox, resid, 1, res);
+  do_place(c, all[pl.od_off + eb.fit], res->pos[0], res->pos[1], res->pos[2], id, true);
+  return 1;
+}
+
+// The greedy loop with every item enqueued back to back: search, k_decide,
+// placement and refresh of item t+1 follow item t on the stream with no host
+// round trip.  k_decide hands a search back to the host (CTL_HALT) when its
+// FFT result cannot be trusted; the kernels of the remaining items then exit
+// at once, the host redoes that item on the exact direct-sum path, places it
+// and resumes the pipeline after it.  Returns the number of items placed.
+//
+// Training records (rec_scores != nullptr): before item t's search the tray
+// features it sees are written to rec_feat + t * kNnTrayFeat (device), and
+// rec_scores[t * n_orient + o] receives the best score of orientation o
+// (+inf when it does not fit or has no feasible offset).
+int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector<int>& off,
+                   const int32_t* order, const spk_orient* orients, int n_orient, int penalty,
+                   int32_t id_offset, spk_result* results, double* rec_scores = nullptr,
+                   unsigned short* rec_feat = nullptr) {
+  const int n_items = (int)bases.size();
+  PackPlan P = plan_pack(c, bases, off, order, orients, n_orient);
+  const std::vector<ItemPlan>& plans = P.items;
+  const std::vector<OrientDesc>& all = P.all;
+  std::vector<std::pair<int, std::vector<double>>> host_rows;   // records of host-path items
+  if (rec_scores) {
+    c->rscores.ensure((size_t)n_items * n_orient);
+    std::vector<double> inf((size_t)n_items * n_orient, (double)INFINITY);
+    upload(c, c->rscores.p, inf.data(), inf.size() * sizeof(double));
+    c->heights.ensure((size_t)c->nx * c->ny);
+  }
 
   int placed = 0;
   int t0 = 0;
@@ -860,18 +973,15 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
     const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
     upload(c, c->ctl.p, ctl0, sizeof(ctl0));
     for (int t = t0; t < n_items; ++t) {
-      const Plan& pl = plans[t];
+      const ItemPlan& pl = plans[t];
       if (pl.m == 0) continue;   // nothing fits: no search, tray unchanged
+      if (rec_feat)
+        nn_tray_features(c->stream, c->tray.p, c->nx, c->ny, c->nz, c->heights.p,
+                         rec_feat + (size_t)t * kNnTrayFeat);
       const OrientDesc* od = c->od_all.p + pl.od_off;
-      enqueue_search(c, od, pl.m, pl.max_ex, pl.max_e, penalty, c->ctl.p, nullptr, nullptr,
-                     nullptr);
-      launch(c, "decide", k_decide, dim3(1), dim3(64), 0, (const SearchOut*)c->sout.p,
-             (const long long*)c->ver.p, c->pout.p, t, c->resid_limit, c->ctl.p);
-      const int nb = std::max(1, std::min(1024, (pl.max_e + 255) / 256));
-      launch(c, "place", k_place, dim3(nb), dim3(256), 0, c->tray.p, (const int*)c->items.p, od,
-             (const SearchOut*)c->sout.p, 0, 0, 0, 0, (int)id_offset, c->nx, c->ny, c->nz, 1,
-             c->zero.p, (const int*)c->ctl.p);
-      refresh_field_and_spectra(c, nullptr, c->ctl.p);
+      enqueue_step(c, od, pl.m, pl.max_ex, pl.max_e, penalty, t, id_offset, 0,
+                   rec_scores ? c->rscores.p + (size_t)t * n_orient : nullptr,
+                   c->map_all.p + pl.od_off);
     }
     int ctl[CTL_WORDS];
     std::vector<PackOut> po(n_items - t0);
@@ -881,7 +991,7 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
     if (c->timing) c->flush_timing();
     const int stop = ctl[CTL_HALT] ? ctl[CTL_HALT_ITEM] : n_items;
     for (int t = t0; t < stop; ++t) {
-      const Plan& pl = plans[t];
+      const ItemPlan& pl = plans[t];
       if (pl.m == 0) {
         to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, &results[t]);
         continue;
@@ -894,37 +1004,273 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
     if (!ctl[CTL_HALT]) break;
     // item `stop` goes through the exact path on the host's orders
     const int h = stop;
-    const Plan& pl = plans[h];
+    const ItemPlan& pl = plans[h];
     const int it = order ? order[h] : h;
-    const float resid = po[h - t0].so.resid;
-    double bs = 1e30;
-    int bi = -1, bo = -1;
-    long long bp = 0;
-    for (int i = 0; i < pl.m; ++i) {
-      const WgBest r = exact_search_one(c, bases[it], all[pl.od_off + i], penalty);
-      if (r.idx != INT_MAX && r.score < bs) {
-        bs = r.score;
-        bi = r.idx;
-        bo = i;
-        bp = ((long long)r.p_hi << 32) | (unsigned int)r.p_lo;
-      }
+    std::vector<int> fit(pl.m);
+    for (int i = 0; i < pl.m; ++i) fit[i] = i;
+    std::vector<double> row;
+    const ExactBest eb = exact_over(c, bases[it], &all[pl.od_off], fit, penalty,
+                                    rec_scores ? &row : nullptr);
+    if (rec_scores) {
+      std::vector<double> full(n_orient, (double)INFINITY);
+      for (int i = 0; i < pl.m; ++i) full[pl.map[i]] = row[i];
+      host_rows.emplace_back(h, std::move(full));
     }
-    if (bo < 0) {
-      to_result(c, false, 0, 0.0, -1, 0, 0, resid, 1, &results[h]);
-    } else {
-      to_result(c, true, bi, bs, pl.map[bo], 0, bp, resid, 1, &results[h]);
-      const spk_result& r = results[h];
-      // do_place takes its descriptor from c->od
-      c->od.ensure(1);
-      do_place(c, all[pl.od_off + bo], r.pos[0], r.pos[1], r.pos[2], id_offset + placed + 1,
-               true);
-      ++placed;
-    }
+    placed += place_exact(c, eb, pl, all, po[h - t0].so.resid, id_offset + placed + 1,
+                          &results[h]);
     t0 = h + 1;
+  }
+  if (rec_scores) {
+    download(c, rec_scores, c->rscores.p, (size_t)n_items * n_orient * sizeof(double));
+    sync(c);
+    for (auto& hr : host_rows)
+      std::copy(hr.second.begin(), hr.second.end(), rec_scores + (size_t)hr.first * n_orient);
   }
   return placed;
 }
 
+// Time a group of launches on the context stream as one named entry.
+template <typename F>
+void timed_group(spk_ctx* c, const char* name, F&& f) {
+  if (!c->timing) {
+    f();
+    return;
+  }
+  hipEvent_t a = c->ev(), b = c->ev();
+  HIP_OK(hipEventRecord(a, c->stream));
+  f();
+  HIP_OK(hipEventRecord(b, c->stream));
+  c->pending.push_back({name, a, b});
+}
+
+// Greedy loop with learned orientation proposals.  The proposer scores the
+// orientations of an item against the tray as it is at that item.  When more
+// than topk orientations fit and the softmax mass of the topk best-scored
+// ones (over the fitting orientations) is at least min_conf, only those are
+// searched, in orientation order so that ties resolve as in the full search;
+// the outcome is the best placement among them.  Every fitting orientation is
+// searched otherwise, and also when no proposal has a feasible offset.  One
+// host synchronisation per item reads the outcome and the logits of the next
+// item, which the device computed right after this item's placement.
+// counts: [0] items decided on their proposals, [1] full searches (too few
+// fitting orientations or low confidence), [2] full searches after
+// infeasible proposals, [3] items decided on the host's exact path.
+int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
+                 const std::vector<int>& off, const int32_t* order, const spk_orient* orients,
+                 int n_orient, int penalty, int32_t id_offset, spk_result* results, int topk,
+                 float min_conf, int32_t* proposals, int32_t* counts) {
+  const int n_items = (int)bases.size();
+  PackPlan P = plan_pack(c, bases, off, order, orients, n_orient);
+  const std::vector<ItemPlan>& plans = P.items;
+  const std::vector<OrientDesc>& all = P.all;
+  hipStream_t s = c->stream;
+  timed_group(c, "proposer_items", [&]() {
+    nn_cast(nn, s);
+    const int bmax = nn->max_batch;
+    c->ifeat.ensure((size_t)std::min(n_items, bmax) * kNnItemFeat);
+    c->ipool.ensure((size_t)n_items * kNnItemPooled);
+    for (int b0 = 0; b0 < n_items; b0 += bmax) {
+      const int nb = std::min(bmax, n_items - b0);
+      for (int j = 0; j < nb; ++j) {
+        const int it = order ? order[b0 + j] : b0 + j;
+        const HostItem& h = bases[it];
+        nn_item_features(s, c->items.p + off[it], h.dims[0], h.dims[1], h.dims[2],
+                         c->ifeat.p + (size_t)j * kNnItemFeat);
+      }
+      nn_item_branch(nn, s, c->ifeat.p, nb, nn->concat.p);
+      HIP_OK(hipMemcpy2DAsync(c->ipool.p + (size_t)b0 * kNnItemPooled,
+                              kNnItemPooled * sizeof(bf16_t), nn->concat.p,
+                              kNnConcat * sizeof(bf16_t), kNnItemPooled * sizeof(bf16_t), nb,
+                              hipMemcpyDeviceToDevice, s));
+    }
+  });
+  c->tfeat.ensure(kNnTrayFeat);
+  c->heights.ensure((size_t)c->nx * c->ny);
+  c->od_prop.ensure(kNnLogits);
+  float logits[kNnLogits];
+  auto enqueue_head = [&](int t) {
+    timed_group(c, "proposer", [&]() {
+      nn_tray_features(s, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, c->tfeat.p);
+      HIP_OK(hipMemcpyAsync(nn->concat.p, c->ipool.p + (size_t)t * kNnItemPooled,
+                            kNnItemPooled * sizeof(bf16_t), hipMemcpyDeviceToDevice, s));
+      nn_tray_head(nn, s, c->tfeat.p, 1);
+    });
+    download(c, logits, nn->logits.p, sizeof(logits));
+  };
+  int placed = 0;
+  auto reset_ctl = [&]() {
+    const int v[CTL_WORDS] = {0, -1, 0, placed};
+    upload(c, c->ctl.p, v, sizeof(v));
+  };
+  int cnt[4] = {0, 0, 0, 0};
+  reset_ctl();
+  bool have = false;   // logits of item t are in `logits` and match the tray
+  for (int t = 0; t < n_items; ++t) {
+    const ItemPlan& pl = plans[t];
+    if (proposals)
+      for (int j = 0; j < topk; ++j) proposals[(size_t)t * topk + j] = -1;
+    if (pl.m == 0) {
+      to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, &results[t]);
+      continue;
+    }
+    std::vector<int> pick;   // fitting indices searched
+    bool use_prop = false;
+    if (pl.m > topk) {
+      if (!have) {
+        enqueue_head(t);
+        sync(c);
+      }
+      float mx = -INFINITY;
+      for (int i = 0; i < pl.m; ++i) mx = std::max(mx, logits[pl.map[i]]);
+      std::vector<int> rank(pl.m);
+      std::vector<double> e(pl.m);
+      double se = 0.0;
+      for (int i = 0; i < pl.m; ++i) {
+        rank[i] = i;
+        e[i] = std::exp((double)logits[pl.map[i]] - mx);
+        se += e[i];
+      }
+      std::stable_sort(rank.begin(), rank.end(), [&](int a, int b) {
+        return logits[pl.map[a]] > logits[pl.map[b]];
+      });
+      double top = 0.0;
+      for (int j = 0; j < topk; ++j) top += e[rank[j]];
+      if (top / se >= (double)min_conf) {
+        use_prop = true;
+        pick.assign(rank.begin(), rank.begin() + topk);
+        std::sort(pick.begin(), pick.end());
+      }
+    }
+    have = false;
+    if (!use_prop) {
+      pick.resize(pl.m);
+      for (int i = 0; i < pl.m; ++i) pick[i] = i;
+    }
+    const int k = (int)pick.size();
+    if (use_prop) {
+      OrientPick op{};
+      int max_ex = 1, max_e = 1;
+      for (int j = 0; j < k; ++j) {
+        op.idx[j] = pick[j];
+        const OrientDesc& d = all[pl.od_off + pick[j]];
+        max_ex = std::max(max_ex, d.ex);
+        max_e = std::max(max_e, d.ex * d.ey * d.ez);
+        if (proposals) proposals[(size_t)t * topk + j] = pl.map[pick[j]];
+      }
+      launch(c, "gather_od", k_gather_od, dim3(1), dim3(32), 0,
+             (const OrientDesc*)c->od_all.p + pl.od_off, op, k, c->od_prop.p);
+      enqueue_step(c, c->od_prop.p, k, max_ex, max_e, penalty, t, id_offset, 1, nullptr, nullptr);
+    } else {
+      enqueue_step(c, c->od_all.p + pl.od_off, pl.m, pl.max_ex, pl.max_e, penalty, t, id_offset,
+                   0, nullptr, nullptr);
+    }
+    // the next searched item's logits, on the tray this step leaves
+    int tn = t + 1;
+    while (tn < n_items && plans[tn].m == 0) ++tn;
+    const bool next_head = tn < n_items && plans[tn].m > topk;
+    if (next_head) enqueue_head(tn);
+    int ctl[CTL_WORDS];
+    PackOut po;
+    download(c, ctl, c->ctl.p, sizeof(ctl));
+    download(c, &po, c->pout.p + t, sizeof(po));
+    sync(c);
+    if (c->timing) c->flush_timing();
+    if (ctl[CTL_HALT] == 0) {
+      const SearchOut& so = po.so;
+      to_result(c, so.found, so.idx, so.score, so.found ? pl.map[pick[so.orient]] : -1,
+                po.collision, so.prox, so.resid, 0, &results[t]);
+      if (so.found) ++placed;
+      ++cnt[use_prop ? 0 : 1];
+      have = next_head;
+      continue;
+    }
+    const int code = ctl[CTL_HALT];
+    float resid = po.so.resid;
+    reset_ctl();
+    if (code == 2) {
+      // no proposal is feasible: search every fitting orientation
+      ++cnt[2];
+      enqueue_step(c, c->od_all.p + pl.od_off, pl.m, pl.max_ex, pl.max_e, penalty, t, id_offset,
+                   0, nullptr, nullptr);
+      download(c, ctl, c->ctl.p, sizeof(ctl));
+      download(c, &po, c->pout.p + t, sizeof(po));
+      sync(c);
+      if (c->timing) c->flush_timing();
+      if (ctl[CTL_HALT] == 0) {
+        const SearchOut& so = po.so;
+        to_result(c, so.found, so.idx, so.score, so.found ? pl.map[so.orient] : -1,
+                  po.collision, so.prox, so.resid, 0, &results[t]);
+        if (so.found) ++placed;
+        reset_ctl();
+        continue;
+      }
+      resid = po.so.resid;
+      reset_ctl();
+      pick.resize(pl.m);
+      for (int i = 0; i < pl.m; ++i) pick[i] = i;
+      use_prop = false;
+    }
+    // the FFT outcome was not trusted: exact direct sums over the same
+    // orientations, then over all of them if no proposal is feasible
+    ++cnt[3];
+    const int it = order ? order[t] : t;
+    ExactBest eb = exact_over(c, bases[it], &all[pl.od_off], pick, penalty, nullptr);
+    if (eb.fit < 0 && use_prop) {
+      ++cnt[2];
+      std::vector<int> every(pl.m);
+      for (int i = 0; i < pl.m; ++i) every[i] = i;
+      eb = exact_over(c, bases[it], &all[pl.od_off], every, penalty, nullptr);
+    }
+    placed += place_exact(c, eb, pl, all, resid, id_offset + placed + 1, &results[t]);
+    reset_ctl();
+  }
+  if (counts)
+    for (int i = 0; i < 4; ++i) counts[i] = cnt[i];
+  return placed;
+}
+
+// Parses and uploads the concatenated items of a pack call.
+void load_items(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_items,
+                const int32_t* order, std::vector<HostItem>& bases, std::vector<int>& off) {
+  if (!items || !dims || n_items <= 0) throw SpkError{SPK_EINVAL, "bad pack arguments"};
+  size_t tot = 0;
+  for (int t = 0; t < n_items; ++t) {
+    const int* p = items + tot;
+    bases.push_back(scan_item(p, dims[3 * t], dims[3 * t + 1], dims[3 * t + 2]));
+    off.push_back((int)tot);
+    tot += (size_t)dims[3 * t] * dims[3 * t + 1] * dims[3 * t + 2];
+  }
+  c->items.ensure(std::max<size_t>(tot, 1));
+  upload(c, c->items.p, items, tot * sizeof(int));
+  for (int t = 0; t < n_items; ++t) {
+    const int it = order ? order[t] : t;
+    if (it < 0 || it >= n_items) throw SpkError{SPK_EINVAL, "order index out of range"};
+  }
+}
+
+}  // namespace
+
+namespace {
+int dev_check(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(SPK_EDEVICE, "no HIP device");
+  if (device < 0 || device >= n) return fail(SPK_EINVAL, "device index out of range");
+  return SPK_OK;
+}
+
+template <typename F>
+int dev_guarded(int device, F&& f) {
+  const int rc = dev_check(device);
+  if (rc != SPK_OK) return rc;
+  try {
+    DeviceGuard g(device);
+    return f();
+  } catch (const HipError& e) {
+    return fail(e.code, e.msg);
+  } catch (const SpkError& e) {
+    return fail(e.code, e.msg);
+  }
+}
 }  // namespace
 
 // =============================================================================
@@ -978,6 +1324,8 @@ int spk_ctx_destroy(spk_ctx* c) {
     c->ver.release(); c->flag.release(); c->taps.release(); c->big.release();
     c->ffree.release(); c->freach.release(); c->pgrid.release(); c->exits.release();
     c->od_all.release(); c->pout.release(); c->ctl.release(); c->zero.release();
+    c->heights.release(); c->map_all.release(); c->tfeat.release(); c->ifeat.release();
+    c->ipool.release(); c->rscores.release(); c->od_prop.release();
     if (c->stream) hipStreamDestroy(c->stream);
   }
   delete c;
@@ -1094,23 +1442,10 @@ int spk_pack(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_items,
              int interlocking_free, int32_t id_offset, spk_result* results) {
   return guarded(c, [&]() {
     if (!c->valid) throw SpkError{SPK_ESTATE, "no tray set in this context"};
-    if (!items || !dims || n_items <= 0 || !results || n_orient <= 0)
-      throw SpkError{SPK_EINVAL, "bad pack arguments"};
+    if (!results || n_orient <= 0) throw SpkError{SPK_EINVAL, "bad pack arguments"};
     std::vector<HostItem> bases;
     std::vector<int> off;
-    size_t tot = 0;
-    for (int t = 0; t < n_items; ++t) {
-      const int* p = items + tot;
-      bases.push_back(scan_item(p, dims[3 * t], dims[3 * t + 1], dims[3 * t + 2]));
-      off.push_back((int)tot);
-      tot += (size_t)dims[3 * t] * dims[3 * t + 1] * dims[3 * t + 2];
-    }
-    c->items.ensure(std::max<size_t>(tot, 1));
-    upload(c, c->items.p, items, tot * sizeof(int));
-    for (int t = 0; t < n_items; ++t) {
-      const int it = order ? order[t] : t;
-      if (it < 0 || it >= n_items) throw SpkError{SPK_EINVAL, "order index out of range"};
-    }
+    load_items(c, items, dims, n_items, order, bases, off);
     int placed = 0;
     if (interlocking_free) {
       // the flood fill iterates to its fixpoint under host control: one
@@ -1445,6 +1780,263 @@ int spk_plan_info(spk_ctx* c, int* px, int* py, int* pz, int* digit_base) {
   if (pz) *pz = c->pz.n;
   if (digit_base) *digit_base = 0;
   return SPK_OK;
+}
+
+// ---- device memory for callers without a HIP runtime of their own -----------
+
+
+int spk_dev_alloc(int device, int64_t bytes, void** out) {
+  if (!out || bytes <= 0) return fail(SPK_EINVAL, "bad allocation arguments");
+  *out = nullptr;
+  return dev_guarded(device, [&]() {
+    HIP_OK(hipMalloc(out, (size_t)bytes));
+    return SPK_OK;
+  });
+}
+
+int spk_dev_free(int device, void* ptr) {
+  if (!ptr) return SPK_OK;
+  return dev_guarded(device, [&]() {
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipFree(ptr));
+    return SPK_OK;
+  });
+}
+
+int spk_dev_copy(int device, void* dst, const void* src, int64_t bytes, int kind) {
+  if (!dst || !src || bytes < 0 || kind < 0 || kind > 2) return fail(SPK_EINVAL, "bad copy arguments");
+  return dev_guarded(device, [&]() {
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                          : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    HIP_OK(hipDeviceSynchronize());
+    if (bytes) HIP_OK(hipMemcpy(dst, src, (size_t)bytes, k));
+    HIP_OK(hipDeviceSynchronize());
+    return SPK_OK;
+  });
+}
+
+int spk_dev_gather(int device, void* dst, const void* src, int64_t row_bytes, int64_t src_rows,
+                   const int64_t* idx, int n) {
+  if (!dst || !src || !idx || n <= 0 || row_bytes <= 0 || row_bytes % 16)
+    return fail(SPK_EINVAL, "bad gather arguments (row size must be a multiple of 16 bytes)");
+  for (int i = 0; i < n; ++i)
+    if (idx[i] < 0 || idx[i] >= src_rows) return fail(SPK_EINVAL, "gather index out of range");
+  return dev_guarded(device, [&]() {
+    long long* d_idx = nullptr;
+    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipMalloc((void**)&d_idx, (size_t)n * sizeof(long long)));
+    const hipError_t e1 = hipMemcpy(d_idx, idx, (size_t)n * sizeof(long long), hipMemcpyHostToDevice);
+    hipError_t e2 = hipSuccess;
+    if (e1 == hipSuccess) {
+      const long long words = row_bytes / 16;
+      hipLaunchKernelGGL(spknn::k_gather_rows,
+                         dim3((unsigned)std::min<long long>((words + 255) / 256, 1024), n),
+                         dim3(256), 0, nullptr, (uint4*)dst, (const uint4*)src, words,
+                         (const long long*)d_idx, n);
+      e2 = hipGetLastError();
+      if (e2 == hipSuccess) e2 = hipDeviceSynchronize();
+    }
+    hipFree(d_idx);
+    HIP_OK(e1);
+    HIP_OK(e2);
+    return SPK_OK;
+  });
+}
+
+// ---- learned orientation proposer -----------------------------------------
+
+int spk_nn_create(int device, int max_batch, spk_nn** out) {
+  if (!out) return fail(SPK_EINVAL, "null out pointer");
+  *out = nullptr;
+  if (max_batch <= 0 || max_batch > 4096) return fail(SPK_EINVAL, "max_batch must be in [1, 4096]");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(SPK_EDEVICE, "no HIP device");
+  if (device < 0 || device >= n) return fail(SPK_EINVAL, "device index out of range");
+  spk_nn* nn = new spk_nn;
+  nn->device = device;
+  const int rc = nn_guarded(nn, [&]() {
+    HIP_OK(hipStreamCreate(&nn->stream));
+    nn_build(nn, max_batch);
+    return SPK_OK;
+  });
+  if (rc != SPK_OK) {
+    const std::string msg = g_err;
+    spk_nn_destroy(nn);
+    return fail(rc, msg);
+  }
+  *out = nn;
+  return SPK_OK;
+}
+
+int spk_nn_destroy(spk_nn* nn) {
+  if (!nn) return SPK_OK;
+  {
+    DeviceGuard g(nn->device);
+    if (nn->stream) hipStreamSynchronize(nn->stream);
+    nn->params.release(); nn->grads.release(); nn->m1.release(); nn->m2.release();
+    nn->logits.release(); nn->loss_rows.release(); nn->part.release(); nn->wbf.release();
+    nn->concat.release(); nn->g_concat.release();
+    for (int i = 0; i < NN_LAYERS; ++i) { nn->act[i].release(); nn->grad[i].release(); }
+    if (nn->stream) hipStreamDestroy(nn->stream);
+  }
+  delete nn;
+  return SPK_OK;
+}
+
+int spk_nn_num_params(spk_nn* nn, int64_t* n) {
+  if (!nn || !n) return fail(SPK_EINVAL, "null argument");
+  *n = (int64_t)nn->n_params;
+  return SPK_OK;
+}
+
+int spk_nn_set_params(spk_nn* nn, const float* params, int64_t n) {
+  return nn_guarded(nn, [&]() {
+    if (!params || n != (int64_t)nn->n_params)
+      throw SpkError{SPK_EINVAL, "parameter count does not match the network"};
+    HIP_OK(hipMemcpyAsync(nn->params.p, params, n * sizeof(float), hipMemcpyHostToDevice,
+                          nn->stream));
+    HIP_OK(hipMemsetAsync(nn->m1.p, 0, n * sizeof(float), nn->stream));
+    HIP_OK(hipMemsetAsync(nn->m2.p, 0, n * sizeof(float), nn->stream));
+    HIP_OK(hipStreamSynchronize(nn->stream));
+    nn->step = 0;
+    nn->dirty = true;
+    return SPK_OK;
+  });
+}
+
+int spk_nn_get_params(spk_nn* nn, float* params, int64_t n) {
+  return nn_guarded(nn, [&]() {
+    if (!params || n != (int64_t)nn->n_params)
+      throw SpkError{SPK_EINVAL, "parameter count does not match the network"};
+    HIP_OK(hipMemcpyAsync(params, nn->params.p, n * sizeof(float), hipMemcpyDeviceToHost,
+                          nn->stream));
+    HIP_OK(hipStreamSynchronize(nn->stream));
+    return SPK_OK;
+  });
+}
+
+int spk_nn_forward(spk_nn* nn, const uint16_t* item_feat, const uint16_t* tray_feat, int batch,
+                   float* logits) {
+  return nn_guarded(nn, [&]() {
+    nn_check_batch(nn, batch);
+    if (!item_feat || !tray_feat || !logits) throw SpkError{SPK_EINVAL, "null argument"};
+    nn_forward(nn, nn->stream, item_feat, tray_feat, batch);
+    HIP_OK(hipMemcpyAsync(logits, nn->logits.p, (size_t)batch * kNnLogits * sizeof(float),
+                          hipMemcpyDeviceToHost, nn->stream));
+    HIP_OK(hipStreamSynchronize(nn->stream));
+    return SPK_OK;
+  });
+}
+
+int spk_nn_gradient(spk_nn* nn, const uint16_t* item_feat, const uint16_t* tray_feat,
+                    const float* target, const int32_t* mask, int batch, float* grads,
+                    float* loss) {
+  return nn_guarded(nn, [&]() {
+    nn_check_batch(nn, batch);
+    if (!item_feat || !tray_feat || !target || !mask || !grads)
+      throw SpkError{SPK_EINVAL, "null argument"};
+    nn_forward(nn, nn->stream, item_feat, tray_feat, batch);
+    nn_backward(nn, nn->stream, item_feat, tray_feat, target, mask, batch);
+    HIP_OK(hipMemcpyAsync(grads, nn->grads.p, nn->n_params * sizeof(float),
+                          hipMemcpyDeviceToHost, nn->stream));
+    const float l = nn_loss_sum(nn, nn->stream, batch);
+    if (loss) *loss = l;
+    return SPK_OK;
+  });
+}
+
+int spk_nn_train_step(spk_nn* nn, const uint16_t* item_feat, const uint16_t* tray_feat,
+                      const float* target, const int32_t* mask, int batch, float lr,
+                      float* loss) {
+  return nn_guarded(nn, [&]() {
+    nn_check_batch(nn, batch);
+    if (!item_feat || !tray_feat || !target || !mask)
+      throw SpkError{SPK_EINVAL, "null argument"};
+    if (!(lr >= 0.f)) throw SpkError{SPK_EINVAL, "learning rate must be >= 0"};
+    hipStream_t s = nn->stream;
+    nn_forward(nn, s, item_feat, tray_feat, batch);
+    nn_backward(nn, s, item_feat, tray_feat, target, mask, batch);
+    nn->step += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)nn->step), c2 = 1.f - std::pow(b2, (float)nn->step);
+    const long long n = (long long)nn->n_params;
+    nn_launch(s, spknn::k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, nn->params.p,
+              (const float*)nn->grads.p, nn->m1.p, nn->m2.p, n, lr, b1, b2, 1e-8f, c1, c2);
+    nn->dirty = true;
+    const float l = nn_loss_sum(nn, s, batch);
+    if (loss) *loss = l;
+    return SPK_OK;
+  });
+}
+
+int spk_nn_item_features(spk_nn* nn, const int32_t* item, int ix, int iy, int iz, uint16_t* out) {
+  return nn_guarded(nn, [&]() {
+    check_dims(ix, iy, iz);
+    if (!item || !out) throw SpkError{SPK_EINVAL, "null argument"};
+    DevBuf<int> d;
+    const size_t n = (size_t)ix * iy * iz;
+    d.ensure(n);
+    HIP_OK(hipMemcpyAsync(d.p, item, n * sizeof(int), hipMemcpyHostToDevice, nn->stream));
+    nn_item_features(nn->stream, d.p, ix, iy, iz, out);
+    HIP_OK(hipStreamSynchronize(nn->stream));
+    d.release();
+    return SPK_OK;
+  });
+}
+
+int spk_tray_features(spk_ctx* c, uint16_t* out) {
+  return guarded(c, [&]() {
+    if (!c->valid) throw SpkError{SPK_ESTATE, "no tray set in this context"};
+    if (!out) throw SpkError{SPK_EINVAL, "null argument"};
+    c->heights.ensure((size_t)c->nx * c->ny);
+    nn_tray_features(c->stream, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, out);
+    sync(c);
+    return SPK_OK;
+  });
+}
+
+int spk_pack_records(spk_ctx* c, const spk_pack_args* a, uint16_t* tray_feat, double* scores) {
+  return guarded(c, [&]() {
+    if (!c->valid) throw SpkError{SPK_ESTATE, "no tray set in this context"};
+    if (!a || !a->results || a->n_orient <= 0 || !tray_feat || !scores)
+      throw SpkError{SPK_EINVAL, "bad pack arguments"};
+    if (a->interlocking_free)
+      throw SpkError{SPK_EINVAL, "training records come from the plain (not interlocking-free) loop"};
+    std::vector<HostItem> bases;
+    std::vector<int> off;
+    load_items(c, a->items, a->dims, a->n_items, a->order, bases, off);
+    const int placed = pack_pipelined(c, bases, off, a->order, a->orients, a->n_orient, a->penalty,
+                                      a->id_offset, a->results, scores, tray_feat);
+    sync(c);
+    if (c->timing) c->flush_timing();
+    c->generation += (uint64_t)placed;
+    return SPK_OK;
+  });
+}
+
+int spk_pack_learned(spk_ctx* c, spk_nn* nn, const spk_pack_args* a, int topk, float min_conf,
+                     int32_t* proposals, int32_t* counts) {
+  return guarded(c, [&]() {
+    if (!c->valid) throw SpkError{SPK_ESTATE, "no tray set in this context"};
+    if (!nn) throw SpkError{SPK_EINVAL, "null proposer"};
+    if (nn->device != c->device)
+      throw SpkError{SPK_EINVAL, "proposer and context are on different devices"};
+    if (!a || !a->results || a->n_orient <= 0 || a->n_orient > kNnLogits)
+      throw SpkError{SPK_EINVAL, "bad pack arguments (1 to 24 orientations)"};
+    if (topk < 1 || topk > kNnLogits) throw SpkError{SPK_EINVAL, "topk must be in [1, 24]"};
+    if (a->interlocking_free)
+      throw SpkError{SPK_EINVAL, "learned proposals run the plain (not interlocking-free) loop"};
+    std::vector<HostItem> bases;
+    std::vector<int> off;
+    load_items(c, a->items, a->dims, a->n_items, a->order, bases, off);
+    const int placed = pack_learned(c, nn, bases, off, a->order, a->orients, a->n_orient,
+                                    a->penalty, a->id_offset, a->results, topk, min_conf,
+                                    proposals, counts);
+    sync(c);
+    if (c->timing) c->flush_timing();
+    c->generation += (uint64_t)placed;
+    return SPK_OK;
+  });
 }
 
 }  // extern "C"

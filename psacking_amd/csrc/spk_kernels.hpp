@@ -868,9 +868,13 @@ k_place(int* __restrict__ tray, const int* __restrict__ items, const OrientDesc*
 // resid_limit of its integer, the digit base covered the distance range and
 // the exact re-check of the winner (k_verify) agrees; then the item is placed
 // by the following kernels (when found) and CTL_PLACED counts it.
+//
+// With halt_not_found (a search over a proposed subset of orientations) a
+// trusted search that found no feasible offset also stops the loop, with
+// CTL_HALT = 2, so that the host can search every orientation of the item.
 __global__ void k_decide(const SearchOut* __restrict__ so, const long long* __restrict__ ver,
                          PackOut* __restrict__ out, int t, float resid_limit,
-                         int* __restrict__ ctl) {
+                         int* __restrict__ ctl, int halt_not_found) {
   if (threadIdx.x != 0 || ctl[CTL_HALT]) return;
   const SearchOut s = *so;
   const bool bad = s.unsupported || !(s.resid <= resid_limit) ||
@@ -879,14 +883,37 @@ __global__ void k_decide(const SearchOut* __restrict__ so, const long long* __re
   r.so = s;
   r.collision = ver[0];
   out[t] = r;
-  if (bad) {
-    ctl[CTL_HALT] = 1;
+  if (bad || (halt_not_found && !s.found)) {
+    ctl[CTL_HALT] = bad ? 1 : 2;
     ctl[CTL_HALT_ITEM] = t;
     ctl[CTL_SKIP_REFRESH] = 1;
     return;
   }
   ctl[CTL_SKIP_REFRESH] = s.found ? 0 : 1;
   if (s.found) ctl[CTL_PLACED] += 1;
+}
+
+// Up to 24 orientation indices passed by value.
+struct OrientPick {
+  int idx[24];
+};
+
+// dst[i] = src[pick.idx[i]] for i < k (the proposed orientations of one item)
+__global__ void k_gather_od(const OrientDesc* __restrict__ src, OrientPick pick, int k,
+                            OrientDesc* __restrict__ dst) {
+  const int i = threadIdx.x;
+  if (i < k) dst[i] = src[pick.idx[i]];
+}
+
+// Per-orientation best scores of an accepted search: out[map[i]] = score of
+// fitting orientation i (+inf when it has no feasible offset).
+__global__ void k_record_scores(const WgBest* __restrict__ per, int m, const int* __restrict__ map,
+                                double* __restrict__ out, const int* __restrict__ ctl) {
+  if (search_off(ctl)) return;
+  const int i = threadIdx.x;
+  if (i >= m) return;
+  const WgBest r = per[i];
+  out[map[i]] = r.idx == INT_MAX ? (double)INFINITY : r.score;
 }
 
 // reset the distance-field statistics before a refresh

@@ -1,0 +1,327 @@
+// Host side of the learned orientation proposer (kernels in spk_nn.hpp).
+// Included by spk_api.hip after its error, buffer and launch helpers.
+//
+// Network (262,760 parameters, channels-last, every conv 3x3x3 stride 2 pad 1,
+// ReLU after every layer but the last):
+//   item  [32,32,32,1]  -> I1 [16^3,16] -> I2 [8^3,32] -> I3 [4^3,48] -> avg2 [2^3,48] = 384
+//   tray  [64,32,32,2]  -> T1 [32,16,16,16] -> T2 [16,8,8,32] -> T3 [8,4,4,48] -> avg2 [4,2,2,48] = 768
+//   concat 1152 -> F1 128 -> F2 24 orientation logits
+// Parameters are stored layer by layer, W [cout][ks^3*cin] (tap-major,
+// channel-minor columns; tap = (t0*3 + t1)*3 + t2) followed by b [cout].
+#pragma once
+
+#include "spk_nn.hpp"
+
+namespace {
+
+using spknn::bf16_t;
+using spknn::ConvDesc;
+
+enum { NN_I1, NN_I2, NN_I3, NN_T1, NN_T2, NN_T3, NN_F1, NN_F2, NN_LAYERS };
+constexpr int kNnItemFeat = 32768;     // 32^3 x 1
+constexpr int kNnTrayFeat = 131072;    // 64x32x32 x 2
+constexpr int kNnItemPooled = 384;
+constexpr int kNnConcat = 1152;
+constexpr int kNnLogits = 24;
+
+int round32(int v) { return (v + 31) & ~31; }
+
+ConvDesc conv_desc(int i0, int i1, int i2, int cin, int cout) {
+  ConvDesc d{};
+  d.i0 = i0; d.i1 = i1; d.i2 = i2; d.cin = cin;
+  d.o0 = (i0 - 1) / 2 + 1; d.o1 = (i1 - 1) / 2 + 1; d.o2 = (i2 - 1) / 2 + 1; d.cout = cout;
+  d.ks = 3; d.st = 2; d.pad = 1;
+  d.K = 27 * cin; d.Kp = round32(d.K);
+  d.Kt = 27 * cout; d.Ktp = round32(d.Kt);
+  return d;
+}
+
+ConvDesc dense_desc(int cin, int cout) {
+  ConvDesc d{};
+  d.i0 = d.i1 = d.i2 = 1; d.cin = cin;
+  d.o0 = d.o1 = d.o2 = 1; d.cout = cout;
+  d.ks = 1; d.st = 1; d.pad = 0;
+  d.K = cin; d.Kp = round32(cin);
+  d.Kt = cout; d.Ktp = round32(cout);
+  return d;
+}
+
+struct NnLayer {
+  ConvDesc d;
+  size_t w_off = 0, b_off = 0;     // master parameters (floats)
+  size_t wb_off = 0, wt_off = 0;   // bf16 working copies
+  int relu = 1;
+  size_t in_size() const { return (size_t)d.i0 * d.i1 * d.i2 * d.cin; }
+  size_t out_size() const { return (size_t)d.o0 * d.o1 * d.o2 * d.cout; }
+};
+
+size_t wgrad_chunks_max(const ConvDesc& d) {
+  const int kb = (d.K + 63) / 64;
+  return (size_t)((512 + kb - 1) / kb);
+}
+
+}  // namespace
+
+struct spk_nn {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int max_batch = 0;
+  NnLayer L[NN_LAYERS];
+  size_t n_params = 0, n_wbf = 0;
+  DevBuf<float> params, grads, m1, m2, logits, loss_rows, part;
+  DevBuf<bf16_t> wbf, concat, g_concat;
+  DevBuf<bf16_t> act[NN_LAYERS], grad[NN_LAYERS];
+  bool dirty = true;   // bf16 copies out of date
+  long long step = 0;
+};
+
+namespace {
+
+template <typename K, typename... Args>
+void nn_launch(hipStream_t s, K kernel, dim3 grid, dim3 block, size_t shmem, Args... args) {
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+  hipLaunchKernelGGL(kernel, grid, block, shmem, s, args...);
+  HIP_OK(hipGetLastError());
+}
+
+void nn_build(spk_nn* nn, int max_batch) {
+  nn->max_batch = max_batch;
+  nn->L[NN_I1].d = conv_desc(32, 32, 32, 1, 16);
+  nn->L[NN_I2].d = conv_desc(16, 16, 16, 16, 32);
+  nn->L[NN_I3].d = conv_desc(8, 8, 8, 32, 48);
+  nn->L[NN_T1].d = conv_desc(64, 32, 32, 2, 16);
+  nn->L[NN_T2].d = conv_desc(32, 16, 16, 16, 32);
+  nn->L[NN_T3].d = conv_desc(16, 8, 8, 32, 48);
+  nn->L[NN_F1].d = dense_desc(kNnConcat, 128);
+  nn->L[NN_F2].d = dense_desc(128, kNnLogits);
+  nn->L[NN_F2].relu = 0;
+  size_t p = 0, w = 0, part = 0;
+  for (auto& l : nn->L) {
+    l.w_off = p;
+    p += (size_t)l.d.cout * l.d.K;
+    l.b_off = p;
+    p += l.d.cout;
+    l.wb_off = w;
+    w += (size_t)l.d.cout * l.d.Kp;
+    l.wt_off = w;
+    w += (size_t)l.d.cin * l.d.Ktp;
+    part = std::max(part, wgrad_chunks_max(l.d) * l.d.cout * (l.d.K + 1));
+    if (l.d.cout % 8) throw SpkError{SPK_EINVAL, "proposer layer width must be a multiple of 8"};
+  }
+  nn->n_params = p;
+  nn->n_wbf = w;
+  nn->params.ensure(p);
+  nn->grads.ensure(p);
+  nn->m1.ensure(p);
+  nn->m2.ensure(p);
+  nn->wbf.ensure(w);
+  nn->part.ensure(part);
+  const size_t B = (size_t)max_batch;
+  for (int i = 0; i < NN_LAYERS; ++i) {
+    nn->act[i].ensure(B * nn->L[i].out_size());
+    nn->grad[i].ensure(B * nn->L[i].out_size());
+  }
+  nn->concat.ensure(B * kNnConcat);
+  nn->g_concat.ensure(B * kNnConcat);
+  nn->logits.ensure(B * kNnLogits);
+  nn->loss_rows.ensure(B);
+  HIP_OK(hipMemsetAsync(nn->params.p, 0, p * sizeof(float), nn->stream));
+  HIP_OK(hipMemsetAsync(nn->m1.p, 0, p * sizeof(float), nn->stream));
+  HIP_OK(hipMemsetAsync(nn->m2.p, 0, p * sizeof(float), nn->stream));
+  HIP_OK(hipStreamSynchronize(nn->stream));
+  nn->dirty = true;
+}
+
+void nn_cast(spk_nn* nn, hipStream_t s) {
+  if (!nn->dirty) return;
+  for (auto& l : nn->L) {
+    const int n = l.d.cout * l.d.Kp + l.d.cin * l.d.Ktp;
+    nn_launch(s, spknn::k_cast_weights, dim3((n + 255) / 256), dim3(256), 0,
+              (const float*)nn->params.p + l.w_off, l.d, nn->wbf.p + l.wb_off,
+              nn->wbf.p + l.wt_off);
+  }
+  nn->dirty = false;
+}
+
+int nt_for(int n) { return n <= 16 ? 1 : n <= 32 ? 2 : n <= 48 ? 3 : 4; }
+
+// forward of one layer over a batch of B rows of inputs
+void nn_fwd(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* X, bf16_t* Y,
+            float* Yf) {
+  ConvDesc d = l.d;
+  d.B = B;
+  const int M = B * d.o0 * d.o1 * d.o2, N = d.cout, NT = nt_for(N);
+  const dim3 grid((M + 127) / 128, (N + 16 * NT - 1) / (16 * NT));
+  const bf16_t* W = nn->wbf.p + l.wb_off;
+  const float* bias = nn->params.p + l.b_off;
+  const bool vec = d.cin % 8 == 0;
+#define SPK_NN_FWD(V, T)                                                                         \
+  nn_launch(s, spknn::k_igemm<spknn::MODE_FWD, V, T>, grid, dim3(256), 0, X, W, bias, d, M, N, \
+            d.Kp, d.Kp, l.relu, (const bf16_t*)nullptr, Y, Yf)
+  if (vec) {
+    switch (NT) {
+      case 1: SPK_NN_FWD(true, 1); break;
+      case 2: SPK_NN_FWD(true, 2); break;
+      case 3: SPK_NN_FWD(true, 3); break;
+      default: SPK_NN_FWD(true, 4); break;
+    }
+  } else {
+    switch (NT) {
+      case 1: SPK_NN_FWD(false, 1); break;
+      case 2: SPK_NN_FWD(false, 2); break;
+      case 3: SPK_NN_FWD(false, 3); break;
+      default: SPK_NN_FWD(false, 4); break;
+    }
+  }
+#undef SPK_NN_FWD
+}
+
+// input gradient of one layer: dX = (dY * W) masked by the layer input > 0
+void nn_bwd_data(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* dY,
+                 const bf16_t* mask, bf16_t* dX) {
+  ConvDesc d = l.d;
+  d.B = B;
+  const int M = B * d.i0 * d.i1 * d.i2, N = d.cin, NT = nt_for(N);
+  const dim3 grid((M + 127) / 128, (N + 16 * NT - 1) / (16 * NT));
+  const bf16_t* Wt = nn->wbf.p + l.wt_off;
+#define SPK_NN_BWD(T)                                                                            \
+  nn_launch(s, spknn::k_igemm<spknn::MODE_BWD, true, T>, grid, dim3(256), 0, dY, Wt,            \
+            (const float*)nullptr, d, M, N, d.Ktp, d.Ktp, 0, mask, dX, (float*)nullptr)
+  switch (NT) {
+    case 1: SPK_NN_BWD(1); break;
+    case 2: SPK_NN_BWD(2); break;
+    case 3: SPK_NN_BWD(3); break;
+    default: SPK_NN_BWD(4); break;
+  }
+#undef SPK_NN_BWD
+}
+
+// weight and bias gradients of one layer, written (not accumulated) into grads
+void nn_wgrad(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* X,
+              const bf16_t* dY) {
+  ConvDesc d = l.d;
+  d.B = B;
+  const int M = B * d.o0 * d.o1 * d.o2, N = d.cout, K = d.K;
+  const int kb = (K + 63) / 64;
+  int chunks = std::max(1, std::min((M + 31) / 32, (512 + kb - 1) / kb));
+  const int MC = (((M + chunks - 1) / chunks) + 31) & ~31;
+  chunks = (M + MC - 1) / MC;
+  float* part = nn->part.p;
+  float* partb = part + (size_t)chunks * N * K;
+  nn_launch(s, spknn::k_wgrad, dim3(kb, chunks), dim3(256), 0, X, dY, d, M, N, MC, part);
+  nn_launch(s, spknn::k_reduce_parts, dim3((N * K + 255) / 256), dim3(256), 0,
+            (const float*)part, chunks, N * K, nn->grads.p + l.w_off);
+  nn_launch(s, spknn::k_bgrad, dim3(chunks, N), dim3(256), 0, dY, M, N, MC, partb);
+  nn_launch(s, spknn::k_reduce_parts, dim3((N + 255) / 256), dim3(256), 0, (const float*)partb,
+            chunks, N, nn->grads.p + l.b_off);
+}
+
+void nn_pool_fwd(hipStream_t s, const NnLayer& l, int B, const bf16_t* X, bf16_t* out, int off) {
+  const ConvDesc& d = l.d;
+  const int n = B * (d.o0 / 2) * (d.o1 / 2) * (d.o2 / 2) * d.cout;
+  nn_launch(s, spknn::k_pool_fwd, dim3((n + 255) / 256), dim3(256), 0, X, B, d.o0, d.o1, d.o2,
+            d.cout, out, kNnConcat, off);
+}
+
+void nn_pool_bwd(hipStream_t s, const NnLayer& l, int B, const bf16_t* g, int off,
+                 const bf16_t* X, bf16_t* dX) {
+  const ConvDesc& d = l.d;
+  const size_t n = (size_t)B * d.o0 * d.o1 * d.o2 * d.cout;
+  nn_launch(s, spknn::k_pool_bwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g, kNnConcat,
+            off, X, B, d.o0, d.o1, d.o2, d.cout, dX);
+}
+
+// item branch: B item feature grids -> pooled features in rows of `out`
+// (row pitch kNnConcat, columns [0, 384))
+void nn_item_branch(spk_nn* nn, hipStream_t s, const bf16_t* item, int B, bf16_t* out) {
+  nn_fwd(nn, s, nn->L[NN_I1], B, item, nn->act[NN_I1].p, nullptr);
+  nn_fwd(nn, s, nn->L[NN_I2], B, nn->act[NN_I1].p, nn->act[NN_I2].p, nullptr);
+  nn_fwd(nn, s, nn->L[NN_I3], B, nn->act[NN_I2].p, nn->act[NN_I3].p, nullptr);
+  nn_pool_fwd(s, nn->L[NN_I3], B, nn->act[NN_I3].p, out, 0);
+}
+
+// tray branch + head on nn->concat whose item columns are filled
+void nn_tray_head(spk_nn* nn, hipStream_t s, const bf16_t* tray, int B) {
+  nn_fwd(nn, s, nn->L[NN_T1], B, tray, nn->act[NN_T1].p, nullptr);
+  nn_fwd(nn, s, nn->L[NN_T2], B, nn->act[NN_T1].p, nn->act[NN_T2].p, nullptr);
+  nn_fwd(nn, s, nn->L[NN_T3], B, nn->act[NN_T2].p, nn->act[NN_T3].p, nullptr);
+  nn_pool_fwd(s, nn->L[NN_T3], B, nn->act[NN_T3].p, nn->concat.p, kNnItemPooled);
+  nn_fwd(nn, s, nn->L[NN_F1], B, nn->concat.p, nn->act[NN_F1].p, nullptr);
+  nn_fwd(nn, s, nn->L[NN_F2], B, nn->act[NN_F1].p, nn->act[NN_F2].p, nn->logits.p);
+}
+
+void nn_forward(spk_nn* nn, hipStream_t s, const bf16_t* item, const bf16_t* tray, int B) {
+  nn_cast(nn, s);
+  nn_item_branch(nn, s, item, B, nn->concat.p);
+  nn_tray_head(nn, s, tray, B);
+}
+
+// loss, then every parameter gradient into nn->grads
+void nn_backward(spk_nn* nn, hipStream_t s, const bf16_t* item, const bf16_t* tray,
+                 const float* target, const int32_t* mask, int B) {
+  nn_launch(s, spknn::k_loss, dim3((B + 63) / 64), dim3(64), 0, (const float*)nn->logits.p, target,
+            mask, B, kNnLogits, nn->grad[NN_F2].p, nn->loss_rows.p);
+  const NnLayer* L = nn->L;
+  nn_wgrad(nn, s, L[NN_F2], B, nn->act[NN_F1].p, nn->grad[NN_F2].p);
+  nn_bwd_data(nn, s, L[NN_F2], B, nn->grad[NN_F2].p, nn->act[NN_F1].p, nn->grad[NN_F1].p);
+  nn_wgrad(nn, s, L[NN_F1], B, nn->concat.p, nn->grad[NN_F1].p);
+  nn_bwd_data(nn, s, L[NN_F1], B, nn->grad[NN_F1].p, nullptr, nn->g_concat.p);
+  nn_pool_bwd(s, L[NN_I3], B, nn->g_concat.p, 0, nn->act[NN_I3].p, nn->grad[NN_I3].p);
+  nn_pool_bwd(s, L[NN_T3], B, nn->g_concat.p, kNnItemPooled, nn->act[NN_T3].p, nn->grad[NN_T3].p);
+  // item branch
+  nn_wgrad(nn, s, L[NN_I3], B, nn->act[NN_I2].p, nn->grad[NN_I3].p);
+  nn_bwd_data(nn, s, L[NN_I3], B, nn->grad[NN_I3].p, nn->act[NN_I2].p, nn->grad[NN_I2].p);
+  nn_wgrad(nn, s, L[NN_I2], B, nn->act[NN_I1].p, nn->grad[NN_I2].p);
+  nn_bwd_data(nn, s, L[NN_I2], B, nn->grad[NN_I2].p, nn->act[NN_I1].p, nn->grad[NN_I1].p);
+  nn_wgrad(nn, s, L[NN_I1], B, item, nn->grad[NN_I1].p);
+  // tray branch
+  nn_wgrad(nn, s, L[NN_T3], B, nn->act[NN_T2].p, nn->grad[NN_T3].p);
+  nn_bwd_data(nn, s, L[NN_T3], B, nn->grad[NN_T3].p, nn->act[NN_T2].p, nn->grad[NN_T2].p);
+  nn_wgrad(nn, s, L[NN_T2], B, nn->act[NN_T1].p, nn->grad[NN_T2].p);
+  nn_bwd_data(nn, s, L[NN_T2], B, nn->grad[NN_T2].p, nn->act[NN_T1].p, nn->grad[NN_T1].p);
+  nn_wgrad(nn, s, L[NN_T1], B, tray, nn->grad[NN_T1].p);
+}
+
+float nn_loss_sum(spk_nn* nn, hipStream_t s, int B) {
+  std::vector<float> rows(B);
+  HIP_OK(hipMemcpyAsync(rows.data(), nn->loss_rows.p, B * sizeof(float), hipMemcpyDeviceToHost, s));
+  HIP_OK(hipStreamSynchronize(s));
+  double t = 0.0;
+  for (float r : rows) t += r;
+  return (float)(t / B);
+}
+
+// item feature grids of one (base-orientation) item already on the device
+void nn_item_features(hipStream_t s, const int* item, int ix, int iy, int iz, bf16_t* out) {
+  nn_launch(s, spknn::k_item_feat, dim3(kNnItemFeat / 256), dim3(256), 0, item, ix, iy, iz, out);
+}
+
+void nn_tray_features(hipStream_t s, const int* tray, int nx, int ny, int nz, int* heights,
+                      bf16_t* out) {
+  nn_launch(s, spknn::k_heightmap, dim3((nx * ny + 255) / 256), dim3(256), 0, tray, nx, ny, nz,
+            heights);
+  nn_launch(s, spknn::k_tray_feat, dim3(kNnTrayFeat / 2 / 256), dim3(256), 0, tray,
+            (const int*)heights, nx, ny, nz, out);
+}
+
+template <typename F>
+int nn_guarded(spk_nn* nn, F&& f) {
+  if (!nn) return fail(SPK_EINVAL, "null proposer");
+  try {
+    DeviceGuard g(nn->device);
+    return f();
+  } catch (const HipError& e) {
+    return fail(e.code, e.msg);
+  } catch (const SpkError& e) {
+    return fail(e.code, e.msg);
+  } catch (const std::bad_alloc&) {
+    return fail(SPK_ENOMEM, "host allocation failed");
+  }
+}
+
+void nn_check_batch(const spk_nn* nn, int batch) {
+  if (batch <= 0 || batch > nn->max_batch)
+    throw SpkError{SPK_EINVAL, "batch must be in [1, max_batch]"};
+}
+
+}  // namespace

@@ -1,0 +1,281 @@
+"""Learned orientation proposer: Python side of the spk_nn_* C-ABI.
+
+No reference counterpart: the reference searches all orientations of every
+item (spectral_packer/packer.py:487-505).  Here a 3-D CNN trained on the plain
+packer's own per-orientation scores proposes the top-k orientations and the
+exact FFT search judges them (``pack_learned``); the plain search over every
+fitting orientation is the fallback when confidence is low or no proposal is
+feasible.
+
+Device buffers (features, targets, masks) are ``DeviceArray`` objects whose
+memory libspk.so allocates (spk_dev_*), so one HIP runtime serves the whole
+path; the network, its training and inference are the HIP kernels of
+libspk.so.  Parameters are float32 vectors in the layout of
+``spk_nn_set_params`` (layer by layer, W [cout][27*cin | cin] then b [cout]).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+
+ITEM_FEATURES = 32768
+TRAY_FEATURES = 131072
+N_LOGITS = 24
+N_PARAMS = 262760
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "proposer.npy")
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise RuntimeError(f"libspk error {rc}: {_lib.last_error()}")
+
+
+class DeviceArray:
+    """A device buffer of libspk.so with a numpy shape and dtype (bf16 data is
+    held as uint16 bit patterns)."""
+
+    def __init__(self, shape, dtype, device: int = 0):
+        self.shape = tuple(int(v) for v in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.device = int(device)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self._lib = _lib.load()
+        p = C.c_void_p()
+        _check(self._lib.spk_dev_alloc(self.device, max(self.nbytes, 16), C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray, device: int = 0) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype, device)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        _check(self._lib.spk_dev_copy(self.device, self.ptr, a.ctypes.data_as(C.c_void_p),
+                                      self.nbytes, 0))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        _check(self._lib.spk_dev_copy(self.device, out.ctypes.data_as(C.c_void_p), self.ptr,
+                                      self.nbytes, 1))
+        return out
+
+    def row(self, i: int) -> C.c_void_p:
+        """pointer to row i (first axis)"""
+        per = self.nbytes // self.shape[0]
+        return C.c_void_p(self.ptr.value + int(i) * per)
+
+    def gather_from(self, src: "DeviceArray", idx) -> None:
+        """self rows = src rows idx (len(idx) == self.shape[0])"""
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        per = src.nbytes // src.shape[0]
+        if per != self.nbytes // self.shape[0] or len(idx) > self.shape[0]:
+            raise ValueError("row size mismatch")
+        _check(self._lib.spk_dev_gather(self.device, self.ptr, src.ptr, per, src.shape[0],
+                                        idx.ctypes.data_as(C.POINTER(C.c_int64)), len(idx)))
+
+    def close(self) -> None:
+        if getattr(self, "ptr", None) is not None and self.ptr.value:
+            self._lib.spk_dev_free(self.device, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ptr(t) -> C.c_void_p:
+    return t.ptr if isinstance(t, DeviceArray) else C.c_void_p(int(t))
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-uniform weights, zero biases (float32, N_PARAMS)."""
+    fans = [27, 432, 864, 54, 432, 864, 1152, 128]
+    couts = [16, 32, 48, 16, 32, 48, 128, 24]
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, n in zip(fans, couts):
+        lim = np.sqrt(6.0 / k)
+        out.append(rng.uniform(-lim, lim, n * k).astype(np.float32))
+        out.append(np.zeros(n, np.float32))
+    p = np.concatenate(out)
+    assert p.size == N_PARAMS
+    return p
+
+
+class Proposer:
+    """One network on one device; one thread at a time."""
+
+    def __init__(self, device: int = 0, max_batch: int = 256, params: Optional[np.ndarray] = None):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        _check(self._lib.spk_nn_create(int(device), int(max_batch), C.byref(h)))
+        self._h = h
+        self.device = int(device)
+        self.max_batch = int(max_batch)
+        n = C.c_int64()
+        _check(self._lib.spk_nn_num_params(h, C.byref(n)))
+        self.n_params = int(n.value)
+        if params is not None:
+            self.set_params(params)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.spk_nn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(self._lib.spk_nn_set_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float)),
+                                           p.size))
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n_params, np.float32)
+        _check(self._lib.spk_nn_get_params(self._h, p.ctypes.data_as(C.POINTER(C.c_float)),
+                                           p.size))
+        return p
+
+    def save(self, path: str = WEIGHTS) -> None:
+        os.makedirs(os.path.dirname(path), exist_ok=True)
+        np.save(path, self.get_params())
+
+    def load(self, path: str = WEIGHTS) -> None:
+        self.set_params(np.load(path, allow_pickle=False))
+
+    # ---- batches of device features -------------------------------------
+    def forward(self, item_feat: DeviceArray, tray_feat: DeviceArray, batch: Optional[int] = None) -> np.ndarray:
+        B = int(item_feat.shape[0] if batch is None else batch)
+        out = np.empty((B, N_LOGITS), np.float32)
+        _check(self._lib.spk_nn_forward(self._h, _ptr(item_feat), _ptr(tray_feat), B,
+                                        out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def gradient(self, item_feat, tray_feat, target, mask) -> Tuple[float, np.ndarray]:
+        B = int(item_feat.shape[0])
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_float()
+        _check(self._lib.spk_nn_gradient(self._h, _ptr(item_feat), _ptr(tray_feat), _ptr(target),
+                                         _ptr(mask), B, g.ctypes.data_as(C.POINTER(C.c_float)),
+                                         C.byref(loss)))
+        return float(loss.value), g
+
+    def train_step(self, item_feat, tray_feat, target, mask, lr: float,
+                   batch: Optional[int] = None) -> float:
+        B = int(item_feat.shape[0] if batch is None else batch)
+        loss = C.c_float()
+        _check(self._lib.spk_nn_train_step(self._h, _ptr(item_feat), _ptr(tray_feat),
+                                           _ptr(target), _ptr(mask), B, float(lr),
+                                           C.byref(loss)))
+        return float(loss.value)
+
+    def item_features(self, item: np.ndarray, out: Optional[DeviceArray] = None, row: int = 0):
+        """bf16 bits [32768] on the device (into row `row` of `out` if given)."""
+        it = np.ascontiguousarray(item, dtype=np.int32)
+        if out is None:
+            out = DeviceArray((1, ITEM_FEATURES), np.uint16, self.device)
+        _check(self._lib.spk_nn_item_features(self._h, it.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              *it.shape, out.row(row)))
+        return out
+
+
+def tray_features(session, out: Optional[DeviceArray] = None, row: int = 0):
+    """bf16 bits [131072] of the session's cached tray on the device."""
+    if out is None:
+        out = DeviceArray((1, TRAY_FEATURES), np.uint16)
+    _check(_lib.load().spk_tray_features(C.c_void_p(session.handle()), out.row(row)))
+    return out
+
+
+class _Args:
+    """spk_pack_args over numpy buffers kept alive by this object."""
+
+    def __init__(self, items: Sequence[np.ndarray], order, perms, flips, penalty: str,
+                 id_offset: int = 0):
+        grids = [np.ascontiguousarray(it, dtype=np.int32) for it in items]
+        for g in grids:
+            if g.ndim != 3:
+                raise ValueError("items must be 3-D")
+        self.flat = np.concatenate([g.reshape(-1) for g in grids]) if grids else np.zeros(0, np.int32)
+        self.dims = np.array([g.shape for g in grids], np.int32).reshape(-1)
+        self.order = np.ascontiguousarray(order, dtype=np.int32)
+        P = np.asarray(perms, np.int32).reshape(-1, 3)
+        F = np.asarray(flips, np.int32).reshape(-1, 3)
+        self.orients = (_lib.SpkOrient * len(P))()
+        for o in range(len(P)):
+            self.orients[o].perm[:] = [int(v) for v in P[o]]
+            self.orients[o].flip[:] = [int(v) for v in F[o]]
+        self.n = len(grids)
+        self.results = (_lib.SpkResult * max(1, self.n))()
+        a = _lib.SpkPackArgs()
+        a.items = self.flat.ctypes.data_as(C.POINTER(C.c_int32))
+        a.dims = self.dims.ctypes.data_as(C.POINTER(C.c_int32))
+        a.n_items = self.n
+        a.order = self.order.ctypes.data_as(C.POINTER(C.c_int32))
+        a.orients = self.orients
+        a.n_orient = len(P)
+        a.penalty = {"pow": 0, "cube": 1}[penalty]
+        a.interlocking_free = 0
+        a.id_offset = int(id_offset)
+        a.results = self.results
+        self.args = a
+
+    def result_dicts(self) -> List[dict]:
+        out = []
+        for t in range(self.n):
+            r = self.results[t]
+            out.append({"position": tuple(r.pos), "found": bool(r.found), "score": r.score,
+                        "orientation": r.orient, "collision": r.collision,
+                        "proximity": r.proximity, "residual": r.residual, "path": r.path})
+        return out
+
+
+def pack_records(session, items, order, perms, flips, penalty: str = "cube", id_offset: int = 0,
+                 feats: Optional[DeviceArray] = None, row0: int = 0):
+    """Plain greedy pack that also returns, per processed item, the tray
+    features it saw (bf16 bits on the device, rows row0.. of `feats`, a new
+    [n, 131072] array by default) and every orientation's best score (numpy
+    [n, n_orient], +inf when infeasible)."""
+    a = _Args(items, order, perms, flips, penalty, id_offset)
+    if feats is None:
+        feats = DeviceArray((a.n, TRAY_FEATURES), np.uint16)
+    if row0 + a.n > feats.shape[0]:
+        raise ValueError("feature buffer too small")
+    scores = np.empty((a.n, a.args.n_orient), np.float64)
+    _check(_lib.load().spk_pack_records(C.c_void_p(session.handle()), C.byref(a.args),
+                                        feats.row(row0), scores.ctypes.data_as(C.POINTER(C.c_double))))
+    return a.result_dicts(), feats, scores
+
+
+def pack_learned(session, proposer: Proposer, items, order, perms, flips, penalty: str = "cube",
+                 topk: int = 4, min_conf: float = 0.0, id_offset: int = 0):
+    """Greedy pack with learned top-k orientation proposals.  Returns
+    (results, proposals [n, topk] (-1 = full search), counts[4])."""
+    a = _Args(items, order, perms, flips, penalty, id_offset)
+    props = np.empty((a.n, topk), np.int32)
+    counts = np.zeros(4, np.int32)
+    _check(_lib.load().spk_pack_learned(C.c_void_p(session.handle()), proposer.handle,
+                                        C.byref(a.args), int(topk), float(min_conf),
+                                        props.ctypes.data_as(C.POINTER(C.c_int32)),
+                                        counts.ctypes.data_as(C.POINTER(C.c_int32))))
+    return a.result_dicts(), props, counts
