@@ -1,15 +1,22 @@
 """Throughput benchmark: items/s of the greedy spectral packer on one MI355X.
 
 Workload (default): 240x123x100 int32 tray, 350 seeded unions of 3-6
-ellipsoids/capsules (bounding boxes 20-50 voxels, fill 0.30-0.40), all 24
-orientations per item in one batched search, proximity from the rounded
-Euclidean distance to all occupied voxels, FP64 score combine.  A step is one
-item: its 24-orientation search, the placement when a feasible offset exists,
-and the refresh of the tray's distance field and spectra.  W warmup items
-run first on the same tray; the next K items are timed.
+ellipsoids/capsules (bounding boxes 20-50 voxels, fill 0.30-0.40), 24
+orientations, proximity from the rounded Euclidean distance to all occupied
+voxels, FP64 score combine.  A step is one item: its orientation search, the
+placement when a feasible offset exists, and the refresh of the tray's
+distance field and spectra.
 
-Prints one JSON line (rank 0) with the roofline of the dominant kernel
-(HIP events on the library's stream) and a bounded CPU baseline of the
+Two loops run over the same K items (default all 350, volume order) from an
+empty tray, after W warmup items on a throwaway tray:
+  plain    every fitting orientation searched in one batched launch,
+  learned  the proposer's top-k orientations searched (full search when its
+           confidence is low or no proposal is feasible).
+The metric's value is the learned loop's items/s when its final tray density
+is within 0.5 percentage points of the plain loop's, else the plain loop's.
+
+Prints one JSON line with the roofline of the dominant kernel of the reported
+loop (HIP events on the library's stream) and a bounded CPU baseline of the
 reference CPU path restated in oracle/ref_cpu_path.py.
 """
 from __future__ import annotations
@@ -29,6 +36,12 @@ if ROOT not in sys.path:
 
 METRIC = "items placed/s @240×123×100 tray, 24 rot; learned vs plain at ≤0.5pp density"
 HBM_PEAK_GBS = 8000.0
+
+
+KERNELS = ["dist_zy", "dist_x", "tray_slab_fwd", "tray_rows_fwd", "tray_x_fwd", "item_slab_fwd",
+           "item_rows_fwd", "pencil_y_fwd", "corr_x", "score_yz", "pencil_y_inv", "rows_score",
+           "select", "verify", "decide", "place", "direct_search", "gather_od", "proposer",
+           "proposer_items", "record", "feas_begin", "feas_x", "feas_yz", "feas_decide"]
 
 
 def device_sync():
@@ -130,7 +143,7 @@ def bench_uneven(args) -> int:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=350)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="readme",
                     choices=["readme", "blobs", "boxes", "uneven"])
@@ -142,6 +155,13 @@ def main(argv=None) -> int:
                     help="seconds of CPU work for the baseline sample (0 = skip)")
     ap.add_argument("--plan", type=int, default=0, choices=[0, 1, 2],
                     help="FFT plan: 0 auto, 1 yz slabs, 2 z rows + y pencils")
+    ap.add_argument("--proposer", default="auto", choices=["auto", "plain", "learned"],
+                    help="auto: time the plain and the learned loop and report the learned one "
+                         "when its density is within 0.5 pp of the plain one")
+    ap.add_argument("--topk", type=int, default=4)
+    ap.add_argument("--min-conf", type=float, default=0.0)
+    ap.add_argument("--weights", default=os.path.join(ROOT, "psacking_amd", "weights",
+                                                      "proposer.npy"))
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args(argv)
@@ -166,6 +186,7 @@ def main(argv=None) -> int:
         return bench_uneven(args)
 
     import psacking_amd as P
+    from psacking_amd import proposer as PR
     from psacking_amd import workloads as W
     from psacking_amd.rotations import orientation_maps
 
@@ -174,94 +195,158 @@ def main(argv=None) -> int:
     items = [np.ascontiguousarray(it.astype(np.int32)) for it in w.items]
     vols = [int((it > 0).sum()) for it in items]
     order = [int(i) for i in np.argsort(vols)[::-1]]
-    W_ = min(args.warmup, len(items) - 1)
-    K = max(1, min(args.steps, len(items) - W_))
+    K = max(1, min(args.steps, len(items)))
+    W_ = max(0, min(args.warmup, len(items)))
     perms, flips = orientation_maps(n_orient)
     penalty = "cube" if n_orient > 1 else "pow"
-
-    s = P.Session(0)
-    s.set_option("plan", args.plan)
-    tray0 = np.zeros(w.tray_size, np.int32)
-    t0 = time.perf_counter()
-    s.set_tray(tray0, args.metric, args.sources, 0)
-    device_sync()
-    t_setup = time.perf_counter() - t0
-
-    # warmup: first W items of the volume-sorted sequence, same tray
-    warm_items = [items[i] for i in order[:W_]]
-    res_w = s.pack(warm_items, np.arange(W_, dtype=np.int32), perms, flips, penalty, False, 0)
-    placed_w = sum(1 for r in res_w if r["found"])
-
-    timed_items = [items[i] for i in order[W_:W_ + K]]
-    s.timing_reset()
-    s.timing(True)
-    device_sync()
-    t1 = time.perf_counter()
-    res = s.pack(timed_items, np.arange(K, dtype=np.int32), perms, flips, penalty, False, placed_w)
-    device_sync()
-    dt = time.perf_counter() - t1
-    s.timing(False)
-    placed = sum(1 for r in res if r["found"])
-    fallback = sum(1 for r in res if r["path"] != 0)
-    max_resid = max(float(r["residual"]) for r in res)
-
-    # per-kernel event timing over the timed region
-    names = ["dist_zy", "dist_x", "tray_slab_fwd", "tray_rows_fwd", "tray_x_fwd",
-             "item_slab_fwd", "item_rows_fwd", "pencil_y_fwd", "corr_x", "score_yz",
-             "pencil_y_inv", "rows_score", "select", "verify", "decide", "place",
-             "direct_search"]
-    kt = {n: s.timing_get(n) for n in names}
-    busy = sum(v[0] for v in kt.values())
-    dom = max(("score_yz", "corr_x"), key=lambda n: kt[n][0])
     Nx, Ny, Nz = w.tray_size
-    Ph = Nx * Ny * (Nz // 2 + 1)
     V = Nx * Ny * Nz
-    # orientations searched per launch (orientations that fit the tray)
-    fits = []
-    for it in timed_items:
-        dims = it.shape
-        f = sum(1 for o in range(n_orient)
-                if all(dims[perms[o][a]] <= w.tray_size[a] for a in range(3)))
-        fits.append(f)
-    m_avg = float(np.mean(fits)) if fits else 0.0
-    ms_dom, n_dom = kt[dom]
-    avg_s = (ms_dom / max(1, n_dom)) / 1e3
-    achieved = kernel_bytes(dom, Ph, V, m_avg) / avg_s / 1e9 if avg_s > 0 else 0.0
+    Ph = Nx * Ny * (Nz // 2 + 1)
+    want_plain = args.proposer in ("auto", "plain")
+    want_learned = args.proposer in ("auto", "learned") and n_orient > 1
+    nn = None
+    if want_learned:
+        if os.path.exists(args.weights):
+            nn = PR.Proposer(0, 64, np.load(args.weights, allow_pickle=False))
+        elif args.proposer == "learned":
+            raise FileNotFoundError(args.weights)
+        else:
+            want_learned = False
 
-    for n in names:
-        if kt[n][1]:
-            print(f"[kernel] {n:14s} {kt[n][0]:9.2f} ms  {kt[n][1]:6d} launches  "
-                  f"{kt[n][0] / max(1, kt[n][1]) * 1e3:8.1f} us/launch  "
-                  f"{100 * kt[n][0] / max(busy, 1e-9):5.1f} %", file=sys.stderr)
-    print(f"[bench] warmup {W_} items ({placed_w} placed), timed {K} items ({placed} placed), "
-          f"{dt:.3f} s, kernels busy {busy / 1e3:.3f} s, setup {t_setup:.3f} s, "
-          f"fallbacks {fallback}, max residual {max_resid:.4f}", file=sys.stderr, flush=True)
+    def fresh_session():
+        s = P.Session(0)
+        s.set_option("plan", args.plan)
+        t0 = time.perf_counter()
+        s.set_tray(np.zeros(w.tray_size, np.int32), args.metric, args.sources, 0)
+        device_sync()
+        return s, time.perf_counter() - t0
+
+    # warmup on a throwaway tray: the first W items of the sequence
+    if W_:
+        warm = [items[i] for i in order[:W_]]
+        if want_plain:
+            s, _ = fresh_session()
+            s.pack(warm, np.arange(W_, dtype=np.int32), perms, flips, penalty, False, 0)
+        if want_learned:
+            s, _ = fresh_session()
+            PR.pack_learned(s, nn, warm, np.arange(W_), perms, flips, penalty, args.topk,
+                            args.min_conf)
+        device_sync()
+
+    timed = [items[i] for i in order[:K]]
+    fits = [sum(1 for o in range(n_orient)
+                if all(it.shape[perms[o][a]] <= w.tray_size[a] for a in range(3)))
+            for it in timed]
+    runs = {}
+
+    def finish(name, s, dt, res, t_setup, searched):
+        t = s.tray()
+        kt = {n: s.timing_get(n) for n in KERNELS}
+        runs[name] = {"dt": dt, "res": res, "density": float((t > 0).sum()) / V, "kt": kt,
+                      "setup": t_setup, "searched": searched,
+                      "placed": sum(1 for r in res if r["found"]),
+                      "fallbacks": sum(1 for r in res if r["path"] != 0),
+                      "max_resid": max(float(r["residual"]) for r in res)}
+
+    if want_plain:
+        s, t_setup = fresh_session()
+        s.timing_reset()
+        s.timing(True)
+        device_sync()
+        t1 = time.perf_counter()
+        res = s.pack(timed, np.arange(K, dtype=np.int32), perms, flips, penalty, False, 0)
+        device_sync()
+        dt = time.perf_counter() - t1
+        s.timing(False)
+        finish("plain", s, dt, res, t_setup, sum(fits))
+    if want_learned:
+        s, t_setup = fresh_session()
+        s.timing_reset()
+        s.timing(True)
+        device_sync()
+        t1 = time.perf_counter()
+        res, props, counts = PR.pack_learned(s, nn, timed, np.arange(K), perms, flips, penalty,
+                                             args.topk, args.min_conf)
+        device_sync()
+        dt = time.perf_counter() - t1
+        s.timing(False)
+        searched = 0
+        for t, r in enumerate(res):
+            p = [int(o) for o in props[t] if o >= 0]
+            if not p:
+                searched += fits[t]
+            else:
+                searched += len(p)
+                if not r["found"] or r["orientation"] not in p:
+                    searched += fits[t]
+        finish("learned", s, dt, res, t_setup, searched)
+        runs["learned"]["counts"] = [int(c) for c in counts]
+
+    for name, r in runs.items():
+        busy = sum(v[0] for v in r["kt"].values())
+        for n in KERNELS:
+            if r["kt"][n][1]:
+                print(f"[kernel:{name}] {n:14s} {r['kt'][n][0]:9.2f} ms  {r['kt'][n][1]:6d} launches  "
+                      f"{r['kt'][n][0] / max(1, r['kt'][n][1]) * 1e3:8.1f} us/launch  "
+                      f"{100 * r['kt'][n][0] / max(busy, 1e-9):5.1f} %", file=sys.stderr)
+        print(f"[bench:{name}] {K} items ({r['placed']} placed), {r['dt']:.3f} s = "
+              f"{K / r['dt']:.1f} items/s, kernels busy {busy / 1e3:.3f} s, density "
+              f"{100 * r['density']:.2f} %, fallbacks {r['fallbacks']}, max residual "
+              f"{r['max_resid']:.4f}" + (f", counts {r['counts']}" if "counts" in r else ""),
+              file=sys.stderr, flush=True)
+
+    agree = None
+    if "plain" in runs and "learned" in runs:
+        a, b = runs["plain"]["res"], runs["learned"]["res"]
+        same = [x["found"] == y["found"] and (not x["found"] or (
+            tuple(x["position"]) == tuple(y["position"]) and x["orientation"] == y["orientation"]))
+            for x, y in zip(a, b)]
+        first = next((i for i, v in enumerate(same) if not v), len(same))
+        agree = {"placements_identical": float(np.mean(same)), "first_divergence": first,
+                 "density_plain": runs["plain"]["density"],
+                 "density_learned": runs["learned"]["density"],
+                 "density_loss_pp": 100 * (runs["plain"]["density"] - runs["learned"]["density"]),
+                 "items_per_s_plain": K / runs["plain"]["dt"],
+                 "items_per_s_learned": K / runs["learned"]["dt"],
+                 "speedup": runs["plain"]["dt"] / runs["learned"]["dt"]}
+        print(f"[bench] learned vs plain: {agree}", file=sys.stderr, flush=True)
+
+    # the metric's value: learned items/s when its density is within 0.5 pp
+    # of the plain method's, else the plain method's items/s
+    if "learned" in runs and (agree is None or agree["density_loss_pp"] <= 0.5):
+        used = "learned"
+    else:
+        used = "plain" if "plain" in runs else "learned"
+    r = runs[used]
+    dom = max(("score_yz", "corr_x"), key=lambda n: r["kt"][n][0])
+    ms_dom, n_dom = r["kt"][dom]
+    avg_s = (ms_dom / max(1, n_dom)) / 1e3
+    m_avg = r["searched"] / max(1, n_dom)
+    achieved = kernel_bytes(dom, Ph, V, m_avg) / avg_s / 1e9 if avg_s > 0 else 0.0
 
     cpu = None
     if args.cpu_budget > 0:
         from oracle import ref_cpu_path as R
         cores = min(16, len(os.sched_getaffinity(0)))
-        item0 = items[order[W_]]
-        # the tray state the first timed item saw is the warmup result; the
-        # reference path's cost does not depend on tray contents
+        item0 = items[order[0]]
+        # the reference path's cost does not depend on tray contents
         sec_item, nsearch, spent = R.time_item_sample(item0, np.zeros(w.tray_size, np.int32),
                                                       n_orient, workers=cores,
                                                       budget_s=args.cpu_budget)
         cpu = {"value": 1.0 / sec_item, "unit": "items/s", "cores": cores, "kind": "port",
                "sample": (f"reference CPU path restated (scipy pocketfft C2C double at 2N+1, "
                           f"plan per call): {nsearch} of {n_orient} orientation searches of item "
-                          f"{order[W_]} timed ({spent:.1f} s), extrapolated to one item")}
+                          f"{order[0]} timed ({spent:.1f} s), extrapolated to one item")}
         print(f"[cpu] {cpu}", file=sys.stderr, flush=True)
 
-    value = K / dt
     out = {
         "metric": METRIC,
-        "value": value,
+        "value": K / r["dt"],
         "unit": "items/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": W_,
-        "ms_per_step": dt / K * 1e3,
+        "ms_per_step": r["dt"] / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -271,10 +356,18 @@ def main(argv=None) -> int:
             "workload": (f"{w.name}: {Nx}x{Ny}x{Nz} int32 tray, {len(items)} seeded items "
                          f"(seed {w.seed}), {n_orient} orientations, proximity "
                          f"{args.metric}/{args.sources}, FP32 FFT + FP64 score"),
+            "path": used,
+            "proposer": (None if used != "learned" else
+                         {"topk": args.topk, "min_conf": args.min_conf,
+                          "counts": runs["learned"].get("counts"),
+                          "weights": os.path.relpath(args.weights, ROOT)}),
+            "learned_vs_plain": agree,
             "items_timed": K,
-            "items_placed_timed": placed,
-            "exact_fallbacks": fallback,
-            "max_fft_residual": max_resid,
+            "items_placed_timed": r["placed"],
+            "tray_density": r["density"],
+            "exact_fallbacks": r["fallbacks"],
+            "max_fft_residual": r["max_resid"],
+            "setup_s": r["setup"],
         },
         "roofline": {
             "bound": "hbm",

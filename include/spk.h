@@ -167,7 +167,13 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  for every search, which the tests use to check that path).
  *   "plan"         FFT plan of the tray: 0 auto, 1 yz slabs in one block,
  *                  2 z rows + y pencils (large y*z planes); applies at the
- *                  next spk_tray_set. */
+ *                  next spk_tray_set.
+ *   "feas"         feasibility pass of the greedy loops (slab plan): 0 off,
+ *                  1 (default) for an item after one that placed nothing,
+ *                  2 every item.  The pass tests every orientation for a
+ *                  zero-collision offset at about a third of the scored
+ *                  search's cost; an item with none is recorded as not found
+ *                  without the scored search.  Results are unchanged. */
 int spk_set_option(spk_ctx* ctx, const char* name, double value);
 
 /* Plan actually used for the cached tray: FFT sizes and digit base. */
@@ -179,7 +185,7 @@ int spk_plan_info(spk_ctx* ctx, int* px, int* py, int* pz, int* digit_base);
  * Every call synchronises the device.  kind: 0 host->device, 1 device->host,
  * 2 device->device.  spk_dev_gather copies rows idx[0..n) (host array, each
  * in [0, src_rows)) of src into consecutive rows of dst; row_bytes must be a
- * multiple of 16. */
+ * multiple of 4. */
 int spk_dev_alloc(int device, int64_t bytes, void** out);
 int spk_dev_free(int device, void* ptr);
 int spk_dev_copy(int device, void* dst, const void* src, int64_t bytes, int kind);
@@ -233,14 +239,17 @@ int spk_pack_records(spk_ctx* ctx, const spk_pack_args* args, uint16_t* tray_fea
                      double* scores);
 
 /* The greedy loop with learned proposals (plain, not interlocking-free; at
- * most 24 orientations).  For each item with more than topk fitting
- * orientations whose topk best-scored ones hold a softmax mass >= min_conf,
- * only those are searched and the best placement among them is taken; every
- * fitting orientation is searched otherwise, or when none of them is
- * feasible.  proposals[t*topk + j] (optional) lists the searched proposals
- * (-1 when the item was searched over all its orientations); counts[4]
- * (optional): items decided on proposals, full searches by rule, full
- * searches after infeasible proposals, exact host-path decisions. */
+ * most 24 orientations).  Candidates are an item's fitting orientations,
+ * narrowed to the feasible ones when the feasibility pass ran first (it does
+ * after an item that placed nothing, unless option "feas" is 0).  With more
+ * than topk candidates whose topk best-scored ones hold a softmax mass >=
+ * min_conf, only those are searched and the best placement among them is
+ * taken; all candidates are searched otherwise, and the other candidates
+ * also when no proposal is feasible.  proposals[t*topk + j] (optional) lists
+ * the searched proposals (-1 when the item was searched over all its
+ * candidates); counts[5] (optional): items decided on proposals, searches of
+ * all candidates, searches after infeasible proposals, exact host-path
+ * decisions, items the feasibility pass found infeasible. */
 int spk_pack_learned(spk_ctx* ctx, spk_nn* nn, const spk_pack_args* args, int topk,
                      float min_conf, int32_t* proposals, int32_t* counts);
 

@@ -169,6 +169,11 @@ struct spk_ctx {
   DevBuf<unsigned short> ifeat, ipool;   // item features [n][32^3], pooled item rows [n][384]
   DevBuf<double> rscores;                // per-orientation best scores [n][n_orient]
   DevBuf<OrientDesc> od_prop;            // descriptors of the proposed orientations
+  DevBuf<OrientDesc> od_rest;            // descriptors of the other orientations searched
+  // feasibility pass
+  DevBuf<int> fstate;                    // residual bits, unused, per-orientation flags
+  DevBuf<int> ctl_on;                    // control words that enable the pass outside the loop
+  int feas_mode = 1;                     // 0 off, 1 after an item that placed nothing, 2 always
   // timing
   bool timing = false;
   std::vector<Timed> pending;
@@ -352,6 +357,8 @@ void plan_tray(spk_ctx* c) {
     allow_shmem(k_dist_zy, dyn);
     allow_shmem(k_corr_x, dyn);
     allow_shmem(k_tray_x_fwd, dyn);
+    allow_shmem(k_feas_x, dyn);
+    allow_shmem(k_feas_yz, dyn);
     c->lds_attrs = true;
   }
 }
@@ -609,6 +616,7 @@ void ensure_search_ws(spk_ctx* c, int m, int max_ex) {
   c->per.ensure(m);
   c->sout.ensure(1);
   c->ver.ensure(3);
+  c->fstate.ensure(kFeasHeader + std::max(m, 24));
 }
 
 // Enqueue one batched search of m orientation descriptors (device array od):
@@ -616,10 +624,9 @@ void ensure_search_ws(spk_ctx* c, int m, int max_ex) {
 // winner.  Results land in c->per, c->sout and c->ver.  With fgrid/pgrid
 // (interlocking-free) the feasibility and proximity grids are written and the
 // flood fill + masked argmin run before the selection.
-void enqueue_search(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
-                    const int* ctl, unsigned char* fgrid, long long* pgrid,
-                    const std::vector<OrientDesc>* host_ods) {
-  const int py = c->py.n, pz = c->pz.n, hz = pz / 2 + 1;
+// item spectra of m orientation descriptors into c->ispec
+void enqueue_item_spectra(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const int* ctl) {
+  const int py = c->py.n, pz = c->pz.n;
   const float2* tw = c->tw.p;
   const size_t ysh = (size_t)(py + py * c->Ty) * sizeof(float2);
   if (c->split) {
@@ -636,10 +643,46 @@ void enqueue_search(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
            slab_shmem(c), (const int*)c->items.p, od, c->ispec.p, max_ex, c->py, c->pz,
            tw + c->px.n, tw + c->px.n + py, ctl);
   }
+}
+
+// x-pass chunking: enough blocks to fill the chip, few orientations per block
+int corr_chunks(const spk_ctx* c, int m, int* chunk) {
+  const int py = c->py.n, pz = c->pz.n;
   const int tiles = (pz + c->T - 1) / c->T;
   int nchunk = std::max(1, std::min(m, (2048 + tiles * py - 1) / (tiles * py)));
-  const int chunk = (m + nchunk - 1) / nchunk;
-  nchunk = (m + chunk - 1) / chunk;
+  *chunk = (m + nchunk - 1) / nchunk;
+  return (m + *chunk - 1) / *chunk;
+}
+
+// Feasibility pass over m descriptors whose item spectra are in c->ispec
+// (slab plan only): per-orientation flags and the residual in c->fstate.
+void enqueue_feas(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const int* ctl) {
+  const int py = c->py.n, pz = c->pz.n;
+  const float2* tw = c->tw.p;
+  const int np = (m + 1) / 2;
+  const int tiles = (pz + c->T - 1) / c->T;
+  int chunk = 0;
+  const int nchunk = corr_chunks(c, np, &chunk);
+  const float scale = (float)(1.0 / ((double)c->px.n * py * pz));
+  launch(c, "feas_x", k_feas_x, dim3(tiles, py, nchunk), dim3(kPencilThreads), pencil_shmem(c),
+         (const float2*)c->g1.p, (const float2*)c->ispec.p, od, c->y1.p, m, chunk, max_ex, c->nx,
+         c->px, py, pz, c->T, scale, tw, ctl);
+  launch(c, "feas_yz", k_feas_yz, dim3(c->nx, np), dim3(kSlabThreads), slab_shmem(c),
+         (const float2*)c->y1.p, od, m, c->nx, c->py, c->pz, tw + c->px.n, tw + c->px.n + py,
+         c->fstate.p, ctl);
+}
+
+// Scored search (pass A, pass B, selection, exact re-check of the winner) of
+// m descriptors whose item spectra are in c->ispec.
+void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
+                    const int* ctl, unsigned char* fgrid, long long* pgrid,
+                    const std::vector<OrientDesc>* host_ods) {
+  const int py = c->py.n, pz = c->pz.n, hz = pz / 2 + 1;
+  const float2* tw = c->tw.p;
+  const size_t ysh = (size_t)(py + py * c->Ty) * sizeof(float2);
+  const int tiles = (pz + c->T - 1) / c->T;
+  int chunk = 0;
+  const int nchunk = corr_chunks(c, m, &chunk);
   const float scale = (float)(1.0 / ((double)c->px.n * py * pz));
   launch(c, "corr_x", k_corr_x, dim3(tiles, py, nchunk), dim3(kPencilThreads), pencil_shmem(c),
          (const float2*)c->g1.p, (const float2*)c->g2.p, (const float2*)c->ispec.p, od, c->y1.p,
@@ -675,6 +718,13 @@ void enqueue_search(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
   launch(c, "verify", k_verify, dim3(std::min(512, (max_e + 255) / 256)), dim3(256), 0,
          (const int*)c->tray.p, (const int*)c->phi.p, (const int*)c->items.p, od,
          (const SearchOut*)c->sout.p, c->ver.p, c->ny, c->nz, ctl);
+}
+
+void enqueue_search(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
+                    const int* ctl, unsigned char* fgrid, long long* pgrid,
+                    const std::vector<OrientDesc>* host_ods) {
+  enqueue_item_spectra(c, od, m, max_ex, ctl);
+  enqueue_scored(c, od, m, max_ex, max_e, penalty, ctl, fgrid, pgrid, host_ods);
 }
 
 // items: per entry base item + orientation; all on host.
@@ -880,10 +930,27 @@ PackPlan plan_pack(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
 // One item of the device-driven loop: batched search over m descriptors (od),
 // k_decide, optional per-orientation score record, placement of the accepted
 // winner and refresh of the distance field and spectra.
+//
+// With the feasibility pass on (slab plan, feas_mode 1 or 2), k_feas_begin
+// decides on the device whether this item runs it (mode 1: when the previous
+// item placed nothing); when it runs and finds no feasible orientation,
+// k_feas_decide records the item as not found and the scored search,
+// placement and refresh are skipped.
 void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
                   int t, int32_t id_offset, int halt_not_found, double* rec_row,
-                  const int* rec_map) {
-  enqueue_search(c, od, m, max_ex, max_e, penalty, c->ctl.p, nullptr, nullptr, nullptr);
+                  const int* rec_map, int feas_mode = -1) {
+  if (feas_mode < 0) feas_mode = c->feas_mode;
+  if (feas_mode && !c->split) {
+    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, m, feas_mode,
+           c->ctl.p);
+    enqueue_item_spectra(c, od, m, max_ex, c->ctl.p);
+    enqueue_feas(c, od, m, max_ex, c->ctl.p);
+    launch(c, "feas_decide", k_feas_decide, dim3(1), dim3(64), 0, (const int*)c->fstate.p, m,
+           c->pout.p, t, c->resid_limit, c->ctl.p);
+    enqueue_scored(c, od, m, max_ex, max_e, penalty, c->ctl.p, nullptr, nullptr, nullptr);
+  } else {
+    enqueue_search(c, od, m, max_ex, max_e, penalty, c->ctl.p, nullptr, nullptr, nullptr);
+  }
   launch(c, "decide", k_decide, dim3(1), dim3(64), 0, (const SearchOut*)c->sout.p,
          (const long long*)c->ver.p, c->pout.p, t, c->resid_limit, c->ctl.p, halt_not_found);
   if (rec_row)
@@ -1044,17 +1111,22 @@ void timed_group(spk_ctx* c, const char* name, F&& f) {
 }
 
 // Greedy loop with learned orientation proposals.  The proposer scores the
-// orientations of an item against the tray as it is at that item.  When more
-// than topk orientations fit and the softmax mass of the topk best-scored
-// ones (over the fitting orientations) is at least min_conf, only those are
-// searched, in orientation order so that ties resolve as in the full search;
-// the outcome is the best placement among them.  Every fitting orientation is
-// searched otherwise, and also when no proposal has a feasible offset.  One
-// host synchronisation per item reads the outcome and the logits of the next
-// item, which the device computed right after this item's placement.
-// counts: [0] items decided on their proposals, [1] full searches (too few
-// fitting orientations or low confidence), [2] full searches after
-// infeasible proposals, [3] items decided on the host's exact path.
+// orientations of an item against the tray as it is at that item; the exact
+// search judges its top-k.  Candidates are the item's fitting orientations,
+// narrowed to the feasible ones when the feasibility pass ran first.  When
+// more than topk candidates remain and the softmax mass of the topk
+// best-scored ones (over the candidates) is at least min_conf, only those
+// are searched, in orientation order so that ties resolve as in the full
+// search, and the outcome is the best placement among them; all candidates
+// are searched otherwise, and the remaining ones also when no proposal is
+// feasible.  The feasibility pass runs first after an item that placed
+// nothing (most such items cannot be placed at all), else only to screen the
+// remaining orientations after infeasible proposals.  One host
+// synchronisation per item (two when the pass runs first) reads the outcome
+// and the logits of the next item, computed right after this item's
+// placement.  counts: [0] items decided on their proposals, [1] searches of
+// all candidates, [2] searches after infeasible proposals, [3] items decided
+// on the host's exact path, [4] items the feasibility pass found infeasible.
 int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
                  const std::vector<int>& off, const int32_t* order, const spk_orient* orients,
                  int n_orient, int penalty, int32_t id_offset, spk_result* results, int topk,
@@ -1087,7 +1159,17 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   c->tfeat.ensure(kNnTrayFeat);
   c->heights.ensure((size_t)c->nx * c->ny);
   c->od_prop.ensure(kNnLogits);
+  c->od_rest.ensure(kNnLogits);
+  c->ctl_on.ensure(CTL_WORDS);
+  {
+    int on[CTL_WORDS] = {0};
+    on[CTL_HALT_ITEM] = -1;
+    on[CTL_FEAS] = 1;
+    upload(c, c->ctl_on.p, on, sizeof(on));
+  }
+  const bool feas_plan = c->feas_mode != 0 && !c->split;
   float logits[kNnLogits];
+  int fsv[kFeasHeader + kNnLogits];
   auto enqueue_head = [&](int t) {
     timed_group(c, "proposer", [&]() {
       nn_tray_features(s, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, c->tfeat.p);
@@ -1097,14 +1179,60 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     });
     download(c, logits, nn->logits.p, sizeof(logits));
   };
+  // descriptors of the listed fitting orientations of item t, on the device
+  auto gather = [&](const ItemPlan& pl, const std::vector<int>& list, OrientDesc* dst,
+                    int* max_ex, int* max_e) -> const OrientDesc* {
+    *max_ex = 1;
+    *max_e = 1;
+    for (int i : list) {
+      const OrientDesc& d = all[pl.od_off + i];
+      *max_ex = std::max(*max_ex, d.ex);
+      *max_e = std::max(*max_e, d.ex * d.ey * d.ez);
+    }
+    if ((int)list.size() == pl.m) return c->od_all.p + pl.od_off;   // the whole list, in order
+    OrientPick op{};
+    for (size_t j = 0; j < list.size(); ++j) op.idx[j] = list[j];
+    launch(c, "gather_od", k_gather_od, dim3(1), dim3(32), 0,
+           (const OrientDesc*)c->od_all.p + pl.od_off, op, (int)list.size(), dst);
+    return dst;
+  };
+  // feasibility of the listed orientations: false when the pass' residual is
+  // out of bounds (nothing learned), else `list` keeps the feasible ones
+  auto screen = [&](const ItemPlan& pl, std::vector<int>& list) -> bool {
+    int max_ex, max_e;
+    const OrientDesc* od = gather(pl, list, c->od_rest.p, &max_ex, &max_e);
+    const int k = (int)list.size();
+    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, k, 2, (int*)nullptr);
+    enqueue_item_spectra(c, od, k, max_ex, nullptr);
+    enqueue_feas(c, od, k, max_ex, c->ctl_on.p);
+    download(c, fsv, c->fstate.p, (kFeasHeader + k) * sizeof(int));
+    sync(c);
+    float res;
+    std::memcpy(&res, &fsv[0], sizeof(res));
+    if (!(res <= c->resid_limit)) return false;
+    std::vector<int> keep;
+    for (int j = 0; j < k; ++j)
+      if (fsv[kFeasHeader + j]) keep.push_back(list[j]);
+    list.swap(keep);
+    return true;
+  };
   int placed = 0;
   auto reset_ctl = [&]() {
-    const int v[CTL_WORDS] = {0, -1, 0, placed};
+    int v[CTL_WORDS] = {0};
+    v[CTL_HALT_ITEM] = -1;
+    v[CTL_PLACED] = placed;
     upload(c, c->ctl.p, v, sizeof(v));
   };
-  int cnt[4] = {0, 0, 0, 0};
+  auto accept = [&](int t, const ItemPlan& pl, const std::vector<int>& list, const PackOut& po) {
+    const SearchOut& so = po.so;
+    to_result(c, so.found, so.idx, so.score, so.found ? pl.map[list[so.orient]] : -1,
+              po.collision, so.prox, so.resid, 0, &results[t]);
+    if (so.found) ++placed;
+  };
+  int cnt[5] = {0, 0, 0, 0, 0};
   reset_ctl();
-  bool have = false;   // logits of item t are in `logits` and match the tray
+  bool have = false;        // logits of item t are in `logits` and match the tray
+  bool prev_placed = true;
   for (int t = 0; t < n_items; ++t) {
     const ItemPlan& pl = plans[t];
     if (proposals)
@@ -1113,20 +1241,38 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, &results[t]);
       continue;
     }
-    std::vector<int> pick;   // fitting indices searched
+    const bool have_t = have;   // logits of item t computed after the previous step
+    have = false;
+    std::vector<int> cand(pl.m);
+    for (int i = 0; i < pl.m; ++i) cand[i] = i;
+    bool screened = false;
+    if (feas_plan && !prev_placed) {
+      screened = screen(pl, cand);
+      if (screened && cand.empty()) {
+        ++cnt[4];
+        float res;
+        std::memcpy(&res, &fsv[0], sizeof(res));
+        to_result(c, false, 0, 0.0, -1, 0, 0, res, 0, &results[t]);
+        continue;
+      }
+      if (!screened) {
+        cand.resize(pl.m);
+        for (int i = 0; i < pl.m; ++i) cand[i] = i;
+      }
+    }
+    std::vector<int> pick;
     bool use_prop = false;
-    if (pl.m > topk) {
-      if (!have) {
+    if ((int)cand.size() > topk) {
+      if (!have_t) {
         enqueue_head(t);
         sync(c);
       }
       float mx = -INFINITY;
-      for (int i = 0; i < pl.m; ++i) mx = std::max(mx, logits[pl.map[i]]);
-      std::vector<int> rank(pl.m);
-      std::vector<double> e(pl.m);
+      for (int i : cand) mx = std::max(mx, logits[pl.map[i]]);
+      std::vector<int> rank = cand;
       double se = 0.0;
-      for (int i = 0; i < pl.m; ++i) {
-        rank[i] = i;
+      std::vector<double> e(pl.m, 0.0);
+      for (int i : cand) {
         e[i] = std::exp((double)logits[pl.map[i]] - mx);
         se += e[i];
       }
@@ -1141,29 +1287,13 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
         std::sort(pick.begin(), pick.end());
       }
     }
-    have = false;
-    if (!use_prop) {
-      pick.resize(pl.m);
-      for (int i = 0; i < pl.m; ++i) pick[i] = i;
-    }
-    const int k = (int)pick.size();
-    if (use_prop) {
-      OrientPick op{};
-      int max_ex = 1, max_e = 1;
-      for (int j = 0; j < k; ++j) {
-        op.idx[j] = pick[j];
-        const OrientDesc& d = all[pl.od_off + pick[j]];
-        max_ex = std::max(max_ex, d.ex);
-        max_e = std::max(max_e, d.ex * d.ey * d.ez);
-        if (proposals) proposals[(size_t)t * topk + j] = pl.map[pick[j]];
-      }
-      launch(c, "gather_od", k_gather_od, dim3(1), dim3(32), 0,
-             (const OrientDesc*)c->od_all.p + pl.od_off, op, k, c->od_prop.p);
-      enqueue_step(c, c->od_prop.p, k, max_ex, max_e, penalty, t, id_offset, 1, nullptr, nullptr);
-    } else {
-      enqueue_step(c, c->od_all.p + pl.od_off, pl.m, pl.max_ex, pl.max_e, penalty, t, id_offset,
-                   0, nullptr, nullptr);
-    }
+    if (!use_prop) pick = cand;
+    int max_ex, max_e;
+    const OrientDesc* od = gather(pl, pick, c->od_prop.p, &max_ex, &max_e);
+    if (use_prop && proposals)
+      for (size_t j = 0; j < pick.size(); ++j) proposals[(size_t)t * topk + j] = pl.map[pick[j]];
+    enqueue_step(c, od, (int)pick.size(), max_ex, max_e, penalty, t, id_offset, use_prop ? 1 : 0,
+                 nullptr, nullptr, 0);
     // the next searched item's logits, on the tray this step leaves
     int tn = t + 1;
     while (tn < n_items && plans[tn].m == 0) ++tn;
@@ -1176,42 +1306,51 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     sync(c);
     if (c->timing) c->flush_timing();
     if (ctl[CTL_HALT] == 0) {
-      const SearchOut& so = po.so;
-      to_result(c, so.found, so.idx, so.score, so.found ? pl.map[pick[so.orient]] : -1,
-                po.collision, so.prox, so.resid, 0, &results[t]);
-      if (so.found) ++placed;
+      accept(t, pl, pick, po);
       ++cnt[use_prop ? 0 : 1];
+      prev_placed = po.so.found;
       have = next_head;
       continue;
     }
     const int code = ctl[CTL_HALT];
     float resid = po.so.resid;
     reset_ctl();
+    std::vector<int> rest;
     if (code == 2) {
-      // no proposal is feasible: search every fitting orientation
+      // no proposal is feasible: the other candidates, screened for
+      // feasibility first unless they already were
       ++cnt[2];
-      enqueue_step(c, c->od_all.p + pl.od_off, pl.m, pl.max_ex, pl.max_e, penalty, t, id_offset,
-                   0, nullptr, nullptr);
-      download(c, ctl, c->ctl.p, sizeof(ctl));
-      download(c, &po, c->pout.p + t, sizeof(po));
-      sync(c);
-      if (c->timing) c->flush_timing();
-      if (ctl[CTL_HALT] == 0) {
-        const SearchOut& so = po.so;
-        to_result(c, so.found, so.idx, so.score, so.found ? pl.map[so.orient] : -1,
-                  po.collision, so.prox, so.resid, 0, &results[t]);
-        if (so.found) ++placed;
-        reset_ctl();
+      for (int i : cand)
+        if (!std::binary_search(pick.begin(), pick.end(), i)) rest.push_back(i);
+      if (feas_plan && !screened && screen(pl, rest) && rest.empty()) {
+        ++cnt[4];
+        to_result(c, false, 0, 0.0, -1, 0, 0, resid, 0, &results[t]);
+        prev_placed = false;
         continue;
       }
-      resid = po.so.resid;
-      reset_ctl();
+      if (!rest.empty()) {
+        const OrientDesc* odr = gather(pl, rest, c->od_rest.p, &max_ex, &max_e);
+        enqueue_step(c, odr, (int)rest.size(), max_ex, max_e, penalty, t, id_offset, 0, nullptr,
+                     nullptr, 0);
+        download(c, ctl, c->ctl.p, sizeof(ctl));
+        download(c, &po, c->pout.p + t, sizeof(po));
+        sync(c);
+        if (c->timing) c->flush_timing();
+        if (ctl[CTL_HALT] == 0) {
+          accept(t, pl, rest, po);
+          prev_placed = po.so.found;
+          reset_ctl();
+          continue;
+        }
+        resid = po.so.resid;
+        reset_ctl();
+      }
       pick.resize(pl.m);
       for (int i = 0; i < pl.m; ++i) pick[i] = i;
       use_prop = false;
     }
     // the FFT outcome was not trusted: exact direct sums over the same
-    // orientations, then over all of them if no proposal is feasible
+    // orientations, then over all fitting ones if no proposal is feasible
     ++cnt[3];
     const int it = order ? order[t] : t;
     ExactBest eb = exact_over(c, bases[it], &all[pl.od_off], pick, penalty, nullptr);
@@ -1222,10 +1361,11 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       eb = exact_over(c, bases[it], &all[pl.od_off], every, penalty, nullptr);
     }
     placed += place_exact(c, eb, pl, all, resid, id_offset + placed + 1, &results[t]);
+    prev_placed = eb.fit >= 0;
     reset_ctl();
   }
   if (counts)
-    for (int i = 0; i < 4; ++i) counts[i] = cnt[i];
+    for (int i = 0; i < 5; ++i) counts[i] = cnt[i];
   return placed;
 }
 
@@ -1325,7 +1465,8 @@ int spk_ctx_destroy(spk_ctx* c) {
     c->ffree.release(); c->freach.release(); c->pgrid.release(); c->exits.release();
     c->od_all.release(); c->pout.release(); c->ctl.release(); c->zero.release();
     c->heights.release(); c->map_all.release(); c->tfeat.release(); c->ifeat.release();
-    c->ipool.release(); c->rscores.release(); c->od_prop.release();
+    c->ipool.release(); c->rscores.release(); c->od_prop.release(); c->od_rest.release();
+    c->fstate.release(); c->ctl_on.release();
     if (c->stream) hipStreamDestroy(c->stream);
   }
   delete c;
@@ -1770,6 +1911,13 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->valid = false;
     return SPK_OK;
   }
+  if (std::string(name) == "feas") {
+    const int v = (int)value;
+    if (v < 0 || v > 2)
+      return fail(SPK_EINVAL, "feas must be 0 (off), 1 (after an item that placed nothing) or 2");
+    c->feas_mode = v;
+    return SPK_OK;
+  }
   return fail(SPK_EINVAL, std::string("unknown option ") + name);
 }
 
@@ -1817,8 +1965,8 @@ int spk_dev_copy(int device, void* dst, const void* src, int64_t bytes, int kind
 
 int spk_dev_gather(int device, void* dst, const void* src, int64_t row_bytes, int64_t src_rows,
                    const int64_t* idx, int n) {
-  if (!dst || !src || !idx || n <= 0 || row_bytes <= 0 || row_bytes % 16)
-    return fail(SPK_EINVAL, "bad gather arguments (row size must be a multiple of 16 bytes)");
+  if (!dst || !src || !idx || n <= 0 || row_bytes <= 0 || row_bytes % 4)
+    return fail(SPK_EINVAL, "bad gather arguments (row size must be a multiple of 4 bytes)");
   for (int i = 0; i < n; ++i)
     if (idx[i] < 0 || idx[i] >= src_rows) return fail(SPK_EINVAL, "gather index out of range");
   return dev_guarded(device, [&]() {
@@ -1828,11 +1976,16 @@ int spk_dev_gather(int device, void* dst, const void* src, int64_t row_bytes, in
     const hipError_t e1 = hipMemcpy(d_idx, idx, (size_t)n * sizeof(long long), hipMemcpyHostToDevice);
     hipError_t e2 = hipSuccess;
     if (e1 == hipSuccess) {
-      const long long words = row_bytes / 16;
-      hipLaunchKernelGGL(spknn::k_gather_rows,
-                         dim3((unsigned)std::min<long long>((words + 255) / 256, 1024), n),
-                         dim3(256), 0, nullptr, (uint4*)dst, (const uint4*)src, words,
-                         (const long long*)d_idx, n);
+      const bool wide = row_bytes % 16 == 0 && (uintptr_t)dst % 16 == 0 &&
+                        (uintptr_t)src % 16 == 0;
+      const long long words = row_bytes / (wide ? 16 : 4);
+      const dim3 grid((unsigned)std::min<long long>((words + 255) / 256, 1024), n);
+      if (wide)
+        hipLaunchKernelGGL(spknn::k_gather_rows<uint4>, grid, dim3(256), 0, nullptr, (uint4*)dst,
+                           (const uint4*)src, words, (const long long*)d_idx, n);
+      else
+        hipLaunchKernelGGL(spknn::k_gather_rows<unsigned>, grid, dim3(256), 0, nullptr,
+                           (unsigned*)dst, (const unsigned*)src, words, (const long long*)d_idx, n);
       e2 = hipGetLastError();
       if (e2 == hipSuccess) e2 = hipDeviceSynchronize();
     }

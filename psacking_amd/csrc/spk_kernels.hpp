@@ -96,9 +96,22 @@ struct SearchOut {           // written by k_select, read by verify/place
 
 // Control words of the device-driven greedy loop (spk_pack).  Kernels take
 // a nullable pointer: outside that loop they always run.
-enum { CTL_HALT = 0, CTL_HALT_ITEM = 1, CTL_SKIP_REFRESH = 2, CTL_PLACED = 3, CTL_WORDS = 4 };
-// search kernels stop once a search has been handed back to the host
-__device__ __forceinline__ bool search_off(const int* ctl) { return ctl && ctl[CTL_HALT]; }
+// CTL_SKIP_SEARCH: the feasibility pass found no feasible orientation, so the
+// item's scored search is skipped; CTL_FEAS: the feasibility pass runs for
+// the current item.
+enum {
+  CTL_HALT = 0, CTL_HALT_ITEM = 1, CTL_SKIP_REFRESH = 2, CTL_PLACED = 3, CTL_SKIP_SEARCH = 4,
+  CTL_FEAS = 5, CTL_WORDS = 6
+};
+// search kernels stop once a search has been handed back to the host, and
+// for an item the feasibility pass has already decided
+__device__ __forceinline__ bool search_off(const int* ctl) {
+  return ctl && (ctl[CTL_HALT] | ctl[CTL_SKIP_SEARCH]);
+}
+// feasibility kernels run when the loop enabled them for the current item
+__device__ __forceinline__ bool feas_off(const int* ctl) {
+  return ctl && (ctl[CTL_HALT] || !ctl[CTL_FEAS]);
+}
 // refresh kernels also stop when the last item placed nothing
 __device__ __forceinline__ bool refresh_off(const int* ctl) {
   return ctl && (ctl[CTL_HALT] | ctl[CTL_SKIP_REFRESH]);
@@ -711,6 +724,187 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
 }
 
 // ---------------------------------------------------------------------------
+// Feasibility pass: does any valid offset of an orientation have zero
+// collision with the occupied voxels?  Collision-only correlations with the
+// binary tray B, two orientations per complex transform (orientation 2p in
+// the real part, 2p+1 in the imaginary part), about a third of the scored
+// search's work per orientation.  FFT(B) comes from the cached spectrum
+// G1 = FFT(B + i h) of two real fields: FFT(B)[k] = (G1[k] + conj(G1[-k])) / 2.
+// State fs[0] = max rounding residual (float bits), fs[1] unused,
+// fs[2 + o] = 1 when orientation o has a zero-collision valid offset.
+// ---------------------------------------------------------------------------
+constexpr int kFeasHeader = 2;
+
+// start of an item in the greedy loop: the scored search runs unless the
+// feasibility pass decides otherwise; mode 0 never runs the pass, 1 runs it
+// after an item that placed nothing, 2 always
+__global__ void k_feas_begin(int* __restrict__ fs, int m, int mode, int* __restrict__ ctl) {
+  if (ctl && ctl[CTL_HALT]) return;
+  const int i = threadIdx.x;
+  if (i == 0 && ctl) {
+    ctl[CTL_SKIP_SEARCH] = 0;
+    ctl[CTL_FEAS] = mode == 2 ? 1 : (mode == 1 ? ctl[CTL_SKIP_REFRESH] : 0);
+  }
+  for (int j = i; j < kFeasHeader + m; j += blockDim.x) fs[j] = 0;
+}
+
+// pass A: grid (ceil(pz/T), py, n_chunks); chunks of orientation pairs
+__global__ void __launch_bounds__(kPencilThreads)
+k_feas_x(const float2* __restrict__ g1, const float2* __restrict__ ispec,
+         const OrientDesc* __restrict__ od, float2* __restrict__ y1, int m, int chunk, int max_ex,
+         int nx, AxisPlan px_, int py, int pz, int T, float scale, const float2* __restrict__ tw_x,
+         const int* __restrict__ ctl) {
+  if (feas_off(ctl)) return;
+  extern __shared__ float2 smem[];
+  const int px = px_.n;
+  float2* twx = smem;
+  float2* buf = twx + px;
+  const int ky = blockIdx.y, tid = threadIdx.x;
+  const int kz0 = blockIdx.x * T;
+  const int np = (m + 1) / 2;
+  const int p0 = blockIdx.z * chunk;
+  const int p1 = min(np, p0 + chunk);
+  load_table(twx, tw_x, px, tid, kPencilThreads);
+  const int kym = ky ? py - ky : 0;
+  float2 G[kPencilMaxEl];
+#pragma unroll
+  for (int s = 0; s < kPencilMaxEl; ++s) {
+    const int e = tid + s * kPencilThreads;
+    const int x = e / T, t = e - x * T;
+    const int kz = kz0 + t;
+    G[s] = make_float2(0.f, 0.f);
+    if (e < px * T && kz < pz) {
+      const float2 a = g1[((size_t)x * py + ky) * pz + kz];
+      const int xm = x ? px - x : 0, kzm = kz ? pz - kz : 0;
+      const float2 b = g1[((size_t)xm * py + kym) * pz + kzm];
+      G[s] = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+    }
+  }
+  for (int p = p0; p < p1; ++p) {
+    const int oa = 2 * p, ob = 2 * p + 1;
+    const bool hasb = ob < m;
+    const OrientDesc da = od[oa];
+    const OrientDesc db = od[hasb ? ob : oa];
+    const float2* sa = ispec + (size_t)oa * max_ex * py * pz;
+    const float2* sb = ispec + (size_t)(hasb ? ob : oa) * max_ex * py * pz;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kPencilMaxEl; ++s) {
+      const int e = tid + s * kPencilThreads;
+      if (e < px * T) {
+        const int x = e / T, t = e - x * T;
+        const int kz = kz0 + t;
+        float2 va = make_float2(0.f, 0.f), vb = make_float2(0.f, 0.f);
+        if (kz < pz) {
+          if (x < da.ex) va = sa[((size_t)x * py + ky) * pz + kz];
+          if (hasb && x < db.ex) vb = sb[((size_t)x * py + ky) * pz + kz];
+        }
+        buf[e] = make_float2(va.x - vb.y, va.y + vb.x);   // va + i vb
+      }
+    }
+    __syncthreads();
+    lds_fft<true, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
+#pragma unroll
+    for (int s = 0; s < kPencilMaxEl; ++s) {
+      const int e = tid + s * kPencilThreads;
+      if (e < px * T) buf[e] = cscale(cmulc(G[s], buf[e]), scale);
+    }
+    __syncthreads();
+    lds_fft<false, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
+    const int nvx = max(da.nvx, hasb ? db.nvx : 0);
+    float2* d1 = y1 + (size_t)p * nx * py * pz;
+    for (int e = tid; e < nvx * T; e += kPencilThreads) {
+      const int x = e / T, t = e - x * T;
+      const int kz = kz0 + t;
+      if (kz < pz) d1[((size_t)x * py + ky) * pz + kz] = buf[e];
+    }
+  }
+}
+
+// pass B: grid (nx, n_pairs).  Inverse yz of one x slice of a pair, then
+// flags orientations with a zero-collision valid offset in it.
+__global__ void __launch_bounds__(kSlabThreads)
+k_feas_yz(const float2* __restrict__ y1, const OrientDesc* __restrict__ od, int m, int nx,
+          AxisPlan py_, AxisPlan pz_, const float2* __restrict__ tw_y,
+          const float2* __restrict__ tw_z, int* __restrict__ fs, const int* __restrict__ ctl) {
+  if (feas_off(ctl)) return;
+  extern __shared__ float2 smem[];
+  const int py = py_.n, pz = pz_.n;
+  const int x = blockIdx.x, p = blockIdx.y, tid = threadIdx.x;
+  const int oa = 2 * p, ob = 2 * p + 1;
+  const bool hasb = ob < m;
+  const OrientDesc da = od[oa];
+  const OrientDesc db = od[hasb ? ob : oa];
+  const bool xa = x < da.nvx, xb = hasb && x < db.nvx;
+  if (!xa && !xb) return;
+  float2* twy = smem;
+  float2* twz = twy + py;
+  float2* buf = twz + pz;
+  load_table(twy, tw_y, py, tid, kSlabThreads);
+  load_table(twz, tw_z, pz, tid, kSlabThreads);
+  const float2* s1 = y1 + ((size_t)p * nx + x) * py * pz;
+  stage_rows<kSlabThreads, kSlabMaxEl>(buf, s1, py * pz, pz, pz, tid);
+  __syncthreads();
+  const int nvy = max(xa ? da.nvy : 0, xb ? db.nvy : 0);
+  lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, py_, pz, pz, 1, false, twy, tid);
+  lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, nvy, pz, true, twz, tid);
+  float res = 0.f;
+  int fa = 0, fb = 0;
+  const float inv_pz = 1.0f / (float)pz;
+#pragma unroll
+  for (int s = 0; s < kSlabMaxEl; ++s) {
+    const int e = tid + s * kSlabThreads;
+    const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
+    if (e < py * pz) {
+      const float2 v = buf[e];
+      if (xa && j < da.nvy && k < da.nvz) {
+        const float r = rintf(v.x);
+        res = fmaxf(res, fabsf(v.x - r));
+        fa |= r == 0.f;
+      }
+      if (xb && j < db.nvy && k < db.nvz) {
+        const float r = rintf(v.y);
+        res = fmaxf(res, fabsf(v.y - r));
+        fb |= r == 0.f;
+      }
+    }
+  }
+  fa = __syncthreads_or(fa);
+  fb = __syncthreads_or(fb);
+  for (int o = 32; o > 0; o >>= 1) res = fmaxf(res, __shfl_xor(res, o));
+  __shared__ float wr[kSlabThreads / 64];
+  if ((tid & 63) == 0) wr[tid >> 6] = res;
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < kSlabThreads / 64; ++w) res = fmaxf(res, wr[w]);
+    atomicMax(&fs[0], __float_as_int(res));
+    if (fa) fs[kFeasHeader + oa] = 1;
+    if (fb) fs[kFeasHeader + ob] = 1;
+  }
+}
+
+// greedy loop: with the pass run and its residual within resid_limit, an
+// item with no feasible orientation is decided here (not found) and its
+// scored search, placement and refresh are skipped
+__global__ void k_feas_decide(const int* __restrict__ fs, int m, PackOut* __restrict__ out, int t,
+                              float resid_limit, int* __restrict__ ctl) {
+  if (threadIdx.x != 0 || feas_off(ctl)) return;
+  const float res = __int_as_float(fs[0]);
+  int any = 0;
+  for (int o = 0; o < m; ++o) any |= fs[kFeasHeader + o];
+  if (!(res <= resid_limit) || any) return;
+  PackOut r{};
+  r.so.found = 0;
+  r.so.idx = INT_MAX;
+  r.so.score = 0.0;
+  r.so.resid = res;
+  r.collision = 0;
+  out[t] = r;
+  ctl[CTL_SKIP_SEARCH] = 1;
+  ctl[CTL_SKIP_REFRESH] = 1;
+}
+
+// ---------------------------------------------------------------------------
 // selection: one block of kSelectThreads; wave w reduces orientations w,
 // w+waves, ... over their x records, then thread 0 takes the first strictly
 // smaller score across orientations (list order).
@@ -875,7 +1069,8 @@ k_place(int* __restrict__ tray, const int* __restrict__ items, const OrientDesc*
 __global__ void k_decide(const SearchOut* __restrict__ so, const long long* __restrict__ ver,
                          PackOut* __restrict__ out, int t, float resid_limit,
                          int* __restrict__ ctl, int halt_not_found) {
-  if (threadIdx.x != 0 || ctl[CTL_HALT]) return;
+  // an item the feasibility pass decided (nothing feasible) keeps its record
+  if (threadIdx.x != 0 || ctl[CTL_HALT] || ctl[CTL_SKIP_SEARCH]) return;
   const SearchOut s = *so;
   const bool bad = s.unsupported || !(s.resid <= resid_limit) ||
                    (s.found && (ver[1] != 0 || ver[2] != s.prox));

@@ -403,13 +403,14 @@ __global__ void k_cast_weights(const float* __restrict__ w, ConvDesc d, bf16_t* 
   }
 }
 
-// dst row i = src row idx[i]; rows of `words` 16-byte words
-__global__ void k_gather_rows(uint4* __restrict__ dst, const uint4* __restrict__ src,
+// dst row i = src row idx[i]; rows of `words` words of type T (uint4 or unsigned)
+template <typename T>
+__global__ void k_gather_rows(T* __restrict__ dst, const T* __restrict__ src,
                               long long words, const long long* __restrict__ idx, int n) {
   const int i = blockIdx.y;
   if (i >= n) return;
-  const uint4* s = src + idx[i] * words;
-  uint4* d = dst + (long long)i * words;
+  const T* s = src + idx[i] * words;
+  T* d = dst + (long long)i * words;
   for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < words;
        w += (long long)gridDim.x * blockDim.x)
     d[w] = s[w];

@@ -270,12 +270,39 @@ def pack_records(session, items, order, perms, flips, penalty: str = "cube", id_
 def pack_learned(session, proposer: Proposer, items, order, perms, flips, penalty: str = "cube",
                  topk: int = 4, min_conf: float = 0.0, id_offset: int = 0):
     """Greedy pack with learned top-k orientation proposals.  Returns
-    (results, proposals [n, topk] (-1 = full search), counts[4])."""
+    (results, proposals [n, topk] (-1 = all candidates searched), counts[5]:
+    items decided on proposals, searches of all candidates, searches after
+    infeasible proposals, exact host-path decisions, items the feasibility
+    pass found infeasible)."""
     a = _Args(items, order, perms, flips, penalty, id_offset)
     props = np.empty((a.n, topk), np.int32)
-    counts = np.zeros(4, np.int32)
+    counts = np.zeros(5, np.int32)
     _check(_lib.load().spk_pack_learned(C.c_void_p(session.handle()), proposer.handle,
                                         C.byref(a.args), int(topk), float(min_conf),
                                         props.ctypes.data_as(C.POINTER(C.c_int32)),
                                         counts.ctypes.data_as(C.POINTER(C.c_int32))))
     return a.result_dicts(), props, counts
+
+
+def soft_targets(scores: np.ndarray, tau: float = 0.02) -> Tuple[np.ndarray, np.ndarray]:
+    """Training targets from per-orientation best scores (+inf = infeasible):
+    q_o proportional to exp(-(s_o - s_min) / (tau * (|s_min| + 1000))) over
+    the feasible orientations (sum 1); mask bit o set when orientation o is
+    feasible.  Rows with no feasible orientation get mask 0 and q = 0."""
+    s = np.asarray(scores, np.float64)
+    feas = np.isfinite(s)
+    smin = np.min(np.where(feas, s, np.inf), axis=1, keepdims=True)
+    smin = np.where(np.isfinite(smin), smin, 0.0)
+    scale = tau * (np.abs(smin) + 1000.0)
+    w = np.where(feas, np.exp(-(np.where(feas, s, smin) - smin) / scale), 0.0)
+    tot = w.sum(axis=1, keepdims=True)
+    q = np.where(tot > 0, w / np.where(tot > 0, tot, 1.0), 0.0)
+    mask = (feas.astype(np.int64) << np.arange(s.shape[1])[None, :]).sum(axis=1)
+    return q.astype(np.float32), mask.astype(np.int32)
+
+
+def load_trained(device: int = 0, path: str = WEIGHTS, max_batch: int = 64) -> Proposer:
+    """The proposer with the committed trained weights."""
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: train with `python -m psacking_amd.train_proposer`")
+    return Proposer(device, max_batch, np.load(path, allow_pickle=False))

@@ -406,3 +406,32 @@ def test_pack_many_uneven_trays(P):
                 assert (tuple(r["position"]), r["orientation"], r["score"]) == \
                     (e.position, e.orientation_index, e.score)
         np.testing.assert_array_equal(s.tray(), exp.tray)
+
+
+@pytest.mark.parametrize("feas", [0, 1, 2])
+def test_feasibility_pass_keeps_results(P, feas):
+    """A tray that fills up: later items have no feasible orientation.  The
+    feasibility pass (off, after a non-placement, every item) must leave every
+    outcome and the final tray equal to the oracle's."""
+    from psacking_amd import workloads as W
+    from psacking_amd.rotations import orientation_maps
+    w = W.blobs(seed=51, n=14, tray=(48, 40, 32))
+    vols = [int((it > 0).sum()) for it in w.items]
+    order = [int(i) for i in np.argsort(vols)[::-1]]
+    perms, flips = orientation_maps(24)
+    s = P.Session(0)
+    s.set_option("feas", feas)
+    s.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
+    say("feasibility pass mode", feas)
+    res = s.pack(w.items, np.array(order, np.int32), perms, flips, "cube", False, 0)
+    exp = O.pack_voxels(w.items, w.tray_size, num_orientations=24, order=order,
+                        metric="edt", sources="nonzero")
+    by_item = {p.item_index: p for p in exp.placements}
+    assert sum(1 for p in exp.placements if not p.success) >= 3
+    for t, r in enumerate(res):
+        e = by_item[order[t]]
+        assert r["found"] == e.success and r["path"] == 0
+        if e.success:
+            assert (tuple(r["position"]), r["orientation"], r["score"]) == \
+                (e.position, e.orientation_index, e.score)
+    np.testing.assert_array_equal(s.tray(), exp.tray)

@@ -229,3 +229,21 @@ def test_pack_learned_replays_on_oracle(P, PR, topk, min_conf, resid):
     exp, tray, _, _ = _replay(w.items, order, 24, w.tray_size, proposals=props)
     _check(res, exp)
     np.testing.assert_array_equal(s.tray(), tray)
+
+
+def test_pack_learned_on_a_filling_tray(P, PR):
+    """Most late items fit nowhere: the feasibility pass decides them; every
+    placement still replays exactly on the oracle."""
+    from psacking_amd.rotations import orientation_maps
+    w, order = _workload(51, 14, (48, 40, 32))
+    perms, flips = orientation_maps(24)
+    nn = PR.Proposer(0, 8, PR.init_params(52))
+    s = P.Session(0)
+    s.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
+    say("pack learned on a filling tray")
+    res, props, counts = PR.pack_learned(s, nn, w.items, order, perms, flips, "cube", topk=4)
+    say("counts", list(counts))
+    assert counts[4] > 0
+    exp, tray, _, _ = _replay(w.items, order, 24, w.tray_size, proposals=props)
+    _check(res, exp)
+    np.testing.assert_array_equal(s.tray(), tray)

@@ -148,3 +148,15 @@ def test_tray_features_definition(shape):
                     below += z < h[x, y]
         assert f[a, b, c, 0] == N.bf16(np.float32(occ) / np.float32(cnt))
         assert f[a, b, c, 1] == N.bf16(np.float32(below) / np.float32(cnt))
+
+
+def test_product_soft_targets_match_oracle():
+    from psacking_amd import proposer
+    rng = np.random.default_rng(9)
+    s = rng.uniform(0, 5e6, (50, 24))
+    s[rng.random((50, 24)) < 0.4] = np.inf
+    s[3] = np.inf
+    q1, m1 = proposer.soft_targets(s)
+    q2, m2 = N.soft_targets(s)
+    np.testing.assert_array_equal(q1, q2)
+    np.testing.assert_array_equal(m1, m2)
