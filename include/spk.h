@@ -194,17 +194,21 @@ int spk_dev_gather(int device, void* dst, const void* src, int64_t row_bytes, in
 
 /* ---- Learned orientation proposer (no reference counterpart) --------------
  * The reference searches every orientation (spectral_packer/packer.py:487-505,
- * fft3.cu:1549-1632).  A 3-D CNN (262,760 parameters, bf16 weights and
- * activations, fp32 accumulation) maps the item's occupancy at pitch 2
- * (32^3 cells) and the tray's occupancy and below-surface fraction on a
- * 64x32x32 grid to 24 orientation logits; the exact search judges its top-k
- * proposals.  Feature buffers, targets and masks are DEVICE pointers (bf16
- * bits as uint16); logits, parameters and gradients are host buffers.  One
- * proposer is used by one thread at a time. */
+ * fft3.cu:1549-1632).  Here a 3-D CNN orientation scorer (260,696 parameters,
+ * bf16 weights and activations, fp32 accumulation) rates each orientation of
+ * an item against the tray: an item branch on the oriented item's occupancy
+ * at pitch 2 (32^3 cells), a tray branch on the tray's occupancy and
+ * below-surface fraction on a 64x32x32 grid, and a head on both; the 24
+ * orientation logits of a record form one softmax, and the exact search
+ * judges the top-k.  Feature buffers, targets and masks are DEVICE pointers
+ * (bf16 bits as uint16); logits, parameters and gradients are host buffers.
+ * A batch of B records holds B*24 oriented item grids (record b, orientation
+ * o at row b*24 + o) and B tray grids.  One proposer is used by one thread at
+ * a time. */
 typedef struct spk_nn spk_nn;
-#define SPK_NN_ITEM_FEATURES 32768    /* [32][32][32][1] */
-#define SPK_NN_TRAY_FEATURES 131072   /* [64][32][32][2] */
-#define SPK_NN_LOGITS 24
+#define SPK_NN_ITEM_FEATURES 32768    /* [32][32][32][1] per oriented item */
+#define SPK_NN_TRAY_FEATURES 131072   /* [64][32][32][2] per tray */
+#define SPK_NN_LOGITS 24              /* orientations per record */
 
 int spk_nn_create(int device, int max_batch, spk_nn** out);
 int spk_nn_destroy(spk_nn* nn);
@@ -213,7 +217,7 @@ int spk_nn_num_params(spk_nn* nn, int64_t* n);
  * setting them clears the optimizer state. */
 int spk_nn_set_params(spk_nn* nn, const float* params, int64_t n);
 int spk_nn_get_params(spk_nn* nn, float* params, int64_t n);
-/* logits [batch][24] of batch (item, tray) feature pairs */
+/* logits [batch][24] */
 int spk_nn_forward(spk_nn* nn, const uint16_t* item_feat, const uint16_t* tray_feat, int batch,
                    float* logits);
 /* soft-target cross entropy over the orientations whose mask bit is set:
@@ -225,9 +229,10 @@ int spk_nn_gradient(spk_nn* nn, const uint16_t* item_feat, const uint16_t* tray_
 int spk_nn_train_step(spk_nn* nn, const uint16_t* item_feat, const uint16_t* tray_feat,
                       const float* target, const int32_t* mask, int batch, float lr,
                       float* loss);
-/* features of a host item (base orientation) into a device buffer */
+/* features of a host item in one orientation (NULL = as given) into a
+ * device buffer of SPK_NN_ITEM_FEATURES bf16 */
 int spk_nn_item_features(spk_nn* nn, const int32_t* item, int ix, int iy, int iz,
-                         uint16_t* out);
+                         const spk_orient* orient, uint16_t* out);
 /* features of the context's cached tray into a device buffer */
 int spk_tray_features(spk_ctx* ctx, uint16_t* out);
 

@@ -1,10 +1,15 @@
 """CPU oracle of the learned orientation proposer.  TEST INFRASTRUCTURE ONLY.
 
 The reference has no learned component (it searches every orientation,
-spectral_packer/packer.py:487-505); this is the restatement of the network the
-HIP library trains and runs (psacking_amd/csrc/spk_nn.hpp, spk_nn_host.hpp),
-used by tests to pin its features, forward pass, loss and gradients.
+spectral_packer/packer.py:487-505); this is the restatement of the
+orientation scorer the HIP library trains and runs (psacking_amd/csrc/
+spk_nn.hpp, spk_nn_host.hpp), used by tests to pin its features, forward
+pass, loss and gradients.
 
+* A record is one tray and the 24 orientations of one item: the item branch
+  runs on each oriented item, the tray branch once, the head on rows
+  [item_o | tray], channel 0 of its output is orientation o's logit, and the
+  24 logits form one softmax.
 * ``emulate=True`` rounds to bfloat16 (round to nearest even) exactly where the
   device stores bf16: input features, weights, every activation, the logit
   gradient and every activation gradient; products accumulate in float64
@@ -46,11 +51,21 @@ def from_bits(h) -> np.ndarray:
 # ---------------------------------------------------------------------------
 # features
 # ---------------------------------------------------------------------------
-def item_features(item: np.ndarray) -> np.ndarray:
-    """[32,32,32] occupancy of the item at pitch 2: cell (a,b,c) = (# voxels > 0
-    among x in {2a,2a+1}, y in {2b,2b+1}, z in {2c,2c+1}) / 8; voxels beyond
-    64 per axis are outside the grid."""
-    occ = (np.asarray(item) > 0)
+def oriented(item: np.ndarray, perm, flip) -> np.ndarray:
+    """The item seen through an index map: output axis a runs along base axis
+    perm[a], reversed when flip[a]."""
+    g = np.transpose(np.asarray(item), perm)
+    for a in range(3):
+        if flip[a]:
+            g = np.flip(g, axis=a)
+    return np.ascontiguousarray(g)
+
+
+def item_features(item: np.ndarray, perm=(0, 1, 2), flip=(0, 0, 0)) -> np.ndarray:
+    """[32,32,32] occupancy of the (oriented) item at pitch 2: cell (a,b,c) =
+    (# voxels > 0 among x in {2a,2a+1}, y in {2b,2b+1}, z in {2c,2c+1}) / 8;
+    voxels beyond 64 per axis are outside the grid."""
+    occ = (oriented(item, perm, flip) > 0)
     g = np.zeros((2 * ITEM_CELLS,) * 3, dtype=np.int32)
     sx, sy, sz = (min(s, 2 * ITEM_CELLS) for s in occ.shape)
     g[:sx, :sy, :sz] = occ[:sx, :sy, :sz]
@@ -143,7 +158,7 @@ LAYERS: List[Layer] = [
     conv("I3", (8, 8, 8), 32, 48),
     conv("T1", (64, 32, 32), 2, 16), conv("T2", (32, 16, 16), 16, 32),
     conv("T3", (16, 8, 8), 32, 48),
-    dense("F1", 1152, 128), dense("F2", 128, N_LOGITS, relu=False),
+    dense("F1", 1152, 128), dense("F2", 128, 8, relu=False),
 ]
 
 
@@ -260,10 +275,12 @@ def _act(v, emulate):
 
 
 def forward(params, item_feat, tray_feat, emulate=True):
-    """item_feat [B,32,32,32], tray_feat [B,64,32,32,2] -> logits [B,24], cache."""
+    """item_feat [B,24,32,32,32] (oriented items), tray_feat [B,64,32,32,2]
+    -> logits [B,24], cache."""
     L = {l.name: l for l in LAYERS}
-    B = item_feat.shape[0]
-    xi = np.asarray(item_feat, np.float64).reshape(B, 32, 32, 32, 1)
+    B = tray_feat.shape[0]
+    R = B * N_LOGITS
+    xi = np.asarray(item_feat, np.float64).reshape(R, 32, 32, 32, 1)
     xt = np.asarray(tray_feat, np.float64).reshape(B, 64, 32, 32, 2)
     if emulate:
         xi, xt = _act(xi, True), _act(xt, True)
@@ -275,28 +292,27 @@ def forward(params, item_feat, tray_feat, emulate=True):
         cols = im2col(X, l)
         pre = cols @ W.T + b
         y = np.maximum(pre, 0.0) if l.relu else pre
-        out = y.reshape((B,) + l.o + (l.cout,)) if l.ks == 3 else y
-        return out
+        return y.reshape((X.shape[0],) + l.o + (l.cout,)) if l.ks == 3 else y
 
     a = xi
     for n in ("I1", "I2", "I3"):
         a = _act(run(n, a), emulate)
         cache[n] = a
-    pi = _act(pool2(a), emulate).reshape(B, -1)
+    pi = _act(pool2(a), emulate).reshape(R, -1)
     a = xt
     for n in ("T1", "T2", "T3"):
         a = _act(run(n, a), emulate)
         cache[n] = a
     pt = _act(pool2(a), emulate).reshape(B, -1)
-    concat = np.concatenate([pi, pt], axis=1)
+    concat = np.concatenate([pi, np.repeat(pt, N_LOGITS, axis=0)], axis=1)
     cache["C"] = concat
     h1 = _act(run("F1", concat), emulate)
     cache["F1"] = h1
-    logits = run("F2", h1)
+    out = run("F2", h1)
     if emulate:
-        logits = logits.astype(np.float32).astype(np.float64)
-    cache["F2"] = logits
-    return logits, cache
+        out = out.astype(np.float32).astype(np.float64)
+    cache["F2"] = out
+    return out[:, 0].reshape(B, N_LOGITS), cache
 
 
 def loss_and_dlogits(logits, target, mask):
@@ -324,7 +340,9 @@ def backward(params, cache, dlogits, emulate=True):
     sl = param_slices()
     g = np.zeros(N_PARAMS, np.float64)
     B = dlogits.shape[0]
-    dz = _act(dlogits, emulate)
+    R = B * N_LOGITS
+    dz = np.zeros((R, 8))
+    dz[:, 0] = _act(dlogits, emulate).reshape(R)
 
     def wgrad(name, X, dY):
         l = L[name]
@@ -334,28 +352,30 @@ def backward(params, cache, dlogits, emulate=True):
         g[w] = (dY2.T @ cols).reshape(-1)
         g[b] = dY2.sum(axis=0)
 
-    def dgrad(name, dY, mask_act):
+    def dgrad(name, dY, mask_act, rows):
         l = L[name]
         W, _ = _layer_params(params, l, emulate)
         dcols = dY.reshape(-1, l.cout) @ W
-        dX = col2im(dcols, l, B)
+        dX = col2im(dcols, l, rows)
         if mask_act is not None:
             dX = np.where(mask_act > 0, dX, 0.0)
         return _act(dX, emulate)
 
     wgrad("F2", cache["F1"], dz)
-    dh1 = dgrad("F2", dz, cache["F1"])
+    dh1 = dgrad("F2", dz, cache["F1"], R)
     wgrad("F1", cache["C"], dh1)
-    dc = dgrad("F1", dh1, None)
-    for branch, off, n3 in (("I", 0, 384), ("T", 384, 768)):
+    dc = dgrad("F1", dh1, None, R)
+    d_item = dc[:, :384]
+    d_tray = _act(dc[:, 384:].reshape(B, N_LOGITS, 768).sum(axis=1), emulate)
+    for branch, dp, rows in (("I", d_item, R), ("T", d_tray, B)):
         a3 = cache[branch + "3"]
-        d3 = unpool2(dc[:, off:off + n3].reshape((B,) + tuple(s // 2 for s in a3.shape[1:4]) +
-                                                 (a3.shape[4],)), a3.shape)
+        d3 = unpool2(dp.reshape((rows,) + tuple(s // 2 for s in a3.shape[1:4]) + (a3.shape[4],)),
+                     a3.shape)
         d3 = _act(np.where(a3 > 0, d3, 0.0), emulate)
         wgrad(branch + "3", cache[branch + "2"], d3)
-        d2 = dgrad(branch + "3", d3, cache[branch + "2"])
+        d2 = dgrad(branch + "3", d3, cache[branch + "2"], rows)
         wgrad(branch + "2", cache[branch + "1"], d2)
-        d1 = dgrad(branch + "2", d2, cache[branch + "1"])
+        d1 = dgrad(branch + "2", d2, cache[branch + "1"], rows)
         wgrad(branch + "1", cache[branch + "0"], d1)
     return g
 

@@ -70,7 +70,7 @@ def load() -> C.CDLL:
     lib.spk_nn_forward.argtypes = [vp, vp, vp, i32, fp]
     lib.spk_nn_gradient.argtypes = [vp, vp, vp, vp, vp, i32, fp, fp]
     lib.spk_nn_train_step.argtypes = [vp, vp, vp, vp, vp, i32, f32, fp]
-    lib.spk_nn_item_features.argtypes = [vp, ip, i32, i32, i32, vp]
+    lib.spk_nn_item_features.argtypes = [vp, ip, i32, i32, i32, C.POINTER(SpkOrient), vp]
     lib.spk_tray_features.argtypes = [vp, vp]
     lib.spk_pack_records.argtypes = [vp, C.POINTER(SpkPackArgs), vp, dp]
     lib.spk_pack_learned.argtypes = [vp, vp, C.POINTER(SpkPackArgs), i32, f32, ip, ip]

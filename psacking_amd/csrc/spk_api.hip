@@ -15,6 +15,7 @@
 
 #include "../../include/spk.h"
 #include "spk_kernels.hpp"
+#include "spk_nn.hpp"
 
 using namespace spk;
 
@@ -166,7 +167,8 @@ struct spk_ctx {
   DevBuf<int> heights;                   // column heights [nx][ny]
   DevBuf<int> map_all;                   // orientation index of each od_all entry
   DevBuf<unsigned short> tfeat;          // tray features [64][32][32][2] (bf16)
-  DevBuf<unsigned short> ifeat, ipool;   // item features [n][32^3], pooled item rows [n][384]
+  DevBuf<unsigned short> ifeat, ipool;   // oriented item features, pooled rows [n][24][384]
+  DevBuf<spknn::ItemMap> imaps;          // orientation maps of the items [n][24]
   DevBuf<double> rscores;                // per-orientation best scores [n][n_orient]
   DevBuf<OrientDesc> od_prop;            // descriptors of the proposed orientations
   DevBuf<OrientDesc> od_rest;            // descriptors of the other orientations searched
@@ -1136,24 +1138,37 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   const std::vector<ItemPlan>& plans = P.items;
   const std::vector<OrientDesc>& all = P.all;
   hipStream_t s = c->stream;
+  // pooled features of every item in each of its orientations: 24 rows per
+  // item (rows past n_orient repeat the base item; their logits are unused)
   timed_group(c, "proposer_items", [&]() {
     nn_cast(nn, s);
+    std::vector<spknn::ItemMap> maps((size_t)n_items * kNnLogits);
+    for (int t = 0; t < n_items; ++t) {
+      const HostItem& h = bases[order ? order[t] : t];
+      for (int o = 0; o < kNnLogits; ++o) {
+        spknn::ItemMap& mp = maps[(size_t)t * kNnLogits + o];
+        mp.bx = h.dims[0]; mp.by = h.dims[1]; mp.bz = h.dims[2];
+        const spk_orient* so = (orients && o < n_orient) ? &orients[o] : nullptr;
+        for (int a = 0; a < 3; ++a) {
+          mp.perm[a] = so ? so->perm[a] : a;
+          mp.flip[a] = so ? (so->flip[a] ? 1 : 0) : 0;
+        }
+      }
+    }
+    c->imaps.ensure(maps.size());
+    upload(c, c->imaps.p, maps.data(), maps.size() * sizeof(spknn::ItemMap));
     const int bmax = nn->max_batch;
-    c->ifeat.ensure((size_t)std::min(n_items, bmax) * kNnItemFeat);
-    c->ipool.ensure((size_t)n_items * kNnItemPooled);
+    c->ifeat.ensure((size_t)std::min(n_items, bmax) * kNnLogits * kNnItemFeat);
+    c->ipool.ensure((size_t)n_items * kNnLogits * kNnItemPooled);
     for (int b0 = 0; b0 < n_items; b0 += bmax) {
       const int nb = std::min(bmax, n_items - b0);
       for (int j = 0; j < nb; ++j) {
         const int it = order ? order[b0 + j] : b0 + j;
-        const HostItem& h = bases[it];
-        nn_item_features(s, c->items.p + off[it], h.dims[0], h.dims[1], h.dims[2],
-                         c->ifeat.p + (size_t)j * kNnItemFeat);
+        nn_item_features(s, c->items.p + off[it], c->imaps.p + (size_t)(b0 + j) * kNnLogits,
+                         kNnLogits, c->ifeat.p + (size_t)j * kNnLogits * kNnItemFeat);
       }
-      nn_item_branch(nn, s, c->ifeat.p, nb, nn->concat.p);
-      HIP_OK(hipMemcpy2DAsync(c->ipool.p + (size_t)b0 * kNnItemPooled,
-                              kNnItemPooled * sizeof(bf16_t), nn->concat.p,
-                              kNnConcat * sizeof(bf16_t), kNnItemPooled * sizeof(bf16_t), nb,
-                              hipMemcpyDeviceToDevice, s));
+      nn_item_branch(nn, s, c->ifeat.p, nb * kNnLogits,
+                     c->ipool.p + (size_t)b0 * kNnLogits * kNnItemPooled);
     }
   });
   c->tfeat.ensure(kNnTrayFeat);
@@ -1173,9 +1188,8 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   auto enqueue_head = [&](int t) {
     timed_group(c, "proposer", [&]() {
       nn_tray_features(s, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, c->tfeat.p);
-      HIP_OK(hipMemcpyAsync(nn->concat.p, c->ipool.p + (size_t)t * kNnItemPooled,
-                            kNnItemPooled * sizeof(bf16_t), hipMemcpyDeviceToDevice, s));
-      nn_tray_head(nn, s, c->tfeat.p, 1);
+      nn_tray_branch(nn, s, c->tfeat.p, 1);
+      nn_head(nn, s, c->ipool.p + (size_t)t * kNnLogits * kNnItemPooled, 1);
     });
     download(c, logits, nn->logits.p, sizeof(logits));
   };
@@ -1466,6 +1480,7 @@ int spk_ctx_destroy(spk_ctx* c) {
     c->od_all.release(); c->pout.release(); c->ctl.release(); c->zero.release();
     c->heights.release(); c->map_all.release(); c->tfeat.release(); c->ifeat.release();
     c->ipool.release(); c->rscores.release(); c->od_prop.release(); c->od_rest.release();
+    c->imaps.release();
     c->fstate.release(); c->ctl_on.release();
     if (c->stream) hipStreamDestroy(c->stream);
   }
@@ -2122,17 +2137,32 @@ int spk_nn_train_step(spk_nn* nn, const uint16_t* item_feat, const uint16_t* tra
   });
 }
 
-int spk_nn_item_features(spk_nn* nn, const int32_t* item, int ix, int iy, int iz, uint16_t* out) {
+int spk_nn_item_features(spk_nn* nn, const int32_t* item, int ix, int iy, int iz,
+                         const spk_orient* orient, uint16_t* out) {
   return nn_guarded(nn, [&]() {
     check_dims(ix, iy, iz);
     if (!item || !out) throw SpkError{SPK_EINVAL, "null argument"};
+    spknn::ItemMap mp{};
+    mp.bx = ix; mp.by = iy; mp.bz = iz;
+    bool seen[3] = {false, false, false};
+    for (int a = 0; a < 3; ++a) {
+      mp.perm[a] = orient ? orient->perm[a] : a;
+      mp.flip[a] = orient ? (orient->flip[a] ? 1 : 0) : 0;
+      if (mp.perm[a] < 0 || mp.perm[a] > 2 || seen[mp.perm[a]])
+        throw SpkError{SPK_EINVAL, "orientation perm must be a permutation of 0,1,2"};
+      seen[mp.perm[a]] = true;
+    }
     DevBuf<int> d;
+    DevBuf<spknn::ItemMap> dm;
     const size_t n = (size_t)ix * iy * iz;
     d.ensure(n);
+    dm.ensure(1);
     HIP_OK(hipMemcpyAsync(d.p, item, n * sizeof(int), hipMemcpyHostToDevice, nn->stream));
-    nn_item_features(nn->stream, d.p, ix, iy, iz, out);
+    HIP_OK(hipMemcpyAsync(dm.p, &mp, sizeof(mp), hipMemcpyHostToDevice, nn->stream));
+    nn_item_features(nn->stream, d.p, dm.p, 1, out);
     HIP_OK(hipStreamSynchronize(nn->stream));
     d.release();
+    dm.release();
     return SPK_OK;
   });
 }

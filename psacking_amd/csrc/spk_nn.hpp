@@ -337,35 +337,73 @@ __global__ void k_pool_bwd(const bf16_t* __restrict__ g, int ld, int off, const 
   dX[i] = f2bf(v);
 }
 
-// Soft-target cross entropy over the valid orientations of each row
-// (mask bit o set).  dz = (softmax - q) / B on valid entries, 0 elsewhere.
-__global__ void k_loss(const float* __restrict__ z, const float* __restrict__ q,
+// Soft-target cross entropy per record over its valid orientations (mask
+// bit o set).  The logit of orientation o of record b is z[(b*L + o)*zs]
+// (channel 0 of the scorer's output row); dz = (softmax - q) / B on valid
+// entries and 0 elsewhere, written to the scorer's output gradient (all zs
+// channels of the row, the unused ones 0); logits also go compactly to
+// zc[b*L + o].
+__global__ void k_loss(const float* __restrict__ z, int zs, const float* __restrict__ q,
                        const int32_t* __restrict__ mask, int B, int L, bf16_t* __restrict__ dz,
-                       float* __restrict__ loss_rows) {
+                       float* __restrict__ loss_rows, float* __restrict__ zc) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= B) return;
   const unsigned mk = (unsigned)mask[b];
-  const float* zr = z + (size_t)b * L;
+  const float* zr = z + (size_t)b * L * zs;
   const float* qr = q + (size_t)b * L;
   float mx = -INFINITY;
   for (int o = 0; o < L; ++o)
-    if (mk >> o & 1u) mx = fmaxf(mx, zr[o]);
+    if (mk >> o & 1u) mx = fmaxf(mx, zr[o * zs]);
   float se = 0.f;
   for (int o = 0; o < L; ++o)
-    if (mk >> o & 1u) se += expf(zr[o] - mx);
+    if (mk >> o & 1u) se += expf(zr[o * zs] - mx);
   const float lse = mx + logf(se);
   float loss = 0.f;
   const float inv = 1.f / (float)B;
   for (int o = 0; o < L; ++o) {
     float g = 0.f;
+    const float zo = zr[o * zs];
     if (mk >> o & 1u) {
-      const float p = expf(zr[o] - lse);
-      loss -= qr[o] * (zr[o] - lse);
+      const float p = expf(zo - lse);
+      loss -= qr[o] * (zo - lse);
       g = (p - qr[o]) * inv;
     }
-    dz[(size_t)b * L + o] = f2bf(g);
+    bf16_t* d = dz + ((size_t)b * L + o) * zs;
+    d[0] = f2bf(g);
+    for (int c = 1; c < zs; ++c) d[c] = 0;
+    zc[(size_t)b * L + o] = zo;
   }
   loss_rows[b] = mk ? loss : 0.f;
+}
+
+// compact logits: zc[r] = z[r*zs] (channel 0 of each scorer row)
+__global__ void k_logits_compact(const float* __restrict__ z, int zs, int rows,
+                                 float* __restrict__ zc) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < rows) zc[r] = z[(size_t)r * zs];
+}
+
+// scorer input rows: concat[r] = [item row r (ni) | tray row r / R (nt)]
+__global__ void k_concat_rows(const bf16_t* __restrict__ item, int ni, const bf16_t* __restrict__ tray,
+                              int nt, int rows, int R, bf16_t* __restrict__ out) {
+  const int w = ni + nt;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)rows * w) return;
+  const int r = (int)(i / w), c = (int)(i - (size_t)r * w);
+  out[i] = c < ni ? item[(size_t)r * ni + c] : tray[(size_t)(r / R) * nt + (c - ni)];
+}
+
+// gradient of the tray rows: sum of the tray part of the R scorer rows of
+// each record, g[b][j] = sum_o gc[(b*R + o)*w + ni + j]
+__global__ void k_tray_gsum(const bf16_t* __restrict__ gc, int ni, int nt, int B, int R,
+                            bf16_t* __restrict__ g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B * nt) return;
+  const int b = i / nt, j = i - b * nt;
+  const int w = ni + nt;
+  float s = 0.f;
+  for (int o = 0; o < R; ++o) s += bf2f(gc[((size_t)b * R + o) * w + ni + j]);
+  g[i] = f2bf(s);
 }
 
 // Adam (bias-corrected) on the fp32 master parameters
@@ -422,20 +460,40 @@ __global__ void k_gather_rows(T* __restrict__ dst, const T* __restrict__ src,
 constexpr int kItemCells = 32;                 // item grid 32^3, pitch 2 voxels
 constexpr int kTray0 = 64, kTray1 = 32, kTray2 = 32;
 
-// item occupancy: cell (a,b,c) = (# of voxels > 0 among x in {2a,2a+1},
-// y in {2b,2b+1}, z in {2c,2c+1} inside the item) / 8
-__global__ void k_item_feat(const int* __restrict__ item, int ix, int iy, int iz,
+// An orientation of a base item as an index map: oriented axis a runs along
+// base axis perm[a], reversed when flip[a] != 0.
+struct ItemMap {
+  int bx, by, bz;
+  int perm[3], flip[3];
+};
+
+// occupancy of the oriented item: cell (a,b,c) = (# of oriented voxels > 0
+// among x in {2a,2a+1}, y in {2b,2b+1}, z in {2c,2c+1}) / 8.  Grid (128, n):
+// item map i -> out + i * 32^3
+__global__ void k_item_feat(const int* __restrict__ item, const ItemMap* __restrict__ maps,
                             bf16_t* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= kItemCells * kItemCells * kItemCells) return;
+  const ItemMap mp = maps[blockIdx.y];
+  const int bd[3] = {mp.bx, mp.by, mp.bz};
+  int ext[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) ext[a] = bd[mp.perm[a]];
   const int c = i % kItemCells, b = (i / kItemCells) % kItemCells, a = i / (kItemCells * kItemCells);
   int cnt = 0;
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    const int x = 2 * a + (e >> 2), y = 2 * b + ((e >> 1) & 1), z = 2 * c + (e & 1);
-    if (x < ix && y < iy && z < iz && item[((size_t)x * iy + y) * iz + z] > 0) ++cnt;
+    const int w[3] = {2 * a + (e >> 2), 2 * b + ((e >> 1) & 1), 2 * c + (e & 1)};
+    if (w[0] >= ext[0] || w[1] >= ext[1] || w[2] >= ext[2]) continue;
+    int u[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int ax = mp.perm[d];
+      u[ax] = mp.flip[d] ? bd[ax] - 1 - w[d] : w[d];
+    }
+    if (item[((size_t)u[0] * mp.by + u[1]) * mp.bz + u[2]] > 0) ++cnt;
   }
-  out[i] = f2bf((float)cnt * 0.125f);
+  out[(size_t)blockIdx.y * (kItemCells * kItemCells * kItemCells) + i] = f2bf((float)cnt * 0.125f);
 }
 
 // column heights: h[x][y] = 1 + max{z : tray > 0}, 0 for an empty column

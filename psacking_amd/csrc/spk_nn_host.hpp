@@ -1,11 +1,15 @@
 // Host side of the learned orientation proposer (kernels in spk_nn.hpp).
 // Included by spk_api.hip after its error, buffer and launch helpers.
 //
-// Network (262,760 parameters, channels-last, every conv 3x3x3 stride 2 pad 1,
-// ReLU after every layer but the last):
-//   item  [32,32,32,1]  -> I1 [16^3,16] -> I2 [8^3,32] -> I3 [4^3,48] -> avg2 [2^3,48] = 384
-//   tray  [64,32,32,2]  -> T1 [32,16,16,16] -> T2 [16,8,8,32] -> T3 [8,4,4,48] -> avg2 [4,2,2,48] = 768
-//   concat 1152 -> F1 128 -> F2 24 orientation logits
+// An orientation scorer shared by all orientations (260,696 parameters,
+// channels-last, every conv 3x3x3 stride 2 pad 1, ReLU after every layer but
+// the last):
+//   oriented item [32,32,32,1] -> I1 [16^3,16] -> I2 [8^3,32] -> I3 [4^3,48] -> avg2 [2^3,48] = 384
+//   tray [64,32,32,2] -> T1 [32,16,16,16] -> T2 [16,8,8,32] -> T3 [8,4,4,48] -> avg2 [4,2,2,48] = 768
+//   [item 384 | tray 768] -> F1 128 -> F2 8 (channel 0 = the orientation's logit)
+// A record is one tray and the 24 orientations of one item: the item branch
+// runs on the 24 oriented items, the tray branch once, the head on 24 rows,
+// and the 24 logits form one softmax.
 // Parameters are stored layer by layer, W [cout][ks^3*cin] (tap-major,
 // channel-minor columns; tap = (t0*3 + t1)*3 + t2) followed by b [cout].
 #pragma once
@@ -21,8 +25,10 @@ enum { NN_I1, NN_I2, NN_I3, NN_T1, NN_T2, NN_T3, NN_F1, NN_F2, NN_LAYERS };
 constexpr int kNnItemFeat = 32768;     // 32^3 x 1
 constexpr int kNnTrayFeat = 131072;    // 64x32x32 x 2
 constexpr int kNnItemPooled = 384;
+constexpr int kNnTrayPooled = 768;
 constexpr int kNnConcat = 1152;
-constexpr int kNnLogits = 24;
+constexpr int kNnLogits = 24;          // orientations per record
+constexpr int kNnOut = 8;              // scorer outputs per row (channel 0 used)
 
 int round32(int v) { return (v + 31) & ~31; }
 
@@ -65,11 +71,11 @@ size_t wgrad_chunks_max(const ConvDesc& d) {
 struct spk_nn {
   int device = 0;
   hipStream_t stream = nullptr;
-  int max_batch = 0;
+  int max_batch = 0;           // records per batch (24 scorer rows each)
   NnLayer L[NN_LAYERS];
   size_t n_params = 0, n_wbf = 0;
-  DevBuf<float> params, grads, m1, m2, logits, loss_rows, part;
-  DevBuf<bf16_t> wbf, concat, g_concat;
+  DevBuf<float> params, grads, m1, m2, out, logits, loss_rows, part;
+  DevBuf<bf16_t> wbf, ipool, tpool, concat, g_concat, g_tpool;
   DevBuf<bf16_t> act[NN_LAYERS], grad[NN_LAYERS];
   bool dirty = true;   // bf16 copies out of date
   long long step = 0;
@@ -93,7 +99,7 @@ void nn_build(spk_nn* nn, int max_batch) {
   nn->L[NN_T2].d = conv_desc(32, 16, 16, 16, 32);
   nn->L[NN_T3].d = conv_desc(16, 8, 8, 32, 48);
   nn->L[NN_F1].d = dense_desc(kNnConcat, 128);
-  nn->L[NN_F2].d = dense_desc(128, kNnLogits);
+  nn->L[NN_F2].d = dense_desc(128, kNnOut);
   nn->L[NN_F2].relu = 0;
   size_t p = 0, w = 0, part = 0;
   for (auto& l : nn->L) {
@@ -116,14 +122,19 @@ void nn_build(spk_nn* nn, int max_batch) {
   nn->m2.ensure(p);
   nn->wbf.ensure(w);
   nn->part.ensure(part);
-  const size_t B = (size_t)max_batch;
+  const size_t B = (size_t)max_batch, R = B * kNnLogits;
   for (int i = 0; i < NN_LAYERS; ++i) {
-    nn->act[i].ensure(B * nn->L[i].out_size());
-    nn->grad[i].ensure(B * nn->L[i].out_size());
+    const bool tray = i == NN_T1 || i == NN_T2 || i == NN_T3;
+    nn->act[i].ensure((tray ? B : R) * nn->L[i].out_size());
+    nn->grad[i].ensure((tray ? B : R) * nn->L[i].out_size());
   }
-  nn->concat.ensure(B * kNnConcat);
-  nn->g_concat.ensure(B * kNnConcat);
-  nn->logits.ensure(B * kNnLogits);
+  nn->ipool.ensure(R * kNnItemPooled);
+  nn->tpool.ensure(B * kNnTrayPooled);
+  nn->g_tpool.ensure(B * kNnTrayPooled);
+  nn->concat.ensure(R * kNnConcat);
+  nn->g_concat.ensure(R * kNnConcat);
+  nn->out.ensure(R * kNnOut);
+  nn->logits.ensure(R);
   nn->loss_rows.ensure(B);
   HIP_OK(hipMemsetAsync(nn->params.p, 0, p * sizeof(float), nn->stream));
   HIP_OK(hipMemsetAsync(nn->m1.p, 0, p * sizeof(float), nn->stream));
@@ -216,65 +227,83 @@ void nn_wgrad(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* 
             chunks, N, nn->grads.p + l.b_off);
 }
 
-void nn_pool_fwd(hipStream_t s, const NnLayer& l, int B, const bf16_t* X, bf16_t* out, int off) {
+void nn_pool_fwd(hipStream_t s, const NnLayer& l, int B, const bf16_t* X, bf16_t* out, int ld,
+                 int off) {
   const ConvDesc& d = l.d;
   const int n = B * (d.o0 / 2) * (d.o1 / 2) * (d.o2 / 2) * d.cout;
   nn_launch(s, spknn::k_pool_fwd, dim3((n + 255) / 256), dim3(256), 0, X, B, d.o0, d.o1, d.o2,
-            d.cout, out, kNnConcat, off);
+            d.cout, out, ld, off);
 }
 
-void nn_pool_bwd(hipStream_t s, const NnLayer& l, int B, const bf16_t* g, int off,
+void nn_pool_bwd(hipStream_t s, const NnLayer& l, int B, const bf16_t* g, int ld, int off,
                  const bf16_t* X, bf16_t* dX) {
   const ConvDesc& d = l.d;
   const size_t n = (size_t)B * d.o0 * d.o1 * d.o2 * d.cout;
-  nn_launch(s, spknn::k_pool_bwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g, kNnConcat,
-            off, X, B, d.o0, d.o1, d.o2, d.cout, dX);
+  nn_launch(s, spknn::k_pool_bwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g, ld, off, X,
+            B, d.o0, d.o1, d.o2, d.cout, dX);
 }
 
-// item branch: B item feature grids -> pooled features in rows of `out`
-// (row pitch kNnConcat, columns [0, 384))
-void nn_item_branch(spk_nn* nn, hipStream_t s, const bf16_t* item, int B, bf16_t* out) {
-  nn_fwd(nn, s, nn->L[NN_I1], B, item, nn->act[NN_I1].p, nullptr);
-  nn_fwd(nn, s, nn->L[NN_I2], B, nn->act[NN_I1].p, nn->act[NN_I2].p, nullptr);
-  nn_fwd(nn, s, nn->L[NN_I3], B, nn->act[NN_I2].p, nn->act[NN_I3].p, nullptr);
-  nn_pool_fwd(s, nn->L[NN_I3], B, nn->act[NN_I3].p, out, 0);
+// item branch: n oriented item grids -> pooled rows (pitch 384) in out
+void nn_item_branch(spk_nn* nn, hipStream_t s, const bf16_t* item, int n, bf16_t* out) {
+  nn_fwd(nn, s, nn->L[NN_I1], n, item, nn->act[NN_I1].p, nullptr);
+  nn_fwd(nn, s, nn->L[NN_I2], n, nn->act[NN_I1].p, nn->act[NN_I2].p, nullptr);
+  nn_fwd(nn, s, nn->L[NN_I3], n, nn->act[NN_I2].p, nn->act[NN_I3].p, nullptr);
+  nn_pool_fwd(s, nn->L[NN_I3], n, nn->act[NN_I3].p, out, kNnItemPooled, 0);
 }
 
-// tray branch + head on nn->concat whose item columns are filled
-void nn_tray_head(spk_nn* nn, hipStream_t s, const bf16_t* tray, int B) {
+// tray branch: B tray grids -> pooled rows (pitch 768) in nn->tpool
+void nn_tray_branch(spk_nn* nn, hipStream_t s, const bf16_t* tray, int B) {
   nn_fwd(nn, s, nn->L[NN_T1], B, tray, nn->act[NN_T1].p, nullptr);
   nn_fwd(nn, s, nn->L[NN_T2], B, nn->act[NN_T1].p, nn->act[NN_T2].p, nullptr);
   nn_fwd(nn, s, nn->L[NN_T3], B, nn->act[NN_T2].p, nn->act[NN_T3].p, nullptr);
-  nn_pool_fwd(s, nn->L[NN_T3], B, nn->act[NN_T3].p, nn->concat.p, kNnItemPooled);
-  nn_fwd(nn, s, nn->L[NN_F1], B, nn->concat.p, nn->act[NN_F1].p, nullptr);
-  nn_fwd(nn, s, nn->L[NN_F2], B, nn->act[NN_F1].p, nn->act[NN_F2].p, nn->logits.p);
+  nn_pool_fwd(s, nn->L[NN_T3], B, nn->act[NN_T3].p, nn->tpool.p, kNnTrayPooled, 0);
 }
 
+// head on B records: rows [item row r | tray row r/24] -> F1 -> F2 -> logits
+void nn_head(spk_nn* nn, hipStream_t s, const bf16_t* ipool, int B) {
+  const int rows = B * kNnLogits;
+  const size_t n = (size_t)rows * kNnConcat;
+  nn_launch(s, spknn::k_concat_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ipool,
+            kNnItemPooled, (const bf16_t*)nn->tpool.p, kNnTrayPooled, rows, kNnLogits,
+            nn->concat.p);
+  nn_fwd(nn, s, nn->L[NN_F1], rows, nn->concat.p, nn->act[NN_F1].p, nullptr);
+  nn_fwd(nn, s, nn->L[NN_F2], rows, nn->act[NN_F1].p, nn->act[NN_F2].p, nn->out.p);
+  nn_launch(s, spknn::k_logits_compact, dim3((rows + 255) / 256), dim3(256), 0,
+            (const float*)nn->out.p, kNnOut, rows, nn->logits.p);
+}
+
+// item [B*24 oriented item grids], tray [B grids] -> nn->logits [B][24]
 void nn_forward(spk_nn* nn, hipStream_t s, const bf16_t* item, const bf16_t* tray, int B) {
   nn_cast(nn, s);
-  nn_item_branch(nn, s, item, B, nn->concat.p);
-  nn_tray_head(nn, s, tray, B);
+  nn_item_branch(nn, s, item, B * kNnLogits, nn->ipool.p);
+  nn_tray_branch(nn, s, tray, B);
+  nn_head(nn, s, nn->ipool.p, B);
 }
 
 // loss, then every parameter gradient into nn->grads
 void nn_backward(spk_nn* nn, hipStream_t s, const bf16_t* item, const bf16_t* tray,
                  const float* target, const int32_t* mask, int B) {
-  nn_launch(s, spknn::k_loss, dim3((B + 63) / 64), dim3(64), 0, (const float*)nn->logits.p, target,
-            mask, B, kNnLogits, nn->grad[NN_F2].p, nn->loss_rows.p);
+  const int R = B * kNnLogits;
+  nn_launch(s, spknn::k_loss, dim3((B + 63) / 64), dim3(64), 0, (const float*)nn->out.p, kNnOut,
+            target, mask, B, kNnLogits, nn->grad[NN_F2].p, nn->loss_rows.p, nn->logits.p);
   const NnLayer* L = nn->L;
-  nn_wgrad(nn, s, L[NN_F2], B, nn->act[NN_F1].p, nn->grad[NN_F2].p);
-  nn_bwd_data(nn, s, L[NN_F2], B, nn->grad[NN_F2].p, nn->act[NN_F1].p, nn->grad[NN_F1].p);
-  nn_wgrad(nn, s, L[NN_F1], B, nn->concat.p, nn->grad[NN_F1].p);
-  nn_bwd_data(nn, s, L[NN_F1], B, nn->grad[NN_F1].p, nullptr, nn->g_concat.p);
-  nn_pool_bwd(s, L[NN_I3], B, nn->g_concat.p, 0, nn->act[NN_I3].p, nn->grad[NN_I3].p);
-  nn_pool_bwd(s, L[NN_T3], B, nn->g_concat.p, kNnItemPooled, nn->act[NN_T3].p, nn->grad[NN_T3].p);
-  // item branch
-  nn_wgrad(nn, s, L[NN_I3], B, nn->act[NN_I2].p, nn->grad[NN_I3].p);
-  nn_bwd_data(nn, s, L[NN_I3], B, nn->grad[NN_I3].p, nn->act[NN_I2].p, nn->grad[NN_I2].p);
-  nn_wgrad(nn, s, L[NN_I2], B, nn->act[NN_I1].p, nn->grad[NN_I2].p);
-  nn_bwd_data(nn, s, L[NN_I2], B, nn->grad[NN_I2].p, nn->act[NN_I1].p, nn->grad[NN_I1].p);
-  nn_wgrad(nn, s, L[NN_I1], B, item, nn->grad[NN_I1].p);
-  // tray branch
+  nn_wgrad(nn, s, L[NN_F2], R, nn->act[NN_F1].p, nn->grad[NN_F2].p);
+  nn_bwd_data(nn, s, L[NN_F2], R, nn->grad[NN_F2].p, nn->act[NN_F1].p, nn->grad[NN_F1].p);
+  nn_wgrad(nn, s, L[NN_F1], R, nn->concat.p, nn->grad[NN_F1].p);
+  nn_bwd_data(nn, s, L[NN_F1], R, nn->grad[NN_F1].p, nullptr, nn->g_concat.p);
+  // item branch (R rows)
+  nn_pool_bwd(s, L[NN_I3], R, nn->g_concat.p, kNnConcat, 0, nn->act[NN_I3].p, nn->grad[NN_I3].p);
+  nn_wgrad(nn, s, L[NN_I3], R, nn->act[NN_I2].p, nn->grad[NN_I3].p);
+  nn_bwd_data(nn, s, L[NN_I3], R, nn->grad[NN_I3].p, nn->act[NN_I2].p, nn->grad[NN_I2].p);
+  nn_wgrad(nn, s, L[NN_I2], R, nn->act[NN_I1].p, nn->grad[NN_I2].p);
+  nn_bwd_data(nn, s, L[NN_I2], R, nn->grad[NN_I2].p, nn->act[NN_I1].p, nn->grad[NN_I1].p);
+  nn_wgrad(nn, s, L[NN_I1], R, item, nn->grad[NN_I1].p);
+  // tray branch (B rows): the tray part of the 24 rows of a record summed
+  nn_launch(s, spknn::k_tray_gsum, dim3((B * kNnTrayPooled + 255) / 256), dim3(256), 0,
+            (const bf16_t*)nn->g_concat.p, kNnItemPooled, kNnTrayPooled, B, kNnLogits,
+            nn->g_tpool.p);
+  nn_pool_bwd(s, L[NN_T3], B, nn->g_tpool.p, kNnTrayPooled, 0, nn->act[NN_T3].p,
+              nn->grad[NN_T3].p);
   nn_wgrad(nn, s, L[NN_T3], B, nn->act[NN_T2].p, nn->grad[NN_T3].p);
   nn_bwd_data(nn, s, L[NN_T3], B, nn->grad[NN_T3].p, nn->act[NN_T2].p, nn->grad[NN_T2].p);
   nn_wgrad(nn, s, L[NN_T2], B, nn->act[NN_T1].p, nn->grad[NN_T2].p);
@@ -291,9 +320,10 @@ float nn_loss_sum(spk_nn* nn, hipStream_t s, int B) {
   return (float)(t / B);
 }
 
-// item feature grids of one (base-orientation) item already on the device
-void nn_item_features(hipStream_t s, const int* item, int ix, int iy, int iz, bf16_t* out) {
-  nn_launch(s, spknn::k_item_feat, dim3(kNnItemFeat / 256), dim3(256), 0, item, ix, iy, iz, out);
+// feature grids of n oriented items (maps, device) of one item on the device
+void nn_item_features(hipStream_t s, const int* item, const spknn::ItemMap* maps, int n,
+                      bf16_t* out) {
+  nn_launch(s, spknn::k_item_feat, dim3(kNnItemFeat / 256, n), dim3(256), 0, item, maps, out);
 }
 
 void nn_tray_features(hipStream_t s, const int* tray, int nx, int ny, int nz, int* heights,

@@ -26,7 +26,7 @@ from . import _lib
 ITEM_FEATURES = 32768
 TRAY_FEATURES = 131072
 N_LOGITS = 24
-N_PARAMS = 262760
+N_PARAMS = 260696
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "proposer.npy")
 
 
@@ -74,13 +74,15 @@ class DeviceArray:
         per = self.nbytes // self.shape[0]
         return C.c_void_p(self.ptr.value + int(i) * per)
 
-    def gather_from(self, src: "DeviceArray", idx) -> None:
-        """self rows = src rows idx (len(idx) == self.shape[0])"""
+    def gather_from(self, src: "DeviceArray", idx, row0: int = 0) -> None:
+        """rows row0 .. row0+len(idx)-1 of self = rows idx of src"""
         idx = np.ascontiguousarray(idx, dtype=np.int64)
         per = src.nbytes // src.shape[0]
-        if per != self.nbytes // self.shape[0] or len(idx) > self.shape[0]:
-            raise ValueError("row size mismatch")
-        _check(self._lib.spk_dev_gather(self.device, self.ptr, src.ptr, per, src.shape[0],
+        if per != self.nbytes // self.shape[0] or row0 < 0 or row0 + len(idx) > self.shape[0]:
+            raise ValueError("row size or count mismatch")
+        if len(idx) == 0:
+            return
+        _check(self._lib.spk_dev_gather(self.device, self.row(row0), src.ptr, per, src.shape[0],
                                         idx.ctypes.data_as(C.POINTER(C.c_int64)), len(idx)))
 
     def close(self) -> None:
@@ -102,7 +104,7 @@ def _ptr(t) -> C.c_void_p:
 def init_params(seed: int = 0) -> np.ndarray:
     """He-uniform weights, zero biases (float32, N_PARAMS)."""
     fans = [27, 432, 864, 54, 432, 864, 1152, 128]
-    couts = [16, 32, 48, 16, 32, 48, 128, 24]
+    couts = [16, 32, 48, 16, 32, 48, 128, 8]
     rng = np.random.default_rng(seed)
     out = []
     for k, n in zip(fans, couts):
@@ -165,14 +167,15 @@ class Proposer:
 
     # ---- batches of device features -------------------------------------
     def forward(self, item_feat: DeviceArray, tray_feat: DeviceArray, batch: Optional[int] = None) -> np.ndarray:
-        B = int(item_feat.shape[0] if batch is None else batch)
+        """logits [B, 24]; item_feat holds B*24 oriented item rows, tray_feat B rows."""
+        B = int(tray_feat.shape[0] if batch is None else batch)
         out = np.empty((B, N_LOGITS), np.float32)
         _check(self._lib.spk_nn_forward(self._h, _ptr(item_feat), _ptr(tray_feat), B,
                                         out.ctypes.data_as(C.POINTER(C.c_float))))
         return out
 
     def gradient(self, item_feat, tray_feat, target, mask) -> Tuple[float, np.ndarray]:
-        B = int(item_feat.shape[0])
+        B = int(tray_feat.shape[0])
         g = np.empty(self.n_params, np.float32)
         loss = C.c_float()
         _check(self._lib.spk_nn_gradient(self._h, _ptr(item_feat), _ptr(tray_feat), _ptr(target),
@@ -182,21 +185,38 @@ class Proposer:
 
     def train_step(self, item_feat, tray_feat, target, mask, lr: float,
                    batch: Optional[int] = None) -> float:
-        B = int(item_feat.shape[0] if batch is None else batch)
+        B = int(tray_feat.shape[0] if batch is None else batch)
         loss = C.c_float()
         _check(self._lib.spk_nn_train_step(self._h, _ptr(item_feat), _ptr(tray_feat),
                                            _ptr(target), _ptr(mask), B, float(lr),
                                            C.byref(loss)))
         return float(loss.value)
 
-    def item_features(self, item: np.ndarray, out: Optional[DeviceArray] = None, row: int = 0):
-        """bf16 bits [32768] on the device (into row `row` of `out` if given)."""
+    def item_features(self, item: np.ndarray, out: Optional[DeviceArray] = None, row: int = 0,
+                      perm=None, flip=None):
+        """bf16 bits [32768] of the item in one orientation (index map perm,
+        flip; default as given) on the device (into row `row` of `out`)."""
         it = np.ascontiguousarray(item, dtype=np.int32)
         if out is None:
             out = DeviceArray((1, ITEM_FEATURES), np.uint16, self.device)
+        o = None
+        if perm is not None:
+            o = _lib.SpkOrient()
+            o.perm[:] = [int(v) for v in perm]
+            o.flip[:] = [int(v) for v in (flip if flip is not None else (0, 0, 0))]
         _check(self._lib.spk_nn_item_features(self._h, it.ctypes.data_as(C.POINTER(C.c_int32)),
-                                              *it.shape, out.row(row)))
+                                              *it.shape, C.byref(o) if o is not None else None,
+                                              out.row(row)))
         return out
+
+    def item_features_all(self, item: np.ndarray, perms, flips, out: DeviceArray, row0: int) -> None:
+        """rows row0 .. row0+23 of `out` = the item in orientations 0..23 (a
+        missing orientation repeats the base item)."""
+        for o in range(N_LOGITS):
+            if o < len(perms):
+                self.item_features(item, out, row0 + o, perms[o], flips[o])
+            else:
+                self.item_features(item, out, row0 + o)
 
 
 def tray_features(session, out: Optional[DeviceArray] = None, row: int = 0):

@@ -12,6 +12,8 @@
    orientation's best score (spk_pack_records); a run stops when a chunk of
    50 items places fewer than 5.  Items with no feasible orientation are not
    records.
+   Each record keeps the tray features in a dense device store; the item
+   enters as its 24 oriented feature grids.
 3. Training: soft-target cross entropy (proposer.soft_targets), Adam, batch
    256, learning rate 1e-3 decayed linearly to 1e-4 over the last 40 % of the
    steps; every step is the library's own forward, backward and Adam kernels.
@@ -62,7 +64,7 @@ def log(*a):
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--records", type=int, default=100000)
-    ap.add_argument("--pool", type=int, default=6000)
+    ap.add_argument("--pool", type=int, default=4000)
     ap.add_argument("--per-run", type=int, default=600)
     ap.add_argument("--epochs", type=int, default=6)
     ap.add_argument("--batch", type=int, default=256)
@@ -73,6 +75,8 @@ def main(argv=None) -> int:
     ap.add_argument("--report", default="")
     ap.add_argument("--max-record-seconds", type=float, default=480.0,
                     help="stop generating records after this long")
+    ap.add_argument("--max-train-seconds", type=float, default=300.0,
+                    help="stop training after the epoch that passes this long")
     args = ap.parse_args(argv)
 
     t0 = time.time()
@@ -88,19 +92,21 @@ def main(argv=None) -> int:
 
     perms, flips = orientation_maps(24)
     nn = PR.Proposer(0, args.batch, PR.init_params(args.seed))
-    ifeat = PR.DeviceArray((len(pool), PR.ITEM_FEATURES), np.uint16)
+    # every pool item in its 24 orientations: rows 24*i .. 24*i+23
+    ifeat = PR.DeviceArray((len(pool) * PR.N_LOGITS, PR.ITEM_FEATURES), np.uint16)
     for i, g in enumerate(pool):
-        nn.item_features(g, out=ifeat, row=i)
-    cap = int(args.records * 1.6) + args.per_run
+        nn.item_features_all(g, perms, flips, ifeat, i * PR.N_LOGITS)
+    log(f"oriented item features: {time.time() - t0:.1f} s")
+    cap = args.records + 50
     tfeat = PR.DeviceArray((cap, PR.TRAY_FEATURES), np.uint16)
+    scratch = PR.DeviceArray((50, PR.TRAY_FEATURES), np.uint16)
     rng = np.random.default_rng(args.seed)
     rec_item, rec_score, rec_run = [], [], []
     row = 0
     run = 0
     t1 = time.time()
     last = t1
-    while (sum(len(r) for r in rec_item) < args.records and row + args.per_run <= cap and
-           time.time() - t1 < args.max_record_seconds):
+    while (row < args.records and time.time() - t1 < args.max_record_seconds):
         pick = rng.integers(0, len(pool), args.per_run)
         pick = pick[np.argsort(-vols[pick], kind="stable")]
         s = P.Session(0)
@@ -110,26 +116,27 @@ def main(argv=None) -> int:
         for c0 in range(0, len(pick), 50):
             chunk = pick[c0:c0 + 50]
             res, _, sc = PR.pack_records(s, [pool[i] for i in chunk], np.arange(len(chunk)),
-                                         perms, flips, "cube", id_offset=placed, feats=tfeat,
-                                         row0=row)
+                                         perms, flips, "cube", id_offset=placed, feats=scratch)
             n_ok = sum(1 for r in res if r["found"])
             placed += n_ok
-            for j in range(len(chunk)):
-                if np.isfinite(sc[j]).any():
-                    items_run.append(int(chunk[j]))
-                    scores_run.append(sc[j])
-                    rows_run.append(row + j)
-            row += len(chunk)
-            if n_ok < 5:
+            keep = [j for j in range(len(chunk)) if np.isfinite(sc[j]).any()]
+            keep = keep[:cap - row]
+            tfeat.gather_from(scratch, keep, row0=row)
+            for j in keep:
+                items_run.append(int(chunk[j]))
+                scores_run.append(sc[j])
+                rows_run.append(row)
+                row += 1
+            if n_ok < 5 or row >= cap:
                 break
-        rec_item.append(np.array(rows_run))
-        rec_score.append((np.array(items_run), np.array(scores_run)))
+        rec_item.append(np.array(rows_run, dtype=np.int64))
+        rec_score.append((np.array(items_run, dtype=np.int64),
+                          np.array(scores_run, dtype=np.float64).reshape(-1, len(perms))))
         rec_run.append(run)
         run += 1
         if time.time() - last > 20:
             last = time.time()
-            log(f"records {sum(len(r) for r in rec_item)} from {run} runs, {row} rows, "
-                f"{time.time() - t1:.0f} s")
+            log(f"records {row} from {run} runs, {time.time() - t1:.0f} s")
     n_runs = len(rec_item)
     n_val = max(1, n_runs // 20)
     def cat(runs):
@@ -148,7 +155,11 @@ def main(argv=None) -> int:
     dq_tr = PR.DeviceArray.from_numpy(q_tr)
     dm_tr = PR.DeviceArray.from_numpy(m_tr)
     B = args.batch
-    bi = PR.DeviceArray((B, PR.ITEM_FEATURES), np.uint16)
+    bi = PR.DeviceArray((B * PR.N_LOGITS, PR.ITEM_FEATURES), np.uint16)
+    orow = np.arange(PR.N_LOGITS)
+
+    def item_rows(items):
+        return (np.asarray(items)[:, None] * PR.N_LOGITS + orow[None, :]).reshape(-1)
     bt = PR.DeviceArray((B, PR.TRAY_FEATURES), np.uint16)
     bq = PR.DeviceArray((B, PR.N_LOGITS), np.float32)
     bm = PR.DeviceArray((B,), np.int32)
@@ -158,7 +169,7 @@ def main(argv=None) -> int:
         for b0 in range(0, len(va_rows), B):
             sl = slice(b0, min(b0 + B, len(va_rows)))
             k = sl.stop - sl.start
-            bi.gather_from(ifeat, np.pad(va_items[sl], (0, B - k)))
+            bi.gather_from(ifeat, item_rows(np.pad(va_items[sl], (0, B - k))))
             bt.gather_from(tfeat, np.pad(va_rows[sl], (0, B - k)))
             z = nn.forward(bi, bt, batch=k)
             s = va_scores[sl]
@@ -189,7 +200,7 @@ def main(argv=None) -> int:
             idx = perm[k * B:(k + 1) * B]
             frac = step / max(1, total)
             lr = args.lr if frac < 0.6 else args.lr * (1.0 - (frac - 0.6) / 0.4 * 0.9)
-            bi.gather_from(ifeat, tr_items[idx])
+            bi.gather_from(ifeat, item_rows(tr_items[idx]))
             bt.gather_from(tfeat, tr_rows[idx])
             bq.gather_from(dq_tr, idx)
             bm.gather_from(dm_tr, idx)
@@ -203,6 +214,9 @@ def main(argv=None) -> int:
                      "top1": a1, "top4": a4})
         log(f"epoch {ep + 1}: train loss {np.mean(run_loss):.4f}, held-out loss {vl:.4f}, "
             f"top-1 {a1:.3f}, top-4 {a4:.3f}, {time.time() - t2:.0f} s")
+        if time.time() - t2 > args.max_train_seconds:
+            log("training time limit reached")
+            break
     t_train = time.time() - t2
     nn.save(args.out)
     log(f"weights -> {args.out}")

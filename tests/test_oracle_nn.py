@@ -15,7 +15,7 @@ from oracle import spk_nn_oracle as N
 
 def test_param_count():
     from psacking_amd import proposer
-    assert N.N_PARAMS == 262760 == proposer.N_PARAMS
+    assert N.N_PARAMS == 260696 == proposer.N_PARAMS
     assert proposer.init_params(3).size == N.N_PARAMS
 
 
@@ -56,7 +56,7 @@ def _case(seed, B=2):
     rng = np.random.default_rng(seed)
     params = N.init_params(seed).astype(np.float64)
     params += rng.normal(0, 0.01, params.size)   # non-zero biases
-    item = (rng.random((B, 32, 32, 32)) < 0.3) * rng.integers(1, 9, (B, 32, 32, 32)) / 8.0
+    item = (rng.random((B, 24, 32, 32, 32)) < 0.3) * rng.integers(1, 9, (B, 24, 32, 32, 32)) / 8.0
     tray = rng.random((B, 64, 32, 32, 2))
     scores = rng.uniform(0, 1e6, (B, 24))
     scores[rng.random((B, 24)) < 0.3] = np.inf
@@ -65,13 +65,15 @@ def _case(seed, B=2):
 
 
 def test_backward_matches_finite_differences():
-    params, item, tray, q, mask = _case(4)
+    params, item, tray, q, mask = _case(4, B=2)
     loss, g, _ = N.loss_grad(params, item, tray, q, mask, emulate=False)
     sl = N.param_slices()
     rng = np.random.default_rng(5)
     idx = []
     for name, (w, b) in sl.items():
-        idx += list(rng.integers(w.start, w.stop, 4)) + [b.start + int(rng.integers(0, b.stop - b.start))]
+        # F2's channel 0 is the logit; channels 1-7 carry no gradient
+        hi = b.start + 1 if name == "F2" else b.stop
+        idx += list(rng.integers(w.start, w.stop, 3)) + [b.start + int(rng.integers(0, hi - b.start))]
     eps = 1e-6
     for i in idx:
         p1, p2 = params.copy(), params.copy()
@@ -160,3 +162,14 @@ def test_product_soft_targets_match_oracle():
     q2, m2 = N.soft_targets(s)
     np.testing.assert_array_equal(q1, q2)
     np.testing.assert_array_equal(m1, m2)
+
+
+def test_oriented_items_follow_the_orientation_order():
+    from oracle import spk_oracle as O
+    from psacking_amd.rotations import orientation_maps
+    rng = np.random.default_rng(10)
+    item = rng.integers(0, 3, (5, 7, 4)).astype(np.int32)
+    perms, flips = orientation_maps(24)
+    ref = O.get_orientations(item, 24)
+    for o in range(24):
+        np.testing.assert_array_equal(N.oriented(item, perms[o], flips[o]), np.asarray(ref[o]))
