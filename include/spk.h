@@ -173,7 +173,14 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  2 every item.  The pass tests every orientation for a
  *                  zero-collision offset at about a third of the scored
  *                  search's cost; an item with none is recorded as not found
- *                  without the scored search.  Results are unchanged. */
+ *                  without the scored search.  Results are unchanged.
+ *   "certificate"  1: before an item's search, compare the tray field's
+ *                  maximum with the item's inscribed-ball bound (the largest
+ *                  distance from an item voxel to a voxel outside it); when
+ *                  the field (distance to every occupied voxel, sources
+ *                  "nonzero") is smaller everywhere, no orientation fits and
+ *                  the item is skipped.  Exact; pays off when free space is
+ *                  scattered in small gaps.  Default 0. */
 int spk_set_option(spk_ctx* ctx, const char* name, double value);
 
 /* Plan actually used for the cached tray: FFT sizes and digit base. */

@@ -176,6 +176,11 @@ struct spk_ctx {
   DevBuf<int> fstate;                    // residual bits, unused, per-orientation flags
   DevBuf<int> ctl_on;                    // control words that enable the pass outside the loop
   int feas_mode = 1;                     // 0 off, 1 after an item that placed nothing, 2 always
+  // free-space certificate
+  bool phi_internal = false;             // phi is this library's distance field of the tray
+  bool certificate = false;              // option "certificate"
+  DevBuf<FieldStats> theta;              // per item: inscribed-ball bound (mx)
+  DevBuf<int> cgrid, cdist, ctmp;        // padded item complement and its distance field
   // timing
   bool timing = false;
   std::vector<Timed> pending;
@@ -419,6 +424,8 @@ void build_spectra(spk_ctx* c, const int* ctl = nullptr) {
 // item placed nothing or the loop was handed back to the host.
 void refresh_field_and_spectra(spk_ctx* c, const int* dist_host, const int* ctl = nullptr) {
   const size_t V = (size_t)c->nx * c->ny * c->nz;
+  if (dist_host) c->phi_internal = false;
+  else if (!ctl) c->phi_internal = true;   // (a gated refresh keeps the flag as it was)
   if (dist_host) {
     upload(c, c->phi.p, dist_host, V * sizeof(int));
     launch(c, "stats_init", k_stats_init, dim3(1), dim3(64), 0, c->stats.p, (const int*)nullptr);
@@ -929,6 +936,42 @@ PackPlan plan_pack(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
   return P;
 }
 
+// Inscribed-ball bounds of the items, in processing order, for the
+// free-space certificate (k_feas_begin): theta[t].mx = largest distance from
+// a voxel of item t to a voxel outside it, in the tray field's metric.  Null
+// when the certificate is off (option "certificate") or does not apply: the
+// tray field must be this library's distance to every occupied voxel
+// (sources "nonzero", not caller-supplied).
+const FieldStats* item_bounds(spk_ctx* c, const std::vector<HostItem>& bases,
+                              const std::vector<int>& off, const int32_t* order) {
+  if (!c->certificate || !c->phi_internal || c->sources != SPK_SOURCES_NONZERO) return nullptr;
+  const int n = (int)bases.size();
+  c->theta.ensure(std::max(n, 1));
+  size_t maxv = 1;
+  for (const auto& b : bases)
+    maxv = std::max(maxv, (size_t)(b.dims[0] + 2) * (b.dims[1] + 2) * (b.dims[2] + 2));
+  c->cgrid.ensure(maxv);
+  c->cdist.ensure(maxv);
+  c->ctmp.ensure(maxv);
+  for (int t = 0; t < n; ++t) {
+    const int it = order ? order[t] : t;
+    const HostItem& h = bases[it];
+    const int X = h.dims[0] + 2, Y = h.dims[1] + 2, Z = h.dims[2] + 2;
+    if ((long long)Y * Z > 32768 || X > 1024) {
+      // too large for the distance kernels: no bound (mx = INT_MIN never certifies)
+      launch(c, "stats_init", k_stats_init, dim3(1), dim3(64), 0, c->theta.p + t,
+             (const int*)nullptr);
+      continue;
+    }
+    const int nb = std::min(1024, (X * Y * Z + 255) / 256);
+    launch(c, "complement", k_complement, dim3(nb), dim3(256), 0, (const int*)c->items.p + off[it],
+           h.dims[0], h.dims[1], h.dims[2], c->cgrid.p);
+    run_distance(c, c->cgrid.p, c->cdist.p, c->ctmp.p, c->theta.p + t, X, Y, Z, c->metric,
+                 SPK_SOURCES_NONZERO, nullptr);
+  }
+  return c->theta.p;
+}
+
 // One item of the device-driven loop: batched search over m descriptors (od),
 // k_decide, optional per-orientation score record, placement of the accepted
 // winner and refresh of the distance field and spectra.
@@ -940,15 +983,20 @@ PackPlan plan_pack(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
 // placement and refresh are skipped.
 void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
                   int t, int32_t id_offset, int halt_not_found, double* rec_row,
-                  const int* rec_map, int feas_mode = -1) {
+                  const int* rec_map, int feas_mode = -1,
+                  const FieldStats* item_bound = nullptr) {
   if (feas_mode < 0) feas_mode = c->feas_mode;
-  if (feas_mode && !c->split) {
-    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, m, feas_mode,
-           c->ctl.p);
+  const bool feas = feas_mode && !c->split;
+  if (feas || item_bound) {
+    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, m,
+           feas ? feas_mode : 0, c->ctl.p, (const FieldStats*)c->stats.p, item_bound, c->pout.p,
+           t);
     enqueue_item_spectra(c, od, m, max_ex, c->ctl.p);
-    enqueue_feas(c, od, m, max_ex, c->ctl.p);
-    launch(c, "feas_decide", k_feas_decide, dim3(1), dim3(64), 0, (const int*)c->fstate.p, m,
-           c->pout.p, t, c->resid_limit, c->ctl.p);
+    if (feas) {
+      enqueue_feas(c, od, m, max_ex, c->ctl.p);
+      launch(c, "feas_decide", k_feas_decide, dim3(1), dim3(64), 0, (const int*)c->fstate.p, m,
+             c->pout.p, t, c->resid_limit, c->ctl.p);
+    }
     enqueue_scored(c, od, m, max_ex, max_e, penalty, c->ctl.p, nullptr, nullptr, nullptr);
   } else {
     enqueue_search(c, od, m, max_ex, max_e, penalty, c->ctl.p, nullptr, nullptr, nullptr);
@@ -1028,6 +1076,7 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
   PackPlan P = plan_pack(c, bases, off, order, orients, n_orient);
   const std::vector<ItemPlan>& plans = P.items;
   const std::vector<OrientDesc>& all = P.all;
+  const FieldStats* bounds = item_bounds(c, bases, off, order);
   std::vector<std::pair<int, std::vector<double>>> host_rows;   // records of host-path items
   if (rec_scores) {
     c->rscores.ensure((size_t)n_items * n_orient);
@@ -1050,7 +1099,7 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
       const OrientDesc* od = c->od_all.p + pl.od_off;
       enqueue_step(c, od, pl.m, pl.max_ex, pl.max_e, penalty, t, id_offset, 0,
                    rec_scores ? c->rscores.p + (size_t)t * n_orient : nullptr,
-                   c->map_all.p + pl.od_off);
+                   c->map_all.p + pl.od_off, -1, bounds ? bounds + t : nullptr);
     }
     int ctl[CTL_WORDS];
     std::vector<PackOut> po(n_items - t0);
@@ -1138,6 +1187,14 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   const std::vector<ItemPlan>& plans = P.items;
   const std::vector<OrientDesc>& all = P.all;
   hipStream_t s = c->stream;
+  // free-space certificate, checked on the host with the tray field's
+  // statistics read at every synchronisation
+  const FieldStats* bounds = item_bounds(c, bases, off, order);
+  std::vector<FieldStats> hb(bounds ? n_items : 0);
+  FieldStats tst{};
+  if (bounds) download(c, hb.data(), bounds, (size_t)n_items * sizeof(FieldStats));
+  download(c, &tst, c->stats.p, sizeof(tst));
+  sync(c);
   // pooled features of every item in each of its orientations: 24 rows per
   // item (rows past n_orient repeat the base item; their logits are unused)
   timed_group(c, "proposer_items", [&]() {
@@ -1216,7 +1273,8 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     int max_ex, max_e;
     const OrientDesc* od = gather(pl, list, c->od_rest.p, &max_ex, &max_e);
     const int k = (int)list.size();
-    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, k, 2, (int*)nullptr);
+    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, k, 2, (int*)nullptr,
+           (const FieldStats*)nullptr, (const FieldStats*)nullptr, (PackOut*)nullptr, 0);
     enqueue_item_spectra(c, od, k, max_ex, nullptr);
     enqueue_feas(c, od, k, max_ex, c->ctl_on.p);
     download(c, fsv, c->fstate.p, (kFeasHeader + k) * sizeof(int));
@@ -1257,6 +1315,12 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     }
     const bool have_t = have;   // logits of item t computed after the previous step
     have = false;
+    if (bounds && tst.mx < hb[t].mx) {
+      ++cnt[4];
+      to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, &results[t]);
+      prev_placed = false;
+      continue;
+    }
     std::vector<int> cand(pl.m);
     for (int i = 0; i < pl.m; ++i) cand[i] = i;
     bool screened = false;
@@ -1317,6 +1381,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     PackOut po;
     download(c, ctl, c->ctl.p, sizeof(ctl));
     download(c, &po, c->pout.p + t, sizeof(po));
+    download(c, &tst, c->stats.p, sizeof(tst));
     sync(c);
     if (c->timing) c->flush_timing();
     if (ctl[CTL_HALT] == 0) {
@@ -1348,6 +1413,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
                      nullptr, 0);
         download(c, ctl, c->ctl.p, sizeof(ctl));
         download(c, &po, c->pout.p + t, sizeof(po));
+        download(c, &tst, c->stats.p, sizeof(tst));
         sync(c);
         if (c->timing) c->flush_timing();
         if (ctl[CTL_HALT] == 0) {
@@ -1377,6 +1443,8 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     placed += place_exact(c, eb, pl, all, resid, id_offset + placed + 1, &results[t]);
     prev_placed = eb.fit >= 0;
     reset_ctl();
+    download(c, &tst, c->stats.p, sizeof(tst));
+    sync(c);
   }
   if (counts)
     for (int i = 0; i < 5; ++i) counts[i] = cnt[i];
@@ -1482,6 +1550,7 @@ int spk_ctx_destroy(spk_ctx* c) {
     c->ipool.release(); c->rscores.release(); c->od_prop.release(); c->od_rest.release();
     c->imaps.release();
     c->fstate.release(); c->ctl_on.release();
+    c->theta.release(); c->cgrid.release(); c->cdist.release(); c->ctmp.release();
     if (c->stream) hipStreamDestroy(c->stream);
   }
   delete c;
@@ -1924,6 +1993,10 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->plan_mode = v;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "certificate") {
+    c->certificate = value != 0.0;
     return SPK_OK;
   }
   if (std::string(name) == "feas") {

@@ -737,15 +737,53 @@ constexpr int kFeasHeader = 2;
 
 // start of an item in the greedy loop: the scored search runs unless the
 // feasibility pass decides otherwise; mode 0 never runs the pass, 1 runs it
-// after an item that placed nothing, 2 always
-__global__ void k_feas_begin(int* __restrict__ fs, int m, int mode, int* __restrict__ ctl) {
+// after an item that placed nothing, 2 always.
+//
+// Free-space certificate (item_st != nullptr, the tray field being the
+// distance to the nearest occupied voxel): item_st->mx = rint(sqrt(e)) for
+// the largest e such that a discrete ball {|y - c|^2 <= e - 1} (L1 ball for
+// the L1 metric) lies inside the item.  A collision-free placement puts that
+// ball on free tray voxels, so the tray field at its centre is >= item_st->mx;
+// when the tray field's maximum is smaller, no orientation is feasible (the
+// ball is invariant under the cube's symmetries) and the item is recorded as
+// not found with no further work.
+__global__ void k_feas_begin(int* __restrict__ fs, int m, int mode, int* __restrict__ ctl,
+                             const FieldStats* __restrict__ tray_st,
+                             const FieldStats* __restrict__ item_st, PackOut* __restrict__ out,
+                             int t) {
   if (ctl && ctl[CTL_HALT]) return;
   const int i = threadIdx.x;
+  const bool cert = ctl && item_st && tray_st->mx < item_st->mx;
   if (i == 0 && ctl) {
-    ctl[CTL_SKIP_SEARCH] = 0;
-    ctl[CTL_FEAS] = mode == 2 ? 1 : (mode == 1 ? ctl[CTL_SKIP_REFRESH] : 0);
+    if (cert) {
+      PackOut r{};
+      r.so.found = 0;
+      r.so.idx = INT_MAX;
+      out[t] = r;
+      ctl[CTL_SKIP_SEARCH] = 1;
+      ctl[CTL_SKIP_REFRESH] = 1;
+      ctl[CTL_FEAS] = 0;
+    } else {
+      ctl[CTL_SKIP_SEARCH] = 0;
+      ctl[CTL_FEAS] = mode == 2 ? 1 : (mode == 1 ? ctl[CTL_SKIP_REFRESH] : 0);
+    }
   }
   for (int j = i; j < kFeasHeader + m; j += blockDim.x) fs[j] = 0;
+}
+
+// padded complement of an item: out (ix+2, iy+2, iz+2) = 1 where the item has
+// no voxel > 0 (its one-voxel border included), 0 on the item's voxels
+__global__ void k_complement(const int* __restrict__ item, int ix, int iy, int iz,
+                             int* __restrict__ out) {
+  const int X = ix + 2, Y = iy + 2, Z = iz + 2;
+  const int n = X * Y * Z;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int z = e % Z, y = (e / Z) % Y, x = e / (Y * Z);
+    int v = 1;
+    if (x >= 1 && y >= 1 && z >= 1 && x <= ix && y <= iy && z <= iz)
+      v = item[((size_t)(x - 1) * iy + (y - 1)) * iz + (z - 1)] > 0 ? 0 : 1;
+    out[e] = v;
+  }
 }
 
 // pass A: grid (ceil(pz/T), py, n_chunks); chunks of orientation pairs

@@ -85,7 +85,24 @@ class DeviceArray:
         _check(self._lib.spk_dev_gather(self.device, self.row(row0), src.ptr, per, src.shape[0],
                                         idx.ctypes.data_as(C.POINTER(C.c_int64)), len(idx)))
 
+    def view(self, r0: int, n: int) -> "DeviceArray":
+        """rows r0 .. r0+n-1 as an array that does not own its memory (keep
+        this array alive while the view is used)"""
+        if r0 < 0 or n <= 0 or r0 + n > self.shape[0]:
+            raise ValueError("rows out of range")
+        v = object.__new__(DeviceArray)
+        v.shape = (n,) + self.shape[1:]
+        v.dtype = self.dtype
+        v.device = self.device
+        v.nbytes = n * (self.nbytes // self.shape[0])
+        v._lib = self._lib
+        v.ptr = self.row(r0)
+        v._owner = self
+        return v
+
     def close(self) -> None:
+        if getattr(self, "_owner", None) is not None:
+            return
         if getattr(self, "ptr", None) is not None and self.ptr.value:
             self._lib.spk_dev_free(self.device, self.ptr)
             self.ptr = None

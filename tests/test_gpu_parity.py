@@ -435,3 +435,36 @@ def test_feasibility_pass_keeps_results(P, feas):
             assert (tuple(r["position"]), r["orientation"], r["score"]) == \
                 (e.position, e.orientation_index, e.score)
     np.testing.assert_array_equal(s.tray(), exp.tray)
+
+
+def test_free_space_certificate_keeps_results(P):
+    """Certificate on: a tray with scattered occupied voxels where large
+    solid items fit nowhere (they are skipped without a search) and small
+    ones fit; every outcome and the final tray equal the oracle's."""
+    from psacking_amd.rotations import orientation_maps
+    rng = np.random.default_rng(60)
+    shape = (30, 28, 24)
+    tray = (rng.random(shape) < 0.12).astype(np.int32)
+    items = [np.ones((8, 7, 9), np.int32), np.ones((2, 1, 2), np.int32),
+             np.ones((9, 9, 8), np.int32), np.ones((1, 2, 1), np.int32),
+             np.ones((2, 2, 1), np.int32)]
+    order = list(range(len(items)))
+    perms, flips = orientation_maps(24)
+    for cert in (0, 1):
+        s = P.Session(0)
+        s.set_option("certificate", cert)
+        s.set_tray(tray, "edt", "nonzero", 0)
+        say("certificate", cert)
+        res = s.pack(items, np.array(order, np.int32), perms, flips, "cube", False,
+                     int(tray.max()))
+        exp = O.pack_voxels(items, shape, num_orientations=24, order=order, metric="edt",
+                            sources="nonzero", initial_tray=tray)
+        by_item = {p.item_index: p for p in exp.placements}
+        assert not by_item[0].success and by_item[1].success
+        for t, r in enumerate(res):
+            e = by_item[order[t]]
+            assert r["found"] == e.success
+            if e.success:
+                assert (tuple(r["position"]), r["orientation"], r["score"]) == \
+                    (e.position, e.orientation_index, e.score)
+        np.testing.assert_array_equal(s.tray(), exp.tray)

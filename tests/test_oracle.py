@@ -255,3 +255,35 @@ def test_reference_cpu_path_as_written_matches_oracle():
         assert got == (exp.position, exp.found, exp.score)
         np.testing.assert_array_equal(R.corr3_as_written(tray, O.pad_to(item, N)),
                                       O.corr_exact(tray, O.pad_to(item, N)))
+
+
+@pytest.mark.parametrize("metric", ["edt", "l1"])
+def test_free_space_certificate_is_sound(metric):
+    """The device skips an item when the tray field's maximum (distance to the
+    nearest occupied voxel) is below the item's inscribed-ball bound (largest
+    distance from an item voxel to a voxel outside it).  Check on random trays
+    and items that the rule never rejects an item with a feasible orientation,
+    and that it does reject some."""
+    rng = np.random.default_rng(77)
+    certified = 0
+    for case in range(60):
+        shape = tuple(int(v) for v in rng.integers(10, 16, 3))
+        tray = (rng.random(shape) < rng.uniform(0.05, 0.4)).astype(np.int32)
+        dims = tuple(int(v) for v in rng.integers(5, 10, 3))
+        item = np.ones(dims, np.int32)
+        # a few holes on the faces keep the items irregular
+        item[0][rng.random(item[0].shape) < 0.3] = 0
+        comp = np.ones(tuple(d + 2 for d in dims), np.int32)
+        comp[1:-1, 1:-1, 1:-1] = (item == 0)
+        theta = int(O.distance_field(comp, metric, "nonzero").max())
+        phi_max = int(O.distance_field(tray, metric, "nonzero").max())
+        if phi_max >= theta:
+            continue
+        certified += 1
+        for g in O.get_orientations(item, 24):
+            g = O.as_grid(g)
+            if any(g.shape[a] > shape[a] for a in range(3)):
+                continue
+            assert not O.search(g, tray, O.distance_field(tray, metric, "nonzero"),
+                                penalty="cube").found, (case, metric)
+    assert certified >= 3
