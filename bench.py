@@ -54,15 +54,18 @@ def device_sync():
 def kernel_bytes(name: str, Ph: int, V: int, m: float) -> float:
     """Algorithmic HBM bytes of one launch, on the tray size itself.
 
-    Per orientation search the path must read the two cached complex64 tray
+    Scored search, per orientation: read the two cached complex64 tray
     spectra (16*Ph), write the two products after the x pass (16*Ph) and read
     them back in the yz pass (16*Ph), Ph = Nx*Ny*(Nz//2+1).  The x-pass kernel
-    does the first two for a batch of m orientations, the yz-pass kernel the
-    third.
+    does the first two for a batch of m orientations (the spectra once), the
+    yz-pass kernel the third.
+    Feasibility pass, per orientation pair (m = pairs): the x pass reads the
+    tray spectrum once (16*Ph) and writes one complex64 product per pair
+    (16*Ph); the yz pass reads it back.
     """
-    if name == "score_yz":
+    if name in ("score_yz", "feas_yz"):
         return 16.0 * Ph * m
-    if name == "corr_x":
+    if name in ("corr_x", "feas_x"):
         return 16.0 * Ph + 16.0 * Ph * m
     raise KeyError(name)
 
@@ -242,7 +245,9 @@ def main(argv=None) -> int:
     def finish(name, s, dt, res, t_setup, searched):
         t = s.tray()
         kt = {n: s.timing_get(n) for n in KERNELS}
+        units = {n: s.timing_units(n) for n in ("corr_x", "score_yz", "feas_x", "feas_yz")}
         runs[name] = {"dt": dt, "res": res, "density": float((t > 0).sum()) / V, "kt": kt,
+                      "units": units,
                       "setup": t_setup, "searched": searched,
                       "placed": sum(1 for r in res if r["found"]),
                       "fallbacks": sum(1 for r in res if r["path"] != 0),
@@ -318,11 +323,21 @@ def main(argv=None) -> int:
     else:
         used = "plain" if "plain" in runs else "learned"
     r = runs[used]
-    dom = max(("score_yz", "corr_x"), key=lambda n: r["kt"][n][0])
+    dom = max(("score_yz", "corr_x", "feas_x", "feas_yz"), key=lambda n: r["kt"][n][0])
     ms_dom, n_dom = r["kt"][dom]
     avg_s = (ms_dom / max(1, n_dom)) / 1e3
-    m_avg = r["searched"] / max(1, n_dom)
+    m_avg = r["units"][dom] / max(1, n_dom)
     achieved = kernel_bytes(dom, Ph, V, m_avg) / avg_s / 1e9 if avg_s > 0 else 0.0
+    # measured HBM bytes per launch of this kernel from the committed counter
+    # runs (FETCH_SIZE/WRITE_SIZE scaled by a copy kernel), scaled to this
+    # run's units per launch
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{w.name}_{Nx}x{Ny}x{Nz}_{used}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            tk = json.load(fh)["kernels"].get(dom)
+        if tk and tk["units_per_launch"] > 0:
+            traffic = tk["hbm_bytes_per_launch"] * m_avg / tk["units_per_launch"]
 
     cpu = None
     if args.cpu_budget > 0:
@@ -376,9 +391,11 @@ def main(argv=None) -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "algorithmic_bytes": kernel_bytes(dom, Ph, V, m_avg),
             "avg_launch_us": avg_s * 1e6,
-            "orients_per_launch": m_avg,
+            "units_per_launch": m_avg,
+            "unit_of_work": "orientation pair" if dom.startswith("feas") else "orientation",
         },
         "cpu_baseline": cpu,
     }

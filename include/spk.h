@@ -159,6 +159,10 @@ int spk_save_vox(const int32_t* grid, int nx, int ny, int nz, const char* path);
 /* Per-kernel HIP-event timing on the context's stream (off by default). */
 int spk_timing_enable(spk_ctx* ctx, int on);
 int spk_timing_get(spk_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
+/* orientations handed to a search kernel while timing was on (orientation
+ * pairs for the feasibility pass), to turn launch times into per-orientation
+ * rates; launches that the device loop gates off still count */
+int spk_timing_units(spk_ctx* ctx, const char* kernel, int64_t* units);
 int spk_timing_reset(spk_ctx* ctx);
 /* Tuning/diagnostic options of a context (no reference counterpart):
  *   "resid_limit"  largest accepted distance of an FFT output from its

@@ -443,6 +443,11 @@ class Session {
     check(spk_timing_get(ctx_, name.c_str(), &ms, &n));
     return py::make_tuple(ms, n);
   }
+  int64_t timing_units(const std::string& name) {
+    int64_t u = 0;
+    check(spk_timing_units(ctx_, name.c_str(), &u));
+    return u;
+  }
   void timing_reset() { check(spk_timing_reset(ctx_)); }
   py::tuple plan() {
     int px, py_, pz, b;
@@ -563,6 +568,7 @@ PYBIND11_MODULE(_core, m) {
       .def("set_option", &Session::set_option, py::arg("name"), py::arg("value"))
       .def("timing", &Session::timing, py::arg("on"))
       .def("timing_get", &Session::timing_get, py::arg("name"))
+      .def("timing_units", &Session::timing_units, py::arg("name"))
       .def("timing_reset", &Session::timing_reset)
       .def("plan", &Session::plan)
       .def("handle", [](const Session& s) { return (uintptr_t)s.ctx(); },

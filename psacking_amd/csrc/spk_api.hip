@@ -186,6 +186,7 @@ struct spk_ctx {
   std::vector<Timed> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, long long>> acc;
+  std::map<std::string, long long> units;   // orientations (pairs) handed to each kernel
   // host-side copy of item buffer bookkeeping
   std::vector<int> h_items;
 
@@ -673,6 +674,10 @@ void enqueue_feas(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const int
   int chunk = 0;
   const int nchunk = corr_chunks(c, np, &chunk);
   const float scale = (float)(1.0 / ((double)c->px.n * py * pz));
+  if (c->timing) {
+    c->units["feas_x"] += np;
+    c->units["feas_yz"] += np;
+  }
   launch(c, "feas_x", k_feas_x, dim3(tiles, py, nchunk), dim3(kPencilThreads), pencil_shmem(c),
          (const float2*)c->g1.p, (const float2*)c->ispec.p, od, c->y1.p, m, chunk, max_ex, c->nx,
          c->px, py, pz, c->T, scale, tw, ctl);
@@ -693,6 +698,10 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
   int chunk = 0;
   const int nchunk = corr_chunks(c, m, &chunk);
   const float scale = (float)(1.0 / ((double)c->px.n * py * pz));
+  if (c->timing) {
+    c->units["corr_x"] += m;
+    c->units[c->split ? "rows_score" : "score_yz"] += m;
+  }
   launch(c, "corr_x", k_corr_x, dim3(tiles, py, nchunk), dim3(kPencilThreads), pencil_shmem(c),
          (const float2*)c->g1.p, (const float2*)c->g2.p, (const float2*)c->ispec.p, od, c->y1.p,
          c->y2.p, m, chunk, max_ex, c->nx, c->px, py, pz, c->T, scale, tw, ctl);
@@ -1973,10 +1982,19 @@ int spk_timing_get(spk_ctx* c, const char* kernel, double* total_ms, int64_t* la
   });
 }
 
+int spk_timing_units(spk_ctx* c, const char* kernel, int64_t* units) {
+  return guarded(c, [&]() {
+    auto it = c->units.find(kernel ? kernel : "");
+    if (units) *units = it == c->units.end() ? 0 : it->second;
+    return SPK_OK;
+  });
+}
+
 int spk_timing_reset(spk_ctx* c) {
   return guarded(c, [&]() {
     c->flush_timing();
     c->acc.clear();
+    c->units.clear();
     return SPK_OK;
   });
 }
