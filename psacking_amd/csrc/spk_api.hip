@@ -54,11 +54,11 @@ bool smooth(int n) {
 }
 
 // Radices with a register butterfly (spk_fft.hpp).
-const int kRadices[] = {16, 15, 14, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
+const int kRadices[] = {25, 16, 15, 14, 12, 10, 9, 8, 7, 6, 5, 4, 3, 2};
 
 // Fewest stages first (each stage is one LDS round trip and two barriers),
 // then the most even split; larger radices run first.
-AxisPlan make_plan(int n) {
+AxisPlan make_plan(int n, int rmax = kMaxRadix) {
   std::map<int, std::vector<int>> memo;
   std::function<std::vector<int>(int)> best = [&](int m) -> std::vector<int> {
     if (m == 1) return {};
@@ -67,7 +67,7 @@ AxisPlan make_plan(int n) {
     std::vector<int> b;
     bool have = false;
     for (int r : kRadices) {
-      if (m % r) continue;
+      if (r > rmax || m % r) continue;
       std::vector<int> c = best(m / r);
       if (!c.empty() && c[0] == -1) continue;
       c.push_back(r);
@@ -313,7 +313,9 @@ void plan_tray(spk_ctx* c) {
     throw SpkError{SPK_EUNSUPPORTED, "tray axis longer than 1024 is not supported"};
   if ((long long)c->ny * c->nz > 32768)
     throw SpkError{SPK_EUNSUPPORTED, "tray y*z plane larger than 32768 voxels"};
-  c->px = make_plan(px);
+  // the x axis is transformed by the x-pencil kernels, whose butterflies stop
+  // at kPencilRadix (so do y pencils of the split plan, re-planned below)
+  c->px = make_plan(px, kPencilRadix);
   c->py = make_plan(py);
   c->pz = make_plan(pz);
   auto tile = [](int n) {
@@ -357,6 +359,7 @@ void plan_tray(spk_ctx* c) {
   c->split = c->plan_mode == 2 || (c->plan_mode == 0 && !slab_fits);
   if (!c->split && !slab_fits)
     throw SpkError{SPK_EUNSUPPORTED, "the slab plan does not fit this tray's yz plane"};
+  if (c->split) c->py = make_plan(py, kPencilRadix);
   if (!c->lds_attrs) {
     const size_t dyn = (size_t)lds_max - 1024;   // leave room for static shared arrays
     allow_shmem(k_tray_slab_fwd, dyn);
@@ -702,7 +705,8 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
     c->units["corr_x"] += m;
     c->units[c->split ? "rows_score" : "score_yz"] += m;
   }
-  launch(c, "corr_x", k_corr_x, dim3(tiles, py, nchunk), dim3(kPencilThreads), pencil_shmem(c),
+  launch(c, "corr_x", k_corr_x, dim3(tiles, py, nchunk), dim3(kPencilThreads),
+         pencil_shmem(c) + (size_t)c->px.n * c->T * sizeof(float2),
          (const float2*)c->g1.p, (const float2*)c->g2.p, (const float2*)c->ispec.p, od, c->y1.p,
          c->y2.p, m, chunk, max_ex, c->nx, c->px, py, pz, c->T, scale, tw, ctl);
   const double* pen = c->pen.p + (penalty == SPK_PENALTY_CUBE ? c->nz : 0);

@@ -19,6 +19,8 @@ namespace spk {
 constexpr int kMaxStages = 12;
 
 constexpr int kRadixBits = 5;
+constexpr int kMaxRadix = 25;       // largest butterfly (a 5x5 composite)
+constexpr int kPencilRadix = 16;    // largest butterfly in the x-pencil kernels
 constexpr unsigned long long kRadixMask = 31ULL;
 
 struct AxisPlan {
@@ -231,6 +233,7 @@ template <bool FWD> struct Bfly<12, FWD> : BflyCT<4, 3, FWD> {};
 template <bool FWD> struct Bfly<14, FWD> : BflyCT<2, 7, FWD> {};
 template <bool FWD> struct Bfly<15, FWD> : BflyCT<3, 5, FWD> {};
 template <bool FWD> struct Bfly<16, FWD> : BflyCT<4, 4, FWD> {};
+template <bool FWD> struct Bfly<25, FWD> : BflyCT<5, 5, FWD> {};
 
 // n / d for 0 <= n < 2^22, d >= 1, with inv = 1/d: the float quotient is off
 // by at most one, fixed by the remainder test.
@@ -312,7 +315,7 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n, int Ns, int es, in
 
 // Full transform of length p.n (all stages).  Caller has synchronised the
 // buffer; returns synchronised.
-template <bool FWD, int NT, int MAXEL>
+template <bool FWD, int NT, int MAXEL, int RMAX = kMaxRadix>
 __device__ __forceinline__ void lds_fft(float2* buf, const AxisPlan& p, int es, int batch, int bs,
                                         bool jfast,
                         const float2* tw, int tid) {
@@ -324,11 +327,17 @@ __device__ __forceinline__ void lds_fft(float2* buf, const AxisPlan& p, int es, 
     // live at once (the radix cases then spill at 128 VGPRs).
     int t = tid;
     asm volatile("" : "+v"(t));
-#define SPK_STAGE(RR) \
-  case RR: fft_stage<RR, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
+    // radices above RMAX are not compiled in (the host plans such axes
+    // without them): the largest butterfly sets the kernel's register need
+#define SPK_STAGE(RR)                                                              \
+  case RR:                                                                         \
+    if constexpr (RR <= RMAX)                                                      \
+      fft_stage<RR, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t);  \
+    break;
     switch (R) {
       SPK_STAGE(2) SPK_STAGE(3) SPK_STAGE(4) SPK_STAGE(5) SPK_STAGE(6) SPK_STAGE(7)
       SPK_STAGE(9) SPK_STAGE(10) SPK_STAGE(12) SPK_STAGE(14) SPK_STAGE(15) SPK_STAGE(16)
+      SPK_STAGE(25)
       default: fft_stage<8, FWD, NT, MAXEL>(buf, p.n, Ns, es, batch, bs, jfast, tw, t); break;
     }
 #undef SPK_STAGE

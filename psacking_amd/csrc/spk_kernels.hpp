@@ -29,8 +29,8 @@ namespace spk {
 constexpr int kSlabThreads = 1024;   // yz-slab kernels
 constexpr int kSlabMaxEl = 16;       // elements per thread -> slab <= 16384 complex
 constexpr int kSlabMax = kSlabThreads * kSlabMaxEl;
-constexpr int kPencilThreads = 256;  // x-pencil kernels
-constexpr int kPencilMaxEl = 16;     // px * T <= 4096
+constexpr int kPencilThreads = 512;  // x-pencil kernels
+constexpr int kPencilMaxEl = 8;      // px * T <= 4096
 constexpr int kPencilMax = kPencilThreads * kPencilMaxEl;
 constexpr int kMaxAxis = 1024;       // largest FFT length along one axis
 constexpr int kDistThreads = 1024;
@@ -461,7 +461,7 @@ k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
     }
   }
   __syncthreads();
-  lds_fft<true, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
+  lds_fft<true, kPencilThreads, kPencilMaxEl, kPencilRadix>(buf, px_, T, T, 1, false, twx, tid);
   for (int e = tid; e < px * T; e += kPencilThreads) {
     const int x = e / T, t = e - x * T;
     const int kz = kz0 + t;
@@ -514,6 +514,8 @@ k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od
 // ---------------------------------------------------------------------------
 // pass A: grid (ceil(pz/T), py, n_chunks).  x pencils of T consecutive kz.
 // ---------------------------------------------------------------------------
+// LDS: twiddles, then two pencil buffers (the G1 and G2 products of one
+// orientation, both formed right after the item's forward FFT).
 __global__ void __launch_bounds__(kPencilThreads)
 k_corr_x(const float2* __restrict__ g1, const float2* __restrict__ g2,
          const float2* __restrict__ ispec, const OrientDesc* __restrict__ od,
@@ -525,6 +527,7 @@ k_corr_x(const float2* __restrict__ g1, const float2* __restrict__ g2,
   const int px = px_.n, hz = pz / 2 + 1;
   float2* twx = smem;
   float2* buf = twx + px;
+  float2* buf2 = buf + px * T;
   const int ky = blockIdx.y, tid = threadIdx.x;
   const int kz0 = blockIdx.x * T;
   const int o0 = blockIdx.z * chunk;
@@ -560,21 +563,19 @@ k_corr_x(const float2* __restrict__ g1, const float2* __restrict__ g2,
       }
     }
     __syncthreads();
-    lds_fft<true, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
-    float2 I[kPencilMaxEl];
+    lds_fft<true, kPencilThreads, kPencilMaxEl, kPencilRadix>(buf, px_, T, T, 1, false, twx, tid);
+    // element e of both products belongs to the thread that read it
 #pragma unroll
     for (int s = 0; s < kPencilMaxEl; ++s) {
       const int e = tid + s * kPencilThreads;
-      I[s] = e < px * T ? buf[e] : make_float2(0.f, 0.f);
+      if (e < px * T) {
+        const float2 I = buf[e];
+        buf[e] = cscale(cmulc(G1[s], I), scale);
+        if (has2) buf2[e] = cscale(cmulc(G2[s], I), scale);
+      }
     }
     __syncthreads();
-#pragma unroll
-    for (int s = 0; s < kPencilMaxEl; ++s) {
-      const int e = tid + s * kPencilThreads;
-      if (e < px * T) buf[e] = cscale(cmulc(G1[s], I[s]), scale);
-    }
-    __syncthreads();
-    lds_fft<false, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
+    lds_fft<false, kPencilThreads, kPencilMaxEl, kPencilRadix>(buf, px_, T, T, 1, false, twx, tid);
     float2* d1 = y1 + (size_t)o * nx * py * pz;
     for (int e = tid; e < d.nvx * T; e += kPencilThreads) {
       const int x = e / T, t = e - x * T;
@@ -582,19 +583,13 @@ k_corr_x(const float2* __restrict__ g1, const float2* __restrict__ g2,
       if (kz < pz) d1[((size_t)x * py + ky) * pz + kz] = buf[e];
     }
     if (has2) {
-      __syncthreads();
-#pragma unroll
-      for (int s = 0; s < kPencilMaxEl; ++s) {
-        const int e = tid + s * kPencilThreads;
-        if (e < px * T) buf[e] = cscale(cmulc(G2[s], I[s]), scale);
-      }
-      __syncthreads();
-      lds_fft<false, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
+      lds_fft<false, kPencilThreads, kPencilMaxEl, kPencilRadix>(buf2, px_, T, T, 1, false, twx,
+                                                                 tid);
       float2* d2 = y2 + (size_t)o * nx * py * hz;
       for (int e = tid; e < d.nvx * T; e += kPencilThreads) {
         const int x = e / T, t = e - x * T;
         const int kz = kz0 + t;
-        if (kz < hz) d2[((size_t)x * py + ky) * hz + kz] = buf[e];
+        if (kz < hz) d2[((size_t)x * py + ky) * hz + kz] = buf2[e];
       }
     }
   }
@@ -841,14 +836,14 @@ k_feas_x(const float2* __restrict__ g1, const float2* __restrict__ ispec,
       }
     }
     __syncthreads();
-    lds_fft<true, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
+    lds_fft<true, kPencilThreads, kPencilMaxEl, kPencilRadix>(buf, px_, T, T, 1, false, twx, tid);
 #pragma unroll
     for (int s = 0; s < kPencilMaxEl; ++s) {
       const int e = tid + s * kPencilThreads;
       if (e < px * T) buf[e] = cscale(cmulc(G[s], buf[e]), scale);
     }
     __syncthreads();
-    lds_fft<false, kPencilThreads, kPencilMaxEl>(buf, px_, T, T, 1, false, twx, tid);
+    lds_fft<false, kPencilThreads, kPencilMaxEl, kPencilRadix>(buf, px_, T, T, 1, false, twx, tid);
     const int nvx = max(da.nvx, hasb ? db.nvx : 0);
     float2* d1 = y1 + (size_t)p * nx * py * pz;
     for (int e = tid; e < nvx * T; e += kPencilThreads) {
@@ -1214,7 +1209,7 @@ k_pencil(float2* __restrict__ data, AxisPlan p_, int B, int T, int n_in, int n_o
     }
   }
   __syncthreads();
-  lds_fft<FWD, kPencilThreads, kPencilMaxEl>(buf, p_, T, T, 1, false, tw, tid);
+  lds_fft<FWD, kPencilThreads, kPencilMaxEl, kPencilRadix>(buf, p_, T, T, 1, false, tw, tid);
   for (int e = tid; e < n_out * T; e += kPencilThreads) {
     const int y = e / T, t = e - y * T;
     if (b0 + t < B) base[(size_t)y * B + t] = buf[e];
