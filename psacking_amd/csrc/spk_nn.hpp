@@ -116,11 +116,17 @@ __device__ __forceinline__ bf16x8 as_bf8(uint4 v) { return __builtin_bit_cast(bf
 // BWD: X = output gradient dY, Wm = Wt [N = cin][ldw = Ktp], rows = input
 //      positions; mask (layer input, post-ReLU) zeroes the gradient where the
 //      activation was not positive.
+//
+// Split K (part != nullptr, forward only): block z reduces k in
+// [z*kchunk, (z+1)*kchunk) and writes its fp32 partial tile to
+// part[z][m][n]; k_split_finalize sums the slices in order and applies the
+// epilogue.  Used when M is small (inference at batch 1), where the K loop of
+// one or two blocks would otherwise be the whole kernel's latency.
 template <int MODE, bool VEC, int NT>
 __global__ void __launch_bounds__(256)
 k_igemm(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wm, const float* __restrict__ bias,
         ConvDesc d, int M, int N, int Kloop, int ldw, int relu, const bf16_t* __restrict__ mask,
-        bf16_t* __restrict__ Y, float* __restrict__ Yf) {
+        bf16_t* __restrict__ Y, float* __restrict__ Yf, int kchunk, float* __restrict__ part) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int mrow0 = (blockIdx.x * 4 + wave) * 32;
   const int nbase = blockIdx.y * 16 * NT;
@@ -142,7 +148,8 @@ k_igemm(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wm, const float
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-  for (int k0 = 0; k0 < Kloop; k0 += 32) {
+  const int k_begin = blockIdx.z * kchunk, k_end = min(Kloop, k_begin + kchunk);
+  for (int k0 = k_begin; k0 < k_end; k0 += 32) {
     const int k = k0 + lk;
     bf16x8 a[2], b[NT];
 #pragma unroll
@@ -191,7 +198,9 @@ k_igemm(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wm, const float
         if (m >= M) continue;
         float v = acc[r][t][j];
         const size_t o = (size_t)m * N + n;
-        if (MODE == MODE_FWD) {
+        if (part) {
+          part[(size_t)blockIdx.z * M * N + o] = v;
+        } else if (MODE == MODE_FWD) {
           v += bias[n];
           if (relu) v = fmaxf(v, 0.f);
           if (Yf) Yf[o] = v;
@@ -202,6 +211,20 @@ k_igemm(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wm, const float
         }
       }
     }
+}
+
+// epilogue of a split-K forward: Y = relu(sum_z part[z] + bias)
+__global__ void k_split_finalize(const float* __restrict__ part, int ks, int M, int N,
+                                 const float* __restrict__ bias, int relu, bf16_t* __restrict__ Y,
+                                 float* __restrict__ Yf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M * N) return;
+  float v = 0.f;
+  for (int z = 0; z < ks; ++z) v += part[(size_t)z * M * N + i];
+  v += bias[i % N];
+  if (relu) v = fmaxf(v, 0.f);
+  if (Yf) Yf[i] = v;
+  if (Y) Y[i] = f2bf(v);
 }
 
 // Weight gradient partial sums over row chunks.  Block (kb, chunk) computes

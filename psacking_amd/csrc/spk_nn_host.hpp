@@ -162,13 +162,25 @@ void nn_fwd(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* X,
   ConvDesc d = l.d;
   d.B = B;
   const int M = B * d.o0 * d.o1 * d.o2, N = d.cout, NT = nt_for(N);
-  const dim3 grid((M + 127) / 128, (N + 16 * NT - 1) / (16 * NT));
+  const int gx = (M + 127) / 128, gy = (N + 16 * NT - 1) / (16 * NT);
+  // split K when the row blocks alone cannot fill the chip: slices of >= 2
+  // k-steps, at most 256 blocks, partials in nn->part
+  const int ksteps = d.Kp / 32;
+  int ks = 1;
+  if (gx * gy < 64 && ksteps >= 4) {
+    ks = std::min(ksteps / 2, std::max(1, 256 / (gx * gy)));
+    if ((size_t)ks * M * N > nn->part.cap) ks = 1;
+  }
+  const int kchunk = ((ksteps + ks - 1) / ks) * 32;
+  ks = (d.Kp + kchunk - 1) / kchunk;
+  float* part = ks > 1 ? nn->part.p : nullptr;
+  const dim3 grid(gx, gy, ks);
   const bf16_t* W = nn->wbf.p + l.wb_off;
   const float* bias = nn->params.p + l.b_off;
   const bool vec = d.cin % 8 == 0;
 #define SPK_NN_FWD(V, T)                                                                         \
   nn_launch(s, spknn::k_igemm<spknn::MODE_FWD, V, T>, grid, dim3(256), 0, X, W, bias, d, M, N, \
-            d.Kp, d.Kp, l.relu, (const bf16_t*)nullptr, Y, Yf)
+            d.Kp, d.Kp, l.relu, (const bf16_t*)nullptr, Y, Yf, kchunk, part)
   if (vec) {
     switch (NT) {
       case 1: SPK_NN_FWD(true, 1); break;
@@ -185,6 +197,9 @@ void nn_fwd(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* X,
     }
   }
 #undef SPK_NN_FWD
+  if (part)
+    nn_launch(s, spknn::k_split_finalize, dim3((M * N + 255) / 256), dim3(256), 0,
+              (const float*)part, ks, M, N, bias, l.relu, Y, Yf);
 }
 
 // input gradient of one layer: dX = (dY * W) masked by the layer input > 0
@@ -197,7 +212,8 @@ void nn_bwd_data(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_
   const bf16_t* Wt = nn->wbf.p + l.wt_off;
 #define SPK_NN_BWD(T)                                                                            \
   nn_launch(s, spknn::k_igemm<spknn::MODE_BWD, true, T>, grid, dim3(256), 0, dY, Wt,            \
-            (const float*)nullptr, d, M, N, d.Ktp, d.Ktp, 0, mask, dX, (float*)nullptr)
+            (const float*)nullptr, d, M, N, d.Ktp, d.Ktp, 0, mask, dX, (float*)nullptr, d.Ktp,   \
+            (float*)nullptr)
   switch (NT) {
     case 1: SPK_NN_BWD(1); break;
     case 2: SPK_NN_BWD(2); break;
