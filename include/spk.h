@@ -178,6 +178,12 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  zero-collision offset at about a third of the scored
  *                  search's cost; an item with none is recorded as not found
  *                  without the scored search.  Results are unchanged.
+ *   "autotune"     1 (default): on the first spk_tray_set of a tray shape in
+ *                  the process, time each admissible x-pencil tile on a
+ *                  24-orientation search and feasibility pass and keep the
+ *                  fastest; 0: fixed tile.
+ *   "tile"         0 (default) tuned or default tile; 4, 8 or 16 force the
+ *                  x-pencil tile (applies at the next spk_tray_set).
  *   "certificate"  1: before an item's search, compare the tray field's
  *                  maximum with the item's inscribed-ball bound (the largest
  *                  distance from an item voxel to a voxel outside it); when
@@ -189,6 +195,9 @@ int spk_set_option(spk_ctx* ctx, const char* name, double value);
 
 /* Plan actually used for the cached tray: FFT sizes and digit base. */
 int spk_plan_info(spk_ctx* ctx, int* px, int* py, int* pz, int* digit_base);
+/* x-pencil tile and plan in use (split = 1 for z rows + y pencils); the tile
+ * is the autotuner's choice when option "autotune" is on (the default). */
+int spk_tile_info(spk_ctx* ctx, int* tile, int* split);
 
 /* ---- Device memory (no reference counterpart) -------------------------------
  * For callers that hold no HIP runtime of their own (ctypes, other

@@ -19,6 +19,7 @@ EXPORTS = (
     "spk_place", "spk_pack", "spk_pack_many", "spk_distance", "spk_correlate", "spk_collision_grid",
     "spk_interlock_mask", "spk_place_grid", "spk_bounds", "spk_save_vox", "spk_timing_enable",
     "spk_timing_get", "spk_timing_units", "spk_timing_reset", "spk_set_option", "spk_plan_info",
+    "spk_tile_info",
     "spk_dev_alloc", "spk_dev_free", "spk_dev_copy", "spk_dev_gather",
     "spk_nn_create", "spk_nn_destroy", "spk_nn_num_params", "spk_nn_set_params",
     "spk_nn_get_params", "spk_nn_forward", "spk_nn_gradient", "spk_nn_train_step",

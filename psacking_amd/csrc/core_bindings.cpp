@@ -454,6 +454,11 @@ class Session {
     check(spk_plan_info(ctx_, &px, &py_, &pz, &b));
     return py::make_tuple(px, py_, pz);
   }
+  py::tuple tile() {
+    int t = 0, split = 0;
+    check(spk_tile_info(ctx_, &t, &split));
+    return py::make_tuple(t, split);
+  }
 
   spk_ctx* ctx() const { return ctx_; }
 
@@ -571,6 +576,7 @@ PYBIND11_MODULE(_core, m) {
       .def("timing_units", &Session::timing_units, py::arg("name"))
       .def("timing_reset", &Session::timing_reset)
       .def("plan", &Session::plan)
+      .def("tile", &Session::tile)
       .def("handle", [](const Session& s) { return (uintptr_t)s.ctx(); },
            "the spk_ctx* of this session, for C-ABI calls (ctypes)");
 

@@ -9,6 +9,8 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <mutex>
+#include <tuple>
 #include <string>
 #include <thread>
 #include <vector>
@@ -176,6 +178,8 @@ struct spk_ctx {
   DevBuf<int> fstate;                    // residual bits, unused, per-orientation flags
   DevBuf<int> ctl_on;                    // control words that enable the pass outside the loop
   int feas_mode = 1;                     // 0 off, 1 after an item that placed nothing, 2 always
+  bool autotune = true;                  // time x-pencil tiles per tray shape (option "autotune")
+  int tile_force = 0;                    // option "tile": 0 tuned/default, else 4, 8 or 16
   // free-space certificate
   bool phi_internal = false;             // phi is this library's distance field of the tray
   bool certificate = false;              // option "certificate"
@@ -1464,6 +1468,114 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   return placed;
 }
 
+// ---- autotuner -------------------------------------------------------------
+// The x-pencil tile T (kz columns per block of the pencil kernels) trades
+// blocks in flight against per-block work and write granularity, and the
+// best value depends on the tray shape.  On the first spk_tray_set of a shape
+// every admissible T in {4, 8, 16} is timed on a real 24-orientation search
+// (and, on the slab plan, a feasibility pass) of a box item sized to the tray;
+// the fastest is kept for that shape for the rest of the process.
+struct TuneKey {
+  int nx, ny, nz, split;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(nx, ny, nz, split) < std::tie(o.nx, o.ny, o.nz, o.split);
+  }
+};
+std::map<TuneKey, int> g_tile;
+std::mutex g_tile_mu;
+
+// the 24 proper rotations as index maps
+std::vector<spk_orient> rotation_maps() {
+  std::vector<spk_orient> out;
+  const int perms[6][3] = {{0, 1, 2}, {1, 2, 0}, {2, 0, 1}, {0, 2, 1}, {2, 1, 0}, {1, 0, 2}};
+  for (int p = 0; p < 6; ++p)
+    for (int f = 0; f < 8; ++f) {
+      const int flips = (f & 1) + ((f >> 1) & 1) + ((f >> 2) & 1);
+      const bool even_perm = p < 3;
+      if (((flips & 1) == 0) != even_perm) continue;   // determinant +1
+      spk_orient o;
+      for (int a = 0; a < 3; ++a) {
+        o.perm[a] = perms[p][a];
+        o.flip[a] = (f >> a) & 1;
+      }
+      out.push_back(o);
+    }
+  return out;
+}
+
+void autotune_tile(spk_ctx* c) {
+  const TuneKey key{c->nx, c->ny, c->nz, c->split ? 1 : 0};
+  {
+    std::lock_guard<std::mutex> g(g_tile_mu);
+    auto it = g_tile.find(key);
+    if (it != g_tile.end()) {
+      c->T = it->second;
+      return;
+    }
+  }
+  const int px = c->px.n;
+  std::vector<int> cands;
+  for (int T : {4, 8, 16})
+    if (px * T <= kPencilMax && (T <= 2 * c->pz.n || T == 4)) cands.push_back(T);
+  if (cands.size() > 1) {
+    // box item about a fifth of the tray's smallest side, slightly uneven
+    const int a = std::max(2, std::min({c->nx, c->ny, c->nz}) / 5);
+    const int d0 = std::min(c->nx, a), d1 = std::min(c->ny, std::max(2, a * 4 / 5)),
+              d2 = std::min(c->nz, std::max(2, a * 6 / 5));
+    std::vector<int> box((size_t)d0 * d1 * d2, 1);
+    c->items.ensure(box.size());
+    upload(c, c->items.p, box.data(), box.size() * sizeof(int));
+    const HostItem h = scan_item(box.data(), d0, d1, d2);
+    std::vector<OrientDesc> ods;
+    int max_ex = 1, max_e = 1;
+    for (const spk_orient& o : rotation_maps()) {
+      OrientDesc d = make_desc(c, h, 0, &o);
+      if (!fits(c, d)) continue;
+      ods.push_back(d);
+      max_ex = std::max(max_ex, d.ex);
+      max_e = std::max(max_e, d.ex * d.ey * d.ez);
+    }
+    const int m = (int)ods.size();
+    if (m > 0) {
+      c->od.ensure(m);
+      upload(c, c->od.p, ods.data(), m * sizeof(OrientDesc));
+      ensure_search_ws(c, m, max_ex);
+      hipEvent_t e0, e1;
+      HIP_OK(hipEventCreate(&e0));
+      HIP_OK(hipEventCreate(&e1));
+      const bool timing = c->timing;
+      c->timing = false;
+      float best_ms = 1e30f;
+      int best_T = c->T;
+      for (int T : cands) {
+        c->T = T;
+        auto run = [&]() {
+          enqueue_search(c, c->od.p, m, max_ex, max_e, SPK_PENALTY_CUBE, nullptr, nullptr,
+                         nullptr, nullptr);
+          if (!c->split) enqueue_feas(c, c->od.p, m, max_ex, nullptr);
+        };
+        run();   // warm-up
+        HIP_OK(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < 2; ++r) run();
+        HIP_OK(hipEventRecord(e1, c->stream));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+          best_ms = ms;
+          best_T = T;
+        }
+      }
+      c->timing = timing;
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      c->T = best_T;
+    }
+  }
+  std::lock_guard<std::mutex> g(g_tile_mu);
+  g_tile[key] = c->T;
+}
+
 // Parses and uploads the concatenated items of a pack call.
 void load_items(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_items,
                 const int32_t* order, std::vector<HostItem>& bases, std::vector<int>& off) {
@@ -1591,6 +1703,14 @@ int spk_tray_set(spk_ctx* c, const int32_t* tray, int nx, int ny, int nz, const 
     c->stats.ensure(1);
     upload(c, c->tray.p, tray, V * sizeof(int));
     refresh_field_and_spectra(c, dist);
+    if (c->tile_force) {
+      if ((long long)c->px.n * c->tile_force > kPencilMax)
+        throw SpkError{SPK_EUNSUPPORTED, "tile too large for this tray's x axis"};
+      c->T = c->tile_force;
+    } else if (reshape && c->autotune) {
+      c->valid = true;
+      autotune_tile(c);
+    }
     sync(c);
     if (c->timing) c->flush_timing();
     c->generation = generation;
@@ -2017,6 +2137,19 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->valid = false;
     return SPK_OK;
   }
+  if (std::string(name) == "autotune") {
+    c->autotune = value != 0.0;
+    return SPK_OK;
+  }
+  if (std::string(name) == "tile") {
+    const int v = (int)value;
+    if (v != 0 && v != 4 && v != 8 && v != 16)
+      return fail(SPK_EINVAL, "tile must be 0 (tuned), 4, 8 or 16");
+    c->tile_force = v;
+    c->tw.release();   // re-plan at the next spk_tray_set
+    c->valid = false;
+    return SPK_OK;
+  }
   if (std::string(name) == "certificate") {
     c->certificate = value != 0.0;
     return SPK_OK;
@@ -2037,6 +2170,13 @@ int spk_plan_info(spk_ctx* c, int* px, int* py, int* pz, int* digit_base) {
   if (py) *py = c->py.n;
   if (pz) *pz = c->pz.n;
   if (digit_base) *digit_base = 0;
+  return SPK_OK;
+}
+
+int spk_tile_info(spk_ctx* c, int* tile, int* split) {
+  if (!c) return fail(SPK_EINVAL, "null context");
+  if (tile) *tile = c->T;
+  if (split) *split = c->split ? 1 : 0;
   return SPK_OK;
 }
 

@@ -333,7 +333,8 @@ def test_search_forced_exact_interlocking(P):
     np.testing.assert_array_equal(s.tray(), exp.tray)
 
 
-def _pack_with_plan(P, w, n_orient, plan, metric="l1", sources="eq1", interlocking_free=False):
+def _pack_with_plan(P, w, n_orient, plan, metric="l1", sources="eq1", interlocking_free=False,
+                    tile=0):
     from psacking_amd.rotations import orientation_maps
     items = w.items
     vols = [int((it > 0).sum()) for it in items]
@@ -341,6 +342,7 @@ def _pack_with_plan(P, w, n_orient, plan, metric="l1", sources="eq1", interlocki
     perms, flips = orientation_maps(n_orient)
     s = P.Session(0)
     s.set_option("plan", plan)
+    s.set_option("tile", tile)
     s.set_tray(np.zeros(w.tray_size, np.int32), metric, sources, 0)
     res = s.pack(items, np.array(order, np.int32), perms, flips,
                  "pow" if n_orient == 1 and not interlocking_free else "cube", interlocking_free, 0)
@@ -354,6 +356,8 @@ def _pack_with_plan(P, w, n_orient, plan, metric="l1", sources="eq1", interlocki
             assert (tuple(r["position"]), r["orientation"], r["score"]) == \
                 (e.position, e.orientation_index, e.score)
     np.testing.assert_array_equal(s.tray(), exp.tray)
+    if tile:
+        assert s.tile()[0] == tile
 
 
 @pytest.mark.parametrize("plan", [1, 2])
@@ -468,3 +472,24 @@ def test_free_space_certificate_keeps_results(P):
                 assert (tuple(r["position"]), r["orientation"], r["score"]) == \
                     (e.position, e.orientation_index, e.score)
         np.testing.assert_array_equal(s.tray(), exp.tray)
+
+
+def test_autotuned_tile_is_cached_and_exact(P):
+    from psacking_amd import workloads as W
+    w = W.blobs(seed=61, n=5, tray=(72, 54, 44))
+    s1 = P.Session(0)
+    s1.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
+    t1, split = s1.tile()
+    say("autotuned tile", t1, "split", split)
+    assert t1 in (4, 8, 16) and split == 0
+    s2 = P.Session(0)
+    s2.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
+    assert s2.tile() == (t1, split)
+    s3 = P.Session(0)
+    s3.set_option("autotune", 0)
+    s3.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
+    assert s3.tile()[0] == 16
+    # every admissible tile gives the oracle's placements
+    for tile in (4, 8, 16):
+        say("forced tile", tile)
+        _pack_with_plan(P, w, 24, 0, "edt", "nonzero", tile=tile)
