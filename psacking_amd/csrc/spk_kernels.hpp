@@ -1509,15 +1509,39 @@ k_ff_tile(const unsigned char* __restrict__ fr_all, unsigned char* __restrict__ 
   __syncthreads();
   const int c = ((tx + 1) * HY + (ty + 1)) * HZ + (tz + 1);
   const bool inside = (bx + tx < nx) && (by + ty < ny) && (bz + tz < nz);
+  // A z line of the tile is 32 consecutive lanes (half a wavefront): each
+  // round takes one step from the 6 neighbours, then carries reached cells
+  // along runs of free cells of the line with a segmented OR-scan over the
+  // lanes (both directions, no barrier), so only x and y steps cost a round.
+  const int zl = tid & (FFZ - 1);
+  const bool free_c = inside && sf[c];
   bool mine = false;
   for (int it = 0; it < 4096; ++it) {
-    int upd = 0;
-    if (inside && sf[c] && !sr[c]) {
-      if (sr[c - 1] | sr[c + 1] | sr[c - HZ] | sr[c + HZ] | sr[c - HY * HZ] | sr[c + HY * HZ]) {
-        sr[c] = 1;
-        mine = true;
-        upd = 1;
+    int gen = 0;
+    if (free_c) {
+      gen = sr[c] | sr[c - 1] | sr[c + 1] | sr[c - HZ] | sr[c + HZ] | sr[c - HY * HZ] |
+            sr[c + HY * HZ];
+      gen = gen ? 1 : 0;
+    }
+    int gf = gen, pf = free_c ? 1 : 0, gb = gen, pb = pf;
+#pragma unroll
+    for (int d = 1; d < FFZ; d <<= 1) {
+      const int g1 = __shfl_up(gf, d, FFZ), p1 = __shfl_up(pf, d, FFZ);
+      const int g2 = __shfl_down(gb, d, FFZ), p2 = __shfl_down(pb, d, FFZ);
+      if (zl >= d) {
+        gf |= pf & g1;
+        pf &= p1;
       }
+      if (zl + d < FFZ) {
+        gb |= pb & g2;
+        pb &= p2;
+      }
+    }
+    const int upd = (free_c && !sr[c] && (gf | gb)) ? 1 : 0;
+    __syncthreads();
+    if (upd) {
+      sr[c] = 1;
+      mine = true;
     }
     if (!__syncthreads_or(upd)) break;
   }
