@@ -27,6 +27,10 @@
  *                         behind interlocking_free_positions (bindings.cpp:561-648)
  *   spk_bounds            get_voxel_grid_bounds_flat (voxelGrid.cpp:279-299)
  *   spk_save_vox          saveAsVox (packing.cpp:275-305)
+ *   spk_voxelize_stl      py_voxelize_stl (bindings.cpp:285-302): LibSL STL
+ *                         reading, voxelize (packing.cpp:526-588) and
+ *                         make_voxel_grid_tight (voxelGrid.cpp:166-178)
+ *   spk_voxelize_mesh     voxelize (packing.cpp:526-588) on an in-memory mesh
  */
 #ifndef SPK_H
 #define SPK_H
@@ -155,6 +159,24 @@ int spk_place_grid(spk_ctx* ctx, const int32_t* tray, int nx, int ny, int nz,
 /* Host grid utilities. */
 int spk_bounds(const int32_t* grid, int nx, int ny, int nz, int32_t lo[3], int32_t hi[3]);
 int spk_save_vox(const int32_t* grid, int nx, int ny, int nz, const char* path);
+
+/* Surface voxelization of a triangle mesh on the GPU, bit-exact with the
+ * reference: exact vertex merge, fixed-point transform into a box of
+ * `resolution` voxels along the longest axis with a 2.5% margin, three-view
+ * rasterization with one parity bit per view and voxel, binarisation, tight
+ * crop.  dims receives the cropped shape (x, y, z).  With out == NULL only dims
+ * are computed; otherwise out (x-major int32 0/1) must hold capacity >=
+ * dims[0]*dims[1]*dims[2] values.  resolution lies in [1, 1024].  A mesh that
+ * marks no voxel fails with SPK_EINVAL and the reference's message.
+ * spk_voxelize_mesh: positions are n_vertices float32 triples; triangles are
+ * n_triangles int32 index triples.  spk_voxelize_stl reads a binary or ASCII
+ * STL file with the reference's rules (binary exactly when the file size is
+ * 84 + 50 * facet count). */
+int spk_voxelize_mesh(int device, const float* positions, int64_t n_vertices,
+                      const int32_t* triangles, int64_t n_triangles, int resolution,
+                      int32_t* dims, int32_t* out, int64_t capacity);
+int spk_voxelize_stl(int device, const char* path, int resolution, int32_t* dims, int32_t* out,
+                     int64_t capacity);
 
 /* Per-kernel HIP-event timing on the context's stream (off by default). */
 int spk_timing_enable(spk_ctx* ctx, int on);

@@ -42,6 +42,7 @@ from ._core import (  # noqa: F401
     voxelize_stl,
 )
 from ._core import __version__  # noqa: F401
+from .mesh import voxelize_mesh  # noqa: F401
 from .packer import BinPacker, PackingResult, PlacementInfo  # noqa: F401
 from .rotations import (  # noqa: F401
     get_24_orientations,
@@ -61,5 +62,5 @@ __all__ = [
     "get_bounds", "pack_many", "save_vox", "gpu_tray_init", "gpu_tray_cleanup", "gpu_tray_is_initialized",
     "gpu_tray_search", "interlocking_free_positions", "get_orientations",
     "get_24_orientations", "orientation_maps", "make_contiguous", "VOXEL_RESOLUTION",
-    "HEIGHT_PENALTY",
+    "HEIGHT_PENALTY", "voxelize_mesh",
 ]

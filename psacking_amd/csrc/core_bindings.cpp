@@ -184,9 +184,17 @@ py::array_t<int32_t> py_place_in_tray(py::object item, py::object tray, py::tupl
 }
 
 py::array_t<int32_t> py_voxelize_stl(const std::string& path, int resolution) {
-  (void)resolution;
-  throw std::runtime_error("Voxelization failed for " + path +
-                           ": mesh voxelization is not part of this build (pass voxel grids)");
+  // bindings.cpp:285-302: every failure surfaces as RuntimeError with this prefix
+  auto fail_vox = [&]() {
+    throw std::runtime_error("Voxelization failed for " + path + ": " + spk_last_error());
+  };
+  int32_t dims[3] = {0, 0, 0};
+  if (spk_voxelize_stl(0, path.c_str(), resolution, dims, nullptr, 0) != SPK_OK) fail_vox();
+  auto out = new_grid(dims[0], dims[1], dims[2]);
+  if (spk_voxelize_stl(0, path.c_str(), resolution, dims, out.mutable_data(), (int64_t)out.size()) !=
+      SPK_OK)
+    fail_vox();
+  return out;
 }
 
 py::array_t<int32_t> correlate(py::object a, py::object b, int convolve) {

@@ -1620,6 +1620,8 @@ int dev_guarded(int device, F&& f) {
 }
 }  // namespace
 
+#include "spk_vox.hpp"
+
 // =============================================================================
 // C-ABI
 // =============================================================================
@@ -2453,6 +2455,38 @@ int spk_pack_learned(spk_ctx* c, spk_nn* nn, const spk_pack_args* a, int topk, f
     sync(c);
     if (c->timing) c->flush_timing();
     c->generation += (uint64_t)placed;
+    return SPK_OK;
+  });
+}
+
+// ---------------------------------------------------------------- voxelizer
+
+int spk_voxelize_mesh(int device, const float* positions, int64_t n_vertices,
+                      const int32_t* triangles, int64_t n_triangles, int resolution,
+                      int32_t* dims, int32_t* out, int64_t capacity) {
+  if (!dims) return fail(SPK_EINVAL, "null dims");
+  if (!triangles && n_triangles) return fail(SPK_EINVAL, "null triangles");
+  return dev_guarded(device, [&]() -> int {
+    try {
+      spkvox::voxelize(positions, n_vertices, triangles, n_triangles, resolution, dims, out, capacity);
+    } catch (const spkvox::VoxError& e) {
+      return fail(e.code, e.msg);
+    }
+    return SPK_OK;
+  });
+}
+
+int spk_voxelize_stl(int device, const char* path, int resolution, int32_t* dims, int32_t* out,
+                     int64_t capacity) {
+  if (!path || !dims) return fail(SPK_EINVAL, "null path or dims");
+  return dev_guarded(device, [&]() -> int {
+    try {
+      const std::vector<float> pos = spkvox::load_stl(path);
+      const int64_t nv = (int64_t)pos.size() / 3;
+      spkvox::voxelize(pos.data(), nv, nullptr, nv / 3, resolution, dims, out, capacity);
+    } catch (const spkvox::VoxError& e) {
+      return fail(e.code, e.msg);
+    }
     return SPK_OK;
   });
 }
