@@ -85,6 +85,7 @@ def bench_uneven(args) -> int:
     for w in ws:
         s = P.Session(0)
         s.set_option("plan", args.plan)
+        s.set_option("certificate", args.certificate)
         vols = [int((it > 0).sum()) for it in w.items]
         sessions.append(s)
         lists.append([np.ascontiguousarray(it.astype(np.int32)) for it in w.items])
@@ -161,6 +162,8 @@ def main(argv=None) -> int:
     ap.add_argument("--proposer", default="auto", choices=["auto", "plain", "learned"],
                     help="auto: time the plain and the learned loop and report the learned one "
                          "when its density is within 0.5 pp of the plain one")
+    ap.add_argument("--certificate", type=int, default=0, choices=[0, 1],
+                    help="free-space certificate before the feasibility pass (exact)")
     ap.add_argument("--topk", type=int, default=4)
     ap.add_argument("--min-conf", type=float, default=0.0)
     ap.add_argument("--weights", default=os.path.join(ROOT, "psacking_amd", "weights",
@@ -219,6 +222,7 @@ def main(argv=None) -> int:
     def fresh_session():
         s = P.Session(0)
         s.set_option("plan", args.plan)
+        s.set_option("certificate", args.certificate)
         t0 = time.perf_counter()
         s.set_tray(np.zeros(w.tray_size, np.int32), args.metric, args.sources, 0)
         device_sync()
@@ -253,28 +257,48 @@ def main(argv=None) -> int:
                       "fallbacks": sum(1 for r in res if r["path"] != 0),
                       "max_resid": max(float(r["residual"]) for r in res)}
 
-    if want_plain:
+    # Each loop runs twice on a fresh tray: the timed run with per-kernel HIP
+    # events off (the reported value), then an identical run with them on for
+    # the kernel table and the roofline.  Both runs must place identically.
+    def run_plain(timing):
         s, t_setup = fresh_session()
         s.timing_reset()
-        s.timing(True)
+        s.timing(timing)
         device_sync()
         t1 = time.perf_counter()
         res = s.pack(timed, np.arange(K, dtype=np.int32), perms, flips, penalty, False, 0)
         device_sync()
         dt = time.perf_counter() - t1
         s.timing(False)
-        finish("plain", s, dt, res, t_setup, sum(fits))
-    if want_learned:
+        return s, t_setup, dt, res
+
+    def run_learned(timing):
         s, t_setup = fresh_session()
         s.timing_reset()
-        s.timing(True)
+        s.timing(timing)
         device_sync()
         t1 = time.perf_counter()
-        res, props, counts = PR.pack_learned(s, nn, timed, np.arange(K), perms, flips, penalty,
-                                             args.topk, args.min_conf)
+        out = PR.pack_learned(s, nn, timed, np.arange(K), perms, flips, penalty, args.topk,
+                              args.min_conf)
         device_sync()
         dt = time.perf_counter() - t1
         s.timing(False)
+        return (s, t_setup, dt) + tuple(out)
+
+    def identical(a, b):
+        key = lambda r: (r["found"], tuple(r["position"]), r["orientation"], r["score"])
+        return [key(x) for x in a] == [key(x) for x in b]
+
+    if want_plain:
+        _, _, dt, res0 = run_plain(False)
+        s, t_setup, dt_ev, res = run_plain(True)
+        assert identical(res0, res), "plain runs with and without kernel events differ"
+        finish("plain", s, dt, res, t_setup, sum(fits))
+        runs["plain"]["dt_events"] = dt_ev
+    if want_learned:
+        _, _, dt, res0, _, _ = run_learned(False)
+        s, t_setup, dt_ev, res, props, counts = run_learned(True)
+        assert identical(res0, res), "learned runs with and without kernel events differ"
         searched = 0
         for t, r in enumerate(res):
             p = [int(o) for o in props[t] if o >= 0]
@@ -286,6 +310,7 @@ def main(argv=None) -> int:
                     searched += fits[t]
         finish("learned", s, dt, res, t_setup, searched)
         runs["learned"]["counts"] = [int(c) for c in counts]
+        runs["learned"]["dt_events"] = dt_ev
 
     for name, r in runs.items():
         busy = sum(v[0] for v in r["kt"].values())
@@ -295,7 +320,8 @@ def main(argv=None) -> int:
                       f"{r['kt'][n][0] / max(1, r['kt'][n][1]) * 1e3:8.1f} us/launch  "
                       f"{100 * r['kt'][n][0] / max(busy, 1e-9):5.1f} %", file=sys.stderr)
         print(f"[bench:{name}] {K} items ({r['placed']} placed), {r['dt']:.3f} s = "
-              f"{K / r['dt']:.1f} items/s, kernels busy {busy / 1e3:.3f} s, density "
+              f"{K / r['dt']:.1f} items/s ({r['dt_events']:.3f} s with kernel events), "
+              f"kernels busy {busy / 1e3:.3f} s, density "
               f"{100 * r['density']:.2f} %, fallbacks {r['fallbacks']}, max residual "
               f"{r['max_resid']:.4f}" + (f", counts {r['counts']}" if "counts" in r else ""),
               file=sys.stderr, flush=True)

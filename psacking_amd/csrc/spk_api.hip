@@ -1332,7 +1332,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     }
     const bool have_t = have;   // logits of item t computed after the previous step
     have = false;
-    if (bounds && tst.mx < hb[t].mx) {
+    if (bounds && tst.mxw < hb[t].mx) {
       ++cnt[4];
       to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, &results[t]);
       prev_placed = false;

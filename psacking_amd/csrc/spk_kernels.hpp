@@ -42,6 +42,9 @@ constexpr int kDistThreads = 1024;
 struct FieldStats {          // reduction of the distance field
   unsigned long long sum;    // two's complement int64 sum
   int mn, mx;
+  int mxw;                   // max over voxels of min(field, distance to the outside
+                             // of the grid): the largest ball of free voxels inside
+  int pad;
 };
 
 struct FieldParams {         // derived on device identically by every block
@@ -172,9 +175,33 @@ __device__ __forceinline__ bool better(double s1, int i1, double s2, int i2) {
 // distance field: separable exact min-plus transforms
 //   L1 : g(k) = min_k' f(k') + |k-k'|      (reference sweeps, fft3.cu:887-942)
 //   EDT: g(k) = min_k' f(k') + (k-k')^2    (squared distances, exact ints)
+// The first (z) pass sees only sources (f = 0) and non-sources (f = big): two
+// sweeps per row give the nearest source.  The y and x passes search outward
+// from each voxel and stop at the first offset d whose cost alone reaches the
+// best value found, which no farther k' can beat because f >= 0; the result is
+// the same minimum as the full scan at a cost proportional to the distance.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ int dcost(int d, int edt) { return edt ? d * d : (d < 0 ? -d : d); }
+
+// min over k' of f(k') + cost(k - k') along a strided line of n values, f >= 0
+__device__ __forceinline__ int outward_min(const int* line, int stride, int n, int k, int edt) {
+  int best = line[k * stride];
+  for (int d = 1;; ++d) {
+    const int c = dcost(d, edt);
+    const bool lo = k - d >= 0, hi = k + d < n;
+    if (c >= best || (!lo && !hi)) break;
+    if (lo) {
+      const int v = line[(k - d) * stride] + c;
+      best = v < best ? v : best;
+    }
+    if (hi) {
+      const int v = line[(k + d) * stride] + c;
+      best = v < best ? v : best;
+    }
+  }
+  return best;
+}
 
 // one x-slice: init (sources) + z pass + y pass, in LDS; writes tmp int grid.
 __global__ void __launch_bounds__(kDistThreads)
@@ -198,47 +225,88 @@ k_dist_zy(const int* __restrict__ tray, int* __restrict__ out, int nx, int ny, i
       const int e = tid + t * kDistThreads;
       if (e < S) {
         const bool s = sources ? (v[t] != 0) : (v[t] == 1);
-        sh[e] = s ? 0 : big;
+        sh[e] = s ? 0 : 1;
       }
+    }
+  }
+  __syncthreads();
+  // z pass: each thread owns a run of consecutive elements; block-wide scans
+  // give every element the index of the last source at or before it (max scan)
+  // and of the first source at or after it (min scan).  A source in another z
+  // row is farther than the row boundary and counts as none.
+  {
+    constexpr int kW = kDistThreads / 64;
+    __shared__ int wlast[kW], wfirst[kW];
+    const int per = (S + kDistThreads - 1) / kDistThreads;   // <= 32
+    const int base = tid * per;
+    int last = -1, first = INT_MAX;
+    for (int t = 0; t < per; ++t) {
+      const int e = base + t;
+      if (e < S && sh[e] == 0) {
+        last = e;
+        first = first == INT_MAX ? e : first;
+      }
+    }
+    // inclusive wave scans: max of last from the left, min of first from the right
+    const int lane = tid & 63, wave = tid >> 6;
+    int incl_last = last, incl_first = first;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int a = __shfl_up(incl_last, d, 64);
+      const int b = __shfl_down(incl_first, d, 64);
+      if (lane >= d) incl_last = a > incl_last ? a : incl_last;
+      if (lane + d < 64) incl_first = b < incl_first ? b : incl_first;
+    }
+    if (lane == 63) wlast[wave] = incl_last;
+    if (lane == 0) wfirst[wave] = incl_first;
+    __syncthreads();
+    int before = __shfl_up(incl_last, 1, 64);     // exclusive within the wave
+    if (lane == 0) before = -1;
+    int after = __shfl_down(incl_first, 1, 64);
+    if (lane == 63) after = INT_MAX;
+    for (int w = 0; w < wave; ++w) before = wlast[w] > before ? wlast[w] : before;
+    for (int w = wave + 1; w < kW; ++w) after = wfirst[w] < after ? wfirst[w] : after;
+    int dz[32];
+    int prev = before;
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+      const int e = base + t;
+      if (t < per && e < S) {
+        if (sh[e] == 0) prev = e;
+        const int r0 = (e / nz) * nz;
+        dz[t] = prev >= r0 ? e - prev : INT_MAX;
+      }
+    }
+    int nxt = after;
+#pragma unroll
+    for (int t = 31; t >= 0; --t) {
+      const int e = base + t;
+      if (t < per && e < S) {
+        const int src = sh[e] == 0;
+        if (src) nxt = e;
+        const int r1 = (e / nz) * nz + nz - 1;
+        const int db = nxt <= r1 ? nxt - e : INT_MAX;
+        const int d = dz[t] < db ? dz[t] : db;
+        dz[t] = d == INT_MAX ? big : dcost(d, edt);
+        if (dz[t] > big) dz[t] = big;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 32; ++t) {
+      const int e = base + t;
+      if (t < per && e < S) sh[e] = dz[t];
     }
   }
   __syncthreads();
   constexpr int MAXR = 32;   // elements per thread: ny*nz <= 32768
   int res[MAXR];
-  // z pass (rows)
-#pragma unroll
-  for (int t = 0; t < MAXR; ++t) {
-    const int e = tid + t * kDistThreads;
-    if (e < S) {
-      const int j = e / nz, k = e - j * nz;
-      const int* row = sh + j * nz;
-      int best = row[k];
-      for (int kk = 0; kk < nz; ++kk) {
-        const int c = row[kk] + dcost(k - kk, edt);
-        best = c < best ? c : best;
-      }
-      res[t] = best;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int t = 0; t < MAXR; ++t) {
-    const int e = tid + t * kDistThreads;
-    if (e < S) sh[e] = res[t];
-  }
-  __syncthreads();
   // y pass (columns)
 #pragma unroll
   for (int t = 0; t < MAXR; ++t) {
     const int e = tid + t * kDistThreads;
     if (e < S) {
       const int j = e / nz, k = e - j * nz;
-      int best = sh[e];
-      for (int jj = 0; jj < ny; ++jj) {
-        const int c = sh[jj * nz + k] + dcost(j - jj, edt);
-        best = c < best ? c : best;
-      }
-      res[t] = best;
+      res[t] = outward_min(sh + k, nz, ny, j, edt);
     }
   }
 #pragma unroll
@@ -276,15 +344,12 @@ k_dist_x(const int* __restrict__ in, int* __restrict__ phi, FieldStats* __restri
   }
   __syncthreads();
   long long lsum = 0;
-  int lmn = INT_MAX, lmx = INT_MIN;
+  int lmn = INT_MAX, lmx = INT_MIN, lmw = INT_MIN;
+  const int wy = min(j + 1, ny - j);   // distance to the outside across a y face
   for (int e = tid; e < E; e += kDistThreads) {
     const int x = e / TK, t = e - x * TK;
     if (t >= tk) continue;
-    int best = sh[e];
-    for (int xx = 0; xx < nx; ++xx) {
-      const int c = sh[xx * TK + t] + dcost(x - xx, edt);
-      best = c < best ? c : best;
-    }
+    const int best = outward_min(sh + t, TK, nx, x, edt);
     int v;
     if (edt) {
       v = (best >= fill * fill) ? fill : (int)rint(sqrt((double)best));
@@ -296,34 +361,44 @@ k_dist_x(const int* __restrict__ in, int* __restrict__ phi, FieldStats* __restri
     lsum += v;
     lmn = v < lmn ? v : lmn;
     lmx = v > lmx ? v : lmx;
+    // distance to the nearest voxel outside the grid, in any metric: the
+    // nearest such voxel is straight across the closest face
+    const int k = k0 + t;
+    const int w = min(min(wy, min(k + 1, nz - k)), min(x + 1, nx - x));
+    const int vw = v < w ? v : w;
+    lmw = vw > lmw ? vw : lmw;
   }
   // block reduction of stats
   __shared__ long long rs[kDistThreads / 64];
-  __shared__ int rmn[kDistThreads / 64], rmx[kDistThreads / 64];
+  __shared__ int rmn[kDistThreads / 64], rmx[kDistThreads / 64], rmw[kDistThreads / 64];
   for (int o = 32; o > 0; o >>= 1) {
     lsum += __shfl_xor(lsum, o);
-    const int a = __shfl_xor(lmn, o), b = __shfl_xor(lmx, o);
+    const int a = __shfl_xor(lmn, o), b = __shfl_xor(lmx, o), c = __shfl_xor(lmw, o);
     lmn = a < lmn ? a : lmn;
     lmx = b > lmx ? b : lmx;
+    lmw = c > lmw ? c : lmw;
   }
   __syncthreads();
   if ((tid & 63) == 0) {
     rs[tid >> 6] = lsum;
     rmn[tid >> 6] = lmn;
     rmx[tid >> 6] = lmx;
+    rmw[tid >> 6] = lmw;
   }
   __syncthreads();
   if (tid == 0) {
     long long s = 0;
-    int a = INT_MAX, b = INT_MIN;
+    int a = INT_MAX, b = INT_MIN, c = INT_MIN;
     for (int w = 0; w < kDistThreads / 64; ++w) {
       s += rs[w];
       a = rmn[w] < a ? rmn[w] : a;
       b = rmx[w] > b ? rmx[w] : b;
+      c = rmw[w] > c ? rmw[w] : c;
     }
     atomicAdd(&st->sum, (unsigned long long)s);
     atomicMin(&st->mn, a);
     atomicMax(&st->mx, b);
+    atomicMax(&st->mxw, c);
   }
 }
 
@@ -348,6 +423,7 @@ k_field_stats(const int* __restrict__ phi, long long n, FieldStats* __restrict__
     atomicAdd(&st->sum, (unsigned long long)lsum);
     atomicMin(&st->mn, lmn);
     atomicMax(&st->mx, lmx);
+    atomicMax(&st->mxw, INT_MAX);   // no bound for a field of unknown meaning
   }
 }
 
@@ -738,17 +814,18 @@ constexpr int kFeasHeader = 2;
 // distance to the nearest occupied voxel): item_st->mx = rint(sqrt(e)) for
 // the largest e such that a discrete ball {|y - c|^2 <= e - 1} (L1 ball for
 // the L1 metric) lies inside the item.  A collision-free placement puts that
-// ball on free tray voxels, so the tray field at its centre is >= item_st->mx;
-// when the tray field's maximum is smaller, no orientation is feasible (the
-// ball is invariant under the cube's symmetries) and the item is recorded as
-// not found with no further work.
+// ball on free voxels inside the tray, so at its centre the distance to the
+// nearest occupied or outside voxel, min(field, wall distance), is
+// >= item_st->mx; when the maximum of that quantity (tray_st->mxw) is smaller,
+// no orientation is feasible (the ball is invariant under the cube's
+// symmetries) and the item is recorded as not found with no further work.
 __global__ void k_feas_begin(int* __restrict__ fs, int m, int mode, int* __restrict__ ctl,
                              const FieldStats* __restrict__ tray_st,
                              const FieldStats* __restrict__ item_st, PackOut* __restrict__ out,
                              int t) {
   if (ctl && ctl[CTL_HALT]) return;
   const int i = threadIdx.x;
-  const bool cert = ctl && item_st && tray_st->mx < item_st->mx;
+  const bool cert = ctl && item_st && tray_st->mxw < item_st->mx;
   if (i == 0 && ctl) {
     if (cert) {
       PackOut r{};
@@ -1150,6 +1227,7 @@ __global__ void k_stats_init(FieldStats* __restrict__ st, const int* __restrict_
   st->sum = 0ULL;
   st->mn = INT_MAX;
   st->mx = INT_MIN;
+  st->mxw = INT_MIN;
 }
 
 // ---------------------------------------------------------------------------

@@ -259,11 +259,12 @@ def test_reference_cpu_path_as_written_matches_oracle():
 
 @pytest.mark.parametrize("metric", ["edt", "l1"])
 def test_free_space_certificate_is_sound(metric):
-    """The device skips an item when the tray field's maximum (distance to the
-    nearest occupied voxel) is below the item's inscribed-ball bound (largest
-    distance from an item voxel to a voxel outside it).  Check on random trays
-    and items that the rule never rejects an item with a feasible orientation,
-    and that it does reject some."""
+    """The device skips an item when the tray's free-ball bound, the maximum
+    over voxels of min(distance to the nearest occupied voxel, distance to the
+    nearest voxel outside the tray), is below the item's inscribed-ball bound
+    (largest distance from an item voxel to a voxel outside it).  Check on
+    random trays and items that the rule never rejects an item with a feasible
+    orientation, and that it does reject some."""
     rng = np.random.default_rng(77)
     certified = 0
     for case in range(60):
@@ -276,8 +277,11 @@ def test_free_space_certificate_is_sound(metric):
         comp = np.ones(tuple(d + 2 for d in dims), np.int32)
         comp[1:-1, 1:-1, 1:-1] = (item == 0)
         theta = int(O.distance_field(comp, metric, "nonzero").max())
-        phi_max = int(O.distance_field(tray, metric, "nonzero").max())
-        if phi_max >= theta:
+        phi = O.distance_field(tray, metric, "nonzero")
+        wall = np.minimum.reduce(np.meshgrid(*[np.minimum(np.arange(n) + 1, n - np.arange(n))
+                                               for n in shape], indexing="ij"))
+        psi_max = int(np.minimum(phi, wall).max())
+        if psi_max >= theta:
             continue
         certified += 1
         for g in O.get_orientations(item, 24):
@@ -286,4 +290,4 @@ def test_free_space_certificate_is_sound(metric):
                 continue
             assert not O.search(g, tray, O.distance_field(tray, metric, "nonzero"),
                                 penalty="cube").found, (case, metric)
-    assert certified >= 3
+    assert certified >= 10
