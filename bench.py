@@ -164,6 +164,8 @@ def main(argv=None) -> int:
                          "when its density is within 0.5 pp of the plain one")
     ap.add_argument("--certificate", type=int, default=0, choices=[0, 1],
                     help="free-space certificate before the feasibility pass (exact)")
+    ap.add_argument("--opt", action="append", default=[], metavar="NAME=VALUE",
+                    help="extra context option (spk_set_option), repeatable")
     ap.add_argument("--topk", type=int, default=4)
     ap.add_argument("--min-conf", type=float, default=0.0)
     ap.add_argument("--weights", default=os.path.join(ROOT, "psacking_amd", "weights",
@@ -223,6 +225,9 @@ def main(argv=None) -> int:
         s = P.Session(0)
         s.set_option("plan", args.plan)
         s.set_option("certificate", args.certificate)
+        for kv in args.opt:
+            k, v = kv.split("=")
+            s.set_option(k, float(v))
         t0 = time.perf_counter()
         s.set_tray(np.zeros(w.tray_size, np.int32), args.metric, args.sources, 0)
         device_sync()

@@ -404,6 +404,61 @@ def place_in_tray(item, tray, position, item_id: int = 1) -> np.ndarray:
 
 
 # ---------------------------------------------------------------------------
+# coarse infeasibility test (no reference counterpart; exact pruning)
+# ---------------------------------------------------------------------------
+
+
+def coarse_core(item, s: int = 2) -> np.ndarray:
+    """Cells Y (on the s-grid of the item) whose fine block may sit anywhere
+    in the (2s-1)^3 neighbourhood of s*Y without leaving the item: K[Y] = 1
+    iff every voxel of [sY-(s-1), sY+(s-1)]^3 is an item voxel (> 0)."""
+    occ = as_grid(item) > 0
+    er = ndimage.binary_erosion(np.pad(occ, s), structure=np.ones((2 * s - 1,) * 3),
+                                border_value=0)[s:-s, s:-s, s:-s]
+    return er[::s, ::s, ::s].astype(np.int32)
+
+
+def coarse_tray(tray, s: int = 2) -> np.ndarray:
+    """1 for every s^3 block of the tray holding an occupied voxel (blocks
+    past the tray's end hold only the voxels inside it)."""
+    t = as_grid(tray) != 0
+    n = np.array(t.shape)
+    nc = -(-n // s)
+    pad = np.zeros(nc * s, bool)
+    pad[:n[0], :n[1], :n[2]] = t
+    return pad.reshape(nc[0], s, nc[1], s, nc[2], s).any(axis=(1, 3, 5)).astype(np.int32)
+
+
+def coarse_infeasible(item, tray, s: int = 2) -> bool:
+    """True only if the (oriented) item has no collision-free in-bounds offset.
+
+    A placement at q = s*Q + r (0 <= r < s per axis) covers, for every cell Y of
+    coarse_core(item), the whole tray block Q + Y, so a collision-free one
+    needs coarse_tray(tray)[Q + Y] = 0 for all those Y: a coarse correlation
+    that is positive at every coarse offset Q = floor(q / s) of a valid q
+    proves the item infeasible.  The test can miss infeasible items, never
+    the reverse."""
+    g = as_grid(item)
+    t = as_grid(tray)
+    n = np.array(t.shape)
+    lo, hi = get_bounds(g)
+    if hi[0] < 0:
+        return False
+    h = np.array(hi)
+    if np.any(h > n - 1):
+        return True
+    K = coarse_core(g, s)
+    if not K.any():
+        return False
+    B = coarse_tray(t, s)
+    qmax = (n - 1 - h) // s           # largest coarse offset of a valid fine offset
+    # core cells past the coarse tray would need item voxels past the tray
+    K = pad_to(K[:B.shape[0], :B.shape[1], :B.shape[2]], B.shape)
+    c = corr_exact(B, K)
+    return not (c[:qmax[0] + 1, :qmax[1] + 1, :qmax[2] + 1] == 0).any()
+
+
+# ---------------------------------------------------------------------------
 # interlocking-free mask
 # ---------------------------------------------------------------------------
 

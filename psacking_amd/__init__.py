@@ -14,7 +14,7 @@ try:
 except ImportError as e:  # pragma: no cover - exercised only when unbuilt
     raise ImportError(
         "psacking_amd native modules are not built (libspk.so / _core): run "
-        "`python -m psacking_amd.build`") from e
+        "`python psacking_amd/build.py`") from e
 
 from ._core import (  # noqa: F401
     HEIGHT_PENALTY,

@@ -54,7 +54,7 @@ def load() -> C.CDLL:
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
-        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m psacking_amd.build`")
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python psacking_amd/build.py`")
     lib = C.CDLL(LIB_PATH)
     lib.spk_last_error.restype = C.c_char_p
     vp, i32, i64, f32 = C.c_void_p, C.c_int, C.c_int64, C.c_float

@@ -3,7 +3,8 @@
 * libspk.so   -- HIP kernels + C-ABI (include/spk.h), hipcc for gfx950
 * _core*.so   -- pybind11 module over the C-ABI, host C++ (g++), rpath $ORIGIN
 
-Run ``python -m psacking_amd.build`` (or ``__graft_entry__.build()``).
+Run ``python psacking_amd/build.py`` (or ``__graft_entry__.build()``); it imports
+nothing from the package, so it works on a tree that has never been built.
 Always recompiles.
 """
 from __future__ import annotations

@@ -180,6 +180,21 @@ struct spk_ctx {
   int feas_mode = 1;                     // 0 off, 1 after an item that placed nothing, 2 always
   bool autotune = true;                  // time x-pencil tiles per tray shape (option "autotune")
   int tile_force = 0;                    // option "tile": 0 tuned/default, else 4, 8 or 16
+  int yz_rmax = kMaxRadix;               // option "yz_radix": largest y/z butterfly
+  int pad_y = 0, pad_z = 0;              // options "pad_y", "pad_z": padded sizes (0 = smallest smooth)
+  int feas_group = 0;                    // option "feas_group": pairs per feasibility launch pair
+  // coarse feasibility pass (2x2x2 blocks; option "coarse")
+  struct Coarse {
+    bool on = true;
+    bool ready = false;                  // planned for the current tray (slab plan only)
+    int nx = 0, ny = 0, nz = 0;          // coarse tray
+    AxisPlan px{}, py{}, pz{};
+    int T = 16;
+    DevBuf<float2> tw, t1, g1, ispec, y1;
+    DevBuf<int> grid, zero, fs, er, tab;
+    DevBuf<FieldStats> zst;
+    DevBuf<OrientDesc> od;
+  } co;
   // free-space certificate
   bool phi_internal = false;             // phi is this library's distance field of the tray
   bool certificate = false;              // option "certificate"
@@ -309,10 +324,74 @@ size_t pencil_shmem(const spk_ctx* c) {
   return (size_t)(c->px.n + c->px.n * c->T) * sizeof(float2);
 }
 
+// ---- coarse feasibility pass: plans for the 2x2x2-block tray --------------
+void plan_coarse(spk_ctx* c) {
+  spk_ctx::Coarse& q = c->co;
+  q.ready = false;
+  if (!q.on || c->split) return;
+  q.nx = (c->nx + 1) / 2;
+  q.ny = (c->ny + 1) / 2;
+  q.nz = (c->nz + 1) / 2;
+  const int px = next_smooth(q.nx), py = next_smooth(q.ny), pz = next_smooth(q.nz);
+  if ((long long)py * pz > kSlabMax || px * 16 > kPencilMax) return;
+  q.px = make_plan(px, kPencilRadix);
+  q.py = make_plan(py);
+  q.pz = make_plan(pz);
+  q.T = 16;
+  std::vector<float2> tw(px + py + pz);
+  auto fill = [&](int off, int n) {
+    for (int k = 0; k < n; ++k) {
+      const double a = -2.0 * M_PI * (double)k / (double)n;
+      tw[off + k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+  };
+  fill(0, px);
+  fill(px, py);
+  fill(px + py, pz);
+  q.tw.ensure(tw.size());
+  upload(c, q.tw.p, tw.data(), tw.size() * sizeof(float2));
+  const size_t cv = (size_t)q.nx * q.ny * q.nz;
+  q.grid.ensure(cv);
+  q.zero.ensure(cv);
+  HIP_OK(hipMemsetAsync(q.zero.p, 0, cv * sizeof(int), c->stream));
+  q.zst.ensure(1);
+  HIP_OK(hipMemsetAsync(q.zst.p, 0, sizeof(FieldStats), c->stream));
+  q.t1.ensure((size_t)q.nx * py * pz);
+  q.g1.ensure((size_t)px * py * pz);
+  q.fs.ensure(kFeasHeader + 24);
+  q.ready = true;
+}
+
+size_t coarse_slab_shmem(const spk_ctx* c) {
+  return (size_t)(c->co.py.n + c->co.pz.n + c->co.py.n * c->co.pz.n) * sizeof(float2);
+}
+size_t coarse_pencil_shmem(const spk_ctx* c) {
+  return (size_t)(c->co.px.n + c->co.px.n * c->co.T) * sizeof(float2);
+}
+
+// spectrum FFT(Bc) of the coarse tray, after every change of the tray
+void build_coarse_spectra(spk_ctx* c, const int* ctl) {
+  spk_ctx::Coarse& q = c->co;
+  if (!q.ready) return;
+  const int py = q.py.n, pz = q.pz.n;
+  const int cv = q.nx * q.ny * q.nz;
+  launch(c, "coarse_pool", k_coarse_pool, dim3(std::min(1024, (cv + 255) / 256)), dim3(256), 0,
+         (const int*)c->tray.p, c->nx, c->ny, c->nz, q.grid.p, q.nx, q.ny, q.nz, ctl);
+  // field 0 of the tray slab kernel with a zero distance field: B + i*0
+  launch(c, "coarse_slab_fwd", k_tray_slab_fwd, dim3(q.nx, 1), dim3(kSlabThreads),
+         coarse_slab_shmem(c), (const int*)q.grid.p, (const int*)q.zero.p,
+         (const FieldStats*)q.zst.p, q.t1.p, q.t1.p, q.nx, q.ny, q.nz, q.py, q.pz,
+         q.tw.p + q.px.n, q.tw.p + q.px.n + py, ctl);
+  launch(c, "coarse_x_fwd", k_tray_x_fwd, dim3((pz + q.T - 1) / q.T, py, 1),
+         dim3(kPencilThreads), coarse_pencil_shmem(c), (const float2*)q.t1.p,
+         (const float2*)q.t1.p, q.g1.p, q.g1.p, q.nx, q.px, py, pz, q.T, q.tw.p, ctl);
+}
+
 // ---- FFT planning for a tray --------------------------------------------
 void plan_tray(spk_ctx* c) {
-  const int px = next_smooth(c->nx), pz = next_smooth(c->nz);
-  int py = next_smooth(c->ny);
+  const int px = next_smooth(c->nx);
+  const int pz = c->pad_z >= c->nz && smooth(c->pad_z) ? c->pad_z : next_smooth(c->nz);
+  int py = c->pad_y >= c->ny && smooth(c->pad_y) ? c->pad_y : next_smooth(c->ny);
   if (px > kMaxAxis || py > kMaxAxis || pz > kMaxAxis)
     throw SpkError{SPK_EUNSUPPORTED, "tray axis longer than 1024 is not supported"};
   if ((long long)c->ny * c->nz > 32768)
@@ -320,8 +399,8 @@ void plan_tray(spk_ctx* c) {
   // the x axis is transformed by the x-pencil kernels, whose butterflies stop
   // at kPencilRadix (so do y pencils of the split plan, re-planned below)
   c->px = make_plan(px, kPencilRadix);
-  c->py = make_plan(py);
-  c->pz = make_plan(pz);
+  c->py = make_plan(py, c->yz_rmax);
+  c->pz = make_plan(pz, c->yz_rmax);
   auto tile = [](int n) {
     int T = 16;
     while (T > 1 && n * T > kPencilMax) T >>= 1;
@@ -367,15 +446,17 @@ void plan_tray(spk_ctx* c) {
   if (!c->lds_attrs) {
     const size_t dyn = (size_t)lds_max - 1024;   // leave room for static shared arrays
     allow_shmem(k_tray_slab_fwd, dyn);
-    allow_shmem(k_item_slab_fwd, dyn);
+    allow_shmem(k_item_slab_fwd<1>, dyn);
     allow_shmem(k_score_yz, dyn);
     allow_shmem(k_dist_zy, dyn);
     allow_shmem(k_corr_x, dyn);
     allow_shmem(k_tray_x_fwd, dyn);
     allow_shmem(k_feas_x, dyn);
     allow_shmem(k_feas_yz, dyn);
+    allow_shmem(k_item_slab_fwd<2>, dyn);
     c->lds_attrs = true;
   }
+  plan_coarse(c);
 }
 
 // ---- distance field --------------------------------------------------------
@@ -425,6 +506,7 @@ void build_spectra(spk_ctx* c, const int* ctl = nullptr) {
   launch(c, "tray_x_fwd", k_tray_x_fwd, dim3((pz + c->T - 1) / c->T, py, 2),
          dim3(kPencilThreads), pencil_shmem(c), (const float2*)c->t1.p, (const float2*)c->t2.p,
          c->g1.p, c->g2.p, c->nx, c->px, py, pz, c->T, tw, ctl);
+  build_coarse_spectra(c, ctl);
 }
 
 // Distance field (computed, or the caller's) + statistics + cached spectra.
@@ -656,7 +738,7 @@ void enqueue_item_spectra(spk_ctx* c, const OrientDesc* od, int m, int max_ex, c
            dim3(kPencilThreads), ysh, c->ispec.p, c->py, pz, c->Ty, py, py, tw + c->px.n, od,
            max_ex, (int)PENCIL_ITEM, ctl, 0);
   } else {
-    launch(c, "item_slab_fwd", k_item_slab_fwd, dim3(max_ex, m), dim3(kSlabThreads),
+    launch(c, "item_slab_fwd", k_item_slab_fwd<1>, dim3(max_ex, m), dim3(kSlabThreads),
            slab_shmem(c), (const int*)c->items.p, od, c->ispec.p, max_ex, c->py, c->pz,
            tw + c->px.n, tw + c->px.n + py, ctl);
   }
@@ -673,24 +755,87 @@ int corr_chunks(const spk_ctx* c, int m, int* chunk) {
 
 // Feasibility pass over m descriptors whose item spectra are in c->ispec
 // (slab plan only): per-orientation flags and the residual in c->fstate.
+// Pairs go through in groups of c->feas_group (0: all in one launch pair):
+// every group reuses the start of y1, so a group's intermediate can stay in
+// the last-level cache between its two kernels.
 void enqueue_feas(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const int* ctl) {
   const int py = c->py.n, pz = c->pz.n;
   const float2* tw = c->tw.p;
   const int np = (m + 1) / 2;
   const int tiles = (pz + c->T - 1) / c->T;
-  int chunk = 0;
-  const int nchunk = corr_chunks(c, np, &chunk);
   const float scale = (float)(1.0 / ((double)c->px.n * py * pz));
   if (c->timing) {
     c->units["feas_x"] += np;
     c->units["feas_yz"] += np;
   }
-  launch(c, "feas_x", k_feas_x, dim3(tiles, py, nchunk), dim3(kPencilThreads), pencil_shmem(c),
-         (const float2*)c->g1.p, (const float2*)c->ispec.p, od, c->y1.p, m, chunk, max_ex, c->nx,
-         c->px, py, pz, c->T, scale, tw, ctl);
-  launch(c, "feas_yz", k_feas_yz, dim3(c->nx, np), dim3(kSlabThreads), slab_shmem(c),
-         (const float2*)c->y1.p, od, m, c->nx, c->py, c->pz, tw + c->px.n, tw + c->px.n + py,
-         c->fstate.p, ctl);
+  const int group = c->feas_group > 0 ? c->feas_group : np;
+  for (int g0 = 0; g0 < np; g0 += group) {
+    const int gp = std::min(group, np - g0);
+    const int ob = 2 * g0, gm = std::min(m - ob, 2 * gp);
+    int chunk = 0;
+    const int nchunk = corr_chunks(c, gp, &chunk);
+    const int* cf = c->co.ready ? c->co.fs.p + kFeasHeader + ob : nullptr;
+    launch(c, "feas_x", k_feas_x, dim3(tiles, py, nchunk), dim3(kPencilThreads), pencil_shmem(c),
+           (const float2*)c->g1.p, (const float2*)c->ispec.p + (size_t)ob * max_ex * py * pz,
+           od + ob, c->y1.p, gm, chunk, max_ex, c->nx, c->px, py, pz, c->T, scale, tw, cf, 0,
+           ctl);
+    launch(c, "feas_yz", k_feas_yz, dim3(c->nx, gp), dim3(kSlabThreads), slab_shmem(c),
+           (const float2*)c->y1.p, od + ob, gm, c->nx, c->py, c->pz, tw + c->px.n,
+           tw + c->px.n + py, c->fstate.p, ob, cf, 0, ctl);
+  }
+}
+
+// Coarse feasibility pass over the m descriptors od (device), before the
+// full-resolution pass: coarse descriptors, spectra of the items' eroded
+// cores sampled every second voxel, collision-only correlation with FFT(Bc),
+// then k_coarse_decide (prune pairs, or skip the full pass).  The eroded
+// items (k_erode_items) must be in c->co.er.  ctl must enable the pass.
+void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const int* ctl) {
+  spk_ctx::Coarse& q = c->co;
+  if (!q.ready) return;
+  const int py = q.py.n, pz = q.pz.n;
+  const int mx = (max_ex + 1) / 2;
+  const int np = (m + 1) / 2;
+  q.od.ensure(24);
+  q.ispec.ensure((size_t)m * mx * py * pz);
+  q.y1.ensure((size_t)np * q.nx * py * pz);
+  launch(c, "coarse_od", k_coarse_od, dim3(1), dim3(64), 0, od, m, q.od.p, ctl);
+  launch(c, "coarse_item_fwd", k_item_slab_fwd<2>, dim3(mx, m), dim3(kSlabThreads),
+         coarse_slab_shmem(c), (const int*)q.er.p, (const OrientDesc*)q.od.p, q.ispec.p, mx,
+         q.py, q.pz, q.tw.p + q.px.n, q.tw.p + q.px.n + py, ctl);
+  const int tiles = (pz + q.T - 1) / q.T;
+  const int nchunk = std::max(1, std::min(np, (2048 + tiles * py - 1) / (tiles * py)));
+  const int chunk = (np + nchunk - 1) / nchunk;
+  const float scale = (float)(1.0 / ((double)q.px.n * py * pz));
+  launch(c, "coarse_feas_x", k_feas_x, dim3(tiles, py, (np + chunk - 1) / chunk),
+         dim3(kPencilThreads), coarse_pencil_shmem(c), (const float2*)q.g1.p,
+         (const float2*)q.ispec.p, (const OrientDesc*)q.od.p, q.y1.p, m, chunk, mx, q.nx, q.px,
+         py, pz, q.T, scale, (const float2*)q.tw.p, (const int*)nullptr, 1, ctl);
+  launch(c, "coarse_feas_yz", k_feas_yz, dim3(q.nx, np), dim3(kSlabThreads),
+         coarse_slab_shmem(c), (const float2*)q.y1.p, (const OrientDesc*)q.od.p, m, q.nx, q.py,
+         q.pz, (const float2*)q.tw.p + q.px.n, (const float2*)q.tw.p + q.px.n + py, q.fs.p, 0,
+         (const int*)nullptr, 1, ctl);
+  launch(c, "coarse_decide", k_coarse_decide, dim3(1), dim3(64), 0, (const int*)q.fs.p, m,
+         c->fstate.p, c->resid_limit, (int*)ctl);
+}
+
+// eroded copies of the loaded items for the coarse pass (same layout)
+void erode_items(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector<int>& off) {
+  spk_ctx::Coarse& q = c->co;
+  if (!q.ready || bases.empty()) return;
+  size_t tot = 0;
+  std::vector<int> tab(4 * bases.size());
+  for (size_t t = 0; t < bases.size(); ++t) {
+    tab[4 * t] = off[t];
+    for (int a = 0; a < 3; ++a) tab[4 * t + 1 + a] = bases[t].dims[a];
+    tot = std::max(tot, (size_t)off[t] + (size_t)bases[t].dims[0] * bases[t].dims[1] *
+                                             bases[t].dims[2]);
+  }
+  q.er.ensure(std::max<size_t>(tot, 1));
+  q.tab.ensure(tab.size());
+  upload(c, q.tab.p, tab.data(), tab.size() * sizeof(int));
+  launch(c, "erode_items", k_erode_items, dim3((unsigned)bases.size()), dim3(256), 0,
+         (const int*)c->items.p, (const int*)q.tab.p, q.er.p);
 }
 
 // Scored search (pass A, pass B, selection, exact re-check of the winner) of
@@ -1005,9 +1150,10 @@ void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e
   if (feas_mode < 0) feas_mode = c->feas_mode;
   const bool feas = feas_mode && !c->split;
   if (feas || item_bound) {
-    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, m,
-           feas ? feas_mode : 0, c->ctl.p, (const FieldStats*)c->stats.p, item_bound, c->pout.p,
-           t);
+    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p,
+           c->co.ready ? c->co.fs.p : (int*)nullptr, m, feas ? feas_mode : 0, c->ctl.p,
+           (const FieldStats*)c->stats.p, item_bound, c->pout.p, t);
+    if (feas) enqueue_coarse(c, od, m, max_ex, c->ctl.p);
     enqueue_item_spectra(c, od, m, max_ex, c->ctl.p);
     if (feas) {
       enqueue_feas(c, od, m, max_ex, c->ctl.p);
@@ -1290,9 +1436,13 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     int max_ex, max_e;
     const OrientDesc* od = gather(pl, list, c->od_rest.p, &max_ex, &max_e);
     const int k = (int)list.size();
-    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, k, 2, (int*)nullptr,
+    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p,
+           c->co.ready ? c->co.fs.p : (int*)nullptr, k, 2, (int*)nullptr,
            (const FieldStats*)nullptr, (const FieldStats*)nullptr, (PackOut*)nullptr, 0);
-    enqueue_item_spectra(c, od, k, max_ex, nullptr);
+    // the coarse pass writes CTL_PRUNE / CTL_COARSE_DONE of ctl_on; without
+    // it they stay 0 there
+    enqueue_coarse(c, od, k, max_ex, c->ctl_on.p);
+    enqueue_item_spectra(c, od, k, max_ex, c->co.ready ? c->ctl_on.p : nullptr);
     enqueue_feas(c, od, k, max_ex, c->ctl_on.p);
     download(c, fsv, c->fstate.p, (kFeasHeader + k) * sizeof(int));
     sync(c);
@@ -1593,6 +1743,7 @@ void load_items(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_ite
     const int it = order ? order[t] : t;
     if (it < 0 || it >= n_items) throw SpkError{SPK_EINVAL, "order index out of range"};
   }
+  erode_items(c, bases, off);
 }
 
 }  // namespace
@@ -2150,6 +2301,32 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->tile_force = v;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "yz_radix" || std::string(name) == "pad_y" ||
+      std::string(name) == "pad_z") {
+    const int v = (int)value;
+    if (std::string(name) == "yz_radix") {
+      if (v < 2 || v > kMaxRadix) return fail(SPK_EINVAL, "yz_radix must lie in [2, 25]");
+      c->yz_rmax = v;
+    } else {
+      if (v < 0 || v > kMaxAxis || (v && !smooth(v)))
+        return fail(SPK_EINVAL, "padded size must be 0 or a 2^a 3^b 5^c 7^d length <= 1024");
+      (std::string(name) == "pad_y" ? c->pad_y : c->pad_z) = v;
+    }
+    c->tw.release();   // re-plan at the next spk_tray_set
+    c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "coarse") {
+    c->co.on = value != 0.0;
+    c->tw.release();   // re-plan at the next spk_tray_set
+    c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "feas_group") {
+    if (value < 0 || value > 12) return fail(SPK_EINVAL, "feas_group must lie in [0, 12]");
+    c->feas_group = (int)value;
     return SPK_OK;
   }
   if (std::string(name) == "certificate") {

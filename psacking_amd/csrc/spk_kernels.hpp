@@ -101,10 +101,12 @@ struct SearchOut {           // written by k_select, read by verify/place
 // a nullable pointer: outside that loop they always run.
 // CTL_SKIP_SEARCH: the feasibility pass found no feasible orientation, so the
 // item's scored search is skipped; CTL_FEAS: the feasibility pass runs for
-// the current item.
+// the current item; CTL_PRUNE: the coarse pass ran and the full-resolution
+// pass may skip orientation pairs it proved infeasible; CTL_COARSE_DONE: the
+// coarse pass proved every orientation infeasible (no full-resolution pass).
 enum {
   CTL_HALT = 0, CTL_HALT_ITEM = 1, CTL_SKIP_REFRESH = 2, CTL_PLACED = 3, CTL_SKIP_SEARCH = 4,
-  CTL_FEAS = 5, CTL_WORDS = 6
+  CTL_FEAS = 5, CTL_PRUNE = 6, CTL_COARSE_DONE = 7, CTL_WORDS = 8
 };
 // search kernels stop once a search has been handed back to the host, and
 // for an item the feasibility pass has already decided
@@ -118,6 +120,10 @@ __device__ __forceinline__ bool feas_off(const int* ctl) {
 // refresh kernels also stop when the last item placed nothing
 __device__ __forceinline__ bool refresh_off(const int* ctl) {
   return ctl && (ctl[CTL_HALT] | ctl[CTL_SKIP_REFRESH]);
+}
+// full-resolution work made unnecessary by the coarse pass
+__device__ __forceinline__ bool coarse_done(const int* ctl) {
+  return ctl && ctl[CTL_COARSE_DONE];
 }
 
 struct PackOut {             // one item of the greedy loop, as the device decided it
@@ -548,12 +554,16 @@ k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
 // ---------------------------------------------------------------------------
 // item spectra: grid (max_ex, n_orient).  I~[o][x][ky][kz] = FFT_yz(item_o[x])
 // ---------------------------------------------------------------------------
+// SUB = 1: the oriented items; SUB = 2: the coarse cores (items holds the
+// eroded items, descriptors carry coarse extents, voxel 2*(x, j, k) is read),
+// run by the coarse feasibility pass.
+template <int SUB>
 __global__ void __launch_bounds__(kSlabThreads)
 k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od,
                 float2* __restrict__ ispec, int max_ex, AxisPlan py_, AxisPlan pz_,
                 const float2* __restrict__ tw_y, const float2* __restrict__ tw_z,
                 const int* __restrict__ ctl) {
-  if (search_off(ctl)) return;
+  if (SUB == 1 ? (search_off(ctl) || coarse_done(ctl)) : feas_off(ctl)) return;
   extern __shared__ float2 smem[];
   const int py = py_.n, pz = pz_.n;
   const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
@@ -572,7 +582,8 @@ k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od
     for (int s = 0; s < kSlabMaxEl; ++s) {
       const int e = tid + s * kSlabThreads;
       const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
-      iv[s] = (e < py * pz && j < d.ey && k < d.ez) ? base[oriented_index(d, x, j, k)] : 0;
+      iv[s] = (e < py * pz && j < d.ey && k < d.ez)
+                  ? base[oriented_index(d, SUB * x, SUB * j, SUB * k)] : 0;
     }
 #pragma unroll
     for (int s = 0; s < kSlabMaxEl; ++s) {
@@ -819,14 +830,16 @@ constexpr int kFeasHeader = 2;
 // >= item_st->mx; when the maximum of that quantity (tray_st->mxw) is smaller,
 // no orientation is feasible (the ball is invariant under the cube's
 // symmetries) and the item is recorded as not found with no further work.
-__global__ void k_feas_begin(int* __restrict__ fs, int m, int mode, int* __restrict__ ctl,
-                             const FieldStats* __restrict__ tray_st,
+__global__ void k_feas_begin(int* __restrict__ fs, int* __restrict__ cfs, int m, int mode,
+                             int* __restrict__ ctl, const FieldStats* __restrict__ tray_st,
                              const FieldStats* __restrict__ item_st, PackOut* __restrict__ out,
                              int t) {
   if (ctl && ctl[CTL_HALT]) return;
   const int i = threadIdx.x;
   const bool cert = ctl && item_st && tray_st->mxw < item_st->mx;
   if (i == 0 && ctl) {
+    ctl[CTL_PRUNE] = 0;
+    ctl[CTL_COARSE_DONE] = 0;
     if (cert) {
       PackOut r{};
       r.so.found = 0;
@@ -840,7 +853,10 @@ __global__ void k_feas_begin(int* __restrict__ fs, int m, int mode, int* __restr
       ctl[CTL_FEAS] = mode == 2 ? 1 : (mode == 1 ? ctl[CTL_SKIP_REFRESH] : 0);
     }
   }
-  for (int j = i; j < kFeasHeader + m; j += blockDim.x) fs[j] = 0;
+  for (int j = i; j < kFeasHeader + m; j += blockDim.x) {
+    fs[j] = 0;
+    if (cfs) cfs[j] = 0;
+  }
 }
 
 // padded complement of an item: out (ix+2, iy+2, iz+2) = 1 where the item has
@@ -863,8 +879,11 @@ __global__ void __launch_bounds__(kPencilThreads)
 k_feas_x(const float2* __restrict__ g1, const float2* __restrict__ ispec,
          const OrientDesc* __restrict__ od, float2* __restrict__ y1, int m, int chunk, int max_ex,
          int nx, AxisPlan px_, int py, int pz, int T, float scale, const float2* __restrict__ tw_x,
-         const int* __restrict__ ctl) {
-  if (feas_off(ctl)) return;
+         const int* __restrict__ cf, int coarse_pass, const int* __restrict__ ctl) {
+  if (feas_off(ctl) || (!coarse_pass && coarse_done(ctl))) return;
+  // pairs the coarse pass proved infeasible in both orientations are skipped
+  // here and in k_feas_yz (cf: coarse flags of this launch's orientations)
+  const bool prune = !coarse_pass && cf && ctl && ctl[CTL_PRUNE];
   extern __shared__ float2 smem[];
   const int px = px_.n;
   float2* twx = smem;
@@ -893,6 +912,7 @@ k_feas_x(const float2* __restrict__ g1, const float2* __restrict__ ispec,
   for (int p = p0; p < p1; ++p) {
     const int oa = 2 * p, ob = 2 * p + 1;
     const bool hasb = ob < m;
+    if (prune && !cf[oa] && !(hasb && cf[ob])) continue;
     const OrientDesc da = od[oa];
     const OrientDesc db = od[hasb ? ob : oa];
     const float2* sa = ispec + (size_t)oa * max_ex * py * pz;
@@ -932,17 +952,20 @@ k_feas_x(const float2* __restrict__ g1, const float2* __restrict__ ispec,
 }
 
 // pass B: grid (nx, n_pairs).  Inverse yz of one x slice of a pair, then
-// flags orientations with a zero-collision valid offset in it.
+// flags orientations with a zero-collision valid offset in it (od and the
+// pairs start at orientation o_base of the item's list).
 __global__ void __launch_bounds__(kSlabThreads)
 k_feas_yz(const float2* __restrict__ y1, const OrientDesc* __restrict__ od, int m, int nx,
           AxisPlan py_, AxisPlan pz_, const float2* __restrict__ tw_y,
-          const float2* __restrict__ tw_z, int* __restrict__ fs, const int* __restrict__ ctl) {
-  if (feas_off(ctl)) return;
+          const float2* __restrict__ tw_z, int* __restrict__ fs, int o_base,
+          const int* __restrict__ cf, int coarse_pass, const int* __restrict__ ctl) {
+  if (feas_off(ctl) || (!coarse_pass && coarse_done(ctl))) return;
   extern __shared__ float2 smem[];
   const int py = py_.n, pz = pz_.n;
   const int x = blockIdx.x, p = blockIdx.y, tid = threadIdx.x;
   const int oa = 2 * p, ob = 2 * p + 1;
   const bool hasb = ob < m;
+  if (!coarse_pass && cf && ctl && ctl[CTL_PRUNE] && !cf[oa] && !(hasb && cf[ob])) return;
   const OrientDesc da = od[oa];
   const OrientDesc db = od[hasb ? ob : oa];
   const bool xa = x < da.nvx, xb = hasb && x < db.nvx;
@@ -988,8 +1011,86 @@ k_feas_yz(const float2* __restrict__ y1, const OrientDesc* __restrict__ od, int 
   if (tid == 0) {
     for (int w = 1; w < kSlabThreads / 64; ++w) res = fmaxf(res, wr[w]);
     atomicMax(&fs[0], __float_as_int(res));
-    if (fa) fs[kFeasHeader + oa] = 1;
-    if (fb) fs[kFeasHeader + ob] = 1;
+    if (fa) fs[kFeasHeader + o_base + oa] = 1;
+    if (fb) fs[kFeasHeader + o_base + ob] = 1;
+  }
+}
+
+// Coarse feasibility pass, after its k_feas_yz: cfs holds its residual and
+// flags.  Untrusted (residual above resid_limit): nothing is pruned.  No
+// orientation coarse-feasible: every orientation is infeasible, the
+// full-resolution pass is skipped and fs[0] takes the coarse residual, so
+// that the pass's reader (k_feas_decide or the host) sees a trusted result
+// with no feasible orientation.  Otherwise the full pass prunes pairs.
+__global__ void k_coarse_decide(const int* __restrict__ cfs, int m, int* __restrict__ fs,
+                                float resid_limit, int* __restrict__ ctl) {
+  if (threadIdx.x != 0 || feas_off(ctl)) return;
+  const float res = __int_as_float(cfs[0]);
+  int any = 0;
+  for (int o = 0; o < m; ++o) any |= cfs[kFeasHeader + o];
+  const bool trusted = res <= resid_limit;
+  ctl[CTL_PRUNE] = trusted && any ? 1 : 0;
+  ctl[CTL_COARSE_DONE] = trusted && !any ? 1 : 0;
+  if (trusted && !any) fs[0] = __float_as_int(res);
+}
+
+// Coarse descriptors from full-resolution ones: extents ceil(e / 2) (the
+// core's cells), valid coarse offsets floor((nv - 1) / 2) + 1.
+__global__ void k_coarse_od(const OrientDesc* __restrict__ od, int m,
+                            OrientDesc* __restrict__ out, const int* __restrict__ ctl) {
+  const int o = threadIdx.x;
+  if (o >= m || feas_off(ctl)) return;
+  OrientDesc d = od[o];
+  d.ex = (d.ex + 1) / 2;
+  d.ey = (d.ey + 1) / 2;
+  d.ez = (d.ez + 1) / 2;
+  d.nvx = d.nvx > 0 ? (d.nvx - 1) / 2 + 1 : 0;
+  d.nvy = d.nvy > 0 ? (d.nvy - 1) / 2 + 1 : 0;
+  d.nvz = d.nvz > 0 ? (d.nvz - 1) / 2 + 1 : 0;
+  out[o] = d;
+}
+
+// Coarse tray: 1 for every 2x2x2 block holding an occupied voxel.
+__global__ void __launch_bounds__(256)
+k_coarse_pool(const int* __restrict__ tray, int nx, int ny, int nz, int* __restrict__ out,
+              int cx, int cy, int cz, const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
+  const int n = cx * cy * cz;
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
+    const int k = e % cz, j = (e / cz) % cy, i = e / (cy * cz);
+    int v = 0;
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b)
+        for (int c = 0; c < 2; ++c) {
+          const int x = 2 * i + a, y = 2 * j + b, z = 2 * k + c;
+          if (x < nx && y < ny && z < nz && tray[((size_t)x * ny + y) * nz + z] != 0) v = 1;
+        }
+    out[e] = v;
+  }
+}
+
+// Items eroded by a 3x3x3 block: er[v] = 1 iff every voxel within distance 1
+// (per axis) of v is an item voxel (> 0); voxels outside the item's box are
+// empty.  One block per item; tab[4*i .. 4*i+3] = offset, dims.
+__global__ void __launch_bounds__(256)
+k_erode_items(const int* __restrict__ items, const int* __restrict__ tab,
+              int* __restrict__ er) {
+  const int it = blockIdx.x;
+  const int off = tab[4 * it], bx = tab[4 * it + 1], by = tab[4 * it + 2], bz = tab[4 * it + 3];
+  const int* src = items + off;
+  const int n = bx * by * bz;
+  for (int e = threadIdx.x; e < n; e += 256) {
+    const int z = e % bz, y = (e / bz) % by, x = e / (by * bz);
+    int v = 1;
+    for (int a = -1; a <= 1 && v; ++a)
+      for (int b = -1; b <= 1 && v; ++b)
+        for (int c = -1; c <= 1 && v; ++c) {
+          const int X = x + a, Y = y + b, Z = z + c;
+          if (X < 0 || Y < 0 || Z < 0 || X >= bx || Y >= by || Z >= bz ||
+              src[((size_t)X * by + Y) * bz + Z] <= 0)
+            v = 0;
+        }
+    er[off + e] = v;
   }
 }
 

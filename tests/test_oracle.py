@@ -291,3 +291,29 @@ def test_free_space_certificate_is_sound(metric):
             assert not O.search(g, tray, O.distance_field(tray, metric, "nonzero"),
                                 penalty="cube").found, (case, metric)
     assert certified >= 10
+
+
+def test_coarse_infeasibility_is_sound():
+    """The coarse test (2x2x2 blocks: occupied-any tray, eroded item core)
+    never rejects an orientation that has a collision-free offset, and it
+    does reject some infeasible ones."""
+    rng = np.random.default_rng(91)
+    proved = infeasible = 0
+    for case in range(140):
+        shape = tuple(int(v) for v in rng.integers(9, 15, 3))
+        tray = (rng.random(shape) < rng.uniform(0.02, 0.25)).astype(np.int32)
+        # a solid slab from the floor leaves a free layer of random depth
+        tray[:, :, :int(rng.integers(0, shape[2] - 3))] = 2
+        dims = tuple(int(v) for v in rng.integers(3, 9, 3))
+        item = np.ones(dims, np.int32)
+        item[rng.random(dims) < 0.1] = 0
+        for g in O.get_orientations(item, 24)[:6]:
+            g = O.as_grid(g)
+            if any(g.shape[a] > shape[a] for a in range(3)):
+                continue
+            c = O.coarse_infeasible(g, tray)
+            found = O.search(g, tray, O.distance_field(tray, "edt", "nonzero"), penalty="cube").found
+            assert not (c and found), case
+            infeasible += not found
+            proved += c
+    assert proved >= 10 and infeasible > proved
