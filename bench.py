@@ -41,7 +41,9 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = ["dist_zy", "dist_x", "tray_slab_fwd", "tray_rows_fwd", "tray_x_fwd", "item_slab_fwd",
            "item_rows_fwd", "pencil_y_fwd", "corr_x", "score_yz", "pencil_y_inv", "rows_score",
            "select", "verify", "decide", "place", "direct_search", "gather_od", "proposer",
-           "proposer_items", "record", "feas_begin", "feas_x", "feas_yz", "feas_decide"]
+           "proposer_items", "record", "feas_begin", "feas_x", "feas_yz", "feas_decide",
+           "coarse_od", "coarse_item_fwd", "coarse_feas_x", "coarse_feas_yz", "coarse_decide",
+           "coarse_pool", "coarse_slab_fwd", "coarse_x_fwd", "erode_items"]
 
 
 def device_sync():
@@ -62,10 +64,13 @@ def kernel_bytes(name: str, Ph: int, V: int, m: float) -> float:
     Feasibility pass, per orientation pair (m = pairs): the x pass reads the
     tray spectrum once (16*Ph) and writes one complex64 product per pair
     (16*Ph); the yz pass reads it back.
+    Coarse pass (2x2x2 blocks): the same per pair on the coarse tray, whose
+    Ph the caller passes.  Pairs are counted on the device (pruned pairs are
+    not transformed).
     """
-    if name in ("score_yz", "feas_yz"):
+    if name in ("score_yz", "feas_yz", "coarse_feas_yz"):
         return 16.0 * Ph * m
-    if name in ("corr_x", "feas_x"):
+    if name in ("corr_x", "feas_x", "coarse_feas_x"):
         return 16.0 * Ph + 16.0 * Ph * m
     raise KeyError(name)
 
@@ -254,7 +259,8 @@ def main(argv=None) -> int:
     def finish(name, s, dt, res, t_setup, searched):
         t = s.tray()
         kt = {n: s.timing_get(n) for n in KERNELS}
-        units = {n: s.timing_units(n) for n in ("corr_x", "score_yz", "feas_x", "feas_yz")}
+        units = {n: s.timing_units(n) for n in ("corr_x", "score_yz", "feas_x", "feas_yz",
+                                                 "coarse_feas_x", "coarse_feas_yz")}
         runs[name] = {"dt": dt, "res": res, "density": float((t > 0).sum()) / V, "kt": kt,
                       "units": units,
                       "setup": t_setup, "searched": searched,
@@ -354,11 +360,14 @@ def main(argv=None) -> int:
     else:
         used = "plain" if "plain" in runs else "learned"
     r = runs[used]
-    dom = max(("score_yz", "corr_x", "feas_x", "feas_yz"), key=lambda n: r["kt"][n][0])
+    cands = ("score_yz", "corr_x", "feas_x", "feas_yz", "coarse_feas_x", "coarse_feas_yz")
+    dom = max(cands, key=lambda n: r["kt"][n][0])
     ms_dom, n_dom = r["kt"][dom]
     avg_s = (ms_dom / max(1, n_dom)) / 1e3
     m_avg = r["units"][dom] / max(1, n_dom)
-    achieved = kernel_bytes(dom, Ph, V, m_avg) / avg_s / 1e9 if avg_s > 0 else 0.0
+    ph_dom = Ph if not dom.startswith("coarse") else \
+        ((Nx + 1) // 2) * ((Ny + 1) // 2) * (((Nz + 1) // 2) // 2 + 1)
+    achieved = kernel_bytes(dom, ph_dom, V, m_avg) / avg_s / 1e9 if avg_s > 0 else 0.0
     # measured HBM bytes per launch of this kernel from the committed counter
     # runs (FETCH_SIZE/WRITE_SIZE scaled by a copy kernel), scaled to this
     # run's units per launch
@@ -423,7 +432,7 @@ def main(argv=None) -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "algorithmic_bytes": kernel_bytes(dom, Ph, V, m_avg),
+            "algorithmic_bytes": kernel_bytes(dom, ph_dom, V, m_avg),
             "avg_launch_us": avg_s * 1e6,
             "units_per_launch": m_avg,
             "unit_of_work": "orientation pair" if dom.startswith("feas") else "orientation",

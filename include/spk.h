@@ -31,6 +31,7 @@
  *                         reading, voxelize (packing.cpp:526-588) and
  *                         make_voxel_grid_tight (voxelGrid.cpp:166-178)
  *   spk_voxelize_mesh     voxelize (packing.cpp:526-588) on an in-memory mesh
+ *   spk_feasible          (none: the loops' feasibility test as a query)
  */
 #ifndef SPK_H
 #define SPK_H
@@ -102,6 +103,20 @@ int spk_search(spk_ctx* ctx, const int32_t* const* items, const int32_t* dims, i
 int spk_search_oriented(spk_ctx* ctx, const int32_t* item, int ix, int iy, int iz,
                         const spk_orient* orients, int n, int penalty,
                         int interlocking_free, spk_result* best, spk_result* per_orient);
+
+/* Which orientations of an item fit the context's tray without collision
+ * (no reference counterpart: the feasibility test the greedy loops use to
+ * skip items that fit nowhere).  full[o] = 1 when orientation o has a valid
+ * offset with no occupied tray voxel under the item (the full-resolution
+ * collision pass, FP32 FFT rounded to integers; *residual = its largest
+ * rounding distance, to be compared with option "resid_limit").
+ * column[o] (may be NULL) = 0 when the column test already excludes o: the
+ * tray reduced to 2x2 column blocks at full height against the item's
+ * x-y-eroded core (exact bit-mask test, tray z <= 128; 1 everywhere when it
+ * does not apply).  n <= 24; needs the slab plan. */
+int spk_feasible(spk_ctx* ctx, const int32_t* item, int ix, int iy, int iz,
+                 const spk_orient* orients, int n, int32_t* column, int32_t* full,
+                 float* residual);
 
 /* Write id into the cached tray where the (oriented) item is > 0 and the tray
  * is 0, then refresh the distance field and spectra.  SPK_EOVERLAP (tray left

@@ -429,6 +429,58 @@ def coarse_tray(tray, s: int = 2) -> np.ndarray:
     return pad.reshape(nc[0], s, nc[1], s, nc[2], s).any(axis=(1, 3, 5)).astype(np.int32)
 
 
+def coarse_columns(tray) -> np.ndarray:
+    """Occupancy of the tray's 2x2 column blocks at full z resolution:
+    C[X, Y, z] = 1 if any voxel (2X+u, 2Y+v, z), u, v in {0, 1}, inside the
+    tray is occupied."""
+    t = as_grid(tray) != 0
+    nx, ny, nz = t.shape
+    pad = np.zeros((nx + nx % 2, ny + ny % 2, nz), bool)
+    pad[:nx, :ny] = t
+    return pad.reshape(pad.shape[0] // 2, 2, pad.shape[1] // 2, 2, nz).any(axis=(1, 3))
+
+
+def column_core(item) -> np.ndarray:
+    """K[X, Y, z] = 1 iff the item has a voxel (> 0) at (2X+dx, 2Y+dy, z) for
+    every dx, dy in {-1, 0, 1} (an x-y erosion by a 3x3 block, sampled at even
+    x and y)."""
+    occ = as_grid(item) > 0
+    er = ndimage.binary_erosion(np.pad(occ, ((1, 1), (1, 1), (0, 0))),
+                                structure=np.ones((3, 3, 1)), border_value=0)[1:-1, 1:-1, :]
+    return er[::2, ::2, :]
+
+
+def column_infeasible(item, tray) -> bool:
+    """True only if the (oriented) item has no collision-free in-bounds offset.
+
+    A placement at (2*Qx + rx, 2*Qy + ry, qz), rx, ry in {0, 1}, covers for
+    every cell (X, Y, z) of column_core(item) the whole column block
+    (Qx + X, Qy + Y) at height qz + z, so a collision-free one needs
+    coarse_columns(tray)[Qx + X, Qy + Y, qz + z] = 0 there.  If every
+    (Qx, Qy, qz) of a valid offset meets an occupied block, the item fits
+    nowhere.  Never rejects a feasible item; may miss infeasible ones."""
+    g = as_grid(item)
+    t = as_grid(tray)
+    n = np.array(t.shape)
+    lo, hi = get_bounds(g)
+    if hi[0] < 0:
+        return False
+    h = np.array(hi)
+    if np.any(h > n - 1):
+        return True
+    K = column_core(g)
+    if not K.any():
+        return False
+    C = coarse_columns(t).astype(np.int64)
+    qx, qy = (n[0] - 1 - h[0]) // 2, (n[1] - 1 - h[1]) // 2
+    qz = n[2] - 1 - h[2]
+    Kp = np.zeros(C.shape, np.int64)
+    kx, ky, kz = (min(K.shape[a], C.shape[a]) for a in range(3))
+    Kp[:kx, :ky, :kz] = K[:kx, :ky, :kz]
+    c = corr_exact(C, Kp)
+    return not (c[:qx + 1, :qy + 1, :qz + 1] == 0).any()
+
+
 def coarse_infeasible(item, tray, s: int = 2) -> bool:
     """True only if the (oriented) item has no collision-free in-bounds offset.
 

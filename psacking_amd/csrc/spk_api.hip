@@ -183,17 +183,13 @@ struct spk_ctx {
   int yz_rmax = kMaxRadix;               // option "yz_radix": largest y/z butterfly
   int pad_y = 0, pad_z = 0;              // options "pad_y", "pad_z": padded sizes (0 = smallest smooth)
   int feas_group = 0;                    // option "feas_group": pairs per feasibility launch pair
-  // coarse feasibility pass (2x2x2 blocks; option "coarse")
+  // column test before the feasibility pass (option "coarse")
   struct Coarse {
     bool on = true;
-    bool ready = false;                  // planned for the current tray (slab plan only)
-    int nx = 0, ny = 0, nz = 0;          // coarse tray
-    AxisPlan px{}, py{}, pz{};
-    int T = 16;
-    DevBuf<float2> tw, t1, g1, ispec, y1;
-    DevBuf<int> grid, zero, fs, er, tab;
-    DevBuf<FieldStats> zst;
-    DevBuf<OrientDesc> od;
+    bool ready = false;                  // tray z <= 128 and the slab plan
+    int cx = 0, cy = 0;                  // 2x2 column blocks of the tray
+    DevBuf<M128> cols, cm;               // block masks, core masks of a search
+    DevBuf<int> fs;                      // residual slot (0), per-orientation flags
   } co;
   // free-space certificate
   bool phi_internal = false;             // phi is this library's distance field of the tray
@@ -206,6 +202,9 @@ struct spk_ctx {
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, long long>> acc;
   std::map<std::string, long long> units;   // orientations (pairs) handed to each kernel
+  DevBuf<unsigned long long> pairs;          // feasibility pairs transformed: full, coarse
+  Mailbox* mail_h = nullptr;                 // pinned host mailbox and its device address
+  Mailbox* mail_d = nullptr;
   // host-side copy of item buffer bookkeeping
   std::vector<int> h_items;
 
@@ -324,67 +323,39 @@ size_t pencil_shmem(const spk_ctx* c) {
   return (size_t)(c->px.n + c->px.n * c->T) * sizeof(float2);
 }
 
-// ---- coarse feasibility pass: plans for the 2x2x2-block tray --------------
+// ---- host mailbox ------------------------------------------------------------
+// Gathers the listed device words into the pinned mailbox and waits for the
+// stream; the host then reads c->mail_h.
+const Mailbox& post_and_sync(spk_ctx* c, const int* ctl, const PackOut* po, const FieldStats* st,
+                             const float* logits, const int* fs, int nfs) {
+  if (!c->mail_h) {
+    HIP_OK(hipHostMalloc((void**)&c->mail_h, sizeof(Mailbox), hipHostMallocMapped));
+    HIP_OK(hipHostGetDevicePointer((void**)&c->mail_d, c->mail_h, 0));
+  }
+  launch(c, "post", k_post, dim3(1), dim3(64), 0, ctl, po, st, logits, fs, nfs, c->mail_d);
+  sync(c);
+  return *c->mail_h;
+}
+
+// ---- column test: 2x2 column blocks of the tray ----------------------------
 void plan_coarse(spk_ctx* c) {
   spk_ctx::Coarse& q = c->co;
   q.ready = false;
-  if (!q.on || c->split) return;
-  q.nx = (c->nx + 1) / 2;
-  q.ny = (c->ny + 1) / 2;
-  q.nz = (c->nz + 1) / 2;
-  const int px = next_smooth(q.nx), py = next_smooth(q.ny), pz = next_smooth(q.nz);
-  if ((long long)py * pz > kSlabMax || px * 16 > kPencilMax) return;
-  q.px = make_plan(px, kPencilRadix);
-  q.py = make_plan(py);
-  q.pz = make_plan(pz);
-  q.T = 16;
-  std::vector<float2> tw(px + py + pz);
-  auto fill = [&](int off, int n) {
-    for (int k = 0; k < n; ++k) {
-      const double a = -2.0 * M_PI * (double)k / (double)n;
-      tw[off + k] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
-  };
-  fill(0, px);
-  fill(px, py);
-  fill(px + py, pz);
-  q.tw.ensure(tw.size());
-  upload(c, q.tw.p, tw.data(), tw.size() * sizeof(float2));
-  const size_t cv = (size_t)q.nx * q.ny * q.nz;
-  q.grid.ensure(cv);
-  q.zero.ensure(cv);
-  HIP_OK(hipMemsetAsync(q.zero.p, 0, cv * sizeof(int), c->stream));
-  q.zst.ensure(1);
-  HIP_OK(hipMemsetAsync(q.zst.p, 0, sizeof(FieldStats), c->stream));
-  q.t1.ensure((size_t)q.nx * py * pz);
-  q.g1.ensure((size_t)px * py * pz);
+  if (!q.on || c->split || c->nz > 128) return;
+  q.cx = (c->nx + 1) / 2;
+  q.cy = (c->ny + 1) / 2;
+  q.cols.ensure((size_t)q.cx * q.cy);
   q.fs.ensure(kFeasHeader + 24);
   q.ready = true;
 }
 
-size_t coarse_slab_shmem(const spk_ctx* c) {
-  return (size_t)(c->co.py.n + c->co.pz.n + c->co.py.n * c->co.pz.n) * sizeof(float2);
-}
-size_t coarse_pencil_shmem(const spk_ctx* c) {
-  return (size_t)(c->co.px.n + c->co.px.n * c->co.T) * sizeof(float2);
-}
-
-// spectrum FFT(Bc) of the coarse tray, after every change of the tray
+// block masks of the tray, after every change of the tray
 void build_coarse_spectra(spk_ctx* c, const int* ctl) {
   spk_ctx::Coarse& q = c->co;
   if (!q.ready) return;
-  const int py = q.py.n, pz = q.pz.n;
-  const int cv = q.nx * q.ny * q.nz;
-  launch(c, "coarse_pool", k_coarse_pool, dim3(std::min(1024, (cv + 255) / 256)), dim3(256), 0,
-         (const int*)c->tray.p, c->nx, c->ny, c->nz, q.grid.p, q.nx, q.ny, q.nz, ctl);
-  // field 0 of the tray slab kernel with a zero distance field: B + i*0
-  launch(c, "coarse_slab_fwd", k_tray_slab_fwd, dim3(q.nx, 1), dim3(kSlabThreads),
-         coarse_slab_shmem(c), (const int*)q.grid.p, (const int*)q.zero.p,
-         (const FieldStats*)q.zst.p, q.t1.p, q.t1.p, q.nx, q.ny, q.nz, q.py, q.pz,
-         q.tw.p + q.px.n, q.tw.p + q.px.n + py, ctl);
-  launch(c, "coarse_x_fwd", k_tray_x_fwd, dim3((pz + q.T - 1) / q.T, py, 1),
-         dim3(kPencilThreads), coarse_pencil_shmem(c), (const float2*)q.t1.p,
-         (const float2*)q.t1.p, q.g1.p, q.g1.p, q.nx, q.px, py, pz, q.T, q.tw.p, ctl);
+  const int n = q.cx * q.cy;
+  launch(c, "tray_columns", k_tray_columns, dim3((n + 3) / 4), dim3(256), 0, (const int*)c->tray.p,
+         c->nx, c->ny, c->nz, q.cols.p, q.cx, q.cy, ctl);
 }
 
 // ---- FFT planning for a tray --------------------------------------------
@@ -446,14 +417,13 @@ void plan_tray(spk_ctx* c) {
   if (!c->lds_attrs) {
     const size_t dyn = (size_t)lds_max - 1024;   // leave room for static shared arrays
     allow_shmem(k_tray_slab_fwd, dyn);
-    allow_shmem(k_item_slab_fwd<1>, dyn);
+    allow_shmem(k_item_slab_fwd, dyn);
     allow_shmem(k_score_yz, dyn);
     allow_shmem(k_dist_zy, dyn);
     allow_shmem(k_corr_x, dyn);
     allow_shmem(k_tray_x_fwd, dyn);
     allow_shmem(k_feas_x, dyn);
     allow_shmem(k_feas_yz, dyn);
-    allow_shmem(k_item_slab_fwd<2>, dyn);
     c->lds_attrs = true;
   }
   plan_coarse(c);
@@ -738,7 +708,7 @@ void enqueue_item_spectra(spk_ctx* c, const OrientDesc* od, int m, int max_ex, c
            dim3(kPencilThreads), ysh, c->ispec.p, c->py, pz, c->Ty, py, py, tw + c->px.n, od,
            max_ex, (int)PENCIL_ITEM, ctl, 0);
   } else {
-    launch(c, "item_slab_fwd", k_item_slab_fwd<1>, dim3(max_ex, m), dim3(kSlabThreads),
+    launch(c, "item_slab_fwd", k_item_slab_fwd, dim3(max_ex, m), dim3(kSlabThreads),
            slab_shmem(c), (const int*)c->items.p, od, c->ispec.p, max_ex, c->py, c->pz,
            tw + c->px.n, tw + c->px.n + py, ctl);
   }
@@ -764,10 +734,7 @@ void enqueue_feas(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const int
   const int np = (m + 1) / 2;
   const int tiles = (pz + c->T - 1) / c->T;
   const float scale = (float)(1.0 / ((double)c->px.n * py * pz));
-  if (c->timing) {
-    c->units["feas_x"] += np;
-    c->units["feas_yz"] += np;
-  }
+  unsigned long long* pairs = c->timing ? c->pairs.p : nullptr;
   const int group = c->feas_group > 0 ? c->feas_group : np;
   for (int g0 = 0; g0 < np; g0 += group) {
     const int gp = std::min(group, np - g0);
@@ -781,62 +748,37 @@ void enqueue_feas(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const int
            ctl);
     launch(c, "feas_yz", k_feas_yz, dim3(c->nx, gp), dim3(kSlabThreads), slab_shmem(c),
            (const float2*)c->y1.p, od + ob, gm, c->nx, c->py, c->pz, tw + c->px.n,
-           tw + c->px.n + py, c->fstate.p, ob, cf, 0, ctl);
+           tw + c->px.n + py, c->fstate.p, ob, cf, 0, pairs, ctl);
   }
 }
 
-// Coarse feasibility pass over the m descriptors od (device), before the
-// full-resolution pass: coarse descriptors, spectra of the items' eroded
-// cores sampled every second voxel, collision-only correlation with FFT(Bc),
-// then k_coarse_decide (prune pairs, or skip the full pass).  The eroded
-// items (k_erode_items) must be in c->co.er.  ctl must enable the pass.
+// control words that enable the feasibility pass outside the greedy loop
+const int* ctl_feas_on(spk_ctx* c) {
+  c->ctl_on.ensure(CTL_WORDS);
+  int on[CTL_WORDS] = {0};
+  on[CTL_HALT_ITEM] = -1;
+  on[CTL_FEAS] = 1;
+  upload(c, c->ctl_on.p, on, sizeof(on));
+  return c->ctl_on.p;
+}
+
+// Column test over the m descriptors od (device), before the
+// full-resolution feasibility pass: core masks of the oriented items, the
+// test, then k_coarse_decide (prune pairs, or skip the full pass).  ctl must
+// enable the pass.
 void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const int* ctl) {
   spk_ctx::Coarse& q = c->co;
   if (!q.ready) return;
-  const int py = q.py.n, pz = q.pz.n;
-  const int mx = (max_ex + 1) / 2;
-  const int np = (m + 1) / 2;
-  q.od.ensure(24);
-  q.ispec.ensure((size_t)m * mx * py * pz);
-  q.y1.ensure((size_t)np * q.nx * py * pz);
-  launch(c, "coarse_od", k_coarse_od, dim3(1), dim3(64), 0, od, m, q.od.p, ctl);
-  launch(c, "coarse_item_fwd", k_item_slab_fwd<2>, dim3(mx, m), dim3(kSlabThreads),
-         coarse_slab_shmem(c), (const int*)q.er.p, (const OrientDesc*)q.od.p, q.ispec.p, mx,
-         q.py, q.pz, q.tw.p + q.px.n, q.tw.p + q.px.n + py, ctl);
-  const int tiles = (pz + q.T - 1) / q.T;
-  const int nchunk = std::max(1, std::min(np, (2048 + tiles * py - 1) / (tiles * py)));
-  const int chunk = (np + nchunk - 1) / nchunk;
-  const float scale = (float)(1.0 / ((double)q.px.n * py * pz));
-  launch(c, "coarse_feas_x", k_feas_x, dim3(tiles, py, (np + chunk - 1) / chunk),
-         dim3(kPencilThreads), coarse_pencil_shmem(c), (const float2*)q.g1.p,
-         (const float2*)q.ispec.p, (const OrientDesc*)q.od.p, q.y1.p, m, chunk, mx, q.nx, q.px,
-         py, pz, q.T, scale, (const float2*)q.tw.p, (const int*)nullptr, 1, ctl);
-  launch(c, "coarse_feas_yz", k_feas_yz, dim3(q.nx, np), dim3(kSlabThreads),
-         coarse_slab_shmem(c), (const float2*)q.y1.p, (const OrientDesc*)q.od.p, m, q.nx, q.py,
-         q.pz, (const float2*)q.tw.p + q.px.n, (const float2*)q.tw.p + q.px.n + py, q.fs.p, 0,
-         (const int*)nullptr, 1, ctl);
+  const int ccx = (max_ex + 1) / 2, ccy = q.cy;
+  q.cm.ensure((size_t)m * ccx * ccy);
+  launch(c, "core_columns", k_core_columns, dim3((ccx * ccy + 3) / 4, m), dim3(256), 0,
+         (const int*)c->items.p, od, m, q.cm.p, ccx, ccy, ctl);
+  launch(c, "column_test", k_column_test, dim3((q.cx * q.cy + 255) / 256, m), dim3(256), 0,
+         (const M128*)q.cols.p, q.cx, q.cy, (const M128*)q.cm.p, ccx, ccy, od, m, q.fs.p, ctl);
   launch(c, "coarse_decide", k_coarse_decide, dim3(1), dim3(64), 0, (const int*)q.fs.p, m,
          c->fstate.p, c->resid_limit, (int*)ctl);
 }
 
-// eroded copies of the loaded items for the coarse pass (same layout)
-void erode_items(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector<int>& off) {
-  spk_ctx::Coarse& q = c->co;
-  if (!q.ready || bases.empty()) return;
-  size_t tot = 0;
-  std::vector<int> tab(4 * bases.size());
-  for (size_t t = 0; t < bases.size(); ++t) {
-    tab[4 * t] = off[t];
-    for (int a = 0; a < 3; ++a) tab[4 * t + 1 + a] = bases[t].dims[a];
-    tot = std::max(tot, (size_t)off[t] + (size_t)bases[t].dims[0] * bases[t].dims[1] *
-                                             bases[t].dims[2]);
-  }
-  q.er.ensure(std::max<size_t>(tot, 1));
-  q.tab.ensure(tab.size());
-  upload(c, q.tab.p, tab.data(), tab.size() * sizeof(int));
-  launch(c, "erode_items", k_erode_items, dim3((unsigned)bases.size()), dim3(256), 0,
-         (const int*)c->items.p, (const int*)q.tab.p, q.er.p);
-}
 
 // Scored search (pass A, pass B, selection, exact re-check of the winner) of
 // m descriptors whose item spectra are in c->ispec.
@@ -1395,13 +1337,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   c->heights.ensure((size_t)c->nx * c->ny);
   c->od_prop.ensure(kNnLogits);
   c->od_rest.ensure(kNnLogits);
-  c->ctl_on.ensure(CTL_WORDS);
-  {
-    int on[CTL_WORDS] = {0};
-    on[CTL_HALT_ITEM] = -1;
-    on[CTL_FEAS] = 1;
-    upload(c, c->ctl_on.p, on, sizeof(on));
-  }
+  ctl_feas_on(c);
   const bool feas_plan = c->feas_mode != 0 && !c->split;
   float logits[kNnLogits];
   int fsv[kFeasHeader + kNnLogits];
@@ -1411,7 +1347,6 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       nn_tray_branch(nn, s, c->tfeat.p, 1);
       nn_head(nn, s, c->ipool.p + (size_t)t * kNnLogits * kNnItemPooled, 1);
     });
-    download(c, logits, nn->logits.p, sizeof(logits));
   };
   // descriptors of the listed fitting orientations of item t, on the device
   auto gather = [&](const ItemPlan& pl, const std::vector<int>& list, OrientDesc* dst,
@@ -1444,8 +1379,9 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     enqueue_coarse(c, od, k, max_ex, c->ctl_on.p);
     enqueue_item_spectra(c, od, k, max_ex, c->co.ready ? c->ctl_on.p : nullptr);
     enqueue_feas(c, od, k, max_ex, c->ctl_on.p);
-    download(c, fsv, c->fstate.p, (kFeasHeader + k) * sizeof(int));
-    sync(c);
+    const Mailbox& mb = post_and_sync(c, nullptr, nullptr, nullptr, nullptr, c->fstate.p,
+                                      kFeasHeader + k);
+    std::memcpy(fsv, mb.fs, (kFeasHeader + k) * sizeof(int));
     float res;
     std::memcpy(&res, &fsv[0], sizeof(res));
     if (!(res <= c->resid_limit)) return false;
@@ -1510,7 +1446,8 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     if ((int)cand.size() > topk) {
       if (!have_t) {
         enqueue_head(t);
-        sync(c);
+        const Mailbox& mb = post_and_sync(c, nullptr, nullptr, nullptr, nn->logits.p, nullptr, 0);
+        std::memcpy(logits, mb.logits, sizeof(logits));
       }
       float mx = -INFINITY;
       for (int i : cand) mx = std::max(mx, logits[pl.map[i]]);
@@ -1546,10 +1483,14 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     if (next_head) enqueue_head(tn);
     int ctl[CTL_WORDS];
     PackOut po;
-    download(c, ctl, c->ctl.p, sizeof(ctl));
-    download(c, &po, c->pout.p + t, sizeof(po));
-    download(c, &tst, c->stats.p, sizeof(tst));
-    sync(c);
+    {
+      const Mailbox& mb = post_and_sync(c, c->ctl.p, c->pout.p + t, c->stats.p,
+                                        next_head ? nn->logits.p : nullptr, nullptr, 0);
+      std::memcpy(ctl, mb.ctl, sizeof(ctl));
+      po = mb.po;
+      tst = mb.st;
+      if (next_head) std::memcpy(logits, mb.logits, sizeof(logits));
+    }
     if (c->timing) c->flush_timing();
     if (ctl[CTL_HALT] == 0) {
       accept(t, pl, pick, po);
@@ -1578,10 +1519,13 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
         const OrientDesc* odr = gather(pl, rest, c->od_rest.p, &max_ex, &max_e);
         enqueue_step(c, odr, (int)rest.size(), max_ex, max_e, penalty, t, id_offset, 0, nullptr,
                      nullptr, 0);
-        download(c, ctl, c->ctl.p, sizeof(ctl));
-        download(c, &po, c->pout.p + t, sizeof(po));
-        download(c, &tst, c->stats.p, sizeof(tst));
-        sync(c);
+        {
+          const Mailbox& mb = post_and_sync(c, c->ctl.p, c->pout.p + t, c->stats.p, nullptr,
+                                            nullptr, 0);
+          std::memcpy(ctl, mb.ctl, sizeof(ctl));
+          po = mb.po;
+          tst = mb.st;
+        }
         if (c->timing) c->flush_timing();
         if (ctl[CTL_HALT] == 0) {
           accept(t, pl, rest, po);
@@ -1610,8 +1554,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     placed += place_exact(c, eb, pl, all, resid, id_offset + placed + 1, &results[t]);
     prev_placed = eb.fit >= 0;
     reset_ctl();
-    download(c, &tst, c->stats.p, sizeof(tst));
-    sync(c);
+    tst = post_and_sync(c, nullptr, nullptr, c->stats.p, nullptr, nullptr, 0).st;
   }
   if (counts)
     for (int i = 0; i < 5; ++i) counts[i] = cnt[i];
@@ -1743,7 +1686,6 @@ void load_items(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_ite
     const int it = order ? order[t] : t;
     if (it < 0 || it >= n_items) throw SpkError{SPK_EINVAL, "order index out of range"};
   }
-  erode_items(c, bases, off);
 }
 
 }  // namespace
@@ -1829,6 +1771,9 @@ int spk_ctx_destroy(spk_ctx* c) {
     c->imaps.release();
     c->fstate.release(); c->ctl_on.release();
     c->theta.release(); c->cgrid.release(); c->cdist.release(); c->ctmp.release();
+    c->pairs.release();
+    c->co.cols.release(); c->co.cm.release(); c->co.fs.release();
+    if (c->mail_h) (void)hipHostFree(c->mail_h);
     if (c->stream) hipStreamDestroy(c->stream);
   }
   delete c;
@@ -1928,6 +1873,62 @@ int spk_search_oriented(spk_ctx* c, const int32_t* item, int ix, int iy, int iz,
     std::vector<Entry> entries;
     for (int o = 0; o < n; ++o) entries.push_back(Entry{bases[0], orients ? &orients[o] : nullptr, 0});
     do_search(c, bases, off, false, entries, penalty, interlocking_free, best, per_orient);
+    return SPK_OK;
+  });
+}
+
+int spk_feasible(spk_ctx* c, const int32_t* item, int ix, int iy, int iz,
+                 const spk_orient* orients, int n, int32_t* column, int32_t* full,
+                 float* residual) {
+  return guarded(c, [&]() {
+    if (!c->valid) throw SpkError{SPK_ESTATE, "no tray set in this context"};
+    if (!item || n <= 0 || n > 24 || !full) throw SpkError{SPK_EINVAL, "bad feasibility arguments"};
+    if (c->split) throw SpkError{SPK_EUNSUPPORTED, "feasibility pass needs the slab plan"};
+    const HostItem it = scan_item(item, ix, iy, iz);
+    const size_t sz = (size_t)ix * iy * iz;
+    c->items.ensure(std::max<size_t>(sz, 1));
+    upload(c, c->items.p, item, sz * sizeof(int));
+    std::vector<OrientDesc> ods;
+    std::vector<int> fit;
+    int max_ex = 1;
+    for (int o = 0; o < n; ++o) {
+      const OrientDesc d = make_desc(c, it, 0, orients ? &orients[o] : nullptr);
+      const bool fits = it.hi[0] >= 0 && d.ex <= c->nx && d.ey <= c->ny && d.ez <= c->nz &&
+                        d.nvx > 0 && d.nvy > 0 && d.nvz > 0;
+      if (column) column[o] = 0;
+      full[o] = 0;
+      if (!fits) continue;
+      ods.push_back(d);
+      fit.push_back(o);
+      max_ex = std::max(max_ex, d.ex);
+    }
+    if (residual) *residual = 0.f;
+    const int k = (int)ods.size();
+    if (k == 0) return SPK_OK;
+    c->od_prop.ensure(24);
+    upload(c, c->od_prop.p, ods.data(), ods.size() * sizeof(OrientDesc));
+    ensure_search_ws(c, k, max_ex);
+    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p,
+           c->co.ready ? c->co.fs.p : (int*)nullptr, k, 2, (int*)nullptr,
+           (const FieldStats*)nullptr, (const FieldStats*)nullptr, (PackOut*)nullptr, 0);
+    enqueue_coarse(c, c->od_prop.p, k, max_ex, ctl_feas_on(c));
+    // the full-resolution pass over every fitting orientation (no pruning)
+    enqueue_item_spectra(c, c->od_prop.p, k, max_ex, nullptr);
+    enqueue_feas(c, c->od_prop.p, k, max_ex, nullptr);
+    std::vector<int> cf(kFeasHeader + k, 1);
+    if (c->co.ready) {
+      const Mailbox& mc = post_and_sync(c, nullptr, nullptr, nullptr, nullptr, c->co.fs.p,
+                                        kFeasHeader + k);
+      std::memcpy(cf.data(), mc.fs, cf.size() * sizeof(int));
+    }
+    const Mailbox& mb = post_and_sync(c, nullptr, nullptr, nullptr, nullptr, c->fstate.p,
+                                      kFeasHeader + k);
+    for (int j = 0; j < k; ++j) {
+      if (column) column[fit[j]] = cf[kFeasHeader + j] ? 1 : 0;
+      full[fit[j]] = mb.fs[kFeasHeader + j] ? 1 : 0;
+    }
+    if (residual) std::memcpy(residual, &mb.fs[0], sizeof(float));
+    if (c->timing) c->flush_timing();
     return SPK_OK;
   });
 }
@@ -2244,9 +2245,14 @@ int spk_save_vox(const int32_t* grid, int nx, int ny, int nz, const char* path) 
 }
 
 int spk_timing_enable(spk_ctx* c, int on) {
-  if (!c) return fail(SPK_EINVAL, "null context");
-  c->timing = on != 0;
-  return SPK_OK;
+  return guarded(c, [&]() {
+    if (on && !c->pairs.p) {
+      c->pairs.ensure(2);
+      HIP_OK(hipMemsetAsync(c->pairs.p, 0, 2 * sizeof(unsigned long long), c->stream));
+    }
+    c->timing = on != 0;
+    return SPK_OK;
+  });
 }
 
 int spk_timing_get(spk_ctx* c, const char* kernel, double* total_ms, int64_t* launches) {
@@ -2261,8 +2267,21 @@ int spk_timing_get(spk_ctx* c, const char* kernel, double* total_ms, int64_t* la
 
 int spk_timing_units(spk_ctx* c, const char* kernel, int64_t* units) {
   return guarded(c, [&]() {
-    auto it = c->units.find(kernel ? kernel : "");
-    if (units) *units = it == c->units.end() ? 0 : it->second;
+    const std::string k = kernel ? kernel : "";
+    long long v = 0;
+    if (k == "feas_x" || k == "feas_yz" || k == "coarse_feas_x" || k == "coarse_feas_yz") {
+      // counted on the device: pairs the coarse pass pruned never run
+      unsigned long long h[2] = {0, 0};
+      if (c->pairs.p) {
+        download(c, h, c->pairs.p, sizeof(h));
+        sync(c);
+      }
+      v = (long long)h[k.rfind("coarse", 0) == 0 ? 1 : 0];
+    } else {
+      auto it = c->units.find(k);
+      v = it == c->units.end() ? 0 : it->second;
+    }
+    if (units) *units = v;
     return SPK_OK;
   });
 }
@@ -2272,6 +2291,7 @@ int spk_timing_reset(spk_ctx* c) {
     c->flush_timing();
     c->acc.clear();
     c->units.clear();
+    if (c->pairs.p) HIP_OK(hipMemsetAsync(c->pairs.p, 0, 2 * sizeof(unsigned long long), c->stream));
     return SPK_OK;
   });
 }

@@ -554,16 +554,12 @@ k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
 // ---------------------------------------------------------------------------
 // item spectra: grid (max_ex, n_orient).  I~[o][x][ky][kz] = FFT_yz(item_o[x])
 // ---------------------------------------------------------------------------
-// SUB = 1: the oriented items; SUB = 2: the coarse cores (items holds the
-// eroded items, descriptors carry coarse extents, voxel 2*(x, j, k) is read),
-// run by the coarse feasibility pass.
-template <int SUB>
 __global__ void __launch_bounds__(kSlabThreads)
 k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od,
                 float2* __restrict__ ispec, int max_ex, AxisPlan py_, AxisPlan pz_,
                 const float2* __restrict__ tw_y, const float2* __restrict__ tw_z,
                 const int* __restrict__ ctl) {
-  if (SUB == 1 ? (search_off(ctl) || coarse_done(ctl)) : feas_off(ctl)) return;
+  if (search_off(ctl) || coarse_done(ctl)) return;
   extern __shared__ float2 smem[];
   const int py = py_.n, pz = pz_.n;
   const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
@@ -582,8 +578,7 @@ k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od
     for (int s = 0; s < kSlabMaxEl; ++s) {
       const int e = tid + s * kSlabThreads;
       const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
-      iv[s] = (e < py * pz && j < d.ey && k < d.ez)
-                  ? base[oriented_index(d, SUB * x, SUB * j, SUB * k)] : 0;
+      iv[s] = (e < py * pz && j < d.ey && k < d.ez) ? base[oriented_index(d, x, j, k)] : 0;
     }
 #pragma unroll
     for (int s = 0; s < kSlabMaxEl; ++s) {
@@ -958,7 +953,8 @@ __global__ void __launch_bounds__(kSlabThreads)
 k_feas_yz(const float2* __restrict__ y1, const OrientDesc* __restrict__ od, int m, int nx,
           AxisPlan py_, AxisPlan pz_, const float2* __restrict__ tw_y,
           const float2* __restrict__ tw_z, int* __restrict__ fs, int o_base,
-          const int* __restrict__ cf, int coarse_pass, const int* __restrict__ ctl) {
+          const int* __restrict__ cf, int coarse_pass, unsigned long long* __restrict__ pairs,
+          const int* __restrict__ ctl) {
   if (feas_off(ctl) || (!coarse_pass && coarse_done(ctl))) return;
   extern __shared__ float2 smem[];
   const int py = py_.n, pz = pz_.n;
@@ -966,6 +962,8 @@ k_feas_yz(const float2* __restrict__ y1, const OrientDesc* __restrict__ od, int 
   const int oa = 2 * p, ob = 2 * p + 1;
   const bool hasb = ob < m;
   if (!coarse_pass && cf && ctl && ctl[CTL_PRUNE] && !cf[oa] && !(hasb && cf[ob])) return;
+  // pairs transformed (both passes), counted when timing asks for it
+  if (pairs && x == 0 && tid == 0) atomicAdd(pairs, 1ULL);
   const OrientDesc da = od[oa];
   const OrientDesc db = od[hasb ? ob : oa];
   const bool xa = x < da.nvx, xb = hasb && x < db.nvx;
@@ -1016,12 +1014,174 @@ k_feas_yz(const float2* __restrict__ y1, const OrientDesc* __restrict__ od, int 
   }
 }
 
-// Coarse feasibility pass, after its k_feas_yz: cfs holds its residual and
-// flags.  Untrusted (residual above resid_limit): nothing is pruned.  No
-// orientation coarse-feasible: every orientation is infeasible, the
-// full-resolution pass is skipped and fs[0] takes the coarse residual, so
-// that the pass's reader (k_feas_decide or the host) sees a trusted result
-// with no feasible orientation.  Otherwise the full pass prunes pairs.
+// What the host-driven loops read after each synchronisation, gathered by
+// one kernel into pinned host memory (no device-to-host copies).
+constexpr int kMailLogits = 24;
+constexpr int kMailFlags = kFeasHeader + 24;
+struct Mailbox {
+  int ctl[CTL_WORDS];
+  FieldStats st;
+  PackOut po;
+  float logits[kMailLogits];
+  int fs[kMailFlags];
+};
+
+// any pointer may be null (that part is left as it was)
+__global__ void k_post(const int* __restrict__ ctl, const PackOut* __restrict__ po,
+                       const FieldStats* __restrict__ st, const float* __restrict__ logits,
+                       const int* __restrict__ fs, int nfs, Mailbox* __restrict__ mb) {
+  const int i = threadIdx.x;
+  if (ctl && i < CTL_WORDS) mb->ctl[i] = ctl[i];
+  if (logits && i < kMailLogits) mb->logits[i] = logits[i];
+  if (fs)
+    for (int j = i; j < nfs && j < kMailFlags; j += blockDim.x) mb->fs[j] = fs[j];
+  if (i == 0) {
+    if (st) mb->st = *st;
+    if (po) mb->po = *po;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Column test: an exact, conservative pre-pass of the feasibility pass on bit
+// masks.  The tray is reduced to 2x2 column blocks at full z resolution (bit z
+// of block (X, Y) set when any of its voxels at height z is occupied), each
+// oriented item to its column core (bit z of core column (X, Y) set when the
+// item has a voxel at (2X+dx, 2Y+dy, z) for all dx, dy in {-1, 0, 1}).  A
+// placement at (2Qx + rx, 2Qy + ry, qz) fills every block (Qx+X, Qy+Y) at the
+// heights qz + z of the core completely, so a collision-free placement needs
+// those block bits clear; an orientation for which every (Qx, Qy, qz) of a
+// valid offset meets a set bit has no collision-free offset.  Heights are
+// 128-bit masks (tray z <= 128); per core column the set bits form runs
+// [a, a+L), and the blocked qz of a run are (window-OR of length L of the
+// block's mask) >> a.
+// ---------------------------------------------------------------------------
+struct M128 {
+  unsigned long long lo, hi;
+};
+__device__ __forceinline__ M128 m_or(M128 a, M128 b) { return {a.lo | b.lo, a.hi | b.hi}; }
+__device__ __forceinline__ M128 m_shr(M128 a, int s) {   // bit i of the result = bit i+s of a
+  if (s == 0) return a;
+  if (s >= 64) return {a.hi >> (s - 64), 0ULL};
+  return {(a.lo >> s) | (a.hi << (64 - s)), a.hi >> s};
+}
+// bit i = OR of bits i .. i+L-1 of t (L >= 1)
+__device__ __forceinline__ M128 m_window(M128 t, int L) {
+  M128 w = t;
+  int len = 1;
+  while (2 * len <= L) {
+    w = m_or(w, m_shr(w, len));
+    len *= 2;
+  }
+  if (len < L) w = m_or(w, m_shr(w, L - len));
+  return w;
+}
+// bits [lo, hi) of a 64-bit word, 0 <= lo, hi <= 64
+__device__ __forceinline__ unsigned long long bits64(int lo, int hi) {
+  if (hi <= lo) return 0ULL;
+  const unsigned long long upper = hi >= 64 ? ~0ULL : ((1ULL << hi) - 1ULL);
+  const unsigned long long lower = lo >= 64 ? ~0ULL : ((1ULL << lo) - 1ULL);
+  return upper & ~lower;
+}
+__device__ __forceinline__ int m_ctz(M128 a) {
+  return a.lo ? __ffsll((long long)a.lo) - 1 : (a.hi ? 64 + __ffsll((long long)a.hi) - 1 : 128);
+}
+
+// one wavefront per 2x2 column block: lanes over z, ballots form the masks
+__global__ void __launch_bounds__(256)
+k_tray_columns(const int* __restrict__ tray, int nx, int ny, int nz, M128* __restrict__ cols,
+               int cx, int cy, const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
+  const int lane = threadIdx.x & 63;
+  const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (col >= cx * cy) return;
+  const int X = col / cy, Y = col - X * cy;
+  M128 m{0ULL, 0ULL};
+  for (int w = 0; w < 2; ++w) {
+    const int z = w * 64 + lane;
+    bool occ = false;
+    for (int u = 0; u < 2; ++u)
+      for (int v = 0; v < 2; ++v) {
+        const int x = 2 * X + u, y = 2 * Y + v;
+        if (x < nx && y < ny && z < nz && tray[((size_t)x * ny + y) * nz + z] != 0) occ = true;
+      }
+    const unsigned long long b = __ballot(occ);
+    if (w == 0) m.lo = b; else m.hi = b;
+  }
+  if (lane == 0) cols[col] = m;
+}
+
+// core columns of m oriented items, cm[(o * ccx + X) * ccy + Y]; one
+// wavefront per core column, lanes over z
+__global__ void __launch_bounds__(256)
+k_core_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od, int m,
+               M128* __restrict__ cm, int ccx, int ccy, const int* __restrict__ ctl) {
+  if (feas_off(ctl)) return;
+  const int lane = threadIdx.x & 63;
+  const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int o = blockIdx.y;
+  if (o >= m || col >= ccx * ccy) return;
+  const OrientDesc d = od[o];
+  const int X = col / ccy, Y = col - X * ccy;
+  const int* base = items + d.item_off;
+  M128 r{0ULL, 0ULL};
+  for (int w = 0; w < 2; ++w) {
+    const int z = w * 64 + lane;
+    bool full = z < d.ez && 2 * X + 1 < d.ex && 2 * Y + 1 < d.ey && X >= 0;
+    for (int dx = -1; dx <= 1 && full; ++dx)
+      for (int dy = -1; dy <= 1 && full; ++dy) {
+        const int x = 2 * X + dx, y = 2 * Y + dy;
+        if (x < 0 || y < 0 || base[oriented_index(d, x, y, z)] <= 0) full = false;
+      }
+    const unsigned long long b = __ballot(full);
+    if (w == 0) r.lo = b; else r.hi = b;
+  }
+  if (lane == 0) cm[((size_t)o * ccx + X) * ccy + Y] = r;
+}
+
+// grid (ceil(cx * cy / 256), m): thread (Qx, Qy) clears the heights qz that
+// some core run blocks and flags the orientation when any valid qz is left
+__global__ void __launch_bounds__(256)
+k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restrict__ cm,
+              int ccx, int ccy, const OrientDesc* __restrict__ od, int m, int* __restrict__ cfs,
+              const int* __restrict__ ctl) {
+  if (feas_off(ctl)) return;
+  const int o = blockIdx.y;
+  if (o >= m) return;
+  const OrientDesc d = od[o];
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  const int Qx = q / cy, Qy = q - Qx * cy;
+  const int nqx = d.nvx > 0 ? (d.nvx - 1) / 2 + 1 : 0, nqy = d.nvy > 0 ? (d.nvy - 1) / 2 + 1 : 0;
+  if (Qx >= nqx || Qy >= nqy || d.nvz <= 0) return;
+  const int kx = (d.ex + 1) / 2, ky = (d.ey + 1) / 2;
+  M128 bad{0ULL, 0ULL};
+  const M128* core = cm + (size_t)o * ccx * ccy;
+  for (int X = 0; X < kx; ++X)
+    for (int Y = 0; Y < ky; ++Y) {
+      M128 k = core[X * ccy + Y];
+      if (!(k.lo | k.hi)) continue;
+      const M128 t = cols[(Qx + X) * cy + (Qy + Y)];
+      while (k.lo | k.hi) {
+        const int a = m_ctz(k);
+        const M128 rest = m_shr(k, a);
+        const int L = m_ctz(M128{~rest.lo, ~rest.hi});
+        bad = m_or(bad, m_shr(m_window(t, L), a));
+        // clear the run: bits a .. a+L-1
+        k.lo &= ~bits64(min(a, 64), min(a + L, 64));
+        k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
+      }
+    }
+  const int nv = d.nvz;
+  const unsigned long long vlo = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1ULL);
+  const unsigned long long vhi = nv >= 128 ? ~0ULL : (nv > 64 ? ((1ULL << (nv - 64)) - 1ULL) : 0ULL);
+  if ((~bad.lo & vlo) | (~bad.hi & vhi)) cfs[kFeasHeader + o] = 1;
+}
+
+// Coarse (column) test decision, after k_column_test: cfs holds its flags (its
+// residual slot stays 0: the test is exact).  No orientation left: every
+// orientation is infeasible, the full-resolution pass is skipped and fs[0]
+// takes the test's residual (0), so that the pass's reader (k_feas_decide or
+// the host) sees a trusted result with no feasible orientation.  Otherwise
+// the full pass prunes orientation pairs the test excluded.
 __global__ void k_coarse_decide(const int* __restrict__ cfs, int m, int* __restrict__ fs,
                                 float resid_limit, int* __restrict__ ctl) {
   if (threadIdx.x != 0 || feas_off(ctl)) return;
@@ -1032,66 +1192,6 @@ __global__ void k_coarse_decide(const int* __restrict__ cfs, int m, int* __restr
   ctl[CTL_PRUNE] = trusted && any ? 1 : 0;
   ctl[CTL_COARSE_DONE] = trusted && !any ? 1 : 0;
   if (trusted && !any) fs[0] = __float_as_int(res);
-}
-
-// Coarse descriptors from full-resolution ones: extents ceil(e / 2) (the
-// core's cells), valid coarse offsets floor((nv - 1) / 2) + 1.
-__global__ void k_coarse_od(const OrientDesc* __restrict__ od, int m,
-                            OrientDesc* __restrict__ out, const int* __restrict__ ctl) {
-  const int o = threadIdx.x;
-  if (o >= m || feas_off(ctl)) return;
-  OrientDesc d = od[o];
-  d.ex = (d.ex + 1) / 2;
-  d.ey = (d.ey + 1) / 2;
-  d.ez = (d.ez + 1) / 2;
-  d.nvx = d.nvx > 0 ? (d.nvx - 1) / 2 + 1 : 0;
-  d.nvy = d.nvy > 0 ? (d.nvy - 1) / 2 + 1 : 0;
-  d.nvz = d.nvz > 0 ? (d.nvz - 1) / 2 + 1 : 0;
-  out[o] = d;
-}
-
-// Coarse tray: 1 for every 2x2x2 block holding an occupied voxel.
-__global__ void __launch_bounds__(256)
-k_coarse_pool(const int* __restrict__ tray, int nx, int ny, int nz, int* __restrict__ out,
-              int cx, int cy, int cz, const int* __restrict__ ctl) {
-  if (refresh_off(ctl)) return;
-  const int n = cx * cy * cz;
-  for (int e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
-    const int k = e % cz, j = (e / cz) % cy, i = e / (cy * cz);
-    int v = 0;
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b)
-        for (int c = 0; c < 2; ++c) {
-          const int x = 2 * i + a, y = 2 * j + b, z = 2 * k + c;
-          if (x < nx && y < ny && z < nz && tray[((size_t)x * ny + y) * nz + z] != 0) v = 1;
-        }
-    out[e] = v;
-  }
-}
-
-// Items eroded by a 3x3x3 block: er[v] = 1 iff every voxel within distance 1
-// (per axis) of v is an item voxel (> 0); voxels outside the item's box are
-// empty.  One block per item; tab[4*i .. 4*i+3] = offset, dims.
-__global__ void __launch_bounds__(256)
-k_erode_items(const int* __restrict__ items, const int* __restrict__ tab,
-              int* __restrict__ er) {
-  const int it = blockIdx.x;
-  const int off = tab[4 * it], bx = tab[4 * it + 1], by = tab[4 * it + 2], bz = tab[4 * it + 3];
-  const int* src = items + off;
-  const int n = bx * by * bz;
-  for (int e = threadIdx.x; e < n; e += 256) {
-    const int z = e % bz, y = (e / bz) % by, x = e / (by * bz);
-    int v = 1;
-    for (int a = -1; a <= 1 && v; ++a)
-      for (int b = -1; b <= 1 && v; ++b)
-        for (int c = -1; c <= 1 && v; ++c) {
-          const int X = x + a, Y = y + b, Z = z + c;
-          if (X < 0 || Y < 0 || Z < 0 || X >= bx || Y >= by || Z >= bz ||
-              src[((size_t)X * by + Y) * bz + Z] <= 0)
-            v = 0;
-        }
-    er[off + e] = v;
-  }
 }
 
 // greedy loop: with the pass run and its residual within resid_limit, an

@@ -343,3 +343,27 @@ def load_trained(device: int = 0, path: str = WEIGHTS, max_batch: int = 64) -> P
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path} is missing: train with `python -m psacking_amd.train_proposer`")
     return Proposer(device, max_batch, np.load(path, allow_pickle=False))
+
+
+def feasible(session, item, perms, flips):
+    """Which orientations of `item` fit the session's tray with no collision:
+    (column, full, residual) where full[o] = 1 when orientation o has a
+    collision-free valid offset, column[o] = 0 when the column test (2x2
+    column blocks, eroded core) already excludes it, residual = the FFT
+    rounding distance of the full pass."""
+    g = np.ascontiguousarray(item, dtype=np.int32)
+    n = len(perms)
+    orients = (_lib.SpkOrient * n)()
+    for o in range(n):
+        for a in range(3):
+            orients[o].perm[a] = int(perms[o][a])
+            orients[o].flip[a] = int(flips[o][a])
+    column = np.zeros(n, np.int32)
+    full = np.zeros(n, np.int32)
+    res = C.c_float(0.0)
+    ip = C.POINTER(C.c_int32)
+    _check(_lib.load().spk_feasible(C.c_void_p(session.handle()), g.ctypes.data_as(ip),
+                                    g.shape[0], g.shape[1], g.shape[2], orients, n,
+                                    column.ctypes.data_as(ip), full.ctypes.data_as(ip),
+                                    C.byref(res)))
+    return column, full, float(res.value)

@@ -495,3 +495,53 @@ def test_autotuned_tile_is_cached_and_exact(P):
     for tile in (4, 8, 16):
         say("forced tile", tile)
         _pack_with_plan(P, w, 24, 0, "edt", "nonzero", tile=tile)
+
+
+def _full_feasible(g, tray):
+    """Oracle: a valid offset with no occupied voxel under the item."""
+    n = np.array(tray.shape)
+    lo, hi = O.get_bounds(g)
+    h = np.array(hi)
+    if h[0] < 0 or np.any(h > n - 1):
+        return False
+    c = O.corr_exact((tray != 0).astype(np.int64), O.pad_to((g > 0).astype(np.int64), tray.shape))
+    return bool((c[:n[0] - h[0], :n[1] - h[1], :n[2] - h[2]] == 0).any())
+
+
+def test_feasibility_flags_match_oracle(P):
+    """spk_feasible against the oracle on trays filled to several levels:
+    full flags = exact feasibility of every orientation, column flags = the
+    oracle's column test, and the column test never excludes a feasible
+    orientation."""
+    from psacking_amd import proposer as PR
+    from psacking_amd.rotations import orientation_maps
+    perms, flips = orientation_maps(24)
+    rng = np.random.default_rng(71)
+    shape = (36, 30, 26)
+    checked = excluded = feasible = 0
+    for level in (0, 8, 14, 18, 21):
+        tray = (rng.random(shape) < 0.04).astype(np.int32)
+        tray[:, :, :level] = 3
+        s = P.Session(0)
+        s.set_tray(tray, "edt", "nonzero", 0)
+        for case in range(6):
+            dims = tuple(int(v) for v in rng.integers(4, 12, 3))
+            item = np.ones(dims, np.int32)
+            item[rng.random(dims) < 0.15] = 0
+            say("feasibility flags level", level, "case", case, dims)
+            column, full, res = PR.feasible(s, item, perms, flips)
+            assert res <= 0.25
+            for o, g in enumerate(O.get_orientations(item, 24)):
+                g = O.as_grid(g)
+                if any(g.shape[a] > shape[a] for a in range(3)):
+                    assert column[o] == 0 and full[o] == 0
+                    continue
+                exp_full = _full_feasible(g, tray)
+                exp_col = not O.column_infeasible(g, tray)
+                assert full[o] == exp_full, (level, case, o)
+                assert column[o] == exp_col, (level, case, o)
+                assert column[o] or not full[o]
+                checked += 1
+                excluded += not column[o]
+                feasible += bool(full[o])
+    assert checked > 500 and excluded > 50 and feasible > 50

@@ -293,6 +293,34 @@ def test_free_space_certificate_is_sound(metric):
     assert certified >= 10
 
 
+def test_column_infeasibility_is_sound_and_stronger():
+    """The column test (2x2 column blocks at full z resolution, x-y eroded
+    core) never rejects an orientation with a collision-free offset, rejects
+    some infeasible ones, and rejects everything the 2x2x2-block test does."""
+    rng = np.random.default_rng(93)
+    proved = cube_proved = infeasible = 0
+    for case in range(140):
+        shape = tuple(int(v) for v in rng.integers(9, 15, 3))
+        tray = (rng.random(shape) < rng.uniform(0.02, 0.25)).astype(np.int32)
+        tray[:, :, :int(rng.integers(0, shape[2] - 3))] = 2
+        dims = tuple(int(v) for v in rng.integers(3, 9, 3))
+        item = np.ones(dims, np.int32)
+        item[rng.random(dims) < 0.1] = 0
+        for g in O.get_orientations(item, 24)[:6]:
+            g = O.as_grid(g)
+            if any(g.shape[a] > shape[a] for a in range(3)):
+                continue
+            c = O.column_infeasible(g, tray)
+            b = O.coarse_infeasible(g, tray)
+            found = O.search(g, tray, O.distance_field(tray, "edt", "nonzero"), penalty="cube").found
+            assert not (c and found), case
+            assert c or not b, case
+            infeasible += not found
+            proved += c
+            cube_proved += b
+    assert proved > cube_proved >= 10 and infeasible > proved
+
+
 def test_coarse_infeasibility_is_sound():
     """The coarse test (2x2x2 blocks: occupied-any tray, eroded item core)
     never rejects an orientation that has a collision-free offset, and it
