@@ -762,11 +762,12 @@ const int* ctl_feas_on(spk_ctx* c) {
   return c->ctl_on.p;
 }
 
-// Column test over the m descriptors od (device), before the
-// full-resolution feasibility pass: core masks of the oriented items, the
-// test, then k_coarse_decide (prune pairs, or skip the full pass).  ctl must
-// enable the pass.
-void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const int* ctl) {
+// Column test over the m descriptors od (device), before the feasibility
+// pass and the scored search: core masks of the oriented items, the test,
+// then k_coarse_decide (prune orientations, or decide the item: see there).
+// ctl must not be null.
+void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const int* ctl,
+                    PackOut* out = nullptr, int t = 0, int halt_not_found = 0) {
   spk_ctx::Coarse& q = c->co;
   if (!q.ready) return;
   const int ccx = (max_ex + 1) / 2, ccy = q.cy;
@@ -776,7 +777,7 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const i
   launch(c, "column_test", k_column_test, dim3((q.cx * q.cy + 255) / 256, m), dim3(256), 0,
          (const M128*)q.cols.p, q.cx, q.cy, (const M128*)q.cm.p, ccx, ccy, od, m, q.fs.p, ctl);
   launch(c, "coarse_decide", k_coarse_decide, dim3(1), dim3(64), 0, (const int*)q.fs.p, m,
-         c->fstate.p, c->resid_limit, (int*)ctl);
+         c->fstate.p, out, t, halt_not_found, (int*)ctl);
 }
 
 
@@ -796,10 +797,12 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
     c->units["corr_x"] += m;
     c->units[c->split ? "rows_score" : "score_yz"] += m;
   }
+  // column-test flags of this list (pruning needs CTL_PRUNE set by its decide)
+  const int* cf = c->co.ready && !fgrid ? c->co.fs.p + kFeasHeader : nullptr;
   launch(c, "corr_x", k_corr_x, dim3(tiles, py, nchunk), dim3(kPencilThreads),
          pencil_shmem(c) + (size_t)c->px.n * c->T * sizeof(float2),
          (const float2*)c->g1.p, (const float2*)c->g2.p, (const float2*)c->ispec.p, od, c->y1.p,
-         c->y2.p, m, chunk, max_ex, c->nx, c->px, py, pz, c->T, scale, tw, ctl);
+         c->y2.p, m, chunk, max_ex, c->nx, c->px, py, pz, c->T, scale, tw, cf, ctl);
   const double* pen = c->pen.p + (penalty == SPK_PENALTY_CUBE ? c->nz : 0);
   const int rs = rec_stride(c);
   if (c->split) {
@@ -817,7 +820,7 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
     launch(c, "score_yz", k_score_yz, dim3(c->nx, m), dim3(kSlabThreads), score_shmem(c),
            (const float2*)c->y1.p, (const float2*)c->y2.p, od, (const FieldStats*)c->stats.p,
            pen, c->rec.p, c->nx, c->ny, c->nz, c->py, c->pz, tw + c->px.n, tw + c->px.n + py,
-           fgrid, pgrid, ctl);
+           fgrid, pgrid, cf, ctl);
   }
   launch(c, "select", k_select, dim3(1), dim3(kSelectThreads), 0, (const WgBest*)c->rec.p, od, m,
          rs, c->split ? c->Rs : 0, c->per.p, c->sout.p, c->ver.p, ctl);
@@ -1091,11 +1094,11 @@ void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e
                   const FieldStats* item_bound = nullptr) {
   if (feas_mode < 0) feas_mode = c->feas_mode;
   const bool feas = feas_mode && !c->split;
-  if (feas || item_bound) {
+  if (feas || item_bound || c->co.ready) {
     launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p,
            c->co.ready ? c->co.fs.p : (int*)nullptr, m, feas ? feas_mode : 0, c->ctl.p,
            (const FieldStats*)c->stats.p, item_bound, c->pout.p, t);
-    if (feas) enqueue_coarse(c, od, m, max_ex, c->ctl.p);
+    enqueue_coarse(c, od, m, max_ex, c->ctl.p, c->pout.p, t, halt_not_found);
     enqueue_item_spectra(c, od, m, max_ex, c->ctl.p);
     if (feas) {
       enqueue_feas(c, od, m, max_ex, c->ctl.p);

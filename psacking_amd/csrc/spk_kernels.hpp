@@ -603,8 +603,11 @@ k_corr_x(const float2* __restrict__ g1, const float2* __restrict__ g2,
          const float2* __restrict__ ispec, const OrientDesc* __restrict__ od,
          float2* __restrict__ y1, float2* __restrict__ y2, int n_orient, int chunk, int max_ex,
          int nx, AxisPlan px_, int py, int pz, int T, float scale, const float2* __restrict__ tw_x,
-         const int* __restrict__ ctl) {
+         const int* __restrict__ cf, const int* __restrict__ ctl) {
   if (search_off(ctl)) return;
+  // orientations the column test excluded have no feasible offset: skipped
+  // here, and k_score_yz records them as having none
+  const bool prune = cf && ctl && ctl[CTL_PRUNE];
   extern __shared__ float2 smem[];
   const int px = px_.n, hz = pz / 2 + 1;
   float2* twx = smem;
@@ -630,6 +633,7 @@ k_corr_x(const float2* __restrict__ g1, const float2* __restrict__ g2,
     }
   }
   for (int o = o0; o < o1; ++o) {
+    if (prune && !cf[o]) continue;
     const OrientDesc d = od[o];
     const float2* is = ispec + (size_t)o * max_ex * py * pz;
     __syncthreads();
@@ -721,7 +725,8 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
            const double* __restrict__ pen, WgBest* __restrict__ rec, int nx, int ny, int nz,
            AxisPlan py_, AxisPlan pz_, const float2* __restrict__ tw_y,
            const float2* __restrict__ tw_z, unsigned char* __restrict__ fgrid,
-           long long* __restrict__ pgrid, const int* __restrict__ ctl) {
+           long long* __restrict__ pgrid, const int* __restrict__ cf,
+           const int* __restrict__ ctl) {
   if (search_off(ctl)) return;
   extern __shared__ float2 smem[];
   const int py = py_.n, pz = pz_.n, hz = pz / 2 + 1;
@@ -729,6 +734,14 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
   const OrientDesc d = od[o];
   WgBest* out = rec + (size_t)o * nx + x;
   if (x >= d.nvx) return;
+  if (cf && ctl && ctl[CTL_PRUNE] && !cf[o]) {   // excluded by the column test
+    if (tid == 0) {
+      WgBest b;
+      b.score = 1e300; b.idx = INT_MAX; b.p_lo = 0; b.p_hi = 0; b.resid = 0.f; b.unsupported = 0;
+      *out = b;
+    }
+    return;
+  }
   float2* twy = smem;
   float2* twz = twy + py;
   float2* buf = twz + pz;
@@ -1115,7 +1128,7 @@ k_tray_columns(const int* __restrict__ tray, int nx, int ny, int nz, M128* __res
 __global__ void __launch_bounds__(256)
 k_core_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od, int m,
                M128* __restrict__ cm, int ccx, int ccy, const int* __restrict__ ctl) {
-  if (feas_off(ctl)) return;
+  if (search_off(ctl)) return;
   const int lane = threadIdx.x & 63;
   const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int o = blockIdx.y;
@@ -1144,7 +1157,7 @@ __global__ void __launch_bounds__(256)
 k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restrict__ cm,
               int ccx, int ccy, const OrientDesc* __restrict__ od, int m, int* __restrict__ cfs,
               const int* __restrict__ ctl) {
-  if (feas_off(ctl)) return;
+  if (search_off(ctl)) return;
   const int o = blockIdx.y;
   if (o >= m) return;
   const OrientDesc d = od[o];
@@ -1176,22 +1189,36 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
   if ((~bad.lo & vlo) | (~bad.hi & vhi)) cfs[kFeasHeader + o] = 1;
 }
 
-// Coarse (column) test decision, after k_column_test: cfs holds its flags (its
-// residual slot stays 0: the test is exact).  No orientation left: every
-// orientation is infeasible, the full-resolution pass is skipped and fs[0]
-// takes the test's residual (0), so that the pass's reader (k_feas_decide or
-// the host) sees a trusted result with no feasible orientation.  Otherwise
-// the full pass prunes orientation pairs the test excluded.
+// Column test decision, after k_column_test (cfs: its flags; its residual
+// slot stays 0, the test being exact).  Some orientation left: the
+// feasibility pass and the scored search skip the excluded ones (CTL_PRUNE).
+// None left: every orientation is infeasible (CTL_COARSE_DONE); fs[0] = 0 so
+// that a reader of the feasibility state sees a trusted empty result, and in
+// the greedy loop (out != nullptr) the item is recorded as not found and its
+// search and refresh are skipped, or, for a search over proposals
+// (halt_not_found), the loop stops with CTL_HALT = 2 as k_decide would.
 __global__ void k_coarse_decide(const int* __restrict__ cfs, int m, int* __restrict__ fs,
-                                float resid_limit, int* __restrict__ ctl) {
-  if (threadIdx.x != 0 || feas_off(ctl)) return;
-  const float res = __int_as_float(cfs[0]);
+                                PackOut* __restrict__ out, int t, int halt_not_found,
+                                int* __restrict__ ctl) {
+  if (threadIdx.x != 0 || search_off(ctl)) return;
   int any = 0;
   for (int o = 0; o < m; ++o) any |= cfs[kFeasHeader + o];
-  const bool trusted = res <= resid_limit;
-  ctl[CTL_PRUNE] = trusted && any ? 1 : 0;
-  ctl[CTL_COARSE_DONE] = trusted && !any ? 1 : 0;
-  if (trusted && !any) fs[0] = __float_as_int(res);
+  ctl[CTL_PRUNE] = any ? 1 : 0;
+  ctl[CTL_COARSE_DONE] = any ? 0 : 1;
+  if (any) return;
+  fs[0] = 0;
+  if (!out) return;
+  PackOut r{};
+  r.so.found = 0;
+  r.so.idx = INT_MAX;
+  out[t] = r;
+  if (halt_not_found) {
+    ctl[CTL_HALT] = 2;
+    ctl[CTL_HALT_ITEM] = t;
+  } else {
+    ctl[CTL_SKIP_SEARCH] = 1;
+  }
+  ctl[CTL_SKIP_REFRESH] = 1;
 }
 
 // greedy loop: with the pass run and its residual within resid_limit, an
