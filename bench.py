@@ -42,8 +42,7 @@ KERNELS = ["dist_zy", "dist_x", "tray_slab_fwd", "tray_rows_fwd", "tray_x_fwd", 
            "item_rows_fwd", "pencil_y_fwd", "corr_x", "score_yz", "pencil_y_inv", "rows_score",
            "select", "verify", "decide", "place", "direct_search", "gather_od", "proposer",
            "proposer_items", "record", "feas_begin", "feas_x", "feas_yz", "feas_decide",
-           "coarse_od", "coarse_item_fwd", "coarse_feas_x", "coarse_feas_yz", "coarse_decide",
-           "coarse_pool", "coarse_slab_fwd", "coarse_x_fwd", "erode_items"]
+           "tray_columns", "core_columns", "column_test", "coarse_decide", "post"]
 
 
 def device_sync():
@@ -64,13 +63,12 @@ def kernel_bytes(name: str, Ph: int, V: int, m: float) -> float:
     Feasibility pass, per orientation pair (m = pairs): the x pass reads the
     tray spectrum once (16*Ph) and writes one complex64 product per pair
     (16*Ph); the yz pass reads it back.
-    Coarse pass (2x2x2 blocks): the same per pair on the coarse tray, whose
-    Ph the caller passes.  Pairs are counted on the device (pruned pairs are
-    not transformed).
+    Feasibility pairs are counted on the device: pairs the column test
+    excluded in both orientations are not transformed.
     """
-    if name in ("score_yz", "feas_yz", "coarse_feas_yz"):
+    if name in ("score_yz", "feas_yz"):
         return 16.0 * Ph * m
-    if name in ("corr_x", "feas_x", "coarse_feas_x"):
+    if name in ("corr_x", "feas_x"):
         return 16.0 * Ph + 16.0 * Ph * m
     raise KeyError(name)
 
@@ -259,8 +257,7 @@ def main(argv=None) -> int:
     def finish(name, s, dt, res, t_setup, searched):
         t = s.tray()
         kt = {n: s.timing_get(n) for n in KERNELS}
-        units = {n: s.timing_units(n) for n in ("corr_x", "score_yz", "feas_x", "feas_yz",
-                                                 "coarse_feas_x", "coarse_feas_yz")}
+        units = {n: s.timing_units(n) for n in ("corr_x", "score_yz", "feas_x", "feas_yz")}
         runs[name] = {"dt": dt, "res": res, "density": float((t > 0).sum()) / V, "kt": kt,
                       "units": units,
                       "setup": t_setup, "searched": searched,
@@ -360,13 +357,12 @@ def main(argv=None) -> int:
     else:
         used = "plain" if "plain" in runs else "learned"
     r = runs[used]
-    cands = ("score_yz", "corr_x", "feas_x", "feas_yz", "coarse_feas_x", "coarse_feas_yz")
+    cands = ("score_yz", "corr_x", "feas_x", "feas_yz")
     dom = max(cands, key=lambda n: r["kt"][n][0])
     ms_dom, n_dom = r["kt"][dom]
     avg_s = (ms_dom / max(1, n_dom)) / 1e3
     m_avg = r["units"][dom] / max(1, n_dom)
-    ph_dom = Ph if not dom.startswith("coarse") else \
-        ((Nx + 1) // 2) * ((Ny + 1) // 2) * (((Nz + 1) // 2) // 2 + 1)
+    ph_dom = Ph
     achieved = kernel_bytes(dom, ph_dom, V, m_avg) / avg_s / 1e9 if avg_s > 0 else 0.0
     # measured HBM bytes per launch of this kernel from the committed counter
     # runs (FETCH_SIZE/WRITE_SIZE scaled by a copy kernel), scaled to this

@@ -2272,14 +2272,14 @@ int spk_timing_units(spk_ctx* c, const char* kernel, int64_t* units) {
   return guarded(c, [&]() {
     const std::string k = kernel ? kernel : "";
     long long v = 0;
-    if (k == "feas_x" || k == "feas_yz" || k == "coarse_feas_x" || k == "coarse_feas_yz") {
-      // counted on the device: pairs the coarse pass pruned never run
+    if (k == "feas_x" || k == "feas_yz") {
+      // counted on the device: pairs the column test excluded never run
       unsigned long long h[2] = {0, 0};
       if (c->pairs.p) {
         download(c, h, c->pairs.p, sizeof(h));
         sync(c);
       }
-      v = (long long)h[k.rfind("coarse", 0) == 0 ? 1 : 0];
+      v = (long long)h[0];
     } else {
       auto it = c->units.find(k);
       v = it == c->units.end() ? 0 : it->second;

@@ -1151,42 +1151,91 @@ k_core_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od,
   if (lane == 0) cm[((size_t)o * ccx + X) * ccy + Y] = r;
 }
 
-// grid (ceil(cx * cy / 256), m): thread (Qx, Qy) clears the heights qz that
-// some core run blocks and flags the orientation when any valid qz is left
+// grid (ceil(cx * cy / 256), m).  Block (b, o): orientation o, offsets
+// q = 256 b + thread over the valid (Qx, Qy) rectangle.  The block first lists
+// the core's runs in LDS (order is irrelevant: they are OR-ed); each thread
+// then ORs the heights every run blocks, stops once every valid height is
+// blocked, and flags the orientation when one is left.  A block starts only
+// while the orientation is still unflagged.
+constexpr int kColRuns = 2048;
+
+__device__ __forceinline__ int col_runs_of(M128 k, int X, int Y, unsigned* runs, int* nruns) {
+  int n = 0;
+  while (k.lo | k.hi) {
+    const int a = m_ctz(k);
+    const M128 rest = m_shr(k, a);
+    const int L = m_ctz(M128{~rest.lo, ~rest.hi});
+    const int slot = atomicAdd(nruns, 1);
+    if (slot < kColRuns)
+      runs[slot] = (unsigned)X | ((unsigned)Y << 7) | ((unsigned)a << 14) | ((unsigned)(L - 1) << 21);
+    k.lo &= ~bits64(min(a, 64), min(a + L, 64));
+    k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
+    ++n;
+  }
+  return n;
+}
+
 __global__ void __launch_bounds__(256)
 k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restrict__ cm,
               int ccx, int ccy, const OrientDesc* __restrict__ od, int m, int* __restrict__ cfs,
               const int* __restrict__ ctl) {
   if (search_off(ctl)) return;
-  const int o = blockIdx.y;
+  const int o = blockIdx.y, tid = threadIdx.x;
   if (o >= m) return;
   const OrientDesc d = od[o];
-  const int q = blockIdx.x * 256 + threadIdx.x;
-  const int Qx = q / cy, Qy = q - Qx * cy;
   const int nqx = d.nvx > 0 ? (d.nvx - 1) / 2 + 1 : 0, nqy = d.nvy > 0 ? (d.nvy - 1) / 2 + 1 : 0;
-  if (Qx >= nqx || Qy >= nqy || d.nvz <= 0) return;
+  const int nq = nqx * nqy;
+  const int q0 = blockIdx.x * 256;
+  if (q0 >= nq || d.nvz <= 0) return;
+  volatile int* flag = cfs + kFeasHeader + o;
+  __shared__ unsigned runs[kColRuns];
+  __shared__ int nruns, skip;
+  if (tid == 0) {
+    nruns = 0;
+    skip = *flag;   // one read, so that the whole block takes the same branch
+  }
+  __syncthreads();
+  if (skip) return;
   const int kx = (d.ex + 1) / 2, ky = (d.ey + 1) / 2;
-  M128 bad{0ULL, 0ULL};
   const M128* core = cm + (size_t)o * ccx * ccy;
-  for (int X = 0; X < kx; ++X)
-    for (int Y = 0; Y < ky; ++Y) {
-      M128 k = core[X * ccy + Y];
-      if (!(k.lo | k.hi)) continue;
-      const M128 t = cols[(Qx + X) * cy + (Qy + Y)];
-      while (k.lo | k.hi) {
-        const int a = m_ctz(k);
-        const M128 rest = m_shr(k, a);
-        const int L = m_ctz(M128{~rest.lo, ~rest.hi});
-        bad = m_or(bad, m_shr(m_window(t, L), a));
-        // clear the run: bits a .. a+L-1
-        k.lo &= ~bits64(min(a, 64), min(a + L, 64));
-        k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
-      }
-    }
+  for (int c = tid; c < kx * ky; c += 256) {
+    const int X = c / ky, Y = c - X * ky;
+    col_runs_of(core[X * ccy + Y], X, Y, runs, &nruns);
+  }
+  __syncthreads();
+  const int nr = nruns;
+  const int q = q0 + tid;
+  if (q >= nq) return;
+  const int Qx = q / nqy, Qy = q - Qx * nqy;
   const int nv = d.nvz;
   const unsigned long long vlo = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1ULL);
   const unsigned long long vhi = nv >= 128 ? ~0ULL : (nv > 64 ? ((1ULL << (nv - 64)) - 1ULL) : 0ULL);
-  if ((~bad.lo & vlo) | (~bad.hi & vhi)) cfs[kFeasHeader + o] = 1;
+  M128 bad{0ULL, 0ULL};
+  if (nr <= kColRuns) {
+    for (int r = 0; r < nr; ++r) {
+      const unsigned e = runs[r];
+      const int X = e & 127, Y = (e >> 7) & 127, a = (e >> 14) & 127, L = ((e >> 21) & 127) + 1;
+      const M128 t = cols[(Qx + X) * cy + (Qy + Y)];
+      bad = m_or(bad, m_shr(m_window(t, L), a));
+      if ((r & 15) == 15 && !((~bad.lo & vlo) | (~bad.hi & vhi))) return;   // all blocked
+    }
+  } else {   // more runs than the LDS list holds: straight from the core masks
+    for (int X = 0; X < kx; ++X)
+      for (int Y = 0; Y < ky; ++Y) {
+        M128 k = core[X * ccy + Y];
+        if (!(k.lo | k.hi)) continue;
+        const M128 t = cols[(Qx + X) * cy + (Qy + Y)];
+        while (k.lo | k.hi) {
+          const int a = m_ctz(k);
+          const M128 rest = m_shr(k, a);
+          const int L = m_ctz(M128{~rest.lo, ~rest.hi});
+          bad = m_or(bad, m_shr(m_window(t, L), a));
+          k.lo &= ~bits64(min(a, 64), min(a + L, 64));
+          k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
+        }
+      }
+  }
+  if ((~bad.lo & vlo) | (~bad.hi & vhi)) *flag = 1;
 }
 
 // Column test decision, after k_column_test (cfs: its flags; its residual
