@@ -344,7 +344,7 @@ void plan_coarse(spk_ctx* c) {
   if (!q.on || c->split || c->nz > 128) return;
   q.cx = (c->nx + 1) / 2;
   q.cy = (c->ny + 1) / 2;
-  q.cols.ensure((size_t)q.cx * q.cy);
+  q.cols.ensure((size_t)kColLevels * q.cx * q.cy);
   q.fs.ensure(kFeasHeader + 24);
   q.ready = true;
 }
@@ -438,9 +438,8 @@ void run_distance(spk_ctx* c, const int* d_grid, int* d_out, int* d_tmp, FieldSt
   const int fill = nx + ny + nz + 10;
   const int edt = metric == SPK_METRIC_EDT ? 1 : 0;
   const int big = edt ? fill * fill : fill;
-  launch(c, "stats_init", k_stats_init, dim3(1), dim3(64), 0, d_st, ctl);
   launch(c, "dist_zy", k_dist_zy, dim3(nx), dim3(kDistThreads), (size_t)ny * nz * sizeof(int),
-         (const int*)d_grid, d_tmp, nx, ny, nz, edt, sources, big, ctl);
+         (const int*)d_grid, d_tmp, nx, ny, nz, edt, sources, big, d_st, ctl);
   constexpr int TK = 16;
   launch(c, "dist_x", k_dist_x<TK>, dim3((nz + TK - 1) / TK, ny), dim3(kDistThreads),
          (size_t)nx * TK * sizeof(int), (const int*)d_tmp, d_out, d_st, nx, ny, nz, edt, fill,
@@ -774,7 +773,7 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const i
   q.cm.ensure((size_t)m * ccx * ccy);
   launch(c, "core_columns", k_core_columns, dim3((ccx * ccy + 3) / 4, m), dim3(256), 0,
          (const int*)c->items.p, od, m, q.cm.p, ccx, ccy, ctl);
-  launch(c, "column_test", k_column_test, dim3((q.cx * q.cy + 255) / 256, m), dim3(256), 0,
+  launch(c, "column_test", k_column_test, dim3((q.cx * q.cy + 63) / 64, m), dim3(256), 0,
          (const M128*)q.cols.p, q.cx, q.cy, (const M128*)q.cm.p, ccx, ccy, od, m, q.fs.p, ctl);
   launch(c, "coarse_decide", k_coarse_decide, dim3(1), dim3(64), 0, (const int*)q.fs.p, m,
          c->fstate.p, out, t, halt_not_found, (int*)ctl);

@@ -212,8 +212,14 @@ __device__ __forceinline__ int outward_min(const int* line, int stride, int n, i
 // one x-slice: init (sources) + z pass + y pass, in LDS; writes tmp int grid.
 __global__ void __launch_bounds__(kDistThreads)
 k_dist_zy(const int* __restrict__ tray, int* __restrict__ out, int nx, int ny, int nz, int edt,
-          int sources, int big, const int* __restrict__ ctl) {
+          int sources, int big, FieldStats* __restrict__ st, const int* __restrict__ ctl) {
   if (refresh_off(ctl)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {   // reduced into by the following k_dist_x
+    st->sum = 0ULL;
+    st->mn = INT_MAX;
+    st->mx = INT_MIN;
+    st->mxw = INT_MIN;
+  }
   extern __shared__ int sh[];
   const int i = blockIdx.x;
   const int tid = threadIdx.x;
@@ -1099,7 +1105,10 @@ __device__ __forceinline__ int m_ctz(M128 a) {
   return a.lo ? __ffsll((long long)a.lo) - 1 : (a.hi ? 64 + __ffsll((long long)a.hi) - 1 : 128);
 }
 
-// one wavefront per 2x2 column block: lanes over z, ballots form the masks
+// one wavefront per 2x2 column block: lanes over z, ballots form the masks.
+// cols holds kColLevels planes of cx*cy masks: plane k is the window-OR of
+// length 2^k of the block masks (plane 0 the masks themselves).
+constexpr int kColLevels = 8;
 __global__ void __launch_bounds__(256)
 k_tray_columns(const int* __restrict__ tray, int nx, int ny, int nz, M128* __restrict__ cols,
                int cx, int cy, const int* __restrict__ ctl) {
@@ -1120,7 +1129,11 @@ k_tray_columns(const int* __restrict__ tray, int nx, int ny, int nz, M128* __res
     const unsigned long long b = __ballot(occ);
     if (w == 0) m.lo = b; else m.hi = b;
   }
-  if (lane == 0) cols[col] = m;
+  if (lane < kColLevels) {
+    M128 w = m;
+    for (int k = 0; k < lane; ++k) w = m_or(w, m_shr(w, 1 << k));
+    cols[(size_t)lane * cx * cy + col] = w;
+  }
 }
 
 // core columns of m oriented items, cm[(o * ccx + X) * ccy + Y]; one
@@ -1151,16 +1164,17 @@ k_core_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od,
   if (lane == 0) cm[((size_t)o * ccx + X) * ccy + Y] = r;
 }
 
-// grid (ceil(cx * cy / 256), m).  Block (b, o): orientation o, offsets
-// q = 256 b + thread over the valid (Qx, Qy) rectangle.  The block first lists
-// the core's runs in LDS (order is irrelevant: they are OR-ed); each thread
-// then ORs the heights every run blocks, stops once every valid height is
-// blocked, and flags the orientation when one is left.  A block starts only
-// while the orientation is still unflagged.
+// grid (ceil(4 * cx * cy / 256), m).  Block (b, o): orientation o, 64
+// offsets q = 64 b + thread / 4 over the valid (Qx, Qy) rectangle, four lanes
+// per offset.  The block first lists the core's runs in LDS (order is
+// irrelevant: they are OR-ed); lane j of an offset ORs the heights that runs
+// j, j+4, ... block (window of length L = level 2^k, plus a shifted copy when
+// L is not a power of two), the four lanes combine every 16 runs and stop once
+// every valid height is blocked, and the orientation is flagged when one is
+// left.  A block starts only while the orientation is still unflagged.
 constexpr int kColRuns = 2048;
 
-__device__ __forceinline__ int col_runs_of(M128 k, int X, int Y, unsigned* runs, int* nruns) {
-  int n = 0;
+__device__ __forceinline__ void col_runs_of(M128 k, int X, int Y, unsigned* runs, int* nruns) {
   while (k.lo | k.hi) {
     const int a = m_ctz(k);
     const M128 rest = m_shr(k, a);
@@ -1170,9 +1184,18 @@ __device__ __forceinline__ int col_runs_of(M128 k, int X, int Y, unsigned* runs,
       runs[slot] = (unsigned)X | ((unsigned)Y << 7) | ((unsigned)a << 14) | ((unsigned)(L - 1) << 21);
     k.lo &= ~bits64(min(a, 64), min(a + L, 64));
     k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
-    ++n;
   }
-  return n;
+}
+
+// heights blocked by a run [a, a+L) of core column (X, Y) at offset (Qx, Qy)
+__device__ __forceinline__ M128 run_blocks(const M128* __restrict__ cols, size_t plane, int cy,
+                                           int Qx, int Qy, int X, int Y, int a, int L) {
+  const int k = 31 - __clz(L);
+  const size_t col = (size_t)(Qx + X) * cy + (Qy + Y);
+  M128 w = cols[(size_t)k * plane + col];
+  const int p = 1 << k;
+  if (L > p) w = m_or(w, m_shr(w, L - p));
+  return m_shr(w, a);
 }
 
 __global__ void __launch_bounds__(256)
@@ -1185,7 +1208,7 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
   const OrientDesc d = od[o];
   const int nqx = d.nvx > 0 ? (d.nvx - 1) / 2 + 1 : 0, nqy = d.nvy > 0 ? (d.nvy - 1) / 2 + 1 : 0;
   const int nq = nqx * nqy;
-  const int q0 = blockIdx.x * 256;
+  const int q0 = blockIdx.x * 64;
   if (q0 >= nq || d.nvz <= 0) return;
   volatile int* flag = cfs + kFeasHeader + o;
   __shared__ unsigned runs[kColRuns];
@@ -1204,37 +1227,52 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
   }
   __syncthreads();
   const int nr = nruns;
-  const int q = q0 + tid;
-  if (q >= nq) return;
-  const int Qx = q / nqy, Qy = q - Qx * nqy;
+  const int q = q0 + (tid >> 2), j = tid & 3;
+  const bool active = q < nq;
+  const int Qx = active ? q / nqy : 0, Qy = active ? q - Qx * nqy : 0;
   const int nv = d.nvz;
   const unsigned long long vlo = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1ULL);
   const unsigned long long vhi = nv >= 128 ? ~0ULL : (nv > 64 ? ((1ULL << (nv - 64)) - 1ULL) : 0ULL);
+  const size_t plane = (size_t)cx * cy;
   M128 bad{0ULL, 0ULL};
   if (nr <= kColRuns) {
-    for (int r = 0; r < nr; ++r) {
-      const unsigned e = runs[r];
-      const int X = e & 127, Y = (e >> 7) & 127, a = (e >> 14) & 127, L = ((e >> 21) & 127) + 1;
-      const M128 t = cols[(Qx + X) * cy + (Qy + Y)];
-      bad = m_or(bad, m_shr(m_window(t, L), a));
-      if ((r & 15) == 15 && !((~bad.lo & vlo) | (~bad.hi & vhi))) return;   // all blocked
-    }
-  } else {   // more runs than the LDS list holds: straight from the core masks
-    for (int X = 0; X < kx; ++X)
-      for (int Y = 0; Y < ky; ++Y) {
-        M128 k = core[X * ccy + Y];
-        if (!(k.lo | k.hi)) continue;
-        const M128 t = cols[(Qx + X) * cy + (Qy + Y)];
-        while (k.lo | k.hi) {
-          const int a = m_ctz(k);
-          const M128 rest = m_shr(k, a);
-          const int L = m_ctz(M128{~rest.lo, ~rest.hi});
-          bad = m_or(bad, m_shr(m_window(t, L), a));
-          k.lo &= ~bits64(min(a, 64), min(a + L, 64));
-          k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
+    // rounds of 16 runs per lane; all four lanes of an offset run the same
+    // number of rounds (the combine uses shuffles within the quad)
+    bool done = !active;
+    for (int r0 = 0; r0 < nr; r0 += 64) {
+      if (!done) {
+        for (int r = r0 + j; r < min(nr, r0 + 64); r += 4) {
+          const unsigned e = runs[r];
+          bad = m_or(bad, run_blocks(cols, plane, cy, Qx, Qy, e & 127, (e >> 7) & 127,
+                                     (e >> 14) & 127, ((e >> 21) & 127) + 1));
         }
       }
+      M128 u = bad;
+      for (int s = 1; s < 4; s <<= 1) {
+        u.lo |= __shfl_xor(u.lo, s, 64);
+        u.hi |= __shfl_xor(u.hi, s, 64);
+      }
+      bad = u;
+      if (!((~bad.lo & vlo) | (~bad.hi & vhi))) done = true;
+      if (__all(done)) break;
+    }
+    if (active && j == 0 && ((~bad.lo & vlo) | (~bad.hi & vhi))) *flag = 1;
+    return;
   }
+  // more runs than the LDS list holds: straight from the core masks
+  if (!active || j != 0) return;
+  for (int X = 0; X < kx; ++X)
+    for (int Y = 0; Y < ky; ++Y) {
+      M128 k = core[X * ccy + Y];
+      while (k.lo | k.hi) {
+        const int a = m_ctz(k);
+        const M128 rest = m_shr(k, a);
+        const int L = m_ctz(M128{~rest.lo, ~rest.hi});
+        bad = m_or(bad, run_blocks(cols, plane, cy, Qx, Qy, X, Y, a, L));
+        k.lo &= ~bits64(min(a, 64), min(a + L, 64));
+        k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
+      }
+    }
   if ((~bad.lo & vlo) | (~bad.hi & vhi)) *flag = 1;
 }
 
@@ -1297,13 +1335,14 @@ __global__ void k_feas_decide(const int* __restrict__ fs, int m, PackOut* __rest
 // smaller score across orientations (list order).
 // ---------------------------------------------------------------------------
 constexpr int kSelectThreads = 1024;
-__global__ void __launch_bounds__(kSelectThreads)
-k_select(const WgBest* __restrict__ rec, const OrientDesc* __restrict__ od, int n_orient,
-         int rec_stride, int rows_per_rec, WgBest* __restrict__ per_orient,
-         SearchOut* __restrict__ out, long long* __restrict__ ver, const int* __restrict__ ctl) {
-  if (search_off(ctl)) return;
+// the selection of one block of kSelectThreads; the winner is written to *out
+// and, when sel != nullptr, to *sel (shared memory) for the same block
+__device__ __forceinline__ void select_body(const WgBest* __restrict__ rec,
+                                            const OrientDesc* __restrict__ od, int n_orient,
+                                            int rec_stride, int rows_per_rec,
+                                            WgBest* __restrict__ per_orient,
+                                            SearchOut* __restrict__ out, SearchOut* sel) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (tid < 3) ver[tid] = 0;   // accumulator of the following k_verify
   for (int o = wave; o < n_orient; o += kSelectThreads / 64) {
     const OrientDesc d = od[o];
     // slab plan: one record per valid x slice; split plan: one per block of
@@ -1357,7 +1396,17 @@ k_select(const WgBest* __restrict__ rec, const OrientDesc* __restrict__ od, int 
     }
     if (!r.found) r.score = 0.0;
     *out = r;
+    if (sel) *sel = r;
   }
+}
+
+__global__ void __launch_bounds__(kSelectThreads)
+k_select(const WgBest* __restrict__ rec, const OrientDesc* __restrict__ od, int n_orient,
+         int rec_stride, int rows_per_rec, WgBest* __restrict__ per_orient,
+         SearchOut* __restrict__ out, long long* __restrict__ ver, const int* __restrict__ ctl) {
+  if (search_off(ctl)) return;
+  if (threadIdx.x < 3) ver[threadIdx.x] = 0;   // accumulator of the following k_verify
+  select_body(rec, od, n_orient, rec_stride, rows_per_rec, per_orient, out, nullptr);
 }
 
 // ---------------------------------------------------------------------------
