@@ -183,6 +183,8 @@ struct spk_ctx {
   int yz_rmax = kMaxRadix;               // option "yz_radix": largest y/z butterfly
   int pad_y = 0, pad_z = 0;              // options "pad_y", "pad_z": padded sizes (0 = smallest smooth)
   int feas_group = 0;                    // option "feas_group": pairs per feasibility launch pair
+  int dist_y = 2;                        // option "dist_y": 1 envelope, 2 full scan
+  int dist_x = 0;                        // option "dist_x": 0 outward search, 2 full scan
   // column test before the feasibility pass (option "coarse")
   struct Coarse {
     bool on = true;
@@ -419,7 +421,6 @@ void plan_tray(spk_ctx* c) {
     allow_shmem(k_tray_slab_fwd, dyn);
     allow_shmem(k_item_slab_fwd, dyn);
     allow_shmem(k_score_yz, dyn);
-    allow_shmem(k_dist_zy, dyn);
     allow_shmem(k_corr_x, dyn);
     allow_shmem(k_tray_x_fwd, dyn);
     allow_shmem(k_feas_x, dyn);
@@ -438,12 +439,27 @@ void run_distance(spk_ctx* c, const int* d_grid, int* d_out, int* d_tmp, FieldSt
   const int fill = nx + ny + nz + 10;
   const int edt = metric == SPK_METRIC_EDT ? 1 : 0;
   const int big = edt ? fill * fill : fill;
-  launch(c, "dist_zy", k_dist_zy, dim3(nx), dim3(kDistThreads), (size_t)ny * nz * sizeof(int),
-         (const int*)d_grid, d_tmp, nx, ny, nz, edt, sources, big, d_st, ctl);
+  // y pass: c->dist_y (1 envelope, 2 full scan); the envelope's scratch
+  // (ny*nz int2 behind the slab) needs planes up to 12288 voxels, else the
+  // outward search (0)
+  int ymode = c->dist_y;
+  if (ymode == 1 && (long long)ny * nz > 12288) ymode = 0;
+  const size_t sh = (size_t)ny * nz * (ymode == 1 ? 3 : 1) * sizeof(int);
+  auto zy = [&](auto kern) {
+    allow_shmem(kern, sh);
+    launch(c, "dist_zy", kern, dim3(nx), dim3(kDistThreads), sh, (const int*)d_grid, d_tmp, nx,
+           ny, nz, sources, big, d_st, ctl);
+  };
+  if (edt) {
+    if (ymode == 1) zy(k_dist_zy<1, 1>); else if (ymode == 2) zy(k_dist_zy<1, 2>); else zy(k_dist_zy<1, 0>);
+  } else {
+    if (ymode == 1) zy(k_dist_zy<0, 1>); else if (ymode == 2) zy(k_dist_zy<0, 2>); else zy(k_dist_zy<0, 0>);
+  }
   constexpr int TK = 16;
-  launch(c, "dist_x", k_dist_x<TK>, dim3((nz + TK - 1) / TK, ny), dim3(kDistThreads),
-         (size_t)nx * TK * sizeof(int), (const int*)d_tmp, d_out, d_st, nx, ny, nz, edt, fill,
-         ctl);
+  auto xk = c->dist_x == 2 ? (edt ? k_dist_x<TK, 1, 2> : k_dist_x<TK, 0, 2>)
+                           : (edt ? k_dist_x<TK, 1, 0> : k_dist_x<TK, 0, 0>);
+  launch(c, "dist_x", xk, dim3((nz + TK - 1) / TK, ny), dim3(kDistThreads),
+         (size_t)nx * TK * sizeof(int), (const int*)d_tmp, d_out, d_st, nx, ny, nz, fill, ctl);
 }
 
 void build_spectra(spk_ctx* c, const int* ctl = nullptr) {
@@ -709,7 +725,8 @@ void enqueue_item_spectra(spk_ctx* c, const OrientDesc* od, int m, int max_ex, c
   } else {
     launch(c, "item_slab_fwd", k_item_slab_fwd, dim3(max_ex, m), dim3(kSlabThreads),
            slab_shmem(c), (const int*)c->items.p, od, c->ispec.p, max_ex, c->py, c->pz,
-           tw + c->px.n, tw + c->px.n + py, ctl);
+           tw + c->px.n, tw + c->px.n + py,
+           c->co.ready ? (const int*)c->co.fs.p + kFeasHeader : (const int*)nullptr, ctl);
   }
 }
 
@@ -2344,6 +2361,17 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->co.on = value != 0.0;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "dist_x") {
+    if (value != 0 && value != 2) return fail(SPK_EINVAL, "dist_x must be 0 or 2");
+    c->dist_x = (int)value;
+    return SPK_OK;
+  }
+  if (std::string(name) == "dist_y") {
+    if (value != 0 && value != 1 && value != 2)
+      return fail(SPK_EINVAL, "dist_y must be 0 (outward), 1 (envelope) or 2 (full scan)");
+    c->dist_y = (int)value;
     return SPK_OK;
   }
   if (std::string(name) == "feas_group") {

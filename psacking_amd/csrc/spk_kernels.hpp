@@ -190,13 +190,15 @@ __device__ __forceinline__ bool better(double s1, int i1, double s2, int i2) {
 
 __device__ __forceinline__ int dcost(int d, int edt) { return edt ? d * d : (d < 0 ? -d : d); }
 
-// min over k' of f(k') + cost(k - k') along a strided line of n values, f >= 0
-__device__ __forceinline__ int outward_min(const int* line, int stride, int n, int k, int edt) {
+// min over k' of f(k') + cost(k - k') along a strided line of n values,
+// f >= floor >= 0 (floor: a lower bound of the line, e.g. its minimum)
+__device__ __forceinline__ int outward_min(const int* line, int stride, int n, int k, int edt,
+                                           int floor = 0) {
   int best = line[k * stride];
   for (int d = 1;; ++d) {
     const int c = dcost(d, edt);
     const bool lo = k - d >= 0, hi = k + d < n;
-    if (c >= best || (!lo && !hi)) break;
+    if (c + floor >= best || (!lo && !hi)) break;
     if (lo) {
       const int v = line[(k - d) * stride] + c;
       best = v < best ? v : best;
@@ -209,10 +211,74 @@ __device__ __forceinline__ int outward_min(const int* line, int stride, int n, i
   return best;
 }
 
+// One line of the exact min-plus transform g(q) = min_r f(r) + cost(q - r),
+// f >= 0 read at f[r * fs], r < n; put(q, g) receives every output in order.
+// Squared Euclidean cost: the lower envelope of the parabolas f(r) + (q-r)^2
+// (Felzenszwalb & Huttenlocher, "Distance transforms of sampled functions",
+// 2012), breakpoints compared exactly in 64-bit integers; v (stride vs)
+// holds the envelope's apexes.  L1 cost: a forward and a backward sweep
+// (g = min(f, neighbour + 1)), both through v as scratch.  O(n) per line.
+template <typename Put>
+__device__ __forceinline__ void line_transform(const int* f, int fs, int n, int edt, int2* v,
+                                               int vs, Put put) {
+  if (!edt) {
+    int g = INT_MAX - 1;
+    for (int q = 0; q < n; ++q) {
+      const int a = f[q * fs];
+      g = a < g + 1 ? a : g + 1;
+      v[q * vs].x = g;
+    }
+    g = INT_MAX - 1;
+    for (int q = n - 1; q >= 0; --q) {
+      const int a = v[q * vs].x;
+      g = a < g + 1 ? a : g + 1;
+      v[q * vs].x = g;
+    }
+    for (int q = 0; q < n; ++q) put(q, v[q * vs].x);
+    return;
+  }
+  // apexes with their F = f(r) + r^2 (< 2^31 for the fields built here)
+  int k = 0;
+  v[0] = make_int2(0, f[0]);
+  for (int q = 1; q < n; ++q) {
+    const int Fq = f[q * fs] + q * q;
+    int2 b = v[k * vs];
+    while (k > 0) {
+      // apex b leaves the envelope when the breakpoint of (b, q) is not past
+      // the breakpoint of (a, b): (Fq-Fb)/(2(q-b)) <= (Fb-Fa)/(2(b-a))
+      const int2 a = v[(k - 1) * vs];
+      if ((long long)(Fq - b.y) * (b.x - a.x) <= (long long)(b.y - a.y) * (q - b.x)) {
+        --k;
+        b = a;
+      } else {
+        break;
+      }
+    }
+    ++k;
+    v[k * vs] = make_int2(q, Fq);
+  }
+  int j = 0;
+  int2 a = v[0];
+  int2 b = k > 0 ? v[vs] : a;
+  for (int q = 0; q < n; ++q) {
+    // advance while the breakpoint of (a, b) lies before q
+    while (j < k && (long long)(b.y - a.y) < 2LL * q * (b.x - a.x)) {
+      ++j;
+      a = b;
+      if (j < k) b = v[(j + 1) * vs];
+    }
+    put(q, a.y - a.x * a.x + (q - a.x) * (q - a.x));
+  }
+}
+
 // one x-slice: init (sources) + z pass + y pass, in LDS; writes tmp int grid.
+// Dynamic LDS: ny*nz ints of slab, then (envelope == 1) ny*nz int2 of
+// envelope scratch for the y pass.
+template <int EDT, int YMODE>
 __global__ void __launch_bounds__(kDistThreads)
-k_dist_zy(const int* __restrict__ tray, int* __restrict__ out, int nx, int ny, int nz, int edt,
+k_dist_zy(const int* __restrict__ tray, int* __restrict__ out, int nx, int ny, int nz,
           int sources, int big, FieldStats* __restrict__ st, const int* __restrict__ ctl) {
+  constexpr int edt = EDT;
   if (refresh_off(ctl)) return;
   if (blockIdx.x == 0 && threadIdx.x == 0) {   // reduced into by the following k_dist_x
     st->sum = 0ULL;
@@ -310,29 +376,57 @@ k_dist_zy(const int* __restrict__ tray, int* __restrict__ out, int nx, int ny, i
     }
   }
   __syncthreads();
-  constexpr int MAXR = 32;   // elements per thread: ny*nz <= 32768
-  int res[MAXR];
-  // y pass (columns)
-#pragma unroll
-  for (int t = 0; t < MAXR; ++t) {
-    const int e = tid + t * kDistThreads;
-    if (e < S) {
-      const int j = e / nz, k = e - j * nz;
-      res[t] = outward_min(sh + k, nz, ny, j, edt);
+  int* dst = out + (size_t)i * S;
+  if constexpr (YMODE == 1) {
+    // y pass: one thread per z column, envelope scratch behind the slab
+    int2* scratch = reinterpret_cast<int2*>(sh + S);
+    for (int k = tid; k < nz; k += kDistThreads)
+      line_transform(sh + k, nz, ny, edt, scratch + k, nz,
+                     [&](int j, int g) { dst[j * nz + k] = g; });
+  } else {
+    // y pass per voxel: a full scan (YMODE 2) or an outward search (YMODE 0)
+    // bounded by the column minimum (no offset can beat it)
+    __shared__ int cmin[1024];
+    if constexpr (YMODE == 0) {
+      for (int k = tid; k < nz; k += kDistThreads) cmin[k] = INT_MAX;
+      __syncthreads();
+      for (int e = tid; e < S; e += kDistThreads) atomicMin(&cmin[e % nz], sh[e]);
+      __syncthreads();
     }
-  }
+    constexpr int MAXR = 32;   // elements per thread: ny*nz <= 32768
+    int res[MAXR];
 #pragma unroll
-  for (int t = 0; t < MAXR; ++t) {
-    const int e = tid + t * kDistThreads;
-    if (e < S) out[(size_t)i * S + e] = res[t];
+    for (int t = 0; t < MAXR; ++t) {
+      const int e = tid + t * kDistThreads;
+      if (e < S) {
+        const int j = e / nz, k = e - j * nz;
+        if constexpr (YMODE == 2) {
+          int best = sh[e];
+          for (int jj = 0; jj < ny; ++jj) {
+            const int c = sh[jj * nz + k] + dcost(j - jj, EDT);
+            best = c < best ? c : best;
+          }
+          res[t] = best;
+        } else {
+          res[t] = outward_min(sh + k, nz, ny, j, EDT, cmin[k]);
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < MAXR; ++t) {
+      const int e = tid + t * kDistThreads;
+      if (e < S) dst[e] = res[t];
+    }
   }
 }
 
 // x pass for a tile of TK consecutive k at fixed j; finalize and reduce stats.
-template <int TK>
+// XMODE 0: outward search per voxel (cost ~ distance), 2: full scan (cost nx)
+template <int TK, int EDT, int XMODE>
 __global__ void __launch_bounds__(kDistThreads)
 k_dist_x(const int* __restrict__ in, int* __restrict__ phi, FieldStats* __restrict__ st, int nx,
-         int ny, int nz, int edt, int fill, const int* __restrict__ ctl) {
+         int ny, int nz, int fill, const int* __restrict__ ctl) {
+  constexpr int edt = EDT;
   if (refresh_off(ctl)) return;
   extern __shared__ int sh[];
   const int j = blockIdx.y;
@@ -354,6 +448,13 @@ k_dist_x(const int* __restrict__ in, int* __restrict__ phi, FieldStats* __restri
       if (e < E) sh[e] = v[s];
     }
   }
+  __shared__ int lmin[TK];   // minimum of each line: no offset can beat it
+  if (tid < TK) lmin[tid] = INT_MAX;
+  __syncthreads();
+  for (int e = tid; e < E; e += kDistThreads) {
+    const int t = e % TK;
+    if (t < tk) atomicMin(&lmin[t], sh[e]);
+  }
   __syncthreads();
   long long lsum = 0;
   int lmn = INT_MAX, lmx = INT_MIN, lmw = INT_MIN;
@@ -361,7 +462,16 @@ k_dist_x(const int* __restrict__ in, int* __restrict__ phi, FieldStats* __restri
   for (int e = tid; e < E; e += kDistThreads) {
     const int x = e / TK, t = e - x * TK;
     if (t >= tk) continue;
-    const int best = outward_min(sh + t, TK, nx, x, edt);
+    int best;
+    if constexpr (XMODE == 2) {
+      best = sh[e];
+      for (int xx = 0; xx < nx; ++xx) {
+        const int c = sh[xx * TK + t] + dcost(x - xx, EDT);
+        best = c < best ? c : best;
+      }
+    } else {
+      best = outward_min(sh + t, TK, nx, x, EDT, lmin[t]);
+    }
     int v;
     if (edt) {
       v = (best >= fill * fill) ? fill : (int)rint(sqrt((double)best));
@@ -560,12 +670,15 @@ k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
 // ---------------------------------------------------------------------------
 // item spectra: grid (max_ex, n_orient).  I~[o][x][ky][kz] = FFT_yz(item_o[x])
 // ---------------------------------------------------------------------------
+// cf (nullable): column-test flags of the list; with CTL_PRUNE set, excluded
+// orientations need no spectra (neither pass reads them)
 __global__ void __launch_bounds__(kSlabThreads)
 k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od,
                 float2* __restrict__ ispec, int max_ex, AxisPlan py_, AxisPlan pz_,
                 const float2* __restrict__ tw_y, const float2* __restrict__ tw_z,
-                const int* __restrict__ ctl) {
+                const int* __restrict__ cf, const int* __restrict__ ctl) {
   if (search_off(ctl) || coarse_done(ctl)) return;
+  if (cf && ctl && ctl[CTL_PRUNE] && !cf[blockIdx.y]) return;
   extern __shared__ float2 smem[];
   const int py = py_.n, pz = pz_.n;
   const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
@@ -925,8 +1038,11 @@ k_feas_x(const float2* __restrict__ g1, const float2* __restrict__ ispec,
   }
   for (int p = p0; p < p1; ++p) {
     const int oa = 2 * p, ob = 2 * p + 1;
-    const bool hasb = ob < m;
-    if (prune && !cf[oa] && !(hasb && cf[ob])) continue;
+    // an excluded orientation has no spectra (k_item_slab_fwd skipped it):
+    // its half of the pair is zero
+    const bool use_a = !prune || cf[oa];
+    const bool hasb = ob < m && (!prune || cf[ob]);
+    if (!use_a && !hasb) continue;
     const OrientDesc da = od[oa];
     const OrientDesc db = od[hasb ? ob : oa];
     const float2* sa = ispec + (size_t)oa * max_ex * py * pz;
@@ -940,7 +1056,7 @@ k_feas_x(const float2* __restrict__ g1, const float2* __restrict__ ispec,
         const int kz = kz0 + t;
         float2 va = make_float2(0.f, 0.f), vb = make_float2(0.f, 0.f);
         if (kz < pz) {
-          if (x < da.ex) va = sa[((size_t)x * py + ky) * pz + kz];
+          if (use_a && x < da.ex) va = sa[((size_t)x * py + ky) * pz + kz];
           if (hasb && x < db.ex) vb = sb[((size_t)x * py + ky) * pz + kz];
         }
         buf[e] = make_float2(va.x - vb.y, va.y + vb.x);   // va + i vb
@@ -955,7 +1071,7 @@ k_feas_x(const float2* __restrict__ g1, const float2* __restrict__ ispec,
     }
     __syncthreads();
     lds_fft<false, kPencilThreads, kPencilMaxEl, kPencilRadix>(buf, px_, T, T, 1, false, twx, tid);
-    const int nvx = max(da.nvx, hasb ? db.nvx : 0);
+    const int nvx = max(use_a ? da.nvx : 0, hasb ? db.nvx : 0);
     float2* d1 = y1 + (size_t)p * nx * py * pz;
     for (int e = tid; e < nvx * T; e += kPencilThreads) {
       const int x = e / T, t = e - x * T;
@@ -979,13 +1095,15 @@ k_feas_yz(const float2* __restrict__ y1, const OrientDesc* __restrict__ od, int 
   const int py = py_.n, pz = pz_.n;
   const int x = blockIdx.x, p = blockIdx.y, tid = threadIdx.x;
   const int oa = 2 * p, ob = 2 * p + 1;
-  const bool hasb = ob < m;
-  if (!coarse_pass && cf && ctl && ctl[CTL_PRUNE] && !cf[oa] && !(hasb && cf[ob])) return;
+  const bool prune = !coarse_pass && cf && ctl && ctl[CTL_PRUNE];
+  const bool use_a = !prune || cf[oa];
+  const bool hasb = ob < m && (!prune || cf[ob]);
+  if (!use_a && !hasb) return;
   // pairs transformed (both passes), counted when timing asks for it
   if (pairs && x == 0 && tid == 0) atomicAdd(pairs, 1ULL);
   const OrientDesc da = od[oa];
   const OrientDesc db = od[hasb ? ob : oa];
-  const bool xa = x < da.nvx, xb = hasb && x < db.nvx;
+  const bool xa = use_a && x < da.nvx, xb = hasb && x < db.nvx;
   if (!xa && !xb) return;
   float2* twy = smem;
   float2* twz = twy + py;

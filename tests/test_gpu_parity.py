@@ -545,3 +545,25 @@ def test_feasibility_flags_match_oracle(P):
                 excluded += not column[o]
                 feasible += bool(full[o])
     assert checked > 500 and excluded > 50 and feasible > 50
+
+
+@pytest.mark.parametrize("dist_y,dist_x", [(0, 0), (1, 0), (2, 0), (2, 2)])
+def test_distance_modes_match_oracle(P, dist_y, dist_x):
+    """Every y-pass (outward search, Felzenszwalb envelope, full scan) and
+    x-pass (outward search bounded by the line minimum, full scan) variant of
+    the distance field gives the oracle's field, both metrics, on an empty, a
+    scattered and a filled-from-below tray."""
+    rng = np.random.default_rng(83)
+    shape = (37, 29, 23)
+    trays = [np.zeros(shape, np.int32), (rng.random(shape) < 0.02).astype(np.int32)]
+    t = (rng.random(shape) < 0.01).astype(np.int32)
+    t[:, :, :9] = 2
+    trays.append(t)
+    for metric in ("edt", "l1"):
+        for k, tray in enumerate(trays):
+            say("distance modes", dist_y, dist_x, metric, k)
+            s = P.Session(0)
+            s.set_option("dist_y", dist_y)
+            s.set_option("dist_x", dist_x)
+            s.set_tray(tray, metric, "nonzero", 0)
+            np.testing.assert_array_equal(s.distance(), O.distance_field(tray, metric, "nonzero"))
