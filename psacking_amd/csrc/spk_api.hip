@@ -1119,7 +1119,7 @@ void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e
     if (feas) {
       enqueue_feas(c, od, m, max_ex, c->ctl.p);
       launch(c, "feas_decide", k_feas_decide, dim3(1), dim3(64), 0, (const int*)c->fstate.p, m,
-             c->pout.p, t, c->resid_limit, c->ctl.p);
+             c->pout.p, t, c->resid_limit, c->co.ready ? c->co.fs.p : (int*)nullptr, c->ctl.p);
     }
     enqueue_scored(c, od, m, max_ex, max_e, penalty, c->ctl.p, nullptr, nullptr, nullptr);
   } else {

@@ -1429,13 +1429,23 @@ __global__ void k_coarse_decide(const int* __restrict__ cfs, int m, int* __restr
 // greedy loop: with the pass run and its residual within resid_limit, an
 // item with no feasible orientation is decided here (not found) and its
 // scored search, placement and refresh are skipped
+//
+// With cfs (the column-test flags the search prunes with), a trusted pass
+// that left some orientation feasible narrows the search to exactly those.
 __global__ void k_feas_decide(const int* __restrict__ fs, int m, PackOut* __restrict__ out, int t,
-                              float resid_limit, int* __restrict__ ctl) {
-  if (threadIdx.x != 0 || feas_off(ctl)) return;
+                              float resid_limit, int* __restrict__ cfs, int* __restrict__ ctl) {
+  if (threadIdx.x != 0 || feas_off(ctl) || coarse_done(ctl)) return;
   const float res = __int_as_float(fs[0]);
   int any = 0;
   for (int o = 0; o < m; ++o) any |= fs[kFeasHeader + o];
-  if (!(res <= resid_limit) || any) return;
+  if (!(res <= resid_limit)) return;
+  if (any) {
+    if (cfs) {
+      for (int o = 0; o < m; ++o) cfs[kFeasHeader + o] = fs[kFeasHeader + o];
+      ctl[CTL_PRUNE] = 1;
+    }
+    return;
+  }
   PackOut r{};
   r.so.found = 0;
   r.so.idx = INT_MAX;
