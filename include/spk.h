@@ -110,10 +110,11 @@ int spk_search_oriented(spk_ctx* ctx, const int32_t* item, int ix, int iy, int i
  * offset with no occupied tray voxel under the item (the full-resolution
  * collision pass, FP32 FFT rounded to integers; *residual = its largest
  * rounding distance, to be compared with option "resid_limit").
- * column[o] (may be NULL) = 0 when the column test already excludes o: the
- * tray reduced to 2x2 column blocks at full height against the item's
- * x-y-eroded core (exact bit-mask test, tray z <= 128; 1 everywhere when it
- * does not apply).  n <= 24; needs the slab plan. */
+ * column[o] (may be NULL): the same answer from the column test, an exact
+ * bit-mask test (tray z <= 128): the tray reduced to 2x2 column blocks at
+ * full height against the item's x-y-eroded core, then full-resolution
+ * columns on the offsets the blocks leave open (1 everywhere when it does not
+ * apply).  n <= 24; needs the slab plan. */
 int spk_feasible(spk_ctx* ctx, const int32_t* item, int ix, int iy, int iz,
                  const spk_orient* orients, int n, int32_t* column, int32_t* full,
                  float* residual);

@@ -191,7 +191,9 @@ struct spk_ctx {
     bool ready = false;                  // tray z <= 128 and the slab plan
     int cx = 0, cy = 0;                  // 2x2 column blocks of the tray
     DevBuf<M128> cols, cm;               // block masks, core masks of a search
+    DevBuf<M128> tf, im;                 // full-resolution tray columns, item columns
     DevBuf<int> fs;                      // residual slot (0), per-orientation flags
+    int fine_group = 64;                 // option "fine_group": lanes per full-resolution candidate
   } co;
   // free-space certificate
   bool phi_internal = false;             // phi is this library's distance field of the tray
@@ -347,6 +349,7 @@ void plan_coarse(spk_ctx* c) {
   q.cx = (c->nx + 1) / 2;
   q.cy = (c->ny + 1) / 2;
   q.cols.ensure((size_t)kColLevels * q.cx * q.cy);
+  q.tf.ensure((size_t)kColLevels * c->nx * c->ny);
   q.fs.ensure(kFeasHeader + 24);
   q.ready = true;
 }
@@ -358,6 +361,8 @@ void build_coarse_spectra(spk_ctx* c, const int* ctl) {
   const int n = q.cx * q.cy;
   launch(c, "tray_columns", k_tray_columns, dim3((n + 3) / 4), dim3(256), 0, (const int*)c->tray.p,
          c->nx, c->ny, c->nz, q.cols.p, q.cx, q.cy, ctl);
+  launch(c, "tray_fine", k_tray_fine, dim3((c->nx * c->ny + 3) / 4), dim3(256), 0,
+         (const int*)c->tray.p, c->nx, c->ny, c->nz, q.tf.p, ctl);
 }
 
 // ---- FFT planning for a tray --------------------------------------------
@@ -787,11 +792,25 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const i
   spk_ctx::Coarse& q = c->co;
   if (!q.ready) return;
   const int ccx = (max_ex + 1) / 2, ccy = q.cy;
+  const int mx = max_ex, my = c->ny;
   q.cm.ensure((size_t)m * ccx * ccy);
+  q.im.ensure((size_t)m * mx * my);
   launch(c, "core_columns", k_core_columns, dim3((ccx * ccy + 3) / 4, m), dim3(256), 0,
          (const int*)c->items.p, od, m, q.cm.p, ccx, ccy, ctl);
-  launch(c, "column_test", k_column_test, dim3((q.cx * q.cy + 63) / 64, m), dim3(256), 0,
-         (const M128*)q.cols.p, q.cx, q.cy, (const M128*)q.cm.p, ccx, ccy, od, m, q.fs.p, ctl);
+  launch(c, "item_columns", k_item_columns, dim3((mx * my + 3) / 4, m), dim3(256), 0,
+         (const int*)c->items.p, od, m, q.im.p, mx, my, ctl);
+  auto test = [&](auto kernel) {
+    launch(c, "column_test", kernel, dim3((q.cx * q.cy + 63) / 64, m), dim3(256), 0,
+           (const M128*)q.cols.p, q.cx, q.cy, (const M128*)q.cm.p, ccx, ccy, (const M128*)q.tf.p,
+           c->nx, c->ny, (const M128*)q.im.p, mx, my, od, m, q.fs.p, ctl);
+  };
+  switch (q.fine_group) {
+    case 4: test(k_column_test<4>); break;
+    case 8: test(k_column_test<8>); break;
+    case 32: test(k_column_test<32>); break;
+    case 16: test(k_column_test<16>); break;
+    default: test(k_column_test<64>); break;
+  }
   launch(c, "coarse_decide", k_coarse_decide, dim3(1), dim3(64), 0, (const int*)q.fs.p, m,
          c->fstate.p, out, t, halt_not_found, (int*)ctl);
 }
@@ -1116,7 +1135,8 @@ void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e
            (const FieldStats*)c->stats.p, item_bound, c->pout.p, t);
     enqueue_coarse(c, od, m, max_ex, c->ctl.p, c->pout.p, t, halt_not_found);
     enqueue_item_spectra(c, od, m, max_ex, c->ctl.p);
-    if (feas) {
+    // the column test's flags are exact: no FFT feasibility pass after it
+    if (feas && !c->co.ready) {
       enqueue_feas(c, od, m, max_ex, c->ctl.p);
       launch(c, "feas_decide", k_feas_decide, dim3(1), dim3(64), 0, (const int*)c->fstate.p, m,
              c->pout.p, t, c->resid_limit, c->co.ready ? c->co.fs.p : (int*)nullptr, c->ctl.p);
@@ -1393,14 +1413,20 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p,
            c->co.ready ? c->co.fs.p : (int*)nullptr, k, 2, (int*)nullptr,
            (const FieldStats*)nullptr, (const FieldStats*)nullptr, (PackOut*)nullptr, 0);
-    // the coarse pass writes CTL_PRUNE / CTL_COARSE_DONE of ctl_on; without
-    // it they stay 0 there
-    enqueue_coarse(c, od, k, max_ex, c->ctl_on.p);
-    enqueue_item_spectra(c, od, k, max_ex, c->co.ready ? c->ctl_on.p : nullptr);
-    enqueue_feas(c, od, k, max_ex, c->ctl_on.p);
-    const Mailbox& mb = post_and_sync(c, nullptr, nullptr, nullptr, nullptr, c->fstate.p,
-                                      kFeasHeader + k);
-    std::memcpy(fsv, mb.fs, (kFeasHeader + k) * sizeof(int));
+    if (c->co.ready) {
+      // the column test alone: its flags are exact (residual slot 0)
+      enqueue_coarse(c, od, k, max_ex, c->ctl_on.p);
+      const Mailbox& mb = post_and_sync(c, nullptr, nullptr, nullptr, nullptr, c->co.fs.p,
+                                        kFeasHeader + k);
+      std::memcpy(fsv, mb.fs, (kFeasHeader + k) * sizeof(int));
+      fsv[0] = 0;
+    } else {
+      enqueue_item_spectra(c, od, k, max_ex, nullptr);
+      enqueue_feas(c, od, k, max_ex, c->ctl_on.p);
+      const Mailbox& mb = post_and_sync(c, nullptr, nullptr, nullptr, nullptr, c->fstate.p,
+                                        kFeasHeader + k);
+      std::memcpy(fsv, mb.fs, (kFeasHeader + k) * sizeof(int));
+    }
     float res;
     std::memcpy(&res, &fsv[0], sizeof(res));
     if (!(res <= c->resid_limit)) return false;
@@ -2361,6 +2387,13 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->co.on = value != 0.0;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "fine_group") {
+    const int v = (int)value;
+    if (v != 4 && v != 8 && v != 16 && v != 32 && v != 64)
+      return fail(SPK_EINVAL, "fine_group must be 4, 8, 16, 32 or 64");
+    c->co.fine_group = v;
     return SPK_OK;
   }
   if (std::string(name) == "dist_x") {

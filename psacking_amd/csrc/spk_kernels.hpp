@@ -1254,6 +1254,54 @@ k_tray_columns(const int* __restrict__ tray, int nx, int ny, int nz, M128* __res
   }
 }
 
+// full-resolution column masks of the tray (bit z of column (x, y) set when
+// voxel (x, y, z) is occupied) with their window-OR levels, tf[level][x*ny+y];
+// one wavefront per column
+__global__ void __launch_bounds__(256)
+k_tray_fine(const int* __restrict__ tray, int nx, int ny, int nz, M128* __restrict__ tf,
+            const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
+  const int lane = threadIdx.x & 63;
+  const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (col >= nx * ny) return;
+  M128 m{0ULL, 0ULL};
+  for (int w = 0; w < 2; ++w) {
+    const int z = w * 64 + lane;
+    const bool occ = z < nz && tray[(size_t)col * nz + z] != 0;
+    const unsigned long long b = __ballot(occ);
+    if (w == 0) m.lo = b; else m.hi = b;
+  }
+  if (lane < kColLevels) {
+    M128 w = m;
+    for (int k = 0; k < lane; ++k) w = m_or(w, m_shr(w, 1 << k));
+    tf[(size_t)lane * nx * ny + col] = w;
+  }
+}
+
+// full-resolution column masks of m oriented items (bit z set when the item
+// has a voxel > 0 at (x, y, z)), im[(o * mx + x) * my + y]; one wavefront per
+// column, lanes over z
+__global__ void __launch_bounds__(256)
+k_item_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od, int m,
+               M128* __restrict__ im, int mx, int my, const int* __restrict__ ctl) {
+  if (search_off(ctl)) return;
+  const int lane = threadIdx.x & 63;
+  const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int o = blockIdx.y;
+  if (o >= m || col >= mx * my) return;
+  const OrientDesc d = od[o];
+  const int x = col / my, y = col - x * my;
+  const int* base = items + d.item_off;
+  M128 r{0ULL, 0ULL};
+  for (int w = 0; w < 2; ++w) {
+    const int z = w * 64 + lane;
+    const bool v = x < d.ex && y < d.ey && z < d.ez && base[oriented_index(d, x, y, z)] > 0;
+    const unsigned long long b = __ballot(v);
+    if (w == 0) r.lo = b; else r.hi = b;
+  }
+  if (lane == 0) im[((size_t)o * mx + x) * my + y] = r;
+}
+
 // core columns of m oriented items, cm[(o * ccx + X) * ccy + Y]; one
 // wavefront per core column, lanes over z
 __global__ void __launch_bounds__(256)
@@ -1283,43 +1331,70 @@ k_core_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od,
 }
 
 // grid (ceil(4 * cx * cy / 256), m).  Block (b, o): orientation o, 64
-// offsets q = 64 b + thread / 4 over the valid (Qx, Qy) rectangle, four lanes
-// per offset.  The block first lists the core's runs in LDS (order is
-// irrelevant: they are OR-ed); lane j of an offset ORs the heights that runs
-// j, j+4, ... block (window of length L = level 2^k, plus a shifted copy when
-// L is not a power of two), the four lanes combine every 16 runs and stop once
-// every valid height is blocked, and the orientation is flagged when one is
-// left.  A block starts only while the orientation is still unflagged.
+// offsets q = 64 b + thread / 4 over the valid (Qx, Qy) rectangle of the
+// column blocks, four lanes per offset.  The block first lists the core's runs
+// in LDS (order is irrelevant: they are OR-ed); lane j of an offset ORs the
+// heights that runs j, j+4, ... block (window of length L = level 2^k, plus a
+// shifted copy when L is not a power of two), the four lanes combine every 16
+// runs and stop once every valid height is blocked.
+// Without the full-resolution stage (tf == nullptr) an offset with a height
+// left flags the orientation as possibly feasible.  With it, such an offset's
+// four full-resolution offsets (2 Qx + rx, 2 Qy + ry) become candidates that
+// start from the heights the blocks already exclude and OR in every run of
+// the item's own columns (masks im); one with a height left proves the
+// orientation feasible, and the flag then means exactly that: an orientation
+// left unflagged has no collision-free offset.  A block starts only while the
+// orientation is unflagged.
 constexpr int kColRuns = 2048;
+constexpr int kFineRuns = 4096;
 
-__device__ __forceinline__ void col_runs_of(M128 k, int X, int Y, unsigned* runs, int* nruns) {
+// runs of one column mask appended to an LDS list: x (7 bits), y (8 bits),
+// start a (7 bits), length - 1 (7 bits)
+__device__ __forceinline__ void col_runs_of(M128 k, int X, int Y, unsigned* runs, int* nruns,
+                                            int cap) {
   while (k.lo | k.hi) {
     const int a = m_ctz(k);
     const M128 rest = m_shr(k, a);
     const int L = m_ctz(M128{~rest.lo, ~rest.hi});
     const int slot = atomicAdd(nruns, 1);
-    if (slot < kColRuns)
-      runs[slot] = (unsigned)X | ((unsigned)Y << 7) | ((unsigned)a << 14) | ((unsigned)(L - 1) << 21);
+    if (slot < cap)
+      runs[slot] = (unsigned)X | ((unsigned)Y << 7) | ((unsigned)a << 15) | ((unsigned)(L - 1) << 22);
     k.lo &= ~bits64(min(a, 64), min(a + L, 64));
     k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
   }
 }
 
-// heights blocked by a run [a, a+L) of core column (X, Y) at offset (Qx, Qy)
-__device__ __forceinline__ M128 run_blocks(const M128* __restrict__ cols, size_t plane, int cy,
-                                           int Qx, int Qy, int X, int Y, int a, int L) {
+// heights blocked by a run [a, a+L) at column c of the plane-strided levels lv
+__device__ __forceinline__ M128 run_blocks_at(const M128* __restrict__ lv, size_t plane, size_t c,
+                                              int a, int L) {
   const int k = 31 - __clz(L);
-  const size_t col = (size_t)(Qx + X) * cy + (Qy + Y);
-  M128 w = cols[(size_t)k * plane + col];
+  M128 w = lv[(size_t)k * plane + c];
   const int p = 1 << k;
   if (L > p) w = m_or(w, m_shr(w, L - p));
   return m_shr(w, a);
 }
 
+__device__ __forceinline__ bool m_any_free(M128 bad, unsigned long long vlo,
+                                           unsigned long long vhi) {
+  return ((~bad.lo & vlo) | (~bad.hi & vhi)) != 0ULL;
+}
+
+// OR over the G lanes of an aligned lane group (every lane of the group active)
+template <int G>
+__device__ __forceinline__ M128 group_or(M128 v) {
+  for (int s = 1; s < G; s <<= 1) {
+    v.lo |= __shfl_xor(v.lo, s, G);
+    v.hi |= __shfl_xor(v.hi, s, G);
+  }
+  return v;
+}
+
+template <int G>
 __global__ void __launch_bounds__(256)
 k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restrict__ cm,
-              int ccx, int ccy, const OrientDesc* __restrict__ od, int m, int* __restrict__ cfs,
-              const int* __restrict__ ctl) {
+              int ccx, int ccy, const M128* __restrict__ tf, int nx, int ny,
+              const M128* __restrict__ im, int mx, int my, const OrientDesc* __restrict__ od,
+              int m, int* __restrict__ cfs, const int* __restrict__ ctl) {
   if (search_off(ctl)) return;
   const int o = blockIdx.y, tid = threadIdx.x;
   if (o >= m) return;
@@ -1330,9 +1405,14 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
   if (q0 >= nq || d.nvz <= 0) return;
   volatile int* flag = cfs + kFeasHeader + o;
   __shared__ unsigned runs[kColRuns];
-  __shared__ int nruns, skip;
+  __shared__ unsigned fruns[kFineRuns];
+  __shared__ unsigned cand_list[256];
+  __shared__ M128 cbad[64];
+  __shared__ int nruns, nfine, ncand, skip;
   if (tid == 0) {
     nruns = 0;
+    nfine = 0;
+    ncand = 0;
     skip = *flag;   // one read, so that the whole block takes the same branch
   }
   __syncthreads();
@@ -1341,10 +1421,17 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
   const M128* core = cm + (size_t)o * ccx * ccy;
   for (int c = tid; c < kx * ky; c += 256) {
     const int X = c / ky, Y = c - X * ky;
-    col_runs_of(core[X * ccy + Y], X, Y, runs, &nruns);
+    col_runs_of(core[X * ccy + Y], X, Y, runs, &nruns, kColRuns);
   }
+  const bool fine = tf != nullptr && d.ex <= 128 && d.ey <= 256;
+  const M128* own = im + (size_t)o * mx * my;
+  if (fine)
+    for (int c = tid; c < d.ex * d.ey; c += 256) {
+      const int x = c / d.ey, y = c - x * d.ey;
+      col_runs_of(own[x * my + y], x, y, fruns, &nfine, kFineRuns);
+    }
   __syncthreads();
-  const int nr = nruns;
+  const int nr = nruns, nf = nfine;
   const int q = q0 + (tid >> 2), j = tid & 3;
   const bool active = q < nq;
   const int Qx = active ? q / nqy : 0, Qy = active ? q - Qx * nqy : 0;
@@ -1361,37 +1448,97 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
       if (!done) {
         for (int r = r0 + j; r < min(nr, r0 + 64); r += 4) {
           const unsigned e = runs[r];
-          bad = m_or(bad, run_blocks(cols, plane, cy, Qx, Qy, e & 127, (e >> 7) & 127,
-                                     (e >> 14) & 127, ((e >> 21) & 127) + 1));
+          const int X = e & 127, Y = (e >> 7) & 255, ra = (e >> 15) & 127;
+          const int L = ((e >> 22) & 127) + 1;
+          bad = m_or(bad, run_blocks_at(cols, plane, (size_t)(Qx + X) * cy + (Qy + Y), ra, L));
         }
       }
       M128 u = bad;
-      for (int s = 1; s < 4; s <<= 1) {
-        u.lo |= __shfl_xor(u.lo, s, 64);
-        u.hi |= __shfl_xor(u.hi, s, 64);
+      for (int s2 = 1; s2 < 4; s2 <<= 1) {
+        u.lo |= __shfl_xor(u.lo, s2, 64);
+        u.hi |= __shfl_xor(u.hi, s2, 64);
       }
       bad = u;
-      if (!((~bad.lo & vlo) | (~bad.hi & vhi))) done = true;
+      if (!m_any_free(bad, vlo, vhi)) done = true;
       if (__all(done)) break;
     }
-    if (active && j == 0 && ((~bad.lo & vlo) | (~bad.hi & vhi))) *flag = 1;
+  } else if (active) {
+    // more runs than the LDS list holds: every lane straight from the core masks
+    for (int X = 0; X < kx; ++X)
+      for (int Y = 0; Y < ky; ++Y) {
+        M128 k = core[X * ccy + Y];
+        while (k.lo | k.hi) {
+          const int a = m_ctz(k);
+          const M128 rest = m_shr(k, a);
+          const int L = m_ctz(M128{~rest.lo, ~rest.hi});
+          bad = m_or(bad, run_blocks_at(cols, plane, (size_t)(Qx + X) * cy + (Qy + Y), a, L));
+          k.lo &= ~bits64(min(a, 64), min(a + L, 64));
+          k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
+        }
+      }
+  }
+  const bool cand = active && m_any_free(bad, vlo, vhi);
+  if (!fine) {
+    if (cand && j == 0) *flag = 1;
     return;
   }
-  // more runs than the LDS list holds: straight from the core masks
-  if (!active || j != 0) return;
-  for (int X = 0; X < kx; ++X)
-    for (int Y = 0; Y < ky; ++Y) {
-      M128 k = core[X * ccy + Y];
-      while (k.lo | k.hi) {
-        const int a = m_ctz(k);
-        const M128 rest = m_shr(k, a);
-        const int L = m_ctz(M128{~rest.lo, ~rest.hi});
-        bad = m_or(bad, run_blocks(cols, plane, cy, Qx, Qy, X, Y, a, L));
-        k.lo &= ~bits64(min(a, 64), min(a + L, 64));
-        k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
+  // full-resolution stage: the block lists the full-resolution offsets
+  // (2 Qx + rx, 2 Qy + ry) of every block offset with a height left; groups of
+  // G lanes then take one candidate at a time, split the item's runs between
+  // their lanes and combine in chunks of growing size, so that a refuted
+  // candidate stops after its first blocked chunk and a proof costs nf / G
+  // loads per lane
+  if (cand) {
+    if (j == 0) cbad[tid >> 2] = bad;
+    const int qx = 2 * Qx + (j & 1), qy = 2 * Qy + (j >> 1);
+    if (qx < d.nvx && qy < d.nvy) cand_list[atomicAdd(&ncand, 1)] = (unsigned)tid;
+  }
+  __syncthreads();
+  const int nc = ncand;
+  const int gl = tid & (G - 1);
+  const size_t fplane = (size_t)nx * ny;
+  for (int c = tid / G; c < nc; c += 256 / G) {
+    if (__shfl((int)*flag, 0, G)) break;   // the group leader's read, so the group stays together
+    const int e = (int)cand_list[c];
+    const int qq = q0 + (e >> 2), je = e & 3;
+    const int QX = qq / nqy, QY = qq - QX * nqy;
+    const int qx = 2 * QX + (je & 1), qy = 2 * QY + (je >> 1);
+    M128 fb = cbad[e >> 2];
+    bool blocked = false;
+    if (nf <= kFineRuns) {
+      int step = 4 * G;
+      for (int r0 = 0; r0 < nf && !blocked; r0 += step, step = min(step * 2, 64 * G)) {
+        const int r1 = min(nf, r0 + step);
+        for (int r = r0 + gl; r < r1; r += G) {
+          const unsigned u = fruns[r];
+          const int x = u & 127, y = (u >> 7) & 255, ra = (u >> 15) & 127, L = ((u >> 22) & 127) + 1;
+          fb = m_or(fb, run_blocks_at(tf, fplane, (size_t)(qx + x) * ny + (qy + y), ra, L));
+        }
+        fb = group_or<G>(fb);
+        blocked = !m_any_free(fb, vlo, vhi);
       }
+    } else {
+      // more runs than the LDS list holds: lanes over the item's columns
+      for (int cc = gl; cc < d.ex * d.ey; cc += G) {
+        const int x = cc / d.ey, y = cc - x * d.ey;
+        M128 k = own[x * my + y];
+        while (k.lo | k.hi) {
+          const int a = m_ctz(k);
+          const M128 rest = m_shr(k, a);
+          const int L = m_ctz(M128{~rest.lo, ~rest.hi});
+          fb = m_or(fb, run_blocks_at(tf, fplane, (size_t)(qx + x) * ny + (qy + y), a, L));
+          k.lo &= ~bits64(min(a, 64), min(a + L, 64));
+          k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
+        }
+      }
+      fb = group_or<G>(fb);
+      blocked = !m_any_free(fb, vlo, vhi);
     }
-  if ((~bad.lo & vlo) | (~bad.hi & vhi)) *flag = 1;
+    if (!blocked) {
+      if (gl == 0) *flag = 1;
+      break;
+    }
+  }
 }
 
 // Column test decision, after k_column_test (cfs: its flags; its residual

@@ -508,11 +508,14 @@ def _full_feasible(g, tray):
     return bool((c[:n[0] - h[0], :n[1] - h[1], :n[2] - h[2]] == 0).any())
 
 
-def test_feasibility_flags_match_oracle(P):
-    """spk_feasible against the oracle on trays filled to several levels:
-    full flags = exact feasibility of every orientation, column flags = the
-    oracle's column test, and the column test never excludes a feasible
-    orientation."""
+@pytest.mark.parametrize("fine_group", [64, 4])
+def test_feasibility_flags_match_oracle(P, fine_group):
+    """spk_feasible against the oracle on trays filled to several levels: the
+    full-pass flags and the column test's flags (its block stage, then its
+    full-resolution stage on the offsets the blocks leave open) both equal the
+    exact feasibility of every orientation; the orientations the oracle's block
+    stage alone excludes are among the infeasible ones.  Both lane-group sizes
+    of the full-resolution stage (the default 64 and 4)."""
     from psacking_amd import proposer as PR
     from psacking_amd.rotations import orientation_maps
     perms, flips = orientation_maps(24)
@@ -523,6 +526,7 @@ def test_feasibility_flags_match_oracle(P):
         tray = (rng.random(shape) < 0.04).astype(np.int32)
         tray[:, :, :level] = 3
         s = P.Session(0)
+        s.set_option("fine_group", fine_group)
         s.set_tray(tray, "edt", "nonzero", 0)
         for case in range(6):
             dims = tuple(int(v) for v in rng.integers(4, 12, 3))
@@ -537,12 +541,12 @@ def test_feasibility_flags_match_oracle(P):
                     assert column[o] == 0 and full[o] == 0
                     continue
                 exp_full = _full_feasible(g, tray)
-                exp_col = not O.column_infeasible(g, tray)
                 assert full[o] == exp_full, (level, case, o)
-                assert column[o] == exp_col, (level, case, o)
-                assert column[o] or not full[o]
+                assert column[o] == exp_full, (level, case, o)
+                blocks_exclude = O.column_infeasible(g, tray)
+                assert not (blocks_exclude and exp_full), (level, case, o)
                 checked += 1
-                excluded += not column[o]
+                excluded += blocks_exclude
                 feasible += bool(full[o])
     assert checked > 500 and excluded > 50 and feasible > 50
 
