@@ -181,6 +181,8 @@ struct spk_ctx {
   bool autotune = true;                  // time x-pencil tiles per tray shape (option "autotune")
   int tile_force = 0;                    // option "tile": 0 tuned/default, else 4, 8 or 16
   int yz_rmax = kMaxRadix;               // option "yz_radix": largest y/z butterfly
+  int score_rmax = 10;                   // option "score_radix": largest butterfly of the score pass
+  AxisPlan sy{}, sz{};                   // y and z plans of the score pass
   int pad_y = 0, pad_z = 0;              // options "pad_y", "pad_z": padded sizes (0 = smallest smooth)
   int feas_group = 0;                    // option "feas_group": pairs per feasibility launch pair
   int dist_y = 2;                        // option "dist_y": 1 envelope, 2 full scan
@@ -379,6 +381,8 @@ void plan_tray(spk_ctx* c) {
   c->px = make_plan(px, kPencilRadix);
   c->py = make_plan(py, c->yz_rmax);
   c->pz = make_plan(pz, c->yz_rmax);
+  c->sy = make_plan(py, std::min(c->yz_rmax, c->score_rmax));
+  c->sz = make_plan(pz, std::min(c->yz_rmax, c->score_rmax));
   auto tile = [](int n) {
     int T = 16;
     while (T > 1 && n * T > kPencilMax) T >>= 1;
@@ -425,7 +429,8 @@ void plan_tray(spk_ctx* c) {
     const size_t dyn = (size_t)lds_max - 1024;   // leave room for static shared arrays
     allow_shmem(k_tray_slab_fwd, dyn);
     allow_shmem(k_item_slab_fwd, dyn);
-    allow_shmem(k_score_yz, dyn);
+    allow_shmem(k_score_yz<10>, dyn);
+    allow_shmem(k_score_yz<25>, dyn);
     allow_shmem(k_corr_x, dyn);
     allow_shmem(k_tray_x_fwd, dyn);
     allow_shmem(k_feas_x, dyn);
@@ -852,10 +857,12 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
            (const float2*)c->y2.p, od, (const FieldStats*)c->stats.p, pen, c->rec.p, rs, c->nx,
            c->ny, c->nz, py, c->pz, c->Rs, tw + c->px.n + py, fgrid, pgrid, ctl);
   } else {
-    launch(c, "score_yz", k_score_yz, dim3(c->nx, m), dim3(kSlabThreads), score_shmem(c),
-           (const float2*)c->y1.p, (const float2*)c->y2.p, od, (const FieldStats*)c->stats.p,
-           pen, c->rec.p, c->nx, c->ny, c->nz, c->py, c->pz, tw + c->px.n, tw + c->px.n + py,
-           fgrid, pgrid, cf, ctl);
+    // the score pass has its own factorisations of py and pz (radices up to
+    // c->score_rmax, compiled as the kernel's largest butterfly)
+    launch(c, "score_yz", c->score_rmax == 25 ? k_score_yz<25> : k_score_yz<10>, dim3(c->nx, m),
+           dim3(kSlabThreads), score_shmem(c), (const float2*)c->y1.p, (const float2*)c->y2.p, od,
+           (const FieldStats*)c->stats.p, pen, c->rec.p, c->nx, c->ny, c->nz, c->sy, c->sz,
+           tw + c->px.n, tw + c->px.n + py, fgrid, pgrid, cf, ctl);
   }
   launch(c, "select", k_select, dim3(1), dim3(kSelectThreads), 0, (const WgBest*)c->rec.p, od, m,
          rs, c->split ? c->Rs : 0, c->per.p, c->sout.p, c->ver.p, ctl);
@@ -2364,6 +2371,14 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     if (v != 0 && v != 4 && v != 8 && v != 16)
       return fail(SPK_EINVAL, "tile must be 0 (tuned), 4, 8 or 16");
     c->tile_force = v;
+    c->tw.release();   // re-plan at the next spk_tray_set
+    c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "score_radix") {
+    const int v = (int)value;
+    if (v != 10 && v != 25) return fail(SPK_EINVAL, "score_radix must be 10 or 25");
+    c->score_rmax = v;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
     return SPK_OK;

@@ -393,29 +393,21 @@ k_dist_zy(const int* __restrict__ tray, int* __restrict__ out, int nx, int ny, i
       for (int e = tid; e < S; e += kDistThreads) atomicMin(&cmin[e % nz], sh[e]);
       __syncthreads();
     }
-    constexpr int MAXR = 32;   // elements per thread: ny*nz <= 32768
-    int res[MAXR];
-#pragma unroll
-    for (int t = 0; t < MAXR; ++t) {
-      const int e = tid + t * kDistThreads;
-      if (e < S) {
-        const int j = e / nz, k = e - j * nz;
-        if constexpr (YMODE == 2) {
-          int best = sh[e];
-          for (int jj = 0; jj < ny; ++jj) {
-            const int c = sh[jj * nz + k] + dcost(j - jj, EDT);
-            best = c < best ? c : best;
-          }
-          res[t] = best;
-        } else {
-          res[t] = outward_min(sh + k, nz, ny, j, EDT, cmin[k]);
+    // one element per thread and step, stored at once (keeping the results
+    // in registers until the end spills)
+    for (int e = tid; e < S; e += kDistThreads) {
+      const int j = e / nz, k = e - j * nz;
+      int best;
+      if constexpr (YMODE == 2) {
+        best = sh[e];
+        for (int jj = 0; jj < ny; ++jj) {
+          const int c = sh[jj * nz + k] + dcost(j - jj, EDT);
+          best = c < best ? c : best;
         }
+      } else {
+        best = outward_min(sh + k, nz, ny, j, EDT, cmin[k]);
       }
-    }
-#pragma unroll
-    for (int t = 0; t < MAXR; ++t) {
-      const int e = tid + t * kDistThreads;
-      if (e < S) dst[e] = res[t];
+      dst[e] = best;
     }
   }
 }
@@ -838,6 +830,7 @@ __device__ __forceinline__ void wg_argmin_write(double sc, int idx, long long p,
   }
 }
 
+template <int RMAX>
 __global__ void __launch_bounds__(kSlabThreads)
 k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
            const OrientDesc* __restrict__ od, const FieldStats* __restrict__ st,
@@ -873,7 +866,7 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
   const float2* s2 = y2 + ((size_t)o * nx + x) * py * hz;
   stage_rows<kSlabThreads, kSlabMaxEl>(buf, s2, py * hz, hz, pz, tid);
   __syncthreads();
-  lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, py_, pz, hz, 1, false, twy, tid);
+  lds_fft<false, kSlabThreads, kSlabMaxEl, RMAX>(buf, py_, pz, hz, 1, false, twy, tid);
   const int nvy = d.nvy, nvz = d.nvz;
   for (int e = tid; e < nvy * (pz - hz); e += kSlabThreads) {
     const int j = e / (pz - hz), k = hz + (e - j * (pz - hz));
@@ -881,7 +874,7 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
     buf[j * pz + k] = make_float2(v.x, -v.y);
   }
   __syncthreads();
-  lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, nvy, pz, true, twz, tid);
+  lds_fft<false, kSlabThreads, kSlabMaxEl, RMAX>(buf, pz_, 1, nvy, pz, true, twz, tid);
   // the rounded low-digit sums wait in LDS behind the slab ([ny][nz] ints),
   // not in registers: the second slab FFT needs the whole register budget
   int* lo = reinterpret_cast<int*>(buf + py * pz);
@@ -903,8 +896,8 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
   const float2* s1 = y1 + ((size_t)o * nx + x) * py * pz;
   stage_rows<kSlabThreads, kSlabMaxEl>(buf, s1, py * pz, pz, pz, tid);
   __syncthreads();
-  lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, py_, pz, pz, 1, false, twy, tid);
-  lds_fft<false, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, nvy, pz, true, twz, tid);
+  lds_fft<false, kSlabThreads, kSlabMaxEl, RMAX>(buf, py_, pz, pz, 1, false, twy, tid);
+  lds_fft<false, kSlabThreads, kSlabMaxEl, RMAX>(buf, pz_, 1, nvy, pz, true, twz, tid);
   double best = 1e300;
   int bidx = INT_MAX;
   long long bp = 0;
@@ -1279,8 +1272,9 @@ k_tray_fine(const int* __restrict__ tray, int nx, int ny, int nz, M128* __restri
 }
 
 // full-resolution column masks of m oriented items (bit z set when the item
-// has a voxel > 0 at (x, y, z)), im[(o * mx + x) * my + y]; one wavefront per
-// column, lanes over z
+// has a voxel > 0 at (x, y, z)), im[(o * mx + x) * my + y] for x < ex, y < ey
+// (the only entries the column test reads); one wavefront per column, lanes
+// over z
 __global__ void __launch_bounds__(256)
 k_item_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od, int m,
                M128* __restrict__ im, int mx, int my, const int* __restrict__ ctl) {
@@ -1291,6 +1285,7 @@ k_item_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od,
   if (o >= m || col >= mx * my) return;
   const OrientDesc d = od[o];
   const int x = col / my, y = col - x * my;
+  if (x >= d.ex || y >= d.ey) return;
   const int* base = items + d.item_off;
   M128 r{0ULL, 0ULL};
   for (int w = 0; w < 2; ++w) {
@@ -1302,7 +1297,8 @@ k_item_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od,
   if (lane == 0) im[((size_t)o * mx + x) * my + y] = r;
 }
 
-// core columns of m oriented items, cm[(o * ccx + X) * ccy + Y]; one
+// core columns of m oriented items, cm[(o * ccx + X) * ccy + Y] for
+// X < (ex + 1) / 2, Y < (ey + 1) / 2 (the entries the column test reads); one
 // wavefront per core column, lanes over z
 __global__ void __launch_bounds__(256)
 k_core_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od, int m,
@@ -1314,6 +1310,7 @@ k_core_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od,
   if (o >= m || col >= ccx * ccy) return;
   const OrientDesc d = od[o];
   const int X = col / ccy, Y = col - X * ccy;
+  if (X >= (d.ex + 1) / 2 || Y >= (d.ey + 1) / 2) return;
   const int* base = items + d.item_off;
   M128 r{0ULL, 0ULL};
   for (int w = 0; w < 2; ++w) {
