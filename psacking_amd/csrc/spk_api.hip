@@ -194,8 +194,13 @@ struct spk_ctx {
     int cx = 0, cy = 0;                  // 2x2 column blocks of the tray
     DevBuf<M128> cols, cm;               // block masks, core masks of a search
     DevBuf<M128> tf, im;                 // full-resolution tray columns, item columns
+    DevBuf<unsigned> rl, frl;            // sorted run lists of the core and item columns
+    DevBuf<int> nrun;                    // their lengths
+    DevBuf<int> ht;                      // tile heights of the tray (kSkyTile columns square)
+    int tx = 0, ty = 0;
     DevBuf<int> fs;                      // residual slot (0), per-orientation flags
     int fine_group = 64;                 // option "fine_group": lanes per full-resolution candidate
+    int col_chunks = 1;                  // option "col_chunks": 64-offset chunks per column-test block
   } co;
   // free-space certificate
   bool phi_internal = false;             // phi is this library's distance field of the tray
@@ -352,6 +357,9 @@ void plan_coarse(spk_ctx* c) {
   q.cy = (c->ny + 1) / 2;
   q.cols.ensure((size_t)kColLevels * q.cx * q.cy);
   q.tf.ensure((size_t)kColLevels * c->nx * c->ny);
+  q.tx = (c->nx + kSkyTile - 1) / kSkyTile;
+  q.ty = (c->ny + kSkyTile - 1) / kSkyTile;
+  q.ht.ensure((size_t)q.tx * q.ty);
   q.fs.ensure(kFeasHeader + 24);
   q.ready = true;
 }
@@ -365,6 +373,8 @@ void build_coarse_spectra(spk_ctx* c, const int* ctl) {
          c->nx, c->ny, c->nz, q.cols.p, q.cx, q.cy, ctl);
   launch(c, "tray_fine", k_tray_fine, dim3((c->nx * c->ny + 3) / 4), dim3(256), 0,
          (const int*)c->tray.p, c->nx, c->ny, c->nz, q.tf.p, ctl);
+  launch(c, "tile_heights", k_tile_heights, dim3((q.tx * q.ty + 3) / 4), dim3(256), 0,
+         (const M128*)q.tf.p, c->nx, c->ny, q.ht.p, q.tx, q.ty, ctl);
 }
 
 // ---- FFT planning for a tray --------------------------------------------
@@ -792,22 +802,35 @@ const int* ctl_feas_on(spk_ctx* c) {
 // pass and the scored search: core masks of the oriented items, the test,
 // then k_coarse_decide (prune orientations, or decide the item: see there).
 // ctl must not be null.
-void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const int* ctl,
-                    PackOut* out = nullptr, int t = 0, int halt_not_found = 0) {
+void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_exy,
+                    const int* ctl, PackOut* out = nullptr, int t = 0, int halt_not_found = 0) {
   spk_ctx::Coarse& q = c->co;
   if (!q.ready) return;
   const int ccx = (max_ex + 1) / 2, ccy = q.cy;
   const int mx = max_ex, my = c->ny;
   q.cm.ensure((size_t)m * ccx * ccy);
   q.im.ensure((size_t)m * mx * my);
-  launch(c, "core_columns", k_core_columns, dim3((ccx * ccy + 3) / 4, m), dim3(256), 0,
-         (const int*)c->items.p, od, m, q.cm.p, ccx, ccy, ctl);
-  launch(c, "item_columns", k_item_columns, dim3((mx * my + 3) / 4, m), dim3(256), 0,
-         (const int*)c->items.p, od, m, q.im.p, mx, my, ctl);
+  q.rl.ensure((size_t)m * kColRuns);
+  q.frl.ensure((size_t)m * kFineRuns);
+  q.nrun.ensure(2 * (size_t)m);
+  if (max_exy <= kMaskCols) {
+    launch(c, "item_masks", k_item_masks, dim3(m), dim3(kMaskThreads), 0, (const int*)c->items.p,
+           od, m, q.cm.p, ccx, ccy, q.im.p, mx, my, q.rl.p, q.frl.p, q.nrun.p, ctl);
+  } else {
+    launch(c, "core_columns", k_core_columns, dim3((ccx * ccy + 3) / 4, m), dim3(256), 0,
+           (const int*)c->items.p, od, m, q.cm.p, ccx, ccy, ctl);
+    launch(c, "item_columns", k_item_columns, dim3((mx * my + 3) / 4, m), dim3(256), 0,
+           (const int*)c->items.p, od, m, q.im.p, mx, my, ctl);
+    launch(c, "run_lists", k_run_lists, dim3(m), dim3(kListThreads), 0, (const M128*)q.cm.p, ccx,
+           ccy, (const M128*)q.im.p, mx, my, od, m, 1, q.rl.p, q.frl.p, q.nrun.p, ctl);
+  }
   auto test = [&](auto kernel) {
-    launch(c, "column_test", kernel, dim3((q.cx * q.cy + 63) / 64, m), dim3(256), 0,
+    const int chunks = (q.cx * q.cy + 63) / 64;
+    launch(c, "column_test", kernel, dim3((chunks + q.col_chunks - 1) / q.col_chunks, m),
+           dim3(256), 0,
            (const M128*)q.cols.p, q.cx, q.cy, (const M128*)q.cm.p, ccx, ccy, (const M128*)q.tf.p,
-           c->nx, c->ny, (const M128*)q.im.p, mx, my, od, m, q.fs.p, ctl);
+           c->nx, c->ny, (const M128*)q.im.p, mx, my, od, m, (const unsigned*)q.rl.p,
+           (const unsigned*)q.frl.p, (const int*)q.nrun.p, (const int*)q.ht.p, q.ty, q.fs.p, ctl);
   };
   switch (q.fine_group) {
     case 4: test(k_column_test<4>); break;
@@ -1036,7 +1059,7 @@ void do_place(spk_ctx* c, const OrientDesc& d, int x, int y, int z, int id, bool
 
 // Orientations of every item that fit the tray, in processing order.
 struct ItemPlan {
-  int od_off = 0, m = 0, max_ex = 1, max_e = 1;
+  int od_off = 0, m = 0, max_ex = 1, max_exy = 1, max_e = 1;
   std::vector<int> map;   // fitting index -> orientation index
 };
 
@@ -1065,6 +1088,7 @@ PackPlan plan_pack(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
       map_all.push_back(o);
       pl.map.push_back(o);
       pl.max_ex = std::max(pl.max_ex, d.ex);
+      pl.max_exy = std::max(pl.max_exy, d.ex * d.ey);
       pl.max_e = std::max(pl.max_e, d.ex * d.ey * d.ez);
     }
     pl.m = (int)pl.map.size();
@@ -1130,8 +1154,8 @@ const FieldStats* item_bounds(spk_ctx* c, const std::vector<HostItem>& bases,
 // item placed nothing); when it runs and finds no feasible orientation,
 // k_feas_decide records the item as not found and the scored search,
 // placement and refresh are skipped.
-void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
-                  int t, int32_t id_offset, int halt_not_found, double* rec_row,
+void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_exy, int max_e,
+                  int penalty, int t, int32_t id_offset, int halt_not_found, double* rec_row,
                   const int* rec_map, int feas_mode = -1,
                   const FieldStats* item_bound = nullptr) {
   if (feas_mode < 0) feas_mode = c->feas_mode;
@@ -1140,7 +1164,7 @@ void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e
     launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p,
            c->co.ready ? c->co.fs.p : (int*)nullptr, m, feas ? feas_mode : 0, c->ctl.p,
            (const FieldStats*)c->stats.p, item_bound, c->pout.p, t);
-    enqueue_coarse(c, od, m, max_ex, c->ctl.p, c->pout.p, t, halt_not_found);
+    enqueue_coarse(c, od, m, max_ex, max_exy, c->ctl.p, c->pout.p, t, halt_not_found);
     enqueue_item_spectra(c, od, m, max_ex, c->ctl.p);
     // the column test's flags are exact: no FFT feasibility pass after it
     if (feas && !c->co.ready) {
@@ -1248,7 +1272,7 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
         nn_tray_features(c->stream, c->tray.p, c->nx, c->ny, c->nz, c->heights.p,
                          rec_feat + (size_t)t * kNnTrayFeat);
       const OrientDesc* od = c->od_all.p + pl.od_off;
-      enqueue_step(c, od, pl.m, pl.max_ex, pl.max_e, penalty, t, id_offset, 0,
+      enqueue_step(c, od, pl.m, pl.max_ex, pl.max_exy, pl.max_e, penalty, t, id_offset, 0,
                    rec_scores ? c->rscores.p + (size_t)t * n_orient : nullptr,
                    c->map_all.p + pl.od_off, -1, bounds ? bounds + t : nullptr);
     }
@@ -1352,10 +1376,12 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     nn_cast(nn, s);
     std::vector<spknn::ItemMap> maps((size_t)n_items * kNnLogits);
     for (int t = 0; t < n_items; ++t) {
-      const HostItem& h = bases[order ? order[t] : t];
+      const int it = order ? order[t] : t;
+      const HostItem& h = bases[it];
       for (int o = 0; o < kNnLogits; ++o) {
         spknn::ItemMap& mp = maps[(size_t)t * kNnLogits + o];
         mp.bx = h.dims[0]; mp.by = h.dims[1]; mp.bz = h.dims[2];
+        mp.off = off[it];
         const spk_orient* so = (orients && o < n_orient) ? &orients[o] : nullptr;
         for (int a = 0; a < 3; ++a) {
           mp.perm[a] = so ? so->perm[a] : a;
@@ -1370,11 +1396,8 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     c->ipool.ensure((size_t)n_items * kNnLogits * kNnItemPooled);
     for (int b0 = 0; b0 < n_items; b0 += bmax) {
       const int nb = std::min(bmax, n_items - b0);
-      for (int j = 0; j < nb; ++j) {
-        const int it = order ? order[b0 + j] : b0 + j;
-        nn_item_features(s, c->items.p + off[it], c->imaps.p + (size_t)(b0 + j) * kNnLogits,
-                         kNnLogits, c->ifeat.p + (size_t)j * kNnLogits * kNnItemFeat);
-      }
+      nn_item_features(s, c->items.p, c->imaps.p + (size_t)b0 * kNnLogits, nb * kNnLogits,
+                       c->ifeat.p);
       nn_item_branch(nn, s, c->ifeat.p, nb * kNnLogits,
                      c->ipool.p + (size_t)b0 * kNnLogits * kNnItemPooled);
     }
@@ -1422,7 +1445,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
            (const FieldStats*)nullptr, (const FieldStats*)nullptr, (PackOut*)nullptr, 0);
     if (c->co.ready) {
       // the column test alone: its flags are exact (residual slot 0)
-      enqueue_coarse(c, od, k, max_ex, c->ctl_on.p);
+      enqueue_coarse(c, od, k, max_ex, pl.max_exy, c->ctl_on.p);
       const Mailbox& mb = post_and_sync(c, nullptr, nullptr, nullptr, nullptr, c->co.fs.p,
                                         kFeasHeader + k);
       std::memcpy(fsv, mb.fs, (kFeasHeader + k) * sizeof(int));
@@ -1526,7 +1549,8 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     const OrientDesc* od = gather(pl, pick, c->od_prop.p, &max_ex, &max_e);
     if (use_prop && proposals)
       for (size_t j = 0; j < pick.size(); ++j) proposals[(size_t)t * topk + j] = pl.map[pick[j]];
-    enqueue_step(c, od, (int)pick.size(), max_ex, max_e, penalty, t, id_offset, use_prop ? 1 : 0,
+    enqueue_step(c, od, (int)pick.size(), max_ex, pl.max_exy, max_e, penalty, t, id_offset,
+                 use_prop ? 1 : 0,
                  nullptr, nullptr, 0);
     // the next searched item's logits, on the tray this step leaves
     int tn = t + 1;
@@ -1569,7 +1593,8 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       }
       if (!rest.empty()) {
         const OrientDesc* odr = gather(pl, rest, c->od_rest.p, &max_ex, &max_e);
-        enqueue_step(c, odr, (int)rest.size(), max_ex, max_e, penalty, t, id_offset, 0, nullptr,
+        enqueue_step(c, odr, (int)rest.size(), max_ex, pl.max_exy, max_e, penalty, t, id_offset, 0,
+                     nullptr,
                      nullptr, 0);
         {
           const Mailbox& mb = post_and_sync(c, c->ctl.p, c->pout.p + t, c->stats.p, nullptr,
@@ -1942,7 +1967,7 @@ int spk_feasible(spk_ctx* c, const int32_t* item, int ix, int iy, int iz,
     upload(c, c->items.p, item, sz * sizeof(int));
     std::vector<OrientDesc> ods;
     std::vector<int> fit;
-    int max_ex = 1;
+    int max_ex = 1, max_exy = 1;
     for (int o = 0; o < n; ++o) {
       const OrientDesc d = make_desc(c, it, 0, orients ? &orients[o] : nullptr);
       const bool fits = it.hi[0] >= 0 && d.ex <= c->nx && d.ey <= c->ny && d.ez <= c->nz &&
@@ -1953,6 +1978,7 @@ int spk_feasible(spk_ctx* c, const int32_t* item, int ix, int iy, int iz,
       ods.push_back(d);
       fit.push_back(o);
       max_ex = std::max(max_ex, d.ex);
+      max_exy = std::max(max_exy, d.ex * d.ey);
     }
     if (residual) *residual = 0.f;
     const int k = (int)ods.size();
@@ -1963,7 +1989,7 @@ int spk_feasible(spk_ctx* c, const int32_t* item, int ix, int iy, int iz,
     launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p,
            c->co.ready ? c->co.fs.p : (int*)nullptr, k, 2, (int*)nullptr,
            (const FieldStats*)nullptr, (const FieldStats*)nullptr, (PackOut*)nullptr, 0);
-    enqueue_coarse(c, c->od_prop.p, k, max_ex, ctl_feas_on(c));
+    enqueue_coarse(c, c->od_prop.p, k, max_ex, max_exy, ctl_feas_on(c));
     // the full-resolution pass over every fitting orientation (no pruning)
     enqueue_item_spectra(c, c->od_prop.p, k, max_ex, nullptr);
     enqueue_feas(c, c->od_prop.p, k, max_ex, nullptr);
@@ -2402,6 +2428,12 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->co.on = value != 0.0;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "col_chunks") {
+    const int v = (int)value;
+    if (v < 1 || v > 64) return fail(SPK_EINVAL, "col_chunks must lie in [1, 64]");
+    c->co.col_chunks = v;
     return SPK_OK;
   }
   if (std::string(name) == "fine_group") {

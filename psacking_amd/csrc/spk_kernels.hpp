@@ -1271,6 +1271,28 @@ k_tray_fine(const int* __restrict__ tray, int nx, int ny, int nz, M128* __restri
   }
 }
 
+// tile heights of the tray, after k_tray_fine: ht[X * ty + Y] = max over the
+// columns (x, y) of tile X = x / kSkyTile, Y = y / kSkyTile of 1 + the highest
+// occupied z (0 for empty columns); one wavefront per tile, a lane per column
+constexpr int kSkyTile = 8;
+__global__ void __launch_bounds__(256)
+k_tile_heights(const M128* __restrict__ tf, int nx, int ny, int* __restrict__ ht, int tx, int ty,
+               const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= tx * ty) return;
+  const int X = t / ty, Y = t - X * ty;
+  const int x = X * kSkyTile + (lane >> 3), y = Y * kSkyTile + (lane & 7);
+  int top = 0;
+  if (x < nx && y < ny) {
+    const M128 m = tf[(size_t)x * ny + y];   // level 0: the column's own mask
+    top = m.hi ? 128 - __clzll((long long)m.hi) : (m.lo ? 64 - __clzll((long long)m.lo) : 0);
+  }
+  for (int s = 1; s < 64; s <<= 1) top = max(top, __shfl_xor(top, s, 64));
+  if (lane == 0) ht[t] = top;
+}
+
 // full-resolution column masks of m oriented items (bit z set when the item
 // has a voxel > 0 at (x, y, z)), im[(o * mx + x) * my + y] for x < ex, y < ey
 // (the only entries the column test reads); one wavefront per column, lanes
@@ -1327,38 +1349,40 @@ k_core_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od,
   if (lane == 0) cm[((size_t)o * ccx + X) * ccy + Y] = r;
 }
 
-// grid (ceil(4 * cx * cy / 256), m).  Block (b, o): orientation o, 64
-// offsets q = 64 b + thread / 4 over the valid (Qx, Qy) rectangle of the
-// column blocks, four lanes per offset.  The block first lists the core's runs
-// in LDS (order is irrelevant: they are OR-ed); lane j of an offset ORs the
-// heights that runs j, j+4, ... block (window of length L = level 2^k, plus a
-// shifted copy when L is not a power of two), the four lanes combine every 16
-// runs and stop once every valid height is blocked.
+// grid (chunks, m).  Block (b, o): orientation o, chunks of 64 offsets
+// q = 64 c + thread / 4 over the valid (Qx, Qy) rectangle of the column
+// blocks, c = b, b + chunks, ..., four lanes per offset.  The core's runs come longest first
+// from k_run_lists; lane j of an offset ORs the heights that runs j,
+// j+4, ... block (window of length L = level 2^k, plus a shifted copy when L
+// is not a power of two), the four lanes combine after rounds of growing size
+// and stop once every valid height is blocked.
 // Without the full-resolution stage (tf == nullptr) an offset with a height
 // left flags the orientation as possibly feasible.  With it, such an offset's
 // four full-resolution offsets (2 Qx + rx, 2 Qy + ry) become candidates that
 // start from the heights the blocks already exclude and OR in every run of
 // the item's own columns (masks im); one with a height left proves the
 // orientation feasible, and the flag then means exactly that: an orientation
-// left unflagged has no collision-free offset.  A block starts only while the
-// orientation is unflagged.
+// left unflagged has no collision-free offset.  A block starts a chunk only
+// while the orientation is unflagged.
 constexpr int kColRuns = 2048;
 constexpr int kFineRuns = 4096;
 
-// runs of one column mask appended to an LDS list: x (7 bits), y (8 bits),
-// start a (7 bits), length - 1 (7 bits)
-__device__ __forceinline__ void col_runs_of(M128 k, int X, int Y, unsigned* runs, int* nruns,
-                                            int cap) {
+// every run [a, a+L) of set bits of a column mask, passed to f(a, L)
+template <typename F>
+__device__ __forceinline__ void for_runs(M128 k, F f) {
   while (k.lo | k.hi) {
     const int a = m_ctz(k);
     const M128 rest = m_shr(k, a);
     const int L = m_ctz(M128{~rest.lo, ~rest.hi});
-    const int slot = atomicAdd(nruns, 1);
-    if (slot < cap)
-      runs[slot] = (unsigned)X | ((unsigned)Y << 7) | ((unsigned)a << 15) | ((unsigned)(L - 1) << 22);
+    f(a, L);
     k.lo &= ~bits64(min(a, 64), min(a + L, 64));
     k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
   }
+}
+
+// a run in an LDS list: x (7 bits), y (8 bits), start a (7 bits), length - 1 (7 bits)
+__device__ __forceinline__ unsigned pack_run(int x, int y, int a, int L) {
+  return (unsigned)x | ((unsigned)y << 7) | ((unsigned)a << 15) | ((unsigned)(L - 1) << 22);
 }
 
 // heights blocked by a run [a, a+L) at column c of the plane-strided levels lv
@@ -1376,6 +1400,202 @@ __device__ __forceinline__ bool m_any_free(M128 bad, unsigned long long vlo,
   return ((~bad.lo & vlo) | (~bad.hi & vhi)) != 0ULL;
 }
 
+// Run lists of m oriented items for the column test, one block per
+// orientation: the runs of the core masks (cm) into runs[o * kColRuns ...] and
+// of the item's own column masks (im, when fine) into fruns[o * kFineRuns ...],
+// each sorted longest first by a counting sort over the 128 lengths (a long
+// run blocks the most heights, so most offsets are refuted by the first few).
+// nrun[2 o] and nrun[2 o + 1] receive the list lengths; a length above the
+// capacity (or kColRuns + 1 when the core's x or y extent exceeds the packed
+// fields) means the list was not written and the test reads the masks.
+constexpr int kListThreads = 1024;
+constexpr int kListHold = 4;   // masks per thread kept in registers between the passes
+__global__ void __launch_bounds__(kListThreads)
+k_run_lists(const M128* __restrict__ cm, int ccx, int ccy, const M128* __restrict__ im, int mx,
+            int my, const OrientDesc* __restrict__ od, int m, int fine_on,
+            unsigned* __restrict__ runs, unsigned* __restrict__ fruns, int* __restrict__ nrun,
+            const int* __restrict__ ctl) {
+  if (search_off(ctl)) return;
+  const int o = blockIdx.x, tid = threadIdx.x;
+  if (o >= m) return;
+  const OrientDesc d = od[o];
+  __shared__ int cnt[2][128];   // per run length: count, then next slot
+  __shared__ int ntot[2];
+  if (tid < 256) cnt[tid >> 7][tid & 127] = 0;
+  const int kx = (d.ex + 1) / 2, ky = (d.ey + 1) / 2;
+  const M128* core = cm + (size_t)o * ccx * ccy;
+  const M128* own = im + (size_t)o * mx * my;
+  const bool clist = kx <= 128 && ky <= 256;   // the packed run's x and y fields
+  const bool fine = fine_on && d.ex <= 128 && d.ey <= 256;
+  const int nc = clist ? kx * ky : 0, nf = fine ? d.ex * d.ey : 0;
+  const M128 zero{0ULL, 0ULL};
+  M128 hc[kListHold], hf[kListHold];
+#pragma unroll
+  for (int i = 0; i < kListHold; ++i) {
+    const int c = tid + i * kListThreads;
+    hc[i] = c < nc ? core[(c / ky) * ccy + (c % ky)] : zero;
+    hf[i] = c < nf ? own[(c / d.ey) * my + (c % d.ey)] : zero;
+  }
+  __syncthreads();
+  auto count0 = [&](int, int L) { atomicAdd(&cnt[0][L - 1], 1); };
+  auto count1 = [&](int, int L) { atomicAdd(&cnt[1][L - 1], 1); };
+#pragma unroll
+  for (int i = 0; i < kListHold; ++i) {
+    for_runs(hc[i], count0);
+    for_runs(hf[i], count1);
+  }
+  for (int c = tid + kListHold * kListThreads; c < nc; c += kListThreads)
+    for_runs(core[(c / ky) * ccy + (c % ky)], count0);
+  for (int c = tid + kListHold * kListThreads; c < nf; c += kListThreads)
+    for_runs(own[(c / d.ey) * my + (c % d.ey)], count1);
+  __syncthreads();
+  if (tid < 128) {
+    // wave w scans list w: lane l holds lengths 128 - 2l and 127 - 2l
+    const int w = tid >> 6, lane = tid & 63;
+    const int v0 = cnt[w][127 - 2 * lane], v1 = cnt[w][126 - 2 * lane];
+    int incl = v0 + v1;
+    for (int s2 = 1; s2 < 64; s2 <<= 1) {
+      const int u = __shfl_up(incl, s2, 64);
+      if (lane >= s2) incl += u;
+    }
+    const int excl = incl - v0 - v1;
+    cnt[w][127 - 2 * lane] = excl;
+    cnt[w][126 - 2 * lane] = excl + v0;
+    if (lane == 63) ntot[w] = incl;
+  }
+  __syncthreads();
+  unsigned* cr = runs + (size_t)o * kColRuns;
+  unsigned* fr = fruns + (size_t)o * kFineRuns;
+  const bool wc = ntot[0] <= kColRuns, wf = ntot[1] <= kFineRuns;
+#pragma unroll
+  for (int i = 0; i < kListHold; ++i) {
+    const int c = tid + i * kListThreads;
+    if (wc && c < nc) {
+      const int X = c / ky, Y = c - X * ky;
+      for_runs(hc[i], [&](int a, int L) { cr[atomicAdd(&cnt[0][L - 1], 1)] = pack_run(X, Y, a, L); });
+    }
+    if (wf && c < nf) {
+      const int x = c / d.ey, y = c - x * d.ey;
+      for_runs(hf[i], [&](int a, int L) { fr[atomicAdd(&cnt[1][L - 1], 1)] = pack_run(x, y, a, L); });
+    }
+  }
+  if (wc)
+    for (int c = tid + kListHold * kListThreads; c < nc; c += kListThreads) {
+      const int X = c / ky, Y = c - X * ky;
+      for_runs(core[X * ccy + Y],
+               [&](int a, int L) { cr[atomicAdd(&cnt[0][L - 1], 1)] = pack_run(X, Y, a, L); });
+    }
+  if (wf)
+    for (int c = tid + kListHold * kListThreads; c < nf; c += kListThreads) {
+      const int x = c / d.ey, y = c - x * d.ey;
+      for_runs(own[x * my + y],
+               [&](int a, int L) { fr[atomicAdd(&cnt[1][L - 1], 1)] = pack_run(x, y, a, L); });
+    }
+  if (tid == 0) {
+    nrun[2 * o] = clist ? ntot[0] : kColRuns + 1;
+    nrun[2 * o + 1] = ntot[1];
+  }
+}
+
+// Item column masks, core masks and sorted run lists in one launch, for
+// items whose footprint ex * ey is at most kMaskCols (the masks are built in
+// LDS): one block per orientation.  Thread per item column: bit z of
+// im[(o * mx + x) * my + y] is set when the item has a voxel > 0 at (x, y, z);
+// core (X, Y) is the AND of the 3 x 3 columns around (2 X, 2 Y) (0 outside the
+// item), written to cm[(o * ccx + X) * ccy + Y]; then the run lists of both,
+// as k_run_lists writes them.  Same outputs as k_item_columns, k_core_columns
+// and k_run_lists together.
+constexpr int kMaskThreads = 1024;
+constexpr int kMaskCols = 4096;
+__global__ void __launch_bounds__(kMaskThreads)
+k_item_masks(const int* __restrict__ items, const OrientDesc* __restrict__ od, int m,
+             M128* __restrict__ cm, int ccx, int ccy, M128* __restrict__ im, int mx, int my,
+             unsigned* __restrict__ runs, unsigned* __restrict__ fruns, int* __restrict__ nrun,
+             const int* __restrict__ ctl) {
+  if (search_off(ctl)) return;
+  const int o = blockIdx.x, tid = threadIdx.x;
+  if (o >= m) return;
+  const OrientDesc d = od[o];
+  __shared__ M128 own[kMaskCols];
+  __shared__ M128 core[kMaskCols / 2];   // ceil(ex/2) * ceil(ey/2) <= 2048 when ex * ey <= 4096
+  __shared__ int cnt[2][128];
+  __shared__ int ntot[2];
+  if (tid < 256) cnt[tid >> 7][tid & 127] = 0;
+  const int ne = d.ex * d.ey;
+  const int* base = items + d.item_off;
+  for (int c = tid; c < ne; c += kMaskThreads) {
+    const int x = c / d.ey, y = c - x * d.ey;
+    const int i0 = oriented_index(d, x, y, 0);
+    const int dz = d.ez > 1 ? oriented_index(d, x, y, 1) - i0 : 0;
+    unsigned long long lo = 0ULL, hi = 0ULL;
+#pragma unroll 8
+    for (int z = 0; z < d.ez; ++z) {
+      const unsigned long long b = base[i0 + z * dz] > 0 ? 1ULL : 0ULL;
+      if (z < 64) lo |= b << z; else hi |= b << (z - 64);
+    }
+    own[c] = M128{lo, hi};
+    im[((size_t)o * mx + x) * my + y] = M128{lo, hi};
+  }
+  __syncthreads();
+  const int kx = (d.ex + 1) / 2, ky = (d.ey + 1) / 2;
+  for (int c = tid; c < kx * ky; c += kMaskThreads) {
+    const int X = c / ky, Y = c - X * ky;
+    M128 r{0ULL, 0ULL};
+    if (2 * X + 1 < d.ex && 2 * Y + 1 < d.ey && X > 0 && Y > 0) {
+      r = M128{~0ULL, ~0ULL};
+      for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+          const M128 v = own[(2 * X + dx) * d.ey + (2 * Y + dy)];
+          r.lo &= v.lo;
+          r.hi &= v.hi;
+        }
+    }
+    core[c] = r;
+    cm[((size_t)o * ccx + X) * ccy + Y] = r;
+  }
+  __syncthreads();
+  const bool clist = kx <= 128 && ky <= 256;   // the packed run's x and y fields
+  const bool fine = d.ex <= 128 && d.ey <= 256;
+  if (clist)
+    for (int c = tid; c < kx * ky; c += kMaskThreads)
+      for_runs(core[c], [&](int, int L) { atomicAdd(&cnt[0][L - 1], 1); });
+  if (fine)
+    for (int c = tid; c < ne; c += kMaskThreads)
+      for_runs(own[c], [&](int, int L) { atomicAdd(&cnt[1][L - 1], 1); });
+  __syncthreads();
+  if (tid < 128) {
+    // wave w scans list w: lane l holds lengths 128 - 2l and 127 - 2l
+    const int w = tid >> 6, lane = tid & 63;
+    const int v0 = cnt[w][127 - 2 * lane], v1 = cnt[w][126 - 2 * lane];
+    int incl = v0 + v1;
+    for (int s2 = 1; s2 < 64; s2 <<= 1) {
+      const int u = __shfl_up(incl, s2, 64);
+      if (lane >= s2) incl += u;
+    }
+    const int excl = incl - v0 - v1;
+    cnt[w][127 - 2 * lane] = excl;
+    cnt[w][126 - 2 * lane] = excl + v0;
+    if (lane == 63) ntot[w] = incl;
+  }
+  __syncthreads();
+  unsigned* cr = runs + (size_t)o * kColRuns;
+  unsigned* fr = fruns + (size_t)o * kFineRuns;
+  if (clist && ntot[0] <= kColRuns)
+    for (int c = tid; c < kx * ky; c += kMaskThreads) {
+      const int X = c / ky, Y = c - X * ky;
+      for_runs(core[c], [&](int a, int L) { cr[atomicAdd(&cnt[0][L - 1], 1)] = pack_run(X, Y, a, L); });
+    }
+  if (fine && ntot[1] <= kFineRuns)
+    for (int c = tid; c < ne; c += kMaskThreads) {
+      const int x = c / d.ey, y = c - x * d.ey;
+      for_runs(own[c], [&](int a, int L) { fr[atomicAdd(&cnt[1][L - 1], 1)] = pack_run(x, y, a, L); });
+    }
+  if (tid == 0) {
+    nrun[2 * o] = clist ? ntot[0] : kColRuns + 1;
+    nrun[2 * o + 1] = fine ? ntot[1] : 0;
+  }
+}
+
 // OR over the G lanes of an aligned lane group (every lane of the group active)
 template <int G>
 __device__ __forceinline__ M128 group_or(M128 v) {
@@ -1385,155 +1605,164 @@ __device__ __forceinline__ M128 group_or(M128 v) {
   }
   return v;
 }
+template <int G>
+__device__ __forceinline__ int group_max(int v) {
+  for (int s = 1; s < G; s <<= 1) v = max(v, __shfl_xor(v, s, G));
+  return v;
+}
 
 template <int G>
 __global__ void __launch_bounds__(256)
 k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restrict__ cm,
               int ccx, int ccy, const M128* __restrict__ tf, int nx, int ny,
               const M128* __restrict__ im, int mx, int my, const OrientDesc* __restrict__ od,
-              int m, int* __restrict__ cfs, const int* __restrict__ ctl) {
+              int m, const unsigned* __restrict__ run_lists, const unsigned* __restrict__ fine_lists,
+              const int* __restrict__ nrun, const int* __restrict__ ht, int ty,
+              int* __restrict__ cfs, const int* __restrict__ ctl) {
   if (search_off(ctl)) return;
   const int o = blockIdx.y, tid = threadIdx.x;
   if (o >= m) return;
   const OrientDesc d = od[o];
   const int nqx = d.nvx > 0 ? (d.nvx - 1) / 2 + 1 : 0, nqy = d.nvy > 0 ? (d.nvy - 1) / 2 + 1 : 0;
   const int nq = nqx * nqy;
-  const int q0 = blockIdx.x * 64;
-  if (q0 >= nq || d.nvz <= 0) return;
+  const int nblk = (nq + 63) / 64;
+  if ((int)blockIdx.x >= nblk || d.nvz <= 0) return;
   volatile int* flag = cfs + kFeasHeader + o;
-  __shared__ unsigned runs[kColRuns];
-  __shared__ unsigned fruns[kFineRuns];
   __shared__ unsigned cand_list[256];
   __shared__ M128 cbad[64];
-  __shared__ int nruns, nfine, ncand, skip;
-  if (tid == 0) {
-    nruns = 0;
-    nfine = 0;
-    ncand = 0;
-    skip = *flag;   // one read, so that the whole block takes the same branch
-  }
-  __syncthreads();
-  if (skip) return;
+  __shared__ int ncand, skip;
   const int kx = (d.ex + 1) / 2, ky = (d.ey + 1) / 2;
   const M128* core = cm + (size_t)o * ccx * ccy;
-  for (int c = tid; c < kx * ky; c += 256) {
-    const int X = c / ky, Y = c - X * ky;
-    col_runs_of(core[X * ccy + Y], X, Y, runs, &nruns, kColRuns);
-  }
   const bool fine = tf != nullptr && d.ex <= 128 && d.ey <= 256;
   const M128* own = im + (size_t)o * mx * my;
-  if (fine)
-    for (int c = tid; c < d.ex * d.ey; c += 256) {
-      const int x = c / d.ey, y = c - x * d.ey;
-      col_runs_of(own[x * my + y], x, y, fruns, &nfine, kFineRuns);
-    }
-  __syncthreads();
-  const int nr = nruns, nf = nfine;
-  const int q = q0 + (tid >> 2), j = tid & 3;
-  const bool active = q < nq;
-  const int Qx = active ? q / nqy : 0, Qy = active ? q - Qx * nqy : 0;
+  // sorted run lists of this orientation (k_run_lists); the lanes of a quad
+  // or group read consecutive entries, shared by every quad of the wave
+  const unsigned* runs = run_lists + (size_t)o * kColRuns;
+  const unsigned* fruns = fine_lists + (size_t)o * kFineRuns;
+  const int nr = nrun[2 * o], nf = nrun[2 * o + 1];
   const int nv = d.nvz;
   const unsigned long long vlo = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1ULL);
   const unsigned long long vhi = nv >= 128 ? ~0ULL : (nv > 64 ? ((1ULL << (nv - 64)) - 1ULL) : 0ULL);
   const size_t plane = (size_t)cx * cy;
-  M128 bad{0ULL, 0ULL};
-  if (nr <= kColRuns) {
-    // rounds of 16 runs per lane; all four lanes of an offset run the same
-    // number of rounds (the combine uses shuffles within the quad)
-    bool done = !active;
-    for (int r0 = 0; r0 < nr; r0 += 64) {
-      if (!done) {
-        for (int r = r0 + j; r < min(nr, r0 + 64); r += 4) {
-          const unsigned e = runs[r];
-          const int X = e & 127, Y = (e >> 7) & 255, ra = (e >> 15) & 127;
-          const int L = ((e >> 22) & 127) + 1;
-          bad = m_or(bad, run_blocks_at(cols, plane, (size_t)(Qx + X) * cy + (Qy + Y), ra, L));
-        }
-      }
-      M128 u = bad;
-      for (int s2 = 1; s2 < 4; s2 <<= 1) {
-        u.lo |= __shfl_xor(u.lo, s2, 64);
-        u.hi |= __shfl_xor(u.hi, s2, 64);
-      }
-      bad = u;
-      if (!m_any_free(bad, vlo, vhi)) done = true;
-      if (__all(done)) break;
-    }
-  } else if (active) {
-    // more runs than the LDS list holds: every lane straight from the core masks
-    for (int X = 0; X < kx; ++X)
-      for (int Y = 0; Y < ky; ++Y) {
-        M128 k = core[X * ccy + Y];
-        while (k.lo | k.hi) {
-          const int a = m_ctz(k);
-          const M128 rest = m_shr(k, a);
-          const int L = m_ctz(M128{~rest.lo, ~rest.hi});
-          bad = m_or(bad, run_blocks_at(cols, plane, (size_t)(Qx + X) * cy + (Qy + Y), a, L));
-          k.lo &= ~bits64(min(a, 64), min(a + L, 64));
-          k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
-        }
-      }
-  }
-  const bool cand = active && m_any_free(bad, vlo, vhi);
-  if (!fine) {
-    if (cand && j == 0) *flag = 1;
-    return;
-  }
-  // full-resolution stage: the block lists the full-resolution offsets
-  // (2 Qx + rx, 2 Qy + ry) of every block offset with a height left; groups of
-  // G lanes then take one candidate at a time, split the item's runs between
-  // their lanes and combine in chunks of growing size, so that a refuted
-  // candidate stops after its first blocked chunk and a proof costs nf / G
-  // loads per lane
-  if (cand) {
-    if (j == 0) cbad[tid >> 2] = bad;
-    const int qx = 2 * Qx + (j & 1), qy = 2 * Qy + (j >> 1);
-    if (qx < d.nvx && qy < d.nvy) cand_list[atomicAdd(&ncand, 1)] = (unsigned)tid;
-  }
-  __syncthreads();
-  const int nc = ncand;
-  const int gl = tid & (G - 1);
   const size_t fplane = (size_t)nx * ny;
-  for (int c = tid / G; c < nc; c += 256 / G) {
-    if (__shfl((int)*flag, 0, G)) break;   // the group leader's read, so the group stays together
-    const int e = (int)cand_list[c];
-    const int qq = q0 + (e >> 2), je = e & 3;
-    const int QX = qq / nqy, QY = qq - QX * nqy;
-    const int qx = 2 * QX + (je & 1), qy = 2 * QY + (je >> 1);
-    M128 fb = cbad[e >> 2];
-    bool blocked = false;
-    if (nf <= kFineRuns) {
-      int step = 4 * G;
-      for (int r0 = 0; r0 < nf && !blocked; r0 += step, step = min(step * 2, 64 * G)) {
-        const int r1 = min(nf, r0 + step);
-        for (int r = r0 + gl; r < r1; r += G) {
-          const unsigned u = fruns[r];
-          const int x = u & 127, y = (u >> 7) & 255, ra = (u >> 15) & 127, L = ((u >> 22) & 127) + 1;
-          fb = m_or(fb, run_blocks_at(tf, fplane, (size_t)(qx + x) * ny + (qy + y), ra, L));
+  const int j = tid & 3, gl = tid & (G - 1);
+  // chunks of 64 offsets: blockIdx.x, blockIdx.x + gridDim.x, ...
+  for (int qb = blockIdx.x; qb < nblk; qb += gridDim.x) {
+    const int q0 = qb * 64;
+    __syncthreads();   // the previous chunk's candidate list is consumed
+    if (tid == 0) {
+      ncand = 0;
+      skip = *flag;   // one read, so that the whole block takes the same branch
+    }
+    __syncthreads();
+    if (skip) return;
+    const int q = q0 + (tid >> 2);
+    const bool active = q < nq;
+    const int Qx = active ? q / nqy : 0, Qy = active ? q - Qx * nqy : 0;
+    M128 bad{0ULL, 0ULL};
+    if (nr <= kColRuns) {
+      // rounds of 4, 8, 16, 32, then 64 runs per offset; all four lanes of an
+      // offset run the same rounds (the combine uses shuffles within the quad)
+      bool done = !active;
+      for (int r0 = 0, step = 4; r0 < nr; r0 += step, step = min(2 * step, 64)) {
+        if (!done) {
+          const int r1 = min(nr, r0 + step);
+          for (int r = r0 + j; r < r1; r += 4) {
+            const unsigned e = runs[r];
+            const int X = e & 127, Y = (e >> 7) & 255, ra = (e >> 15) & 127;
+            const int L = ((e >> 22) & 127) + 1;
+            bad = m_or(bad, run_blocks_at(cols, plane, (size_t)(Qx + X) * cy + (Qy + Y), ra, L));
+          }
+        }
+        M128 u = bad;
+        for (int s2 = 1; s2 < 4; s2 <<= 1) {
+          u.lo |= __shfl_xor(u.lo, s2, 64);
+          u.hi |= __shfl_xor(u.hi, s2, 64);
+        }
+        bad = u;
+        if (!m_any_free(bad, vlo, vhi)) done = true;
+        if (__all(done)) break;
+      }
+    } else if (active) {
+      // no run list (too many runs, or extents beyond its fields): every lane
+      // straight from the core masks
+      for (int X = 0; X < kx; ++X)
+        for (int Y = 0; Y < ky; ++Y)
+          for_runs(core[X * ccy + Y], [&](int a, int L) {
+            bad = m_or(bad, run_blocks_at(cols, plane, (size_t)(Qx + X) * cy + (Qy + Y), a, L));
+          });
+    }
+    const bool cand = active && m_any_free(bad, vlo, vhi);
+    if (!fine) {
+      if (cand && j == 0) *flag = 1;
+      continue;
+    }
+    // full-resolution stage: the block lists the full-resolution offsets
+    // (2 Qx + rx, 2 Qy + ry) of every block offset with a height left; groups
+    // of G lanes then take one candidate at a time, split the item's runs
+    // between their lanes and combine in chunks of growing size, so that a
+    // refuted candidate stops after its first blocked chunk and a proof costs
+    // nf / G loads per lane
+    if (cand) {
+      if (j == 0) cbad[tid >> 2] = bad;
+      const int qx = 2 * Qx + (j & 1), qy = 2 * Qy + (j >> 1);
+      if (qx < d.nvx && qy < d.nvy) cand_list[atomicAdd(&ncand, 1)] = (unsigned)tid;
+    }
+    __syncthreads();
+    const int nc = ncand;
+    for (int c = tid / G; c < nc; c += 256 / G) {
+      if (__shfl((int)*flag, 0, G)) break;   // the group leader's read, so the group stays together
+      const int e = (int)cand_list[c];
+      const int qq = q0 + (e >> 2), je = e & 3;
+      const int QX = qq / nqy, QY = qq - QX * nqy;
+      const int qx = 2 * QX + (je & 1), qy = 2 * QY + (je >> 1);
+      // sufficient test first: every tray column under the footprint
+      // [qx, qx + ex) x [qy, qy + ey) is at most h high (h = max of the tile
+      // heights there), so the item's box at height h is empty, a valid
+      // offset when h < nvz
+      {
+        const int tx0 = qx / kSkyTile, tx1 = (qx + d.ex - 1) / kSkyTile;
+        const int ty0 = qy / kSkyTile, ty1 = (qy + d.ey - 1) / kSkyTile;
+        const int nyt = ty1 - ty0 + 1, nt = (tx1 - tx0 + 1) * nyt;
+        int h = 0;
+        for (int i = gl; i < nt; i += G) {
+          const int a = i / nyt;
+          h = max(h, ht[(tx0 + a) * ty + ty0 + (i - a * nyt)]);
+        }
+        if (group_max<G>(h) < d.nvz) {
+          if (gl == 0) *flag = 1;
+          break;
+        }
+      }
+      M128 fb = cbad[e >> 2];
+      bool blocked = false;
+      if (nf <= kFineRuns) {
+        int step = G;
+        for (int r0 = 0; r0 < nf && !blocked; r0 += step, step = min(step * 2, 64 * G)) {
+          const int r1 = min(nf, r0 + step);
+          for (int r = r0 + gl; r < r1; r += G) {
+            const unsigned u = fruns[r];
+            const int x = u & 127, y = (u >> 7) & 255, ra = (u >> 15) & 127, L = ((u >> 22) & 127) + 1;
+            fb = m_or(fb, run_blocks_at(tf, fplane, (size_t)(qx + x) * ny + (qy + y), ra, L));
+          }
+          fb = group_or<G>(fb);
+          blocked = !m_any_free(fb, vlo, vhi);
+        }
+      } else {
+        // no run list: lanes over the item's columns
+        for (int cc = gl; cc < d.ex * d.ey; cc += G) {
+          const int x = cc / d.ey, y = cc - x * d.ey;
+          for_runs(own[x * my + y], [&](int a, int L) {
+            fb = m_or(fb, run_blocks_at(tf, fplane, (size_t)(qx + x) * ny + (qy + y), a, L));
+          });
         }
         fb = group_or<G>(fb);
         blocked = !m_any_free(fb, vlo, vhi);
       }
-    } else {
-      // more runs than the LDS list holds: lanes over the item's columns
-      for (int cc = gl; cc < d.ex * d.ey; cc += G) {
-        const int x = cc / d.ey, y = cc - x * d.ey;
-        M128 k = own[x * my + y];
-        while (k.lo | k.hi) {
-          const int a = m_ctz(k);
-          const M128 rest = m_shr(k, a);
-          const int L = m_ctz(M128{~rest.lo, ~rest.hi});
-          fb = m_or(fb, run_blocks_at(tf, fplane, (size_t)(qx + x) * ny + (qy + y), a, L));
-          k.lo &= ~bits64(min(a, 64), min(a + L, 64));
-          k.hi &= ~bits64(max(a - 64, 0), max(a + L - 64, 0));
-        }
+      if (!blocked) {
+        if (gl == 0) *flag = 1;
+        break;
       }
-      fb = group_or<G>(fb);
-      blocked = !m_any_free(fb, vlo, vhi);
-    }
-    if (!blocked) {
-      if (gl == 0) *flag = 1;
-      break;
     }
   }
 }

@@ -484,10 +484,12 @@ constexpr int kItemCells = 32;                 // item grid 32^3, pitch 2 voxels
 constexpr int kTray0 = 64, kTray1 = 32, kTray2 = 32;
 
 // An orientation of a base item as an index map: oriented axis a runs along
-// base axis perm[a], reversed when flip[a] != 0.
+// base axis perm[a], reversed when flip[a] != 0; the base item's voxels start
+// `off` ints past the item pointer handed to the kernel.
 struct ItemMap {
   int bx, by, bz;
   int perm[3], flip[3];
+  int off;
 };
 
 // occupancy of the oriented item: cell (a,b,c) = (# of oriented voxels > 0
@@ -514,7 +516,7 @@ __global__ void k_item_feat(const int* __restrict__ item, const ItemMap* __restr
       const int ax = mp.perm[d];
       u[ax] = mp.flip[d] ? bd[ax] - 1 - w[d] : w[d];
     }
-    if (item[((size_t)u[0] * mp.by + u[1]) * mp.bz + u[2]] > 0) ++cnt;
+    if (item[(size_t)mp.off + ((size_t)u[0] * mp.by + u[1]) * mp.bz + u[2]] > 0) ++cnt;
   }
   out[(size_t)blockIdx.y * (kItemCells * kItemCells * kItemCells) + i] = f2bf((float)cnt * 0.125f);
 }
