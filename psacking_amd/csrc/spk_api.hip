@@ -197,6 +197,8 @@ struct spk_ctx {
     DevBuf<unsigned> rl, frl;            // sorted run lists of the core and item columns
     DevBuf<int> nrun;                    // their lengths
     DevBuf<int> ht;                      // tile heights of the tray (kSkyTile columns square)
+    DevBuf<M128> lines;                  // line masks of the packed items (k_item_lines)
+    DevBuf<ItemLines> il;
     int tx = 0, ty = 0;
     DevBuf<int> fs;                      // residual slot (0), per-orientation flags
     int fine_group = 64;                 // option "fine_group": lanes per full-resolution candidate
@@ -565,6 +567,7 @@ HostItem scan_item(const int* data, int ix, int iy, int iz) {
 OrientDesc make_desc(const spk_ctx* c, const HostItem& it, int item_off, const spk_orient* o) {
   OrientDesc d{};
   d.item_off = item_off;
+  d.line_off = -1;
   d.bx = it.dims[0]; d.by = it.dims[1]; d.bz = it.dims[2];
   int perm[3] = {0, 1, 2}, flip[3] = {0, 0, 0};
   if (o) {
@@ -815,7 +818,7 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
   q.nrun.ensure(2 * (size_t)m);
   if (max_exy <= kMaskCols) {
     launch(c, "item_masks", k_item_masks, dim3(m), dim3(kMaskThreads), 0, (const int*)c->items.p,
-           od, m, q.cm.p, ccx, ccy, q.im.p, mx, my, q.rl.p, q.frl.p, q.nrun.p, ctl);
+           (const M128*)q.lines.p, od, m, q.cm.p, ccx, ccy, q.im.p, mx, my, q.rl.p, q.frl.p, q.nrun.p, ctl);
   } else {
     launch(c, "core_columns", k_core_columns, dim3((ccx * ccy + 3) / 4, m), dim3(256), 0,
            (const int*)c->items.p, od, m, q.cm.p, ccx, ccy, ctl);
@@ -1077,6 +1080,25 @@ PackPlan plan_pack(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
   P.items.resize(n_items);
   std::vector<int> map_all;
   int ws_m = 1, ws_ex = 1;
+  // line masks of the base items, read by the column test's item masks
+  std::vector<int> line_off(n_items, -1);
+  if (c->co.ready && n_items <= 65535) {
+    std::vector<ItemLines> il(n_items);
+    int tot = 0, maxl = 1;
+    for (int i = 0; i < n_items; ++i) {
+      const HostItem& b = bases[i];
+      il[i] = ItemLines{off[i], b.dims[0], b.dims[1], b.dims[2], tot};
+      line_off[i] = tot;
+      const int nl = b.dims[0] * b.dims[1] + b.dims[0] * b.dims[2] + b.dims[1] * b.dims[2];
+      tot += nl;
+      maxl = std::max(maxl, nl);
+    }
+    c->co.lines.ensure(std::max(tot, 1));
+    c->co.il.ensure(n_items);
+    upload(c, c->co.il.p, il.data(), il.size() * sizeof(ItemLines));
+    launch(c, "item_lines", k_item_lines, dim3((maxl + 255) / 256, n_items), dim3(256), 0,
+           (const int*)c->items.p, (const ItemLines*)c->co.il.p, c->co.lines.p);
+  }
   for (int t = 0; t < n_items; ++t) {
     const int it = order ? order[t] : t;
     ItemPlan& pl = P.items[t];
@@ -1084,6 +1106,7 @@ PackPlan plan_pack(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
     for (int o = 0; o < n_orient; ++o) {
       OrientDesc d = make_desc(c, bases[it], off[it], orients ? &orients[o] : nullptr);
       if (!fits(c, d)) continue;
+      d.line_off = line_off[it];
       P.all.push_back(d);
       map_all.push_back(o);
       pl.map.push_back(o);
@@ -2209,6 +2232,7 @@ int spk_place_grid(spk_ctx* c, const int32_t* tray, int nx, int ny, int nz, cons
     upload(c, g.p, tray, V * sizeof(int));
     upload(c, it.p, item, I * sizeof(int));
     OrientDesc d{};
+    d.line_off = -1;
     d.bx = ix; d.by = iy; d.bz = iz;
     d.perm[0] = 0; d.perm[1] = 1; d.perm[2] = 2;
     d.ex = ix; d.ey = iy; d.ez = iz;

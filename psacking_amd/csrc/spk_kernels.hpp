@@ -74,6 +74,7 @@ struct OrientDesc {          // one searched orientation
   int ex, ey, ez;            // oriented dims
   int hx, hy, hz;            // oriented max index of voxels > 0 (-1 if empty)
   int nvx, nvy, nvz;         // valid offsets per axis: N - 1 - h  (+1), clamped
+  int line_off;              // the base item's line masks (k_item_lines), -1 if none
   long long isum;            // sum of item values
 };
 
@@ -1497,6 +1498,51 @@ k_run_lists(const M128* __restrict__ cm, int ccx, int ccy, const M128* __restric
   }
 }
 
+// Line masks of base items, for the column test's item masks: per item three
+// tables of M128 at lines + line_off, bit k set when the voxel at coordinate k
+// along the line is > 0 (lines of at most 128 voxels; a longer axis gets no
+// table and is never a fitting orientation's z axis when the tray's z is at
+// most 128): lines along z indexed u0 * by + u1, then along y indexed
+// u0 * bz + u2, then along x indexed u1 * bz + u2.  Grid (ceil(max lines /
+// 256), items), a thread per line.
+struct ItemLines {
+  int off;        // voxel offset of the base item in the item buffer
+  int bx, by, bz;
+  int line_off;   // first M128 of its tables
+};
+__global__ void __launch_bounds__(256)
+k_item_lines(const int* __restrict__ items, const ItemLines* __restrict__ il, M128* __restrict__ lines) {
+  const ItemLines L = il[blockIdx.y];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int nz = L.bx * L.by, ny = L.bx * L.bz, nxl = L.by * L.bz;
+  if (i >= nz + ny + nxl) return;
+  const int* b = items + L.off;
+  unsigned long long lo = 0ULL, hi = 0ULL;
+  int start, step, len;
+  if (i < nz) {
+    const int u0 = i / L.by, u1 = i - u0 * L.by;
+    start = (u0 * L.by + u1) * L.bz; step = 1; len = L.bz;
+  } else if (i < nz + ny) {
+    const int j = i - nz, u0 = j / L.bz, u2 = j - u0 * L.bz;
+    start = u0 * L.by * L.bz + u2; step = L.bz; len = L.by;
+  } else {
+    const int j = i - nz - ny, u1 = j / L.bz, u2 = j - u1 * L.bz;
+    start = u1 * L.bz + u2; step = L.by * L.bz; len = L.bx;
+  }
+  if (len <= 128)
+    for (int k = 0; k < len; ++k) {
+      const unsigned long long v = b[start + k * step] > 0 ? 1ULL : 0ULL;
+      if (k < 64) lo |= v << k; else hi |= v << (k - 64);
+    }
+  lines[L.line_off + i] = M128{lo, hi};
+}
+
+// bit i of the result = bit n - 1 - i of a (0 < n <= 128; bits >= n of a are 0)
+__device__ __forceinline__ M128 m_reverse(M128 a, int n) {
+  const M128 r{__brevll(a.hi), __brevll(a.lo)};
+  return m_shr(r, 128 - n);
+}
+
 // Item column masks, core masks and sorted run lists in one launch, for
 // items whose footprint ex * ey is at most kMaskCols (the masks are built in
 // LDS): one block per orientation.  Thread per item column: bit z of
@@ -1504,11 +1550,14 @@ k_run_lists(const M128* __restrict__ cm, int ccx, int ccy, const M128* __restric
 // core (X, Y) is the AND of the 3 x 3 columns around (2 X, 2 Y) (0 outside the
 // item), written to cm[(o * ccx + X) * ccy + Y]; then the run lists of both,
 // as k_run_lists writes them.  Same outputs as k_item_columns, k_core_columns
-// and k_run_lists together.
+// and k_run_lists together.  With line masks (lines != nullptr, d.line_off >=
+// 0) an item column is read as one line of the base item (reversed when the z
+// axis is flipped), else gathered voxel by voxel.
 constexpr int kMaskThreads = 1024;
 constexpr int kMaskCols = 4096;
 __global__ void __launch_bounds__(kMaskThreads)
-k_item_masks(const int* __restrict__ items, const OrientDesc* __restrict__ od, int m,
+k_item_masks(const int* __restrict__ items, const M128* __restrict__ lines,
+             const OrientDesc* __restrict__ od, int m,
              M128* __restrict__ cm, int ccx, int ccy, M128* __restrict__ im, int mx, int my,
              unsigned* __restrict__ runs, unsigned* __restrict__ fruns, int* __restrict__ nrun,
              const int* __restrict__ ctl) {
@@ -1523,6 +1572,23 @@ k_item_masks(const int* __restrict__ items, const OrientDesc* __restrict__ od, i
   if (tid < 256) cnt[tid >> 7][tid & 127] = 0;
   const int ne = d.ex * d.ey;
   const int* base = items + d.item_off;
+  if (lines && d.line_off >= 0 && d.ez <= 128) {
+    // the column is a line of the base item along base axis A = perm[2]
+    const int A = d.perm[2], p = A == 0 ? 1 : 0, q = A == 2 ? 1 : 2;   // p < q: the others
+    const int bd[3] = {d.bx, d.by, d.bz};
+    const M128* tab = lines + d.line_off +
+                      (A == 2 ? 0 : (A == 1 ? d.bx * d.by : d.bx * d.by + d.bx * d.bz));
+    for (int c = tid; c < ne; c += kMaskThreads) {
+      const int x = c / d.ey, y = c - x * d.ey;
+      int u[3] = {0, 0, 0};
+      u[d.perm[0]] = d.flip[0] ? bd[d.perm[0]] - 1 - x : x;
+      u[d.perm[1]] = d.flip[1] ? bd[d.perm[1]] - 1 - y : y;
+      M128 r = tab[u[p] * bd[q] + u[q]];
+      if (d.flip[2]) r = m_reverse(r, d.ez);
+      own[c] = r;
+      im[((size_t)o * mx + x) * my + y] = r;
+    }
+  } else
   for (int c = tid; c < ne; c += kMaskThreads) {
     const int x = c / d.ey, y = c - x * d.ey;
     const int i0 = oriented_index(d, x, y, 0);
