@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -185,6 +186,8 @@ struct spk_ctx {
   AxisPlan sy{}, sz{};                   // y and z plans of the score pass
   int pad_y = 0, pad_z = 0;              // options "pad_y", "pad_z": padded sizes (0 = smallest smooth)
   int feas_group = 0;                    // option "feas_group": pairs per feasibility launch pair
+  bool screen_all = true;                // option "screen_all": learned loop screens every item
+                                         // with the column test before proposing
   int dist_y = 2;                        // option "dist_y": 1 envelope, 2 full scan
   int dist_x = 0;                        // option "dist_x": 0 outward search, 2 full scan
   // column test before the feasibility pass (option "coarse")
@@ -218,6 +221,7 @@ struct spk_ctx {
   DevBuf<unsigned long long> pairs;          // feasibility pairs transformed: full, coarse
   Mailbox* mail_h = nullptr;                 // pinned host mailbox and its device address
   Mailbox* mail_d = nullptr;
+  unsigned mail_seq = 0;                     // sequence number of the last post
   // host-side copy of item buffer bookkeeping
   std::vector<int> h_items;
 
@@ -342,11 +346,30 @@ size_t pencil_shmem(const spk_ctx* c) {
 const Mailbox& post_and_sync(spk_ctx* c, const int* ctl, const PackOut* po, const FieldStats* st,
                              const float* logits, const int* fs, int nfs) {
   if (!c->mail_h) {
-    HIP_OK(hipHostMalloc((void**)&c->mail_h, sizeof(Mailbox), hipHostMallocMapped));
+    HIP_OK(hipHostMalloc((void**)&c->mail_h, sizeof(Mailbox),
+                         hipHostMallocMapped | hipHostMallocCoherent));
     HIP_OK(hipHostGetDevicePointer((void**)&c->mail_d, c->mail_h, 0));
+    c->mail_h->seq = 0;
+    c->mail_seq = 0;
   }
-  launch(c, "post", k_post, dim3(1), dim3(64), 0, ctl, po, st, logits, fs, nfs, c->mail_d);
-  sync(c);
+  const unsigned seq = ++c->mail_seq;
+  launch(c, "post", k_post, dim3(1), dim3(64), 0, ctl, po, st, logits, fs, nfs, seq, c->mail_d);
+  // The host spins on the sequence word (a stream synchronisation wakes the
+  // host tens of microseconds later); after a bounded spin it falls back to
+  // the synchronisation, which also reports a failed kernel.
+  const auto t0 = std::chrono::steady_clock::now();
+  bool seen = false;
+  for (int it = 0; !seen; ++it) {
+    seen = __atomic_load_n(&c->mail_h->seq, __ATOMIC_ACQUIRE) == seq;
+    if (!seen && (it & 4095) == 4095 &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(100))
+      break;
+  }
+  if (!seen) {
+    sync(c);
+    if (__atomic_load_n(&c->mail_h->seq, __ATOMIC_ACQUIRE) != seq)
+      throw SpkError{SPK_EDEVICE, "host mailbox not written"};
+  }
   return *c->mail_h;
 }
 
@@ -1180,14 +1203,16 @@ const FieldStats* item_bounds(spk_ctx* c, const std::vector<HostItem>& bases,
 void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_exy, int max_e,
                   int penalty, int t, int32_t id_offset, int halt_not_found, double* rec_row,
                   const int* rec_map, int feas_mode = -1,
-                  const FieldStats* item_bound = nullptr) {
+                  const FieldStats* item_bound = nullptr, bool known_feasible = false) {
   if (feas_mode < 0) feas_mode = c->feas_mode;
   const bool feas = feas_mode && !c->split;
   if (feas || item_bound || c->co.ready) {
     launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p,
            c->co.ready ? c->co.fs.p : (int*)nullptr, m, feas ? feas_mode : 0, c->ctl.p,
            (const FieldStats*)c->stats.p, item_bound, c->pout.p, t);
-    enqueue_coarse(c, od, m, max_ex, max_exy, c->ctl.p, c->pout.p, t, halt_not_found);
+    // orientations a column test already found feasible need no second one
+    if (!known_feasible)
+      enqueue_coarse(c, od, m, max_ex, max_exy, c->ctl.p, c->pout.p, t, halt_not_found);
     enqueue_item_spectra(c, od, m, max_ex, c->ctl.p);
     // the column test's flags are exact: no FFT feasibility pass after it
     if (feas && !c->co.ready) {
@@ -1525,7 +1550,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     std::vector<int> cand(pl.m);
     for (int i = 0; i < pl.m; ++i) cand[i] = i;
     bool screened = false;
-    if (feas_plan && !prev_placed) {
+    if (feas_plan && (!prev_placed || (c->screen_all && c->co.ready))) {
       screened = screen(pl, cand);
       if (screened && cand.empty()) {
         ++cnt[4];
@@ -1573,8 +1598,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     if (use_prop && proposals)
       for (size_t j = 0; j < pick.size(); ++j) proposals[(size_t)t * topk + j] = pl.map[pick[j]];
     enqueue_step(c, od, (int)pick.size(), max_ex, pl.max_exy, max_e, penalty, t, id_offset,
-                 use_prop ? 1 : 0,
-                 nullptr, nullptr, 0);
+                 use_prop ? 1 : 0, nullptr, nullptr, 0, nullptr, screened && c->co.ready);
     // the next searched item's logits, on the tray this step leaves
     int tn = t + 1;
     while (tn < n_items && plans[tn].m == 0) ++tn;
@@ -2452,6 +2476,10 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->co.on = value != 0.0;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "screen_all") {
+    c->screen_all = value != 0.0;
     return SPK_OK;
   }
   if (std::string(name) == "col_chunks") {

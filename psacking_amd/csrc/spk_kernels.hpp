@@ -1155,12 +1155,16 @@ struct Mailbox {
   PackOut po;
   float logits[kMailLogits];
   int fs[kMailFlags];
+  unsigned seq;              // written last: the post this content belongs to
 };
 
-// any pointer may be null (that part is left as it was)
+// any pointer may be null (that part is left as it was).  Every thread makes
+// its stores visible to the host before thread 0 stores seq, so a host that
+// sees seq can read the rest.
 __global__ void k_post(const int* __restrict__ ctl, const PackOut* __restrict__ po,
                        const FieldStats* __restrict__ st, const float* __restrict__ logits,
-                       const int* __restrict__ fs, int nfs, Mailbox* __restrict__ mb) {
+                       const int* __restrict__ fs, int nfs, unsigned seq,
+                       Mailbox* __restrict__ mb) {
   const int i = threadIdx.x;
   if (ctl && i < CTL_WORDS) mb->ctl[i] = ctl[i];
   if (logits && i < kMailLogits) mb->logits[i] = logits[i];
@@ -1169,6 +1173,12 @@ __global__ void k_post(const int* __restrict__ ctl, const PackOut* __restrict__ 
   if (i == 0) {
     if (st) mb->st = *st;
     if (po) mb->po = *po;
+  }
+  __threadfence_system();
+  __syncthreads();
+  if (i == 0) {
+    __threadfence_system();
+    *reinterpret_cast<volatile unsigned*>(&mb->seq) = seq;
   }
 }
 

@@ -214,18 +214,23 @@ def test_pack_records_matches_oracle(P, PR):
         np.testing.assert_array_equal(bits[t], N.bf16_bits(N.tray_features(trays[t])).reshape(-1))
 
 
-@pytest.mark.parametrize("topk,min_conf,resid", [(4, 0.0, None), (2, 0.0, -1.0), (24, 0.0, None),
-                                                 (4, 1.01, None)])
-def test_pack_learned_replays_on_oracle(P, PR, topk, min_conf, resid):
+@pytest.mark.parametrize("topk,min_conf,resid,screen_all", [(4, 0.0, None, 1), (2, 0.0, -1.0, 1),
+                                                            (24, 0.0, None, 1), (4, 1.01, None, 1),
+                                                            (4, 0.0, None, 0)])
+def test_pack_learned_replays_on_oracle(P, PR, topk, min_conf, resid, screen_all):
+    """screen_all 1 (default): every item is screened by the column test
+    before proposing; 0: only after an item that placed nothing, so proposals
+    may be infeasible and the remaining candidates are searched after them."""
     from psacking_amd.rotations import orientation_maps
     w, order = _workload(47, 7, (56, 48, 40))
     perms, flips = orientation_maps(24)
     nn = PR.Proposer(0, 8, PR.init_params(48))
     s = P.Session(0)
+    s.set_option("screen_all", screen_all)
     if resid is not None:
         s.set_option("resid_limit", resid)
     s.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
-    say("pack learned, topk", topk, "min_conf", min_conf, "resid_limit", resid)
+    say("pack learned, topk", topk, "min_conf", min_conf, "resid_limit", resid, "screen_all", screen_all)
     res, props, counts = PR.pack_learned(s, nn, w.items, order, perms, flips, "cube",
                                          topk=topk, min_conf=min_conf)
     if topk >= 24 or min_conf > 1.0:
