@@ -1458,10 +1458,17 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   const bool feas_plan = c->feas_mode != 0 && !c->split;
   float logits[kNnLogits];
   int fsv[kFeasHeader + kNnLogits];
+  // nn->tpool holds the tray branch of the tray as the stream leaves it: the
+  // head of an item then needs only its own rows (no step or exact placement
+  // was enqueued since the tray branch was)
+  bool tpool_valid = false;
   auto enqueue_head = [&](int t) {
     timed_group(c, "proposer", [&]() {
-      nn_tray_features(s, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, c->tfeat.p);
-      nn_tray_branch(nn, s, c->tfeat.p, 1);
+      if (!tpool_valid) {
+        nn_tray_features(s, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, c->tfeat.p);
+        nn_tray_branch(nn, s, c->tfeat.p, 1);
+        tpool_valid = true;
+      }
       nn_head(nn, s, c->ipool.p + (size_t)t * kNnLogits * kNnItemPooled, 1);
     });
   };
@@ -1599,6 +1606,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       for (size_t j = 0; j < pick.size(); ++j) proposals[(size_t)t * topk + j] = pl.map[pick[j]];
     enqueue_step(c, od, (int)pick.size(), max_ex, pl.max_exy, max_e, penalty, t, id_offset,
                  use_prop ? 1 : 0, nullptr, nullptr, 0, nullptr, screened && c->co.ready);
+    tpool_valid = false;
     // the next searched item's logits, on the tray this step leaves
     int tn = t + 1;
     while (tn < n_items && plans[tn].m == 0) ++tn;
@@ -1641,8 +1649,8 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       if (!rest.empty()) {
         const OrientDesc* odr = gather(pl, rest, c->od_rest.p, &max_ex, &max_e);
         enqueue_step(c, odr, (int)rest.size(), max_ex, pl.max_exy, max_e, penalty, t, id_offset, 0,
-                     nullptr,
-                     nullptr, 0);
+                     nullptr, nullptr, 0);
+        tpool_valid = false;
         {
           const Mailbox& mb = post_and_sync(c, c->ctl.p, c->pout.p + t, c->stats.p, nullptr,
                                             nullptr, 0);
@@ -1676,6 +1684,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       eb = exact_over(c, bases[it], &all[pl.od_off], every, penalty, nullptr);
     }
     placed += place_exact(c, eb, pl, all, resid, id_offset + placed + 1, &results[t]);
+    tpool_valid = false;
     prev_placed = eb.fit >= 0;
     reset_ctl();
     tst = post_and_sync(c, nullptr, nullptr, c->stats.p, nullptr, nullptr, 0).st;
