@@ -828,8 +828,11 @@ const int* ctl_feas_on(spk_ctx* c) {
 // pass and the scored search: core masks of the oriented items, the test,
 // then k_coarse_decide (prune orientations, or decide the item: see there).
 // ctl must not be null.
+// With clear_flags the column test's flags are cleared here (no k_feas_begin
+// before it); callers only pass it for max_exy <= kMaskCols.
 void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_exy,
-                    const int* ctl, PackOut* out = nullptr, int t = 0, int halt_not_found = 0) {
+                    const int* ctl, PackOut* out = nullptr, int t = 0, int halt_not_found = 0,
+                    bool clear_flags = false) {
   spk_ctx::Coarse& q = c->co;
   if (!q.ready) return;
   const int ccx = (max_ex + 1) / 2, ccy = q.cy;
@@ -841,7 +844,8 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
   q.nrun.ensure(2 * (size_t)m);
   if (max_exy <= kMaskCols) {
     launch(c, "item_masks", k_item_masks, dim3(m), dim3(kMaskThreads), 0, (const int*)c->items.p,
-           (const M128*)q.lines.p, od, m, q.cm.p, ccx, ccy, q.im.p, mx, my, q.rl.p, q.frl.p, q.nrun.p, ctl);
+           (const M128*)q.lines.p, od, m, q.cm.p, ccx, ccy, q.im.p, mx, my, q.rl.p, q.frl.p,
+           q.nrun.p, clear_flags ? q.fs.p : (int*)nullptr, ctl);
   } else {
     launch(c, "core_columns", k_core_columns, dim3((ccx * ccy + 3) / 4, m), dim3(256), 0,
            (const int*)c->items.p, od, m, q.cm.p, ccx, ccy, ctl);
@@ -1495,12 +1499,15 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     int max_ex, max_e;
     const OrientDesc* od = gather(pl, list, c->od_rest.p, &max_ex, &max_e);
     const int k = (int)list.size();
-    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p,
-           c->co.ready ? c->co.fs.p : (int*)nullptr, k, 2, (int*)nullptr,
-           (const FieldStats*)nullptr, (const FieldStats*)nullptr, (PackOut*)nullptr, 0);
+    // the item masks clear the flags when they come from k_item_masks
+    const bool fused = c->co.ready && pl.max_exy <= kMaskCols;
+    if (!fused)
+      launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p,
+             c->co.ready ? c->co.fs.p : (int*)nullptr, k, 2, (int*)nullptr,
+             (const FieldStats*)nullptr, (const FieldStats*)nullptr, (PackOut*)nullptr, 0);
     if (c->co.ready) {
       // the column test alone: its flags are exact (residual slot 0)
-      enqueue_coarse(c, od, k, max_ex, pl.max_exy, c->ctl_on.p);
+      enqueue_coarse(c, od, k, max_ex, pl.max_exy, c->ctl_on.p, nullptr, 0, 0, fused);
       const Mailbox& mb = post_and_sync(c, nullptr, nullptr, nullptr, nullptr, c->co.fs.p,
                                         kFeasHeader + k);
       std::memcpy(fsv, mb.fs, (kFeasHeader + k) * sizeof(int));

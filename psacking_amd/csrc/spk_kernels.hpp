@@ -1570,10 +1570,15 @@ k_item_masks(const int* __restrict__ items, const M128* __restrict__ lines,
              const OrientDesc* __restrict__ od, int m,
              M128* __restrict__ cm, int ccx, int ccy, M128* __restrict__ im, int mx, int my,
              unsigned* __restrict__ runs, unsigned* __restrict__ fruns, int* __restrict__ nrun,
-             const int* __restrict__ ctl) {
+             int* __restrict__ cfs, const int* __restrict__ ctl) {
   if (search_off(ctl)) return;
   const int o = blockIdx.x, tid = threadIdx.x;
   if (o >= m) return;
+  if (cfs && tid == 0) {   // clears the column test's flags (cfs: null when already cleared)
+    cfs[kFeasHeader + o] = 0;
+    if (o == 0)
+      for (int h = 0; h < kFeasHeader; ++h) cfs[h] = 0;
+  }
   const OrientDesc d = od[o];
   __shared__ M128 own[kMaskCols];
   __shared__ M128 core[kMaskCols / 2];   // ceil(ex/2) * ceil(ey/2) <= 2048 when ex * ey <= 4096
@@ -1854,9 +1859,12 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
 __global__ void k_coarse_decide(const int* __restrict__ cfs, int m, int* __restrict__ fs,
                                 PackOut* __restrict__ out, int t, int halt_not_found,
                                 int* __restrict__ ctl) {
-  if (threadIdx.x != 0 || search_off(ctl)) return;
-  int any = 0;
-  for (int o = 0; o < m; ++o) any |= cfs[kFeasHeader + o];
+  if (search_off(ctl)) return;
+  // one wavefront: a lane per flag, then a ballot
+  int v = 0;
+  for (int o = threadIdx.x; o < m; o += 64) v |= cfs[kFeasHeader + o];
+  const bool any = __any(v != 0);
+  if (threadIdx.x != 0) return;
   ctl[CTL_PRUNE] = any ? 1 : 0;
   ctl[CTL_COARSE_DONE] = any ? 0 : 1;
   if (any) return;
