@@ -829,10 +829,11 @@ const int* ctl_feas_on(spk_ctx* c) {
 // then k_coarse_decide (prune orientations, or decide the item: see there).
 // ctl must not be null.
 // With clear_flags the column test's flags are cleared here (no k_feas_begin
-// before it); callers only pass it for max_exy <= kMaskCols.
+// before it); callers only pass it for max_exy <= kMaskCols.  Without decide
+// the flags are only left for the host (no k_coarse_decide).
 void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_exy,
                     const int* ctl, PackOut* out = nullptr, int t = 0, int halt_not_found = 0,
-                    bool clear_flags = false) {
+                    bool clear_flags = false, bool decide = true) {
   spk_ctx::Coarse& q = c->co;
   if (!q.ready) return;
   const int ccx = (max_ex + 1) / 2, ccy = q.cy;
@@ -869,7 +870,8 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
     case 16: test(k_column_test<16>); break;
     default: test(k_column_test<64>); break;
   }
-  launch(c, "coarse_decide", k_coarse_decide, dim3(1), dim3(64), 0, (const int*)q.fs.p, m,
+  if (decide)
+    launch(c, "coarse_decide", k_coarse_decide, dim3(1), dim3(64), 0, (const int*)q.fs.p, m,
          c->fstate.p, out, t, halt_not_found, (int*)ctl);
 }
 
@@ -1507,7 +1509,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
              (const FieldStats*)nullptr, (const FieldStats*)nullptr, (PackOut*)nullptr, 0);
     if (c->co.ready) {
       // the column test alone: its flags are exact (residual slot 0)
-      enqueue_coarse(c, od, k, max_ex, pl.max_exy, c->ctl_on.p, nullptr, 0, 0, fused);
+      enqueue_coarse(c, od, k, max_ex, pl.max_exy, c->ctl_on.p, nullptr, 0, 0, fused, false);
       const Mailbox& mb = post_and_sync(c, nullptr, nullptr, nullptr, nullptr, c->co.fs.p,
                                         kFeasHeader + k);
       std::memcpy(fsv, mb.fs, (kFeasHeader + k) * sizeof(int));
