@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -222,6 +223,9 @@ struct spk_ctx {
   Mailbox* mail_h = nullptr;                 // pinned host mailbox and its device address
   Mailbox* mail_d = nullptr;
   unsigned mail_seq = 0;                     // sequence number of the last post
+  int* stage = nullptr;                      // pinned staging buffer of the items
+  size_t stage_cap = 0;
+  hipEvent_t stage_ev = nullptr;             // recorded after its last copy
   // host-side copy of item buffer bookkeeping
   std::vector<int> h_items;
 
@@ -567,23 +571,32 @@ struct HostItem {
   long long sum;
 };
 
+// Bounds and sum of the voxels > 0 of an item.  Row by row: the minimum and
+// the sum of a row vectorise; only rows with a positive voxel look for their
+// first and last one.
 HostItem scan_item(const int* data, int ix, int iy, int iz) {
   if (!data && (long long)ix * iy * iz > 0) throw SpkError{SPK_EINVAL, "null item"};
   check_dims(ix, iy, iz);
   HostItem h{data, {ix, iy, iz}, {INT_MAX, INT_MAX, INT_MAX}, {-1, -1, -1}, 0};
   for (int i = 0; i < ix; ++i)
-    for (int j = 0; j < iy; ++j)
+    for (int j = 0; j < iy; ++j) {
+      const int* row = data + ((size_t)i * iy + j) * iz;
+      int mn = INT_MAX;
+      long long s = 0;
       for (int k = 0; k < iz; ++k) {
-        const int v = data[((size_t)i * iy + j) * iz + k];
-        if (v < 0)
-          throw SpkError{SPK_EINVAL, "negative item voxel values are not supported"};
-        if (v > 0) {
-          h.sum += v;
-          h.lo[0] = std::min(h.lo[0], i); h.hi[0] = std::max(h.hi[0], i);
-          h.lo[1] = std::min(h.lo[1], j); h.hi[1] = std::max(h.hi[1], j);
-          h.lo[2] = std::min(h.lo[2], k); h.hi[2] = std::max(h.hi[2], k);
-        }
+        mn = std::min(mn, row[k]);
+        s += row[k];
       }
+      if (mn < 0) throw SpkError{SPK_EINVAL, "negative item voxel values are not supported"};
+      if (s == 0) continue;
+      int k0 = 0, k1 = iz - 1;
+      while (row[k0] == 0) ++k0;
+      while (row[k1] == 0) --k1;
+      h.sum += s;
+      h.lo[0] = std::min(h.lo[0], i); h.hi[0] = std::max(h.hi[0], i);
+      h.lo[1] = std::min(h.lo[1], j); h.hi[1] = std::max(h.hi[1], j);
+      h.lo[2] = std::min(h.lo[2], k0); h.hi[2] = std::max(h.hi[2], k1);
+    }
   return h;
 }
 
@@ -1812,18 +1825,70 @@ void autotune_tile(spk_ctx* c) {
 }
 
 // Parses and uploads the concatenated items of a pack call.
+// Items of a pack call: host threads scan them (bounds, sums) and copy them
+// into the context's pinned staging buffer, which one asynchronous copy then
+// moves to the device (a copy from pageable memory is staged and synchronous).
 void load_items(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_items,
                 const int32_t* order, std::vector<HostItem>& bases, std::vector<int>& off) {
   if (!items || !dims || n_items <= 0) throw SpkError{SPK_EINVAL, "bad pack arguments"};
   size_t tot = 0;
+  off.resize(n_items);
   for (int t = 0; t < n_items; ++t) {
-    const int* p = items + tot;
-    bases.push_back(scan_item(p, dims[3 * t], dims[3 * t + 1], dims[3 * t + 2]));
-    off.push_back((int)tot);
+    check_dims(dims[3 * t], dims[3 * t + 1], dims[3 * t + 2]);
+    off[t] = (int)tot;
     tot += (size_t)dims[3 * t] * dims[3 * t + 1] * dims[3 * t + 2];
   }
+  bases.resize(n_items);
   c->items.ensure(std::max<size_t>(tot, 1));
-  upload(c, c->items.p, items, tot * sizeof(int));
+  if (c->stage_ev) HIP_OK(hipEventSynchronize(c->stage_ev));   // the previous copy is done
+  if (c->stage_cap < tot) {
+    if (c->stage) HIP_OK(hipHostFree(c->stage));
+    c->stage = nullptr;
+    c->stage_cap = 0;
+    HIP_OK(hipHostMalloc((void**)&c->stage, std::max<size_t>(tot, 1) * sizeof(int)));
+    c->stage_cap = tot;
+  }
+  // helper threads: at most 8 per call and 15 across concurrent calls (the
+  // multi-instance batch loads from one host thread per instance)
+  static std::atomic<int> g_helpers{0};
+  const int want = std::min({7, (int)std::thread::hardware_concurrency() - 1, (n_items - 1) / 16});
+  int got = 0;
+  while (got < want) {
+    int cur = g_helpers.load();
+    if (cur >= 15) break;
+    if (g_helpers.compare_exchange_weak(cur, cur + 1)) ++got;
+  }
+  const int nth = got + 1;
+  std::vector<SpkError> errs(nth, SpkError{SPK_OK, ""});
+  auto work = [&](int w) {
+    try {
+      for (int t = w; t < n_items; t += nth) {
+        const size_t n = (size_t)dims[3 * t] * dims[3 * t + 1] * dims[3 * t + 2];
+        bases[t] = scan_item(items + off[t], dims[3 * t], dims[3 * t + 1], dims[3 * t + 2]);
+        if (n) std::memcpy(c->stage + off[t], items + off[t], n * sizeof(int));
+      }
+    } catch (const SpkError& e) {
+      errs[w] = e;
+    }
+  };
+  std::vector<std::thread> th;
+  int w_next = 1;
+  try {
+    for (; w_next < nth; ++w_next) th.emplace_back(work, w_next);
+  } catch (const std::exception&) {
+    // no thread: the calling thread does the rest
+  }
+  work(0);
+  for (int w = w_next; w < nth; ++w) work(w);
+  for (auto& x : th) x.join();
+  g_helpers -= got;
+  for (const auto& e : errs)
+    if (e.code != SPK_OK) throw e;
+  if (tot)
+    HIP_OK(hipMemcpyAsync(c->items.p, c->stage, tot * sizeof(int), hipMemcpyHostToDevice,
+                          c->stream));
+  if (!c->stage_ev) HIP_OK(hipEventCreateWithFlags(&c->stage_ev, hipEventDisableTiming));
+  HIP_OK(hipEventRecord(c->stage_ev, c->stream));
   for (int t = 0; t < n_items; ++t) {
     const int it = order ? order[t] : t;
     if (it < 0 || it >= n_items) throw SpkError{SPK_EINVAL, "order index out of range"};
@@ -1916,6 +1981,9 @@ int spk_ctx_destroy(spk_ctx* c) {
     c->pairs.release();
     c->co.cols.release(); c->co.cm.release(); c->co.fs.release();
     if (c->mail_h) (void)hipHostFree(c->mail_h);
+    if (c->stage_ev) (void)hipEventSynchronize(c->stage_ev);
+    if (c->stage) (void)hipHostFree(c->stage);
+    if (c->stage_ev) (void)hipEventDestroy(c->stage_ev);
     if (c->stream) hipStreamDestroy(c->stream);
   }
   delete c;
