@@ -19,6 +19,7 @@
  *                         rotations.py:159-268 done as index maps on device
  *   spk_place             place_in_tray (packing.cpp:773-788)
  *   spk_pack              BinPacker.pack_voxels greedy loop (packer.py:447-529)
+ *   spk_pack_gather       the same, items as one pointer per item array
  *   spk_distance          calculate_distance (fft3.cu:944-1004)
  *   spk_correlate         dft_conv3 / dft_corr3 (fft3.cu:643-846,
  *                         packing.cpp:225-239)
@@ -133,7 +134,18 @@ int spk_pack(spk_ctx* ctx, const int32_t* items, const int32_t* dims, int n_item
              int penalty, int interlocking_free, int32_t id_offset,
              spk_result* results);
 
-/* Arguments of one spk_pack call, for spk_pack_many. */
+/* spk_pack with the items given as one pointer per item instead of one
+ * concatenated buffer: the bindings' list of item arrays (bindings.cpp
+ * pack_voxels' vector of grids, packer.py:447-529) goes in without a copy.
+ * items[t] holds dims[3t] * dims[3t+1] * dims[3t+2] values. */
+int spk_pack_gather(spk_ctx* ctx, const int32_t* const* items, const int32_t* dims,
+                    int n_items, const int32_t* order, const spk_orient* orients,
+                    int n_orient, int penalty, int interlocking_free, int32_t id_offset,
+                    spk_result* results);
+
+/* Arguments of one spk_pack call, for spk_pack_many (and the learned and
+ * records loops).  The items are either concatenated in items, or, when
+ * item_ptrs is not null, one pointer per item there (items is then unused). */
 typedef struct {
   const int32_t* items;
   const int32_t* dims;
@@ -145,6 +157,7 @@ typedef struct {
   int interlocking_free;
   int32_t id_offset;
   spk_result* results;
+  const int32_t* const* item_ptrs;
 } spk_pack_args;
 
 /* Many independent packing problems at once (no reference counterpart: the

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libspk.so")
 EXPORTS = (
     "spk_last_error", "spk_version", "spk_ctx_create", "spk_ctx_destroy", "spk_tray_set",
     "spk_tray_get", "spk_tray_info", "spk_tray_clear", "spk_search", "spk_search_oriented",
-    "spk_place", "spk_pack", "spk_pack_many", "spk_distance", "spk_correlate", "spk_collision_grid",
+    "spk_place", "spk_pack", "spk_pack_gather", "spk_pack_many", "spk_distance", "spk_correlate", "spk_collision_grid",
     "spk_interlock_mask", "spk_place_grid", "spk_bounds", "spk_save_vox", "spk_timing_enable",
     "spk_timing_get", "spk_timing_units", "spk_timing_reset", "spk_set_option", "spk_plan_info",
     "spk_tile_info",
@@ -43,7 +43,8 @@ class SpkPackArgs(C.Structure):
                 ("n_items", C.c_int), ("order", C.POINTER(C.c_int32)),
                 ("orients", C.POINTER(SpkOrient)), ("n_orient", C.c_int), ("penalty", C.c_int),
                 ("interlocking_free", C.c_int), ("id_offset", C.c_int32),
-                ("results", C.POINTER(SpkResult))]
+                ("results", C.POINTER(SpkResult)),
+                ("item_ptrs", C.POINTER(C.POINTER(C.c_int32)))]
 
 
 _lib = None

@@ -413,18 +413,12 @@ class Session {
                 const std::string& penalty, bool interlocking_free, int id_offset) {
     std::vector<Grid> gs = grids_of(items);
     std::vector<int32_t> dims;
-    size_t tot = 0;
+    std::vector<const int32_t*> ptrs;   // the arrays themselves: no concatenated copy
     for (auto& g : gs) {
       dims.push_back(g.nx);
       dims.push_back(g.ny);
       dims.push_back(g.nz);
-      tot += g.size();
-    }
-    std::vector<int32_t> flat(tot);
-    size_t off = 0;
-    for (auto& g : gs) {
-      std::memcpy(flat.data() + off, g.ptr(), g.size() * sizeof(int32_t));
-      off += g.size();
+      ptrs.push_back(g.ptr());
     }
     auto ord = py::array_t<int32_t, py::array::c_style | py::array::forcecast>::ensure(order);
     if (!ord || ord.ndim() != 1 || (size_t)ord.shape(0) != gs.size())
@@ -433,7 +427,7 @@ class Session {
     std::vector<spk_result> res(gs.size());
     {
       py::gil_scoped_release nogil;
-      check(spk_pack(ctx_, flat.data(), dims.data(), (int)gs.size(), ord.data(), os.data(),
+      check(spk_pack_gather(ctx_, ptrs.data(), dims.data(), (int)gs.size(), ord.data(), os.data(),
                      (int)os.size(), penalty_of(penalty), interlocking_free ? 1 : 0, id_offset,
                      res.data()));
     }

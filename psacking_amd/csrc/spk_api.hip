@@ -1825,12 +1825,14 @@ void autotune_tile(spk_ctx* c) {
 }
 
 // Parses and uploads the concatenated items of a pack call.
-// Items of a pack call: host threads scan them (bounds, sums) and copy them
+// Items of a pack call, concatenated in items or one pointer per item in
+// ptrs: host threads scan them (bounds, sums) and copy them
 // into the context's pinned staging buffer, which one asynchronous copy then
 // moves to the device (a copy from pageable memory is staged and synchronous).
-void load_items(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_items,
-                const int32_t* order, std::vector<HostItem>& bases, std::vector<int>& off) {
-  if (!items || !dims || n_items <= 0) throw SpkError{SPK_EINVAL, "bad pack arguments"};
+void load_items(spk_ctx* c, const int32_t* items, const int32_t* const* ptrs,
+                const int32_t* dims, int n_items, const int32_t* order,
+                std::vector<HostItem>& bases, std::vector<int>& off) {
+  if ((!items && !ptrs) || !dims || n_items <= 0) throw SpkError{SPK_EINVAL, "bad pack arguments"};
   size_t tot = 0;
   off.resize(n_items);
   for (int t = 0; t < n_items; ++t) {
@@ -1864,8 +1866,9 @@ void load_items(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_ite
     try {
       for (int t = w; t < n_items; t += nth) {
         const size_t n = (size_t)dims[3 * t] * dims[3 * t + 1] * dims[3 * t + 2];
-        bases[t] = scan_item(items + off[t], dims[3 * t], dims[3 * t + 1], dims[3 * t + 2]);
-        if (n) std::memcpy(c->stage + off[t], items + off[t], n * sizeof(int));
+        const int32_t* src = ptrs ? ptrs[t] : items + off[t];
+        bases[t] = scan_item(src, dims[3 * t], dims[3 * t + 1], dims[3 * t + 2]);
+        if (n) std::memcpy(c->stage + off[t], src, n * sizeof(int));
       }
     } catch (const SpkError& e) {
       errs[w] = e;
@@ -2160,15 +2163,16 @@ int spk_place(spk_ctx* c, const int32_t* item, int ix, int iy, int iz, const spk
   });
 }
 
-int spk_pack(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_items,
-             const int32_t* order, const spk_orient* orients, int n_orient, int penalty,
-             int interlocking_free, int32_t id_offset, spk_result* results) {
+namespace {
+int pack_impl(spk_ctx* c, const int32_t* items, const int32_t* const* ptrs, const int32_t* dims,
+              int n_items, const int32_t* order, const spk_orient* orients, int n_orient,
+              int penalty, int interlocking_free, int32_t id_offset, spk_result* results) {
   return guarded(c, [&]() {
     if (!c->valid) throw SpkError{SPK_ESTATE, "no tray set in this context"};
     if (!results || n_orient <= 0) throw SpkError{SPK_EINVAL, "bad pack arguments"};
     std::vector<HostItem> bases;
     std::vector<int> off;
-    load_items(c, items, dims, n_items, order, bases, off);
+    load_items(c, items, ptrs, dims, n_items, order, bases, off);
     int placed = 0;
     if (interlocking_free) {
       // the flood fill iterates to its fixpoint under host control: one
@@ -2197,6 +2201,22 @@ int spk_pack(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_items,
     return SPK_OK;
   });
 }
+}  // namespace
+
+int spk_pack(spk_ctx* c, const int32_t* items, const int32_t* dims, int n_items,
+             const int32_t* order, const spk_orient* orients, int n_orient, int penalty,
+             int interlocking_free, int32_t id_offset, spk_result* results) {
+  return pack_impl(c, items, nullptr, dims, n_items, order, orients, n_orient, penalty,
+                   interlocking_free, id_offset, results);
+}
+
+int spk_pack_gather(spk_ctx* c, const int32_t* const* items, const int32_t* dims, int n_items,
+                    const int32_t* order, const spk_orient* orients, int n_orient, int penalty,
+                    int interlocking_free, int32_t id_offset, spk_result* results) {
+  if (!items) return fail(SPK_EINVAL, "bad pack arguments");
+  return pack_impl(c, nullptr, items, dims, n_items, order, orients, n_orient, penalty,
+                   interlocking_free, id_offset, results);
+}
 
 int spk_pack_many(spk_ctx* const* ctxs, int n, const spk_pack_args* args) {
   if (!ctxs || !args || n <= 0) return fail(SPK_EINVAL, "bad pack_many arguments");
@@ -2212,7 +2232,8 @@ int spk_pack_many(spk_ctx* const* ctxs, int n, const spk_pack_args* args) {
   for (int i = 0; i < n; ++i) {
     th.emplace_back([&, i]() {
       const spk_pack_args& a = args[i];
-      rc[i] = spk_pack(ctxs[i], a.items, a.dims, a.n_items, a.order, a.orients, a.n_orient,
+      rc[i] = pack_impl(ctxs[i], a.items, a.item_ptrs, a.dims, a.n_items, a.order, a.orients,
+                        a.n_orient,
                        a.penalty, a.interlocking_free, a.id_offset, a.results);
       if (rc[i] != SPK_OK) msg[i] = spk_last_error();
     });
@@ -2869,7 +2890,7 @@ int spk_pack_records(spk_ctx* c, const spk_pack_args* a, uint16_t* tray_feat, do
       throw SpkError{SPK_EINVAL, "training records come from the plain (not interlocking-free) loop"};
     std::vector<HostItem> bases;
     std::vector<int> off;
-    load_items(c, a->items, a->dims, a->n_items, a->order, bases, off);
+    load_items(c, a->items, a->item_ptrs, a->dims, a->n_items, a->order, bases, off);
     const int placed = pack_pipelined(c, bases, off, a->order, a->orients, a->n_orient, a->penalty,
                                       a->id_offset, a->results, scores, tray_feat);
     sync(c);
@@ -2893,7 +2914,7 @@ int spk_pack_learned(spk_ctx* c, spk_nn* nn, const spk_pack_args* a, int topk, f
       throw SpkError{SPK_EINVAL, "learned proposals run the plain (not interlocking-free) loop"};
     std::vector<HostItem> bases;
     std::vector<int> off;
-    load_items(c, a->items, a->dims, a->n_items, a->order, bases, off);
+    load_items(c, a->items, a->item_ptrs, a->dims, a->n_items, a->order, bases, off);
     const int placed = pack_learned(c, nn, bases, off, a->order, a->orients, a->n_orient,
                                     a->penalty, a->id_offset, a->results, topk, min_conf,
                                     proposals, counts);

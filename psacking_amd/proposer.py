@@ -253,7 +253,10 @@ class _Args:
         for g in grids:
             if g.ndim != 3:
                 raise ValueError("items must be 3-D")
-        self.flat = np.concatenate([g.reshape(-1) for g in grids]) if grids else np.zeros(0, np.int32)
+        # one pointer per item array (spk_pack_args.item_ptrs): no concatenated copy
+        self.grids = grids
+        self.ptrs = (C.POINTER(C.c_int32) * max(1, len(grids)))(
+            *[g.ctypes.data_as(C.POINTER(C.c_int32)) for g in grids])
         self.dims = np.array([g.shape for g in grids], np.int32).reshape(-1)
         self.order = np.ascontiguousarray(order, dtype=np.int32)
         P = np.asarray(perms, np.int32).reshape(-1, 3)
@@ -265,7 +268,8 @@ class _Args:
         self.n = len(grids)
         self.results = (_lib.SpkResult * max(1, self.n))()
         a = _lib.SpkPackArgs()
-        a.items = self.flat.ctypes.data_as(C.POINTER(C.c_int32))
+        a.items = None
+        a.item_ptrs = self.ptrs
         a.dims = self.dims.ctypes.data_as(C.POINTER(C.c_int32))
         a.n_items = self.n
         a.order = self.order.ctypes.data_as(C.POINTER(C.c_int32))
