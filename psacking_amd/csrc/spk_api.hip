@@ -219,7 +219,8 @@ struct spk_ctx {
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, long long>> acc;
   std::map<std::string, long long> units;   // orientations (pairs) handed to each kernel
-  DevBuf<unsigned long long> pairs;          // feasibility pairs transformed: full, coarse
+  DevBuf<unsigned long long> pairs;          // feasibility pairs transformed (full, coarse),
+                                             // orientations the score pass searched
   Mailbox* mail_h = nullptr;                 // pinned host mailbox and its device address
   Mailbox* mail_d = nullptr;
   unsigned mail_seq = 0;                     // sequence number of the last post
@@ -930,7 +931,8 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
     launch(c, "score_yz", c->score_rmax == 25 ? k_score_yz<25> : k_score_yz<10>, dim3(c->nx, m),
            dim3(kSlabThreads), score_shmem(c), (const float2*)c->y1.p, (const float2*)c->y2.p, od,
            (const FieldStats*)c->stats.p, pen, c->rec.p, c->nx, c->ny, c->nz, c->sy, c->sz,
-           tw + c->px.n, tw + c->px.n + py, fgrid, pgrid, cf, ctl);
+           tw + c->px.n, tw + c->px.n + py, fgrid, pgrid, cf,
+           c->timing && c->pairs.p ? c->pairs.p + 2 : (unsigned long long*)nullptr, ctl);
   }
   launch(c, "select", k_select, dim3(1), dim3(kSelectThreads), 0, (const WgBest*)c->rec.p, od, m,
          rs, c->split ? c->Rs : 0, c->per.p, c->sout.p, c->ver.p, ctl);
@@ -2480,8 +2482,8 @@ int spk_save_vox(const int32_t* grid, int nx, int ny, int nz, const char* path) 
 int spk_timing_enable(spk_ctx* c, int on) {
   return guarded(c, [&]() {
     if (on && !c->pairs.p) {
-      c->pairs.ensure(2);
-      HIP_OK(hipMemsetAsync(c->pairs.p, 0, 2 * sizeof(unsigned long long), c->stream));
+      c->pairs.ensure(3);
+      HIP_OK(hipMemsetAsync(c->pairs.p, 0, 3 * sizeof(unsigned long long), c->stream));
     }
     c->timing = on != 0;
     return SPK_OK;
@@ -2502,14 +2504,15 @@ int spk_timing_units(spk_ctx* c, const char* kernel, int64_t* units) {
   return guarded(c, [&]() {
     const std::string k = kernel ? kernel : "";
     long long v = 0;
-    if (k == "feas_x" || k == "feas_yz") {
-      // counted on the device: pairs the column test excluded never run
-      unsigned long long h[2] = {0, 0};
+    if (k == "feas_x" || k == "feas_yz" || ((k == "score_yz" || k == "corr_x") && c->pairs.p)) {
+      // counted on the device: pairs and orientations the column test
+      // excluded never run
+      unsigned long long h[3] = {0, 0, 0};
       if (c->pairs.p) {
         download(c, h, c->pairs.p, sizeof(h));
         sync(c);
       }
-      v = (long long)h[0];
+      v = (long long)(k == "score_yz" || k == "corr_x" ? h[2] : h[0]);
     } else {
       auto it = c->units.find(k);
       v = it == c->units.end() ? 0 : it->second;
@@ -2524,7 +2527,7 @@ int spk_timing_reset(spk_ctx* c) {
     c->flush_timing();
     c->acc.clear();
     c->units.clear();
-    if (c->pairs.p) HIP_OK(hipMemsetAsync(c->pairs.p, 0, 2 * sizeof(unsigned long long), c->stream));
+    if (c->pairs.p) HIP_OK(hipMemsetAsync(c->pairs.p, 0, 3 * sizeof(unsigned long long), c->stream));
     return SPK_OK;
   });
 }

@@ -839,7 +839,7 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
            AxisPlan py_, AxisPlan pz_, const float2* __restrict__ tw_y,
            const float2* __restrict__ tw_z, unsigned char* __restrict__ fgrid,
            long long* __restrict__ pgrid, const int* __restrict__ cf,
-           const int* __restrict__ ctl) {
+           unsigned long long* __restrict__ searched, const int* __restrict__ ctl) {
   if (search_off(ctl)) return;
   extern __shared__ float2 smem[];
   const int py = py_.n, pz = pz_.n, hz = pz / 2 + 1;
@@ -855,6 +855,7 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
     }
     return;
   }
+  if (searched && x == 0 && tid == 0) atomicAdd(searched, 1ULL);   // an orientation searched
   float2* twy = smem;
   float2* twz = twy + py;
   float2* buf = twz + pz;
