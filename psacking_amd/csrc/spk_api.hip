@@ -575,18 +575,30 @@ struct HostItem {
 // Bounds and sum of the voxels > 0 of an item.  Row by row: the minimum and
 // the sum of a row vectorise; only rows with a positive voxel look for their
 // first and last one.
-HostItem scan_item(const int* data, int ix, int iy, int iz) {
+// copy (not null): the item is also copied there, in the same pass.
+HostItem scan_item(const int* data, int ix, int iy, int iz, int* copy = nullptr) {
   if (!data && (long long)ix * iy * iz > 0) throw SpkError{SPK_EINVAL, "null item"};
   check_dims(ix, iy, iz);
   HostItem h{data, {ix, iy, iz}, {INT_MAX, INT_MAX, INT_MAX}, {-1, -1, -1}, 0};
   for (int i = 0; i < ix; ++i)
     for (int j = 0; j < iy; ++j) {
-      const int* row = data + ((size_t)i * iy + j) * iz;
+      const size_t r0 = ((size_t)i * iy + j) * iz;
+      const int* row = data + r0;
       int mn = INT_MAX;
       long long s = 0;
-      for (int k = 0; k < iz; ++k) {
-        mn = std::min(mn, row[k]);
-        s += row[k];
+      if (copy) {
+        int* dst = copy + r0;
+        for (int k = 0; k < iz; ++k) {
+          const int v = row[k];
+          dst[k] = v;
+          mn = std::min(mn, v);
+          s += v;
+        }
+      } else {
+        for (int k = 0; k < iz; ++k) {
+          mn = std::min(mn, row[k]);
+          s += row[k];
+        }
       }
       if (mn < 0) throw SpkError{SPK_EINVAL, "negative item voxel values are not supported"};
       if (s == 0) continue;
@@ -1852,10 +1864,10 @@ void load_items(spk_ctx* c, const int32_t* items, const int32_t* const* ptrs,
     HIP_OK(hipHostMalloc((void**)&c->stage, std::max<size_t>(tot, 1) * sizeof(int)));
     c->stage_cap = tot;
   }
-  // helper threads: at most 8 per call and 15 across concurrent calls (the
+  // helper threads: at most 15 per call and 15 across concurrent calls (the
   // multi-instance batch loads from one host thread per instance)
   static std::atomic<int> g_helpers{0};
-  const int want = std::min({7, (int)std::thread::hardware_concurrency() - 1, (n_items - 1) / 16});
+  const int want = std::min({15, (int)std::thread::hardware_concurrency() - 1, (n_items - 1) / 8});
   int got = 0;
   while (got < want) {
     int cur = g_helpers.load();
@@ -1869,8 +1881,8 @@ void load_items(spk_ctx* c, const int32_t* items, const int32_t* const* ptrs,
       for (int t = w; t < n_items; t += nth) {
         const size_t n = (size_t)dims[3 * t] * dims[3 * t + 1] * dims[3 * t + 2];
         const int32_t* src = ptrs ? ptrs[t] : items + off[t];
-        bases[t] = scan_item(src, dims[3 * t], dims[3 * t + 1], dims[3 * t + 2]);
-        if (n) std::memcpy(c->stage + off[t], src, n * sizeof(int));
+        bases[t] = scan_item(src, dims[3 * t], dims[3 * t + 1], dims[3 * t + 2],
+                             n ? c->stage + off[t] : nullptr);
       }
     } catch (const SpkError& e) {
       errs[w] = e;
