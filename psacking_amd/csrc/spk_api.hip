@@ -213,6 +213,7 @@ struct spk_ctx {
   bool certificate = false;              // option "certificate"
   DevBuf<FieldStats> theta;              // per item: inscribed-ball bound (mx)
   DevBuf<int> cgrid, cdist, ctmp;        // padded item complement and its distance field
+  DevBuf<int> gates;                     // control words gating speculative screens
   // timing
   bool timing = false;
   std::vector<Timed> pending;
@@ -346,36 +347,50 @@ size_t pencil_shmem(const spk_ctx* c) {
 }
 
 // ---- host mailbox ------------------------------------------------------------
-// Gathers the listed device words into the pinned mailbox and waits for the
-// stream; the host then reads c->mail_h.
-const Mailbox& post_and_sync(spk_ctx* c, const int* ctl, const PackOut* po, const FieldStats* st,
-                             const float* logits, const int* fs, int nfs) {
+// kMailSlots pinned mailboxes.  post_async gathers the listed device words
+// into one of them (k_post, stream-ordered) and returns the post's sequence
+// number; wait_post waits until that slot holds that post.  Two slots let the
+// learned loop keep one post in flight while it reads the other.
+constexpr int kMailSlots = 2;
+unsigned post_async(spk_ctx* c, int slot, const int* ctl, const PackOut* po, const FieldStats* st,
+                    const float* logits, const int* fs, int nfs) {
   if (!c->mail_h) {
-    HIP_OK(hipHostMalloc((void**)&c->mail_h, sizeof(Mailbox),
+    HIP_OK(hipHostMalloc((void**)&c->mail_h, kMailSlots * sizeof(Mailbox),
                          hipHostMallocMapped | hipHostMallocCoherent));
     HIP_OK(hipHostGetDevicePointer((void**)&c->mail_d, c->mail_h, 0));
-    c->mail_h->seq = 0;
+    for (int i = 0; i < kMailSlots; ++i) c->mail_h[i].seq = 0;
     c->mail_seq = 0;
   }
   const unsigned seq = ++c->mail_seq;
-  launch(c, "post", k_post, dim3(1), dim3(64), 0, ctl, po, st, logits, fs, nfs, seq, c->mail_d);
-  // The host spins on the sequence word (a stream synchronisation wakes the
-  // host tens of microseconds later); after a bounded spin it falls back to
-  // the synchronisation, which also reports a failed kernel.
+  launch(c, "post", k_post, dim3(1), dim3(64), 0, ctl, po, st, logits, fs, nfs, seq,
+         c->mail_d + slot);
+  return seq;
+}
+
+// The host spins on the sequence word (a stream synchronisation wakes the
+// host tens of microseconds later); after a bounded spin it falls back to the
+// synchronisation, which also reports a failed kernel.
+const Mailbox& wait_post(spk_ctx* c, int slot, unsigned seq) {
+  const Mailbox* m = c->mail_h + slot;
   const auto t0 = std::chrono::steady_clock::now();
   bool seen = false;
   for (int it = 0; !seen; ++it) {
-    seen = __atomic_load_n(&c->mail_h->seq, __ATOMIC_ACQUIRE) == seq;
+    seen = __atomic_load_n(&m->seq, __ATOMIC_ACQUIRE) == seq;
     if (!seen && (it & 4095) == 4095 &&
         std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(100))
       break;
   }
   if (!seen) {
     sync(c);
-    if (__atomic_load_n(&c->mail_h->seq, __ATOMIC_ACQUIRE) != seq)
+    if (__atomic_load_n(&m->seq, __ATOMIC_ACQUIRE) != seq)
       throw SpkError{SPK_EDEVICE, "host mailbox not written"};
   }
-  return *c->mail_h;
+  return *m;
+}
+
+const Mailbox& post_and_sync(spk_ctx* c, const int* ctl, const PackOut* po, const FieldStats* st,
+                             const float* logits, const int* fs, int nfs) {
+  return wait_post(c, 0, post_async(c, 0, ctl, po, st, logits, fs, nfs));
 }
 
 // ---- column test: 2x2 column blocks of the tray ----------------------------
@@ -1557,6 +1572,32 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     list.swap(keep);
     return true;
   };
+  // Pipelined screens (column test, every item screened, no certificate):
+  // behind an item's screen the next screened item's screen is enqueued
+  // before the host has read the first one's flags.  k_spec_gate lets it run
+  // only when the first item fits nowhere, i.e. when the tray stays as it is;
+  // otherwise it exits at once and its post is ignored.  The two screens post
+  // into different mailbox slots.
+  const bool pipe = c->co.ready && c->screen_all && feas_plan && !bounds;
+  c->gates.ensure(kMailSlots * CTL_WORDS);
+  auto enqueue_screen = [&](const ItemPlan& pl, const int* gate, int slot) -> unsigned {
+    const OrientDesc* od = c->od_all.p + pl.od_off;   // every fitting orientation
+    const bool fused = pl.max_exy <= kMaskCols;
+    if (!fused)
+      launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, c->co.fs.p, pl.m, 2,
+             (int*)nullptr, (const FieldStats*)nullptr, (const FieldStats*)nullptr,
+             (PackOut*)nullptr, 0);
+    enqueue_coarse(c, od, pl.m, pl.max_ex, pl.max_exy, gate, nullptr, 0, 0, fused, false);
+    const unsigned seq = post_async(c, slot, nullptr, nullptr, nullptr, nullptr, c->co.fs.p,
+                                    kFeasHeader + pl.m);
+    launch(c, "spec_gate", k_spec_gate, dim3(1), dim3(64), 0, (const int*)c->co.fs.p, pl.m,
+           c->gates.p + slot * CTL_WORDS);
+    return seq;
+  };
+  struct Spec {
+    int t = -1, slot = 0;
+    unsigned seq = 0;
+  } spec;
   int placed = 0;
   auto reset_ctl = [&]() {
     int v[CTL_WORDS] = {0};
@@ -1594,7 +1635,39 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     for (int i = 0; i < pl.m; ++i) cand[i] = i;
     bool screened = false;
     if (feas_plan && (!prev_placed || (c->screen_all && c->co.ready))) {
-      screened = screen(pl, cand);
+      if (pipe) {
+        int slot = 0;
+        unsigned seq = 0;
+        if (spec.t == t) {   // enqueued behind the previous screen, which found nothing
+          slot = spec.slot;
+          seq = spec.seq;
+        } else {
+          seq = enqueue_screen(pl, c->ctl_on.p, slot);
+        }
+        spec.t = -1;
+        int tn = t + 1;
+        while (tn < n_items && plans[tn].m == 0) ++tn;
+        if (tn < n_items) {
+          spec.t = tn;
+          spec.slot = slot ^ 1;
+          spec.seq = enqueue_screen(plans[tn], c->gates.p + slot * CTL_WORDS, spec.slot);
+        }
+        const Mailbox& mb = wait_post(c, slot, seq);
+        std::memcpy(fsv, mb.fs, (kFeasHeader + pl.m) * sizeof(int));
+        fsv[0] = 0;
+        // as screen(): a residual (0 for the exact column test) above the
+        // limit leaves the candidates as they are
+        screened = 0.f <= c->resid_limit;
+        if (screened) {
+          std::vector<int> keep;
+          for (int j = 0; j < pl.m; ++j)
+            if (fsv[kFeasHeader + j]) keep.push_back(j);
+          cand.swap(keep);
+        }
+        if (!cand.empty()) spec.t = -1;   // the item is searched: the next screen is redone
+      } else {
+        screened = screen(pl, cand);
+      }
       if (screened && cand.empty()) {
         ++cnt[4];
         float res;

@@ -1884,6 +1884,18 @@ __global__ void k_coarse_decide(const int* __restrict__ cfs, int m, int* __restr
   ctl[CTL_SKIP_REFRESH] = 1;
 }
 
+// Gate of a speculative screen: gate[CTL_SKIP_SEARCH] = 1 when the screen
+// before it (flags cfs of m orientations) found a feasible orientation.  The
+// next item's screen, enqueued before the host read those flags, then exits
+// at once; it runs only when this item fits nowhere, i.e. when the tray stays
+// as it is.  One wavefront.
+__global__ void k_spec_gate(const int* __restrict__ cfs, int m, int* __restrict__ gate) {
+  int v = 0;
+  for (int o = threadIdx.x; o < m; o += 64) v |= cfs[kFeasHeader + o];
+  const bool any = __any(v != 0);
+  if (threadIdx.x < CTL_WORDS) gate[threadIdx.x] = threadIdx.x == CTL_SKIP_SEARCH && any ? 1 : 0;
+}
+
 // greedy loop: with the pass run and its residual within resid_limit, an
 // item with no feasible orientation is decided here (not found) and its
 // scored search, placement and refresh are skipped
