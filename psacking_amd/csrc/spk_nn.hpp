@@ -521,15 +521,24 @@ __global__ void k_item_feat(const int* __restrict__ item, const ItemMap* __restr
   out[(size_t)blockIdx.y * (kItemCells * kItemCells * kItemCells) + i] = f2bf((float)cnt * 0.125f);
 }
 
-// column heights: h[x][y] = 1 + max{z : tray > 0}, 0 for an empty column
+// column heights: h[x][y] = 1 + max{z : tray > 0}, 0 for an empty column.
+// One wavefront per column: 64 heights per ballot, from the top down.
 __global__ void k_heightmap(const int* __restrict__ tray, int nx, int ny, int nz,
                             int* __restrict__ h) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (i >= nx * ny) return;
   const int* col = tray + (size_t)i * nz;
-  int z = nz - 1;
-  while (z >= 0 && col[z] <= 0) --z;
-  h[i] = z + 1;
+  int top = 0;
+  for (int z0 = ((nz - 1) / 64) * 64; z0 >= 0; z0 -= 64) {
+    const int z = z0 + lane;
+    const unsigned long long b = __ballot(z < nz && col[z] > 0);
+    if (b) {
+      top = z0 + 64 - __clzll((long long)b);
+      break;
+    }
+  }
+  if (lane == 0) h[i] = top;
 }
 
 __device__ __forceinline__ void cell_range(int a, int n, int G, int& lo, int& hi) {

@@ -344,8 +344,8 @@ void nn_item_features(hipStream_t s, const int* item, const spknn::ItemMap* maps
 
 void nn_tray_features(hipStream_t s, const int* tray, int nx, int ny, int nz, int* heights,
                       bf16_t* out) {
-  nn_launch(s, spknn::k_heightmap, dim3((nx * ny + 255) / 256), dim3(256), 0, tray, nx, ny, nz,
-            heights);
+  nn_launch(s, spknn::k_heightmap, dim3((nx * ny + 3) / 4), dim3(256), 0, tray, nx, ny, nz,
+            heights);   // a wavefront per column
   nn_launch(s, spknn::k_tray_feat, dim3(kNnTrayFeat / 2 / 256), dim3(256), 0, tray,
             (const int*)heights, nx, ny, nz, out);
 }
