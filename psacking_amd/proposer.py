@@ -255,8 +255,7 @@ class _Args:
                 raise ValueError("items must be 3-D")
         # one pointer per item array (spk_pack_args.item_ptrs): no concatenated copy
         self.grids = grids
-        self.ptrs = (C.POINTER(C.c_int32) * max(1, len(grids)))(
-            *[g.ctypes.data_as(C.POINTER(C.c_int32)) for g in grids])
+        self.ptrs = np.array([g.__array_interface__["data"][0] for g in grids] or [0], np.uintp)
         self.dims = np.array([g.shape for g in grids], np.int32).reshape(-1)
         self.order = np.ascontiguousarray(order, dtype=np.int32)
         P = np.asarray(perms, np.int32).reshape(-1, 3)
@@ -269,7 +268,7 @@ class _Args:
         self.results = (_lib.SpkResult * max(1, self.n))()
         a = _lib.SpkPackArgs()
         a.items = None
-        a.item_ptrs = self.ptrs
+        a.item_ptrs = self.ptrs.ctypes.data_as(C.POINTER(C.POINTER(C.c_int32)))
         a.dims = self.dims.ctypes.data_as(C.POINTER(C.c_int32))
         a.n_items = self.n
         a.order = self.order.ctypes.data_as(C.POINTER(C.c_int32))
