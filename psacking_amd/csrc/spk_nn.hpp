@@ -152,6 +152,20 @@ k_igemm(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wm, const float
   for (int k0 = k_begin; k0 < k_end; k0 += 32) {
     const int k = k0 + lk;
     bf16x8 a[2], b[NT];
+    // element-wise im2col (cin % 8 != 0): the taps of the lane's 8 columns do
+    // not depend on the row, so they are decomposed once for both rows
+    int tp0[8], tp1[8], tp2[8], tch[8];
+    if (MODE == MODE_FWD && !VEC) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int kk = k + j;
+        const int tap = kk / d.cin, rest = tap % (d.ks * d.ks);
+        tch[j] = kk - tap * d.cin;
+        tp0[j] = kk < d.K ? tap / (d.ks * d.ks) : -(1 << 20);   // out of range: never in bounds
+        tp1[j] = rest / d.ks;
+        tp2[j] = rest - tp1[j] * d.ks;
+      }
+    }
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       uint4 v = make_uint4(0, 0, 0, 0);
@@ -163,7 +177,14 @@ k_igemm(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wm, const float
         } else {
           bf16_t e[8];
 #pragma unroll
-          for (int j = 0; j < 8; ++j) e[j] = col_fwd(X, d, rb[r], r0[r], r1[r], r2[r], k + j);
+          for (int j = 0; j < 8; ++j) {
+            const int x0 = r0[r] * d.st - d.pad + tp0[j], x1 = r1[r] * d.st - d.pad + tp1[j];
+            const int x2 = r2[r] * d.st - d.pad + tp2[j];
+            e[j] = (unsigned)x0 < (unsigned)d.i0 && (unsigned)x1 < (unsigned)d.i1 &&
+                           (unsigned)x2 < (unsigned)d.i2
+                       ? X[((((size_t)rb[r] * d.i0 + x0) * d.i1 + x1) * d.i2 + x2) * d.cin + tch[j]]
+                       : (bf16_t)0;
+          }
           v.x = e[0] | ((unsigned)e[1] << 16);
           v.y = e[2] | ((unsigned)e[3] << 16);
           v.z = e[4] | ((unsigned)e[5] << 16);
