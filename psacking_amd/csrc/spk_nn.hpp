@@ -154,16 +154,18 @@ k_igemm(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wm, const float
     bf16x8 a[2], b[NT];
     // element-wise im2col (cin % 8 != 0): the taps of the lane's 8 columns do
     // not depend on the row, so they are decomposed once for both rows
-    int tp0[8], tp1[8], tp2[8], tch[8];
+    int tp0[8], tp1[8], tp2[8], toff[8];
     if (MODE == MODE_FWD && !VEC) {
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const int kk = k + j;
         const int tap = kk / d.cin, rest = tap % (d.ks * d.ks);
-        tch[j] = kk - tap * d.cin;
         tp0[j] = kk < d.K ? tap / (d.ks * d.ks) : -(1 << 20);   // out of range: never in bounds
         tp1[j] = rest / d.ks;
         tp2[j] = rest - tp1[j] * d.ks;
+        // element offset from the row's corner (tap (0,0,0), channel 0)
+        toff[j] = kk < d.K ? ((tp0[j] * d.i1 + tp1[j]) * d.i2 + tp2[j]) * d.cin + (kk - tap * d.cin)
+                           : 0;
       }
     }
 #pragma unroll
@@ -176,14 +178,15 @@ k_igemm(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wm, const float
           v = col_fwd8(X, d, rb[r], r0[r], r1[r], r2[r], k);
         } else {
           bf16_t e[8];
+          const int c0 = r0[r] * d.st - d.pad, c1 = r1[r] * d.st - d.pad, c2 = r2[r] * d.st - d.pad;
+          const long long corner =
+              ((((long long)rb[r] * d.i0 + c0) * d.i1 + c1) * d.i2 + c2) * d.cin;
 #pragma unroll
           for (int j = 0; j < 8; ++j) {
-            const int x0 = r0[r] * d.st - d.pad + tp0[j], x1 = r1[r] * d.st - d.pad + tp1[j];
-            const int x2 = r2[r] * d.st - d.pad + tp2[j];
-            e[j] = (unsigned)x0 < (unsigned)d.i0 && (unsigned)x1 < (unsigned)d.i1 &&
-                           (unsigned)x2 < (unsigned)d.i2
-                       ? X[((((size_t)rb[r] * d.i0 + x0) * d.i1 + x1) * d.i2 + x2) * d.cin + tch[j]]
-                       : (bf16_t)0;
+            const bool in = (unsigned)(c0 + tp0[j]) < (unsigned)d.i0 &&
+                            (unsigned)(c1 + tp1[j]) < (unsigned)d.i1 &&
+                            (unsigned)(c2 + tp2[j]) < (unsigned)d.i2;
+            e[j] = in ? X[corner + toff[j]] : (bf16_t)0;
           }
           v.x = e[0] | ((unsigned)e[1] << 16);
           v.y = e[2] | ((unsigned)e[3] << 16);
