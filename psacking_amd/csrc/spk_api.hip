@@ -1724,8 +1724,16 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     int ctl[CTL_WORDS];
     PackOut po;
     {
-      const Mailbox& mb = post_and_sync(c, c->ctl.p, c->pout.p + t, c->stats.p,
-                                        next_head ? nn->logits.p : nullptr, nullptr, 0);
+      const unsigned sq = post_async(c, 0, c->ctl.p, c->pout.p + t, c->stats.p,
+                                     next_head ? nn->logits.p : nullptr, nullptr, 0);
+      // the next screened item's screen, behind this step and its post: valid
+      // whatever the step places (undone below if the host takes over)
+      if (pipe && tn < n_items) {
+        spec.t = tn;
+        spec.slot = 1;
+        spec.seq = enqueue_screen(plans[tn], c->ctl_on.p, 1);
+      }
+      const Mailbox& mb = wait_post(c, 0, sq);
       std::memcpy(ctl, mb.ctl, sizeof(ctl));
       po = mb.po;
       tst = mb.st;
@@ -1739,6 +1747,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       have = next_head;
       continue;
     }
+    spec.t = -1;   // the paths below may change the tray: the next screen is redone
     const int code = ctl[CTL_HALT];
     float resid = po.so.resid;
     reset_ctl();
