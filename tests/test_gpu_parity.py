@@ -574,3 +574,42 @@ def test_distance_modes_match_oracle(P, dist_y, dist_x):
             s.set_option("dist_x", dist_x)
             s.set_tray(tray, metric, "nonzero", 0)
             np.testing.assert_array_equal(s.distance(), O.distance_field(tray, metric, "nonzero"))
+
+
+def test_pack_flat_and_gather_agree(P):
+    """spk_pack (items concatenated in one buffer) and spk_pack_gather (one
+    pointer per item array, what Session.pack passes) give the same results
+    and the same tray."""
+    import ctypes as C
+    from psacking_amd import _lib
+    from psacking_amd import workloads as W
+    from psacking_amd.rotations import orientation_maps
+    w = W.blobs(seed=16, n=12, tray=(64, 48, 40))
+    perms, flips = orientation_maps(24)
+    order = np.arange(len(w.items), dtype=np.int32)
+    say("pack flat vs gather, 12 items")
+    s1 = P.Session(0)
+    s1.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
+    res = s1.pack(list(w.items), order, perms, flips, "cube", False, 0)
+    s2 = P.Session(0)
+    s2.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
+    grids = [np.ascontiguousarray(it, dtype=np.int32) for it in w.items]
+    flat = np.concatenate([g.reshape(-1) for g in grids])
+    dims = np.array([g.shape for g in grids], np.int32).reshape(-1)
+    orients = (_lib.SpkOrient * len(perms))()
+    for o in range(len(perms)):
+        orients[o].perm[:] = [int(v) for v in perms[o]]
+        orients[o].flip[:] = [int(v) for v in flips[o]]
+    out = (_lib.SpkResult * len(grids))()
+    ip = C.POINTER(C.c_int32)
+    rc = _lib.load().spk_pack(C.c_void_p(s2.handle()), flat.ctypes.data_as(ip), dims.ctypes.data_as(ip),
+                              len(grids), order.ctypes.data_as(ip), orients, len(perms), 1, 0, 0, out)
+    assert rc == 0, _lib.last_error()
+    placed = 0
+    for r, g in zip(res, out):
+        assert r["found"] == bool(g.found)
+        if r["found"]:
+            placed += 1
+            assert (tuple(r["position"]), r["orientation"], r["score"]) == (tuple(g.pos), g.orient, g.score)
+    assert placed > 0
+    np.testing.assert_array_equal(s1.tray(), s2.tray())
