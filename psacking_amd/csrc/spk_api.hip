@@ -184,6 +184,7 @@ struct spk_ctx {
   int tile_force = 0;                    // option "tile": 0 tuned/default, else 4, 8 or 16
   int yz_rmax = kMaxRadix;               // option "yz_radix": largest y/z butterfly
   int score_rmax = 10;                   // option "score_radix": largest butterfly of the score pass
+  int corr_chunk = 0;                    // option "corr_chunk": orientations per pass-A block (0 auto)
   AxisPlan sy{}, sz{};                   // y and z plans of the score pass
   int pad_y = 0, pad_z = 0;              // options "pad_y", "pad_z": padded sizes (0 = smallest smooth)
   int feas_group = 0;                    // option "feas_group": pairs per feasibility launch pair
@@ -822,6 +823,7 @@ int corr_chunks(const spk_ctx* c, int m, int* chunk) {
   const int py = c->py.n, pz = c->pz.n;
   const int tiles = (pz + c->T - 1) / c->T;
   int nchunk = std::max(1, std::min(m, (2048 + tiles * py - 1) / (tiles * py)));
+  if (c->corr_chunk > 0) nchunk = (m + c->corr_chunk - 1) / c->corr_chunk;   // forced
   *chunk = (m + nchunk - 1) / nchunk;
   return (m + *chunk - 1) / *chunk;
 }
@@ -2680,6 +2682,11 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->co.on = value != 0.0;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "corr_chunk") {
+    if (value < 0 || value > 24) return fail(SPK_EINVAL, "corr_chunk must lie in [0, 24]");
+    c->corr_chunk = (int)value;
     return SPK_OK;
   }
   if (std::string(name) == "screen_all") {
