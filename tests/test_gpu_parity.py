@@ -613,3 +613,41 @@ def test_pack_flat_and_gather_agree(P):
             assert (tuple(r["position"]), r["orientation"], r["score"]) == (tuple(g.pos), g.orient, g.score)
     assert placed > 0
     np.testing.assert_array_equal(s1.tray(), s2.tray())
+
+
+def test_feasibility_flags_large_footprint(P):
+    """Items whose footprint exceeds what the item-mask kernel holds in LDS
+    (ex * ey > 4096 columns) take the three-kernel path (core masks, item
+    masks, run lists); the column test's flags still equal exact
+    feasibility, on trays with sparse obstacles and one empty above a height."""
+    from psacking_amd import proposer as PR
+    from psacking_amd.rotations import orientation_maps
+    perms, flips = orientation_maps(24)
+    rng = np.random.default_rng(91)
+    shape = (92, 84, 30)
+    item = np.ones((70, 62, 5), np.int32)
+    item[rng.random(item.shape) < 0.1] = 0
+    item[10:22, 5:17, :] = 0
+    item[:, :, 3:] *= (rng.random((70, 62, 1)) < 0.5).astype(np.int32)
+    checked = big = feasible = 0
+    for case, p in enumerate((0.002, 0.004, 0.03)):
+        tray = (rng.random(shape) < p).astype(np.int32)
+        tray[:, :, :6] = 2
+        if case == 0:
+            tray[:, :, 20:] = 0
+        say("large-footprint feasibility, case", case)
+        s = P.Session(0)
+        s.set_tray(tray, "edt", "nonzero", 0)
+        column, full, res = PR.feasible(s, item, perms, flips)
+        assert res <= 0.25
+        for o, g in enumerate(O.get_orientations(item, 24)):
+            g = O.as_grid(g)
+            if any(g.shape[a] > shape[a] for a in range(3)):
+                assert column[o] == 0 and full[o] == 0
+                continue
+            exp = _full_feasible(g, tray)
+            assert full[o] == exp and column[o] == exp, (case, o)
+            checked += 1
+            big += g.shape[0] * g.shape[1] > 4096
+            feasible += bool(exp)
+    assert big > 0 and feasible > 0 and feasible < checked
