@@ -267,7 +267,7 @@ def test_orientation_count_case(P, kats):
 
 
 def _session_pack(P, w, n_orient, resid_limit=None, metric="l1", sources="eq1", n=None,
-                  interlocking_free=False):
+                  interlocking_free=False, options=()):
     from psacking_amd.rotations import orientation_maps
     items = w.items if n is None else w.items[:n]
     vols = [int((it > 0).sum()) for it in items]
@@ -276,6 +276,8 @@ def _session_pack(P, w, n_orient, resid_limit=None, metric="l1", sources="eq1", 
     s = P.Session(0)
     if resid_limit is not None:
         s.set_option("resid_limit", resid_limit)
+    for name, value in options:
+        s.set_option(name, value)
     s.set_tray(np.zeros(w.tray_size, np.int32), metric, sources, 0)
     res = s.pack(items, np.array(order, np.int32), perms, flips,
                  "pow" if n_orient == 1 and not interlocking_free else "cube", interlocking_free, 0)
@@ -302,11 +304,14 @@ def test_pack_forced_exact_path(P):
     np.testing.assert_array_equal(s.tray(), exp.tray)
 
 
-def test_pack_pipelined_matches_oracle(P):
+@pytest.mark.parametrize("options", [(), (("corr_chunk", 1), ("col_chunks", 3), ("score_radix", 25))])
+def test_pack_pipelined_matches_oracle(P, options):
+    """Default options, and pass A one orientation per block, column-test
+    blocks over several offset chunks, radix-25 score pass."""
     from psacking_amd import workloads as W
     w = W.blobs(seed=15, n=10, tray=(64, 48, 40))
-    say("pack pipelined")
-    s, res, exp, order = _session_pack(P, w, 24, metric="edt", sources="nonzero")
+    say("pack pipelined", options)
+    s, res, exp, order = _session_pack(P, w, 24, metric="edt", sources="nonzero", options=options)
     by_item = {p.item_index: p for p in exp.placements}
     for t, r in enumerate(res):
         e = by_item[order[t]]
