@@ -354,7 +354,7 @@ size_t pencil_shmem(const spk_ctx* c) {
 // learned loop keep one post in flight while it reads the other.
 constexpr int kMailSlots = 2;
 unsigned post_async(spk_ctx* c, int slot, const int* ctl, const PackOut* po, const FieldStats* st,
-                    const float* logits, const int* fs, int nfs) {
+                    const float* logits, const int* fs, int nfs, int* gate = nullptr) {
   if (!c->mail_h) {
     HIP_OK(hipHostMalloc((void**)&c->mail_h, kMailSlots * sizeof(Mailbox),
                          hipHostMallocMapped | hipHostMallocCoherent));
@@ -364,7 +364,7 @@ unsigned post_async(spk_ctx* c, int slot, const int* ctl, const PackOut* po, con
   }
   const unsigned seq = ++c->mail_seq;
   launch(c, "post", k_post, dim3(1), dim3(64), 0, ctl, po, st, logits, fs, nfs, seq,
-         c->mail_d + slot);
+         c->mail_d + slot, gate);
   return seq;
 }
 
@@ -1576,10 +1576,11 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   };
   // Pipelined screens (column test, every item screened, no certificate):
   // behind an item's screen the next screened item's screen is enqueued
-  // before the host has read the first one's flags.  k_spec_gate lets it run
-  // only when the first item fits nowhere, i.e. when the tray stays as it is;
-  // otherwise it exits at once and its post is ignored.  The two screens post
-  // into different mailbox slots.
+  // before the host has read the first one's flags.  The first one's post
+  // (k_post) writes the gate that lets it run only when the first item fits
+  // nowhere, i.e. when the tray stays as it is; otherwise it exits at once
+  // and its post is ignored.  The two screens post into different mailbox
+  // slots.
   const bool pipe = c->co.ready && c->screen_all && feas_plan && !bounds;
   c->gates.ensure(kMailSlots * CTL_WORDS);
   auto enqueue_screen = [&](const ItemPlan& pl, const int* gate, int slot) -> unsigned {
@@ -1590,11 +1591,9 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
              (int*)nullptr, (const FieldStats*)nullptr, (const FieldStats*)nullptr,
              (PackOut*)nullptr, 0);
     enqueue_coarse(c, od, pl.m, pl.max_ex, pl.max_exy, gate, nullptr, 0, 0, fused, false);
-    const unsigned seq = post_async(c, slot, nullptr, nullptr, nullptr, nullptr, c->co.fs.p,
-                                    kFeasHeader + pl.m);
-    launch(c, "spec_gate", k_spec_gate, dim3(1), dim3(64), 0, (const int*)c->co.fs.p, pl.m,
-           c->gates.p + slot * CTL_WORDS);
-    return seq;
+    // the post also writes the gate of the screen that may follow
+    return post_async(c, slot, nullptr, nullptr, nullptr, nullptr, c->co.fs.p, kFeasHeader + pl.m,
+                      c->gates.p + slot * CTL_WORDS);
   };
   struct Spec {
     int t = -1, slot = 0;

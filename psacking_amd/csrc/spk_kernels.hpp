@@ -1161,16 +1161,28 @@ struct Mailbox {
 
 // any pointer may be null (that part is left as it was).  Every thread makes
 // its stores visible to the host before thread 0 stores seq, so a host that
-// sees seq can read the rest.
+// sees seq can read the rest.  One wavefront.
+//
+// gate (with fs, the flags of a screen): the control words of the next
+// item's speculative screen, gate[CTL_SKIP_SEARCH] = 1 when some orientation
+// of this screen is feasible.  That screen, enqueued before the host read
+// these flags, then exits at once; it runs only when this item fits nowhere,
+// i.e. when the tray stays as it is.
 __global__ void k_post(const int* __restrict__ ctl, const PackOut* __restrict__ po,
                        const FieldStats* __restrict__ st, const float* __restrict__ logits,
                        const int* __restrict__ fs, int nfs, unsigned seq,
-                       Mailbox* __restrict__ mb) {
+                       Mailbox* __restrict__ mb, int* __restrict__ gate) {
   const int i = threadIdx.x;
   if (ctl && i < CTL_WORDS) mb->ctl[i] = ctl[i];
   if (logits && i < kMailLogits) mb->logits[i] = logits[i];
   if (fs)
     for (int j = i; j < nfs && j < kMailFlags; j += blockDim.x) mb->fs[j] = fs[j];
+  if (gate && fs) {
+    int v = 0;
+    for (int j = kFeasHeader + i; j < nfs; j += blockDim.x) v |= fs[j];
+    const bool any = __any(v != 0);
+    if (i < CTL_WORDS) gate[i] = i == CTL_SKIP_SEARCH && any ? 1 : 0;
+  }
   if (i == 0) {
     if (st) mb->st = *st;
     if (po) mb->po = *po;
@@ -1882,18 +1894,6 @@ __global__ void k_coarse_decide(const int* __restrict__ cfs, int m, int* __restr
     ctl[CTL_SKIP_SEARCH] = 1;
   }
   ctl[CTL_SKIP_REFRESH] = 1;
-}
-
-// Gate of a speculative screen: gate[CTL_SKIP_SEARCH] = 1 when the screen
-// before it (flags cfs of m orientations) found a feasible orientation.  The
-// next item's screen, enqueued before the host read those flags, then exits
-// at once; it runs only when this item fits nowhere, i.e. when the tray stays
-// as it is.  One wavefront.
-__global__ void k_spec_gate(const int* __restrict__ cfs, int m, int* __restrict__ gate) {
-  int v = 0;
-  for (int o = threadIdx.x; o < m; o += 64) v |= cfs[kFeasHeader + o];
-  const bool any = __any(v != 0);
-  if (threadIdx.x < CTL_WORDS) gate[threadIdx.x] = threadIdx.x == CTL_SKIP_SEARCH && any ? 1 : 0;
 }
 
 // greedy loop: with the pass run and its residual within resid_limit, an
