@@ -271,7 +271,7 @@ int spk_dev_gather(int device, void* dst, const void* src, int64_t row_bytes, in
  * at pitch 2 (32^3 cells), a tray branch on the tray's occupancy and
  * below-surface fraction on a 64x32x32 grid, and a head on both; the 24
  * orientation logits of a record form one softmax, and the exact search
- * judges the top-k.  Feature buffers, targets and masks are DEVICE pointers
+ * evaluates the top-k.  Feature buffers, targets and masks are DEVICE pointers
  * (bf16 bits as uint16); logits, parameters and gradients are host buffers.
  * A batch of B records holds B*24 oriented item grids (record b, orientation
  * o at row b*24 + o) and B tray grids.  One proposer is used by one thread at

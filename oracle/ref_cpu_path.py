@@ -1,4 +1,4 @@
-"""The reference CPU search path restated as written.  TEST/BASELINE ONLY.
+"""The reference CPU search path restated as written.  Test and CPU-baseline use only.
 
 Used only by bench.py's ``cpu_baseline`` leg and by tests.  It follows the
 reference's FFTW branch step by step, with pocketfft (scipy.fft, complex128)

@@ -1436,7 +1436,7 @@ void timed_group(spk_ctx* c, const char* name, F&& f) {
 
 // Greedy loop with learned orientation proposals.  The proposer scores the
 // orientations of an item against the tray as it is at that item; the exact
-// search judges its top-k.  Candidates are the item's fitting orientations,
+// search evaluates its top-k.  Candidates are the item's fitting orientations,
 // narrowed to the feasible ones when the feasibility pass ran first.  When
 // more than topk candidates remain and the softmax mass of the topk
 // best-scored ones (over the candidates) is at least min_conf, only those

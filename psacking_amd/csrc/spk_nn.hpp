@@ -1,7 +1,7 @@
 // Kernels of the learned orientation proposer: a small 3-D CNN that scores
 // the 24 orientations of an item against the current tray.  There is no
 // counterpart in the reference (spectral_packer/packer.py:487-505 searches
-// every orientation); the exact FFT search judges the proposals.
+// every orientation); the exact FFT search evaluates the proposals.
 //
 // Storage: activations, weights and activation gradients are bf16 (ushort
 // bits, round-to-nearest-even from fp32), channels-last [batch][d0][d1][d2][c].

@@ -3,7 +3,7 @@
 No reference counterpart: the reference searches all orientations of every
 item (spectral_packer/packer.py:487-505).  Here a 3-D CNN trained on the plain
 packer's own per-orientation scores proposes the top-k orientations and the
-exact FFT search judges them (``pack_learned``); the plain search over every
+exact FFT search evaluates them (``pack_learned``); the plain search over every
 fitting orientation is the fallback when confidence is low or no proposal is
 feasible.
 
