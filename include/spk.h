@@ -210,9 +210,11 @@ int spk_voxelize_stl(int device, const char* path, int resolution, int32_t* dims
 /* Per-kernel HIP-event timing on the context's stream (off by default). */
 int spk_timing_enable(spk_ctx* ctx, int on);
 int spk_timing_get(spk_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
-/* orientations handed to a search kernel while timing was on (orientation
- * pairs for the feasibility pass), to turn launch times into per-orientation
- * rates; launches that the device loop gates off still count */
+/* units of work a kernel did while timing was on, to turn launch times into
+ * per-unit rates: for "score_yz" and "corr_x" the orientations the score pass
+ * searched, for the feasibility pass the orientation pairs it transformed
+ * (both counted on the device: orientations the column test excluded do not
+ * count); for other kernels the orientations handed to them */
 int spk_timing_units(spk_ctx* ctx, const char* kernel, int64_t* units);
 int spk_timing_reset(spk_ctx* ctx);
 /* Tuning/diagnostic options of a context (no reference counterpart):
@@ -241,7 +243,25 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  the field (distance to every occupied voxel, sources
  *                  "nonzero") is smaller everywhere, no orientation fits and
  *                  the item is skipped.  Exact; pays off when free space is
- *                  scattered in small gaps.  Default 0. */
+ *                  scattered in small gaps.  Default 0.
+ *   "coarse"       1 (default): the exact column test before every search
+ *                  (slab plan, tray z <= 128), in place of the feasibility
+ *                  pass; 0 off.  Applies at the next spk_tray_set.
+ *   "fine_group"   lanes per candidate offset in the column test's
+ *                  full-resolution stage: 4, 8, 16, 32 or 64 (default).
+ *   "col_chunks"   64-offset chunks per column-test block (default 1).
+ *   "screen_all"   1 (default): the learned loop screens every item with the
+ *                  column test before proposing; 0: only after an item that
+ *                  placed nothing.
+ *   "score_radix"  largest y/z butterfly of the score pass: 10 (default) or
+ *                  25; "yz_radix" the same for the other y/z passes (2..25),
+ *                  "pad_y"/"pad_z" padded FFT sizes (0: smallest 2^a3^b5^c7^d).
+ *   "corr_chunk"   orientations per pass-A block (0: automatic).
+ *   "feas_group"   orientation pairs per feasibility launch pair (0: all).
+ *   "dist_y"       distance y pass: 0 outward search, 1 Felzenszwalb
+ *                  envelope, 2 full scan (default); "dist_x": 0 outward
+ *                  search (default), 2 full scan.
+ * None of them changes a result. */
 int spk_set_option(spk_ctx* ctx, const char* name, double value);
 
 /* Plan actually used for the cached tray: FFT sizes and digit base. */
