@@ -120,7 +120,7 @@ def bench_uneven(args) -> int:
           f"setup {t_setup:.3f} s, fallbacks {fallback}, mean tray density "
           f"{np.mean(dens):.4f}", file=sys.stderr, flush=True)
     out = {
-        "metric": "items/s over 48 concurrent uneven trays, 24 rot, interlocking-free",
+        "metric": f"items/s over {len(ws)} concurrent uneven trays, 24 rot, interlocking-free",
         "value": n_items / dt,
         "unit": "items/s",
         "n_gpus": 1,
