@@ -215,7 +215,6 @@ struct spk_ctx {
   DevBuf<FieldStats> theta;              // per item: inscribed-ball bound (mx)
   DevBuf<int> cgrid, cdist, ctmp;        // padded item complement and its distance field
   DevBuf<int> gates;                     // control words gating speculative screens
-  DevBuf<int> ffch, ffact;               // flood fill: per-grid change flags, grids still changing
   // timing
   bool timing = false;
   std::vector<Timed> pending;
@@ -734,8 +733,7 @@ WgBest exact_search_one(spk_ctx* c, const HostItem& it, const OrientDesc& d, int
 void flood_fill(spk_ctx* c, int m, const std::vector<int>& exits, int nx, int ny, int nz) {
   const size_t V = (size_t)nx * ny * nz;
   c->freach.ensure((size_t)m * V);
-  c->ffch.ensure(m);
-  c->ffact.ensure(m);
+  c->flag.ensure(1);
   c->exits.ensure((size_t)3 * m);
   upload(c, c->exits.p, exits.data(), exits.size() * sizeof(int));
   const int nb = (int)std::max<size_t>(1, std::min<size_t>(2048, (V + 255) / 256));
@@ -743,26 +741,16 @@ void flood_fill(spk_ctx* c, int m, const std::vector<int>& exits, int nx, int ny
          c->freach.p, (const int*)c->exits.p, nx, ny, nz);
   const int xt = (nx + FFX - 1) / FFX;
   const dim3 grid((nz + FFZ - 1) / FFZ, (ny + FFY - 1) / FFY, xt * m);
-  // Rounds over the grids still changing: a grid whose round reached no new
-  // cell is at its fixpoint (the grids are independent) and leaves the list.
-  // A round that reports a change reached at least one new cell, so at most
-  // m * V rounds report one and the loop always ends.
-  std::vector<int> act(m), ch(m);
-  for (int o = 0; o < m; ++o) act[o] = o;
-  for (size_t it = 0; it <= (size_t)m * V && !act.empty(); ++it) {
-    const int na = (int)act.size();
-    HIP_OK(hipMemsetAsync(c->ffch.p, 0, (size_t)m * sizeof(int), c->stream));
-    if (na < m) upload(c, c->ffact.p, act.data(), (size_t)na * sizeof(int));
-    const dim3 g(grid.x, grid.y, (unsigned)(xt * na));
-    launch(c, "ff_tile", k_ff_tile, g, dim3(FFX * FFY * FFZ), 0,
-           (const unsigned char*)c->ffree.p, c->freach.p, nx, ny, nz, xt,
-           na < m ? (const int*)c->ffact.p : (const int*)nullptr, c->ffch.p);
-    download(c, ch.data(), c->ffch.p, (size_t)m * sizeof(int));
+  // a round that reports a change reached at least one new cell, so at most
+  // V rounds report one and the loop always ends
+  for (size_t it = 0; it <= (size_t)m * V; ++it) {
+    HIP_OK(hipMemsetAsync(c->flag.p, 0, sizeof(int), c->stream));
+    launch(c, "ff_tile", k_ff_tile, grid, dim3(FFX * FFY * FFZ), 0,
+           (const unsigned char*)c->ffree.p, c->freach.p, nx, ny, nz, xt, c->flag.p);
+    int ch = 0;
+    download(c, &ch, c->flag.p, sizeof(int));
     sync(c);
-    std::vector<int> next;
-    for (int o : act)
-      if (ch[o]) next.push_back(o);
-    act.swap(next);
+    if (!ch) break;
   }
 }
 
@@ -2094,7 +2082,7 @@ int spk_ctx_destroy(spk_ctx* c) {
     c->co.cols.release(); c->co.cm.release(); c->co.fs.release();
     c->co.tf.release(); c->co.im.release(); c->co.rl.release(); c->co.frl.release();
     c->co.nrun.release(); c->co.ht.release(); c->co.lines.release(); c->co.il.release();
-    c->gates.release(); c->ffch.release(); c->ffact.release();
+    c->gates.release();
     if (c->mail_h) (void)hipHostFree(c->mail_h);
     if (c->stage_ev) (void)hipEventSynchronize(c->stage_ev);
     if (c->stage) (void)hipHostFree(c->stage);

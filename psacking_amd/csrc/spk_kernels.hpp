@@ -2480,25 +2480,21 @@ k_ff_seed(const unsigned char* __restrict__ fr, unsigned char* __restrict__ reac
 }
 
 // tile 4 x 8 x 32 with a one-cell halo; local fixpoint in LDS.
-// grid (ceil(nz/32), ceil(ny/8), ceil(nx/4) * n): the z index also selects the
-// grid, o = active[z / xtiles] (o = z / xtiles without a list); changed[o] is
-// set when a cell of grid o was reached in this round.
+// grid (ceil(nz/32), ceil(ny/8), ceil(nx/4) * m): the z index also selects the grid o.
 constexpr int FFX = 4, FFY = 8, FFZ = 32;
 __global__ void __launch_bounds__(FFX * FFY * FFZ)
 k_ff_tile(const unsigned char* __restrict__ fr_all, unsigned char* __restrict__ reached_all,
-          int nx, int ny, int nz, int xtiles, const int* __restrict__ active,
-          int* __restrict__ changed) {
+          int nx, int ny, int nz, int xtiles, int* __restrict__ changed) {
   constexpr int HX = FFX + 2, HY = FFY + 2, HZ = FFZ + 2;
   __shared__ unsigned char sf[HX * HY * HZ], sr[HX * HY * HZ];
   __shared__ int gch;
   const int tid = threadIdx.x;
   const int tz = tid % FFZ, ty = (tid / FFZ) % FFY, tx = tid / (FFZ * FFY);
-  const int oi = blockIdx.z / xtiles;
-  const int o = active ? active[oi] : oi;
+  const int o = blockIdx.z / xtiles;
   const size_t V = (size_t)nx * ny * nz;
   const unsigned char* fr = fr_all + o * V;
   unsigned char* reached = reached_all + o * V;
-  const int bx = (blockIdx.z - oi * xtiles) * FFX, by = blockIdx.y * FFY, bz = blockIdx.x * FFZ;
+  const int bx = (blockIdx.z - o * xtiles) * FFX, by = blockIdx.y * FFY, bz = blockIdx.x * FFZ;
   for (int e = tid; e < HX * HY * HZ; e += FFX * FFY * FFZ) {
     const int hz_ = e % HZ, hy_ = (e / HZ) % HY, hx_ = e / (HZ * HY);
     const int X = bx + hx_ - 1, Y = by + hy_ - 1, Z = bz + hz_ - 1;
@@ -2556,7 +2552,7 @@ k_ff_tile(const unsigned char* __restrict__ fr_all, unsigned char* __restrict__ 
     gch = 1;
   }
   __syncthreads();
-  if (tid == 0 && gch) atomicOr(&changed[o], 1);
+  if (tid == 0 && gch) atomicOr(changed, 1);
 }
 
 // grid (nx, m): argmin of double(p) + penalty[k] over reached offsets of slice x
