@@ -318,13 +318,41 @@ class SearchResult:
     proximity: Optional[int] = None
 
 
+def accept_bound(penalty: str) -> float:
+    """Scores at or above this bound count as not found.
+
+    "pow" (the 1-orientation CPU path): the loop starts at bestVal = INF = 1e9
+    and takes only strictly smaller scores (packing.cpp:626-637,
+    constants.h:21).  "cube" (the cached GPU path): found iff the best score
+    is < 1e29 (fft3.cu:1361, 1447).
+    """
+    return float(INF_BOUND) if penalty == "pow" else 1e29
+
+
+def _pick(c, p, hi, penalty: str, mask=None) -> SearchResult:
+    """Score, validity and argmin over the tray box from exact c and p."""
+    N = c.shape
+    valid = (c == 0) & ~_oob_mask(N, hi)
+    if mask is not None:
+        valid &= (np.asarray(mask) == 1)
+    pen = penalty_table(N[2], penalty)
+    score = p.astype(np.float64) + pen[None, None, :]
+    score = np.where(valid, score, np.inf)
+    flat = int(np.argmin(score))          # first occurrence = lowest linear index
+    if not score.flat[flat] < accept_bound(penalty):
+        return SearchResult((-1, -1, -1), False, 0.0)
+    pos = tuple(int(v) for v in np.unravel_index(flat, N))
+    return SearchResult(pos, True, float(score.flat[flat]), int(c.flat[flat]), int(p.flat[flat]))
+
+
 def search(item, tray, phi=None, penalty: str = "pow", mask=None) -> SearchResult:
     """One-orientation search (packing.cpp:590-642 / fft3.cu:1299-1373).
 
     Valid offsets: q + hi <= N-1 per axis, collision c[q] == 0 (ID-weighted
     correlation of the raw tray with the raw item) and, if given, mask[q]==1.
     Score = double(p[q]) + penalty[k]; the minimum with the lowest linear
-    index wins.  Not found -> ((-1,-1,-1), False, 0.0).
+    index wins, if it is below accept_bound(penalty).  Not found ->
+    ((-1,-1,-1), False, 0.0).
     """
     t = as_grid(tray)
     it = as_grid(item)
@@ -337,17 +365,69 @@ def search(item, tray, phi=None, penalty: str = "pow", mask=None) -> SearchResul
     _, hi = get_bounds(itp)
     c = corr_exact(t, itp)
     p = corr_exact(phi, itp)
-    valid = (c == 0) & ~_oob_mask(t.shape, hi)
-    if mask is not None:
-        valid &= (np.asarray(mask) == 1)
-    pen = penalty_table(t.shape[2], penalty)
-    score = p.astype(np.float64) + pen[None, None, :]
-    score = np.where(valid, score, np.inf)
-    flat = int(np.argmin(score))          # first occurrence = lowest linear index
-    if not np.isfinite(score.flat[flat]):
+    return _pick(c, p, hi, penalty, mask)
+
+
+# ---------------------------------------------------------------------------
+# many searches against one tray state (large trays)
+# ---------------------------------------------------------------------------
+
+
+class TrayState:
+    """FP64 spectra of one tray state, for searches that read valid offsets.
+
+    A valid offset q has q + hi <= N - 1 on every axis, so every item voxel
+    lands at q + y <= N - 1 < P: the circular correlation on P >= N (the
+    smallest fast size) equals the linear one at every valid offset, which is
+    all that search() and the interlocking mask read.  The tray and field
+    spectra are computed once per state instead of once per call, which is
+    what makes the oracle usable at 240x123x100 with 24 orientations.
+    search_state / interlocking_mask_state equal search /
+    interlocking_free_positions (pinned in tests/test_oracle.py).
+    """
+
+    def __init__(self, tray, phi):
+        self.tray = as_grid(tray)
+        self.phi = as_grid(phi)
+        self.N = self.tray.shape
+        self.P = tuple(int(sfft.next_fast_len(int(n), real=True)) for n in self.N)
+        self.T = sfft.rfftn(self.tray.astype(np.float64), self.P, workers=-1)
+        self.F = sfft.rfftn(self.phi.astype(np.float64), self.P, workers=-1)
+
+    def _round(self, spec) -> np.ndarray:
+        N = self.N
+        r = sfft.irfftn(spec, self.P, workers=-1)[:N[0], :N[1], :N[2]]
+        ri = np.rint(r)
+        resid = float(np.abs(r - ri).max()) if r.size else 0.0
+        if resid > 0.05:
+            raise ArithmeticError(f"oracle FP64 FFT not exact (residual {resid})")
+        return ri.astype(np.int64)
+
+    def correlate(self, item, want_p: bool = True):
+        """(c, p, hi): tray and field correlations with the item (int64 over
+        the tray box, exact at valid offsets) and the item's upper bounds."""
+        it = as_grid(item)
+        _, hi = get_bounds(it)
+        I = np.conj(sfft.rfftn(it.astype(np.float64), self.P, workers=-1))
+        c = self._round(self.T * I)
+        p = self._round(self.F * I) if want_p else None
+        return c, p, hi
+
+
+def search_state(state: TrayState, item, penalty: str = "cube", mask=None) -> SearchResult:
+    """search(item, state.tray, state.phi, penalty, mask) on cached spectra."""
+    it = as_grid(item)
+    if any(it.shape[d] > state.N[d] for d in range(3)):
         return SearchResult((-1, -1, -1), False, 0.0)
-    pos = tuple(int(v) for v in np.unravel_index(flat, t.shape))
-    return SearchResult(pos, True, float(score.flat[flat]), int(c.flat[flat]), int(p.flat[flat]))
+    c, p, hi = state.correlate(it)
+    return _pick(c, p, hi, penalty, mask)
+
+
+def interlocking_mask_state(state: TrayState, obj) -> np.ndarray:
+    """interlocking_free_positions(state.tray, obj) on cached spectra."""
+    c, _, _ = state.correlate(obj, want_p=False)
+    free, seed = _if_labels_from(c, as_grid(obj))
+    return _reach(free, seed)
 
 
 def search_batch(orientations: Sequence[np.ndarray], tray, phi, penalty: str = "cube",
@@ -515,13 +595,17 @@ def coarse_infeasible(item, tray, s: int = 2) -> bool:
 # ---------------------------------------------------------------------------
 
 
-def _if_labels(tray, obj):
-    t = as_grid(tray)
-    o = as_grid(obj)
+def _if_labels_from(coll: np.ndarray, o: np.ndarray):
+    """Free offsets (collision 0, in bounds) and seeds from a collision grid.
+
+    Object max from voxels != 0, 0 for an empty object (the reference's own
+    test oracle treats an empty object as fitting everywhere,
+    tests/test_flood_fill.py:80-83); seeds are free cells on i|j|k == 0 or
+    == max(0, N-1-objmax) (fft3.cu:300-343).
+    """
     nz_idx = np.argwhere(o != 0)
     omax = nz_idx.max(axis=0) if len(nz_idx) else np.zeros(3, dtype=np.int64)
-    coll = corr_exact(t, pad_to(o, t.shape))
-    N = t.shape
+    N = coll.shape
     i = np.arange(N[0])[:, None, None]
     j = np.arange(N[1])[None, :, None]
     k = np.arange(N[2])[None, None, :]
@@ -533,6 +617,24 @@ def _if_labels(tray, obj):
     return free, seed
 
 
+def _if_labels(tray, obj):
+    t = as_grid(tray)
+    o = as_grid(obj)
+    coll = corr_exact(t, pad_to(o, t.shape))
+    return _if_labels_from(coll, o)
+
+
+def _reach(free: np.ndarray, seed: np.ndarray) -> np.ndarray:
+    """Free cells 6-connected to a seed (connected-component labelling)."""
+    lab, n = ndimage.label(free, structure=ndimage.generate_binary_structure(3, 1))
+    if n == 0:
+        return np.zeros(free.shape, dtype=np.int32)
+    keep = np.zeros(n + 1, dtype=bool)
+    keep[np.unique(lab[seed])] = True
+    keep[0] = False
+    return keep[lab].astype(np.int32)
+
+
 def interlocking_free_positions(tray, obj) -> np.ndarray:
     """Interlocking-free mask (bindings.cpp:561-648, fft3.cu:234-402).
 
@@ -542,13 +644,7 @@ def interlocking_free_positions(tray, obj) -> np.ndarray:
     own oracle tests/test_flood_fill.py:67-189 computes).
     """
     free, seed = _if_labels(tray, obj)
-    lab, n = ndimage.label(free, structure=ndimage.generate_binary_structure(3, 1))
-    if n == 0:
-        return np.zeros(free.shape, dtype=np.int32)
-    keep = np.zeros(n + 1, dtype=bool)
-    keep[np.unique(lab[seed])] = True
-    keep[0] = False
-    return keep[lab].astype(np.int32)
+    return _reach(free, seed)
 
 
 def interlocking_free_positions_bfs(tray, obj) -> np.ndarray:
@@ -682,28 +778,22 @@ def pack_voxels(items: Sequence[np.ndarray], tray_size, num_orientations: int = 
     num_placed = 0
     for orig in order:
         item = proc[orig]
-        orients = get_orientations(item, num_orientations)
+        orients = [as_grid(g) for g in get_orientations(item, num_orientations)]
         single = (num_orientations == 1 and not interlocking_free)
         phi = distance_field(tray, metric, sources)
-        best = None
-        if interlocking_free:
-            valid = [(o, as_grid(g)) for o, g in enumerate(orients)
-                     if not any(g.shape[d] > tray.shape[d] for d in range(3))]
-            if valid:
-                pos, found, score, bo = search_batch([g for _, g in valid], tray, phi,
-                                                     penalty="cube", interlocking_free=True)
-                if found:
-                    best = (pos, score, valid[bo][0], valid[bo][1])
-        else:
-            best_score = float("inf")
-            for o, g in enumerate(orients):
-                g = as_grid(g)
-                if any(g.shape[d] > tray.shape[d] for d in range(3)):
-                    continue
-                r = search(g, tray, phi, penalty="pow" if single else "cube")
-                if r.found and r.score < best_score:
-                    best_score = r.score
-                    best = (r.position, r.score, o, g)
+        state = TrayState(tray, phi)
+        # orientations larger than the tray are skipped (fft3.cu:1563-1568,
+        # packer.py:491-494); a later orientation replaces the best only on a
+        # strictly smaller score (fft3.cu:1579, packer.py:501)
+        best, best_score = None, float("inf")
+        for o, g in enumerate(orients):
+            if any(g.shape[d] > tray.shape[d] for d in range(3)):
+                continue
+            mask = interlocking_mask_state(state, g) if interlocking_free else None
+            r = search_state(state, g, penalty="pow" if single else "cube", mask=mask)
+            if r.found and r.score < best_score:
+                best_score = r.score
+                best = (r.position, r.score, o, g)
         if best is not None:
             item_id = id_offset + num_placed + 1
             tray = place_in_tray(best[3], tray, best[0], item_id)

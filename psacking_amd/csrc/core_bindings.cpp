@@ -155,7 +155,8 @@ py::tuple py_fft_search_batch(py::list orientations, py::object tray, py::object
   Grid tr = to_grid(tray), d = to_grid(tray_distance);
   ensure_cached_tray(tr, d, generation);
   std::vector<Grid> items = grids_of(orientations);
-  if (items.empty()) return py::make_tuple(py::make_tuple(-1, -1, -1), false, 0.0);
+  // fft3.cu:1560: the batch starts from best_score = 1e30
+  if (items.empty()) return py::make_tuple(py::make_tuple(-1, -1, -1), false, 1e30);
   spk_result best = search_list(items, SPK_PENALTY_CUBE, 0);
   if (!best.found) return py::make_tuple(py::make_tuple(-1, -1, -1), false, 1e30);
   return result_tuple(best);

@@ -103,10 +103,26 @@ int next_smooth(int n) {
   return m;
 }
 
+// A device buffer that grows on demand and frees itself: owners (contexts,
+// proposers) are destroyed with their device current, after their stream
+// has drained, so every member is released without being listed by hand.
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), cap(o.cap) { o.p = nullptr; o.cap = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p; cap = o.cap;
+      o.p = nullptr; o.cap = 0;
+    }
+    return *this;
+  }
+  ~DevBuf() { release(); }
   void ensure(size_t n) {
     if (n <= cap) return;
     if (p) hipFree(p);
@@ -2065,31 +2081,13 @@ int spk_ctx_destroy(spk_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     for (auto& t : c->pending) { hipEventDestroy(t.a); hipEventDestroy(t.b); }
     for (auto e : c->pool) hipEventDestroy(e);
-    c->tray.release(); c->phi.release(); c->tmpi.release(); c->stats.release();
-    c->tw.release(); c->t1.release(); c->t2.release(); c->g1.release(); c->g2.release();
-    c->pen.release(); c->items.release(); c->od.release(); c->ispec.release();
-    c->y1.release(); c->y2.release(); c->rec.release(); c->per.release(); c->sout.release();
-    c->rec2.release();
-    c->ver.release(); c->flag.release(); c->taps.release(); c->big.release();
-    c->ffree.release(); c->freach.release(); c->pgrid.release(); c->exits.release();
-    c->od_all.release(); c->pout.release(); c->ctl.release(); c->zero.release();
-    c->heights.release(); c->map_all.release(); c->tfeat.release(); c->ifeat.release();
-    c->ipool.release(); c->rscores.release(); c->od_prop.release(); c->od_rest.release();
-    c->imaps.release();
-    c->fstate.release(); c->ctl_on.release();
-    c->theta.release(); c->cgrid.release(); c->cdist.release(); c->ctmp.release();
-    c->pairs.release();
-    c->co.cols.release(); c->co.cm.release(); c->co.fs.release();
-    c->co.tf.release(); c->co.im.release(); c->co.rl.release(); c->co.frl.release();
-    c->co.nrun.release(); c->co.ht.release(); c->co.lines.release(); c->co.il.release();
-    c->gates.release();
     if (c->mail_h) (void)hipHostFree(c->mail_h);
     if (c->stage_ev) (void)hipEventSynchronize(c->stage_ev);
     if (c->stage) (void)hipHostFree(c->stage);
     if (c->stage_ev) (void)hipEventDestroy(c->stage_ev);
     if (c->stream) hipStreamDestroy(c->stream);
+    delete c;   // the device buffers free themselves, with the device current
   }
-  delete c;
   return SPK_OK;
 }
 
@@ -2850,13 +2848,9 @@ int spk_nn_destroy(spk_nn* nn) {
   {
     DeviceGuard g(nn->device);
     if (nn->stream) hipStreamSynchronize(nn->stream);
-    nn->params.release(); nn->grads.release(); nn->m1.release(); nn->m2.release();
-    nn->logits.release(); nn->loss_rows.release(); nn->part.release(); nn->wbf.release();
-    nn->concat.release(); nn->g_concat.release();
-    for (int i = 0; i < NN_LAYERS; ++i) { nn->act[i].release(); nn->grad[i].release(); }
     if (nn->stream) hipStreamDestroy(nn->stream);
+    delete nn;   // the device buffers free themselves, with the device current
   }
-  delete nn;
   return SPK_OK;
 }
 

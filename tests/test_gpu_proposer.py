@@ -97,7 +97,11 @@ def test_forward_matches_oracle(PR):
     got = nn.forward(*_dev_batch(item, tray, q, mask)[:2])
     ref, _ = N.forward(params, item, tray, emulate=True)
     scale = np.abs(ref).max()
-    np.testing.assert_allclose(got, ref, rtol=1e-2, atol=1e-2 * scale)
+    err = float(np.abs(got - ref).max()) / scale
+    say("forward max abs error / max |logit|", f"{err:.2e}")
+    # accumulation order differs, so a bf16 rounding of an activation may
+    # flip (2^-8 relative); the logits stay within a few 1e-3 of the scale
+    assert err < 4e-3, err
 
 
 def test_gradients_match_oracle(PR):
@@ -113,7 +117,7 @@ def test_gradients_match_oracle(PR):
         for part, sl in (("W", w), ("b", b)):
             err = np.linalg.norm(g[sl] - ref[sl]) / max(np.linalg.norm(ref[sl]), 1e-12)
             say(name, part, "relative L2 error", f"{err:.2e}")
-            assert err < 3e-2, (name, part, err)
+            assert err < 1e-3, (name, part, err)
 
 
 def test_training_fits_a_batch(PR):
