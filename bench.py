@@ -12,12 +12,17 @@ empty tray, after W warmup items on a throwaway tray:
   plain    every fitting orientation searched in one batched launch,
   learned  the proposer's top-k orientations searched (full search when its
            confidence is low or no proposal is feasible).
-The metric's value is the learned loop's items/s when its final tray density
-is within 0.5 percentage points of the plain loop's, else the plain loop's.
+The metric's value is the learned loop's items PLACED per second (items that
+fit nowhere are decided too, and count in items_processed_per_s only) when
+the learned loop's final density over a whole run of all items is within 0.5
+percentage points of the plain loop's, else the plain loop's.  When fewer
+items are timed, both loops also run untimed over all items for that gate.
 
 Prints one JSON line with the roofline of the dominant kernel of the reported
-loop (HIP events on the library's stream) and a bounded CPU baseline of the
-reference CPU path restated in oracle/ref_cpu_path.py.
+loop (HIP events on the library's stream), the whole loop's fraction of the
+HBM roofline, and a CPU baseline of the reference CPU path restated in
+oracle/ref_cpu_path.py (a prefix packed in full and compared with the GPU's
+placements, a 1-thread figure, and the 64^3 boxes workload in full).
 """
 from __future__ import annotations
 
@@ -158,8 +163,11 @@ def main(argv=None) -> int:
                     help="uneven workload: number of independent trays")
     ap.add_argument("--metric", default="edt", choices=["edt", "l1"])
     ap.add_argument("--sources", default="nonzero", choices=["nonzero", "eq1"])
-    ap.add_argument("--cpu-budget", type=float, default=20.0,
-                    help="seconds of CPU work for the baseline sample (0 = skip)")
+    ap.add_argument("--cpu-items", type=int, default=2,
+                    help="CPU baseline: items of the timed sequence packed in full on the host "
+                         "(0 = skip the baseline)")
+    ap.add_argument("--cpu-budget", type=float, default=12.0,
+                    help="CPU baseline: seconds of 1-thread searches for the 1-thread figure")
     ap.add_argument("--plan", type=int, default=0, choices=[0, 1, 2],
                     help="FFT plan: 0 auto, 1 yz slabs, 2 z rows + y pencils")
     ap.add_argument("--proposer", default="auto", choices=["auto", "plain", "learned"],
@@ -328,30 +336,60 @@ def main(argv=None) -> int:
                       f"{r['kt'][n][0] / max(1, r['kt'][n][1]) * 1e3:8.1f} us/launch  "
                       f"{100 * r['kt'][n][0] / max(busy, 1e-9):5.1f} %", file=sys.stderr)
         print(f"[bench:{name}] {K} items ({r['placed']} placed), {r['dt']:.3f} s = "
-              f"{K / r['dt']:.1f} items/s ({r['dt_events']:.3f} s with kernel events), "
+              f"{r['placed'] / r['dt']:.1f} items placed/s, {K / r['dt']:.1f} items processed/s "
+              f"({r['dt_events']:.3f} s with kernel events), "
               f"kernels busy {busy / 1e3:.3f} s, density "
               f"{100 * r['density']:.2f} %, fallbacks {r['fallbacks']}, max residual "
               f"{r['max_resid']:.4f}" + (f", counts {r['counts']}" if "counts" in r else ""),
               file=sys.stderr, flush=True)
 
+    # Density gate and agreement.  The gate compares the final densities of a
+    # whole run over every item: when fewer items are timed, both loops also
+    # run untimed over all of them.  Agreement counts only items that at
+    # least one loop placed.
+    def agreement(a, b):
+        rows = [(x, y) for x, y in zip(a, b) if x["found"] or y["found"]]
+        same = [x["found"] == y["found"] and tuple(x["position"]) == tuple(y["position"]) and
+                x["orientation"] == y["orientation"] for x, y in rows]
+        first = next((i for i, v in enumerate(same) if not v), len(same))
+        return (float(np.mean(same)) if same else 1.0), first, len(rows)
+
     agree = None
     if "plain" in runs and "learned" in runs:
         a, b = runs["plain"]["res"], runs["learned"]["res"]
-        same = [x["found"] == y["found"] and (not x["found"] or (
-            tuple(x["position"]) == tuple(y["position"]) and x["orientation"] == y["orientation"]))
-            for x, y in zip(a, b)]
-        first = next((i for i, v in enumerate(same) if not v), len(same))
-        agree = {"placements_identical": float(np.mean(same)), "first_divergence": first,
+        ident, first, n_rows = agreement(a, b)
+        agree = {"placements_identical": ident, "first_divergence": first,
+                 "items_placed_by_either": n_rows,
                  "density_plain": runs["plain"]["density"],
                  "density_learned": runs["learned"]["density"],
-                 "density_loss_pp": 100 * (runs["plain"]["density"] - runs["learned"]["density"]),
-                 "items_per_s_plain": K / runs["plain"]["dt"],
-                 "items_per_s_learned": K / runs["learned"]["dt"],
+                 "items_placed_per_s_plain": runs["plain"]["placed"] / runs["plain"]["dt"],
+                 "items_placed_per_s_learned": runs["learned"]["placed"] / runs["learned"]["dt"],
                  "speedup": runs["plain"]["dt"] / runs["learned"]["dt"]}
+        if K < len(items):
+            full = [items[i] for i in order]
+            s, _ = fresh_session()
+            rp = s.pack(full, np.arange(len(full), dtype=np.int32), perms, flips, penalty,
+                        False, 0)
+            dp = float((s.tray() > 0).sum()) / V
+            s, _ = fresh_session()
+            rl = PR.pack_learned(s, nn, full, np.arange(len(full)), perms, flips, penalty,
+                                 args.topk, args.min_conf)[0]
+            dl = float((s.tray() > 0).sum()) / V
+            fi, ff, fn = agreement(rp, rl)
+            agree["full_run"] = {"items": len(full),
+                                 "placed_plain": sum(1 for r in rp if r["found"]),
+                                 "placed_learned": sum(1 for r in rl if r["found"]),
+                                 "density_plain": dp, "density_learned": dl,
+                                 "placements_identical": fi, "first_divergence": ff}
+        else:
+            dp, dl = runs["plain"]["density"], runs["learned"]["density"]
+        agree["gate_items"] = len(items)
+        agree["density_loss_pp"] = 100 * (dp - dl)
         print(f"[bench] learned vs plain: {agree}", file=sys.stderr, flush=True)
 
-    # the metric's value: learned items/s when its density is within 0.5 pp
-    # of the plain method's, else the plain method's items/s
+    # the metric's value: the learned loop's items placed per second when its
+    # density (whole run) is within 0.5 pp of the plain method's, else the
+    # plain method's
     if "learned" in runs and (agree is None or agree["density_loss_pp"] <= 0.5):
         used = "learned"
     else:
@@ -374,25 +412,19 @@ def main(argv=None) -> int:
             tk = json.load(fh)["kernels"].get(dom)
         if tk and tk["units_per_launch"] > 0:
             traffic = tk["hbm_bytes_per_launch"] * m_avg / tk["units_per_launch"]
+    # whole-loop fraction: algorithmic bytes of the loop's own method (every
+    # orientation it searches: B_OS = 48 Ph each, every placement: B_place =
+    # 48 Ph + 32 V) over the wall time at HBM peak
+    b_loop = 48.0 * Ph * r["searched"] + (48.0 * Ph + 32.0 * V) * r["placed"]
+    loop_frac = b_loop / r["dt"] / (HBM_PEAK_GBS * 1e9)
 
     cpu = None
-    if args.cpu_budget > 0:
-        from oracle import ref_cpu_path as R
-        cores = min(16, len(os.sched_getaffinity(0)))
-        item0 = items[order[0]]
-        # the reference path's cost does not depend on tray contents
-        sec_item, nsearch, spent = R.time_item_sample(item0, np.zeros(w.tray_size, np.int32),
-                                                      n_orient, workers=cores,
-                                                      budget_s=args.cpu_budget)
-        cpu = {"value": 1.0 / sec_item, "unit": "items/s", "cores": cores, "kind": "port",
-               "sample": (f"reference CPU path restated (scipy pocketfft C2C double at 2N+1, "
-                          f"plan per call): {nsearch} of {n_orient} orientation searches of item "
-                          f"{order[0]} timed ({spent:.1f} s), extrapolated to one item")}
-        print(f"[cpu] {cpu}", file=sys.stderr, flush=True)
+    if args.cpu_items > 0:
+        cpu = cpu_baseline(args, w, items, order, perms, flips, P)
 
     out = {
         "metric": METRIC,
-        "value": K / r["dt"],
+        "value": r["placed"] / r["dt"],
         "unit": "items/s",
         "n_gpus": 1,
         "steps": K,
@@ -408,6 +440,7 @@ def main(argv=None) -> int:
                          f"(seed {w.seed}), {n_orient} orientations, proximity "
                          f"{args.metric}/{args.sources}, FP32 FFT + FP64 score"),
             "path": used,
+            "value_is": "items placed / wall seconds over the timed items",
             "proposer": (None if used != "learned" else
                          {"topk": args.topk, "min_conf": args.min_conf,
                           "counts": runs["learned"].get("counts"),
@@ -415,10 +448,13 @@ def main(argv=None) -> int:
             "learned_vs_plain": agree,
             "items_timed": K,
             "items_placed_timed": r["placed"],
+            "items_processed_per_s": K / r["dt"],
             "tray_density": r["density"],
             "exact_fallbacks": r["fallbacks"],
             "max_fft_residual": r["max_resid"],
             "setup_s": r["setup"],
+            "loop_roofline": {"algorithmic_bytes": b_loop, "orientations_searched": r["searched"],
+                              "achieved_GBps": b_loop / r["dt"] / 1e9, "frac": loop_frac},
         },
         "roofline": {
             "bound": "hbm",
@@ -437,6 +473,86 @@ def main(argv=None) -> int:
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def fft_library() -> str:
+    """What the CPU baseline's FFT is: FFTW3 when the host has it (the
+    reference's CPU branch links it), else pocketfft through scipy.fft."""
+    import subprocess
+    try:
+        ld = subprocess.run(["ldconfig", "-p"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                            text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        ld = ""
+    libs = sorted({l.split()[0] for l in ld.splitlines() if "libfftw3" in l})
+    if libs:
+        return f"FFTW3 present ({', '.join(libs)}) but not bound from Python; pocketfft used"
+    return "pocketfft (scipy.fft, complex128); ldconfig lists no libfftw3"
+
+
+def cpu_baseline(args, w, items, order, perms, flips, P):
+    """The reference CPU path restated as written (oracle/ref_cpu_path.py),
+    timed on this host:
+      * value: a prefix of the timed items (24 orientations, L1 field of
+        value-1 voxels, cube penalty, (2N+1)-padded C2C double FFTs with a
+        plan per call) packed in full with every host core this process may
+        use (OMP_NUM_THREADS caps it, as on the GPU host); its placements are
+        compared with the GPU loop on the same items and settings;
+      * value_1thread: the same search at one thread, from the orientation
+        searches of the prefix's first item that fit a bounded budget;
+      * config1: the 64^3 tray with 20 boxes, 1 orientation, packed in full
+        with every core, placements compared with the GPU's."""
+    from oracle import ref_cpu_path as R
+    from psacking_amd import workloads as W
+    nproc = len(os.sched_getaffinity(0))
+    cores = nproc
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = max(1, min(nproc, int(os.environ["OMP_NUM_THREADS"])))
+    n = min(args.cpu_items, len(order))
+    pre = [items[i] for i in order[:n]]
+    t0 = time.perf_counter()
+    cpu_out, cpu_tray = R.pack_as_written(pre, w.tray_size, w.num_orientations, workers=cores)
+    dt = time.perf_counter() - t0
+    s = P.Session(0)
+    s.set_tray(np.zeros(w.tray_size, np.int32), "l1", "eq1", 0)
+    g = s.pack(pre, np.arange(n, dtype=np.int32), perms, flips, "cube", False, 0)
+    match = all(c[0] == bool(r["found"]) and (not c[0] or (
+        tuple(c[1]) == tuple(r["position"]) and c[2] == r["orientation"] and c[3] == r["score"]))
+        for c, r in zip(cpu_out, g)) and np.array_equal(s.tray(), cpu_tray)
+    # one thread: a bounded sample of the same searches, extrapolated to an item
+    sec_item, nsearch, spent = R.time_item_sample(pre[0], np.zeros(w.tray_size, np.int32),
+                                                  w.num_orientations, workers=1,
+                                                  budget_s=args.cpu_budget)
+    # config 1 in full
+    b = W.boxes()
+    bv = [int((it > 0).sum()) for it in b.items]
+    border = [int(i) for i in np.argsort(bv)[::-1]]
+    bi = [b.items[i] for i in border]
+    t1 = time.perf_counter()
+    c1_out, c1_tray = R.pack_as_written(bi, b.tray_size, 1, workers=cores)
+    dt1 = time.perf_counter() - t1
+    gb = P.BinPacker(b.tray_size, num_orientations=1).pack_voxels(bi, sort_by_volume=False)
+    c1_match = np.array_equal(gb.tray, c1_tray) and all(
+        c[0] == p.success and (not c[0] or (tuple(c[1]) == tuple(p.position) and c[3] == p.score))
+        for c, p in zip(c1_out, gb.placements))
+    placed = sum(1 for c in cpu_out if c[0])
+    cpu = {"value": placed / dt, "unit": "items/s", "cores": cores, "kind": "port",
+           "sample": (f"reference CPU path restated as written ({fft_library()}, C2C double at "
+                      f"2N+1, plan per call): the first {n} items of the timed sequence "
+                      f"({placed} placed), {w.num_orientations} orientations, L1/eq1 field, "
+                      f"packed in full in {dt:.1f} s on {cores} threads (nproc {nproc})"),
+           "items_processed_per_s": n / dt,
+           "placements_match_gpu": bool(match),
+           "value_1thread": 1.0 / sec_item,
+           "sample_1thread": (f"{nsearch} of {w.num_orientations} orientation searches of the "
+                              f"first item at 1 thread ({spent:.1f} s), extrapolated to one item"),
+           "config1": {"workload": "boxes: 64^3 tray, 20 boxes (seed 1), 1 orientation",
+                       "items_per_s": len(bi) / dt1,
+                       "items_placed": sum(1 for c in c1_out if c[0]),
+                       "seconds": dt1, "cores": cores,
+                       "placements_match_gpu": bool(c1_match)}}
+    print(f"[cpu] {cpu}", file=sys.stderr, flush=True)
+    return cpu
 
 
 if __name__ == "__main__":

@@ -46,8 +46,13 @@ def corr3_as_written(a: np.ndarray, b: np.ndarray, workers: int = 1) -> np.ndarr
     return np.ascontiguousarray(r[::-1, ::-1, ::-1])
 
 
-def search_as_written(item, tray, phi=None, workers: int = 1):
-    """fft_search_placement (packing.cpp:590-642) with the CPU FFT path."""
+def search_as_written(item, tray, phi=None, workers: int = 1, penalty: str = "pow"):
+    """fft_search_placement (packing.cpp:590-642) with the CPU FFT path.
+
+    penalty "pow" is that function's P * pow(k/L, 3) with the distance field
+    computed inside; "cube" with a field passed in is the per-orientation
+    step of the 24-orientation loop (packer.py:496-498), whose cached search
+    scores ((hp*qz)*qz)*qz (fft3.cu:173), run here on the CPU FFT path."""
     tray = np.asarray(tray, dtype=np.int64)
     N = tray.shape
     L = N[2]
@@ -64,10 +69,15 @@ def search_as_written(item, tray, phi=None, workers: int = 1):
     prox = corr3_as_written(np.asarray(phi, dtype=np.int64), item_p, workers)
     # the strict-< loop over where3d_flat's (i, j, k) order keeps the first
     # minimum; vectorised here so the baseline is not charged Python loop time
-    pen = np.array([100000000.0 * math.pow((kk + 0.0) / (L + 0.0), 3.0) for kk in range(L)])
+    if penalty == "pow":
+        pen = np.array([100000000.0 * math.pow((kk + 0.0) / (L + 0.0), 3.0) for kk in range(L)])
+        bound = 1e9           # bestVal = INF (packing.cpp:626, constants.h:21)
+    else:
+        pen = np.array([((100000000.0 * (kk / L)) * (kk / L)) * (kk / L) for kk in range(L)])
+        bound = 1e29          # fft3.cu:1361
     score = np.where(coll == 0, prox.astype(np.float64) + pen[None, None, :], np.inf)
     flat = int(np.argmin(score))
-    if not np.isfinite(score.flat[flat]):
+    if not score.flat[flat] < bound:
         return (-1, -1, -1), False, 0.0
     best_id = tuple(int(v) for v in np.unravel_index(flat, N))
     return best_id, True, float(score.flat[flat])
@@ -96,3 +106,41 @@ def time_item_sample(item, tray, n_orient: int = 24, workers: int = 1, budget_s:
             break
     per_search = float(np.mean(times))
     return t_dist + per_search * n_orient, len(times), time.perf_counter() - t0
+
+
+def pack_as_written(items, tray_size, n_orient: int = 24, workers: int = 1):
+    """The reference greedy loop (packer.py:447-529) on the CPU FFT path.
+
+    Items in the given order; per item the L1 field of the tray once
+    (packer.py:460-461) when n_orient > 1, each orientation that fits the
+    tray searched as written (search_as_written, "cube" penalty; with one
+    orientation the "pow" search with its own field), the first strictly
+    smaller score wins (packer.py:501), the winner is placed with id
+    num_placed + 1 (packer.py:509-511).  Returns ([(found, pos, orient,
+    score)], final tray).
+    """
+    from .spk_oracle import get_orientations, place_in_tray
+    tray = np.zeros(tuple(tray_size), np.int32)
+    out = []
+    placed = 0
+    for item in items:
+        orients = [np.ascontiguousarray(g.astype(np.int32))
+                   for g in get_orientations(np.asarray(item), n_orient)]
+        phi = calculate_distance(tray) if n_orient > 1 else None
+        best = None
+        for o, g in enumerate(orients):
+            if any(g.shape[d] > tray.shape[d] for d in range(3)):
+                continue
+            if n_orient > 1:
+                pos, found, score = search_as_written(g, tray, phi, workers, "cube")
+            else:
+                pos, found, score = search_as_written(g, tray, None, workers, "pow")
+            if found and (best is None or score < best[2]):
+                best = (pos, o, score, g)
+        if best is None:
+            out.append((False, None, -1, None))
+            continue
+        placed += 1
+        tray = place_in_tray(best[3], tray, best[0], placed)
+        out.append((True, tuple(best[0]), best[1], best[2]))
+    return out, tray
