@@ -170,6 +170,16 @@ int spk_pack_many(spk_ctx* const* ctxs, int n, const spk_pack_args* args);
 /* Standalone full-grid operations (host in, host out). */
 int spk_distance(spk_ctx* ctx, const int32_t* grid, int nx, int ny, int nz, int metric,
                  int sources, int32_t* out);
+/* spk_distance with no device allocation (no counterpart in the reference:
+ * for callers that cannot allocate device memory -- serving loops and
+ * harnesses that forbid hipMalloc).  The grid, the intermediate field and the
+ * result live in a static device array of the library (32 MiB + 256 bytes
+ * per GPU, held until the process ends); grids of more than 4,194,304 voxels
+ * fail with SPK_EUNSUPPORTED.  Calls on one GPU share that array: the library
+ * serialises them with a lock per device, so overlapping calls from several
+ * threads wait for each other.  Same results as spk_distance. */
+int spk_distance_noalloc(spk_ctx* ctx, const int32_t* grid, int nx, int ny, int nz, int metric,
+                         int sources, int32_t* out);
 int spk_correlate(spk_ctx* ctx, const int32_t* a, const int32_t* b, int nx, int ny,
                   int nz, int convolve, int32_t* out);
 int spk_collision_grid(spk_ctx* ctx, const int32_t* tray, int nx, int ny, int nz,

@@ -24,7 +24,7 @@ EXPORTS = (
     "spk_nn_create", "spk_nn_destroy", "spk_nn_num_params", "spk_nn_set_params",
     "spk_nn_get_params", "spk_nn_forward", "spk_nn_gradient", "spk_nn_train_step",
     "spk_nn_item_features", "spk_tray_features", "spk_pack_records", "spk_pack_learned",
-    "spk_voxelize_mesh", "spk_voxelize_stl", "spk_feasible",
+    "spk_voxelize_mesh", "spk_voxelize_stl", "spk_feasible", "spk_distance_noalloc",
 )
 
 
@@ -80,6 +80,8 @@ def load() -> C.CDLL:
     lib.spk_voxelize_mesh.argtypes = [i32, fp, i64, ip, i64, i32, ip, ip, i64]
     lib.spk_voxelize_stl.argtypes = [i32, C.c_char_p, i32, ip, ip, i64]
     lib.spk_feasible.argtypes = [vp, ip, i32, i32, i32, C.POINTER(SpkOrient), i32, ip, ip, fp]
+    for name in ("spk_distance", "spk_distance_noalloc"):
+        getattr(lib, name).argtypes = [vp, ip, i32, i32, i32, i32, i32, ip]
     _lib = lib
     return lib
 

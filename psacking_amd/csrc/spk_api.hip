@@ -247,6 +247,7 @@ struct spk_ctx {
   hipEvent_t stage_ev = nullptr;             // recorded after its last copy
   // host-side copy of item buffer bookkeeping
   std::vector<int> h_items;
+  void* scratch = nullptr;                   // device address of this device's static scratch
 
   hipEvent_t ev() {
     if (!pool.empty()) {
@@ -2013,6 +2014,39 @@ void load_items(spk_ctx* c, const int32_t* items, const int32_t* const* ptrs,
 
 }  // namespace
 
+// ---- static scratch of the allocation-free entry point ---------------------
+// spk_distance_noalloc keeps the grid (overwritten by the result) and the
+// intermediate field of the distance transform in this file-scope array, so
+// it calls no device allocator.  Each GPU the library runs on holds its own
+// copy (32 MiB + 256 bytes) from module load to process end.  Host code never
+// uses the array's name as a device address: spk_ctx_create looks up each
+// device's address once, with that device current, and every launch and copy
+// takes that address.
+constexpr size_t kStaticVoxels = size_t(1) << 22;   // largest grid: 4,194,304 voxels
+constexpr size_t kScratchBytes = 2 * kStaticVoxels * sizeof(int) + 256;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 256;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+std::mutex g_scratch_call_mu[kMaxDevices];   // one allocation-free call per device at a time
+
+// device address of `scratch` on `device` (which must be current): looked up
+// once per device and kept
+void* scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) throw SpkError{SPK_EUNSUPPORTED, "device index too large"};
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw SpkError{SPK_EDEVICE, "hipGetSymbolAddress returned a null scratch address"};
+    g_scratch_addr[device] = p;
+  }
+  return g_scratch_addr[device];
+}
+}  // namespace
+
 namespace {
 int dev_check(int device) {
   int n = 0;
@@ -2064,6 +2098,7 @@ int spk_ctx_create(int device, spk_ctx** out) {
   c->device = device;
   int rc = guarded(c, [&]() {
     HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->scratch = scratch_address(device);
     return SPK_OK;
   });
   if (rc != SPK_OK) {
@@ -2357,6 +2392,38 @@ int spk_distance(spk_ctx* c, const int32_t* grid, int nx, int ny, int nz, int me
     sync(c);
     if (c->timing) c->flush_timing();
     g.release(); o.release(); t.release(); st.release();
+    return SPK_OK;
+  });
+}
+
+int spk_distance_noalloc(spk_ctx* c, const int32_t* grid, int nx, int ny, int nz, int metric,
+                         int sources, int32_t* out) {
+  return guarded(c, [&]() {
+    if (!grid || !out) throw SpkError{SPK_EINVAL, "null grid"};
+    check_dims(nx, ny, nz);
+    const size_t V = (size_t)nx * ny * nz;
+    if (V > kStaticVoxels)
+      throw SpkError{SPK_EUNSUPPORTED,
+                     "spk_distance_noalloc: the grid has more voxels than the static scratch "
+                     "holds (4194304)"};
+    if (!c->scratch) throw SpkError{SPK_ESTATE, "static scratch address not set up"};
+    std::lock_guard<std::mutex> lk(g_scratch_call_mu[c->device]);
+    // per-kernel timing creates events on demand: the launches here are not timed
+    struct TimingOff {
+      spk_ctx* c;
+      bool was;
+      ~TimingOff() { c->timing = was; }
+    } off{c, c->timing};
+    c->timing = false;
+    int* gbuf = static_cast<int*>(c->scratch);   // grid, then the result
+    int* tbuf = gbuf + kStaticVoxels;            // field after the z and y passes
+    FieldStats* st = reinterpret_cast<FieldStats*>(tbuf + kStaticVoxels);
+    upload(c, gbuf, grid, V * sizeof(int));
+    // every value of tbuf and of the result is written before it is read, and
+    // the field statistics are reset by the first kernel: nothing to clear
+    run_distance(c, gbuf, gbuf, tbuf, st, nx, ny, nz, metric, sources);
+    download(c, out, gbuf, V * sizeof(int));
+    sync(c);
     return SPK_OK;
   });
 }
