@@ -268,6 +268,12 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  "pad_y"/"pad_z" padded FFT sizes (0: smallest 2^a3^b5^c7^d).
  *   "corr_chunk"   orientations per pass-A block (0: automatic).
  *   "feas_group"   orientation pairs per feasibility launch pair (0: all).
+ *   "inc_dist"     1 (default): after a placement the distance field is
+ *                  updated from the placed voxels alone (the field is a
+ *                  minimum over sources: Phi = min(Phi, transform of the new
+ *                  sources), computed on the item's box and one pass over the
+ *                  tray), when the field is this library's own and the item's
+ *                  x extent is at most 96; 0: the full transform every time.
  *   "dist_y"       distance y pass: 0 outward search, 1 Felzenszwalb
  *                  envelope, 2 full scan (default); "dist_x": 0 outward
  *                  search (default), 2 full scan.

@@ -208,6 +208,7 @@ struct spk_ctx {
                                          // with the column test before proposing
   int dist_y = 2;                        // option "dist_y": 1 envelope, 2 full scan
   int dist_x = 0;                        // option "dist_x": 0 outward search, 2 full scan
+  bool inc_dist = true;                  // option "inc_dist": incremental field after a placement
   // column test before the feasibility pass (option "coarse")
   struct Coarse {
     bool on = true;
@@ -577,11 +578,53 @@ void build_spectra(spk_ctx* c, const int* ctl = nullptr) {
   build_coarse_spectra(c, ctl);
 }
 
+// The placement a refresh follows, for the incremental distance update: the
+// device's search outcome (so, with od the searched orientations) or an
+// explicit orientation od[orient] at (qx, qy, qz); id as k_place takes it
+// (the loop adds the placed count on the device).
+struct PlaceRef {
+  const SearchOut* so = nullptr;
+  const OrientDesc* od = nullptr;
+  int orient = 0, qx = 0, qy = 0, qz = 0, id = 0;
+  int max_ex = 0;   // largest x extent among the orientations od may name
+};
+
+// The incremental update applies when the field is this library's own
+// transform of the tray (not a caller's) and the item's x extent fits its
+// candidate buffer.
+bool inc_applies(const spk_ctx* c, const PlaceRef* pr) {
+  return pr && c->inc_dist && c->phi_internal && pr->max_ex <= kIncMaxEx &&
+         (long long)c->ny * c->nz <= 32768 && c->nx <= 1024;
+}
+
+// Phi = min(Phi, transform of the placed item's voxels) and the statistics
+// of the result (k_inc_zy, k_inc_x); skipped on the device when the placement
+// adds no source.
+void run_distance_inc(spk_ctx* c, const PlaceRef& pr, const int* ctl) {
+  const int fill = c->nx + c->ny + c->nz + 10;
+  const int edt = c->metric == SPK_METRIC_EDT ? 1 : 0;
+  const int big = edt ? fill * fill : fill;
+  const size_t sh = (size_t)c->ny * c->nz * sizeof(int);
+  auto zy = edt ? k_inc_zy<1> : k_inc_zy<0>;
+  allow_shmem(zy, sh);
+  launch(c, "inc_zy", zy, dim3(std::min(pr.max_ex, c->nx)), dim3(kDistThreads), sh,
+         (const int*)c->tray.p, c->tmpi.p, c->stats.p, pr.so, pr.od, pr.orient, pr.qx, pr.qy,
+         pr.qz, pr.id, c->nx, c->ny, c->nz, c->sources, big, ctl);
+  auto xk = edt ? k_inc_x<1> : k_inc_x<0>;
+  launch(c, "inc_x", xk,
+         dim3((c->nz + kIncLines - 1) / kIncLines, c->ny, (c->nx + kIncChunk - 1) / kIncChunk),
+         dim3(256), 0, (const int*)c->tmpi.p, c->phi.p, c->stats.p, pr.so, pr.od, pr.orient,
+         pr.qx, pr.qy, pr.qz, pr.id, c->nx, c->ny, c->nz, c->sources, big, fill, ctl);
+}
+
 // Distance field (computed, or the caller's) + statistics + cached spectra.
 // With ctl (greedy loop) every kernel is skipped on the device when the last
-// item placed nothing or the loop was handed back to the host.
-void refresh_field_and_spectra(spk_ctx* c, const int* dist_host, const int* ctl = nullptr) {
+// item placed nothing or the loop was handed back to the host.  After a
+// placement (pr), the field is updated incrementally when that applies.
+void refresh_field_and_spectra(spk_ctx* c, const int* dist_host, const int* ctl = nullptr,
+                               const PlaceRef* pr = nullptr) {
   const size_t V = (size_t)c->nx * c->ny * c->nz;
+  const bool inc = !dist_host && inc_applies(c, pr);
   if (dist_host) c->phi_internal = false;
   else if (!ctl) c->phi_internal = true;   // (a gated refresh keeps the flag as it was)
   if (dist_host) {
@@ -590,6 +633,8 @@ void refresh_field_and_spectra(spk_ctx* c, const int* dist_host, const int* ctl 
     const int nb = (int)std::min<size_t>(1024, (V + 255) / 256);
     launch(c, "field_stats", k_field_stats, dim3(nb), dim3(256), 0, (const int*)c->phi.p,
            (long long)V, c->stats.p);
+  } else if (inc) {
+    run_distance_inc(c, *pr, ctl);
   } else {
     run_distance(c, c->tray.p, c->phi.p, c->tmpi.p, c->stats.p, c->nx, c->ny, c->nz, c->metric,
                  c->sources, ctl);
@@ -1147,7 +1192,13 @@ void do_place(spk_ctx* c, const OrientDesc& d, int x, int y, int z, int id, bool
   launch(c, "place", k_place, dim3(nb), dim3(256), 0, c->tray.p, (const int*)c->items.p,
          (const OrientDesc*)c->od.p, (const SearchOut*)nullptr, 0, x, y, z, id, c->nx, c->ny,
          c->nz, 1, c->flag.p, (const int*)nullptr);
-  if (refresh) refresh_field_and_spectra(c, nullptr);
+  if (refresh) {
+    PlaceRef pr;
+    pr.od = c->od.p;
+    pr.qx = x; pr.qy = y; pr.qz = z; pr.id = id;
+    pr.max_ex = d.ex;
+    refresh_field_and_spectra(c, nullptr, nullptr, &pr);
+  }
 }
 
 // Orientations of every item that fit the tray, in processing order.
@@ -1300,7 +1351,12 @@ void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e
   launch(c, "place", k_place, dim3(nb), dim3(256), 0, c->tray.p, (const int*)c->items.p, od,
          (const SearchOut*)c->sout.p, 0, 0, 0, 0, (int)id_offset, c->nx, c->ny, c->nz, 1,
          c->zero.p, (const int*)c->ctl.p);
-  refresh_field_and_spectra(c, nullptr, c->ctl.p);
+  PlaceRef pr;
+  pr.so = c->sout.p;
+  pr.od = od;
+  pr.id = id_offset;
+  pr.max_ex = max_ex;
+  refresh_field_and_spectra(c, nullptr, c->ctl.p, &pr);
 }
 
 struct ExactBest {
@@ -1434,6 +1490,7 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
     for (auto& hr : host_rows)
       std::copy(hr.second.begin(), hr.second.end(), rec_scores + (size_t)hr.first * n_orient);
   }
+  if (placed > 0) c->phi_internal = true;   // a placement recomputed the field on the device
   return placed;
 }
 
@@ -1827,6 +1884,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   }
   if (counts)
     for (int i = 0; i < 5; ++i) counts[i] = cnt[i];
+  if (placed > 0) c->phi_internal = true;   // a placement recomputed the field on the device
   return placed;
 }
 
@@ -2771,6 +2829,10 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     if (v != 4 && v != 8 && v != 16 && v != 32 && v != 64)
       return fail(SPK_EINVAL, "fine_group must be 4, 8, 16, 32 or 64");
     c->co.fine_group = v;
+    return SPK_OK;
+  }
+  if (std::string(name) == "inc_dist") {
+    c->inc_dist = value != 0.0;
     return SPK_OK;
   }
   if (std::string(name) == "dist_x") {

@@ -517,6 +517,243 @@ k_dist_x(const int* __restrict__ in, int* __restrict__ phi, FieldStats* __restri
   }
 }
 
+// ---------------------------------------------------------------------------
+// Incremental distance field after a placement.
+//
+// A placement only adds sources: with sources "nonzero" every voxel the item
+// wrote, with sources "eq1" the voxels of id 1 (so only the placement of id 1
+// adds any).  The field is a minimum over sources, and rounding
+// (rint(sqrt()) of the squared Euclidean distance) and the fill cap are
+// monotone, so  Phi_new = min(Phi_old, D)  with D the transform of the new
+// sources alone.  The new sources lie in the item's occupied box
+// [qx, qx+hx] x [qy, qy+hy] x [qz, qz+hz], which makes D separable with short
+// candidate ranges:
+//   k_inc_zy  one block per slab x in [qx, qx+hx]: z pass on the item's
+//             columns, y pass over its hy+1 rows -> g[x-qx][y][z], all (y,z);
+//   k_inc_x   per (y, z) line: the lower envelope of the hx+1 candidates
+//             along x (L1: prefix/suffix minima), evaluated at every x,
+//             minimum with Phi_old, and the field statistics of the result.
+// Both take the placement as k_place does (the device's search outcome, or
+// an explicit orientation and offset) and skip when it adds no source.
+// ---------------------------------------------------------------------------
+constexpr int kIncMaxEx = 96;        // largest x extent of an item updated this way
+constexpr int kIncLines = 64;        // (y, z) lines per k_inc_x block (one per lane)
+constexpr int kIncChunk = 32;        // x values per k_inc_x block
+
+struct PlacedBox {
+  int qx, qy, qz;                    // offset of the oriented item
+  int hx, hy, hz;                    // its occupied box, relative to the offset
+  int id;                            // id written by the placement
+};
+
+// false when the placement adds no source (nothing placed, empty item, or a
+// placement that is not id 1 under sources "eq1")
+__device__ __forceinline__ bool placed_box(const SearchOut* so, const OrientDesc* od, int orient,
+                                           int qx, int qy, int qz, int id, int ny, int nz,
+                                           int sources, const int* ctl, PlacedBox& b) {
+  if (ctl) id += ctl[CTL_PLACED];
+  OrientDesc d;
+  if (so) {
+    const SearchOut s = *so;
+    if (!s.found) return false;
+    d = od[s.orient];
+    qz = s.idx % nz; qy = (s.idx / nz) % ny; qx = s.idx / (ny * nz);
+  } else {
+    d = od[orient];
+  }
+  if (d.hx < 0 || d.hy < 0 || d.hz < 0) return false;
+  if (!sources && id != 1) return false;
+  b.qx = qx; b.qy = qy; b.qz = qz;
+  b.hx = d.hx; b.hy = d.hy; b.hz = d.hz;
+  b.id = id;
+  return true;
+}
+
+// grid (kIncMaxEx), kDistThreads threads; dynamic LDS (hy+1)*nz ints <= ny*nz.
+// g: [hx+1][ny][nz] ints, >= big where a slab has no new source.
+template <int EDT>
+__global__ void __launch_bounds__(kDistThreads)
+k_inc_zy(const int* __restrict__ tray, int* __restrict__ g, FieldStats* __restrict__ st,
+         const SearchOut* __restrict__ so, const OrientDesc* __restrict__ od, int orient, int qx,
+         int qy, int qz, int id, int nx, int ny, int nz, int sources, int big,
+         const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
+  PlacedBox b;
+  if (!placed_box(so, od, orient, qx, qy, qz, id, ny, nz, sources, ctl, b)) return;
+  const int r = blockIdx.x;
+  if (r > b.hx || b.qx + r >= nx) return;
+  if (r == 0 && threadIdx.x == 0) {   // reduced into by the following k_inc_x
+    st->sum = 0ULL;
+    st->mn = INT_MAX;
+    st->mx = INT_MIN;
+    st->mxw = INT_MIN;
+  }
+  extern __shared__ int g1[];          // [hy+1][nz]: distance along z to the column's sources
+  const int x = b.qx + r, tid = threadIdx.x;
+  const int ry = min(b.hy + 1, ny - b.qy), rz = min(b.hz + 1, nz - b.qz);
+  const int* slab = tray + (size_t)x * ny * nz;
+  for (int e = tid; e < ry * nz; e += kDistThreads) {
+    const int jy = e / nz, k = e - jy * nz;
+    const int* col = slab + (size_t)(b.qy + jy) * nz + b.qz;
+    // nearest new source in this column: outward from k, clipped to the box
+    int best = big;
+    for (int t = 0; t < rz; ++t) {
+      if (col[t] == b.id) {
+        const int c = dcost(k - (b.qz + t), EDT);
+        best = c < best ? c : best;
+      }
+    }
+    g1[e] = best;
+  }
+  __syncthreads();
+  int* out = g + (size_t)r * ny * nz;
+  for (int e = tid; e < ny * nz; e += kDistThreads) {
+    const int j = e / nz, k = e - j * nz;
+    int best = big;
+    for (int jy = 0; jy < ry; ++jy) {
+      const int v = g1[jy * nz + k];
+      if (v >= big) continue;
+      const int c = v + dcost(j - (b.qy + jy), EDT);
+      best = c < best ? c : best;
+    }
+    out[e] = best;
+  }
+}
+
+// grid (ceil(nz / kIncLines), ny, ceil(nx / kIncChunk)), 256 threads: lane l
+// of every wave owns line (y = blockIdx.y, z = z0 + l); wave w evaluates x in
+// its quarter of the block's chunk.  LDS: kIncLines * kIncMaxEx int2.
+template <int EDT>
+__global__ void __launch_bounds__(256)
+k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict__ st,
+        const SearchOut* __restrict__ so, const OrientDesc* __restrict__ od, int orient, int qx,
+        int qy, int qz, int id, int nx, int ny, int nz, int sources, int big, int fill,
+        const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
+  PlacedBox b;
+  if (!placed_box(so, od, orient, qx, qy, qz, id, ny, nz, sources, ctl, b)) return;
+  __shared__ int2 env[kIncMaxEx][kIncLines];   // [entry][line]: apex (x, F) or (A, B) minima
+  __shared__ int nenv[kIncLines];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int y = blockIdx.y, z = blockIdx.x * kIncLines + lane;
+  const int m = min(b.hx + 1, nx - b.qx);      // candidate slabs
+  const bool line_ok = z < nz;
+  if (wave == 0 && line_ok) {
+    const int* gl = g + (size_t)y * nz + z;
+    const size_t gs = (size_t)ny * nz;
+    if (EDT) {
+      // lower envelope of the parabolas f(p) + (x - p)^2, apexes in order of p
+      int k = -1;
+      for (int r = 0; r < m; ++r) {
+        const int f = gl[r * gs];
+        if (f >= big) continue;
+        const int p = b.qx + r;
+        const int F = f + p * p;
+        while (k >= 1) {
+          const int2 a = env[k - 1][lane], c = env[k][lane];
+          if ((long long)(F - c.y) * (c.x - a.x) <= (long long)(c.y - a.y) * (p - c.x)) --k;
+          else break;
+        }
+        ++k;
+        env[k][lane] = make_int2(p, F);
+      }
+      nenv[lane] = k + 1;
+    } else {
+      // A[r] = min over candidates <= r of f - p, B[r] = min over >= r of f + p
+      int a = INT_MAX / 2;
+      for (int r = 0; r < m; ++r) {
+        const int f = gl[r * gs];
+        const int p = b.qx + r;
+        if (f < big) a = min(a, f - p);
+        env[r][lane].x = a;
+      }
+      int c = INT_MAX / 2;
+      for (int r = m - 1; r >= 0; --r) {
+        const int f = gl[r * gs];
+        const int p = b.qx + r;
+        if (f < big) c = min(c, f + p);
+        env[r][lane].y = c;
+      }
+      nenv[lane] = m;
+    }
+  }
+  __syncthreads();
+  long long lsum = 0;
+  int lmn = INT_MAX, lmx = INT_MIN, lmw = INT_MIN;
+  if (line_ok) {
+    const int n = nenv[lane];
+    const int per = kIncChunk / 4;
+    const int x0 = blockIdx.z * kIncChunk + wave * per;
+    const int x1 = min(x0 + per, nx);
+    const int wy = min(y + 1, ny - y), wz = min(z + 1, nz - z);
+    int j = 0;
+    for (int x = x0; x < x1; ++x) {
+      int best = big;
+      if (EDT) {
+        if (n > 0) {
+          // advance while the next apex's parabola is lower at x
+          while (j + 1 < n) {
+            const int2 a = env[j][lane], c = env[j + 1][lane];
+            if ((long long)(c.y - a.y) <= 2LL * x * (c.x - a.x)) ++j;
+            else break;
+          }
+          const int2 a = env[j][lane];
+          best = a.y - a.x * a.x + (x - a.x) * (x - a.x);
+        }
+      } else if (n > 0) {
+        // candidates at or left of x give f - p + x, at or right give f + p - x
+        const int r = x - b.qx;
+        int lo = INT_MAX / 2, hi = INT_MAX / 2;
+        if (r >= 0) lo = env[min(r, n - 1)][lane].x;
+        if (r < n) hi = env[max(r, 0)][lane].y;
+        const long long v = min((long long)lo + x, (long long)hi - x);
+        best = v < big ? (int)v : big;
+      }
+      int d;
+      if (EDT) d = best >= fill * fill ? fill : (int)rint(sqrt((double)best));
+      else d = best;
+      d = d > fill ? fill : d;
+      int* pv = phi + ((size_t)x * ny + y) * nz + z;
+      const int old = *pv;
+      const int v = d < old ? d : old;
+      if (v != old) *pv = v;
+      lsum += v;
+      lmn = v < lmn ? v : lmn;
+      lmx = v > lmx ? v : lmx;
+      const int w = min(min(wy, wz), min(x + 1, nx - x));
+      const int vw = v < w ? v : w;
+      lmw = vw > lmw ? vw : lmw;
+    }
+  }
+  __shared__ long long rs[4];
+  __shared__ int rmn[4], rmx[4], rmw[4];
+  for (int o = 32; o > 0; o >>= 1) {
+    lsum += __shfl_xor(lsum, o);
+    const int a = __shfl_xor(lmn, o), c = __shfl_xor(lmx, o), e = __shfl_xor(lmw, o);
+    lmn = a < lmn ? a : lmn;
+    lmx = c > lmx ? c : lmx;
+    lmw = e > lmw ? e : lmw;
+  }
+  if (lane == 0) {
+    rs[wave] = lsum; rmn[wave] = lmn; rmx[wave] = lmx; rmw[wave] = lmw;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    long long sum = 0;
+    int a = INT_MAX, c = INT_MIN, e = INT_MIN;
+    for (int w = 0; w < 4; ++w) {
+      sum += rs[w];
+      a = rmn[w] < a ? rmn[w] : a;
+      c = rmx[w] > c ? rmx[w] : c;
+      e = rmw[w] > e ? rmw[w] : e;
+    }
+    atomicAdd(&st->sum, (unsigned long long)sum);
+    atomicMin(&st->mn, a);
+    atomicMax(&st->mx, c);
+    atomicMax(&st->mxw, e);
+  }
+}
+
 // stats of a caller-provided distance field
 __global__ void __launch_bounds__(256)
 k_field_stats(const int* __restrict__ phi, long long n, FieldStats* __restrict__ st) {

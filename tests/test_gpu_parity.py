@@ -656,3 +656,39 @@ def test_feasibility_flags_large_footprint(P):
             big += g.shape[0] * g.shape[1] > 4096
             feasible += bool(exp)
     assert big > 0 and feasible > 0 and feasible < checked
+
+
+@pytest.mark.parametrize("metric,sources", [("edt", "nonzero"), ("l1", "nonzero"), ("l1", "eq1"),
+                                            ("edt", "eq1")])
+def test_incremental_distance_matches_oracle(P, metric, sources):
+    """After each placement the context's field (updated from the placed
+    voxels alone, option inc_dist on by default) equals the oracle's full
+    transform of the tray: items placed one pack call at a time by the
+    device loop, then by explicit placements on the host path."""
+    from psacking_amd import workloads as W
+    from psacking_amd.rotations import orientation_maps
+    w = W.blobs(seed=24, n=8, tray=(50, 44, 36))
+    vols = [int((it > 0).sum()) for it in w.items]
+    order = [int(i) for i in np.argsort(vols)[::-1]]
+    perms, flips = orientation_maps(24)
+    s = P.Session(0)
+    s.set_tray(np.zeros(w.tray_size, np.int32), metric, sources, 0)
+    placed = 0
+    say("incremental field, device loop", metric, sources)
+    for i in order[:5]:
+        r = s.pack([w.items[i]], np.zeros(1, np.int32), perms, flips, "cube", False, placed)[0]
+        placed += int(r["found"])
+        np.testing.assert_array_equal(s.distance(), O.distance_field(s.tray(), metric, sources))
+    assert placed >= 3
+    say("incremental field, explicit placements", metric, sources)
+    t = s.tray()
+    for i in order[5:]:
+        item = w.items[i]
+        res = O.search_state(O.TrayState(t, O.distance_field(t, metric, sources)), item, "cube")
+        if not res.found:
+            continue
+        placed += 1
+        s.place(item, perms[0], flips[0], res.position, placed)
+        t = O.place_in_tray(item, t, res.position, placed)
+        np.testing.assert_array_equal(s.tray(), t)
+        np.testing.assert_array_equal(s.distance(), O.distance_field(t, metric, sources))
