@@ -587,14 +587,23 @@ struct PlaceRef {
   const OrientDesc* od = nullptr;
   int orient = 0, qx = 0, qy = 0, qz = 0, id = 0;
   int max_ex = 0;   // largest x extent among the orientations od may name
+  int max_dim = 0;  // largest extent along any axis among them (0: unknown)
 };
+
+// k_inc_zy's LDS: the item box's column flags (rows x z extent) and the
+// z-pass costs (rows x nz), rows and z extent bounded by the item's extents
+size_t inc_zy_shmem(const spk_ctx* c, const PlaceRef& pr) {
+  const size_t ry = pr.max_dim > 0 ? std::min(pr.max_dim, c->ny) : c->ny;
+  const size_t rz = pr.max_dim > 0 ? std::min(pr.max_dim, c->nz) : c->nz;
+  return (ry * rz + ry * c->nz) * sizeof(int);
+}
 
 // The incremental update applies when the field is this library's own
 // transform of the tray (not a caller's) and the item's x extent fits its
 // candidate buffer.
 bool inc_applies(const spk_ctx* c, const PlaceRef* pr) {
-  return pr && c->inc_dist && c->phi_internal && pr->max_ex <= kIncMaxEx &&
-         (long long)c->ny * c->nz <= 32768 && c->nx <= 1024;
+  return pr && c->inc_dist && c->phi_internal && pr->max_ex <= kIncMaxEx && c->nx <= 1024 &&
+         inc_zy_shmem(c, *pr) <= 160 * 1024;
 }
 
 // Phi = min(Phi, transform of the placed item's voxels) and the statistics
@@ -604,7 +613,7 @@ void run_distance_inc(spk_ctx* c, const PlaceRef& pr, const int* ctl) {
   const int fill = c->nx + c->ny + c->nz + 10;
   const int edt = c->metric == SPK_METRIC_EDT ? 1 : 0;
   const int big = edt ? fill * fill : fill;
-  const size_t sh = (size_t)c->ny * c->nz * sizeof(int);
+  const size_t sh = inc_zy_shmem(c, pr);
   auto zy = edt ? k_inc_zy<1> : k_inc_zy<0>;
   allow_shmem(zy, sh);
   launch(c, "inc_zy", zy, dim3(std::min(pr.max_ex, c->nx)), dim3(kDistThreads), sh,
@@ -1197,6 +1206,7 @@ void do_place(spk_ctx* c, const OrientDesc& d, int x, int y, int z, int id, bool
     pr.od = c->od.p;
     pr.qx = x; pr.qy = y; pr.qz = z; pr.id = id;
     pr.max_ex = d.ex;
+    pr.max_dim = std::max(d.ex, std::max(d.ey, d.ez));
     refresh_field_and_spectra(c, nullptr, nullptr, &pr);
   }
 }
@@ -1204,6 +1214,7 @@ void do_place(spk_ctx* c, const OrientDesc& d, int x, int y, int z, int id, bool
 // Orientations of every item that fit the tray, in processing order.
 struct ItemPlan {
   int od_off = 0, m = 0, max_ex = 1, max_exy = 1, max_e = 1;
+  int max_dim = 1;        // largest extent along any axis of its fitting orientations
   std::vector<int> map;   // fitting index -> orientation index
 };
 
@@ -1254,6 +1265,7 @@ PackPlan plan_pack(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
       pl.max_ex = std::max(pl.max_ex, d.ex);
       pl.max_exy = std::max(pl.max_exy, d.ex * d.ey);
       pl.max_e = std::max(pl.max_e, d.ex * d.ey * d.ez);
+      pl.max_dim = std::max(pl.max_dim, std::max(d.ex, std::max(d.ey, d.ez)));
     }
     pl.m = (int)pl.map.size();
     ws_m = std::max(ws_m, pl.m);
@@ -1321,7 +1333,8 @@ const FieldStats* item_bounds(spk_ctx* c, const std::vector<HostItem>& bases,
 void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_exy, int max_e,
                   int penalty, int t, int32_t id_offset, int halt_not_found, double* rec_row,
                   const int* rec_map, int feas_mode = -1,
-                  const FieldStats* item_bound = nullptr, bool known_feasible = false) {
+                  const FieldStats* item_bound = nullptr, bool known_feasible = false,
+                  int max_dim = 0) {
   if (feas_mode < 0) feas_mode = c->feas_mode;
   const bool feas = feas_mode && !c->split;
   if (feas || item_bound || c->co.ready) {
@@ -1356,6 +1369,7 @@ void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e
   pr.od = od;
   pr.id = id_offset;
   pr.max_ex = max_ex;
+  pr.max_dim = max_dim;
   refresh_field_and_spectra(c, nullptr, c->ctl.p, &pr);
 }
 
@@ -1445,7 +1459,8 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
       const OrientDesc* od = c->od_all.p + pl.od_off;
       enqueue_step(c, od, pl.m, pl.max_ex, pl.max_exy, pl.max_e, penalty, t, id_offset, 0,
                    rec_scores ? c->rscores.p + (size_t)t * n_orient : nullptr,
-                   c->map_all.p + pl.od_off, -1, bounds ? bounds + t : nullptr);
+                   c->map_all.p + pl.od_off, -1, bounds ? bounds + t : nullptr, false,
+                   pl.max_dim);
     }
     int ctl[CTL_WORDS];
     std::vector<PackOut> po(n_items - t0);
@@ -1789,7 +1804,8 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     if (use_prop && proposals)
       for (size_t j = 0; j < pick.size(); ++j) proposals[(size_t)t * topk + j] = pl.map[pick[j]];
     enqueue_step(c, od, (int)pick.size(), max_ex, pl.max_exy, max_e, penalty, t, id_offset,
-                 use_prop ? 1 : 0, nullptr, nullptr, 0, nullptr, screened && c->co.ready);
+                 use_prop ? 1 : 0, nullptr, nullptr, 0, nullptr, screened && c->co.ready,
+                 pl.max_dim);
     tpool_valid = false;
     // the next searched item's logits, on the tray this step leaves
     int tn = t + 1;
@@ -1842,7 +1858,7 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       if (!rest.empty()) {
         const OrientDesc* odr = gather(pl, rest, c->od_rest.p, &max_ex, &max_e);
         enqueue_step(c, odr, (int)rest.size(), max_ex, pl.max_exy, max_e, penalty, t, id_offset, 0,
-                     nullptr, nullptr, 0);
+                     nullptr, nullptr, 0, nullptr, false, pl.max_dim);
         tpool_valid = false;
         {
           const Mailbox& mb = post_and_sync(c, c->ctl.p, c->pout.p + t, c->stats.p, nullptr,

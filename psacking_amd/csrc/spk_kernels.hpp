@@ -569,7 +569,8 @@ __device__ __forceinline__ bool placed_box(const SearchOut* so, const OrientDesc
   return true;
 }
 
-// grid (kIncMaxEx), kDistThreads threads; dynamic LDS (hy+1)*nz ints <= ny*nz.
+// grid (kIncMaxEx), kDistThreads threads.  Dynamic LDS: the item box's
+// column data (hy+1)*(hz+1) ints, then g1 (hy+1)*nz ints.
 // g: [hx+1][ny][nz] ints, >= big where a slab has no new source.
 template <int EDT>
 __global__ void __launch_bounds__(kDistThreads)
@@ -588,33 +589,47 @@ k_inc_zy(const int* __restrict__ tray, int* __restrict__ g, FieldStats* __restri
     st->mx = INT_MIN;
     st->mxw = INT_MIN;
   }
-  extern __shared__ int g1[];          // [hy+1][nz]: distance along z to the column's sources
+  extern __shared__ int sm[];
   const int x = b.qx + r, tid = threadIdx.x;
   const int ry = min(b.hy + 1, ny - b.qy), rz = min(b.hz + 1, nz - b.qz);
+  int* src = sm;                    // [ry][rz]: 1 where a new source lies
+  int* g1 = sm + ry * rz;           // [ry][nz]: cost along z to the column's nearest source
   const int* slab = tray + (size_t)x * ny * nz;
-  for (int e = tid; e < ry * nz; e += kDistThreads) {
-    const int jy = e / nz, k = e - jy * nz;
-    const int* col = slab + (size_t)(b.qy + jy) * nz + b.qz;
-    // nearest new source in this column: outward from k, clipped to the box
-    int best = big;
-    for (int t = 0; t < rz; ++t) {
-      if (col[t] == b.id) {
-        const int c = dcost(k - (b.qz + t), EDT);
-        best = c < best ? c : best;
-      }
-    }
-    g1[e] = best;
+  for (int e = tid; e < ry * rz; e += kDistThreads) {
+    const int jy = e / rz, t = e - jy * rz;
+    src[e] = slab[(size_t)(b.qy + jy) * nz + b.qz + t] == b.id ? 1 : 0;
   }
   __syncthreads();
+  // z pass, one thread per column: nearest source at or below, then at or above
+  for (int jy = tid; jy < ry; jy += kDistThreads) {
+    const int* cs = src + jy * rz;
+    int* row = g1 + jy * nz;
+    int last = -1;
+    for (int k = 0; k < nz; ++k) {
+      const int t = k - b.qz;
+      if (t >= 0 && t < rz && cs[t]) last = k;
+      row[k] = last >= 0 ? dcost(k - last, EDT) : big;
+    }
+    int next = -1;
+    for (int k = nz - 1; k >= 0; --k) {
+      const int t = k - b.qz;
+      if (t >= 0 && t < rz && cs[t]) next = k;
+      if (next >= 0) {
+        const int c = dcost(next - k, EDT);
+        row[k] = c < row[k] ? c : row[k];
+      }
+    }
+  }
+  __syncthreads();
+  // y pass over the item's rows
   int* out = g + (size_t)r * ny * nz;
   for (int e = tid; e < ny * nz; e += kDistThreads) {
     const int j = e / nz, k = e - j * nz;
     int best = big;
     for (int jy = 0; jy < ry; ++jy) {
       const int v = g1[jy * nz + k];
-      if (v >= big) continue;
       const int c = v + dcost(j - (b.qy + jy), EDT);
-      best = c < best ? c : best;
+      best = (v < big && c < best) ? c : best;
     }
     out[e] = best;
   }
@@ -622,7 +637,9 @@ k_inc_zy(const int* __restrict__ tray, int* __restrict__ g, FieldStats* __restri
 
 // grid (ceil(nz / kIncLines), ny, ceil(nx / kIncChunk)), 256 threads: lane l
 // of every wave owns line (y = blockIdx.y, z = z0 + l); wave w evaluates x in
-// its quarter of the block's chunk.  LDS: kIncLines * kIncMaxEx int2.
+// its quarter of the block's chunk.  LDS: the candidates of the block's lines
+// (loaded by every thread), then their envelopes (EDT: apex (x, F); L1:
+// prefix/suffix minima A, B).
 template <int EDT>
 __global__ void __launch_bounds__(256)
 k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict__ st,
@@ -632,20 +649,25 @@ k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict
   if (refresh_off(ctl)) return;
   PlacedBox b;
   if (!placed_box(so, od, orient, qx, qy, qz, id, ny, nz, sources, ctl, b)) return;
-  __shared__ int2 env[kIncMaxEx][kIncLines];   // [entry][line]: apex (x, F) or (A, B) minima
+  __shared__ int2 env[kIncMaxEx][kIncLines];
   __shared__ int nenv[kIncLines];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int y = blockIdx.y, z = blockIdx.x * kIncLines + lane;
+  const int y = blockIdx.y, z0 = blockIdx.x * kIncLines, z = z0 + lane;
   const int m = min(b.hx + 1, nx - b.qx);      // candidate slabs
   const bool line_ok = z < nz;
+  // every thread loads candidates: row r of lines z0.. (coalesced along z)
+  for (int e = tid; e < m * kIncLines; e += 256) {
+    const int rr = e / kIncLines, l = e - rr * kIncLines;
+    env[rr][l].x = (z0 + l < nz) ? g[((size_t)rr * ny + y) * nz + z0 + l] : big;
+  }
+  __syncthreads();
   if (wave == 0 && line_ok) {
-    const int* gl = g + (size_t)y * nz + z;
-    const size_t gs = (size_t)ny * nz;
     if (EDT) {
-      // lower envelope of the parabolas f(p) + (x - p)^2, apexes in order of p
+      // lower envelope of the parabolas f(p) + (x - p)^2, apexes in order of
+      // p, written over the candidate rows already read (k <= r)
       int k = -1;
       for (int r = 0; r < m; ++r) {
-        const int f = gl[r * gs];
+        const int f = env[r][lane].x;
         if (f >= big) continue;
         const int p = b.qx + r;
         const int F = f + p * p;
@@ -662,17 +684,17 @@ k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict
       // A[r] = min over candidates <= r of f - p, B[r] = min over >= r of f + p
       int a = INT_MAX / 2;
       for (int r = 0; r < m; ++r) {
-        const int f = gl[r * gs];
+        const int f = env[r][lane].x;
         const int p = b.qx + r;
         if (f < big) a = min(a, f - p);
-        env[r][lane].x = a;
+        env[r][lane].y = a;              // A, kept in .y until the second sweep
       }
       int c = INT_MAX / 2;
       for (int r = m - 1; r >= 0; --r) {
-        const int f = gl[r * gs];
+        const int f = env[r][lane].x;
         const int p = b.qx + r;
         if (f < big) c = min(c, f + p);
-        env[r][lane].y = c;
+        env[r][lane] = make_int2(env[r][lane].y, c);
       }
       nenv[lane] = m;
     }
@@ -682,16 +704,24 @@ k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict
   int lmn = INT_MAX, lmx = INT_MIN, lmw = INT_MIN;
   if (line_ok) {
     const int n = nenv[lane];
-    const int per = kIncChunk / 4;
+    constexpr int per = kIncChunk / 4;
     const int x0 = blockIdx.z * kIncChunk + wave * per;
-    const int x1 = min(x0 + per, nx);
     const int wy = min(y + 1, ny - y), wz = min(z + 1, nz - z);
+    int old[per];
+#pragma unroll
+    for (int i = 0; i < per; ++i) {   // independent loads first
+      const int x = x0 + i;
+      old[i] = x < nx ? phi[((size_t)x * ny + y) * nz + z] : 0;
+    }
     int j = 0;
-    for (int x = x0; x < x1; ++x) {
+#pragma unroll
+    for (int i = 0; i < per; ++i) {
+      const int x = x0 + i;
+      if (x >= nx) break;
       int best = big;
       if (EDT) {
         if (n > 0) {
-          // advance while the next apex's parabola is lower at x
+          // advance while the next apex's parabola is at most as high at x
           while (j + 1 < n) {
             const int2 a = env[j][lane], c = env[j + 1][lane];
             if ((long long)(c.y - a.y) <= 2LL * x * (c.x - a.x)) ++j;
@@ -713,10 +743,8 @@ k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict
       if (EDT) d = best >= fill * fill ? fill : (int)rint(sqrt((double)best));
       else d = best;
       d = d > fill ? fill : d;
-      int* pv = phi + ((size_t)x * ny + y) * nz + z;
-      const int old = *pv;
-      const int v = d < old ? d : old;
-      if (v != old) *pv = v;
+      const int v = d < old[i] ? d : old[i];
+      if (v != old[i]) phi[((size_t)x * ny + y) * nz + z] = v;
       lsum += v;
       lmn = v < lmn ? v : lmn;
       lmx = v > lmx ? v : lmx;
