@@ -273,7 +273,7 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  minimum over sources: Phi = min(Phi, transform of the new
  *                  sources), computed on the item's box and one pass over the
  *                  tray), when the field is this library's own and the item's
- *                  x extent is at most 96; 0: the full transform every time.
+ *                  x extent is at most 128; 0: the full transform every time.
  *   "dist_y"       distance y pass: 0 outward search, 1 Felzenszwalb
  *                  envelope, 2 full scan (default); "dist_x": 0 outward
  *                  search (default), 2 full scan.

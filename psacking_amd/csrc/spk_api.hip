@@ -162,6 +162,7 @@ struct spk_ctx {
   bool lds_attrs = false;
   DevBuf<int> tray, phi, tmpi;
   DevBuf<FieldStats> stats;
+  DevBuf<FieldStats> stats_part;         // per-block partial field statistics
   DevBuf<float2> tw, t1, t2, g1, g2;
   DevBuf<double> pen;          // [2][nz]: pow, cube
   // search workspace
@@ -617,13 +618,20 @@ void run_distance_inc(spk_ctx* c, const PlaceRef& pr, const int* ctl) {
   auto zy = edt ? k_inc_zy<1> : k_inc_zy<0>;
   allow_shmem(zy, sh);
   launch(c, "inc_zy", zy, dim3(std::min(pr.max_ex, c->nx)), dim3(kDistThreads), sh,
-         (const int*)c->tray.p, c->tmpi.p, c->stats.p, pr.so, pr.od, pr.orient, pr.qx, pr.qy,
-         pr.qz, pr.id, c->nx, c->ny, c->nz, c->sources, big, ctl);
+         (const int*)c->tray.p, c->tmpi.p, pr.so, pr.od, pr.orient, pr.qx, pr.qy, pr.qz, pr.id,
+         c->nx, c->ny, c->nz, c->sources, big, ctl);
+  const dim3 grid((c->nz + kIncLines - 1) / kIncLines, c->ny);
+  const int nb = (int)(grid.x * grid.y);
+  c->stats_part.ensure(nb);
   auto xk = edt ? k_inc_x<1> : k_inc_x<0>;
-  launch(c, "inc_x", xk,
-         dim3((c->nz + kIncLines - 1) / kIncLines, c->ny, (c->nx + kIncChunk - 1) / kIncChunk),
-         dim3(256), 0, (const int*)c->tmpi.p, c->phi.p, c->stats.p, pr.so, pr.od, pr.orient,
-         pr.qx, pr.qy, pr.qz, pr.id, c->nx, c->ny, c->nz, c->sources, big, fill, ctl);
+  const size_t xsh = (size_t)pr.max_ex * kIncLines * sizeof(int);
+  allow_shmem(xk, xsh);
+  launch(c, "inc_x", xk, grid, dim3(256), xsh,
+         (const int*)c->tmpi.p, c->phi.p, c->stats_part.p, pr.so, pr.od, pr.orient, pr.qx, pr.qy,
+         pr.qz, pr.id, c->nx, c->ny, c->nz, c->sources, big, fill, ctl);
+  launch(c, "inc_stats", k_inc_stats, dim3(1), dim3(1024), 0, (const FieldStats*)c->stats_part.p,
+         nb, c->stats.p, pr.so, pr.od, pr.orient, pr.qx, pr.qy, pr.qz, pr.id, c->ny, c->nz,
+         c->sources, ctl);
 }
 
 // Distance field (computed, or the caller's) + statistics + cached spectra.

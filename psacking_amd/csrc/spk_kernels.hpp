@@ -536,9 +536,8 @@ k_dist_x(const int* __restrict__ in, int* __restrict__ phi, FieldStats* __restri
 // Both take the placement as k_place does (the device's search outcome, or
 // an explicit orientation and offset) and skip when it adds no source.
 // ---------------------------------------------------------------------------
-constexpr int kIncMaxEx = 96;        // largest x extent of an item updated this way
+constexpr int kIncMaxEx = 128;       // largest x extent of an item updated this way
 constexpr int kIncLines = 64;        // (y, z) lines per k_inc_x block (one per lane)
-constexpr int kIncChunk = 32;        // x values per k_inc_x block
 
 struct PlacedBox {
   int qx, qy, qz;                    // offset of the oriented item
@@ -569,26 +568,30 @@ __device__ __forceinline__ bool placed_box(const SearchOut* so, const OrientDesc
   return true;
 }
 
+// rint(sqrt(d2)) for 0 <= d2 < 2^31 without double arithmetic: r is the
+// rounded root exactly when r^2 - r + 1 <= d2 <= r^2 + r (sqrt(d2) lies in
+// [r - 1/2, r + 1/2), ties are impossible for an integer d2); the float root
+// is within one of r and the two tests correct it.
+__device__ __forceinline__ int round_sqrt(int d2) {
+  long long r = (long long)rintf(sqrtf((float)d2));
+  if (r * r + r < d2) ++r;
+  if (r * r - r + 1 > d2) --r;
+  return (int)r;
+}
+
 // grid (kIncMaxEx), kDistThreads threads.  Dynamic LDS: the item box's
-// column data (hy+1)*(hz+1) ints, then g1 (hy+1)*nz ints.
+// source flags (hy+1)*(hz+1) ints, then g1 (hy+1)*nz ints.
 // g: [hx+1][ny][nz] ints, >= big where a slab has no new source.
 template <int EDT>
 __global__ void __launch_bounds__(kDistThreads)
-k_inc_zy(const int* __restrict__ tray, int* __restrict__ g, FieldStats* __restrict__ st,
-         const SearchOut* __restrict__ so, const OrientDesc* __restrict__ od, int orient, int qx,
-         int qy, int qz, int id, int nx, int ny, int nz, int sources, int big,
-         const int* __restrict__ ctl) {
+k_inc_zy(const int* __restrict__ tray, int* __restrict__ g, const SearchOut* __restrict__ so,
+         const OrientDesc* __restrict__ od, int orient, int qx, int qy, int qz, int id, int nx,
+         int ny, int nz, int sources, int big, const int* __restrict__ ctl) {
   if (refresh_off(ctl)) return;
   PlacedBox b;
   if (!placed_box(so, od, orient, qx, qy, qz, id, ny, nz, sources, ctl, b)) return;
   const int r = blockIdx.x;
   if (r > b.hx || b.qx + r >= nx) return;
-  if (r == 0 && threadIdx.x == 0) {   // reduced into by the following k_inc_x
-    st->sum = 0ULL;
-    st->mn = INT_MAX;
-    st->mx = INT_MIN;
-    st->mxw = INT_MIN;
-  }
   extern __shared__ int sm[];
   const int x = b.qx + r, tid = threadIdx.x;
   const int ry = min(b.hy + 1, ny - b.qy), rz = min(b.hz + 1, nz - b.qz);
@@ -600,25 +603,16 @@ k_inc_zy(const int* __restrict__ tray, int* __restrict__ g, FieldStats* __restri
     src[e] = slab[(size_t)(b.qy + jy) * nz + b.qz + t] == b.id ? 1 : 0;
   }
   __syncthreads();
-  // z pass, one thread per column: nearest source at or below, then at or above
-  for (int jy = tid; jy < ry; jy += kDistThreads) {
+  // z pass over the item's columns
+  for (int e = tid; e < ry * nz; e += kDistThreads) {
+    const int jy = e / nz, k = e - jy * nz;
     const int* cs = src + jy * rz;
-    int* row = g1 + jy * nz;
-    int last = -1;
-    for (int k = 0; k < nz; ++k) {
-      const int t = k - b.qz;
-      if (t >= 0 && t < rz && cs[t]) last = k;
-      row[k] = last >= 0 ? dcost(k - last, EDT) : big;
+    int best = big;
+    for (int t = 0; t < rz; ++t) {
+      const int c = dcost(k - (b.qz + t), EDT);
+      best = (cs[t] && c < best) ? c : best;
     }
-    int next = -1;
-    for (int k = nz - 1; k >= 0; --k) {
-      const int t = k - b.qz;
-      if (t >= 0 && t < rz && cs[t]) next = k;
-      if (next >= 0) {
-        const int c = dcost(next - k, EDT);
-        row[k] = c < row[k] ? c : row[k];
-      }
-    }
+    g1[e] = best;
   }
   __syncthreads();
   // y pass over the item's rows
@@ -635,116 +629,58 @@ k_inc_zy(const int* __restrict__ tray, int* __restrict__ g, FieldStats* __restri
   }
 }
 
-// grid (ceil(nz / kIncLines), ny, ceil(nx / kIncChunk)), 256 threads: lane l
-// of every wave owns line (y = blockIdx.y, z = z0 + l); wave w evaluates x in
-// its quarter of the block's chunk.  LDS: the candidates of the block's lines
-// (loaded by every thread), then their envelopes (EDT: apex (x, F); L1:
-// prefix/suffix minima A, B).
+// grid (ceil(nz / kIncLines), ny), 256 threads: lane l of every wave owns
+// line (y = blockIdx.y, z = z0 + l); wave w evaluates x = w, w + 4, ...
+// Every output is the minimum over the hx+1 candidate slabs (staged in LDS,
+// dynamic: (hx+1) * kIncLines ints); an x whose distance to the candidate
+// range alone reaches the old value cannot lower it and is skipped.
+// part[block] receives the block's field statistics (k_inc_stats reduces).
 template <int EDT>
 __global__ void __launch_bounds__(256)
-k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict__ st,
+k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict__ part,
         const SearchOut* __restrict__ so, const OrientDesc* __restrict__ od, int orient, int qx,
         int qy, int qz, int id, int nx, int ny, int nz, int sources, int big, int fill,
         const int* __restrict__ ctl) {
   if (refresh_off(ctl)) return;
   PlacedBox b;
   if (!placed_box(so, od, orient, qx, qy, qz, id, ny, nz, sources, ctl, b)) return;
-  __shared__ int2 env[kIncMaxEx][kIncLines];
-  __shared__ int nenv[kIncLines];
+  extern __shared__ int cand[];     // [r][line]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int y = blockIdx.y, z0 = blockIdx.x * kIncLines, z = z0 + lane;
   const int m = min(b.hx + 1, nx - b.qx);      // candidate slabs
-  const bool line_ok = z < nz;
-  // every thread loads candidates: row r of lines z0.. (coalesced along z)
   for (int e = tid; e < m * kIncLines; e += 256) {
     const int rr = e / kIncLines, l = e - rr * kIncLines;
-    env[rr][l].x = (z0 + l < nz) ? g[((size_t)rr * ny + y) * nz + z0 + l] : big;
-  }
-  __syncthreads();
-  if (wave == 0 && line_ok) {
-    if (EDT) {
-      // lower envelope of the parabolas f(p) + (x - p)^2, apexes in order of
-      // p, written over the candidate rows already read (k <= r)
-      int k = -1;
-      for (int r = 0; r < m; ++r) {
-        const int f = env[r][lane].x;
-        if (f >= big) continue;
-        const int p = b.qx + r;
-        const int F = f + p * p;
-        while (k >= 1) {
-          const int2 a = env[k - 1][lane], c = env[k][lane];
-          if ((long long)(F - c.y) * (c.x - a.x) <= (long long)(c.y - a.y) * (p - c.x)) --k;
-          else break;
-        }
-        ++k;
-        env[k][lane] = make_int2(p, F);
-      }
-      nenv[lane] = k + 1;
-    } else {
-      // A[r] = min over candidates <= r of f - p, B[r] = min over >= r of f + p
-      int a = INT_MAX / 2;
-      for (int r = 0; r < m; ++r) {
-        const int f = env[r][lane].x;
-        const int p = b.qx + r;
-        if (f < big) a = min(a, f - p);
-        env[r][lane].y = a;              // A, kept in .y until the second sweep
-      }
-      int c = INT_MAX / 2;
-      for (int r = m - 1; r >= 0; --r) {
-        const int f = env[r][lane].x;
-        const int p = b.qx + r;
-        if (f < big) c = min(c, f + p);
-        env[r][lane] = make_int2(env[r][lane].y, c);
-      }
-      nenv[lane] = m;
-    }
+    cand[e] = (z0 + l < nz) ? g[((size_t)rr * ny + y) * nz + z0 + l] : big;
   }
   __syncthreads();
   long long lsum = 0;
   int lmn = INT_MAX, lmx = INT_MIN, lmw = INT_MIN;
-  if (line_ok) {
-    const int n = nenv[lane];
-    constexpr int per = kIncChunk / 4;
-    const int x0 = blockIdx.z * kIncChunk + wave * per;
+  if (z < nz) {
     const int wy = min(y + 1, ny - y), wz = min(z + 1, nz - z);
-    int old[per];
-#pragma unroll
-    for (int i = 0; i < per; ++i) {   // independent loads first
-      const int x = x0 + i;
-      old[i] = x < nx ? phi[((size_t)x * ny + y) * nz + z] : 0;
-    }
-    int j = 0;
-#pragma unroll
-    for (int i = 0; i < per; ++i) {
-      const int x = x0 + i;
-      if (x >= nx) break;
-      int best = big;
-      if (EDT) {
-        if (n > 0) {
-          // advance while the next apex's parabola is at most as high at x
-          while (j + 1 < n) {
-            const int2 a = env[j][lane], c = env[j + 1][lane];
-            if ((long long)(c.y - a.y) <= 2LL * x * (c.x - a.x)) ++j;
-            else break;
-          }
-          const int2 a = env[j][lane];
-          best = a.y - a.x * a.x + (x - a.x) * (x - a.x);
+    const int xl = b.qx, xh = b.qx + m - 1;
+#pragma unroll 4
+    for (int x = wave; x < nx; x += 4) {
+      int* pv = phi + ((size_t)x * ny + y) * nz + z;
+      const int old = *pv;
+      int v = old;
+      const int dx = x < xl ? xl - x : (x > xh ? x - xh : 0);
+      // the new field at x is at least dx (its distance to the item's slabs)
+      if (dx < old) {
+        int best = big;
+        for (int rr = 0; rr < m; ++rr) {
+          const int f = cand[rr * kIncLines + lane];
+          const int c = f + dcost(x - (b.qx + rr), EDT);
+          best = (f < big && c < best) ? c : best;
         }
-      } else if (n > 0) {
-        // candidates at or left of x give f - p + x, at or right give f + p - x
-        const int r = x - b.qx;
-        int lo = INT_MAX / 2, hi = INT_MAX / 2;
-        if (r >= 0) lo = env[min(r, n - 1)][lane].x;
-        if (r < n) hi = env[max(r, 0)][lane].y;
-        const long long v = min((long long)lo + x, (long long)hi - x);
-        best = v < big ? (int)v : big;
+        int d;
+        if (EDT) d = best >= fill * fill ? fill : round_sqrt(best);
+        else d = best;
+        d = d > fill ? fill : d;
+        if (d < old) {
+          v = d;
+          *pv = v;
+        }
       }
-      int d;
-      if (EDT) d = best >= fill * fill ? fill : (int)rint(sqrt((double)best));
-      else d = best;
-      d = d > fill ? fill : d;
-      const int v = d < old[i] ? d : old[i];
-      if (v != old[i]) phi[((size_t)x * ny + y) * nz + z] = v;
       lsum += v;
       lmn = v < lmn ? v : lmn;
       lmx = v > lmx ? v : lmx;
@@ -767,18 +703,62 @@ k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict
   }
   __syncthreads();
   if (tid == 0) {
-    long long sum = 0;
-    int a = INT_MAX, c = INT_MIN, e = INT_MIN;
+    FieldStats f;
+    f.sum = 0ULL;
+    f.mn = INT_MAX; f.mx = INT_MIN; f.mxw = INT_MIN; f.pad = 0;
     for (int w = 0; w < 4; ++w) {
-      sum += rs[w];
-      a = rmn[w] < a ? rmn[w] : a;
-      c = rmx[w] > c ? rmx[w] : c;
-      e = rmw[w] > e ? rmw[w] : e;
+      f.sum += (unsigned long long)rs[w];
+      f.mn = rmn[w] < f.mn ? rmn[w] : f.mn;
+      f.mx = rmx[w] > f.mx ? rmx[w] : f.mx;
+      f.mxw = rmw[w] > f.mxw ? rmw[w] : f.mxw;
     }
-    atomicAdd(&st->sum, (unsigned long long)sum);
-    atomicMin(&st->mn, a);
-    atomicMax(&st->mx, c);
-    atomicMax(&st->mxw, e);
+    part[blockIdx.y * gridDim.x + blockIdx.x] = f;
+  }
+}
+
+// the field statistics from k_inc_x's n block partials (one block of 1024)
+__global__ void __launch_bounds__(1024)
+k_inc_stats(const FieldStats* __restrict__ part, int n, FieldStats* __restrict__ st,
+            const SearchOut* __restrict__ so, const OrientDesc* __restrict__ od, int orient,
+            int qx, int qy, int qz, int id, int ny, int nz, int sources,
+            const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
+  PlacedBox b;
+  if (!placed_box(so, od, orient, qx, qy, qz, id, ny, nz, sources, ctl, b)) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  unsigned long long sum = 0ULL;
+  int mn = INT_MAX, mx = INT_MIN, mw = INT_MIN;
+  for (int i = tid; i < n; i += 1024) {
+    const FieldStats f = part[i];
+    sum += f.sum;
+    mn = f.mn < mn ? f.mn : mn;
+    mx = f.mx > mx ? f.mx : mx;
+    mw = f.mxw > mw ? f.mxw : mw;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sum += __shfl_xor(sum, o);
+    const int a = __shfl_xor(mn, o), c = __shfl_xor(mx, o), e = __shfl_xor(mw, o);
+    mn = a < mn ? a : mn;
+    mx = c > mx ? c : mx;
+    mw = e > mw ? e : mw;
+  }
+  __shared__ unsigned long long rs[16];
+  __shared__ int rmn[16], rmx[16], rmw[16];
+  if (lane == 0) {
+    rs[wave] = sum; rmn[wave] = mn; rmx[wave] = mx; rmw[wave] = mw;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    FieldStats f;
+    f.sum = 0ULL;
+    f.mn = INT_MAX; f.mx = INT_MIN; f.mxw = INT_MIN; f.pad = 0;
+    for (int w = 0; w < 16; ++w) {
+      f.sum += rs[w];
+      f.mn = rmn[w] < f.mn ? rmn[w] : f.mn;
+      f.mx = rmx[w] > f.mx ? rmx[w] : f.mx;
+      f.mxw = rmw[w] > f.mxw ? rmw[w] : f.mxw;
+    }
+    *st = f;
   }
 }
 

@@ -126,7 +126,9 @@ template <int MODE, bool VEC, int NT>
 __global__ void __launch_bounds__(256)
 k_igemm(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wm, const float* __restrict__ bias,
         ConvDesc d, int M, int N, int Kloop, int ldw, int relu, const bf16_t* __restrict__ mask,
-        bf16_t* __restrict__ Y, float* __restrict__ Yf, int kchunk, float* __restrict__ part) {
+        bf16_t* __restrict__ Y, float* __restrict__ Yf, int kchunk, float* __restrict__ part,
+        const int* __restrict__ gate) {
+  if (gate && *gate) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int mrow0 = (blockIdx.x * 4 + wave) * 32;
   const int nbase = blockIdx.y * 16 * NT;
@@ -240,7 +242,8 @@ k_igemm(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wm, const float
 // epilogue of a split-K forward: Y = relu(sum_z part[z] + bias)
 __global__ void k_split_finalize(const float* __restrict__ part, int ks, int M, int N,
                                  const float* __restrict__ bias, int relu, bf16_t* __restrict__ Y,
-                                 float* __restrict__ Yf) {
+                                 float* __restrict__ Yf, const int* __restrict__ gate) {
+  if (gate && *gate) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M * N) return;
   float v = 0.f;
@@ -348,7 +351,8 @@ __global__ void k_reduce_parts(const float* __restrict__ part, int nchunk, int c
 // 2x2x2 average pool of X [B][d0][d1][d2][C] (even sizes) into rows of a wider
 // buffer: out[b*ld + off + ((h0*h1n + h1)*h2n + h2)*C + c]
 __global__ void k_pool_fwd(const bf16_t* __restrict__ X, int B, int d0, int d1, int d2, int C,
-                           bf16_t* __restrict__ out, int ld, int off) {
+                           bf16_t* __restrict__ out, int ld, int off, const int* __restrict__ gate) {
+  if (gate && *gate) return;
   const int h0n = d0 / 2, h1n = d1 / 2, h2n = d2 / 2;
   const int per = h0n * h1n * h2n * C;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -425,14 +429,17 @@ __global__ void k_loss(const float* __restrict__ z, int zs, const float* __restr
 
 // compact logits: zc[r] = z[r*zs] (channel 0 of each scorer row)
 __global__ void k_logits_compact(const float* __restrict__ z, int zs, int rows,
-                                 float* __restrict__ zc) {
+                                 float* __restrict__ zc, const int* __restrict__ gate) {
+  if (gate && *gate) return;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r < rows) zc[r] = z[(size_t)r * zs];
 }
 
 // scorer input rows: concat[r] = [item row r (ni) | tray row r / R (nt)]
 __global__ void k_concat_rows(const bf16_t* __restrict__ item, int ni, const bf16_t* __restrict__ tray,
-                              int nt, int rows, int R, bf16_t* __restrict__ out) {
+                              int nt, int rows, int R, bf16_t* __restrict__ out,
+                              const int* __restrict__ gate) {
+  if (gate && *gate) return;
   const int w = ni + nt;
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)rows * w) return;
@@ -548,7 +555,8 @@ __global__ void k_item_feat(const int* __restrict__ item, const ItemMap* __restr
 // column heights: h[x][y] = 1 + max{z : tray > 0}, 0 for an empty column.
 // One wavefront per column: 64 heights per ballot, from the top down.
 __global__ void k_heightmap(const int* __restrict__ tray, int nx, int ny, int nz,
-                            int* __restrict__ h) {
+                            int* __restrict__ h, const int* __restrict__ gate) {
+  if (gate && *gate) return;
   const int lane = threadIdx.x & 63;
   const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (i >= nx * ny) return;
@@ -574,7 +582,8 @@ __device__ __forceinline__ void cell_range(int a, int n, int G, int& lo, int& hi
 // tray features [64][32][32][2]: channel 0 = occupied fraction of the cell,
 // channel 1 = fraction of the cell below its columns' heights (z < h[x][y])
 __global__ void k_tray_feat(const int* __restrict__ tray, const int* __restrict__ h, int nx, int ny,
-                            int nz, bf16_t* __restrict__ out) {
+                            int nz, bf16_t* __restrict__ out, const int* __restrict__ gate) {
+  if (gate && *gate) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= kTray0 * kTray1 * kTray2) return;
   const int c = i % kTray2, b = (i / kTray2) % kTray1, a = i / (kTray2 * kTray1);

@@ -78,6 +78,9 @@ struct spk_nn {
   DevBuf<bf16_t> wbf, ipool, tpool, concat, g_concat, g_tpool;
   DevBuf<bf16_t> act[NN_LAYERS], grad[NN_LAYERS];
   bool dirty = true;   // bf16 copies out of date
+  // device word that, when nonzero, makes the inference kernels exit at once
+  // (the device-driven learned loop skips items it does not propose for)
+  const int* gate = nullptr;
   long long step = 0;
 };
 
@@ -180,7 +183,7 @@ void nn_fwd(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* X,
   const bool vec = d.cin % 8 == 0;
 #define SPK_NN_FWD(V, T)                                                                         \
   nn_launch(s, spknn::k_igemm<spknn::MODE_FWD, V, T>, grid, dim3(256), 0, X, W, bias, d, M, N, \
-            d.Kp, d.Kp, l.relu, (const bf16_t*)nullptr, Y, Yf, kchunk, part)
+            d.Kp, d.Kp, l.relu, (const bf16_t*)nullptr, Y, Yf, kchunk, part, nn->gate)
   if (vec) {
     switch (NT) {
       case 1: SPK_NN_FWD(true, 1); break;
@@ -199,7 +202,7 @@ void nn_fwd(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* X,
 #undef SPK_NN_FWD
   if (part)
     nn_launch(s, spknn::k_split_finalize, dim3((M * N + 255) / 256), dim3(256), 0,
-              (const float*)part, ks, M, N, bias, l.relu, Y, Yf);
+              (const float*)part, ks, M, N, bias, l.relu, Y, Yf, nn->gate);
 }
 
 // input gradient of one layer: dX = (dY * W) masked by the layer input > 0
@@ -213,7 +216,7 @@ void nn_bwd_data(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_
 #define SPK_NN_BWD(T)                                                                            \
   nn_launch(s, spknn::k_igemm<spknn::MODE_BWD, true, T>, grid, dim3(256), 0, dY, Wt,            \
             (const float*)nullptr, d, M, N, d.Ktp, d.Ktp, 0, mask, dX, (float*)nullptr, d.Ktp,   \
-            (float*)nullptr)
+            (float*)nullptr, (const int*)nullptr)
   switch (NT) {
     case 1: SPK_NN_BWD(1); break;
     case 2: SPK_NN_BWD(2); break;
@@ -244,11 +247,11 @@ void nn_wgrad(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* 
 }
 
 void nn_pool_fwd(hipStream_t s, const NnLayer& l, int B, const bf16_t* X, bf16_t* out, int ld,
-                 int off) {
+                 int off, const int* gate) {
   const ConvDesc& d = l.d;
   const int n = B * (d.o0 / 2) * (d.o1 / 2) * (d.o2 / 2) * d.cout;
   nn_launch(s, spknn::k_pool_fwd, dim3((n + 255) / 256), dim3(256), 0, X, B, d.o0, d.o1, d.o2,
-            d.cout, out, ld, off);
+            d.cout, out, ld, off, gate);
 }
 
 void nn_pool_bwd(hipStream_t s, const NnLayer& l, int B, const bf16_t* g, int ld, int off,
@@ -264,7 +267,7 @@ void nn_item_branch(spk_nn* nn, hipStream_t s, const bf16_t* item, int n, bf16_t
   nn_fwd(nn, s, nn->L[NN_I1], n, item, nn->act[NN_I1].p, nullptr);
   nn_fwd(nn, s, nn->L[NN_I2], n, nn->act[NN_I1].p, nn->act[NN_I2].p, nullptr);
   nn_fwd(nn, s, nn->L[NN_I3], n, nn->act[NN_I2].p, nn->act[NN_I3].p, nullptr);
-  nn_pool_fwd(s, nn->L[NN_I3], n, nn->act[NN_I3].p, out, kNnItemPooled, 0);
+  nn_pool_fwd(s, nn->L[NN_I3], n, nn->act[NN_I3].p, out, kNnItemPooled, 0, nn->gate);
 }
 
 // tray branch: B tray grids -> pooled rows (pitch 768) in nn->tpool
@@ -272,7 +275,7 @@ void nn_tray_branch(spk_nn* nn, hipStream_t s, const bf16_t* tray, int B) {
   nn_fwd(nn, s, nn->L[NN_T1], B, tray, nn->act[NN_T1].p, nullptr);
   nn_fwd(nn, s, nn->L[NN_T2], B, nn->act[NN_T1].p, nn->act[NN_T2].p, nullptr);
   nn_fwd(nn, s, nn->L[NN_T3], B, nn->act[NN_T2].p, nn->act[NN_T3].p, nullptr);
-  nn_pool_fwd(s, nn->L[NN_T3], B, nn->act[NN_T3].p, nn->tpool.p, kNnTrayPooled, 0);
+  nn_pool_fwd(s, nn->L[NN_T3], B, nn->act[NN_T3].p, nn->tpool.p, kNnTrayPooled, 0, nn->gate);
 }
 
 // head on B records: rows [item row r | tray row r/24] -> F1 -> F2 -> logits
@@ -281,11 +284,11 @@ void nn_head(spk_nn* nn, hipStream_t s, const bf16_t* ipool, int B) {
   const size_t n = (size_t)rows * kNnConcat;
   nn_launch(s, spknn::k_concat_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ipool,
             kNnItemPooled, (const bf16_t*)nn->tpool.p, kNnTrayPooled, rows, kNnLogits,
-            nn->concat.p);
+            nn->concat.p, nn->gate);
   nn_fwd(nn, s, nn->L[NN_F1], rows, nn->concat.p, nn->act[NN_F1].p, nullptr);
   nn_fwd(nn, s, nn->L[NN_F2], rows, nn->act[NN_F1].p, nn->act[NN_F2].p, nn->out.p);
   nn_launch(s, spknn::k_logits_compact, dim3((rows + 255) / 256), dim3(256), 0,
-            (const float*)nn->out.p, kNnOut, rows, nn->logits.p);
+            (const float*)nn->out.p, kNnOut, rows, nn->logits.p, nn->gate);
 }
 
 // item [B*24 oriented item grids], tray [B grids] -> nn->logits [B][24]
@@ -343,11 +346,11 @@ void nn_item_features(hipStream_t s, const int* item, const spknn::ItemMap* maps
 }
 
 void nn_tray_features(hipStream_t s, const int* tray, int nx, int ny, int nz, int* heights,
-                      bf16_t* out) {
+                      bf16_t* out, const int* gate = nullptr) {
   nn_launch(s, spknn::k_heightmap, dim3((nx * ny + 3) / 4), dim3(256), 0, tray, nx, ny, nz,
-            heights);   // a wavefront per column
+            heights, gate);   // a wavefront per column
   nn_launch(s, spknn::k_tray_feat, dim3(kNnTrayFeat / 2 / 256), dim3(256), 0, tray,
-            (const int*)heights, nx, ny, nz, out);
+            (const int*)heights, nx, ny, nz, out, gate);
 }
 
 template <typename F>
