@@ -193,6 +193,9 @@ struct spk_ctx {
   DevBuf<double> rscores;                // per-orientation best scores [n][n_orient]
   DevBuf<OrientDesc> od_prop;            // descriptors of the proposed orientations
   DevBuf<OrientDesc> od_rest;            // descriptors of the other orientations searched
+  DevBuf<int> learn_gate;                // device-driven learned loop: proposer gate word,
+  DevBuf<int> learn_pick, learn_used;    // per-item pick (fitting indices) and pick kind
+  bool learn_dev = true;                 // option "learn_dev": device-driven learned loop
   // feasibility pass
   DevBuf<int> fstate;                    // residual bits, unused, per-orientation flags
   DevBuf<int> ctl_on;                    // control words that enable the pass outside the loop
@@ -1548,25 +1551,14 @@ void timed_group(spk_ctx* c, const char* name, F&& f) {
 // placement.  counts: [0] items decided on their proposals, [1] searches of
 // all candidates, [2] searches after infeasible proposals, [3] items decided
 // on the host's exact path, [4] items the feasibility pass found infeasible.
-int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
-                 const std::vector<int>& off, const int32_t* order, const spk_orient* orients,
-                 int n_orient, int penalty, int32_t id_offset, spk_result* results, int topk,
-                 float min_conf, int32_t* proposals, int32_t* counts) {
+// Pooled item-branch features of every item in each of its orientations, on
+// the device (c->ipool): 24 rows per item in processing order (rows past
+// n_orient repeat the base item; their logits are unused).
+void learned_item_pool(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
+                       const std::vector<int>& off, const int32_t* order,
+                       const spk_orient* orients, int n_orient) {
   const int n_items = (int)bases.size();
-  PackPlan P = plan_pack(c, bases, off, order, orients, n_orient);
-  const std::vector<ItemPlan>& plans = P.items;
-  const std::vector<OrientDesc>& all = P.all;
   hipStream_t s = c->stream;
-  // free-space certificate, checked on the host with the tray field's
-  // statistics read at every synchronisation
-  const FieldStats* bounds = item_bounds(c, bases, off, order);
-  std::vector<FieldStats> hb(bounds ? n_items : 0);
-  FieldStats tst{};
-  if (bounds) download(c, hb.data(), bounds, (size_t)n_items * sizeof(FieldStats));
-  download(c, &tst, c->stats.p, sizeof(tst));
-  sync(c);
-  // pooled features of every item in each of its orientations: 24 rows per
-  // item (rows past n_orient repeat the base item; their logits are unused)
   timed_group(c, "proposer_items", [&]() {
     nn_cast(nn, s);
     std::vector<spknn::ItemMap> maps((size_t)n_items * kNnLogits);
@@ -1597,6 +1589,26 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
                      c->ipool.p + (size_t)b0 * kNnLogits * kNnItemPooled);
     }
   });
+}
+
+int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
+                 const std::vector<int>& off, const int32_t* order, const spk_orient* orients,
+                 int n_orient, int penalty, int32_t id_offset, spk_result* results, int topk,
+                 float min_conf, int32_t* proposals, int32_t* counts) {
+  const int n_items = (int)bases.size();
+  PackPlan P = plan_pack(c, bases, off, order, orients, n_orient);
+  const std::vector<ItemPlan>& plans = P.items;
+  const std::vector<OrientDesc>& all = P.all;
+  hipStream_t s = c->stream;
+  // free-space certificate, checked on the host with the tray field's
+  // statistics read at every synchronisation
+  const FieldStats* bounds = item_bounds(c, bases, off, order);
+  std::vector<FieldStats> hb(bounds ? n_items : 0);
+  FieldStats tst{};
+  if (bounds) download(c, hb.data(), bounds, (size_t)n_items * sizeof(FieldStats));
+  download(c, &tst, c->stats.p, sizeof(tst));
+  sync(c);
+  learned_item_pool(c, nn, bases, off, order, orients, n_orient);
   c->tfeat.ensure(kNnTrayFeat);
   c->heights.ensure((size_t)c->nx * c->ny);
   c->od_prop.ensure(kNnLogits);
@@ -1905,6 +1917,153 @@ int pack_learned(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     prev_placed = eb.fit >= 0;
     reset_ctl();
     tst = post_and_sync(c, nullptr, nullptr, c->stats.p, nullptr, nullptr, 0).st;
+  }
+  if (counts)
+    for (int i = 0; i < 5; ++i) counts[i] = cnt[i];
+  if (placed > 0) c->phi_internal = true;   // a placement recomputed the field on the device
+  return placed;
+}
+
+// Device-driven learned loop.  Every item is enqueued back to back with no
+// host round trip: its column test (exact feasibility flags; an item with no
+// feasible orientation is decided there), the proposer on the current tray
+// (its kernels exit at once unless more than topk orientations are
+// feasible), the pick (k_learn_pick: the topk feasible orientations with the
+// highest logits, or every feasible one), the search over the pick, the
+// decision, the placement and the refresh.  The host reads the outcomes once
+// at the end, or at a halt (an untrusted FFT outcome, or a search over
+// proposals that found nothing), where it decides that item on the exact
+// path and resumes.  Same results as pack_learned with the column test
+// screening every item and no confidence threshold.
+int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
+                     const std::vector<int>& off, const int32_t* order, const spk_orient* orients,
+                     int n_orient, int penalty, int32_t id_offset, spk_result* results, int topk,
+                     int32_t* proposals, int32_t* counts) {
+  const int n_items = (int)bases.size();
+  PackPlan P = plan_pack(c, bases, off, order, orients, n_orient);
+  const std::vector<ItemPlan>& plans = P.items;
+  const std::vector<OrientDesc>& all = P.all;
+  hipStream_t s = c->stream;
+  // every search runs over topk slots (padding skipped): size the workspace
+  int ws_ex = 1;
+  for (const ItemPlan& pl : plans) ws_ex = std::max(ws_ex, pl.max_ex);
+  ensure_search_ws(c, topk, ws_ex);
+  learned_item_pool(c, nn, bases, off, order, orients, n_orient);
+  c->tfeat.ensure(kNnTrayFeat);
+  c->heights.ensure((size_t)c->nx * c->ny);
+  c->od_prop.ensure(topk);
+  c->learn_gate.ensure(1);
+  c->learn_pick.ensure((size_t)std::max(1, n_items) * topk);
+  c->learn_used.ensure(std::max(1, n_items));
+  int placed = 0, t0 = 0;
+  int cnt[5] = {0, 0, 0, 0, 0};
+  std::vector<int> lp, lu;
+  while (t0 < n_items) {
+    const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
+    upload(c, c->ctl.p, ctl0, sizeof(ctl0));
+    int* ctl = c->ctl.p;
+    for (int t = t0; t < n_items; ++t) {
+      const ItemPlan& pl = plans[t];
+      if (pl.m == 0) continue;   // nothing fits: no search, tray unchanged
+      const OrientDesc* od_fit = c->od_all.p + pl.od_off;
+      launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, c->co.fs.p, pl.m, 0,
+             ctl, (const FieldStats*)nullptr, (const FieldStats*)nullptr, c->pout.p, t);
+      enqueue_coarse(c, od_fit, pl.m, pl.max_ex, pl.max_exy, ctl, c->pout.p, t, 0);
+      launch(c, "learn_gate", k_learn_gate, dim3(1), dim3(64), 0, (const int*)c->co.fs.p, pl.m,
+             topk, c->learn_gate.p, (const int*)ctl);
+      nn->gate = c->learn_gate.p;
+      timed_group(c, "proposer", [&]() {
+        nn_tray_features(s, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, c->tfeat.p,
+                         c->learn_gate.p);
+        nn_tray_branch(nn, s, c->tfeat.p, 1);
+        nn_head(nn, s, c->ipool.p + (size_t)t * kNnLogits * kNnItemPooled, 1);
+      });
+      nn->gate = nullptr;
+      launch(c, "learn_pick", k_learn_pick, dim3(1), dim3(64), 0, (const float*)nn->logits.p,
+             (const int*)c->map_all.p + pl.od_off, od_fit, pl.m, topk, c->co.fs.p, c->od_prop.p,
+             c->learn_pick.p + (size_t)t * topk, c->learn_used.p + t, (const int*)ctl);
+      const OrientDesc* od = c->od_prop.p;
+      enqueue_item_spectra(c, od, topk, pl.max_ex, ctl);
+      enqueue_scored(c, od, topk, pl.max_ex, pl.max_e, penalty, ctl, nullptr, nullptr, nullptr);
+      launch(c, "decide", k_decide, dim3(1), dim3(64), 0, (const SearchOut*)c->sout.p,
+             (const long long*)c->ver.p, c->pout.p, t, c->resid_limit, ctl, 1);
+      const int nb = std::max(1, std::min(1024, (pl.max_e + 255) / 256));
+      launch(c, "place", k_place, dim3(nb), dim3(256), 0, c->tray.p, (const int*)c->items.p, od,
+             (const SearchOut*)c->sout.p, 0, 0, 0, 0, (int)id_offset, c->nx, c->ny, c->nz, 1,
+             c->zero.p, (const int*)ctl);
+      PlaceRef pr;
+      pr.so = c->sout.p;
+      pr.od = od;
+      pr.id = id_offset;
+      pr.max_ex = pl.max_ex;
+      pr.max_dim = pl.max_dim;
+      refresh_field_and_spectra(c, nullptr, ctl, &pr);
+    }
+    int cw[CTL_WORDS];
+    const int nrem = n_items - t0;
+    std::vector<PackOut> po(nrem);
+    lp.assign((size_t)nrem * topk, -1);
+    lu.assign(nrem, -1);
+    download(c, cw, c->ctl.p, sizeof(cw));
+    download(c, po.data(), c->pout.p + t0, po.size() * sizeof(PackOut));
+    download(c, lp.data(), c->learn_pick.p + (size_t)t0 * topk, lp.size() * sizeof(int));
+    download(c, lu.data(), c->learn_used.p + t0, lu.size() * sizeof(int));
+    sync(c);
+    if (c->timing) c->flush_timing();
+    const int stop = cw[CTL_HALT] ? cw[CTL_HALT_ITEM] : n_items;
+    for (int t = t0; t < stop; ++t) {
+      const ItemPlan& pl = plans[t];
+      if (proposals)
+        for (int j = 0; j < topk; ++j) proposals[(size_t)t * topk + j] = -1;
+      const int r = t - t0;
+      if (pl.m == 0) {
+        to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, &results[t]);
+        continue;
+      }
+      if (lu[r] < 0) {   // the column test found no feasible orientation
+        ++cnt[4];
+        to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, &results[t]);
+        continue;
+      }
+      const SearchOut& so = po[r].so;
+      const int* pk = lp.data() + (size_t)r * topk;
+      to_result(c, so.found, so.idx, so.score, so.found ? pl.map[pk[so.orient]] : -1,
+                po[r].collision, so.prox, so.resid, 0, &results[t]);
+      if (so.found) ++placed;
+      ++cnt[lu[r] ? 0 : 1];
+      if (lu[r] && proposals)
+        for (int j = 0; j < topk; ++j)
+          proposals[(size_t)t * topk + j] = pk[j] >= 0 ? pl.map[pk[j]] : -1;
+    }
+    if (!cw[CTL_HALT]) break;
+    // item `stop` on the exact path: over its pick, then (no proposal
+    // feasible) over every fitting orientation
+    const int h = stop, r = h - t0;
+    const ItemPlan& pl = plans[h];
+    const int it = order ? order[h] : h;
+    const int* pk = lp.data() + (size_t)r * topk;
+    std::vector<int> pick, every(pl.m);
+    for (int j = 0; j < topk; ++j)
+      if (pk[j] >= 0) pick.push_back(pk[j]);
+    for (int i = 0; i < pl.m; ++i) every[i] = i;
+    if (proposals)
+      for (int j = 0; j < topk; ++j)
+        proposals[(size_t)h * topk + j] = (lu[r] > 0 && pk[j] >= 0) ? pl.map[pk[j]] : -1;
+    ExactBest eb;
+    if (cw[CTL_HALT] == 2) {
+      ++cnt[2];
+      eb = exact_over(c, bases[it], &all[pl.od_off], every, penalty, nullptr);
+    } else {
+      ++cnt[3];
+      eb = exact_over(c, bases[it], &all[pl.od_off], pick.empty() ? every : pick, penalty,
+                      nullptr);
+      if (eb.fit < 0 && lu[r] > 0) {
+        ++cnt[2];
+        eb = exact_over(c, bases[it], &all[pl.od_off], every, penalty, nullptr);
+      }
+    }
+    placed += place_exact(c, eb, pl, all, po[r].so.resid, id_offset + placed + 1, &results[h]);
+    t0 = h + 1;
   }
   if (counts)
     for (int i = 0; i < 5; ++i) counts[i] = cnt[i];
@@ -2855,6 +3014,10 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->co.fine_group = v;
     return SPK_OK;
   }
+  if (std::string(name) == "learn_dev") {
+    c->learn_dev = value != 0.0;
+    return SPK_OK;
+  }
   if (std::string(name) == "inc_dist") {
     c->inc_dist = value != 0.0;
     return SPK_OK;
@@ -3168,9 +3331,15 @@ int spk_pack_learned(spk_ctx* c, spk_nn* nn, const spk_pack_args* a, int topk, f
     std::vector<HostItem> bases;
     std::vector<int> off;
     load_items(c, a->items, a->item_ptrs, a->dims, a->n_items, a->order, bases, off);
-    const int placed = pack_learned(c, nn, bases, off, a->order, a->orients, a->n_orient,
-                                    a->penalty, a->id_offset, a->results, topk, min_conf,
-                                    proposals, counts);
+    // the device-driven loop covers the default configuration: the column
+    // test screens every item, no confidence threshold, no certificate
+    const bool dev = c->learn_dev && c->co.ready && c->screen_all && c->feas_mode != 0 &&
+                     !c->split && !(min_conf > 0.f) && !c->certificate;
+    const int placed =
+        dev ? pack_learned_dev(c, nn, bases, off, a->order, a->orients, a->n_orient, a->penalty,
+                               a->id_offset, a->results, topk, proposals, counts)
+            : pack_learned(c, nn, bases, off, a->order, a->orients, a->n_orient, a->penalty,
+                           a->id_offset, a->results, topk, min_conf, proposals, counts);
     sync(c);
     if (c->timing) c->flush_timing();
     c->generation += (uint64_t)placed;

@@ -568,14 +568,14 @@ __device__ __forceinline__ bool placed_box(const SearchOut* so, const OrientDesc
   return true;
 }
 
-// rint(sqrt(d2)) for 0 <= d2 < 2^31 without double arithmetic: r is the
+// rint(sqrt(d2)) for 0 <= d2 < 2^31 without double arithmetic: r >= 1 is the
 // rounded root exactly when r^2 - r + 1 <= d2 <= r^2 + r (sqrt(d2) lies in
-// [r - 1/2, r + 1/2), ties are impossible for an integer d2); the float root
-// is within one of r and the two tests correct it.
+// [r - 1/2, r + 1/2), ties are impossible for an integer d2), and r = 0 when
+// d2 = 0; the float root is within one of r and the two tests correct it.
 __device__ __forceinline__ int round_sqrt(int d2) {
   long long r = (long long)rintf(sqrtf((float)d2));
   if (r * r + r < d2) ++r;
-  if (r * r - r + 1 > d2) --r;
+  if (r > 0 && r * r - r + 1 > d2) --r;
   return (int)r;
 }
 
@@ -2377,6 +2377,72 @@ __global__ void k_gather_od(const OrientDesc* __restrict__ src, OrientPick pick,
                             OrientDesc* __restrict__ dst) {
   const int i = threadIdx.x;
   if (i < k) dst[i] = src[pick.idx[i]];
+}
+
+// ---------------------------------------------------------------------------
+// Device-driven learned loop: the proposal pick as kernels.
+// The column test has left per-orientation flags cfs[kFeasHeader + j] for the
+// item's m fitting orientations (exact: 1 when orientation j has a
+// collision-free valid offset); k_coarse_decide has already decided an item
+// with none.
+// ---------------------------------------------------------------------------
+// gate = 1 (the proposer's inference kernels exit at once) unless the item is
+// still to be searched and more than topk of its orientations are feasible
+__global__ void k_learn_gate(const int* __restrict__ cfs, int m, int topk, int* __restrict__ gate,
+                             const int* __restrict__ ctl) {
+  const int j = threadIdx.x;   // one wavefront, m <= 64
+  const int f = j < m ? cfs[kFeasHeader + j] : 0;
+  const int n = __popcll(__ballot(f != 0));
+  if (j == 0) *gate = (search_off(ctl) || n <= topk) ? 1 : 0;
+}
+
+// The orientations searched for an item: every feasible one when there are at
+// most topk (no proposal needed), else the topk feasible ones with the
+// highest logits (equal logits: the lower index first), in index order.  The
+// descriptors go to od_pick[0, topk), padded with a copy of the first whose
+// flag is 0 (the search skips it), and the flags of the pick list replace
+// cfs[kFeasHeader + i], i < topk.  pick[i] = fitting index of slot i (-1 for
+// padding); *used = 1 when logits chose, 0 when every feasible orientation
+// is searched, -1 when the item was decided before (nothing searched).
+// logits are indexed by orientation, map[j] is fitting j's orientation.
+__global__ void k_learn_pick(const float* __restrict__ logits, const int* __restrict__ map,
+                             const OrientDesc* __restrict__ od_fit, int m, int topk,
+                             int* __restrict__ cfs, OrientDesc* __restrict__ od_pick,
+                             int* __restrict__ pick, int* __restrict__ used,
+                             const int* __restrict__ ctl) {
+  const int j = threadIdx.x;   // one wavefront, m <= 64
+  const bool off = search_off(ctl);
+  const int f = (!off && j < m) ? cfs[kFeasHeader + j] : 0;
+  const unsigned long long feas = __ballot(f != 0);
+  const int n = __popcll(feas);
+  bool sel = f != 0;
+  if (n > topk && sel) {
+    // rank among the feasible orientations: higher logit first, then index
+    const float lj = logits[map[j]];
+    int rank = 0;
+    for (int i = 0; i < m; ++i) {
+      if (!((feas >> i) & 1ULL) || i == j) continue;
+      const float li = logits[map[i]];
+      rank += (li > lj || (li == lj && i < j)) ? 1 : 0;
+    }
+    sel = rank < topk;
+  }
+  const unsigned long long chosen = __ballot(sel);
+  const int k = __popcll(chosen);
+  const int first = chosen ? __ffsll((long long)chosen) - 1 : 0;
+  if (sel) {
+    const int pos = __popcll(chosen & ((1ULL << j) - 1ULL));
+    od_pick[pos] = od_fit[j];
+    pick[pos] = j;
+  }
+  if (j < topk) {
+    if (j >= k) {
+      od_pick[j] = od_fit[first];
+      pick[j] = -1;
+    }
+    cfs[kFeasHeader + j] = j < k ? 1 : 0;
+  }
+  if (j == 0) *used = off ? -1 : (n > topk ? 1 : 0);
 }
 
 // Per-orientation best scores of an accepted search: out[map[i]] = score of
