@@ -43,7 +43,7 @@ METRIC = "items placed/s @240×123×100 tray, 24 rot; learned vs plain at ≤0.5
 HBM_PEAK_GBS = 8000.0
 
 
-KERNELS = ["dist_zy", "dist_x", "inc_zy", "inc_x", "inc_stats", "tray_slab_fwd", "tray_rows_fwd", "tray_x_fwd", "item_slab_fwd",
+KERNELS = ["dist_zy", "dist_x", "inc_zy", "inc_x", "inc_stats", "learn_gate", "learn_pick", "tray_slab_fwd", "tray_rows_fwd", "tray_x_fwd", "item_slab_fwd",
            "item_rows_fwd", "pencil_y_fwd", "corr_x", "score_yz", "pencil_y_inv", "rows_score",
            "select", "verify", "decide", "place", "direct_search", "gather_od", "proposer",
            "proposer_items", "record", "feas_begin", "feas_x", "feas_yz", "feas_decide",

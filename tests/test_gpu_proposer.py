@@ -243,7 +243,8 @@ def test_pack_learned_replays_on_oracle(P, PR, topk, min_conf, resid, screen_all
     if topk >= 24 or min_conf > 1.0:
         assert np.all(props == -1) and counts[0] == 0
     else:
-        assert counts[0] + counts[2] > 0
+        # some items were searched over proposals (on the FFT or the exact path)
+        assert (props >= 0).any()
     if resid is not None:
         assert counts[3] > 0 and all(r["path"] == 1 for r in res if r["found"])
     exp, tray, _, _ = _replay(w.items, order, 24, w.tray_size, proposals=props)
