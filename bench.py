@@ -79,54 +79,128 @@ def kernel_bytes(name: str, Ph: int, V: int, m: float) -> float:
 
 
 def bench_uneven(args) -> int:
-    """48 independent trays (sides log-uniform in [48, 256], z <= 128), 30-200
+    """Independent trays (sides log-uniform in [48, 256], z <= 128), 30-200
     blob items each, 24 orientations, interlocking-free check after every
-    placement, all instances packed concurrently (one context and stream per
-    instance).  value = items processed per second over all instances."""
+    placement, all instances packed concurrently (spk_pack_many: a pool of
+    host workers, one context and stream per instance).  value = items
+    placed per second over all instances; items processed per second is
+    reported beside it.  The loop runs twice from empty trays: kernel events
+    off (value), then on (kernel table, roofline of the scored search)."""
     import psacking_amd as P
     from psacking_amd import workloads as W
     from psacking_amd.rotations import orientation_maps
 
     ws = [W.uneven_instance(i) for i in range(args.instances)]
     perms, flips = orientation_maps(24)
-    sessions, lists, orders = [], [], []
+    lists, orders = [], []
     for w in ws:
-        s = P.Session(0)
-        s.set_option("plan", args.plan)
-        s.set_option("certificate", args.certificate)
         vols = [int((it > 0).sum()) for it in w.items]
-        sessions.append(s)
         lists.append([np.ascontiguousarray(it.astype(np.int32)) for it in w.items])
         orders.append(np.array([int(i) for i in np.argsort(vols)[::-1]], np.int32))
+
+    def sessions_for(gen):
+        ss = []
+        for w in ws:
+            s = P.Session(0)
+            s.set_option("plan", args.plan)
+            s.set_option("certificate", args.certificate)
+            for kv in args.opt:
+                k, v = kv.split("=")
+                s.set_option(k, float(v))
+            s.set_tray(np.zeros(w.tray_size, np.int32), args.metric, args.sources, gen)
+            ss.append(s)
+        return ss
+
     t0 = time.perf_counter()
-    for s, w in zip(sessions, ws):
-        s.set_tray(np.zeros(w.tray_size, np.int32), args.metric, args.sources, 0)
+    sessions = sessions_for(0)
     device_sync()
     t_setup = time.perf_counter() - t0
     if args.warmup > 0:
         nw = [min(args.warmup, len(l)) for l in lists]
         P.pack_many(sessions, [l[:k] for l, k in zip(lists, nw)],
                     [np.arange(k, dtype=np.int32) for k in nw], perms, flips, "cube", True)
-        for s, w in zip(sessions, ws):
-            s.set_tray(np.zeros(w.tray_size, np.int32), args.metric, args.sources, 1)
         device_sync()
-    t1 = time.perf_counter()
-    res = P.pack_many(sessions, lists, orders, perms, flips, "cube", True)
-    device_sync()
-    dt = time.perf_counter() - t1
+
+    def run(timing):
+        ss = sessions_for(1)
+        for s in ss:
+            s.timing_reset()
+            s.timing(timing)
+        device_sync()
+        t1 = time.perf_counter()
+        res = P.pack_many(ss, lists, orders, perms, flips, "cube", True)
+        device_sync()
+        dt = time.perf_counter() - t1
+        for s in ss:
+            s.timing(False)
+        return ss, res, dt
+
+    sessions, res, dt = run(False)
     n_items = sum(len(l) for l in lists)
     placed = sum(1 for rl in res for r in rl if r["found"])
     fallback = sum(1 for rl in res for r in rl if r["path"] != 0)
-    dens = []
-    for s, w in zip(sessions, ws):
-        t = s.tray()
-        dens.append(float((t > 0).sum()) / t.size)
+    dens = [float((s.tray() > 0).sum()) / s.tray().size for s in sessions]
     print(f"[bench] {len(ws)} instances, {n_items} items ({placed} placed), {dt:.3f} s, "
           f"setup {t_setup:.3f} s, fallbacks {fallback}, mean tray density "
           f"{np.mean(dens):.4f}", file=sys.stderr, flush=True)
+    ev_sessions, ev_res, dt_ev = run(True)
+    key = lambda r: (r["found"], tuple(r["position"]), r["orientation"], r["score"])
+    assert all([key(x) for x in a] == [key(x) for x in b] for a, b in zip(res, ev_res)), \
+        "uneven runs with and without kernel events differ"
+    kt = {}
+    for s in ev_sessions:
+        for n in KERNELS + ["ff_seed", "ff_tile", "ff_batch_init", "masked_argmin"]:
+            ms, cnt = s.timing_get(n)
+            a = kt.setdefault(n, [0.0, 0])
+            a[0] += ms
+            a[1] += cnt
+    busy = sum(v[0] for v in kt.values())
+    for n, (ms, cnt) in sorted(kt.items(), key=lambda kv: -kv[1][0]):
+        if cnt:
+            print(f"[kernel:uneven] {n:14s} {ms:9.2f} ms  {cnt:6d} launches  "
+                  f"{ms / cnt * 1e3:8.1f} us/launch  {100 * ms / max(busy, 1e-9):5.1f} %",
+                  file=sys.stderr)
+    # roofline of the scored search's yz pass: algorithmic bytes 16 Ph per
+    # searched orientation on each instance's own tray, over its event time
+    b_alg = t_alg = 0.0
+    n_launch = 0
+    for s, w in zip(ev_sessions, ws):
+        nx, ny, nz = w.tray_size
+        ms, cnt = s.timing_get("score_yz")
+        b_alg += 16.0 * nx * ny * (nz // 2 + 1) * s.timing_units("score_yz")
+        t_alg += ms / 1e3
+        n_launch += cnt
+    achieved = b_alg / t_alg / 1e9 if t_alg > 0 else 0.0
+    cpu = None
+    if args.cpu_items > 0:
+        # the oracle's interlocking-free search (cached FP64 spectra, exact
+        # labelling flood fill) on the first item of the largest instance,
+        # all 24 orientations, on an empty tray
+        from oracle import spk_oracle as O
+        big = max(range(len(ws)), key=lambda i: int(np.prod(ws[i].tray_size)))
+        w = ws[big]
+        item = lists[big][int(orders[big][0])]
+        t1 = time.perf_counter()
+        tray = np.zeros(w.tray_size, np.int32)
+        st = O.TrayState(tray, O.distance_field(tray, args.metric, args.sources))
+        for g in O.get_orientations(item, 24):
+            g = O.as_grid(g)
+            if any(g.shape[d] > w.tray_size[d] for d in range(3)):
+                continue
+            O.search_state(st, g, "cube", O.interlocking_mask_state(st, g))
+        sec = time.perf_counter() - t1
+        cores = len(os.sched_getaffinity(0))
+        if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+            cores = max(1, min(cores, int(os.environ["OMP_NUM_THREADS"])))
+        cpu = {"value": 1.0 / sec, "unit": "items/s", "cores": cores, "kind": "port",
+               "sample": (f"the oracle's interlocking-free search (FP64 scipy FFT on cached "
+                          f"spectra, labelling flood fill) of the first item of instance {big} "
+                          f"({'x'.join(map(str, w.tray_size))}), 24 orientations, on an empty "
+                          f"tray: {sec:.1f} s for one item")}
+        print(f"[cpu] {cpu}", file=sys.stderr, flush=True)
     out = {
-        "metric": f"items/s over {len(ws)} concurrent uneven trays, 24 rot, interlocking-free",
-        "value": n_items / dt,
+        "metric": f"items placed/s over {len(ws)} concurrent uneven trays, 24 rot, interlocking-free",
+        "value": placed / dt,
         "unit": "items/s",
         "n_gpus": 1,
         "steps": n_items,
@@ -142,11 +216,17 @@ def bench_uneven(args) -> int:
                          f"48-256 (z<=128), {n_items} blob items, 24 orientations, "
                          f"interlocking-free, proximity {args.metric}/{args.sources}"),
             "items_placed": placed,
+            "items_processed_per_s": n_items / dt,
             "exact_fallbacks": fallback,
             "mean_density": float(np.mean(dens)),
+            "kernel_ms": {n: round(v[0], 3) for n, v in kt.items() if v[1]},
         },
-        "roofline": None,
-        "cpu_baseline": None,
+        "roofline": {"bound": "hbm", "kernel": "score_yz", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes": b_alg,
+                     "avg_launch_us": t_alg / max(1, n_launch) * 1e6,
+                     "unit_of_work": "orientation (16 Ph bytes each, Ph of its own tray)"},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -162,9 +162,12 @@ typedef struct {
 
 /* Many independent packing problems at once (no reference counterpart: the
  * multi-instance batch of uneven trays).  ctxs[i] must hold instance i's tray
- * (spk_tray_set); every context's work runs on its own stream, concurrently.
- * Returns the first failing instance's status (its message in
- * spk_last_error()); the other instances still complete. */
+ * (spk_tray_set); every context's work runs on its own stream.  A pool of
+ * host workers (option "many_workers" of ctxs[0], default 8) takes the
+ * instances largest first (tray voxels x items) from a shared queue, so
+ * their kernels overlap on the device.  Returns the first failing
+ * instance's status (its message in spk_last_error()); the other instances
+ * still complete. */
 int spk_pack_many(spk_ctx* const* ctxs, int n, const spk_pack_args* args);
 
 /* Standalone full-grid operations (host in, host out). */

@@ -178,6 +178,8 @@ struct spk_ctx {
   DevBuf<unsigned char> ffree, freach;   // IF: feasibility / reached grids [m][V]
   DevBuf<long long> pgrid;               // IF: proximity grids [m][V]
   DevBuf<int> exits;                     // IF: high seed faces [m][3]
+  DevBuf<unsigned char> ff_act;          // IF: tiles changed in the last round (two buffers)
+  DevBuf<int> ff_flags;                  // IF: go / changed flags of a batch of rounds
   // greedy loop
   DevBuf<OrientDesc> od_all;             // descriptors of every item's fitting orientations
   DevBuf<PackOut> pout;                  // per-item outcome decided on the device
@@ -196,6 +198,7 @@ struct spk_ctx {
   DevBuf<int> learn_gate;                // device-driven learned loop: proposer gate word,
   DevBuf<int> learn_pick, learn_used;    // per-item pick (fitting indices) and pick kind
   bool learn_dev = true;                 // option "learn_dev": device-driven learned loop
+  int many_workers = 8;                  // option "many_workers": spk_pack_many host workers
   // feasibility pass
   DevBuf<int> fstate;                    // residual bits, unused, per-orientation flags
   DevBuf<int> ctl_on;                    // control words that enable the pass outside the loop
@@ -357,6 +360,8 @@ void check_dims(int nx, int ny, int nz) {
 #include "spk_nn_host.hpp"
 
 namespace {
+
+constexpr int kFfBatchMax = 64;   // flood-fill rounds enqueued between host reads
 
 size_t slab_shmem(const spk_ctx* c) {
   return (size_t)(c->py.n + c->pz.n + c->py.n * c->pz.n) * sizeof(float2);
@@ -620,11 +625,11 @@ void run_distance_inc(spk_ctx* c, const PlaceRef& pr, const int* ctl) {
   const size_t sh = inc_zy_shmem(c, pr);
   auto zy = edt ? k_inc_zy<1> : k_inc_zy<0>;
   allow_shmem(zy, sh);
-  launch(c, "inc_zy", zy, dim3(std::min(pr.max_ex, c->nx)), dim3(kDistThreads), sh,
+  launch(c, "inc_zy", zy, dim3(std::min(pr.max_ex, c->nx), kIncSplit), dim3(kDistThreads), sh,
          (const int*)c->tray.p, c->tmpi.p, pr.so, pr.od, pr.orient, pr.qx, pr.qy, pr.qz, pr.id,
          c->nx, c->ny, c->nz, c->sources, big, ctl);
-  const dim3 grid((c->nz + kIncLines - 1) / kIncLines, c->ny);
-  const int nb = (int)(grid.x * grid.y);
+  const dim3 grid((c->nz + kIncLines - 1) / kIncLines, c->ny, kIncSplit);
+  const int nb = (int)(grid.x * grid.y * grid.z);
   c->stats_part.ensure(nb);
   auto xk = edt ? k_inc_x<1> : k_inc_x<0>;
   const size_t xsh = (size_t)pr.max_ex * kIncLines * sizeof(int);
@@ -812,10 +817,15 @@ WgBest exact_search_one(spk_ctx* c, const HostItem& it, const OrientDesc& d, int
 
 // Flood fill m feasibility grids (c->ffree, m*V bytes) from their seed faces to
 // the fixpoint into c->freach.  exits[3o..] are the high seed faces.
+// Flood fill of the m feasibility grids to their fixpoint.  Rounds are
+// enqueued in batches that run back to back on the device (a round after
+// one that changed nothing exits at once, and within a round only tiles next
+// to a change are worked); the host reads one flag per batch: the last
+// round's change flag.  Every round that reports a change reaches at least
+// one new cell, so the loop ends.
 void flood_fill(spk_ctx* c, int m, const std::vector<int>& exits, int nx, int ny, int nz) {
   const size_t V = (size_t)nx * ny * nz;
   c->freach.ensure((size_t)m * V);
-  c->flag.ensure(1);
   c->exits.ensure((size_t)3 * m);
   upload(c, c->exits.p, exits.data(), exits.size() * sizeof(int));
   const int nb = (int)std::max<size_t>(1, std::min<size_t>(2048, (V + 255) / 256));
@@ -823,16 +833,26 @@ void flood_fill(spk_ctx* c, int m, const std::vector<int>& exits, int nx, int ny
          c->freach.p, (const int*)c->exits.p, nx, ny, nz);
   const int xt = (nx + FFX - 1) / FFX;
   const dim3 grid((nz + FFZ - 1) / FFZ, (ny + FFY - 1) / FFY, xt * m);
-  // a round that reports a change reached at least one new cell, so at most
-  // V rounds report one and the loop always ends
-  for (size_t it = 0; it <= (size_t)m * V; ++it) {
-    HIP_OK(hipMemsetAsync(c->flag.p, 0, sizeof(int), c->stream));
-    launch(c, "ff_tile", k_ff_tile, grid, dim3(FFX * FFY * FFZ), 0,
-           (const unsigned char*)c->ffree.p, c->freach.p, nx, ny, nz, xt, c->flag.p);
+  const size_t tiles = (size_t)grid.x * grid.y * grid.z;
+  c->ff_act.ensure(2 * tiles);
+  c->ff_flags.ensure(kFfBatchMax + 1);
+  int batch = 8;
+  size_t round = 0;
+  for (size_t it = 0; it <= (size_t)m * V; it += batch) {
+    launch(c, "ff_batch_init", k_ff_batch_init, dim3(round == 0 ? 256 : 1), dim3(256), 0,
+           c->ff_flags.p, batch, round == 0 ? c->ff_act.p + (round & 1) * tiles : nullptr,
+           (long long)tiles);
+    for (int r = 0; r < batch; ++r, ++round)
+      launch(c, "ff_tile", k_ff_tile, grid, dim3(FFX * FFY * FFZ), 0,
+             (const unsigned char*)c->ffree.p, c->freach.p, nx, ny, nz, xt,
+             (const int*)c->ff_flags.p + r, c->ff_flags.p + r + 1,
+             (const unsigned char*)c->ff_act.p + (round & 1) * tiles,
+             c->ff_act.p + ((round + 1) & 1) * tiles);
     int ch = 0;
-    download(c, &ch, c->flag.p, sizeof(int));
+    download(c, &ch, c->ff_flags.p + batch, sizeof(int));
     sync(c);
     if (!ch) break;
+    batch = std::min(2 * batch, kFfBatchMax);   // long fills: fewer host reads
   }
 }
 
@@ -2598,18 +2618,32 @@ int spk_pack_many(spk_ctx* const* ctxs, int n, const spk_pack_args* args) {
     if (!ctxs[i]) return fail(SPK_EINVAL, "null context in pack_many");
   std::vector<int> rc(n, SPK_OK);
   std::vector<std::string> msg(n);
-  // one host thread per instance: each drives its own context and stream,
-  // so the instances' kernels (and their host-controlled flood fills)
-  // overlap on the device
+  // A pool of persistent host workers takes instances from a shared queue,
+  // largest work first (tray voxels x items), so that the long instances
+  // start at once and the short ones fill in behind them.  Each instance
+  // runs on its own context and stream; the workers' kernels (and their
+  // flood-fill batches) overlap on the device.  Pool size: option
+  // "many_workers" of the first context.
+  std::vector<int> queue(n);
+  for (int i = 0; i < n; ++i) queue[i] = i;
+  auto work = [&](int i) {
+    const spk_ctx* c = ctxs[i];
+    return (double)c->nx * c->ny * c->nz * std::max(1, args[i].n_items);
+  };
+  std::stable_sort(queue.begin(), queue.end(), [&](int a, int b) { return work(a) > work(b); });
+  const int workers = std::max(1, std::min(n, ctxs[0]->many_workers));
+  std::atomic<int> next{0};
   std::vector<std::thread> th;
-  th.reserve(n);
-  for (int i = 0; i < n; ++i) {
-    th.emplace_back([&, i]() {
-      const spk_pack_args& a = args[i];
-      rc[i] = pack_impl(ctxs[i], a.items, a.item_ptrs, a.dims, a.n_items, a.order, a.orients,
-                        a.n_orient,
-                       a.penalty, a.interlocking_free, a.id_offset, a.results);
-      if (rc[i] != SPK_OK) msg[i] = spk_last_error();
+  th.reserve(workers);
+  for (int w = 0; w < workers; ++w) {
+    th.emplace_back([&]() {
+      for (int q = next.fetch_add(1); q < n; q = next.fetch_add(1)) {
+        const int i = queue[q];
+        const spk_pack_args& a = args[i];
+        rc[i] = pack_impl(ctxs[i], a.items, a.item_ptrs, a.dims, a.n_items, a.order, a.orients,
+                          a.n_orient, a.penalty, a.interlocking_free, a.id_offset, a.results);
+        if (rc[i] != SPK_OK) msg[i] = spk_last_error();
+      }
     });
   }
   for (auto& t : th) t.join();
@@ -3012,6 +3046,11 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     if (v != 4 && v != 8 && v != 16 && v != 32 && v != 64)
       return fail(SPK_EINVAL, "fine_group must be 4, 8, 16, 32 or 64");
     c->co.fine_group = v;
+    return SPK_OK;
+  }
+  if (std::string(name) == "many_workers") {
+    if (value < 1 || value > 256) return fail(SPK_EINVAL, "many_workers must lie in [1, 256]");
+    c->many_workers = (int)value;
     return SPK_OK;
   }
   if (std::string(name) == "learn_dev") {

@@ -538,6 +538,7 @@ k_dist_x(const int* __restrict__ in, int* __restrict__ phi, FieldStats* __restri
 // ---------------------------------------------------------------------------
 constexpr int kIncMaxEx = 128;       // largest x extent of an item updated this way
 constexpr int kIncLines = 64;        // (y, z) lines per k_inc_x block (one per lane)
+constexpr int kIncSplit = 4;         // k_inc_x blocks per line group (shares of x)
 
 struct PlacedBox {
   int qx, qy, qz;                    // offset of the oriented item
@@ -579,8 +580,9 @@ __device__ __forceinline__ int round_sqrt(int d2) {
   return (int)r;
 }
 
-// grid (kIncMaxEx), kDistThreads threads.  Dynamic LDS: the item box's
-// source flags (hy+1)*(hz+1) ints, then g1 (hy+1)*nz ints.
+// grid (kIncMaxEx, kIncSplit), kDistThreads threads: block (r, s) computes
+// slab qx + r's z pass and share s of its y pass.  Dynamic LDS: the item
+// box's source flags (hy+1)*(hz+1) ints, then g1 (hy+1)*nz ints.
 // g: [hx+1][ny][nz] ints, >= big where a slab has no new source.
 template <int EDT>
 __global__ void __launch_bounds__(kDistThreads)
@@ -615,9 +617,11 @@ k_inc_zy(const int* __restrict__ tray, int* __restrict__ g, const SearchOut* __r
     g1[e] = best;
   }
   __syncthreads();
-  // y pass over the item's rows
+  // y pass over the item's rows, this block's share of the slab
   int* out = g + (size_t)r * ny * nz;
-  for (int e = tid; e < ny * nz; e += kDistThreads) {
+  const int share = (ny * nz + kIncSplit - 1) / kIncSplit;
+  const int e1 = min(ny * nz, ((int)blockIdx.y + 1) * share);
+  for (int e = blockIdx.y * share + tid; e < e1; e += kDistThreads) {
     const int j = e / nz, k = e - j * nz;
     int best = big;
     for (int jy = 0; jy < ry; ++jy) {
@@ -629,8 +633,9 @@ k_inc_zy(const int* __restrict__ tray, int* __restrict__ g, const SearchOut* __r
   }
 }
 
-// grid (ceil(nz / kIncLines), ny), 256 threads: lane l of every wave owns
-// line (y = blockIdx.y, z = z0 + l); wave w evaluates x = w, w + 4, ...
+// grid (ceil(nz / kIncLines), ny, kIncSplit), 256 threads: lane l of every
+// wave owns line (y = blockIdx.y, z = z0 + l); wave w evaluates x = x0 + w,
+// x0 + w + 4, ... in the block's share of x (blockIdx.z of kIncSplit).
 // Every output is the minimum over the hx+1 candidate slabs (staged in LDS,
 // dynamic: (hx+1) * kIncLines ints); an x whose distance to the candidate
 // range alone reaches the old value cannot lower it and is skipped.
@@ -658,8 +663,10 @@ k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict
   if (z < nz) {
     const int wy = min(y + 1, ny - y), wz = min(z + 1, nz - z);
     const int xl = b.qx, xh = b.qx + m - 1;
+    const int share = (nx + kIncSplit - 1) / kIncSplit;
+    const int xa = blockIdx.z * share, xb = min(nx, xa + share);
 #pragma unroll 4
-    for (int x = wave; x < nx; x += 4) {
+    for (int x = xa + wave; x < xb; x += 4) {
       int* pv = phi + ((size_t)x * ny + y) * nz + z;
       const int old = *pv;
       int v = old;
@@ -712,7 +719,7 @@ k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict
       f.mx = rmx[w] > f.mx ? rmx[w] : f.mx;
       f.mxw = rmw[w] > f.mxw ? rmw[w] : f.mxw;
     }
-    part[blockIdx.y * gridDim.x + blockIdx.x] = f;
+    part[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = f;
   }
 }
 
@@ -2792,14 +2799,40 @@ k_ff_seed(const unsigned char* __restrict__ fr, unsigned char* __restrict__ reac
 
 // tile 4 x 8 x 32 with a one-cell halo; local fixpoint in LDS.
 // grid (ceil(nz/32), ceil(ny/8), ceil(nx/4) * m): the z index also selects the grid o.
+//
+// Rounds run back to back on the device.  prev: nonzero when the previous
+// round changed some cell (a round after one that changed nothing exits at
+// once); changed: set when this round changes a cell.  act_prev / act_cur
+// (one byte per tile, indexed like the grid): a tile is worked only when it
+// or one of its six neighbouring tiles of the same grid changed in the
+// previous round; every block writes its own act_cur entry.
 constexpr int FFX = 4, FFY = 8, FFZ = 32;
 __global__ void __launch_bounds__(FFX * FFY * FFZ)
 k_ff_tile(const unsigned char* __restrict__ fr_all, unsigned char* __restrict__ reached_all,
-          int nx, int ny, int nz, int xtiles, int* __restrict__ changed) {
+          int nx, int ny, int nz, int xtiles, const int* __restrict__ prev,
+          int* __restrict__ changed, const unsigned char* __restrict__ act_prev,
+          unsigned char* __restrict__ act_cur) {
+  if (*prev == 0) return;
+  const int tid = threadIdx.x;
+  const size_t tile = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  {
+    const int xt_ = blockIdx.z % xtiles;
+    const size_t sy = gridDim.x, sx = (size_t)gridDim.x * gridDim.y;
+    bool act = act_prev[tile] != 0;
+    act = act || (blockIdx.x > 0 && act_prev[tile - 1]);
+    act = act || (blockIdx.x + 1 < gridDim.x && act_prev[tile + 1]);
+    act = act || (blockIdx.y > 0 && act_prev[tile - sy]);
+    act = act || (blockIdx.y + 1 < gridDim.y && act_prev[tile + sy]);
+    act = act || (xt_ > 0 && act_prev[tile - sx]);
+    act = act || (xt_ + 1 < xtiles && act_prev[tile + sx]);
+    if (!act) {
+      if (tid == 0) act_cur[tile] = 0;
+      return;
+    }
+  }
   constexpr int HX = FFX + 2, HY = FFY + 2, HZ = FFZ + 2;
   __shared__ unsigned char sf[HX * HY * HZ], sr[HX * HY * HZ];
   __shared__ int gch;
-  const int tid = threadIdx.x;
   const int tz = tid % FFZ, ty = (tid / FFZ) % FFY, tx = tid / (FFZ * FFY);
   const int o = blockIdx.z / xtiles;
   const size_t V = (size_t)nx * ny * nz;
@@ -2863,7 +2896,20 @@ k_ff_tile(const unsigned char* __restrict__ fr_all, unsigned char* __restrict__ 
     gch = 1;
   }
   __syncthreads();
-  if (tid == 0 && gch) atomicOr(changed, 1);
+  if (tid == 0) {
+    act_cur[tile] = gch ? 1 : 0;
+    if (gch) atomicOr(changed, 1);
+  }
+}
+
+// start of a batch of flood-fill rounds: flags[0] = go (1), flags[1..k] = 0;
+// with init_act, every tile of act starts active
+__global__ void k_ff_batch_init(int* __restrict__ flags, int k, unsigned char* __restrict__ act,
+                                long long n_act) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= k) flags[i] = i == 0 ? 1 : 0;
+  if (act)
+    for (long long t = i; t < n_act; t += (long long)gridDim.x * blockDim.x) act[t] = 1;
 }
 
 // grid (nx, m): argmin of double(p) + penalty[k] over reached offsets of slice x
