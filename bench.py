@@ -85,7 +85,8 @@ def bench_uneven(args) -> int:
     host workers, one context and stream per instance).  value = items
     placed per second over all instances; items processed per second is
     reported beside it.  The loop runs twice from empty trays: kernel events
-    off (value), then on (kernel table, roofline of the scored search)."""
+    off (value), then on, one instance after another (kernel table, roofline
+    of the scored search)."""
     import psacking_amd as P
     from psacking_amd import workloads as W
     from psacking_amd.rotations import orientation_maps
@@ -126,6 +127,9 @@ def bench_uneven(args) -> int:
         for s in ss:
             s.timing_reset()
             s.timing(timing)
+        # with kernel events the instances run one after another: events on
+        # concurrent streams would also time the other streams' kernels
+        ss[0].set_option("many_workers", 1 if timing else args.many_workers)
         device_sync()
         t1 = time.perf_counter()
         res = P.pack_many(ss, lists, orders, perms, flips, "cube", True)
@@ -241,6 +245,8 @@ def main(argv=None) -> int:
                     choices=["readme", "blobs", "boxes", "uneven"])
     ap.add_argument("--instances", type=int, default=48,
                     help="uneven workload: number of independent trays")
+    ap.add_argument("--many-workers", type=int, default=8,
+                    help="uneven workload: host workers of spk_pack_many")
     ap.add_argument("--metric", default="edt", choices=["edt", "l1"])
     ap.add_argument("--sources", default="nonzero", choices=["nonzero", "eq1"])
     ap.add_argument("--cpu-items", type=int, default=2,

@@ -210,6 +210,8 @@ struct spk_ctx {
   int corr_chunk = 0;                    // option "corr_chunk": orientations per pass-A block (0 auto)
   AxisPlan sy{}, sz{};                   // y and z plans of the score pass
   int pad_y = 0, pad_z = 0;              // options "pad_y", "pad_z": padded sizes (0 = smallest smooth)
+  bool pad_forced = false;               // pad_y / pad_z set by an option: not autotuned
+  bool score_radix_forced = false;       // score_radix set by an option: not autotuned
   int feas_group = 0;                    // option "feas_group": pairs per feasibility launch pair
   bool screen_all = true;                // option "screen_all": learned loop screens every item
                                          // with the column test before proposing
@@ -2104,7 +2106,12 @@ struct TuneKey {
     return std::tie(nx, ny, nz, split) < std::tie(o.nx, o.ny, o.nz, o.split);
   }
 };
-std::map<TuneKey, int> g_tile;
+// the autotuner's choice for a tray shape: x-pencil tile, largest butterfly
+// of the score pass, padded y and z FFT sizes
+struct TuneChoice {
+  int T, score_rmax, py, pz;
+};
+std::map<TuneKey, TuneChoice> g_tile;
 std::mutex g_tile_mu;
 
 // the 24 proper rotations as index maps
@@ -2126,77 +2133,140 @@ std::vector<spk_orient> rotation_maps() {
   return out;
 }
 
+// Autotuner, per tray shape and process: x-pencil tile (4, 8, 16), the
+// score pass's largest butterfly (radix 10 or 25: different stage counts and
+// register pressure) and the padded y/z FFT sizes (the smallest
+// 2^a 3^b 5^c 7^d size or the next one: more points, maybe fewer or cheaper
+// stages), each timed on a batched 24-orientation search plus its
+// feasibility pass with a box item about a fifth of the tray's smallest side.
+// Coordinate search in that order; options that fix a choice ("tile",
+// "score_radix", "pad_y", "pad_z") take it out of the search.  The choice
+// changes no result (every plan gives exact integers).
 void autotune_tile(spk_ctx* c) {
   const TuneKey key{c->nx, c->ny, c->nz, c->split ? 1 : 0};
+  auto apply = [&](const TuneChoice& ch) {
+    if (ch.py != c->py.n || ch.pz != c->pz.n) {
+      c->pad_y = ch.py;
+      c->pad_z = ch.pz;
+      plan_tray(c);
+      build_spectra(c);
+    }
+    c->score_rmax = ch.score_rmax;
+    c->sy = make_plan(c->py.n, std::min(c->yz_rmax, c->score_rmax));
+    c->sz = make_plan(c->pz.n, std::min(c->yz_rmax, c->score_rmax));
+    c->T = ch.T;
+  };
   {
     std::lock_guard<std::mutex> g(g_tile_mu);
     auto it = g_tile.find(key);
     if (it != g_tile.end()) {
-      c->T = it->second;
+      apply(it->second);
       return;
     }
   }
-  const int px = c->px.n;
-  std::vector<int> cands;
-  for (int T : {4, 8, 16})
-    if (px * T <= kPencilMax && (T <= 2 * c->pz.n || T == 4)) cands.push_back(T);
-  if (cands.size() > 1) {
-    // box item about a fifth of the tray's smallest side, slightly uneven
-    const int a = std::max(2, std::min({c->nx, c->ny, c->nz}) / 5);
-    const int d0 = std::min(c->nx, a), d1 = std::min(c->ny, std::max(2, a * 4 / 5)),
-              d2 = std::min(c->nz, std::max(2, a * 6 / 5));
-    std::vector<int> box((size_t)d0 * d1 * d2, 1);
-    c->items.ensure(box.size());
-    upload(c, c->items.p, box.data(), box.size() * sizeof(int));
-    const HostItem h = scan_item(box.data(), d0, d1, d2);
-    std::vector<OrientDesc> ods;
-    int max_ex = 1, max_e = 1;
-    for (const spk_orient& o : rotation_maps()) {
-      OrientDesc d = make_desc(c, h, 0, &o);
-      if (!fits(c, d)) continue;
-      ods.push_back(d);
-      max_ex = std::max(max_ex, d.ex);
-      max_e = std::max(max_e, d.ex * d.ey * d.ez);
-    }
-    const int m = (int)ods.size();
-    if (m > 0) {
-      c->od.ensure(m);
-      upload(c, c->od.p, ods.data(), m * sizeof(OrientDesc));
+  TuneChoice best{c->T, c->score_rmax, c->py.n, c->pz.n};
+  // box item about a fifth of the tray's smallest side, slightly uneven
+  const int a = std::max(2, std::min({c->nx, c->ny, c->nz}) / 5);
+  const int d0 = std::min(c->nx, a), d1 = std::min(c->ny, std::max(2, a * 4 / 5)),
+            d2 = std::min(c->nz, std::max(2, a * 6 / 5));
+  std::vector<int> box((size_t)d0 * d1 * d2, 1);
+  c->items.ensure(box.size());
+  upload(c, c->items.p, box.data(), box.size() * sizeof(int));
+  const HostItem h = scan_item(box.data(), d0, d1, d2);
+  std::vector<OrientDesc> ods;
+  int max_ex = 1, max_e = 1;
+  for (const spk_orient& o : rotation_maps()) {
+    OrientDesc d = make_desc(c, h, 0, &o);
+    if (!fits(c, d)) continue;
+    ods.push_back(d);
+    max_ex = std::max(max_ex, d.ex);
+    max_e = std::max(max_e, d.ex * d.ey * d.ez);
+  }
+  const int m = (int)ods.size();
+  if (m > 0) {
+    c->od.ensure(m);
+    upload(c, c->od.p, ods.data(), m * sizeof(OrientDesc));
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    const bool timing = c->timing;
+    c->timing = false;
+    auto measure = [&]() -> float {
       ensure_search_ws(c, m, max_ex);
-      hipEvent_t e0, e1;
-      HIP_OK(hipEventCreate(&e0));
-      HIP_OK(hipEventCreate(&e1));
-      const bool timing = c->timing;
-      c->timing = false;
-      float best_ms = 1e30f;
-      int best_T = c->T;
-      for (int T : cands) {
-        c->T = T;
-        auto run = [&]() {
-          enqueue_search(c, c->od.p, m, max_ex, max_e, SPK_PENALTY_CUBE, nullptr, nullptr,
-                         nullptr, nullptr);
-          if (!c->split) enqueue_feas(c, c->od.p, m, max_ex, nullptr);
-        };
-        run();   // warm-up
-        HIP_OK(hipEventRecord(e0, c->stream));
-        for (int r = 0; r < 2; ++r) run();
-        HIP_OK(hipEventRecord(e1, c->stream));
-        HIP_OK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms) {
-          best_ms = ms;
-          best_T = T;
+      auto run = [&]() {
+        enqueue_search(c, c->od.p, m, max_ex, max_e, SPK_PENALTY_CUBE, nullptr, nullptr,
+                       nullptr, nullptr);
+        if (!c->split) enqueue_feas(c, c->od.p, m, max_ex, nullptr);
+      };
+      run();   // warm-up
+      HIP_OK(hipEventRecord(e0, c->stream));
+      for (int r = 0; r < 2; ++r) run();
+      HIP_OK(hipEventRecord(e1, c->stream));
+      HIP_OK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+      return ms;
+    };
+    float best_ms = measure();
+    auto consider = [&](const TuneChoice& ch) {
+      apply(ch);
+      const float ms = measure();
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = ch;
+      }
+    };
+    // 1. x-pencil tile
+    if (!c->tile_force) {
+      const TuneChoice base = best;
+      for (int T : {4, 8, 16}) {
+        if (T == base.T || c->px.n * T > kPencilMax || !(T <= 2 * c->pz.n || T == 4)) continue;
+        TuneChoice ch = base;
+        ch.T = T;
+        consider(ch);
+      }
+      apply(best);
+    }
+    // 2. largest butterfly of the score pass (slab plan)
+    if (!c->split && !c->score_radix_forced) {
+      TuneChoice ch = best;
+      ch.score_rmax = best.score_rmax == 10 ? 25 : 10;
+      const AxisPlan ay = make_plan(c->py.n, std::min(c->yz_rmax, ch.score_rmax));
+      const AxisPlan az = make_plan(c->pz.n, std::min(c->yz_rmax, ch.score_rmax));
+      if (ay.nst != c->sy.nst || ay.rad != c->sy.rad || az.nst != c->sz.nst || az.rad != c->sz.rad)
+        consider(ch);
+      apply(best);
+    }
+    // 3. padded y / z sizes (slab plan kept; the spectra are rebuilt)
+    if (!c->split && !c->pad_forced) {
+      const TuneChoice base = best;
+      int lds_max = 0;
+      HIP_OK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock,
+                                   c->device));
+      for (int axis = 0; axis < 2; ++axis) {
+        TuneChoice ch = base;
+        int& p = axis == 0 ? ch.py : ch.pz;
+        p = next_smooth(p + 1);
+        const size_t slab = (size_t)(ch.py + ch.pz + ch.py * ch.pz) * sizeof(float2) +
+                            (size_t)c->ny * c->nz * sizeof(int);
+        if ((long long)ch.py * ch.pz > kSlabMax || slab + 512 > (size_t)lds_max || p > kMaxAxis)
+          continue;
+        consider(ch);
+        if (c->split) {   // the padded plan left the slab plan: not a candidate
+          best_ms = 1e30f;
+          apply(base);
+          best_ms = measure();
+          best = base;
         }
       }
-      c->timing = timing;
-      hipEventDestroy(e0);
-      hipEventDestroy(e1);
-      c->T = best_T;
+      apply(best);
     }
+    c->timing = timing;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
   }
   std::lock_guard<std::mutex> g(g_tile_mu);
-  g_tile[key] = c->T;
+  g_tile[key] = best;
 }
 
 // Parses and uploads the concatenated items of a pack call.
@@ -2400,7 +2470,12 @@ int spk_tray_set(spk_ctx* c, const int32_t* tray, int nx, int ny, int nz, const 
     const bool reshape = !(c->nx == nx && c->ny == ny && c->nz == nz) || c->tw.p == nullptr;
     c->nx = nx; c->ny = ny; c->nz = nz;
     c->metric = metric; c->sources = sources;
-    if (reshape) plan_tray(c);
+    if (reshape) {
+      // sizes and radix the autotuner chose for another shape do not carry over
+      if (!c->pad_forced) c->pad_y = c->pad_z = 0;
+      if (!c->score_radix_forced) c->score_rmax = 10;
+      plan_tray(c);
+    }
     const size_t V = (size_t)nx * ny * nz;
     c->tray.ensure(V);
     c->phi.ensure(V);
@@ -3001,6 +3076,7 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     const int v = (int)value;
     if (v != 10 && v != 25) return fail(SPK_EINVAL, "score_radix must be 10 or 25");
     c->score_rmax = v;
+    c->score_radix_forced = true;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
     return SPK_OK;
@@ -3015,6 +3091,7 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
       if (v < 0 || v > kMaxAxis || (v && !smooth(v)))
         return fail(SPK_EINVAL, "padded size must be 0 or a 2^a 3^b 5^c 7^d length <= 1024");
       (std::string(name) == "pad_y" ? c->pad_y : c->pad_z) = v;
+      c->pad_forced = c->pad_y != 0 || c->pad_z != 0;
     }
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
