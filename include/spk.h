@@ -271,6 +271,16 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  "pad_y"/"pad_z" padded FFT sizes (0: smallest 2^a3^b5^c7^d).
  *   "corr_chunk"   orientations per pass-A block (0: automatic).
  *   "feas_group"   orientation pairs per feasibility launch pair (0: all).
+ *   "graph"        1: the device-driven loops (spk_pack, spk_pack_gather,
+ *                  spk_pack_learned) capture each enqueue of their items as
+ *                  one hipGraph and launch it (off while timing is on);
+ *                  0 (default): the launches go to the stream directly.
+ *   "learn_dev"    1 (default): spk_pack_learned runs device-driven (the
+ *                  pick of the proposals as a kernel, no host round trip per
+ *                  item) when the column test screens every item, min_conf
+ *                  is 0 and the certificate is off; 0: host-driven.
+ *   "many_workers" host workers of spk_pack_many (set on its first
+ *                  context; default 8).
  *   "inc_dist"     1 (default): after a placement the distance field is
  *                  updated from the placed voxels alone (the field is a
  *                  minimum over sources: Phi = min(Phi, transform of the new

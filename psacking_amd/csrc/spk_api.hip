@@ -199,6 +199,7 @@ struct spk_ctx {
   DevBuf<int> learn_pick, learn_used;    // per-item pick (fitting indices) and pick kind
   bool learn_dev = true;                 // option "learn_dev": device-driven learned loop
   int many_workers = 8;                  // option "many_workers": spk_pack_many host workers
+  bool graph = false;                    // option "graph": device loops captured as a hipGraph
   // feasibility pass
   DevBuf<int> fstate;                    // residual bits, unused, per-orientation flags
   DevBuf<int> ctl_on;                    // control words that enable the pass outside the loop
@@ -1406,6 +1407,63 @@ void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e
   refresh_field_and_spectra(c, nullptr, c->ctl.p, &pr);
 }
 
+// Optional hipGraph capture of a device-driven loop's enqueue (option
+// "graph"): every launch of the items from the first undecided one to the
+// last goes into one graph, instantiated and launched once.  The loops'
+// buffers are sized before the capture begins (presize_loop), no copy,
+// memset or synchronisation is enqueued inside it, and per-kernel timing
+// (events) turns the capture off.  On an exception the capture is ended and
+// its graph discarded.
+struct LoopGraph {
+  spk_ctx* c;
+  bool on;
+  bool capturing = false;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  LoopGraph(spk_ctx* c_, bool want) : c(c_), on(want && !c_->timing) {
+    if (on) {
+      HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+      capturing = true;
+    }
+  }
+  // end the capture and launch the graph
+  void run() {
+    if (!on) return;
+    capturing = false;
+    HIP_OK(hipStreamEndCapture(c->stream, &g));
+    HIP_OK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    HIP_OK(hipGraphLaunch(ge, c->stream));
+  }
+  ~LoopGraph() {
+    if (capturing) {
+      hipGraph_t dead = nullptr;
+      (void)hipStreamEndCapture(c->stream, &dead);
+      if (dead) (void)hipGraphDestroy(dead);
+    }
+    if (ge) {
+      (void)hipStreamSynchronize(c->stream);   // the launched graph has finished
+      (void)hipGraphExecDestroy(ge);
+    }
+    if (g) (void)hipGraphDestroy(g);
+  }
+};
+
+// Sizes every buffer an item step of the device-driven loops may grow, for
+// items whose largest search has m orientations of x extent up to max_ex,
+// so that a captured enqueue allocates nothing.
+void presize_loop(spk_ctx* c, int m, int max_ex) {
+  spk_ctx::Coarse& q = c->co;
+  if (q.ready) {
+    const int ccx = (max_ex + 1) / 2;
+    q.cm.ensure((size_t)m * ccx * q.cy);
+    q.im.ensure((size_t)m * max_ex * c->ny);
+    q.rl.ensure((size_t)m * kColRuns);
+    q.frl.ensure((size_t)m * kFineRuns);
+    q.nrun.ensure(2 * (size_t)m);
+  }
+  c->stats_part.ensure((size_t)((c->nz + kIncLines - 1) / kIncLines) * c->ny * kIncSplit);
+}
+
 struct ExactBest {
   int fit = -1;            // index into the item's fitting orientations, -1 = none
   int idx = 0;
@@ -1480,9 +1538,18 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
 
   int placed = 0;
   int t0 = 0;
+  {
+    int mm = 1, mx = 1;
+    for (const ItemPlan& pl : plans) {
+      mm = std::max(mm, pl.m);
+      mx = std::max(mx, pl.max_ex);
+    }
+    presize_loop(c, mm, mx);
+  }
   while (t0 < n_items) {
     const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
     upload(c, c->ctl.p, ctl0, sizeof(ctl0));
+    LoopGraph lg(c, c->graph && !rec_feat && !rec_scores);
     for (int t = t0; t < n_items; ++t) {
       const ItemPlan& pl = plans[t];
       if (pl.m == 0) continue;   // nothing fits: no search, tray unchanged
@@ -1495,6 +1562,7 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
                    c->map_all.p + pl.od_off, -1, bounds ? bounds + t : nullptr, false,
                    pl.max_dim);
     }
+    lg.run();
     int ctl[CTL_WORDS];
     std::vector<PackOut> po(n_items - t0);
     download(c, ctl, c->ctl.p, sizeof(ctl));
@@ -1980,10 +2048,16 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   int placed = 0, t0 = 0;
   int cnt[5] = {0, 0, 0, 0, 0};
   std::vector<int> lp, lu;
+  {
+    int mm = topk;
+    for (const ItemPlan& pl : plans) mm = std::max(mm, pl.m);
+    presize_loop(c, mm, ws_ex);
+  }
   while (t0 < n_items) {
     const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
     upload(c, c->ctl.p, ctl0, sizeof(ctl0));
     int* ctl = c->ctl.p;
+    LoopGraph lg(c, c->graph);
     for (int t = t0; t < n_items; ++t) {
       const ItemPlan& pl = plans[t];
       if (pl.m == 0) continue;   // nothing fits: no search, tray unchanged
@@ -2021,6 +2095,7 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       pr.max_dim = pl.max_dim;
       refresh_field_and_spectra(c, nullptr, ctl, &pr);
     }
+    lg.run();
     int cw[CTL_WORDS];
     const int nrem = n_items - t0;
     std::vector<PackOut> po(nrem);
@@ -3123,6 +3198,10 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     if (v != 4 && v != 8 && v != 16 && v != 32 && v != 64)
       return fail(SPK_EINVAL, "fine_group must be 4, 8, 16, 32 or 64");
     c->co.fine_group = v;
+    return SPK_OK;
+  }
+  if (std::string(name) == "graph") {
+    c->graph = value != 0.0;
     return SPK_OK;
   }
   if (std::string(name) == "many_workers") {

@@ -221,19 +221,24 @@ def test_pack_records_matches_oracle(P, PR):
         np.testing.assert_array_equal(bits[t], N.bf16_bits(N.tray_features(trays[t])).reshape(-1))
 
 
-@pytest.mark.parametrize("topk,min_conf,resid,screen_all", [(4, 0.0, None, 1), (2, 0.0, -1.0, 1),
-                                                            (24, 0.0, None, 1), (4, 1.01, None, 1),
-                                                            (4, 0.0, None, 0)])
-def test_pack_learned_replays_on_oracle(P, PR, topk, min_conf, resid, screen_all):
+@pytest.mark.parametrize("topk,min_conf,resid,screen_all,extra",
+                         [(4, 0.0, None, 1, ()), (2, 0.0, -1.0, 1, ()), (24, 0.0, None, 1, ()),
+                          (4, 1.01, None, 1, ()), (4, 0.0, None, 0, ()),
+                          (4, 0.0, None, 1, (("graph", 1),)), (4, 0.0, None, 1, (("learn_dev", 0),))])
+def test_pack_learned_replays_on_oracle(P, PR, topk, min_conf, resid, screen_all, extra):
     """screen_all 1 (default): every item is screened by the column test
     before proposing; 0: only after an item that placed nothing, so proposals
-    may be infeasible and the remaining candidates are searched after them."""
+    may be infeasible and the remaining candidates are searched after them.
+    The default configuration runs device-driven (learn_dev), also captured
+    as a hipGraph, and host-driven."""
     from psacking_amd.rotations import orientation_maps
     w, order = _workload(47, 7, (56, 48, 40))
     perms, flips = orientation_maps(24)
     nn = PR.Proposer(0, 8, PR.init_params(48))
     s = P.Session(0)
     s.set_option("screen_all", screen_all)
+    for name, value in extra:
+        s.set_option(name, value)
     if resid is not None:
         s.set_option("resid_limit", resid)
     s.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
