@@ -200,6 +200,7 @@ struct spk_ctx {
   bool learn_dev = true;                 // option "learn_dev": device-driven learned loop
   int many_workers = 8;                  // option "many_workers": spk_pack_many host workers
   bool graph = false;                    // option "graph": device loops captured as a hipGraph
+  bool fixed_fft = true;                 // option "fixed_fft": compile-time-plan score kernels
   // feasibility pass
   DevBuf<int> fstate;                    // residual bits, unused, per-orientation flags
   DevBuf<int> ctl_on;                    // control words that enable the pass outside the loop
@@ -517,6 +518,10 @@ void plan_tray(spk_ctx* c) {
     allow_shmem(k_item_slab_fwd, dyn);
     allow_shmem(k_score_yz<10>, dyn);
     allow_shmem(k_score_yz<25>, dyn);
+    allow_shmem(k_score_yz_f<FPlan<125, 5, 5, 5>, FPlan<100, 10, 10>>, dyn);
+    allow_shmem(k_score_yz_f<FPlan<125, 25, 5>, FPlan<100, 10, 10>>, dyn);
+    allow_shmem(k_score_yz_f<FPlan<100, 10, 10>, FPlan<100, 10, 10>>, dyn);
+    allow_shmem(k_score_yz_f<FPlan<64, 8, 8>, FPlan<64, 8, 8>>, dyn);
     allow_shmem(k_corr_x, dyn);
     allow_shmem(k_tray_x_fwd, dyn);
     allow_shmem(k_feas_x, dyn);
@@ -1024,6 +1029,38 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
 }
 
 
+// true when the runtime plan p is the compile-time plan FPlan<N, Rs...>
+template <int N, int... Rs>
+bool plan_is(const AxisPlan& p, FPlan<N, Rs...>) {
+  const int rs[] = {Rs...};
+  if (p.n != N || p.nst != (int)sizeof...(Rs)) return false;
+  for (int s = 0; s < p.nst; ++s)
+    if ((int)((p.rad >> (kRadixBits * s)) & kRadixMask) != rs[s]) return false;
+  return true;
+}
+
+using ScoreKernel = decltype(&k_score_yz<10>);
+
+// the score pass for the context's plans: a compile-time instantiation when
+// one matches (padded sizes of the common trays), else the runtime one
+template <class PY_, class PZ_>
+bool fixed_score(const spk_ctx* c, ScoreKernel* k) {
+  if (!plan_is(c->sy, PY_{}) || !plan_is(c->sz, PZ_{})) return false;
+  *k = k_score_yz_f<PY_, PZ_>;
+  return true;
+}
+
+ScoreKernel score_kernel(const spk_ctx* c) {
+  ScoreKernel k = c->score_rmax == 25 ? k_score_yz<25> : k_score_yz<10>;
+  if (c->fixed_fft) {
+    fixed_score<FPlan<125, 5, 5, 5>, FPlan<100, 10, 10>>(c, &k) ||
+        fixed_score<FPlan<125, 25, 5>, FPlan<100, 10, 10>>(c, &k) ||
+        fixed_score<FPlan<100, 10, 10>, FPlan<100, 10, 10>>(c, &k) ||
+        fixed_score<FPlan<64, 8, 8>, FPlan<64, 8, 8>>(c, &k);
+  }
+  return k;
+}
+
 // Scored search (pass A, pass B, selection, exact re-check of the winner) of
 // m descriptors whose item spectra are in c->ispec.
 void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
@@ -1061,8 +1098,9 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
            c->ny, c->nz, py, c->pz, c->Rs, tw + c->px.n + py, fgrid, pgrid, ctl);
   } else {
     // the score pass has its own factorisations of py and pz (radices up to
-    // c->score_rmax, compiled as the kernel's largest butterfly)
-    launch(c, "score_yz", c->score_rmax == 25 ? k_score_yz<25> : k_score_yz<10>, dim3(c->nx, m),
+    // c->score_rmax, compiled as the kernel's largest butterfly), or a
+    // compile-time plan when one matches them
+    launch(c, "score_yz", score_kernel(c), dim3(c->nx, m),
            dim3(kSlabThreads), score_shmem(c), (const float2*)c->y1.p, (const float2*)c->y2.p, od,
            (const FieldStats*)c->stats.p, pen, c->rec.p, c->nx, c->ny, c->nz, c->sy, c->sz,
            tw + c->px.n, tw + c->px.n + py, fgrid, pgrid, cf,
@@ -3198,6 +3236,10 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     if (v != 4 && v != 8 && v != 16 && v != 32 && v != 64)
       return fail(SPK_EINVAL, "fine_group must be 4, 8, 16, 32 or 64");
     c->co.fine_group = v;
+    return SPK_OK;
+  }
+  if (std::string(name) == "fixed_fft") {
+    c->fixed_fft = value != 0.0;
     return SPK_OK;
   }
   if (std::string(name) == "graph") {

@@ -345,4 +345,102 @@ __device__ __forceinline__ void lds_fft(float2* buf, const AxisPlan& p, int es, 
   }
 }
 
+// ---- compile-time plans ----------------------------------------------------
+// A transform length with its radices fixed at compile time (FPlan<N, R...>,
+// applied in the order listed), or the runtime plan (DynPlan).  With a fixed
+// plan every stride, count and divisor of the stages below is a constant:
+// index arithmetic becomes shifts and multiplications, LDS addresses take
+// immediate offsets and each thread's butterfly loop is fully unrolled.
+struct DynPlan {
+  static constexpr int N = 0;
+};
+template <int N_, int... Rs>
+struct FPlan {
+  static constexpr int N = N_;
+};
+
+template <int A, int B>
+struct CeilDiv {
+  static constexpr int v = (A + B - 1) / B;
+};
+
+// One Stockham stage (the same element mapping as fft_stage) with N, R, NS and
+// the strides fixed.  Rows (JFAST): transforms BS apart, elements contiguous,
+// batch (<= MAXB transforms) at run time.  Columns (!JFAST): transforms
+// contiguous, elements ES apart, BATCH transforms fixed.
+template <int R, bool FWD, int NT, int N, int NS, int ES, int BS, bool JFAST, int BATCH, int MAXB>
+__device__ __forceinline__ void fstage(float2* buf, int batch_rt, const float2* tw, int tid) {
+  constexpr int M = N / R;
+  constexpr int TWS = N / (NS * R);
+  constexpr int MAXT = (JFAST ? MAXB : BATCH) * M;
+  constexpr int MS = CeilDiv<MAXT, NT>::v;
+  constexpr int IN_STEP = JFAST ? M : M * ES;
+  constexpr int OUT_STEP = JFAST ? NS : NS * ES;
+  const int total = JFAST ? batch_rt * M : MAXT;
+  float2 v[MS][R];
+  int ob[MS];
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    const int b = tid + s * NT;
+    ob[s] = -1;
+    if (b < total) {
+      int j, ib, obase;
+      if (JFAST) {
+        const int q = b / M;
+        j = b - q * M;
+        const int k = j % NS;
+        ib = q * BS + j;
+        obase = q * BS + (j - k) * R + k;
+      } else {
+        j = b / BATCH;
+        const int col = b - j * BATCH;
+        const int k = j % NS;
+        ib = col + j * ES;
+        obase = col + ((j - k) * R + k) * ES;
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[s][r] = buf[ib + r * IN_STEP];
+      const int k = j % NS;
+      if (NS > 1 && k != 0) {
+        const int kt = k * TWS;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[r * kt];
+          if (!FWD) w.y = -w.y;
+          v[s][r] = cmul(v[s][r], w);
+        }
+      }
+      Bfly<R, FWD>::run(v[s]);
+      ob[s] = obase;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    if (ob[s] >= 0) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[ob[s] + r * OUT_STEP] = v[s][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <bool FWD, int NT, int N, int NS, int ES, int BS, bool JFAST, int BATCH, int MAXB, int R,
+          int... Rest>
+__device__ __forceinline__ void fixed_fft_stages(float2* buf, int batch, const float2* tw,
+                                                 int tid) {
+  fstage<R, FWD, NT, N, NS, ES, BS, JFAST, BATCH, MAXB>(buf, batch, tw, tid);
+  if constexpr (sizeof...(Rest) > 0)
+    fixed_fft_stages<FWD, NT, N, NS * R, ES, BS, JFAST, BATCH, MAXB, Rest...>(buf, batch, tw, tid);
+}
+
+template <bool FWD, int NT, class P, int ES, int BS, bool JFAST, int BATCH, int MAXB>
+struct FixedFft;
+template <bool FWD, int NT, int N, int... Rs, int ES, int BS, bool JFAST, int BATCH, int MAXB>
+struct FixedFft<FWD, NT, FPlan<N, Rs...>, ES, BS, JFAST, BATCH, MAXB> {
+  __device__ __forceinline__ static void run(float2* buf, int batch, const float2* tw, int tid) {
+    fixed_fft_stages<FWD, NT, N, 1, ES, BS, JFAST, BATCH, MAXB, Rs...>(buf, batch, tw, tid);
+  }
+};
+
 }  // namespace spk

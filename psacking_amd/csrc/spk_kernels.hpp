@@ -1179,6 +1179,135 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
   wg_argmin_write(best, bidx, bp, res, fp.ok ? 0 : 1, out, tid, kSlabThreads);
 }
 
+// The score pass with the y and z plans fixed at compile time (FPlan): the
+// same computation as k_score_yz, for the padded sizes of the common trays,
+// with every index, stride and loop bound a constant.  The host launches it
+// when the runtime plans equal SY and SZ.
+template <class SY, class SZ>
+__global__ void __launch_bounds__(kSlabThreads)
+k_score_yz_f(const float2* __restrict__ y1, const float2* __restrict__ y2,
+             const OrientDesc* __restrict__ od, const FieldStats* __restrict__ st,
+             const double* __restrict__ pen, WgBest* __restrict__ rec, int nx, int ny, int nz,
+             AxisPlan py_, AxisPlan pz_, const float2* __restrict__ tw_y,
+             const float2* __restrict__ tw_z, unsigned char* __restrict__ fgrid,
+             long long* __restrict__ pgrid, const int* __restrict__ cf,
+             unsigned long long* __restrict__ searched, const int* __restrict__ ctl) {
+  constexpr int PY = SY::N, PZ = SZ::N, HZ = PZ / 2 + 1, NT = kSlabThreads;
+  constexpr int EL = CeilDiv<PY * PZ, NT>::v;       // slab elements per thread
+  constexpr int ELH = CeilDiv<PY * HZ, NT>::v;      // half-spectrum elements per thread
+  if (search_off(ctl)) return;
+  extern __shared__ float2 smem[];
+  const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
+  const OrientDesc d = od[o];
+  WgBest* out = rec + (size_t)o * nx + x;
+  if (x >= d.nvx) return;
+  if (cf && ctl && ctl[CTL_PRUNE] && !cf[o]) {   // excluded by the column test
+    if (tid == 0) {
+      WgBest b;
+      b.score = 1e300; b.idx = INT_MAX; b.p_lo = 0; b.p_hi = 0; b.resid = 0.f; b.unsupported = 0;
+      *out = b;
+    }
+    return;
+  }
+  if (searched && x == 0 && tid == 0) atomicAdd(searched, 1ULL);   // an orientation searched
+  float2* twy = smem;
+  float2* twz = twy + PY;
+  float2* buf = twz + PZ;
+  load_table(twy, tw_y, PY, tid, NT);
+  load_table(twz, tw_z, PZ, tid, NT);
+  const FieldParams fp = field_params(st, (long long)nx * ny * nz);
+  const long long addback = ((1LL << fp.lb) / 2 + fp.mu) * d.isum;
+  const int nvy = d.nvy, nvz = d.nvz;
+  // --- low digit (real field, half spectrum) ---
+  {
+    const float2* s2 = y2 + ((size_t)o * nx + x) * PY * HZ;
+    float2 v[ELH];
+#pragma unroll
+    for (int s = 0; s < ELH; ++s) {
+      const int e = tid + s * NT;
+      if (e < PY * HZ) v[s] = s2[e];
+    }
+#pragma unroll
+    for (int s = 0; s < ELH; ++s) {
+      const int e = tid + s * NT;
+      if (e < PY * HZ) {
+        const int j = e / HZ;
+        buf[j * PZ + (e - j * HZ)] = v[s];
+      }
+    }
+  }
+  __syncthreads();
+  FixedFft<false, NT, SY, PZ, 1, false, HZ, 0>::run(buf, 0, twy, tid);
+  for (int e = tid; e < nvy * (PZ - HZ); e += NT) {
+    const int j = e / (PZ - HZ), k = HZ + (e - j * (PZ - HZ));
+    const float2 v = buf[j * PZ + (PZ - k)];
+    buf[j * PZ + k] = make_float2(v.x, -v.y);
+  }
+  __syncthreads();
+  FixedFft<false, NT, SZ, 1, PZ, true, 0, PY>::run(buf, nvy, twz, tid);
+  // the rounded low-digit sums wait in LDS behind the slab ([ny][nz] ints)
+  int* lo = reinterpret_cast<int*>(buf + PY * PZ);
+  float res = 0.f;
+#pragma unroll
+  for (int s = 0; s < EL; ++s) {
+    const int e = tid + s * NT;
+    const int j = e / PZ, k = e - j * PZ;
+    if (j < nvy && k < nvz) {
+      const float v = buf[e].x;
+      const float r = rintf(v);
+      res = fmaxf(res, fabsf(v - r));
+      lo[j * nz + k] = (int)r;
+    }
+  }
+  __syncthreads();
+  // --- collision + high digit (complex field, full spectrum) ---
+  {
+    const float2* s1 = y1 + ((size_t)o * nx + x) * PY * PZ;
+    float2 v[EL];
+#pragma unroll
+    for (int s = 0; s < EL; ++s) {
+      const int e = tid + s * NT;
+      if (e < PY * PZ) v[s] = s1[e];
+    }
+#pragma unroll
+    for (int s = 0; s < EL; ++s) {
+      const int e = tid + s * NT;
+      if (e < PY * PZ) buf[e] = v[s];
+    }
+  }
+  __syncthreads();
+  FixedFft<false, NT, SY, PZ, 1, false, PZ, 0>::run(buf, 0, twy, tid);
+  FixedFft<false, NT, SZ, 1, PZ, true, 0, PY>::run(buf, nvy, twz, tid);
+  double best = 1e300;
+  int bidx = INT_MAX;
+  long long bp = 0;
+  const int lb = fp.lb;
+#pragma unroll
+  for (int s = 0; s < EL; ++s) {
+    const int e = tid + s * NT;
+    const int j = e / PZ, k = e - j * PZ;
+    if (j < nvy && k < nvz) {
+      const float2 v = buf[e];
+      const float c = rintf(v.x), h = rintf(v.y);
+      res = fmaxf(res, fmaxf(fabsf(v.x - c), fabsf(v.y - h)));
+      // b * h with b = 2^lb: a shift of the (two's complement) high digit
+      const long long p =
+          (long long)((unsigned long long)(long long)h << lb) + lo[j * nz + k] + addback;
+      const int idx = (x * ny + j) * nz + k;
+      if (fgrid) {
+        const size_t g = (size_t)o * nx * ny * nz + idx;
+        fgrid[g] = c == 0.f ? 1 : 0;
+        pgrid[g] = p;
+      }
+      if (c == 0.f) {
+        const double sc = (double)p + pen[k];
+        if (better(sc, idx, best, bidx)) { best = sc; bidx = idx; bp = p; }
+      }
+    }
+  }
+  wg_argmin_write(best, bidx, bp, res, fp.ok ? 0 : 1, out, tid, NT);
+}
+
 // ---------------------------------------------------------------------------
 // Feasibility pass: does any valid offset of an orientation have zero
 // collision with the occupied voxels?  Collision-only correlations with the
