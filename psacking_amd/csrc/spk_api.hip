@@ -523,6 +523,9 @@ void plan_tray(spk_ctx* c) {
     allow_shmem(k_score_yz_f<FPlan<100, 10, 10>, FPlan<100, 10, 10>>, dyn);
     allow_shmem(k_score_yz_f<FPlan<64, 8, 8>, FPlan<64, 8, 8>>, dyn);
     allow_shmem(k_corr_x, dyn);
+    allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 16>, dyn);
+    allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 8>, dyn);
+    allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 4>, dyn);
     allow_shmem(k_tray_x_fwd, dyn);
     allow_shmem(k_feas_x, dyn);
     allow_shmem(k_feas_yz, dyn);
@@ -1061,6 +1064,26 @@ ScoreKernel score_kernel(const spk_ctx* c) {
   return k;
 }
 
+using CorrKernel = decltype(&k_corr_x);
+
+template <class PX_, int T_>
+bool fixed_corr(const spk_ctx* c, CorrKernel* k) {
+  if (c->T != T_ || !plan_is(c->px, PX_{})) return false;
+  *k = k_corr_x_f<PX_, T_>;
+  return true;
+}
+
+// pass A for the context's x plan and tile: a compile-time instantiation when
+// one matches, else the runtime one
+CorrKernel corr_kernel(const spk_ctx* c) {
+  CorrKernel k = k_corr_x;
+  if (c->fixed_fft) {
+    fixed_corr<FPlan<240, 16, 15>, 16>(c, &k) || fixed_corr<FPlan<240, 16, 15>, 8>(c, &k) ||
+        fixed_corr<FPlan<240, 16, 15>, 4>(c, &k);
+  }
+  return k;
+}
+
 // Scored search (pass A, pass B, selection, exact re-check of the winner) of
 // m descriptors whose item spectra are in c->ispec.
 void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
@@ -1079,7 +1102,7 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
   }
   // column-test flags of this list (pruning needs CTL_PRUNE set by its decide)
   const int* cf = c->co.ready && !fgrid ? c->co.fs.p + kFeasHeader : nullptr;
-  launch(c, "corr_x", k_corr_x, dim3(tiles, py, nchunk), dim3(kPencilThreads),
+  launch(c, "corr_x", corr_kernel(c), dim3(tiles, py, nchunk), dim3(kPencilThreads),
          pencil_shmem(c) + (size_t)c->px.n * c->T * sizeof(float2),
          (const float2*)c->g1.p, (const float2*)c->g2.p, (const float2*)c->ispec.p, od, c->y1.p,
          c->y2.p, m, chunk, max_ex, c->nx, c->px, py, pz, c->T, scale, tw, cf, ctl);

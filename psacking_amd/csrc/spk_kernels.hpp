@@ -1045,6 +1045,91 @@ k_corr_x(const float2* __restrict__ g1, const float2* __restrict__ g2,
   }
 }
 
+// pass A with the x plan and the tile fixed at compile time (FPlan, TT): the
+// same computation as k_corr_x, launched when the runtime plan and tile match.
+template <class SX, int TT>
+__global__ void __launch_bounds__(kPencilThreads)
+k_corr_x_f(const float2* __restrict__ g1, const float2* __restrict__ g2,
+           const float2* __restrict__ ispec, const OrientDesc* __restrict__ od,
+           float2* __restrict__ y1, float2* __restrict__ y2, int n_orient, int chunk, int max_ex,
+           int nx, AxisPlan px_, int py, int pz, int T_, float scale,
+           const float2* __restrict__ tw_x, const int* __restrict__ cf,
+           const int* __restrict__ ctl) {
+  constexpr int PX = SX::N, T = TT, NT = kPencilThreads;
+  constexpr int EL = CeilDiv<PX * T, NT>::v;
+  if (search_off(ctl)) return;
+  const bool prune = cf && ctl && ctl[CTL_PRUNE];
+  extern __shared__ float2 smem[];
+  const int hz = pz / 2 + 1;
+  float2* twx = smem;
+  float2* buf = twx + PX;
+  float2* buf2 = buf + PX * T;
+  const int ky = blockIdx.y, tid = threadIdx.x;
+  const int kz0 = blockIdx.x * T;
+  const int o0 = blockIdx.z * chunk;
+  const int o1 = min(n_orient, o0 + chunk);
+  load_table(twx, tw_x, PX, tid, NT);
+  const bool has2 = kz0 < hz;
+  float2 G1[EL], G2[EL];
+#pragma unroll
+  for (int s = 0; s < EL; ++s) {
+    const int e = tid + s * NT;
+    const int x = e / T, t = e - x * T;
+    const int kz = kz0 + t;
+    G1[s] = make_float2(0.f, 0.f);
+    G2[s] = make_float2(0.f, 0.f);
+    if (e < PX * T && kz < pz) {
+      G1[s] = g1[((size_t)x * py + ky) * pz + kz];
+      if (kz < hz) G2[s] = g2[((size_t)x * py + ky) * hz + kz];
+    }
+  }
+  for (int o = o0; o < o1; ++o) {
+    if (prune && !cf[o]) continue;
+    const OrientDesc d = od[o];
+    const float2* is = ispec + (size_t)o * max_ex * py * pz;
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < EL; ++s) {
+      const int e = tid + s * NT;
+      if (e < PX * T) {
+        const int x = e / T, t = e - x * T;
+        const int kz = kz0 + t;
+        float2 v = make_float2(0.f, 0.f);
+        if (x < d.ex && kz < pz) v = is[((size_t)x * py + ky) * pz + kz];
+        buf[e] = v;
+      }
+    }
+    __syncthreads();
+    FixedFft<true, NT, SX, T, 1, false, T, 0>::run(buf, 0, twx, tid);
+#pragma unroll
+    for (int s = 0; s < EL; ++s) {
+      const int e = tid + s * NT;
+      if (e < PX * T) {
+        const float2 I = buf[e];
+        buf[e] = cscale(cmulc(G1[s], I), scale);
+        if (has2) buf2[e] = cscale(cmulc(G2[s], I), scale);
+      }
+    }
+    __syncthreads();
+    FixedFft<false, NT, SX, T, 1, false, T, 0>::run(buf, 0, twx, tid);
+    float2* d1 = y1 + (size_t)o * nx * py * pz;
+    for (int e = tid; e < d.nvx * T; e += NT) {
+      const int x = e / T, t = e - x * T;
+      const int kz = kz0 + t;
+      if (kz < pz) d1[((size_t)x * py + ky) * pz + kz] = buf[e];
+    }
+    if (has2) {
+      FixedFft<false, NT, SX, T, 1, false, T, 0>::run(buf2, 0, twx, tid);
+      float2* d2 = y2 + (size_t)o * nx * py * hz;
+      for (int e = tid; e < d.nvx * T; e += NT) {
+        const int x = e / T, t = e - x * T;
+        const int kz = kz0 + t;
+        if (kz < hz) d2[((size_t)x * py + ky) * hz + kz] = buf2[e];
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // pass B: grid (nx, n_orient).  Inverse yz, round, score, argmin.
 // ---------------------------------------------------------------------------
