@@ -455,6 +455,45 @@ void build_coarse_spectra(spk_ctx* c, const int* ctl) {
 }
 
 // ---- FFT planning for a tray --------------------------------------------
+// true when the runtime plan p is the compile-time plan FPlan<N, Rs...>
+template <int N, int... Rs>
+bool plan_is(const AxisPlan& p, FPlan<N, Rs...>) {
+  const int rs[] = {Rs...};
+  if (p.n != N || p.nst != (int)sizeof...(Rs)) return false;
+  for (int s = 0; s < p.nst; ++s)
+    if ((int)((p.rad >> (kRadixBits * s)) & kRadixMask) != rs[s]) return false;
+  return true;
+}
+
+using SlabFwdKernel = decltype(&k_tray_slab_fwd<DynPlan, DynPlan>);
+using ItemFwdKernel = decltype(&k_item_slab_fwd<DynPlan, DynPlan>);
+using XFwdKernel = decltype(&k_tray_x_fwd<DynPlan, 0>);
+
+// the forward slab and x-pencil kernels for the context's plans: the
+// compile-time instantiation for the common padded sizes when the plans (and
+// the tile) match, else the runtime one
+SlabFwdKernel tray_slab_kernel(const spk_ctx* c) {
+  using Y = FPlan<125, 25, 5>;
+  using Z = FPlan<100, 10, 10>;
+  if (c->fixed_fft && plan_is(c->py, Y{}) && plan_is(c->pz, Z{})) return k_tray_slab_fwd<Y, Z>;
+  return k_tray_slab_fwd<DynPlan, DynPlan>;
+}
+ItemFwdKernel item_slab_kernel(const spk_ctx* c) {
+  using Y = FPlan<125, 25, 5>;
+  using Z = FPlan<100, 10, 10>;
+  if (c->fixed_fft && plan_is(c->py, Y{}) && plan_is(c->pz, Z{})) return k_item_slab_fwd<Y, Z>;
+  return k_item_slab_fwd<DynPlan, DynPlan>;
+}
+XFwdKernel tray_x_kernel(const spk_ctx* c) {
+  using X = FPlan<240, 16, 15>;
+  if (c->fixed_fft && plan_is(c->px, X{})) {
+    if (c->T == 16) return k_tray_x_fwd<X, 16>;
+    if (c->T == 8) return k_tray_x_fwd<X, 8>;
+    if (c->T == 4) return k_tray_x_fwd<X, 4>;
+  }
+  return k_tray_x_fwd<DynPlan, 0>;
+}
+
 void plan_tray(spk_ctx* c) {
   const int px = next_smooth(c->nx);
   const int pz = c->pad_z >= c->nz && smooth(c->pad_z) ? c->pad_z : next_smooth(c->nz);
@@ -514,8 +553,13 @@ void plan_tray(spk_ctx* c) {
   if (c->split) c->py = make_plan(py, kPencilRadix);
   if (!c->lds_attrs) {
     const size_t dyn = (size_t)lds_max - 1024;   // leave room for static shared arrays
-    allow_shmem(k_tray_slab_fwd, dyn);
-    allow_shmem(k_item_slab_fwd, dyn);
+    using Y = FPlan<125, 25, 5>;
+    using Z = FPlan<100, 10, 10>;
+    using X = FPlan<240, 16, 15>;
+    allow_shmem(k_tray_slab_fwd<DynPlan, DynPlan>, dyn);
+    allow_shmem(k_tray_slab_fwd<Y, Z>, dyn);
+    allow_shmem(k_item_slab_fwd<DynPlan, DynPlan>, dyn);
+    allow_shmem(k_item_slab_fwd<Y, Z>, dyn);
     allow_shmem(k_score_yz<10>, dyn);
     allow_shmem(k_score_yz<25>, dyn);
     allow_shmem(k_score_yz_f<FPlan<125, 5, 5, 5>, FPlan<100, 10, 10>>, dyn);
@@ -526,7 +570,10 @@ void plan_tray(spk_ctx* c) {
     allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 16>, dyn);
     allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 8>, dyn);
     allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 4>, dyn);
-    allow_shmem(k_tray_x_fwd, dyn);
+    allow_shmem(k_tray_x_fwd<DynPlan, 0>, dyn);
+    allow_shmem(k_tray_x_fwd<X, 16>, dyn);
+    allow_shmem(k_tray_x_fwd<X, 8>, dyn);
+    allow_shmem(k_tray_x_fwd<X, 4>, dyn);
     allow_shmem(k_feas_x, dyn);
     allow_shmem(k_feas_yz, dyn);
     c->lds_attrs = true;
@@ -587,12 +634,12 @@ void build_spectra(spk_ctx* c, const int* ctl = nullptr) {
            dim3(kPencilThreads), sh, c->t2.p, c->py, hz, c->Ty, c->ny, py, tw + c->px.n,
            (const OrientDesc*)nullptr, c->nx, (int)PENCIL_PLAIN, ctl, 1);
   } else {
-    launch(c, "tray_slab_fwd", k_tray_slab_fwd, dim3(c->nx, 2), dim3(kSlabThreads),
+    launch(c, "tray_slab_fwd", tray_slab_kernel(c), dim3(c->nx, 2), dim3(kSlabThreads),
            slab_shmem(c), (const int*)c->tray.p, (const int*)c->phi.p,
            (const FieldStats*)c->stats.p, c->t1.p, c->t2.p, c->nx, c->ny, c->nz, c->py, c->pz,
            tw + c->px.n, tw + c->px.n + py, ctl);
   }
-  launch(c, "tray_x_fwd", k_tray_x_fwd, dim3((pz + c->T - 1) / c->T, py, 2),
+  launch(c, "tray_x_fwd", tray_x_kernel(c), dim3((pz + c->T - 1) / c->T, py, 2),
          dim3(kPencilThreads), pencil_shmem(c), (const float2*)c->t1.p, (const float2*)c->t2.p,
          c->g1.p, c->g2.p, c->nx, c->px, py, pz, c->T, tw, ctl);
   build_coarse_spectra(c, ctl);
@@ -924,7 +971,7 @@ void enqueue_item_spectra(spk_ctx* c, const OrientDesc* od, int m, int max_ex, c
            dim3(kPencilThreads), ysh, c->ispec.p, c->py, pz, c->Ty, py, py, tw + c->px.n, od,
            max_ex, (int)PENCIL_ITEM, ctl, 0);
   } else {
-    launch(c, "item_slab_fwd", k_item_slab_fwd, dim3(max_ex, m), dim3(kSlabThreads),
+    launch(c, "item_slab_fwd", item_slab_kernel(c), dim3(max_ex, m), dim3(kSlabThreads),
            slab_shmem(c), (const int*)c->items.p, od, c->ispec.p, max_ex, c->py, c->pz,
            tw + c->px.n, tw + c->px.n + py,
            c->co.ready ? (const int*)c->co.fs.p + kFeasHeader : (const int*)nullptr, ctl);
@@ -1031,16 +1078,6 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
          c->fstate.p, out, t, halt_not_found, (int*)ctl);
 }
 
-
-// true when the runtime plan p is the compile-time plan FPlan<N, Rs...>
-template <int N, int... Rs>
-bool plan_is(const AxisPlan& p, FPlan<N, Rs...>) {
-  const int rs[] = {Rs...};
-  if (p.n != N || p.nst != (int)sizeof...(Rs)) return false;
-  for (int s = 0; s < p.nst; ++s)
-    if ((int)((p.rad >> (kRadixBits * s)) & kRadixMask) != rs[s]) return false;
-  return true;
-}
 
 using ScoreKernel = decltype(&k_score_yz<10>);
 

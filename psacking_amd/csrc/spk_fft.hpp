@@ -443,4 +443,23 @@ struct FixedFft<FWD, NT, FPlan<N, Rs...>, ES, BS, JFAST, BATCH, MAXB> {
   }
 };
 
+// A transform with the compile-time plan P when P is fixed (the strides and
+// counts then come from the template arguments), else with the runtime plan p.
+template <bool FWD, int NT, int MAXEL, int RMAX, class P, int ES, int BS, bool JFAST, int BATCH,
+          int MAXB>
+__device__ __forceinline__ void fft_axis(float2* buf, const AxisPlan& p, int es, int batch,
+                                         int bs, bool jfast, const float2* tw, int tid) {
+  if constexpr (P::N > 0)
+    FixedFft<FWD, NT, P, ES, BS, JFAST, BATCH, MAXB>::run(buf, batch, tw, tid);
+  else
+    lds_fft<FWD, NT, MAXEL, RMAX>(buf, p, es, batch, bs, jfast, tw, tid);
+}
+
+// e / d and e % d with d = D when D > 0 (a constant), else the runtime d
+template <int D>
+__device__ __forceinline__ int div_by(int e, int d, float inv) {
+  if constexpr (D > 0) return e / D;
+  else return fdiv(e, d, inv);
+}
+
 }  // namespace spk

@@ -800,15 +800,18 @@ k_field_stats(const int* __restrict__ phi, long long n, FieldStats* __restrict__
 
 // grid (nx, 2): field 0 -> slab of B + i*h, full kz; field 1 -> slab of l,
 // only kz < hz kept.  Forward 2D FFT (z rows < ny, then y columns).
+// SY, SZ: compile-time plans of the padded y and z lengths, or DynPlan.
+template <class SY, class SZ>
 __global__ void __launch_bounds__(kSlabThreads)
 k_tray_slab_fwd(const int* __restrict__ tray, const int* __restrict__ phi,
                 const FieldStats* __restrict__ st, float2* __restrict__ t1,
                 float2* __restrict__ t2, int nx, int ny, int nz, AxisPlan py_, AxisPlan pz_,
                 const float2* __restrict__ tw_y, const float2* __restrict__ tw_z,
                 const int* __restrict__ ctl) {
+  constexpr int FY = SY::N, FZ = SZ::N, FH = FZ / 2 + 1;
   if (refresh_off(ctl)) return;
   extern __shared__ float2 smem[];
-  const int py = py_.n, pz = pz_.n, hz = pz / 2 + 1;
+  const int py = FY > 0 ? FY : py_.n, pz = FZ > 0 ? FZ : pz_.n, hz = pz / 2 + 1;
   float2* twy = smem;
   float2* twz = twy + py;
   float2* buf = twz + pz;
@@ -825,7 +828,7 @@ k_tray_slab_fwd(const int* __restrict__ tray, const int* __restrict__ phi,
 #pragma unroll
     for (int s = 0; s < kSlabMaxEl; ++s) {
       const int e = tid + s * kSlabThreads;
-      const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
+      const int j = div_by<FZ>(e, pz, inv_pz), k = e - j * pz;
       dv[s] = 0;
       tv[s] = 0;
       if (e < py * pz && j < ny && k < nz) {
@@ -836,7 +839,7 @@ k_tray_slab_fwd(const int* __restrict__ tray, const int* __restrict__ phi,
 #pragma unroll
     for (int s = 0; s < kSlabMaxEl; ++s) {
       const int e = tid + s * kSlabThreads;
-      const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
+      const int j = div_by<FZ>(e, pz, inv_pz), k = e - j * pz;
       if (e >= py * pz) continue;
       float2 v = make_float2(0.f, 0.f);
       if (j < ny && k < nz) {
@@ -853,9 +856,14 @@ k_tray_slab_fwd(const int* __restrict__ tray, const int* __restrict__ phi,
     }
   }
   __syncthreads();
-  lds_fft<true, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, ny, pz, true, twz, tid);
-  const int ncol = field == 0 ? pz : hz;
-  lds_fft<true, kSlabThreads, kSlabMaxEl>(buf, py_, pz, ncol, 1, false, twy, tid);
+  fft_axis<true, kSlabThreads, kSlabMaxEl, kMaxRadix, SZ, 1, FZ, true, 0, FY>(
+      buf, pz_, 1, ny, pz, true, twz, tid);
+  if (field == 0)
+    fft_axis<true, kSlabThreads, kSlabMaxEl, kMaxRadix, SY, FZ, 1, false, FZ, 0>(
+        buf, py_, pz, pz, 1, false, twy, tid);
+  else
+    fft_axis<true, kSlabThreads, kSlabMaxEl, kMaxRadix, SY, FZ, 1, false, FH, 0>(
+        buf, py_, pz, hz, 1, false, twy, tid);
   if (field == 0) {
     float2* dst = t1 + (size_t)x * py * pz;
     for (int e = tid; e < py * pz; e += kSlabThreads) dst[e] = buf[e];
@@ -869,14 +877,17 @@ k_tray_slab_fwd(const int* __restrict__ tray, const int* __restrict__ phi,
 }
 
 // grid (ceil(ncols/T), py, 2): forward FFT along x of the slab spectra
-// (rows x >= nx are zero), writes G[kx][ky][kz].
+// (rows x >= nx are zero), writes G[kx][ky][kz].  SX, TT: compile-time x plan
+// and tile, or DynPlan and 0.
+template <class SX, int TT>
 __global__ void __launch_bounds__(kPencilThreads)
 k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
              float2* __restrict__ g1, float2* __restrict__ g2, int nx, AxisPlan px_, int py,
-             int pz, int T, const float2* __restrict__ tw_x, const int* __restrict__ ctl) {
+             int pz, int T_, const float2* __restrict__ tw_x, const int* __restrict__ ctl) {
   if (refresh_off(ctl)) return;
   extern __shared__ float2 smem[];
-  const int px = px_.n, hz = pz / 2 + 1;
+  const int px = SX::N > 0 ? SX::N : px_.n, hz = pz / 2 + 1;
+  const int T = TT > 0 ? TT : T_;
   float2* twx = smem;
   float2* buf = twx + px;
   const int field = blockIdx.z, ky = blockIdx.y, tid = threadIdx.x;
@@ -904,7 +915,8 @@ k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
     }
   }
   __syncthreads();
-  lds_fft<true, kPencilThreads, kPencilMaxEl, kPencilRadix>(buf, px_, T, T, 1, false, twx, tid);
+  fft_axis<true, kPencilThreads, kPencilMaxEl, kPencilRadix, SX, TT, 1, false, TT, 0>(
+      buf, px_, T, T, 1, false, twx, tid);
   for (int e = tid; e < px * T; e += kPencilThreads) {
     const int x = e / T, t = e - x * T;
     const int kz = kz0 + t;
@@ -917,15 +929,18 @@ k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
 // ---------------------------------------------------------------------------
 // cf (nullable): column-test flags of the list; with CTL_PRUNE set, excluded
 // orientations need no spectra (neither pass reads them)
+// SY, SZ: compile-time plans of the padded y and z lengths, or DynPlan.
+template <class SY, class SZ>
 __global__ void __launch_bounds__(kSlabThreads)
 k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od,
                 float2* __restrict__ ispec, int max_ex, AxisPlan py_, AxisPlan pz_,
                 const float2* __restrict__ tw_y, const float2* __restrict__ tw_z,
                 const int* __restrict__ cf, const int* __restrict__ ctl) {
+  constexpr int FY = SY::N, FZ = SZ::N;
   if (search_off(ctl) || coarse_done(ctl)) return;
   if (cf && ctl && ctl[CTL_PRUNE] && !cf[blockIdx.y]) return;
   extern __shared__ float2 smem[];
-  const int py = py_.n, pz = pz_.n;
+  const int py = FY > 0 ? FY : py_.n, pz = FZ > 0 ? FZ : pz_.n;
   const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
   const OrientDesc d = od[o];
   if (x >= d.ex) return;
@@ -941,7 +956,7 @@ k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od
 #pragma unroll
     for (int s = 0; s < kSlabMaxEl; ++s) {
       const int e = tid + s * kSlabThreads;
-      const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
+      const int j = div_by<FZ>(e, pz, inv_pz), k = e - j * pz;
       iv[s] = (e < py * pz && j < d.ey && k < d.ez) ? base[oriented_index(d, x, j, k)] : 0;
     }
 #pragma unroll
@@ -951,8 +966,10 @@ k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od
     }
   }
   __syncthreads();
-  lds_fft<true, kSlabThreads, kSlabMaxEl>(buf, pz_, 1, d.ey, pz, true, twz, tid);
-  lds_fft<true, kSlabThreads, kSlabMaxEl>(buf, py_, pz, pz, 1, false, twy, tid);
+  fft_axis<true, kSlabThreads, kSlabMaxEl, kMaxRadix, SZ, 1, FZ, true, 0, FY>(
+      buf, pz_, 1, d.ey, pz, true, twz, tid);
+  fft_axis<true, kSlabThreads, kSlabMaxEl, kMaxRadix, SY, FZ, 1, false, FZ, 0>(
+      buf, py_, pz, pz, 1, false, twy, tid);
   float2* dst = ispec + ((size_t)o * max_ex + x) * py * pz;
   for (int e = tid; e < py * pz; e += kSlabThreads) dst[e] = buf[e];
 }
