@@ -1064,7 +1064,8 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
            dim3(256), 0,
            (const M128*)q.cols.p, q.cx, q.cy, (const M128*)q.cm.p, ccx, ccy, (const M128*)q.tf.p,
            c->nx, c->ny, (const M128*)q.im.p, mx, my, od, m, (const unsigned*)q.rl.p,
-           (const unsigned*)q.frl.p, (const int*)q.nrun.p, (const int*)q.ht.p, q.ty, q.fs.p, ctl);
+           (const unsigned*)q.frl.p, (const int*)q.nrun.p, (const int*)q.ht.p, q.ty, q.fs.p, ctl,
+           q.tx);
   };
   switch (q.fine_group) {
     case 4: test(k_column_test<4>); break;

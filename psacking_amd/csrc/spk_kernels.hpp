@@ -2195,7 +2195,7 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
               const M128* __restrict__ im, int mx, int my, const OrientDesc* __restrict__ od,
               int m, const unsigned* __restrict__ run_lists, const unsigned* __restrict__ fine_lists,
               const int* __restrict__ nrun, const int* __restrict__ ht, int ty,
-              int* __restrict__ cfs, const int* __restrict__ ctl) {
+              int* __restrict__ cfs, const int* __restrict__ ctl, int tx) {
   if (search_off(ctl)) return;
   const int o = blockIdx.y, tid = threadIdx.x;
   if (o >= m) return;
@@ -2205,6 +2205,21 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
   const int nblk = (nq + 63) / 64;
   if ((int)blockIdx.x >= nblk || d.nvz <= 0) return;
   volatile int* flag = cfs + kFeasHeader + o;
+  // whole-tray proof first: when no tray column is higher than H, the item's
+  // box at height H is empty at offset (0, 0), a valid offset when H < nvz
+  {
+    __shared__ int hw[4];
+    int h = 0;
+    for (int i = tid; i < tx * ty; i += 256) h = max(h, ht[i]);
+    for (int s2 = 32; s2 > 0; s2 >>= 1) h = max(h, __shfl_xor(h, s2, 64));
+    if ((tid & 63) == 0) hw[tid >> 6] = h;
+    __syncthreads();
+    h = max(max(hw[0], hw[1]), max(hw[2], hw[3]));
+    if (h < d.nvz) {
+      if (tid == 0) *flag = 1;
+      return;
+    }
+  }
   __shared__ unsigned cand_list[256];
   __shared__ M128 cbad[64];
   __shared__ int ncand, skip;
