@@ -343,18 +343,19 @@ def main(argv=None) -> int:
         device_sync()
 
     timed = [items[i] for i in order[:K]]
-    fits = [sum(1 for o in range(n_orient)
-                if all(it.shape[perms[o][a]] <= w.tray_size[a] for a in range(3)))
-            for it in timed]
     runs = {}
 
-    def finish(name, s, dt, res, t_setup, searched):
+    def finish(name, s, dt, res, t_setup):
         t = s.tray()
         kt = {n: s.timing_get(n) for n in KERNELS}
         units = {n: s.timing_units(n) for n in ("corr_x", "score_yz", "feas_x", "feas_yz")}
+        # orientations the scored search ran, counted on the device (the
+        # column test's pruned orientations and items decided with no FFT
+        # are not counted); feasibility pairs run at about a third of one
         runs[name] = {"dt": dt, "res": res, "density": float((t > 0).sum()) / V, "kt": kt,
                       "units": units,
-                      "setup": t_setup, "searched": searched,
+                      "setup": t_setup, "searched": units["score_yz"],
+                      "feas_pairs": units["feas_yz"],
                       "placed": sum(1 for r in res if r["found"]),
                       "fallbacks": sum(1 for r in res if r["path"] != 0),
                       "max_resid": max(float(r["residual"]) for r in res)}
@@ -395,22 +396,13 @@ def main(argv=None) -> int:
         _, _, dt, res0 = run_plain(False)
         s, t_setup, dt_ev, res = run_plain(True)
         assert identical(res0, res), "plain runs with and without kernel events differ"
-        finish("plain", s, dt, res, t_setup, sum(fits))
+        finish("plain", s, dt, res, t_setup)
         runs["plain"]["dt_events"] = dt_ev
     if want_learned:
         _, _, dt, res0, _, _ = run_learned(False)
         s, t_setup, dt_ev, res, props, counts = run_learned(True)
         assert identical(res0, res), "learned runs with and without kernel events differ"
-        searched = 0
-        for t, r in enumerate(res):
-            p = [int(o) for o in props[t] if o >= 0]
-            if not p:
-                searched += fits[t]
-            else:
-                searched += len(p)
-                if not r["found"] or r["orientation"] not in p:
-                    searched += fits[t]
-        finish("learned", s, dt, res, t_setup, searched)
+        finish("learned", s, dt, res, t_setup)
         runs["learned"]["counts"] = [int(c) for c in counts]
         runs["learned"]["dt_events"] = dt_ev
 
@@ -498,10 +490,12 @@ def main(argv=None) -> int:
             tk = json.load(fh)["kernels"].get(dom)
         if tk and tk["units_per_launch"] > 0:
             traffic = tk["hbm_bytes_per_launch"] * m_avg / tk["units_per_launch"]
-    # whole-loop fraction: algorithmic bytes of the loop's own method (every
-    # orientation it searches: B_OS = 48 Ph each, every placement: B_place =
-    # 48 Ph + 32 V) over the wall time at HBM peak
-    b_loop = 48.0 * Ph * r["searched"] + (48.0 * Ph + 32.0 * V) * r["placed"]
+    # whole-loop fraction: algorithmic bytes of the work the loop did (every
+    # orientation its scored search ran: B_OS = 48 Ph each; every feasibility
+    # pair: 16 Ph; every placement: B_place = 48 Ph + 32 V) over the wall time
+    # at HBM peak
+    b_loop = (48.0 * Ph * r["searched"] + 16.0 * Ph * r["feas_pairs"] +
+              (48.0 * Ph + 32.0 * V) * r["placed"])
     loop_frac = b_loop / r["dt"] / (HBM_PEAK_GBS * 1e9)
 
     cpu = None
@@ -540,6 +534,7 @@ def main(argv=None) -> int:
             "max_fft_residual": r["max_resid"],
             "setup_s": r["setup"],
             "loop_roofline": {"algorithmic_bytes": b_loop, "orientations_searched": r["searched"],
+                              "feasibility_pairs": r["feas_pairs"],
                               "achieved_GBps": b_loop / r["dt"] / 1e9, "frac": loop_frac},
         },
         "roofline": {

@@ -146,6 +146,21 @@ __device__ __forceinline__ int oriented_index(const OrientDesc& d, int w0, int w
   return (u[0] * d.by + u[1]) * d.bz + u[2];
 }
 
+// Workgroups of a launch go to the 8 XCDs in turn (workgroup b to XCD b mod 8)
+// and each XCD has its own L2.  xcd_rank() renumbers them so that each XCD
+// takes one contiguous run of ranks: the pencil kernels give neighbouring
+// ranks neighbouring rows, whose 128-byte lines they share (row pitches such
+// as 100 complex values = 800 bytes split lines between tiles), so a shared
+// line is fetched into one L2 instead of two.
+__device__ __forceinline__ int xcd_rank() {
+  constexpr int kXcd = 8;
+  const int nb = gridDim.x * gridDim.y * gridDim.z;
+  const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  const int q = nb / kXcd, r = nb - q * kXcd;
+  const int xcd = b % kXcd, slot = b / kXcd;
+  return (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
+}
+
 __device__ __forceinline__ void load_table(float2* dst, const float2* src, int n, int tid, int nt) {
   for (int q = tid; q < n; q += nt) dst[q] = src[q];
 }
@@ -890,9 +905,14 @@ k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
   const int T = TT > 0 ? TT : T_;
   float2* twx = smem;
   float2* buf = twx + px;
-  const int field = blockIdx.z, ky = blockIdx.y, tid = threadIdx.x;
+  // rank = (ky * tiles + tile) * 2 + field (xcd_rank): both fields of a tile,
+  // the tiles of one row and neighbouring rows run on one XCD, and the
+  // field-1 tiles past hz (which exit at once) are spread over all XCDs
+  const int rk = xcd_rank(), ntile = gridDim.x;
+  const int field = rk & 1, rest = rk >> 1;
+  const int ky = rest / ntile, tid = threadIdx.x;
   const int ncol = field == 0 ? pz : hz;
-  const int kz0 = blockIdx.x * T;
+  const int kz0 = (rest - ky * ntile) * T;
   if (kz0 >= ncol) return;
   load_table(twx, tw_x, px, tid, kPencilThreads);
   const float2* src = field == 0 ? t1 : t2;
@@ -994,9 +1014,13 @@ k_corr_x(const float2* __restrict__ g1, const float2* __restrict__ g2,
   float2* twx = smem;
   float2* buf = twx + px;
   float2* buf2 = buf + px * T;
-  const int ky = blockIdx.y, tid = threadIdx.x;
-  const int kz0 = blockIdx.x * T;
-  const int o0 = blockIdx.z * chunk;
+  // rank = (ky * tiles + tile) * chunks + chunk: the chunks of one tile, the
+  // tiles of one row and neighbouring rows run on one XCD (xcd_rank)
+  const int rk = xcd_rank(), nch = gridDim.z, ntile = gridDim.x;
+  const int rest = rk / nch;
+  const int ky = rest / ntile, tid = threadIdx.x;
+  const int kz0 = (rest - ky * ntile) * T;
+  const int o0 = (rk - rest * nch) * chunk;
   const int o1 = min(n_orient, o0 + chunk);
   load_table(twx, tw_x, px, tid, kPencilThreads);
   const bool has2 = kz0 < hz;
@@ -1081,9 +1105,13 @@ k_corr_x_f(const float2* __restrict__ g1, const float2* __restrict__ g2,
   float2* twx = smem;
   float2* buf = twx + PX;
   float2* buf2 = buf + PX * T;
-  const int ky = blockIdx.y, tid = threadIdx.x;
-  const int kz0 = blockIdx.x * T;
-  const int o0 = blockIdx.z * chunk;
+  // rank = (ky * tiles + tile) * chunks + chunk: the chunks of one tile, the
+  // tiles of one row and neighbouring rows run on one XCD (xcd_rank)
+  const int rk = xcd_rank(), nch = gridDim.z, ntile = gridDim.x;
+  const int rest = rk / nch;
+  const int ky = rest / ntile, tid = threadIdx.x;
+  const int kz0 = (rest - ky * ntile) * T;
+  const int o0 = (rk - rest * nch) * chunk;
   const int o1 = min(n_orient, o0 + chunk);
   load_table(twx, tw_x, PX, tid, NT);
   const bool has2 = kz0 < hz;
@@ -1185,6 +1213,109 @@ __device__ __forceinline__ void wg_argmin_write(double sc, int idx, long long p,
   }
 }
 
+// ---- score pass helpers ----------------------------------------------------
+// Low digit, z pass over row pairs.  After the y pass, row j of the slab holds
+// X_j, the half spectrum along z (kz < hz) of a real row; row j < h
+// (h = ceil(nvy / 2)) takes Z = X_j + i X_{j+h} over every kz (X at pz - k
+// is conj X at k), so the inverse z transform of row j carries the low-digit
+// sums of row j in its real part and of row j + h in its imaginary part: one
+// complex transform per two real rows.  In place: the thread of (j, k) alone
+// reads X_j[k] and X_{j+h}[k] and writes Z_j[k] and Z_j[pz - k] (a column
+// past the half spectrum).
+template <int NT, int PZ_, int HZ_>
+__device__ __forceinline__ void pair_low_rows(float2* buf, int pz_rt, int nvy, int tid) {
+  const int pz = PZ_ > 0 ? PZ_ : pz_rt, hz = PZ_ > 0 ? HZ_ : pz_rt / 2 + 1;
+  const int h = (nvy + 1) >> 1;
+  const float inv_hz = 1.0f / (float)hz;
+  for (int e = tid; e < h * hz; e += NT) {
+    const int j = PZ_ > 0 ? e / HZ_ : fdiv(e, hz, inv_hz), k = e - j * hz;
+    const float2 a = buf[j * pz + k];
+    const float2 b = j + h < nvy ? buf[(j + h) * pz + k] : make_float2(0.f, 0.f);
+    buf[j * pz + k] = make_float2(a.x - b.y, a.y + b.x);                      // a + i b
+    if (k > 0 && pz - k >= hz) buf[j * pz + pz - k] = make_float2(a.x + b.y, b.x - a.y);
+  }
+}
+
+// The valid offsets (j < nvy, k < nvz) of one slab, column by column: the
+// thread walks columns k = c0, c0 + cols, ... and in each the rows
+// j = g, g + G, ... in increasing order (cols = min(nvz, NT) threads per row
+// group, G = NT / cols groups, threads past cols * G idle).
+struct ColWalk {
+  int cols, G, g, c0;
+};
+template <int NT>
+__device__ __forceinline__ ColWalk col_walk(int nvz, int tid) {
+  ColWalk w;
+  w.cols = nvz < NT ? (nvz > 0 ? nvz : 1) : NT;
+  w.G = NT / w.cols;
+  w.g = tid / w.cols;
+  w.c0 = w.g < w.G ? tid - w.g * w.cols : nvz;
+  return w;
+}
+
+// rounded low-digit sums of the valid offsets into lo[j * nz + k]; returns the
+// largest rounding residual (rows paired by pair_low_rows)
+__device__ __forceinline__ float low_digit_sums(const float2* buf, int ldb, int* lo, int nz,
+                                                int nvy, int nvz, const ColWalk& w) {
+  const int h = (nvy + 1) >> 1;
+  float res = 0.f;
+  for (int k = w.c0; k < nvz; k += w.cols)
+    for (int j = w.g; j < nvy; j += w.G) {
+      const float v = j < h ? buf[j * ldb + k].x : buf[(j - h) * ldb + k].y;
+      const float r = rintf(v);
+      res = fmaxf(res, fabsf(v - r));
+      lo[j * nz + k] = (int)r;
+    }
+  return res;
+}
+
+// Collision and proximity of the valid offsets of slab x, and the slab's best
+// (score, index).  Within a column the score double(p) + pen[k] is a strictly
+// increasing function of the integer p while |p| + |pen| < 2^51 (doubles
+// there are spaced at most 0.5 apart, so integers one apart never round to
+// one value): the column's lowest score belongs to its smallest p, first in
+// row order on ties, and only that offset's score is formed (score_order_ok
+// checks the bound).  Ties across columns resolve by the lowest index.
+__device__ __forceinline__ void slab_scores(const float2* buf, int ldb, const int* lo, int x,
+                                            int ny, int nz, int nvy, int nvz, int lb,
+                                            long long addback, const double* __restrict__ pen,
+                                            unsigned char* __restrict__ fgrid,
+                                            long long* __restrict__ pgrid, size_t gbase,
+                                            const ColWalk& w, float& res, double& best,
+                                            int& bidx, long long& bp) {
+  for (int k = w.c0; k < nvz; k += w.cols) {
+    long long pm = 0;
+    int jm = -1;
+    for (int j = w.g; j < nvy; j += w.G) {
+      const float2 v = buf[j * ldb + k];
+      const float c = rintf(v.x), h = rintf(v.y);
+      res = fmaxf(res, fmaxf(fabsf(v.x - c), fabsf(v.y - h)));
+      // b * h with b = 2^lb: a shift of the (two's complement) high digit
+      const long long p =
+          (long long)((unsigned long long)(long long)h << lb) + lo[j * nz + k] + addback;
+      if (fgrid) {
+        const size_t g = gbase + ((size_t)x * ny + j) * nz + k;
+        fgrid[g] = c == 0.f ? 1 : 0;
+        pgrid[g] = p;
+      }
+      if (c == 0.f && (jm < 0 || p < pm)) { pm = p; jm = j; }
+    }
+    if (jm >= 0) {
+      const double sc = (double)pm + pen[k];
+      const int idx = (x * ny + jm) * nz + k;
+      if (better(sc, idx, best, bidx)) { best = sc; bidx = idx; bp = pm; }
+    }
+  }
+}
+
+// the bound under which slab_scores' column-wise minimum is exact: every
+// proximity is a sum of item values times field values, and the penalty
+// tables hold 1e8 (k / nz)^3 < 2^27
+__device__ __forceinline__ bool score_order_ok(const OrientDesc& d, const FieldStats* st) {
+  const double fm = fmax(fabs((double)st->mx), fabs((double)st->mn));
+  return (double)d.isum * fm < 0x1p50;
+}
+
 template <int RMAX>
 __global__ void __launch_bounds__(kSlabThreads)
 k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
@@ -1216,37 +1347,21 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
   load_table(twy, tw_y, py, tid, kSlabThreads);
   load_table(twz, tw_z, pz, tid, kSlabThreads);
   const FieldParams fp = field_params(st, (long long)nx * ny * nz);
-  const long long b = 1LL << fp.lb;
-  const long long addback = (b / 2 + fp.mu) * d.isum;
-  // --- low digit (real field, half spectrum) ---
+  const long long addback = ((1LL << fp.lb) / 2 + fp.mu) * d.isum;
+  const int nvy = d.nvy, nvz = d.nvz;
+  const ColWalk w = col_walk<kSlabThreads>(nvz, tid);
+  // --- low digit (real field, half spectrum): y pass, rows paired, z pass ---
   const float2* s2 = y2 + ((size_t)o * nx + x) * py * hz;
   stage_rows<kSlabThreads, kSlabMaxEl>(buf, s2, py * hz, hz, pz, tid);
   __syncthreads();
   lds_fft<false, kSlabThreads, kSlabMaxEl, RMAX>(buf, py_, pz, hz, 1, false, twy, tid);
-  const int nvy = d.nvy, nvz = d.nvz;
-  for (int e = tid; e < nvy * (pz - hz); e += kSlabThreads) {
-    const int j = e / (pz - hz), k = hz + (e - j * (pz - hz));
-    const float2 v = buf[j * pz + (pz - k)];
-    buf[j * pz + k] = make_float2(v.x, -v.y);
-  }
+  pair_low_rows<kSlabThreads, 0, 0>(buf, pz, nvy, tid);
   __syncthreads();
-  lds_fft<false, kSlabThreads, kSlabMaxEl, RMAX>(buf, pz_, 1, nvy, pz, true, twz, tid);
+  lds_fft<false, kSlabThreads, kSlabMaxEl, RMAX>(buf, pz_, 1, (nvy + 1) >> 1, pz, true, twz, tid);
   // the rounded low-digit sums wait in LDS behind the slab ([ny][nz] ints),
   // not in registers: the second slab FFT needs the whole register budget
   int* lo = reinterpret_cast<int*>(buf + py * pz);
-  float res = 0.f;
-  const float inv_pz = 1.0f / (float)pz;
-#pragma unroll
-  for (int s = 0; s < kSlabMaxEl; ++s) {
-    const int e = tid + s * kSlabThreads;
-    const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
-    if (j < nvy && k < nvz) {
-      const float v = buf[e].x;
-      const float r = rintf(v);
-      res = fmaxf(res, fabsf(v - r));
-      lo[j * nz + k] = (int)r;
-    }
-  }
+  float res = low_digit_sums(buf, pz, lo, nz, nvy, nvz, w);
   __syncthreads();
   // --- collision + high digit (complex field, full spectrum) ---
   const float2* s1 = y1 + ((size_t)o * nx + x) * py * pz;
@@ -1257,28 +1372,10 @@ k_score_yz(const float2* __restrict__ y1, const float2* __restrict__ y2,
   double best = 1e300;
   int bidx = INT_MAX;
   long long bp = 0;
-#pragma unroll
-  for (int s = 0; s < kSlabMaxEl; ++s) {
-    const int e = tid + s * kSlabThreads;
-    const int j = fdiv(e, pz, inv_pz), k = e - j * pz;
-    if (j < nvy && k < nvz) {
-      const float2 v = buf[e];
-      const float c = rintf(v.x), h = rintf(v.y);
-      res = fmaxf(res, fmaxf(fabsf(v.x - c), fabsf(v.y - h)));
-      const long long p = b * (long long)h + lo[j * nz + k] + addback;
-      const int idx = (x * ny + j) * nz + k;
-      if (fgrid) {
-        const size_t g = (size_t)o * nx * ny * nz + idx;
-        fgrid[g] = c == 0.f ? 1 : 0;
-        pgrid[g] = p;
-      }
-      if (c == 0.f) {
-        const double sc = (double)p + pen[k];
-        if (better(sc, idx, best, bidx)) { best = sc; bidx = idx; bp = p; }
-      }
-    }
-  }
-  wg_argmin_write(best, bidx, bp, res, fp.ok ? 0 : 1, out, tid, kSlabThreads);
+  slab_scores(buf, pz, lo, x, ny, nz, nvy, nvz, fp.lb, addback, pen, fgrid, pgrid,
+              (size_t)o * nx * ny * nz, w, res, best, bidx, bp);
+  wg_argmin_write(best, bidx, bp, res, fp.ok && score_order_ok(d, st) ? 0 : 1, out, tid,
+                  kSlabThreads);
 }
 
 // The score pass with the y and z plans fixed at compile time (FPlan): the
@@ -1320,7 +1417,8 @@ k_score_yz_f(const float2* __restrict__ y1, const float2* __restrict__ y2,
   const FieldParams fp = field_params(st, (long long)nx * ny * nz);
   const long long addback = ((1LL << fp.lb) / 2 + fp.mu) * d.isum;
   const int nvy = d.nvy, nvz = d.nvz;
-  // --- low digit (real field, half spectrum) ---
+  const ColWalk w = col_walk<NT>(nvz, tid);
+  // --- low digit (real field, half spectrum): y pass, rows paired, z pass ---
   {
     const float2* s2 = y2 + ((size_t)o * nx + x) * PY * HZ;
     float2 v[ELH];
@@ -1340,27 +1438,12 @@ k_score_yz_f(const float2* __restrict__ y1, const float2* __restrict__ y2,
   }
   __syncthreads();
   FixedFft<false, NT, SY, PZ, 1, false, HZ, 0>::run(buf, 0, twy, tid);
-  for (int e = tid; e < nvy * (PZ - HZ); e += NT) {
-    const int j = e / (PZ - HZ), k = HZ + (e - j * (PZ - HZ));
-    const float2 v = buf[j * PZ + (PZ - k)];
-    buf[j * PZ + k] = make_float2(v.x, -v.y);
-  }
+  pair_low_rows<NT, PZ, HZ>(buf, PZ, nvy, tid);
   __syncthreads();
-  FixedFft<false, NT, SZ, 1, PZ, true, 0, PY>::run(buf, nvy, twz, tid);
+  FixedFft<false, NT, SZ, 1, PZ, true, 0, PY>::run(buf, (nvy + 1) >> 1, twz, tid);
   // the rounded low-digit sums wait in LDS behind the slab ([ny][nz] ints)
   int* lo = reinterpret_cast<int*>(buf + PY * PZ);
-  float res = 0.f;
-#pragma unroll
-  for (int s = 0; s < EL; ++s) {
-    const int e = tid + s * NT;
-    const int j = e / PZ, k = e - j * PZ;
-    if (j < nvy && k < nvz) {
-      const float v = buf[e].x;
-      const float r = rintf(v);
-      res = fmaxf(res, fabsf(v - r));
-      lo[j * nz + k] = (int)r;
-    }
-  }
+  float res = low_digit_sums(buf, PZ, lo, nz, nvy, nvz, w);
   __syncthreads();
   // --- collision + high digit (complex field, full spectrum) ---
   {
@@ -1383,31 +1466,9 @@ k_score_yz_f(const float2* __restrict__ y1, const float2* __restrict__ y2,
   double best = 1e300;
   int bidx = INT_MAX;
   long long bp = 0;
-  const int lb = fp.lb;
-#pragma unroll
-  for (int s = 0; s < EL; ++s) {
-    const int e = tid + s * NT;
-    const int j = e / PZ, k = e - j * PZ;
-    if (j < nvy && k < nvz) {
-      const float2 v = buf[e];
-      const float c = rintf(v.x), h = rintf(v.y);
-      res = fmaxf(res, fmaxf(fabsf(v.x - c), fabsf(v.y - h)));
-      // b * h with b = 2^lb: a shift of the (two's complement) high digit
-      const long long p =
-          (long long)((unsigned long long)(long long)h << lb) + lo[j * nz + k] + addback;
-      const int idx = (x * ny + j) * nz + k;
-      if (fgrid) {
-        const size_t g = (size_t)o * nx * ny * nz + idx;
-        fgrid[g] = c == 0.f ? 1 : 0;
-        pgrid[g] = p;
-      }
-      if (c == 0.f) {
-        const double sc = (double)p + pen[k];
-        if (better(sc, idx, best, bidx)) { best = sc; bidx = idx; bp = p; }
-      }
-    }
-  }
-  wg_argmin_write(best, bidx, bp, res, fp.ok ? 0 : 1, out, tid, NT);
+  slab_scores(buf, PZ, lo, x, ny, nz, nvy, nvz, fp.lb, addback, pen, fgrid, pgrid,
+              (size_t)o * nx * ny * nz, w, res, best, bidx, bp);
+  wg_argmin_write(best, bidx, bp, res, fp.ok && score_order_ok(d, st) ? 0 : 1, out, tid, NT);
 }
 
 // ---------------------------------------------------------------------------
@@ -1493,10 +1554,13 @@ k_feas_x(const float2* __restrict__ g1, const float2* __restrict__ ispec,
   const int px = px_.n;
   float2* twx = smem;
   float2* buf = twx + px;
-  const int ky = blockIdx.y, tid = threadIdx.x;
-  const int kz0 = blockIdx.x * T;
+  // rank = (ky * tiles + tile) * chunks + chunk, as in k_corr_x
+  const int rk = xcd_rank(), nch = gridDim.z, ntile = gridDim.x;
+  const int rest = rk / nch;
+  const int ky = rest / ntile, tid = threadIdx.x;
+  const int kz0 = (rest - ky * ntile) * T;
   const int np = (m + 1) / 2;
-  const int p0 = blockIdx.z * chunk;
+  const int p0 = (rk - rest * nch) * chunk;
   const int p1 = min(np, p0 + chunk);
   load_table(twx, tw_x, px, tid, kPencilThreads);
   const int kym = ky ? py - ky : 0;
