@@ -279,6 +279,13 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  pick of the proposals as a kernel, no host round trip per
  *                  item) when the column test screens every item, min_conf
  *                  is 0 and the certificate is off; 0: host-driven.
+ *   "prescreen"    items per round of the device-driven loops (spk_pack,
+ *                  spk_pack_gather, spk_pack_learned; default 32): at the
+ *                  start of a round one batched column test over its items
+ *                  decides those that fit nowhere on the tray as it is (they
+ *                  fit nowhere later either: placements only add occupied
+ *                  voxels), and only the others are enqueued; 0: one round,
+ *                  no batched test.
  *   "many_workers" host workers of spk_pack_many (set on its first
  *                  context; default 8).
  *   "inc_dist"     1 (default): after a placement the distance field is

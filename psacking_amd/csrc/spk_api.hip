@@ -198,6 +198,9 @@ struct spk_ctx {
   DevBuf<int> learn_gate;                // device-driven learned loop: proposer gate word,
   DevBuf<int> learn_pick, learn_used;    // per-item pick (fitting indices) and pick kind
   bool learn_dev = true;                 // option "learn_dev": device-driven learned loop
+  int prescreen = 32;                    // option "prescreen": items per round of the device
+                                         // loops, screened together first (0: one round, none)
+  DevBuf<int> pre_fs;                    // flags of a batched screen
   int many_workers = 8;                  // option "many_workers": spk_pack_many host workers
   bool graph = false;                    // option "graph": device loops captured as a hipGraph
   bool fixed_fft = true;                 // option "fixed_fft": compile-time-plan score kernels
@@ -1036,9 +1039,10 @@ const int* ctl_feas_on(spk_ctx* c) {
 // the flags are only left for the host (no k_coarse_decide).
 void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_exy,
                     const int* ctl, PackOut* out = nullptr, int t = 0, int halt_not_found = 0,
-                    bool clear_flags = false, bool decide = true) {
+                    bool clear_flags = false, bool decide = true, int* flags = nullptr) {
   spk_ctx::Coarse& q = c->co;
   if (!q.ready) return;
+  int* fs = flags ? flags : q.fs.p;
   const int ccx = (max_ex + 1) / 2, ccy = q.cy;
   const int mx = max_ex, my = c->ny;
   q.cm.ensure((size_t)m * ccx * ccy);
@@ -1049,7 +1053,7 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
   if (max_exy <= kMaskCols) {
     launch(c, "item_masks", k_item_masks, dim3(m), dim3(kMaskThreads), 0, (const int*)c->items.p,
            (const M128*)q.lines.p, od, m, q.cm.p, ccx, ccy, q.im.p, mx, my, q.rl.p, q.frl.p,
-           q.nrun.p, clear_flags ? q.fs.p : (int*)nullptr, ctl);
+           q.nrun.p, clear_flags ? fs : (int*)nullptr, ctl);
   } else {
     launch(c, "core_columns", k_core_columns, dim3((ccx * ccy + 3) / 4, m), dim3(256), 0,
            (const int*)c->items.p, od, m, q.cm.p, ccx, ccy, ctl);
@@ -1064,7 +1068,7 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
            dim3(256), 0,
            (const M128*)q.cols.p, q.cx, q.cy, (const M128*)q.cm.p, ccx, ccy, (const M128*)q.tf.p,
            c->nx, c->ny, (const M128*)q.im.p, mx, my, od, m, (const unsigned*)q.rl.p,
-           (const unsigned*)q.frl.p, (const int*)q.nrun.p, (const int*)q.ht.p, q.ty, q.fs.p, ctl,
+           (const unsigned*)q.frl.p, (const int*)q.nrun.p, (const int*)q.ht.p, q.ty, fs, ctl,
            q.tx);
   };
   switch (q.fine_group) {
@@ -1075,7 +1079,7 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
     default: test(k_column_test<64>); break;
   }
   if (decide)
-    launch(c, "coarse_decide", k_coarse_decide, dim3(1), dim3(64), 0, (const int*)q.fs.p, m,
+    launch(c, "coarse_decide", k_coarse_decide, dim3(1), dim3(64), 0, (const int*)fs, m,
          c->fstate.p, out, t, halt_not_found, (int*)ctl);
 }
 
@@ -1607,6 +1611,48 @@ int place_exact(spk_ctx* c, const ExactBest& eb, const ItemPlan& pl,
   return 1;
 }
 
+// Items that fit nowhere on the current tray fit nowhere on any later tray of
+// the same run (a placement only adds occupied voxels), so the greedy loops
+// decide them (not found) before their turn, with no per-item work: one
+// batched column test over the fitting orientations of items [t_begin, t_end)
+// on the current tray.  live[t - t_begin] = 1 when the item fits somewhere
+// now or has no fitting orientation to test (then the loop skips it anyway);
+// every item is live when the column test does not apply (split plan, tray
+// z > 128, option "coarse" off).
+std::vector<char> prescreen(spk_ctx* c, const std::vector<ItemPlan>& plans, int t_begin,
+                            int t_end) {
+  std::vector<char> live((size_t)std::max(0, t_end - t_begin), 1);
+  spk_ctx::Coarse& q = c->co;
+  if (!q.ready || !q.on || t_end <= t_begin) return live;
+  int od0 = -1, od1 = 0, max_ex = 1, max_exy = 1;
+  for (int t = t_begin; t < t_end; ++t) {
+    const ItemPlan& pl = plans[t];
+    if (pl.m == 0) continue;
+    if (od0 < 0) od0 = pl.od_off;
+    od1 = pl.od_off + pl.m;
+    max_ex = std::max(max_ex, pl.max_ex);
+    max_exy = std::max(max_exy, pl.max_exy);
+  }
+  if (od0 < 0) return live;
+  const int m = od1 - od0;   // the fitting orientations of a range are contiguous
+  c->pre_fs.ensure((size_t)kFeasHeader + m);
+  HIP_OK(hipMemsetAsync(c->pre_fs.p, 0, ((size_t)kFeasHeader + m) * sizeof(int), c->stream));
+  enqueue_coarse(c, c->od_all.p + od0, m, max_ex, max_exy, nullptr, nullptr, 0, 0, false, false,
+                 c->pre_fs.p);
+  std::vector<int> f((size_t)m);
+  download(c, f.data(), c->pre_fs.p + kFeasHeader, (size_t)m * sizeof(int));
+  sync(c);
+  for (int t = t_begin; t < t_end; ++t) {
+    const ItemPlan& pl = plans[t];
+    if (pl.m == 0) continue;
+    bool any = false;
+    for (int i = 0; i < pl.m && !any; ++i) any = f[(size_t)(pl.od_off - od0 + i)] != 0;
+    live[t - t_begin] = any ? 1 : 0;
+  }
+  return live;
+}
+
+
 // The greedy loop with every item enqueued back to back: search, k_decide,
 // placement and refresh of item t+1 follow item t on the stream with no host
 // round trip.  k_decide hands a search back to the host (CTL_HALT) when its
@@ -1645,13 +1691,23 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
     }
     presize_loop(c, mm, mx);
   }
+  // rounds of c->prescreen items (one round when it is 0 or records are
+  // written): items a batched column test finds infeasible on the tray at
+  // the start of their round are decided there (skip), the others enqueued
+  const bool rounds = c->prescreen > 0 && !rec_feat && !rec_scores;
+  std::vector<char> skip(n_items, 0);
   while (t0 < n_items) {
+    const int t_end = rounds ? std::min(n_items, t0 + c->prescreen) : n_items;
+    if (rounds) {
+      const std::vector<char> live = prescreen(c, plans, t0, t_end);
+      for (int t = t0; t < t_end; ++t) skip[t] = live[t - t0] ? 0 : 1;
+    }
     const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
     upload(c, c->ctl.p, ctl0, sizeof(ctl0));
     LoopGraph lg(c, c->graph && !rec_feat && !rec_scores);
-    for (int t = t0; t < n_items; ++t) {
+    for (int t = t0; t < t_end; ++t) {
       const ItemPlan& pl = plans[t];
-      if (pl.m == 0) continue;   // nothing fits: no search, tray unchanged
+      if (pl.m == 0 || skip[t]) continue;   // nothing fits / fits nowhere: tray unchanged
       if (rec_feat)
         nn_tray_features(c->stream, c->tray.p, c->nx, c->ny, c->nz, c->heights.p,
                          rec_feat + (size_t)t * kNnTrayFeat);
@@ -1663,15 +1719,15 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
     }
     lg.run();
     int ctl[CTL_WORDS];
-    std::vector<PackOut> po(n_items - t0);
+    std::vector<PackOut> po(t_end - t0);
     download(c, ctl, c->ctl.p, sizeof(ctl));
     download(c, po.data(), c->pout.p + t0, po.size() * sizeof(PackOut));
     sync(c);
     if (c->timing) c->flush_timing();
-    const int stop = ctl[CTL_HALT] ? ctl[CTL_HALT_ITEM] : n_items;
+    const int stop = ctl[CTL_HALT] ? ctl[CTL_HALT_ITEM] : t_end;
     for (int t = t0; t < stop; ++t) {
       const ItemPlan& pl = plans[t];
-      if (pl.m == 0) {
+      if (pl.m == 0 || skip[t]) {
         to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, &results[t]);
         continue;
       }
@@ -1680,7 +1736,10 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
                 po[t - t0].collision, s.prox, s.resid, 0, &results[t]);
       if (s.found) ++placed;
     }
-    if (!ctl[CTL_HALT]) break;
+    if (!ctl[CTL_HALT]) {
+      t0 = t_end;
+      continue;
+    }
     // item `stop` goes through the exact path on the host's orders
     const int h = stop;
     const ItemPlan& pl = plans[h];
@@ -2152,14 +2211,23 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     for (const ItemPlan& pl : plans) mm = std::max(mm, pl.m);
     presize_loop(c, mm, ws_ex);
   }
+  // rounds of c->prescreen items, as in pack_pipelined: items a batched
+  // column test finds infeasible at the start of their round are decided
+  // there (the column test would decide them at their turn)
+  std::vector<char> skip(n_items, 0);
   while (t0 < n_items) {
+    const int t_end = c->prescreen > 0 ? std::min(n_items, t0 + c->prescreen) : n_items;
+    if (c->prescreen > 0) {
+      const std::vector<char> live = prescreen(c, plans, t0, t_end);
+      for (int t = t0; t < t_end; ++t) skip[t] = live[t - t0] ? 0 : 1;
+    }
     const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
     upload(c, c->ctl.p, ctl0, sizeof(ctl0));
     int* ctl = c->ctl.p;
     LoopGraph lg(c, c->graph);
-    for (int t = t0; t < n_items; ++t) {
+    for (int t = t0; t < t_end; ++t) {
       const ItemPlan& pl = plans[t];
-      if (pl.m == 0) continue;   // nothing fits: no search, tray unchanged
+      if (pl.m == 0 || skip[t]) continue;   // nothing fits / fits nowhere: tray unchanged
       const OrientDesc* od_fit = c->od_all.p + pl.od_off;
       launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, c->co.fs.p, pl.m, 0,
              ctl, (const FieldStats*)nullptr, (const FieldStats*)nullptr, c->pout.p, t);
@@ -2196,7 +2264,7 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     }
     lg.run();
     int cw[CTL_WORDS];
-    const int nrem = n_items - t0;
+    const int nrem = t_end - t0;
     std::vector<PackOut> po(nrem);
     lp.assign((size_t)nrem * topk, -1);
     lu.assign(nrem, -1);
@@ -2206,13 +2274,18 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
     download(c, lu.data(), c->learn_used.p + t0, lu.size() * sizeof(int));
     sync(c);
     if (c->timing) c->flush_timing();
-    const int stop = cw[CTL_HALT] ? cw[CTL_HALT_ITEM] : n_items;
+    const int stop = cw[CTL_HALT] ? cw[CTL_HALT_ITEM] : t_end;
     for (int t = t0; t < stop; ++t) {
       const ItemPlan& pl = plans[t];
       if (proposals)
         for (int j = 0; j < topk; ++j) proposals[(size_t)t * topk + j] = -1;
       const int r = t - t0;
       if (pl.m == 0) {
+        to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, &results[t]);
+        continue;
+      }
+      if (skip[t]) {   // decided by the batched screen, as the column test would
+        ++cnt[4];
         to_result(c, false, 0, 0.0, -1, 0, 0, 0.f, 0, &results[t]);
         continue;
       }
@@ -2231,7 +2304,10 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
         for (int j = 0; j < topk; ++j)
           proposals[(size_t)t * topk + j] = pk[j] >= 0 ? pl.map[pk[j]] : -1;
     }
-    if (!cw[CTL_HALT]) break;
+    if (!cw[CTL_HALT]) {
+      t0 = t_end;
+      continue;
+    }
     // item `stop` on the exact path: over its pick, then (no proposal
     // feasible) over every fitting orientation
     const int h = stop, r = h - t0;
@@ -3314,6 +3390,11 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
   }
   if (std::string(name) == "learn_dev") {
     c->learn_dev = value != 0.0;
+    return SPK_OK;
+  }
+  if (std::string(name) == "prescreen") {
+    if (value < 0 || value > 1e6) return fail(SPK_EINVAL, "prescreen must lie in [0, 1e6]");
+    c->prescreen = (int)value;
     return SPK_OK;
   }
   if (std::string(name) == "inc_dist") {

@@ -419,12 +419,15 @@ def test_pack_many_uneven_trays(P):
         np.testing.assert_array_equal(s.tray(), exp.tray)
 
 
-@pytest.mark.parametrize("feas,coarse", [(0, 1), (1, 0), (1, 1), (2, 0), (2, 1)])
-def test_feasibility_pass_keeps_results(P, feas, coarse):
+@pytest.mark.parametrize("feas,coarse,prescreen", [(0, 1, 32), (1, 0, 32), (1, 1, 32), (2, 0, 32),
+                                                   (2, 1, 32), (1, 1, 2), (1, 1, 0)])
+def test_feasibility_pass_keeps_results(P, feas, coarse, prescreen):
     """A tray that fills up: later items have no feasible orientation.  The
     feasibility pass (off, after a non-placement, every item), with and
     without its coarse 2x2x2-block pre-pass, must leave every outcome and the
-    final tray equal to the oracle's."""
+    final tray equal to the oracle's; so must rounds of 2 items with a
+    batched column test at the start of each (items that fit nowhere then
+    are decided there) and a single round with none."""
     from psacking_amd import workloads as W
     from psacking_amd.rotations import orientation_maps
     w = W.blobs(seed=51, n=14, tray=(48, 40, 32))
@@ -434,8 +437,9 @@ def test_feasibility_pass_keeps_results(P, feas, coarse):
     s = P.Session(0)
     s.set_option("feas", feas)
     s.set_option("coarse", coarse)
+    s.set_option("prescreen", prescreen)
     s.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
-    say("feasibility pass mode", feas, "coarse", coarse)
+    say("feasibility pass mode", feas, "coarse", coarse, "prescreen", prescreen)
     res = s.pack(w.items, np.array(order, np.int32), perms, flips, "cube", False, 0)
     exp = O.pack_voxels(w.items, w.tray_size, num_orientations=24, order=order,
                         metric="edt", sources="nonzero")

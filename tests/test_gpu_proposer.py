@@ -257,16 +257,20 @@ def test_pack_learned_replays_on_oracle(P, PR, topk, min_conf, resid, screen_all
     np.testing.assert_array_equal(s.tray(), tray)
 
 
-def test_pack_learned_on_a_filling_tray(P, PR):
-    """Most late items fit nowhere: the feasibility pass decides them; every
-    placement still replays exactly on the oracle."""
+@pytest.mark.parametrize("prescreen", [32, 2, 0])
+def test_pack_learned_on_a_filling_tray(P, PR, prescreen):
+    """Most late items fit nowhere: the column test decides them, at their
+    turn or in the batched test at the start of a round (rounds of 32 and
+    of 2 items, and one round with no batched test); every placement still
+    replays exactly on the oracle."""
     from psacking_amd.rotations import orientation_maps
     w, order = _workload(51, 14, (48, 40, 32))
     perms, flips = orientation_maps(24)
     nn = PR.Proposer(0, 8, PR.init_params(52))
     s = P.Session(0)
+    s.set_option("prescreen", prescreen)
     s.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
-    say("pack learned on a filling tray")
+    say("pack learned on a filling tray, prescreen", prescreen)
     res, props, counts = PR.pack_learned(s, nn, w.items, order, perms, flips, "cube", topk=4)
     say("counts", list(counts))
     assert counts[4] > 0
