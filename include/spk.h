@@ -280,7 +280,7 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  item) when the column test screens every item, min_conf
  *                  is 0 and the certificate is off; 0: host-driven.
  *   "prescreen"    items per round of the device-driven loops (spk_pack,
- *                  spk_pack_gather, spk_pack_learned; default 32): at the
+ *                  spk_pack_gather, spk_pack_learned; default 16): at the
  *                  start of a round one batched column test over its items
  *                  decides those that fit nowhere on the tray as it is (they
  *                  fit nowhere later either: placements only add occupied

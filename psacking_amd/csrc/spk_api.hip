@@ -198,7 +198,7 @@ struct spk_ctx {
   DevBuf<int> learn_gate;                // device-driven learned loop: proposer gate word,
   DevBuf<int> learn_pick, learn_used;    // per-item pick (fitting indices) and pick kind
   bool learn_dev = true;                 // option "learn_dev": device-driven learned loop
-  int prescreen = 32;                    // option "prescreen": items per round of the device
+  int prescreen = 16;                    // option "prescreen": items per round of the device
                                          // loops, screened together first (0: one round, none)
   DevBuf<int> pre_fs;                    // flags of a batched screen
   int many_workers = 8;                  // option "many_workers": spk_pack_many host workers

@@ -419,8 +419,8 @@ def test_pack_many_uneven_trays(P):
         np.testing.assert_array_equal(s.tray(), exp.tray)
 
 
-@pytest.mark.parametrize("feas,coarse,prescreen", [(0, 1, 32), (1, 0, 32), (1, 1, 32), (2, 0, 32),
-                                                   (2, 1, 32), (1, 1, 2), (1, 1, 0)])
+@pytest.mark.parametrize("feas,coarse,prescreen", [(0, 1, 16), (1, 0, 16), (1, 1, 16), (2, 0, 16),
+                                                   (2, 1, 16), (1, 1, 2), (1, 1, 0)])
 def test_feasibility_pass_keeps_results(P, feas, coarse, prescreen):
     """A tray that fills up: later items have no feasible orientation.  The
     feasibility pass (off, after a non-placement, every item), with and

@@ -257,10 +257,10 @@ def test_pack_learned_replays_on_oracle(P, PR, topk, min_conf, resid, screen_all
     np.testing.assert_array_equal(s.tray(), tray)
 
 
-@pytest.mark.parametrize("prescreen", [32, 2, 0])
+@pytest.mark.parametrize("prescreen", [16, 2, 0])
 def test_pack_learned_on_a_filling_tray(P, PR, prescreen):
     """Most late items fit nowhere: the column test decides them, at their
-    turn or in the batched test at the start of a round (rounds of 32 and
+    turn or in the batched test at the start of a round (rounds of 16 and
     of 2 items, and one round with no batched test); every placement still
     replays exactly on the oracle."""
     from psacking_amd.rotations import orientation_maps
