@@ -294,6 +294,9 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  sources), computed on the item's box and one pass over the
  *                  tray), when the field is this library's own and the item's
  *                  x extent is at most 128; 0: the full transform every time.
+ *   "inc_masks"    1 (default): after a placement the column test's tray masks
+ *                  are rebuilt for the columns under the placed box only (a
+ *                  placement changes no other column); 0: the whole tray.
  *   "dist_y"       distance y pass: 0 outward search, 1 Felzenszwalb
  *                  envelope, 2 full scan (default); "dist_x": 0 outward
  *                  search (default), 2 full scan.

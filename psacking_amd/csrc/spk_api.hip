@@ -223,6 +223,8 @@ struct spk_ctx {
   int dist_y = 2;                        // option "dist_y": 1 envelope, 2 full scan
   int dist_x = 0;                        // option "dist_x": 0 outward search, 2 full scan
   bool inc_dist = true;                  // option "inc_dist": incremental field after a placement
+  bool inc_masks = true;                 // option "inc_masks": column-test masks updated under
+                                         // the placed box only
   // column test before the feasibility pass (option "coarse")
   struct Coarse {
     bool on = true;
@@ -616,7 +618,11 @@ void run_distance(spk_ctx* c, const int* d_grid, int* d_out, int* d_tmp, FieldSt
          (size_t)nx * TK * sizeof(int), (const int*)d_tmp, d_out, d_st, nx, ny, nz, fill, ctl);
 }
 
-void build_spectra(spk_ctx* c, const int* ctl = nullptr) {
+struct PlaceRef;
+void update_coarse(spk_ctx* c, const int* ctl, const PlaceRef& pr);
+// the cached spectra of the tray, then the column test's tray masks: rebuilt
+// (pr == nullptr) or updated for the placement pr
+void build_spectra(spk_ctx* c, const int* ctl = nullptr, const PlaceRef* pr = nullptr) {
   const int py = c->py.n, pz = c->pz.n, hz = pz / 2 + 1;
   c->t1.ensure((size_t)c->nx * py * pz);
   c->t2.ensure((size_t)c->nx * py * hz);
@@ -645,7 +651,8 @@ void build_spectra(spk_ctx* c, const int* ctl = nullptr) {
   launch(c, "tray_x_fwd", tray_x_kernel(c), dim3((pz + c->T - 1) / c->T, py, 2),
          dim3(kPencilThreads), pencil_shmem(c), (const float2*)c->t1.p, (const float2*)c->t2.p,
          c->g1.p, c->g2.p, c->nx, c->px, py, pz, c->T, tw, ctl);
-  build_coarse_spectra(c, ctl);
+  if (pr && c->inc_masks) update_coarse(c, ctl, *pr);
+  else build_coarse_spectra(c, ctl);
 }
 
 // The placement a refresh follows, for the incremental distance update: the
@@ -659,6 +666,23 @@ struct PlaceRef {
   int max_ex = 0;   // largest x extent among the orientations od may name
   int max_dim = 0;  // largest extent along any axis among them (0: unknown)
 };
+
+// The column test's tray masks after the placement pr: only the columns under
+// the placed box are rebuilt (k_tray_masks_inc), when the box side is known.
+void update_coarse(spk_ctx* c, const int* ctl, const PlaceRef& pr) {
+  spk_ctx::Coarse& q = c->co;
+  if (!q.ready) return;
+  if (pr.max_dim <= 0) {
+    build_coarse_spectra(c, ctl);
+    return;
+  }
+  const int bound = std::min(pr.max_dim, std::max(c->nx, c->ny));
+  const int nb1 = bound / 2 + 1;
+  const int waves = bound * bound + nb1 * nb1;
+  launch(c, "tray_masks_inc", k_tray_masks_inc, dim3((waves + 3) / 4), dim3(256), 0,
+         (const int*)c->tray.p, c->nx, c->ny, c->nz, q.tf.p, q.cols.p, q.cx, q.cy, q.ht.p, q.ty,
+         pr.so, pr.od, pr.orient, pr.qx, pr.qy, bound, ctl);
+}
 
 // k_inc_zy's LDS: the item box's column flags (rows x z extent) and the
 // z-pass costs (rows x nz), rows and z extent bounded by the item's extents
@@ -725,7 +749,7 @@ void refresh_field_and_spectra(spk_ctx* c, const int* dist_host, const int* ctl 
     run_distance(c, c->tray.p, c->phi.p, c->tmpi.p, c->stats.p, c->nx, c->ny, c->nz, c->metric,
                  c->sources, ctl);
   }
-  build_spectra(c, ctl);
+  build_spectra(c, ctl, pr);
 }
 
 // ---- orientation descriptors ----------------------------------------------
@@ -1615,13 +1639,13 @@ int place_exact(spk_ctx* c, const ExactBest& eb, const ItemPlan& pl,
 // the same run (a placement only adds occupied voxels), so the greedy loops
 // decide them (not found) before their turn, with no per-item work: one
 // batched column test over the fitting orientations of items [t_begin, t_end)
-// on the current tray.  live[t - t_begin] = 1 when the item fits somewhere
-// now or has no fitting orientation to test (then the loop skips it anyway);
-// every item is live when the column test does not apply (split plan, tray
-// z > 128, option "coarse" off).
-std::vector<char> prescreen(spk_ctx* c, const std::vector<ItemPlan>& plans, int t_begin,
-                            int t_end) {
-  std::vector<char> live((size_t)std::max(0, t_end - t_begin), 1);
+// on the current tray.  Returns per item of the range the number of its
+// fitting orientations feasible now, an upper bound for the rest of the run:
+// 0 = the item fits nowhere; -1 = not tested (no fitting orientation, or the
+// column test does not apply: split plan, tray z > 128, option "coarse" off).
+std::vector<int> prescreen(spk_ctx* c, const std::vector<ItemPlan>& plans, int t_begin,
+                           int t_end) {
+  std::vector<int> live((size_t)std::max(0, t_end - t_begin), -1);
   spk_ctx::Coarse& q = c->co;
   if (!q.ready || !q.on || t_end <= t_begin) return live;
   int od0 = -1, od1 = 0, max_ex = 1, max_exy = 1;
@@ -1645,9 +1669,9 @@ std::vector<char> prescreen(spk_ctx* c, const std::vector<ItemPlan>& plans, int 
   for (int t = t_begin; t < t_end; ++t) {
     const ItemPlan& pl = plans[t];
     if (pl.m == 0) continue;
-    bool any = false;
-    for (int i = 0; i < pl.m && !any; ++i) any = f[(size_t)(pl.od_off - od0 + i)] != 0;
-    live[t - t_begin] = any ? 1 : 0;
+    int n = 0;
+    for (int i = 0; i < pl.m; ++i) n += f[(size_t)(pl.od_off - od0 + i)] != 0 ? 1 : 0;
+    live[t - t_begin] = n;
   }
   return live;
 }
@@ -1699,8 +1723,8 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
   while (t0 < n_items) {
     const int t_end = rounds ? std::min(n_items, t0 + c->prescreen) : n_items;
     if (rounds) {
-      const std::vector<char> live = prescreen(c, plans, t0, t_end);
-      for (int t = t0; t < t_end; ++t) skip[t] = live[t - t0] ? 0 : 1;
+      const std::vector<int> live = prescreen(c, plans, t0, t_end);
+      for (int t = t0; t < t_end; ++t) skip[t] = live[t - t0] == 0 ? 1 : 0;
     }
     const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
     upload(c, c->ctl.p, ctl0, sizeof(ctl0));
@@ -2214,12 +2238,18 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   // rounds of c->prescreen items, as in pack_pipelined: items a batched
   // column test finds infeasible at the start of their round are decided
   // there (the column test would decide them at their turn)
-  std::vector<char> skip(n_items, 0);
+  // An item with at most topk orientations feasible at the start of its round
+  // has at most topk at its turn: the proposer would be gated off, so it is
+  // not enqueued (k_learn_pick then takes every feasible orientation).
+  std::vector<char> skip(n_items, 0), propose(n_items, 1);
   while (t0 < n_items) {
     const int t_end = c->prescreen > 0 ? std::min(n_items, t0 + c->prescreen) : n_items;
     if (c->prescreen > 0) {
-      const std::vector<char> live = prescreen(c, plans, t0, t_end);
-      for (int t = t0; t < t_end; ++t) skip[t] = live[t - t0] ? 0 : 1;
+      const std::vector<int> live = prescreen(c, plans, t0, t_end);
+      for (int t = t0; t < t_end; ++t) {
+        skip[t] = live[t - t0] == 0 ? 1 : 0;
+        propose[t] = live[t - t0] < 0 || live[t - t0] > topk ? 1 : 0;
+      }
     }
     const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
     upload(c, c->ctl.p, ctl0, sizeof(ctl0));
@@ -2232,16 +2262,18 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, c->co.fs.p, pl.m, 0,
              ctl, (const FieldStats*)nullptr, (const FieldStats*)nullptr, c->pout.p, t);
       enqueue_coarse(c, od_fit, pl.m, pl.max_ex, pl.max_exy, ctl, c->pout.p, t, 0);
-      launch(c, "learn_gate", k_learn_gate, dim3(1), dim3(64), 0, (const int*)c->co.fs.p, pl.m,
-             topk, c->learn_gate.p, (const int*)ctl);
-      nn->gate = c->learn_gate.p;
-      timed_group(c, "proposer", [&]() {
-        nn_tray_features(s, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, c->tfeat.p,
-                         c->learn_gate.p);
-        nn_tray_branch(nn, s, c->tfeat.p, 1);
-        nn_head(nn, s, c->ipool.p + (size_t)t * kNnLogits * kNnItemPooled, 1);
-      });
-      nn->gate = nullptr;
+      if (propose[t]) {
+        launch(c, "learn_gate", k_learn_gate, dim3(1), dim3(64), 0, (const int*)c->co.fs.p,
+               pl.m, topk, c->learn_gate.p, (const int*)ctl);
+        nn->gate = c->learn_gate.p;
+        timed_group(c, "proposer", [&]() {
+          nn_tray_features(s, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, c->tfeat.p,
+                           c->learn_gate.p);
+          nn_tray_branch(nn, s, c->tfeat.p, 1);
+          nn_head(nn, s, c->ipool.p + (size_t)t * kNnLogits * kNnItemPooled, 1);
+        });
+        nn->gate = nullptr;
+      }
       launch(c, "learn_pick", k_learn_pick, dim3(1), dim3(64), 0, (const float*)nn->logits.p,
              (const int*)c->map_all.p + pl.od_off, od_fit, pl.m, topk, c->co.fs.p, c->od_prop.p,
              c->learn_pick.p + (size_t)t * topk, c->learn_used.p + t, (const int*)ctl);
@@ -3399,6 +3431,10 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
   }
   if (std::string(name) == "inc_dist") {
     c->inc_dist = value != 0.0;
+    return SPK_OK;
+  }
+  if (std::string(name) == "inc_masks") {
+    c->inc_masks = value != 0.0;
     return SPK_OK;
   }
   if (std::string(name) == "dist_x") {

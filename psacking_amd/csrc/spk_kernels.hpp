@@ -1864,6 +1864,82 @@ k_tile_heights(const M128* __restrict__ tf, int nx, int ny, int* __restrict__ ht
   if (lane == 0) ht[t] = top;
 }
 
+// The tray masks after one placement.  A placement only adds occupied voxels
+// inside the placed item's box, so only the columns under the box change:
+// their full-resolution masks and the 2x2 block masks over them are rebuilt
+// from the tray, and the tile heights, which can only grow, take the new
+// column tops by atomic maximum.  Waves [0, bound^2) take the box's columns,
+// waves [bound^2, bound^2 + nb1^2) its 2x2 blocks (nb1 = bound / 2 + 1);
+// bound >= the box side along x and y (the grid is sized for it).  The
+// placement is the device's search outcome (so != nullptr: orientation
+// od[so->orient] at so->idx) or orientation od[orient] at (qx, qy).
+__global__ void __launch_bounds__(256)
+k_tray_masks_inc(const int* __restrict__ tray, int nx, int ny, int nz, M128* __restrict__ tf,
+                 M128* __restrict__ cols, int cx, int cy, int* __restrict__ ht, int ty,
+                 const SearchOut* __restrict__ so, const OrientDesc* __restrict__ od, int orient,
+                 int qx, int qy, int bound, const int* __restrict__ ctl) {
+  if (refresh_off(ctl)) return;
+  OrientDesc d;
+  if (so) {
+    const SearchOut s = *so;
+    if (!s.found) return;
+    d = od[s.orient];
+    qy = (s.idx / nz) % ny;
+    qx = s.idx / (ny * nz);
+  } else {
+    d = od[orient];
+  }
+  if (d.hx < 0 || d.hy < 0 || d.hz < 0) return;
+  const int ex = d.hx + 1, ey = d.hy + 1;   // columns [qx, qx + ex) x [qy, qy + ey)
+  const int lane = threadIdx.x & 63;
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int ncol = bound * bound, nb1 = bound / 2 + 1;
+  if (w < ncol) {
+    const int u = w / bound, v = w - u * bound;
+    const int x = qx + u, y = qy + v;
+    if (u >= ex || v >= ey || x >= nx || y >= ny) return;
+    const size_t col = (size_t)x * ny + y;
+    M128 m{0ULL, 0ULL};
+    for (int h = 0; h < 2; ++h) {
+      const int z = h * 64 + lane;
+      const bool occ = z < nz && tray[col * nz + z] != 0;
+      const unsigned long long b = __ballot(occ);
+      if (h == 0) m.lo = b; else m.hi = b;
+    }
+    if (lane < kColLevels) {
+      M128 wv = m;
+      for (int k = 0; k < lane; ++k) wv = m_or(wv, m_shr(wv, 1 << k));
+      tf[(size_t)lane * nx * ny + col] = wv;
+    }
+    if (lane == 0) {
+      const int top = m.hi ? 128 - __clzll((long long)m.hi) : (m.lo ? 64 - __clzll((long long)m.lo) : 0);
+      atomicMax(&ht[(x / kSkyTile) * ty + y / kSkyTile], top);
+    }
+    return;
+  }
+  const int b = w - ncol;
+  if (b >= nb1 * nb1) return;
+  const int X = qx / 2 + b / nb1, Y = qy / 2 + (b - (b / nb1) * nb1);
+  if (X > (qx + ex - 1) / 2 || Y > (qy + ey - 1) / 2 || X >= cx || Y >= cy) return;
+  M128 m{0ULL, 0ULL};
+  for (int h = 0; h < 2; ++h) {
+    const int z = h * 64 + lane;
+    bool occ = false;
+    for (int u = 0; u < 2; ++u)
+      for (int v = 0; v < 2; ++v) {
+        const int x = 2 * X + u, y = 2 * Y + v;
+        if (x < nx && y < ny && z < nz && tray[((size_t)x * ny + y) * nz + z] != 0) occ = true;
+      }
+    const unsigned long long bb = __ballot(occ);
+    if (h == 0) m.lo = bb; else m.hi = bb;
+  }
+  if (lane < kColLevels) {
+    M128 wv = m;
+    for (int k = 0; k < lane; ++k) wv = m_or(wv, m_shr(wv, 1 << k));
+    cols[(size_t)lane * cx * cy + (size_t)X * cy + Y] = wv;
+  }
+}
+
 // full-resolution column masks of m oriented items (bit z set when the item
 // has a voxel > 0 at (x, y, z)), im[(o * mx + x) * my + y] for x < ex, y < ey
 // (the only entries the column test reads); one wavefront per column, lanes

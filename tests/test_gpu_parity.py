@@ -519,6 +519,46 @@ def _full_feasible(g, tray):
     return bool((c[:n[0] - h[0], :n[1] - h[1], :n[2] - h[2]] == 0).any())
 
 
+def test_masks_after_placements_match_oracle(P):
+    """The column test's tray masks are updated under each placed box only
+    (option inc_masks, default): after a pack run the column flags of
+    spk_feasible equal exhaustive feasibility on the packed tray, and the run
+    with whole-tray mask rebuilds gives the same outcomes and tray."""
+    from psacking_amd import proposer as PR
+    from psacking_amd import workloads as W
+    from psacking_amd.rotations import orientation_maps
+    perms, flips = orientation_maps(24)
+    w = W.blobs(seed=52, n=12, tray=(44, 38, 30))
+    vols = [int((it > 0).sum()) for it in w.items]
+    order = np.array([int(i) for i in np.argsort(vols)[::-1]], np.int32)
+    out = []
+    for inc in (1, 0):
+        s = P.Session(0)
+        s.set_option("inc_masks", inc)
+        s.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
+        say("pack with inc_masks", inc)
+        res = s.pack(w.items, order, perms, flips, "cube", False, 0)
+        out.append(([(r["found"], tuple(r["position"]), r["orientation"], r["score"]) for r in res],
+                    s.tray(), s))
+    assert out[0][0] == out[1][0]
+    np.testing.assert_array_equal(out[0][1], out[1][1])
+    tray, s = out[0][1], out[0][2]
+    assert sum(1 for r in out[0][0] if r[0]) >= 5
+    rng = np.random.default_rng(53)
+    for case in range(5):
+        dims = tuple(int(v) for v in rng.integers(4, 14, 3))
+        item = np.ones(dims, np.int32)
+        item[rng.random(dims) < 0.15] = 0
+        say("column flags on the packed tray, case", case, dims)
+        column, full, res = PR.feasible(s, item, perms, flips)
+        for o, g in enumerate(O.get_orientations(item, 24)):
+            g = O.as_grid(g)
+            if any(g.shape[a] > tray.shape[a] for a in range(3)):
+                continue
+            exp = _full_feasible(g, tray)
+            assert column[o] == exp and full[o] == exp, (case, o)
+
+
 @pytest.mark.parametrize("fine_group", [64, 4])
 def test_feasibility_flags_match_oracle(P, fine_group):
     """spk_feasible against the oracle on trays filled to several levels: the
