@@ -443,6 +443,96 @@ struct FixedFft<FWD, NT, FPlan<N, Rs...>, ES, BS, JFAST, BATCH, MAXB> {
   }
 };
 
+// ---- in place, decimation in frequency (compile-time plans) ----------------
+// The transform of FixedFft computed in place.  Stage s works on blocks of
+// length L = N / (R_0 ... R_{s-1}); its butterfly (block, n), n < S = L / R_s,
+// takes the R_s elements block*L + n + r*S, applies the R_s-point DFT,
+// multiplies output m by w_L^(n*m) and writes it back to block*L + n + m*S.
+// No two butterflies of a stage share an element, so a stage needs one
+// barrier (after it) where Stockham's needs two.  The output comes out
+// digit-reversed: X[k], k = m_0 + R_0 (m_1 + R_1 (m_2 + ...)), lands at
+// position sum_s m_s * N / (R_0 ... R_s) (PlanPos<P>::at).  Rows (JFAST):
+// transform q starts at rowmap(q) * BS, elements ES apart; columns (!JFAST):
+// BATCH transforms at consecutive offsets, elements ES apart.
+template <int N, int... Rs> struct IpPos;
+template <int N, int R, int... Rest> struct IpPos<N, R, Rest...> {
+  __device__ __forceinline__ static int at(int k) {
+    if constexpr (sizeof...(Rest) == 0) return k;
+    else return (k % R) * (N / R) + IpPos<N / R, Rest...>::at(k / R);
+  }
+};
+template <class P> struct PlanPos;
+template <int N, int... Rs> struct PlanPos<FPlan<N, Rs...>> {
+  __device__ __forceinline__ static int at(int k) { return IpPos<N, Rs...>::at(k); }
+};
+
+struct IdentityMap {
+  __device__ __forceinline__ int operator()(int q) const { return q; }
+};
+
+template <int R, bool FWD, int NT, int N, int L, int ES, int BS, bool JFAST, int BATCH, int MAXB,
+          class RowMap>
+__device__ __forceinline__ void ipstage(float2* buf, int batch_rt, const float2* tw, int tid,
+                                        RowMap rowmap) {
+  constexpr int S = L / R, M = N / R, TWS = N / L;
+  constexpr int MAXT = (JFAST ? MAXB : BATCH) * M;
+  constexpr int MS = CeilDiv<MAXT, NT>::v;
+  const int total = JFAST ? batch_rt * M : MAXT;
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    const int b = tid + s * NT;
+    if (b < total) {
+      int j, base;
+      if (JFAST) {
+        const int q = b / M;
+        j = b - q * M;
+        base = rowmap(q) * BS;
+      } else {
+        j = b / BATCH;
+        base = b - j * BATCH;
+      }
+      const int blk = j / S, n = j - blk * S;
+      const int i0 = base + (blk * L + n) * ES;
+      float2 v[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[r] = buf[i0 + r * S * ES];
+      Bfly<R, FWD>::run(v);
+      if (S > 1 && n != 0) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[n * r * TWS];
+          if (!FWD) w.y = -w.y;
+          v[r] = cmul(v[r], w);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[i0 + r * S * ES] = v[r];
+    }
+  }
+  __syncthreads();
+}
+
+template <bool FWD, int NT, int N, int L, int ES, int BS, bool JFAST, int BATCH, int MAXB,
+          class RowMap, int R, int... Rest>
+__device__ __forceinline__ void ip_stages(float2* buf, int batch, const float2* tw, int tid,
+                                          RowMap rm) {
+  ipstage<R, FWD, NT, N, L, ES, BS, JFAST, BATCH, MAXB, RowMap>(buf, batch, tw, tid, rm);
+  if constexpr (sizeof...(Rest) > 0)
+    ip_stages<FWD, NT, N, L / R, ES, BS, JFAST, BATCH, MAXB, RowMap, Rest...>(buf, batch, tw, tid,
+                                                                             rm);
+}
+
+template <bool FWD, int NT, class P, int ES, int BS, bool JFAST, int BATCH, int MAXB>
+struct IpFft;
+template <bool FWD, int NT, int N, int... Rs, int ES, int BS, bool JFAST, int BATCH, int MAXB>
+struct IpFft<FWD, NT, FPlan<N, Rs...>, ES, BS, JFAST, BATCH, MAXB> {
+  template <class RowMap>
+  __device__ __forceinline__ static void run(float2* buf, int batch, const float2* tw, int tid,
+                                             RowMap rm) {
+    ip_stages<FWD, NT, N, N, ES, BS, JFAST, BATCH, MAXB, RowMap, Rs...>(buf, batch, tw, tid, rm);
+  }
+};
+
 // A transform with the compile-time plan P when P is fixed (the strides and
 // counts then come from the template arguments), else with the runtime plan p.
 template <bool FWD, int NT, int MAXEL, int RMAX, class P, int ES, int BS, bool JFAST, int BATCH,

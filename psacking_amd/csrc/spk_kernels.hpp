@@ -1222,17 +1222,21 @@ __device__ __forceinline__ void wg_argmin_write(double sc, int idx, long long p,
 // complex transform per two real rows.  In place: the thread of (j, k) alone
 // reads X_j[k] and X_{j+h}[k] and writes Z_j[k] and Z_j[pz - k] (a column
 // past the half spectrum).
-template <int NT, int PZ_, int HZ_>
-__device__ __forceinline__ void pair_low_rows(float2* buf, int pz_rt, int nvy, int tid) {
+// (row(j): the LDS row that holds logical row j, when the y pass leaves its
+// rows permuted)
+template <int NT, int PZ_, int HZ_, class RowMap = IdentityMap>
+__device__ __forceinline__ void pair_low_rows(float2* buf, int pz_rt, int nvy, int tid,
+                                              RowMap row = RowMap()) {
   const int pz = PZ_ > 0 ? PZ_ : pz_rt, hz = PZ_ > 0 ? HZ_ : pz_rt / 2 + 1;
   const int h = (nvy + 1) >> 1;
   const float inv_hz = 1.0f / (float)hz;
   for (int e = tid; e < h * hz; e += NT) {
     const int j = PZ_ > 0 ? e / HZ_ : fdiv(e, hz, inv_hz), k = e - j * hz;
-    const float2 a = buf[j * pz + k];
-    const float2 b = j + h < nvy ? buf[(j + h) * pz + k] : make_float2(0.f, 0.f);
-    buf[j * pz + k] = make_float2(a.x - b.y, a.y + b.x);                      // a + i b
-    if (k > 0 && pz - k >= hz) buf[j * pz + pz - k] = make_float2(a.x + b.y, b.x - a.y);
+    const int rj = row(j) * pz;
+    const float2 a = buf[rj + k];
+    const float2 b = j + h < nvy ? buf[row(j + h) * pz + k] : make_float2(0.f, 0.f);
+    buf[rj + k] = make_float2(a.x - b.y, a.y + b.x);                      // a + i b
+    if (k > 0 && pz - k >= hz) buf[rj + pz - k] = make_float2(a.x + b.y, b.x - a.y);
   }
 }
 
@@ -1255,17 +1259,23 @@ __device__ __forceinline__ ColWalk col_walk(int nvz, int tid) {
 
 // rounded low-digit sums of the valid offsets into lo[j * nz + k]; returns the
 // largest rounding residual (rows paired by pair_low_rows)
+// (row, col: the LDS row and column of logical row j and column k, when the
+// transforms leave them permuted)
+template <class RowMap = IdentityMap, class ColMap = IdentityMap>
 __device__ __forceinline__ float low_digit_sums(const float2* buf, int ldb, int* lo, int nz,
-                                                int nvy, int nvz, const ColWalk& w) {
+                                                int nvy, int nvz, const ColWalk& w,
+                                                RowMap row = RowMap(), ColMap col = ColMap()) {
   const int h = (nvy + 1) >> 1;
   float res = 0.f;
-  for (int k = w.c0; k < nvz; k += w.cols)
+  for (int k = w.c0; k < nvz; k += w.cols) {
+    const int ck = col(k);
     for (int j = w.g; j < nvy; j += w.G) {
-      const float v = j < h ? buf[j * ldb + k].x : buf[(j - h) * ldb + k].y;
+      const float v = j < h ? buf[row(j) * ldb + ck].x : buf[row(j - h) * ldb + ck].y;
       const float r = rintf(v);
       res = fmaxf(res, fabsf(v - r));
       lo[j * nz + k] = (int)r;
     }
+  }
   return res;
 }
 
@@ -1276,18 +1286,21 @@ __device__ __forceinline__ float low_digit_sums(const float2* buf, int ldb, int*
 // one value): the column's lowest score belongs to its smallest p, first in
 // row order on ties, and only that offset's score is formed (score_order_ok
 // checks the bound).  Ties across columns resolve by the lowest index.
+template <class RowMap = IdentityMap, class ColMap = IdentityMap>
 __device__ __forceinline__ void slab_scores(const float2* buf, int ldb, const int* lo, int x,
                                             int ny, int nz, int nvy, int nvz, int lb,
                                             long long addback, const double* __restrict__ pen,
                                             unsigned char* __restrict__ fgrid,
                                             long long* __restrict__ pgrid, size_t gbase,
                                             const ColWalk& w, float& res, double& best,
-                                            int& bidx, long long& bp) {
+                                            int& bidx, long long& bp, RowMap row = RowMap(),
+                                            ColMap col = ColMap()) {
   for (int k = w.c0; k < nvz; k += w.cols) {
     long long pm = 0;
     int jm = -1;
+    const int ck = col(k);
     for (int j = w.g; j < nvy; j += w.G) {
-      const float2 v = buf[j * ldb + k];
+      const float2 v = buf[row(j) * ldb + ck];
       const float c = rintf(v.x), h = rintf(v.y);
       res = fmaxf(res, fmaxf(fabsf(v.x - c), fabsf(v.y - h)));
       // b * h with b = 2^lb: a shift of the (two's complement) high digit
@@ -1437,13 +1450,17 @@ k_score_yz_f(const float2* __restrict__ y1, const float2* __restrict__ y2,
     }
   }
   __syncthreads();
-  FixedFft<false, NT, SY, PZ, 1, false, HZ, 0>::run(buf, 0, twy, tid);
-  pair_low_rows<NT, PZ, HZ>(buf, PZ, nvy, tid);
+  // in-place transforms (one barrier per stage): the y pass leaves logical
+  // row j in LDS row yrow(j), the z pass logical column k in LDS column zcol(k)
+  const auto yrow = [](int j) { return PlanPos<SY>::at(j); };
+  const auto zcol = [](int k) { return PlanPos<SZ>::at(k); };
+  IpFft<false, NT, SY, PZ, 1, false, HZ, 0>::run(buf, 0, twy, tid, IdentityMap());
+  pair_low_rows<NT, PZ, HZ>(buf, PZ, nvy, tid, yrow);
   __syncthreads();
-  FixedFft<false, NT, SZ, 1, PZ, true, 0, PY>::run(buf, (nvy + 1) >> 1, twz, tid);
+  IpFft<false, NT, SZ, 1, PZ, true, 0, PY>::run(buf, (nvy + 1) >> 1, twz, tid, yrow);
   // the rounded low-digit sums wait in LDS behind the slab ([ny][nz] ints)
   int* lo = reinterpret_cast<int*>(buf + PY * PZ);
-  float res = low_digit_sums(buf, PZ, lo, nz, nvy, nvz, w);
+  float res = low_digit_sums(buf, PZ, lo, nz, nvy, nvz, w, yrow, zcol);
   __syncthreads();
   // --- collision + high digit (complex field, full spectrum) ---
   {
@@ -1461,13 +1478,13 @@ k_score_yz_f(const float2* __restrict__ y1, const float2* __restrict__ y2,
     }
   }
   __syncthreads();
-  FixedFft<false, NT, SY, PZ, 1, false, PZ, 0>::run(buf, 0, twy, tid);
-  FixedFft<false, NT, SZ, 1, PZ, true, 0, PY>::run(buf, nvy, twz, tid);
+  IpFft<false, NT, SY, PZ, 1, false, PZ, 0>::run(buf, 0, twy, tid, IdentityMap());
+  IpFft<false, NT, SZ, 1, PZ, true, 0, PY>::run(buf, nvy, twz, tid, yrow);
   double best = 1e300;
   int bidx = INT_MAX;
   long long bp = 0;
   slab_scores(buf, PZ, lo, x, ny, nz, nvy, nvz, fp.lb, addback, pen, fgrid, pgrid,
-              (size_t)o * nx * ny * nz, w, res, best, bidx, bp);
+              (size_t)o * nx * ny * nz, w, res, best, bidx, bp, yrow, zcol);
   wg_argmin_write(best, bidx, bp, res, fp.ok && score_order_ok(d, st) ? 0 : 1, out, tid, NT);
 }
 
