@@ -1826,19 +1826,24 @@ void timed_group(spk_ctx* c, const char* name, F&& f) {
 // Pooled item-branch features of every item in each of its orientations, on
 // the device (c->ipool): 24 rows per item in processing order (rows past
 // n_orient repeat the base item; their logits are unused).
+// which (optional): only these items (positions in processing order), whose
+// rows go to consecutive slots of c->ipool in the order listed.
 void learned_item_pool(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
                        const std::vector<int>& off, const int32_t* order,
-                       const spk_orient* orients, int n_orient) {
-  const int n_items = (int)bases.size();
+                       const spk_orient* orients, int n_orient,
+                       const std::vector<int>* which = nullptr) {
+  const int n_items = which ? (int)which->size() : (int)bases.size();
+  if (n_items == 0) return;
   hipStream_t s = c->stream;
   timed_group(c, "proposer_items", [&]() {
     nn_cast(nn, s);
     std::vector<spknn::ItemMap> maps((size_t)n_items * kNnLogits);
-    for (int t = 0; t < n_items; ++t) {
+    for (int i = 0; i < n_items; ++i) {
+      const int t = which ? (*which)[i] : i;
       const int it = order ? order[t] : t;
       const HostItem& h = bases[it];
       for (int o = 0; o < kNnLogits; ++o) {
-        spknn::ItemMap& mp = maps[(size_t)t * kNnLogits + o];
+        spknn::ItemMap& mp = maps[(size_t)i * kNnLogits + o];
         mp.bx = h.dims[0]; mp.by = h.dims[1]; mp.bz = h.dims[2];
         mp.off = off[it];
         const spk_orient* so = (orients && o < n_orient) ? &orients[o] : nullptr;
@@ -2220,7 +2225,6 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   int ws_ex = 1;
   for (const ItemPlan& pl : plans) ws_ex = std::max(ws_ex, pl.max_ex);
   ensure_search_ws(c, topk, ws_ex);
-  learned_item_pool(c, nn, bases, off, order, orients, n_orient);
   c->tfeat.ensure(kNnTrayFeat);
   c->heights.ensure((size_t)c->nx * c->ny);
   c->od_prop.ensure(topk);
@@ -2241,7 +2245,11 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   // An item with at most topk orientations feasible at the start of its round
   // has at most topk at its turn: the proposer would be gated off, so it is
   // not enqueued (k_learn_pick then takes every feasible orientation).
+  // The item branch of the proposer (every orientation of an item) runs per
+  // round for the round's items that may need proposals, into consecutive
+  // slots of c->ipool (slot[t]).
   std::vector<char> skip(n_items, 0), propose(n_items, 1);
+  std::vector<int> slot(n_items, 0);
   while (t0 < n_items) {
     const int t_end = c->prescreen > 0 ? std::min(n_items, t0 + c->prescreen) : n_items;
     if (c->prescreen > 0) {
@@ -2250,6 +2258,15 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
         skip[t] = live[t - t0] == 0 ? 1 : 0;
         propose[t] = live[t - t0] < 0 || live[t - t0] > topk ? 1 : 0;
       }
+    }
+    {
+      std::vector<int> need;
+      for (int t = t0; t < t_end; ++t)
+        if (plans[t].m > 0 && !skip[t] && propose[t]) {
+          slot[t] = (int)need.size();
+          need.push_back(t);
+        }
+      learned_item_pool(c, nn, bases, off, order, orients, n_orient, &need);
     }
     const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
     upload(c, c->ctl.p, ctl0, sizeof(ctl0));
@@ -2270,7 +2287,7 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
           nn_tray_features(s, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, c->tfeat.p,
                            c->learn_gate.p);
           nn_tray_branch(nn, s, c->tfeat.p, 1);
-          nn_head(nn, s, c->ipool.p + (size_t)t * kNnLogits * kNnItemPooled, 1);
+          nn_head(nn, s, c->ipool.p + (size_t)slot[t] * kNnLogits * kNnItemPooled, 1);
         });
         nn->gate = nullptr;
       }
