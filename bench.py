@@ -47,7 +47,9 @@ KERNELS = ["dist_zy", "dist_x", "inc_zy", "inc_x", "inc_stats", "learn_gate", "l
            "item_rows_fwd", "pencil_y_fwd", "corr_x", "score_yz", "pencil_y_inv", "rows_score",
            "select", "verify", "decide", "place", "direct_search", "gather_od", "proposer",
            "proposer_items", "record", "feas_begin", "feas_x", "feas_yz", "feas_decide",
-           "tray_columns", "tray_fine", "tile_heights", "core_columns", "item_columns", "run_lists", "item_masks", "column_test", "coarse_decide", "post"]
+           "tray_columns", "tray_fine", "tile_heights", "tray_masks_inc", "core_columns", "item_columns",
+           "run_lists", "item_masks", "column_test", "pre_masks", "pre_column_test", "coarse_decide",
+           "post"]
 
 
 def device_sync():

@@ -201,6 +201,7 @@ struct spk_ctx {
   int prescreen = 16;                    // option "prescreen": items per round of the device
                                          // loops, screened together first (0: one round, none)
   DevBuf<int> pre_fs;                    // flags of a batched screen
+  DevBuf<int> pre_aux;                   // its per-item counts and orientation items
   int many_workers = 8;                  // option "many_workers": spk_pack_many host workers
   bool graph = false;                    // option "graph": device loops captured as a hipGraph
   bool fixed_fft = true;                 // option "fixed_fft": compile-time-plan score kernels
@@ -441,7 +442,7 @@ void plan_coarse(spk_ctx* c) {
   q.tf.ensure((size_t)kColLevels * c->nx * c->ny);
   q.tx = (c->nx + kSkyTile - 1) / kSkyTile;
   q.ty = (c->ny + kSkyTile - 1) / kSkyTile;
-  q.ht.ensure((size_t)q.tx * q.ty);
+  q.ht.ensure((size_t)q.tx * q.ty + 1);   // tile heights, then the tray top
   q.fs.ensure(kFeasHeader + 24);
   q.ready = true;
 }
@@ -452,7 +453,7 @@ void build_coarse_spectra(spk_ctx* c, const int* ctl) {
   if (!q.ready) return;
   const int n = q.cx * q.cy;
   launch(c, "tray_columns", k_tray_columns, dim3((n + 3) / 4), dim3(256), 0, (const int*)c->tray.p,
-         c->nx, c->ny, c->nz, q.cols.p, q.cx, q.cy, ctl);
+         c->nx, c->ny, c->nz, q.cols.p, q.cx, q.cy, q.ht.p + (size_t)q.tx * q.ty, ctl);
   launch(c, "tray_fine", k_tray_fine, dim3((c->nx * c->ny + 3) / 4), dim3(256), 0,
          (const int*)c->tray.p, c->nx, c->ny, c->nz, q.tf.p, ctl);
   launch(c, "tile_heights", k_tile_heights, dim3((q.tx * q.ty + 3) / 4), dim3(256), 0,
@@ -680,8 +681,8 @@ void update_coarse(spk_ctx* c, const int* ctl, const PlaceRef& pr) {
   const int nb1 = bound / 2 + 1;
   const int waves = bound * bound + nb1 * nb1;
   launch(c, "tray_masks_inc", k_tray_masks_inc, dim3((waves + 3) / 4), dim3(256), 0,
-         (const int*)c->tray.p, c->nx, c->ny, c->nz, q.tf.p, q.cols.p, q.cx, q.cy, q.ht.p, q.ty,
-         pr.so, pr.od, pr.orient, pr.qx, pr.qy, bound, ctl);
+         (const int*)c->tray.p, c->nx, c->ny, c->nz, q.tf.p, q.cols.p, q.cx, q.cy, q.ht.p, q.tx,
+         q.ty, pr.so, pr.od, pr.orient, pr.qx, pr.qy, bound, ctl);
 }
 
 // k_inc_zy's LDS: the item box's column flags (rows x z extent) and the
@@ -1063,10 +1064,15 @@ const int* ctl_feas_on(spk_ctx* c) {
 // the flags are only left for the host (no k_coarse_decide).
 void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_exy,
                     const int* ctl, PackOut* out = nullptr, int t = 0, int halt_not_found = 0,
-                    bool clear_flags = false, bool decide = true, int* flags = nullptr) {
+                    bool clear_flags = false, bool decide = true, int* flags = nullptr,
+                    const int* pre = nullptr, int* cnt = nullptr, const int* od_item = nullptr,
+                    int cap = 0) {
   spk_ctx::Coarse& q = c->co;
   if (!q.ready) return;
   int* fs = flags ? flags : q.fs.p;
+  // a batched screen (flags given) is timed under names of its own
+  const char* n_masks = flags ? "pre_masks" : "item_masks";
+  const char* n_test = flags ? "pre_column_test" : "column_test";
   const int ccx = (max_ex + 1) / 2, ccy = q.cy;
   const int mx = max_ex, my = c->ny;
   q.cm.ensure((size_t)m * ccx * ccy);
@@ -1075,9 +1081,9 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
   q.frl.ensure((size_t)m * kFineRuns);
   q.nrun.ensure(2 * (size_t)m);
   if (max_exy <= kMaskCols) {
-    launch(c, "item_masks", k_item_masks, dim3(m), dim3(kMaskThreads), 0, (const int*)c->items.p,
+    launch(c, n_masks, k_item_masks, dim3(m), dim3(kMaskThreads), 0, (const int*)c->items.p,
            (const M128*)q.lines.p, od, m, q.cm.p, ccx, ccy, q.im.p, mx, my, q.rl.p, q.frl.p,
-           q.nrun.p, clear_flags ? fs : (int*)nullptr, ctl);
+           q.nrun.p, clear_flags ? fs : (int*)nullptr, ctl, pre);
   } else {
     launch(c, "core_columns", k_core_columns, dim3((ccx * ccy + 3) / 4, m), dim3(256), 0,
            (const int*)c->items.p, od, m, q.cm.p, ccx, ccy, ctl);
@@ -1088,12 +1094,12 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
   }
   auto test = [&](auto kernel) {
     const int chunks = (q.cx * q.cy + 63) / 64;
-    launch(c, "column_test", kernel, dim3((chunks + q.col_chunks - 1) / q.col_chunks, m),
+    launch(c, n_test, kernel, dim3((chunks + q.col_chunks - 1) / q.col_chunks, m),
            dim3(256), 0,
            (const M128*)q.cols.p, q.cx, q.cy, (const M128*)q.cm.p, ccx, ccy, (const M128*)q.tf.p,
            c->nx, c->ny, (const M128*)q.im.p, mx, my, od, m, (const unsigned*)q.rl.p,
            (const unsigned*)q.frl.p, (const int*)q.nrun.p, (const int*)q.ht.p, q.ty, fs, ctl,
-           q.tx);
+           q.tx, pre, cnt, od_item, cap);
   };
   switch (q.fine_group) {
     case 4: test(k_column_test<4>); break;
@@ -1495,7 +1501,7 @@ void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e
                   int penalty, int t, int32_t id_offset, int halt_not_found, double* rec_row,
                   const int* rec_map, int feas_mode = -1,
                   const FieldStats* item_bound = nullptr, bool known_feasible = false,
-                  int max_dim = 0) {
+                  int max_dim = 0, const int* pre = nullptr) {
   if (feas_mode < 0) feas_mode = c->feas_mode;
   const bool feas = feas_mode && !c->split;
   if (feas || item_bound || c->co.ready) {
@@ -1504,7 +1510,8 @@ void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e
            (const FieldStats*)c->stats.p, item_bound, c->pout.p, t);
     // orientations a column test already found feasible need no second one
     if (!known_feasible)
-      enqueue_coarse(c, od, m, max_ex, max_exy, c->ctl.p, c->pout.p, t, halt_not_found);
+      enqueue_coarse(c, od, m, max_ex, max_exy, c->ctl.p, c->pout.p, t, halt_not_found, false,
+                     true, nullptr, pre);
     enqueue_item_spectra(c, od, m, max_ex, c->ctl.p);
     // the column test's flags are exact: no FFT feasibility pass after it
     if (feas && !c->co.ready) {
@@ -1643,9 +1650,15 @@ int place_exact(spk_ctx* c, const ExactBest& eb, const ItemPlan& pl,
 // fitting orientations feasible now, an upper bound for the rest of the run:
 // 0 = the item fits nowhere; -1 = not tested (no fitting orientation, or the
 // column test does not apply: split plan, tray z > 128, option "coarse" off).
+// The flags stay in c->pre_fs (orientation i of item t at
+// kFeasHeader + plans[t].od_off - *od0_out) until the next call, for the
+// per-item column tests of the round (pre_flags): 1 feasible, 0 infeasible,
+// 2 untested.  An item stops being tested once cap of its orientations are
+// proven feasible (the count returned is then at least cap).
 std::vector<int> prescreen(spk_ctx* c, const std::vector<ItemPlan>& plans, int t_begin,
-                           int t_end) {
+                           int t_end, int* od0_out, int cap) {
   std::vector<int> live((size_t)std::max(0, t_end - t_begin), -1);
+  *od0_out = 0;
   spk_ctx::Coarse& q = c->co;
   if (!q.ready || !q.on || t_end <= t_begin) return live;
   int od0 = -1, od1 = 0, max_ex = 1, max_exy = 1;
@@ -1658,11 +1671,19 @@ std::vector<int> prescreen(spk_ctx* c, const std::vector<ItemPlan>& plans, int t
     max_exy = std::max(max_exy, pl.max_exy);
   }
   if (od0 < 0) return live;
+  *od0_out = od0;
   const int m = od1 - od0;   // the fitting orientations of a range are contiguous
+  const int ni = t_end - t_begin;
+  // per-item feasible counts (zero), then each orientation's item
+  std::vector<int> aux((size_t)ni + m, 0);
+  for (int t = t_begin; t < t_end; ++t)
+    for (int i = 0; i < plans[t].m; ++i) aux[(size_t)ni + plans[t].od_off - od0 + i] = t - t_begin;
+  c->pre_aux.ensure(aux.size());
+  upload(c, c->pre_aux.p, aux.data(), aux.size() * sizeof(int));
   c->pre_fs.ensure((size_t)kFeasHeader + m);
   HIP_OK(hipMemsetAsync(c->pre_fs.p, 0, ((size_t)kFeasHeader + m) * sizeof(int), c->stream));
   enqueue_coarse(c, c->od_all.p + od0, m, max_ex, max_exy, nullptr, nullptr, 0, 0, false, false,
-                 c->pre_fs.p);
+                 c->pre_fs.p, nullptr, c->pre_aux.p, c->pre_aux.p + ni, std::max(1, cap));
   std::vector<int> f((size_t)m);
   download(c, f.data(), c->pre_fs.p + kFeasHeader, (size_t)m * sizeof(int));
   sync(c);
@@ -1674,6 +1695,13 @@ std::vector<int> prescreen(spk_ctx* c, const std::vector<ItemPlan>& plans, int t
     live[t - t_begin] = n;
   }
   return live;
+}
+
+// item t's flags from its round's batched screen, or nullptr when it was not
+// screened (the column test then tests every fitting orientation)
+const int* pre_flags(const spk_ctx* c, const std::vector<ItemPlan>& plans, int t, int live,
+                     int od0) {
+  return live > 0 ? c->pre_fs.p + kFeasHeader + (plans[t].od_off - od0) : nullptr;
 }
 
 
@@ -1720,11 +1748,16 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
   // the start of their round are decided there (skip), the others enqueued
   const bool rounds = c->prescreen > 0 && !rec_feat && !rec_scores;
   std::vector<char> skip(n_items, 0);
+  std::vector<const int*> pre(n_items, nullptr);
   while (t0 < n_items) {
     const int t_end = rounds ? std::min(n_items, t0 + c->prescreen) : n_items;
     if (rounds) {
-      const std::vector<int> live = prescreen(c, plans, t0, t_end);
-      for (int t = t0; t < t_end; ++t) skip[t] = live[t - t0] == 0 ? 1 : 0;
+      int od0 = 0;
+      const std::vector<int> live = prescreen(c, plans, t0, t_end, &od0, 1);
+      for (int t = t0; t < t_end; ++t) {
+        skip[t] = live[t - t0] == 0 ? 1 : 0;
+        pre[t] = pre_flags(c, plans, t, live[t - t0], od0);
+      }
     }
     const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
     upload(c, c->ctl.p, ctl0, sizeof(ctl0));
@@ -1739,7 +1772,7 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
       enqueue_step(c, od, pl.m, pl.max_ex, pl.max_exy, pl.max_e, penalty, t, id_offset, 0,
                    rec_scores ? c->rscores.p + (size_t)t * n_orient : nullptr,
                    c->map_all.p + pl.od_off, -1, bounds ? bounds + t : nullptr, false,
-                   pl.max_dim);
+                   pl.max_dim, pre[t]);
     }
     lg.run();
     int ctl[CTL_WORDS];
@@ -2250,13 +2283,16 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   // slots of c->ipool (slot[t]).
   std::vector<char> skip(n_items, 0), propose(n_items, 1);
   std::vector<int> slot(n_items, 0);
+  std::vector<const int*> pre(n_items, nullptr);
   while (t0 < n_items) {
     const int t_end = c->prescreen > 0 ? std::min(n_items, t0 + c->prescreen) : n_items;
     if (c->prescreen > 0) {
-      const std::vector<int> live = prescreen(c, plans, t0, t_end);
+      int od0 = 0;
+      const std::vector<int> live = prescreen(c, plans, t0, t_end, &od0, topk + 1);
       for (int t = t0; t < t_end; ++t) {
         skip[t] = live[t - t0] == 0 ? 1 : 0;
         propose[t] = live[t - t0] < 0 || live[t - t0] > topk ? 1 : 0;
+        pre[t] = pre_flags(c, plans, t, live[t - t0], od0);
       }
     }
     {
@@ -2278,7 +2314,8 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       const OrientDesc* od_fit = c->od_all.p + pl.od_off;
       launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, c->co.fs.p, pl.m, 0,
              ctl, (const FieldStats*)nullptr, (const FieldStats*)nullptr, c->pout.p, t);
-      enqueue_coarse(c, od_fit, pl.m, pl.max_ex, pl.max_exy, ctl, c->pout.p, t, 0);
+      enqueue_coarse(c, od_fit, pl.m, pl.max_ex, pl.max_exy, ctl, c->pout.p, t, 0, false, true,
+                     nullptr, pre[t]);
       if (propose[t]) {
         launch(c, "learn_gate", k_learn_gate, dim3(1), dim3(64), 0, (const int*)c->co.fs.p,
                pl.m, topk, c->learn_gate.p, (const int*)ctl);

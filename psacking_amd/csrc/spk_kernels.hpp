@@ -1810,8 +1810,9 @@ __device__ __forceinline__ int m_ctz(M128 a) {
 constexpr int kColLevels = 8;
 __global__ void __launch_bounds__(256)
 k_tray_columns(const int* __restrict__ tray, int nx, int ny, int nz, M128* __restrict__ cols,
-               int cx, int cy, const int* __restrict__ ctl) {
+               int cx, int cy, int* __restrict__ top, const int* __restrict__ ctl) {
   if (refresh_off(ctl)) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *top = 0;   // k_tile_heights raises it
   const int lane = threadIdx.x & 63;
   const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (col >= cx * cy) return;
@@ -1861,7 +1862,8 @@ k_tray_fine(const int* __restrict__ tray, int nx, int ny, int nz, M128* __restri
 
 // tile heights of the tray, after k_tray_fine: ht[X * ty + Y] = max over the
 // columns (x, y) of tile X = x / kSkyTile, Y = y / kSkyTile of 1 + the highest
-// occupied z (0 for empty columns); one wavefront per tile, a lane per column
+// occupied z (0 for empty columns); one wavefront per tile, a lane per column.
+// ht[tx * ty] is the tray top, the largest of them (reset by k_tray_columns).
 constexpr int kSkyTile = 8;
 __global__ void __launch_bounds__(256)
 k_tile_heights(const M128* __restrict__ tf, int nx, int ny, int* __restrict__ ht, int tx, int ty,
@@ -1878,7 +1880,10 @@ k_tile_heights(const M128* __restrict__ tf, int nx, int ny, int* __restrict__ ht
     top = m.hi ? 128 - __clzll((long long)m.hi) : (m.lo ? 64 - __clzll((long long)m.lo) : 0);
   }
   for (int s = 1; s < 64; s <<= 1) top = max(top, __shfl_xor(top, s, 64));
-  if (lane == 0) ht[t] = top;
+  if (lane == 0) {
+    ht[t] = top;
+    atomicMax(&ht[tx * ty], top);
+  }
 }
 
 // The tray masks after one placement.  A placement only adds occupied voxels
@@ -1892,7 +1897,7 @@ k_tile_heights(const M128* __restrict__ tf, int nx, int ny, int* __restrict__ ht
 // od[so->orient] at so->idx) or orientation od[orient] at (qx, qy).
 __global__ void __launch_bounds__(256)
 k_tray_masks_inc(const int* __restrict__ tray, int nx, int ny, int nz, M128* __restrict__ tf,
-                 M128* __restrict__ cols, int cx, int cy, int* __restrict__ ht, int ty,
+                 M128* __restrict__ cols, int cx, int cy, int* __restrict__ ht, int tx, int ty,
                  const SearchOut* __restrict__ so, const OrientDesc* __restrict__ od, int orient,
                  int qx, int qy, int bound, const int* __restrict__ ctl) {
   if (refresh_off(ctl)) return;
@@ -1931,6 +1936,7 @@ k_tray_masks_inc(const int* __restrict__ tray, int nx, int ny, int nz, M128* __r
     if (lane == 0) {
       const int top = m.hi ? 128 - __clzll((long long)m.hi) : (m.lo ? 64 - __clzll((long long)m.lo) : 0);
       atomicMax(&ht[(x / kSkyTile) * ty + y / kSkyTile], top);
+      atomicMax(&ht[tx * ty], top);
     }
     return;
   }
@@ -2029,6 +2035,7 @@ k_core_columns(const int* __restrict__ items, const OrientDesc* __restrict__ od,
 // left unflagged has no collision-free offset.  A block starts a chunk only
 // while the orientation is unflagged.
 constexpr int kColRuns = 2048;
+constexpr int kRunsLds = 256;   // run-list entries the column test keeps in LDS
 constexpr int kFineRuns = 4096;
 
 // every run [a, a+L) of set bits of a column mask, passed to f(a, L)
@@ -2223,7 +2230,7 @@ k_item_masks(const int* __restrict__ items, const M128* __restrict__ lines,
              const OrientDesc* __restrict__ od, int m,
              M128* __restrict__ cm, int ccx, int ccy, M128* __restrict__ im, int mx, int my,
              unsigned* __restrict__ runs, unsigned* __restrict__ fruns, int* __restrict__ nrun,
-             int* __restrict__ cfs, const int* __restrict__ ctl) {
+             int* __restrict__ cfs, const int* __restrict__ ctl, const int* __restrict__ pre) {
   if (search_off(ctl)) return;
   const int o = blockIdx.x, tid = threadIdx.x;
   if (o >= m) return;
@@ -2232,6 +2239,7 @@ k_item_masks(const int* __restrict__ items, const M128* __restrict__ lines,
     if (o == 0)
       for (int h = 0; h < kFeasHeader; ++h) cfs[h] = 0;
   }
+  if (pre && pre[o] == 0) return;   // infeasible on an earlier tray: no test (k_column_test)
   const OrientDesc d = od[o];
   __shared__ M128 own[kMaskCols];
   __shared__ M128 core[kMaskCols / 2];   // ceil(ex/2) * ceil(ey/2) <= 2048 when ex * ey <= 4096
@@ -2352,30 +2360,40 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
               const M128* __restrict__ im, int mx, int my, const OrientDesc* __restrict__ od,
               int m, const unsigned* __restrict__ run_lists, const unsigned* __restrict__ fine_lists,
               const int* __restrict__ nrun, const int* __restrict__ ht, int ty,
-              int* __restrict__ cfs, const int* __restrict__ ctl, int tx) {
+              int* __restrict__ cfs, const int* __restrict__ ctl, int tx,
+              const int* __restrict__ pre, int* __restrict__ cnt,
+              const int* __restrict__ od_item, int cap) {
   if (search_off(ctl)) return;
   const int o = blockIdx.y, tid = threadIdx.x;
   if (o >= m) return;
+  // pre (nullable): flags of the same orientations on an earlier tray of the
+  // run; one infeasible there is infeasible here (placements only add
+  // occupied voxels) and keeps its flag 0 with no work
+  if (pre && pre[o] == 0) return;
   const OrientDesc d = od[o];
   const int nqx = d.nvx > 0 ? (d.nvx - 1) / 2 + 1 : 0, nqy = d.nvy > 0 ? (d.nvy - 1) / 2 + 1 : 0;
   const int nq = nqx * nqy;
   const int nblk = (nq + 63) / 64;
   if ((int)blockIdx.x >= nblk || d.nvz <= 0) return;
   volatile int* flag = cfs + kFeasHeader + o;
-  // whole-tray proof first: when no tray column is higher than H, the item's
-  // box at height H is empty at offset (0, 0), a valid offset when H < nvz
-  {
-    __shared__ int hw[4];
-    int h = 0;
-    for (int i = tid; i < tx * ty; i += 256) h = max(h, ht[i]);
-    for (int s2 = 32; s2 > 0; s2 >>= 1) h = max(h, __shfl_xor(h, s2, 64));
-    if ((tid & 63) == 0) hw[tid >> 6] = h;
-    __syncthreads();
-    h = max(max(hw[0], hw[1]), max(hw[2], hw[3]));
-    if (h < d.nvz) {
-      if (tid == 0) *flag = 1;
-      return;
-    }
+  // A batched screen of several items (cnt != nullptr, cnt[od_item[o]]
+  // counts the orientations of o's item proven feasible) needs no more tests
+  // of an item once cap of its orientations are: an orientation left
+  // untested then reads 2 (unknown), never 0 (infeasible).
+  volatile int* icnt = cnt ? cnt + od_item[o] : nullptr;
+  auto mark_feasible = [&]() {
+    if (atomicExch((int*)flag, 1) != 1 && icnt) atomicAdd((int*)icnt, 1);
+  };
+  if (icnt && *icnt >= cap) {
+    if (tid == 0) atomicCAS((int*)flag, 0, 2);
+    return;
+  }
+  // whole-tray proof first: when no tray column is higher than H (the tray
+  // top, ht[tx * ty]), the item's box at height H is empty at offset (0, 0),
+  // a valid offset when H < nvz
+  if (ht[tx * ty] < d.nvz) {
+    if (tid == 0) mark_feasible();
+    return;
   }
   __shared__ unsigned cand_list[256];
   __shared__ M128 cbad[64];
@@ -2389,6 +2407,14 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
   const unsigned* runs = run_lists + (size_t)o * kColRuns;
   const unsigned* fruns = fine_lists + (size_t)o * kFineRuns;
   const int nr = nrun[2 * o], nf = nrun[2 * o + 1];
+  // the first kRunsLds entries of each list in LDS: every offset of the block
+  // reads the same lists
+  __shared__ unsigned sruns[kRunsLds], sfruns[kRunsLds];
+  for (int i = tid; i < kRunsLds; i += 256) {
+    if (i < nr && nr <= kColRuns) sruns[i] = runs[i];
+    if (i < nf && nf <= kFineRuns) sfruns[i] = fruns[i];
+  }
+  __syncthreads();
   const int nv = d.nvz;
   const unsigned long long vlo = nv >= 64 ? ~0ULL : ((1ULL << nv) - 1ULL);
   const unsigned long long vhi = nv >= 128 ? ~0ULL : (nv > 64 ? ((1ULL << (nv - 64)) - 1ULL) : 0ULL);
@@ -2401,7 +2427,13 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
     __syncthreads();   // the previous chunk's candidate list is consumed
     if (tid == 0) {
       ncand = 0;
-      skip = *flag;   // one read, so that the whole block takes the same branch
+      // one read, so that the whole block takes the same branch
+      int sk = *flag == 1 ? 1 : 0;
+      if (!sk && icnt && *icnt >= cap) {
+        atomicCAS((int*)flag, 0, 2);
+        sk = 1;
+      }
+      skip = sk;
     }
     __syncthreads();
     if (skip) return;
@@ -2417,7 +2449,7 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
         if (!done) {
           const int r1 = min(nr, r0 + step);
           for (int r = r0 + j; r < r1; r += 4) {
-            const unsigned e = runs[r];
+            const unsigned e = r < kRunsLds ? sruns[r] : runs[r];
             const int X = e & 127, Y = (e >> 7) & 255, ra = (e >> 15) & 127;
             const int L = ((e >> 22) & 127) + 1;
             bad = m_or(bad, run_blocks_at(cols, plane, (size_t)(Qx + X) * cy + (Qy + Y), ra, L));
@@ -2443,7 +2475,7 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
     }
     const bool cand = active && m_any_free(bad, vlo, vhi);
     if (!fine) {
-      if (cand && j == 0) *flag = 1;
+      if (cand && j == 0) mark_feasible();
       continue;
     }
     // full-resolution stage: the block lists the full-resolution offsets
@@ -2479,7 +2511,7 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
           h = max(h, ht[(tx0 + a) * ty + ty0 + (i - a * nyt)]);
         }
         if (group_max<G>(h) < d.nvz) {
-          if (gl == 0) *flag = 1;
+          if (gl == 0) mark_feasible();
           break;
         }
       }
@@ -2490,7 +2522,7 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
         for (int r0 = 0; r0 < nf && !blocked; r0 += step, step = min(step * 2, 64 * G)) {
           const int r1 = min(nf, r0 + step);
           for (int r = r0 + gl; r < r1; r += G) {
-            const unsigned u = fruns[r];
+            const unsigned u = r < kRunsLds ? sfruns[r] : fruns[r];
             const int x = u & 127, y = (u >> 7) & 255, ra = (u >> 15) & 127, L = ((u >> 22) & 127) + 1;
             fb = m_or(fb, run_blocks_at(tf, fplane, (size_t)(qx + x) * ny + (qy + y), ra, L));
           }
@@ -2509,7 +2541,7 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
         blocked = !m_any_free(fb, vlo, vhi);
       }
       if (!blocked) {
-        if (gl == 0) *flag = 1;
+        if (gl == 0) mark_feasible();
         break;
       }
     }
