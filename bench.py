@@ -359,6 +359,8 @@ def main(argv=None) -> int:
                       "setup": t_setup, "searched": units["score_yz"],
                       "feas_pairs": units["feas_yz"],
                       "placed": sum(1 for r in res if r["found"]),
+                      "plan": {"padded": list(s.plan()), "tile": s.tile()[0],
+                               "split": bool(s.tile()[1])},
                       "fallbacks": sum(1 for r in res if r["path"] != 0),
                       "max_resid": max(float(r["residual"]) for r in res)}
 
@@ -535,6 +537,7 @@ def main(argv=None) -> int:
             "exact_fallbacks": r["fallbacks"],
             "max_fft_residual": r["max_resid"],
             "setup_s": r["setup"],
+            "fft_plan": r["plan"],
             "loop_roofline": {"algorithmic_bytes": b_loop, "orientations_searched": r["searched"],
                               "feasibility_pairs": r["feas_pairs"],
                               "achieved_GBps": b_loop / r["dt"] / 1e9, "frac": loop_frac},

@@ -270,6 +270,11 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  25; "yz_radix" the same for the other y/z passes (2..25),
  *                  "pad_y"/"pad_z" padded FFT sizes (0: smallest 2^a3^b5^c7^d).
  *   "corr_chunk"   orientations per pass-A block (0: automatic).
+ *   "reg_x"        pass A when the padded x length is 240 (= 16 * 15): the
+ *                  x transforms in registers with one LDS exchange per
+ *                  transform, 2 (default) two LDS buffers per block, 1 three
+ *                  (two fewer barriers, a third more LDS); 0: the LDS FFT
+ *                  kernel used for every other length.
  *   "feas_group"   orientation pairs per feasibility launch pair (0: all).
  *   "graph"        1: the device-driven loops (spk_pack, spk_pack_gather,
  *                  spk_pack_learned) capture each enqueue of their items as

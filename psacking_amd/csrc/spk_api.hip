@@ -155,6 +155,7 @@ struct spk_ctx {
   int metric = SPK_METRIC_L1, sources = SPK_SOURCES_EQ1;
   AxisPlan px{}, py{}, pz{};
   int T = 16;                  // x-pencil tile
+  int Tc = 16;                 // x-pencil tile of pass A (tuned on its own)
   int Ty = 16;                 // y-pencil tile (split plan)
   int Rf = 1, Rs = 1;          // rows per block: forward rows, score rows (split plan)
   bool split = false;          // split plan in use
@@ -205,6 +206,8 @@ struct spk_ctx {
   int many_workers = 8;                  // option "many_workers": spk_pack_many host workers
   bool graph = false;                    // option "graph": device loops captured as a hipGraph
   bool fixed_fft = true;                 // option "fixed_fft": compile-time-plan score kernels
+  int reg_x = 2;                         // option "reg_x": register pass A for px = 240
+                                         // (0 off, 1 three LDS buffers, 2 two)
   // feasibility pass
   DevBuf<int> fstate;                    // residual bits, unused, per-orientation flags
   DevBuf<int> ctl_on;                    // control words that enable the pass outside the loop
@@ -521,6 +524,7 @@ void plan_tray(spk_ctx* c) {
     return T;
   };
   c->T = tile(px);
+  c->Tc = c->reg_x > 0 && px == 240 ? 4 : c->T;
   c->Ty = tile(py);
   c->Rf = std::max(1, kRowMax / pz);        // forward rows per block
   c->Rs = std::max(1, kRowMax / (2 * pz));  // score rows per block (two row halves)
@@ -576,6 +580,12 @@ void plan_tray(spk_ctx* c) {
     allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 16>, dyn);
     allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 8>, dyn);
     allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 4>, dyn);
+    allow_shmem(k_corr_x240<4, 3>, corr_x240_shmem(4, 3));
+    allow_shmem(k_corr_x240<8, 3>, corr_x240_shmem(8, 3));
+    allow_shmem(k_corr_x240<16, 3>, corr_x240_shmem(16, 3));
+    allow_shmem(k_corr_x240<4, 2>, corr_x240_shmem(4, 2));
+    allow_shmem(k_corr_x240<8, 2>, corr_x240_shmem(8, 2));
+    allow_shmem(k_corr_x240<16, 2>, corr_x240_shmem(16, 2));
     allow_shmem(k_tray_x_fwd<DynPlan, 0>, dyn);
     allow_shmem(k_tray_x_fwd<X, 16>, dyn);
     allow_shmem(k_tray_x_fwd<X, 8>, dyn);
@@ -1007,9 +1017,9 @@ void enqueue_item_spectra(spk_ctx* c, const OrientDesc* od, int m, int max_ex, c
 }
 
 // x-pass chunking: enough blocks to fill the chip, few orientations per block
-int corr_chunks(const spk_ctx* c, int m, int* chunk) {
+int corr_chunks(const spk_ctx* c, int m, int T, int* chunk) {
   const int py = c->py.n, pz = c->pz.n;
-  const int tiles = (pz + c->T - 1) / c->T;
+  const int tiles = (pz + T - 1) / T;
   int nchunk = std::max(1, std::min(m, (2048 + tiles * py - 1) / (tiles * py)));
   if (c->corr_chunk > 0) nchunk = (m + c->corr_chunk - 1) / c->corr_chunk;   // forced
   *chunk = (m + nchunk - 1) / nchunk;
@@ -1033,7 +1043,7 @@ void enqueue_feas(spk_ctx* c, const OrientDesc* od, int m, int max_ex, const int
     const int gp = std::min(group, np - g0);
     const int ob = 2 * g0, gm = std::min(m - ob, 2 * gp);
     int chunk = 0;
-    const int nchunk = corr_chunks(c, gp, &chunk);
+    const int nchunk = corr_chunks(c, gp, c->T, &chunk);
     const int* cf = c->co.ready ? c->co.fs.p + kFeasHeader + ob : nullptr;
     launch(c, "feas_x", k_feas_x, dim3(tiles, py, nchunk), dim3(kPencilThreads), pencil_shmem(c),
            (const float2*)c->g1.p, (const float2*)c->ispec.p + (size_t)ob * max_ex * py * pz,
@@ -1140,20 +1150,34 @@ using CorrKernel = decltype(&k_corr_x);
 
 template <class PX_, int T_>
 bool fixed_corr(const spk_ctx* c, CorrKernel* k) {
-  if (c->T != T_ || !plan_is(c->px, PX_{})) return false;
+  if (c->Tc != T_ || !plan_is(c->px, PX_{})) return false;
   *k = k_corr_x_f<PX_, T_>;
   return true;
 }
 
-// pass A for the context's x plan and tile: a compile-time instantiation when
-// one matches, else the runtime one
-CorrKernel corr_kernel(const spk_ctx* c) {
+// pass A for the context's x plan and tile: the register kernel for
+// px = 240, else a compile-time instantiation when one matches, else the
+// runtime one; with its block size and LDS bytes
+struct CorrLaunch {
+  CorrKernel k;
+  int threads;
+  size_t shmem;
+};
+CorrLaunch corr_launch(const spk_ctx* c) {
+  const int T = c->Tc;
+  if (c->fixed_fft && c->reg_x > 0 && c->px.n == 240 && (T == 4 || T == 8 || T == 16)) {
+    const int nb = c->reg_x == 2 ? 2 : 3;
+    CorrKernel k;
+    if (nb == 3) k = T == 4 ? k_corr_x240<4, 3> : T == 8 ? k_corr_x240<8, 3> : k_corr_x240<16, 3>;
+    else k = T == 4 ? k_corr_x240<4, 2> : T == 8 ? k_corr_x240<8, 2> : k_corr_x240<16, 2>;
+    return {k, 16 * T, corr_x240_shmem(T, nb)};
+  }
   CorrKernel k = k_corr_x;
   if (c->fixed_fft) {
     fixed_corr<FPlan<240, 16, 15>, 16>(c, &k) || fixed_corr<FPlan<240, 16, 15>, 8>(c, &k) ||
         fixed_corr<FPlan<240, 16, 15>, 4>(c, &k);
   }
-  return k;
+  return {k, kPencilThreads, (size_t)(c->px.n + 2 * c->px.n * T) * sizeof(float2)};
 }
 
 // Scored search (pass A, pass B, selection, exact re-check of the winner) of
@@ -1164,9 +1188,9 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
   const int py = c->py.n, pz = c->pz.n, hz = pz / 2 + 1;
   const float2* tw = c->tw.p;
   const size_t ysh = (size_t)(py + py * c->Ty) * sizeof(float2);
-  const int tiles = (pz + c->T - 1) / c->T;
+  const int tiles = (pz + c->Tc - 1) / c->Tc;
   int chunk = 0;
-  const int nchunk = corr_chunks(c, m, &chunk);
+  const int nchunk = corr_chunks(c, m, c->Tc, &chunk);
   const float scale = (float)(1.0 / ((double)c->px.n * py * pz));
   if (c->timing) {
     c->units["corr_x"] += m;
@@ -1174,10 +1198,10 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
   }
   // column-test flags of this list (pruning needs CTL_PRUNE set by its decide)
   const int* cf = c->co.ready && !fgrid ? c->co.fs.p + kFeasHeader : nullptr;
-  launch(c, "corr_x", corr_kernel(c), dim3(tiles, py, nchunk), dim3(kPencilThreads),
-         pencil_shmem(c) + (size_t)c->px.n * c->T * sizeof(float2),
+  const CorrLaunch cl = corr_launch(c);
+  launch(c, "corr_x", cl.k, dim3(tiles, py, nchunk), dim3(cl.threads), cl.shmem,
          (const float2*)c->g1.p, (const float2*)c->g2.p, (const float2*)c->ispec.p, od, c->y1.p,
-         c->y2.p, m, chunk, max_ex, c->nx, c->px, py, pz, c->T, scale, tw, cf, ctl);
+         c->y2.p, m, chunk, max_ex, c->nx, c->px, py, pz, c->Tc, scale, tw, cf, ctl);
   const double* pen = c->pen.p + (penalty == SPK_PENALTY_CUBE ? c->nz : 0);
   const int rs = rec_stride(c);
   if (c->split) {
@@ -2445,7 +2469,7 @@ struct TuneKey {
 // the autotuner's choice for a tray shape: x-pencil tile, largest butterfly
 // of the score pass, padded y and z FFT sizes
 struct TuneChoice {
-  int T, score_rmax, py, pz;
+  int T, Tc, score_rmax, py, pz;
 };
 std::map<TuneKey, TuneChoice> g_tile;
 std::mutex g_tile_mu;
@@ -2469,7 +2493,8 @@ std::vector<spk_orient> rotation_maps() {
   return out;
 }
 
-// Autotuner, per tray shape and process: x-pencil tile (4, 8, 16), the
+// Autotuner, per tray shape and process: x-pencil tile (4, 8, 16) of the
+// pencil kernels, then pass A's own tile (4, 8, 16), the
 // score pass's largest butterfly (radix 10 or 25: different stage counts and
 // register pressure) and the padded y/z FFT sizes (the smallest
 // 2^a 3^b 5^c 7^d size or the next one: more points, maybe fewer or cheaper
@@ -2491,6 +2516,7 @@ void autotune_tile(spk_ctx* c) {
     c->sy = make_plan(c->py.n, std::min(c->yz_rmax, c->score_rmax));
     c->sz = make_plan(c->pz.n, std::min(c->yz_rmax, c->score_rmax));
     c->T = ch.T;
+    c->Tc = ch.Tc;
   };
   {
     std::lock_guard<std::mutex> g(g_tile_mu);
@@ -2500,7 +2526,7 @@ void autotune_tile(spk_ctx* c) {
       return;
     }
   }
-  TuneChoice best{c->T, c->score_rmax, c->py.n, c->pz.n};
+  TuneChoice best{c->T, c->Tc, c->score_rmax, c->py.n, c->pz.n};
   // box item about a fifth of the tray's smallest side, slightly uneven
   const int a = std::max(2, std::min({c->nx, c->ny, c->nz}) / 5);
   const int d0 = std::min(c->nx, a), d1 = std::min(c->ny, std::max(2, a * 4 / 5)),
@@ -2559,6 +2585,18 @@ void autotune_tile(spk_ctx* c) {
         if (T == base.T || c->px.n * T > kPencilMax || !(T <= 2 * c->pz.n || T == 4)) continue;
         TuneChoice ch = base;
         ch.T = T;
+        consider(ch);
+      }
+      apply(best);
+      // 1b. pass A's own tile (the register kernel for px = 240 keeps tile 4,
+      // measured fastest at both 24 and 2-4 orientations per launch)
+      const TuneChoice base2 = best;
+      const bool reg = c->fixed_fft && c->reg_x > 0 && c->px.n == 240;
+      for (int T : {4, 8, 16}) {
+        if (reg) break;
+        if (T == base2.Tc || c->px.n * T > kPencilMax || !(T <= 2 * c->pz.n || T == 4)) continue;
+        TuneChoice ch = base2;
+        ch.Tc = T;
         consider(ch);
       }
       apply(best);
@@ -2823,6 +2861,7 @@ int spk_tray_set(spk_ctx* c, const int32_t* tray, int nx, int ny, int nz, const 
       if ((long long)c->px.n * c->tile_force > kPencilMax)
         throw SpkError{SPK_EUNSUPPORTED, "tile too large for this tray's x axis"};
       c->T = c->tile_force;
+      c->Tc = c->tile_force;
     } else if (reshape && c->autotune) {
       c->valid = true;
       autotune_tile(c);
@@ -3393,6 +3432,12 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->plan_mode = v;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "reg_x") {
+    const int v = (int)value;
+    if (v < 0 || v > 2) return fail(SPK_EINVAL, "reg_x must be 0, 1 or 2");
+    c->reg_x = v;
     return SPK_OK;
   }
   if (std::string(name) == "autotune") {
