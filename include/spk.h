@@ -270,6 +270,10 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  25; "yz_radix" the same for the other y/z passes (2..25),
  *                  "pad_y"/"pad_z" padded FFT sizes (0: smallest 2^a3^b5^c7^d).
  *   "corr_chunk"   orientations per pass-A block (0: automatic).
+ *   "score2"       1 (default): the score pass holds both digit slabs in LDS
+ *                  at once and transforms them in shared stages (compile-time
+ *                  plans, when they fit the device's LDS); 0: one after the
+ *                  other.
  *   "reg_x"        pass A when the padded x length is 240 (= 16 * 15): the
  *                  x transforms in registers with one LDS exchange per
  *                  transform, 2 (default) two LDS buffers per block, 1 three

@@ -206,6 +206,8 @@ struct spk_ctx {
   int many_workers = 8;                  // option "many_workers": spk_pack_many host workers
   bool graph = false;                    // option "graph": device loops captured as a hipGraph
   bool fixed_fft = true;                 // option "fixed_fft": compile-time-plan score kernels
+  bool score2 = true;                    // option "score2": score pass with both slabs in LDS
+  int lds_max = 0;                       // the device's LDS bytes per block
   int reg_x = 2;                         // option "reg_x": register pass A for px = 240
                                          // (0 off, 1 three LDS buffers, 2 two)
   // feasibility pass
@@ -552,6 +554,7 @@ void plan_tray(spk_ctx* c) {
   upload(c, c->pen.p, pen.data(), pen.size() * sizeof(double));
   int lds_max = 0;
   HIP_OK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+  c->lds_max = lds_max;
   if ((size_t)c->ny * c->nz * sizeof(int) > (size_t)lds_max)
     throw SpkError{SPK_EUNSUPPORTED, "tray y*z plane does not fit the device's LDS"};
   // slab plan when a whole yz slab (and its low-digit sums) fits one block's
@@ -576,6 +579,10 @@ void plan_tray(spk_ctx* c) {
     allow_shmem(k_score_yz_f<FPlan<125, 25, 5>, FPlan<100, 10, 10>>, dyn);
     allow_shmem(k_score_yz_f<FPlan<100, 10, 10>, FPlan<100, 10, 10>>, dyn);
     allow_shmem(k_score_yz_f<FPlan<64, 8, 8>, FPlan<64, 8, 8>>, dyn);
+    allow_shmem(k_score_yz2<FPlan<125, 5, 5, 5>, FPlan<100, 10, 10>>, dyn);
+    allow_shmem(k_score_yz2<FPlan<125, 25, 5>, FPlan<100, 10, 10>>, dyn);
+    allow_shmem(k_score_yz2<FPlan<100, 10, 10>, FPlan<100, 10, 10>>, dyn);
+    allow_shmem(k_score_yz2<FPlan<64, 8, 8>, FPlan<64, 8, 8>>, dyn);
     allow_shmem(k_corr_x, dyn);
     allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 16>, dyn);
     allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 8>, dyn);
@@ -1128,15 +1135,22 @@ using ScoreKernel = decltype(&k_score_yz<10>);
 
 // the score pass for the context's plans: a compile-time instantiation when
 // one matches (padded sizes of the common trays), else the runtime one
+// (k_score_yz2, both slabs in LDS, when option "score2" is on and they fit)
+struct ScoreLaunch {
+  ScoreKernel k;
+  size_t shmem;
+};
 template <class PY_, class PZ_>
-bool fixed_score(const spk_ctx* c, ScoreKernel* k) {
+bool fixed_score(const spk_ctx* c, ScoreLaunch* k) {
   if (!plan_is(c->sy, PY_{}) || !plan_is(c->sz, PZ_{})) return false;
-  *k = k_score_yz_f<PY_, PZ_>;
+  const size_t sh2 = score2_shmem(PY_::N, PZ_::N);
+  if (c->score2 && sh2 + 1024 <= (size_t)c->lds_max) *k = {k_score_yz2<PY_, PZ_>, sh2};
+  else *k = {k_score_yz_f<PY_, PZ_>, score_shmem(c)};
   return true;
 }
 
-ScoreKernel score_kernel(const spk_ctx* c) {
-  ScoreKernel k = c->score_rmax == 25 ? k_score_yz<25> : k_score_yz<10>;
+ScoreLaunch score_launch(const spk_ctx* c) {
+  ScoreLaunch k{c->score_rmax == 25 ? k_score_yz<25> : k_score_yz<10>, score_shmem(c)};
   if (c->fixed_fft) {
     fixed_score<FPlan<125, 5, 5, 5>, FPlan<100, 10, 10>>(c, &k) ||
         fixed_score<FPlan<125, 25, 5>, FPlan<100, 10, 10>>(c, &k) ||
@@ -1219,8 +1233,9 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
     // the score pass has its own factorisations of py and pz (radices up to
     // c->score_rmax, compiled as the kernel's largest butterfly), or a
     // compile-time plan when one matches them
-    launch(c, "score_yz", score_kernel(c), dim3(c->nx, m),
-           dim3(kSlabThreads), score_shmem(c), (const float2*)c->y1.p, (const float2*)c->y2.p, od,
+    const ScoreLaunch sl = score_launch(c);
+    launch(c, "score_yz", sl.k, dim3(c->nx, m),
+           dim3(kSlabThreads), sl.shmem, (const float2*)c->y1.p, (const float2*)c->y2.p, od,
            (const FieldStats*)c->stats.p, pen, c->rec.p, c->nx, c->ny, c->nz, c->sy, c->sz,
            tw + c->px.n, tw + c->px.n + py, fgrid, pgrid, cf,
            c->timing && c->pairs.p ? c->pairs.p + 2 : (unsigned long long*)nullptr, ctl);
@@ -3432,6 +3447,10 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->plan_mode = v;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "score2") {
+    c->score2 = value != 0.0;
     return SPK_OK;
   }
   if (std::string(name) == "reg_x") {

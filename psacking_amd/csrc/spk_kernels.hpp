@@ -1685,6 +1685,246 @@ k_score_yz_f(const float2* __restrict__ y1, const float2* __restrict__ y2,
   wg_argmin_write(best, bidx, bp, res, fp.ok && score_order_ok(d, st) ? 0 : 1, out, tid, NT);
 }
 
+// ---- score pass with both digit slabs resident -----------------------------
+// One in-place decimation-in-frequency butterfly (the element mapping of
+// ipstage): R elements i0 + r S es, S = L / R, twiddled by w_N^(n r N / L).
+template <int R, bool FWD, int N, int L>
+__device__ __forceinline__ void ip_bfly(float2* buf, int i0, int es, int n, const float2* tw) {
+  constexpr int S = L / R, TWS = N / L;
+  float2 v[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) v[r] = buf[i0 + r * S * es];
+  Bfly<R, FWD>::run(v);
+  if (S > 1 && n != 0) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      float2 w = tw[n * r * TWS];
+      if (!FWD) w.y = -w.y;
+      v[r] = cmul(v[r], w);
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) buf[i0 + r * S * es] = v[r];
+}
+
+// One stage over the columns of two slabs at once: B1 columns of a (element
+// pitch B1) and B2 columns of b (pitch B2), every column of length N.
+template <int R, bool FWD, int NT, int N, int L, int B1, int B2>
+__device__ __forceinline__ void ipstage_cols2(float2* a, float2* b, const float2* tw, int tid) {
+  constexpr int S = L / R, M = N / R;
+  constexpr int T1 = B1 * M, TOT = (B1 + B2) * M;
+  constexpr int MS = CeilDiv<TOT, NT>::v;
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    const int e = tid + s * NT;
+    if (e < T1) {
+      const int j = e / B1, col = e - j * B1;
+      const int blk = j / S, n = j - blk * S;
+      ip_bfly<R, FWD, N, L>(a, col + (blk * L + n) * B1, B1, n, tw);
+    } else if (e < TOT) {
+      const int f = e - T1;
+      const int j = f / B2, col = f - j * B2;
+      const int blk = j / S, n = j - blk * S;
+      ip_bfly<R, FWD, N, L>(b, col + (blk * L + n) * B2, B2, n, tw);
+    }
+  }
+  __syncthreads();
+}
+template <bool FWD, int NT, int N, int L, int B1, int B2, int R, int... Rest>
+__device__ __forceinline__ void ip_cols2(float2* a, float2* b, const float2* tw, int tid) {
+  ipstage_cols2<R, FWD, NT, N, L, B1, B2>(a, b, tw, tid);
+  if constexpr (sizeof...(Rest) > 0) ip_cols2<FWD, NT, N, L / R, B1, B2, Rest...>(a, b, tw, tid);
+}
+
+// One stage over the rows (length N, contiguous, pitch PITCH) of two slabs:
+// n1 rows of a, logical row q at LDS row rowmap(q), and n2 rows of b at rows
+// 0 .. n2-1 (both counts at most NMAX in total).
+template <int R, bool FWD, int NT, int N, int L, int PITCH, int NMAX, class RowMap>
+__device__ __forceinline__ void ipstage_rows2(float2* a, int n1, float2* b, int n2,
+                                              const float2* tw, int tid, RowMap rowmap) {
+  constexpr int S = L / R, M = N / R;
+  constexpr int MS = CeilDiv<NMAX * M, NT>::v;
+  const int t1 = n1 * M, tot = (n1 + n2) * M;
+#pragma unroll
+  for (int s = 0; s < MS; ++s) {
+    const int e = tid + s * NT;
+    if (e < tot) {
+      const bool first = e < t1;
+      const int f = first ? e : e - t1;
+      const int q = f / M, j = f - q * M;
+      const int blk = j / S, n = j - blk * S;
+      float2* base = first ? a + rowmap(q) * PITCH : b + q * PITCH;
+      ip_bfly<R, FWD, N, L>(base, blk * L + n, 1, n, tw);
+    }
+  }
+  __syncthreads();
+}
+template <bool FWD, int NT, int N, int L, int PITCH, int NMAX, class RowMap, int R, int... Rest>
+__device__ __forceinline__ void ip_rows2(float2* a, int n1, float2* b, int n2, const float2* tw,
+                                         int tid, RowMap rm) {
+  ipstage_rows2<R, FWD, NT, N, L, PITCH, NMAX, RowMap>(a, n1, b, n2, tw, tid, rm);
+  if constexpr (sizeof...(Rest) > 0)
+    ip_rows2<FWD, NT, N, L / R, PITCH, NMAX, RowMap, Rest...>(a, n1, b, n2, tw, tid, rm);
+}
+template <class P> struct IpCols2;
+template <int N, int... Rs> struct IpCols2<FPlan<N, Rs...>> {
+  template <bool FWD, int NT, int B1, int B2>
+  __device__ __forceinline__ static void run(float2* a, float2* b, const float2* tw, int tid) {
+    ip_cols2<FWD, NT, N, N, B1, B2, Rs...>(a, b, tw, tid);
+  }
+};
+template <class P> struct IpRows2;
+template <int N, int... Rs> struct IpRows2<FPlan<N, Rs...>> {
+  template <bool FWD, int NT, int PITCH, int NMAX, class RowMap>
+  __device__ __forceinline__ static void run(float2* a, int n1, float2* b, int n2,
+                                             const float2* tw, int tid, RowMap rm) {
+    ip_rows2<FWD, NT, N, N, PITCH, NMAX, RowMap, Rs...>(a, n1, b, n2, tw, tid, rm);
+  }
+};
+
+// LDS of k_score_yz2: twiddles, the complex slab [PY][PZ] and the low digit's
+// half-spectrum slab [PY][HZ] (after the row pairing: [ceil(nvy/2)][PZ])
+__host__ __device__ constexpr size_t score2_shmem(int py, int pz) {
+  return (size_t)(py + pz + py * pz + py * (pz / 2 + 1)) * sizeof(float2);
+}
+
+// The score pass (same results as k_score_yz_f) with both slabs in LDS at
+// once, so that their transforms share stages: one load phase, the y pass of
+// the PZ complex columns and the HZ low-digit columns in the same stages (a
+// radix-25 y stage then has 5 (PZ + HZ) butterflies for the block's 1024
+// threads instead of 5 PZ or 5 HZ), the low digit's rows paired as in
+// pair_low_rows, and the z pass of the nvy complex rows and the paired rows
+// in the same stages.  The scores then read both slabs directly.
+template <class SY, class SZ>
+__global__ void __launch_bounds__(kSlabThreads)
+k_score_yz2(const float2* __restrict__ y1, const float2* __restrict__ y2,
+            const OrientDesc* __restrict__ od, const FieldStats* __restrict__ st,
+            const double* __restrict__ pen, WgBest* __restrict__ rec, int nx, int ny, int nz,
+            AxisPlan py_, AxisPlan pz_, const float2* __restrict__ tw_y,
+            const float2* __restrict__ tw_z, unsigned char* __restrict__ fgrid,
+            long long* __restrict__ pgrid, const int* __restrict__ cf,
+            unsigned long long* __restrict__ searched, const int* __restrict__ ctl) {
+  constexpr int PY = SY::N, PZ = SZ::N, HZ = PZ / 2 + 1, NT = kSlabThreads;
+  constexpr int EL = CeilDiv<PY * PZ, NT>::v;       // complex slab elements per thread
+  constexpr int ELH = CeilDiv<PY * HZ, NT>::v;      // half-spectrum elements per thread
+  constexpr int HMAX = (PY + 1) / 2;                // paired low-digit rows at most
+  static_assert(HMAX * PZ <= PY * HZ, "paired rows must fit the half-spectrum slab");
+  if (search_off(ctl)) return;
+  extern __shared__ float2 smem[];
+  const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
+  const OrientDesc d = od[o];
+  WgBest* out = rec + (size_t)o * nx + x;
+  if (x >= d.nvx) return;
+  if (cf && ctl && ctl[CTL_PRUNE] && !cf[o]) {   // excluded by the column test
+    if (tid == 0) {
+      WgBest b;
+      b.score = 1e300; b.idx = INT_MAX; b.p_lo = 0; b.p_hi = 0; b.resid = 0.f; b.unsupported = 0;
+      *out = b;
+    }
+    return;
+  }
+  if (searched && x == 0 && tid == 0) atomicAdd(searched, 1ULL);   // an orientation searched
+  float2* twy = smem;
+  float2* twz = twy + PY;
+  float2* hb = twz + PZ;       // complex slab (collision + high digit)
+  float2* lb = hb + PY * PZ;   // low digit
+  load_table(twy, tw_y, PY, tid, NT);
+  load_table(twz, tw_z, PZ, tid, NT);
+  const FieldParams fp = field_params(st, (long long)nx * ny * nz);
+  const long long addback = ((1LL << fp.lb) / 2 + fp.mu) * d.isum;
+  const int nvy = d.nvy, nvz = d.nvz;
+  const int h = (nvy + 1) >> 1;
+  {   // both slabs: every load issued before the first LDS store
+    const float2* s1 = y1 + ((size_t)o * nx + x) * PY * PZ;
+    const float2* s2 = y2 + ((size_t)o * nx + x) * PY * HZ;
+    float2 v[EL], u[ELH];
+#pragma unroll
+    for (int s = 0; s < EL; ++s) {
+      const int e = tid + s * NT;
+      if (e < PY * PZ) v[s] = s1[e];
+    }
+#pragma unroll
+    for (int s = 0; s < ELH; ++s) {
+      const int e = tid + s * NT;
+      if (e < PY * HZ) u[s] = s2[e];
+    }
+#pragma unroll
+    for (int s = 0; s < EL; ++s) {
+      const int e = tid + s * NT;
+      if (e < PY * PZ) hb[e] = v[s];
+    }
+#pragma unroll
+    for (int s = 0; s < ELH; ++s) {
+      const int e = tid + s * NT;
+      if (e < PY * HZ) lb[e] = u[s];
+    }
+  }
+  __syncthreads();
+  // y pass of both slabs; logical row j then sits at LDS row yrow(j)
+  const auto yrow = [](int j) { return PlanPos<SY>::at(j); };
+  const auto zcol = [](int k) { return PlanPos<SZ>::at(k); };
+  IpCols2<SY>::template run<false, NT, PZ, HZ>(hb, lb, twy, tid);
+  {   // low digit: rows j and j + h paired into one complex row (pair_low_rows)
+    constexpr int MP = CeilDiv<HMAX * HZ, NT>::v;
+    float2 a[MP], bq[MP];
+#pragma unroll
+    for (int s = 0; s < MP; ++s) {
+      const int e = tid + s * NT;
+      if (e < h * HZ) {
+        const int j = e / HZ, k = e - j * HZ;
+        a[s] = lb[yrow(j) * HZ + k];
+        bq[s] = j + h < nvy ? lb[yrow(j + h) * HZ + k] : make_float2(0.f, 0.f);
+      }
+    }
+    __syncthreads();   // the paired rows overwrite the half-spectrum layout
+#pragma unroll
+    for (int s = 0; s < MP; ++s) {
+      const int e = tid + s * NT;
+      if (e < h * HZ) {
+        const int j = e / HZ, k = e - j * HZ;
+        lb[j * PZ + k] = make_float2(a[s].x - bq[s].y, a[s].y + bq[s].x);   // a + i b
+        if (k > 0 && PZ - k >= HZ) lb[j * PZ + PZ - k] = make_float2(a[s].x + bq[s].y, bq[s].x - a[s].y);
+      }
+    }
+  }
+  __syncthreads();
+  // z pass: the nvy complex rows (LDS rows yrow(j)) and the h paired rows
+  IpRows2<SZ>::template run<false, NT, PZ, PY + HMAX>(hb, nvy, lb, h, twz, tid, yrow);
+  // scores of the valid offsets (slab_scores, with the low digit read here)
+  const ColWalk w = col_walk<NT>(nvz, tid);
+  float res = 0.f;
+  double best = 1e300;
+  int bidx = INT_MAX;
+  long long bp = 0;
+  const size_t gbase = (size_t)o * nx * ny * nz;
+  for (int k = w.c0; k < nvz; k += w.cols) {
+    long long pm = 0;
+    int jm = -1;
+    const int ck = zcol(k);
+    for (int j = w.g; j < nvy; j += w.G) {
+      const float2 v = hb[yrow(j) * PZ + ck];
+      const float lv = j < h ? lb[j * PZ + ck].x : lb[(j - h) * PZ + ck].y;
+      const float c = rintf(v.x), hh = rintf(v.y), lr = rintf(lv);
+      res = fmaxf(res, fmaxf(fabsf(lv - lr), fmaxf(fabsf(v.x - c), fabsf(v.y - hh))));
+      // b * hh with b = 2^lb: a shift of the (two's complement) high digit
+      const long long p =
+          (long long)((unsigned long long)(long long)hh << fp.lb) + (long long)lr + addback;
+      if (fgrid) {
+        const size_t g = gbase + ((size_t)x * ny + j) * nz + k;
+        fgrid[g] = c == 0.f ? 1 : 0;
+        pgrid[g] = p;
+      }
+      if (c == 0.f && (jm < 0 || p < pm)) { pm = p; jm = j; }
+    }
+    if (jm >= 0) {
+      const double sc = (double)pm + pen[k];
+      const int idx = (x * ny + jm) * nz + k;
+      if (better(sc, idx, best, bidx)) { best = sc; bidx = idx; bp = pm; }
+    }
+  }
+  wg_argmin_write(best, bidx, bp, res, fp.ok && score_order_ok(d, st) ? 0 : 1, out, tid, NT);
+}
+
 // ---------------------------------------------------------------------------
 // Feasibility pass: does any valid offset of an orientation have zero
 // collision with the occupied voxels?  Collision-only correlations with the
