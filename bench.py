@@ -46,7 +46,7 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = ["dist_zy", "dist_x", "inc_zy", "inc_x", "inc_stats", "learn_gate", "learn_pick", "tray_slab_fwd", "tray_rows_fwd", "tray_x_fwd", "item_slab_fwd",
            "item_rows_fwd", "pencil_y_fwd", "corr_x", "score_yz", "pencil_y_inv", "rows_score",
            "select", "verify", "decide", "place", "direct_search", "gather_od", "proposer",
-           "proposer_items", "record", "feas_begin", "feas_x", "feas_yz", "feas_decide",
+           "proposer_items", "record", "spec_add", "feas_begin", "feas_x", "feas_yz", "feas_decide",
            "tray_columns", "tray_fine", "tile_heights", "tray_masks_inc", "core_columns", "item_columns",
            "run_lists", "item_masks", "column_test", "pre_masks", "pre_column_test", "coarse_decide",
            "post"]

@@ -227,7 +227,8 @@ int spk_timing_get(spk_ctx* ctx, const char* kernel, double* total_ms, int64_t* 
  * per-unit rates: for "score_yz" and "corr_x" the orientations the score pass
  * searched, for the feasibility pass the orientation pairs it transformed
  * (both counted on the device: orientations the column test excluded do not
- * count); for other kernels the orientations handed to them */
+ * count), for "spec_add" the incremental tray-spectrum updates that ran; for
+ * other kernels the orientations handed to them */
 int spk_timing_units(spk_ctx* ctx, const char* kernel, int64_t* units);
 int spk_timing_reset(spk_ctx* ctx);
 /* Tuning/diagnostic options of a context (no reference counterpart):
@@ -270,6 +271,13 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  25; "yz_radix" the same for the other y/z passes (2..25),
  *                  "pad_y"/"pad_z" padded FFT sizes (0: smallest 2^a3^b5^c7^d).
  *   "corr_chunk"   orientations per pass-A block (0: automatic).
+ *   "spec_inc"     after a placement in a device-driven loop that leaves the
+ *                  distance field unchanged (sources "eq1", the library's own
+ *                  field, a placed id other than 1, an item of 0/1 values,
+ *                  slab plan), the tray spectrum is updated by the placed
+ *                  voxels alone, G1 += e^{-2 pi i k.q / P} FFT(item), at most
+ *                  this many times in a row before a full transform (default
+ *                  16; 0: always the full transform).
  *   "score2"       1 (default): the score pass holds both digit slabs in LDS
  *                  at once and transforms them in shared stages (compile-time
  *                  plans, when they fit the device's LDS); 0: one after the

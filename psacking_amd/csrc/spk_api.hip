@@ -206,6 +206,7 @@ struct spk_ctx {
   int many_workers = 8;                  // option "many_workers": spk_pack_many host workers
   bool graph = false;                    // option "graph": device loops captured as a hipGraph
   bool fixed_fft = true;                 // option "fixed_fft": compile-time-plan score kernels
+  int spec_inc = 16;                     // option "spec_inc": incremental G1 updates in a row
   bool score2 = true;                    // option "score2": score pass with both slabs in LDS
   int lds_max = 0;                       // the device's LDS bytes per block
   int reg_x = 2;                         // option "reg_x": register pass A for px = 240
@@ -636,8 +637,26 @@ void run_distance(spk_ctx* c, const int* d_grid, int* d_out, int* d_tmp, FieldSt
          (size_t)nx * TK * sizeof(int), (const int*)d_tmp, d_out, d_st, nx, ny, nz, fill, ctl);
 }
 
-struct PlaceRef;
+// The placement a refresh follows, for the incremental distance update: the
+// device's search outcome (so, with od the searched orientations) or an
+// explicit orientation od[orient] at (qx, qy, qz); id as k_place takes it
+// (the loop adds the placed count on the device).
+struct PlaceRef {
+  const SearchOut* so = nullptr;
+  const OrientDesc* od = nullptr;
+  int orient = 0, qx = 0, qy = 0, qz = 0, id = 0;
+  int max_ex = 0;   // largest x extent among the orientations od may name
+  int max_dim = 0;  // largest extent along any axis among them (0: unknown)
+};
+
 void update_coarse(spk_ctx* c, const int* ctl, const PlaceRef& pr);
+// Consecutive incremental updates of G1 a device loop may make (k_decide's
+// spec_max): the field must be this library's own transform of the tray with
+// sources "eq1" (a placement with id != 1 then adds no source and leaves it
+// unchanged), on the slab plan; 0 when that does not hold.
+int spec_max(const spk_ctx* c) {
+  return (!c->split && c->sources == SPK_SOURCES_EQ1 && c->phi_internal) ? c->spec_inc : 0;
+}
 // the cached spectra of the tray, then the column test's tray masks: rebuilt
 // (pr == nullptr) or updated for the placement pr
 void build_spectra(spk_ctx* c, const int* ctl = nullptr, const PlaceRef* pr = nullptr) {
@@ -669,21 +688,17 @@ void build_spectra(spk_ctx* c, const int* ctl = nullptr, const PlaceRef* pr = nu
   launch(c, "tray_x_fwd", tray_x_kernel(c), dim3((pz + c->T - 1) / c->T, py, 2),
          dim3(kPencilThreads), pencil_shmem(c), (const float2*)c->t1.p, (const float2*)c->t2.p,
          c->g1.p, c->g2.p, c->nx, c->px, py, pz, c->T, tw, ctl);
+  // after a placement in a device loop: the incremental update instead, when
+  // k_decide chose it (the two kernels above then exit at once)
+  if (ctl && pr && pr->so && spec_max(c) > 0)
+    launch(c, "spec_add", k_spec_add, dim3((pz + c->T - 1) / c->T, py), dim3(kPencilThreads),
+           pencil_shmem(c), c->g1.p, (const float2*)c->ispec.p, pr->od, pr->so, pr->max_ex,
+           c->ny, c->nz, c->px, py, pz, c->T, tw,
+           c->timing && c->pairs.p ? c->pairs.p + 3 : (unsigned long long*)nullptr, ctl);
   if (pr && c->inc_masks) update_coarse(c, ctl, *pr);
   else build_coarse_spectra(c, ctl);
 }
 
-// The placement a refresh follows, for the incremental distance update: the
-// device's search outcome (so, with od the searched orientations) or an
-// explicit orientation od[orient] at (qx, qy, qz); id as k_place takes it
-// (the loop adds the placed count on the device).
-struct PlaceRef {
-  const SearchOut* so = nullptr;
-  const OrientDesc* od = nullptr;
-  int orient = 0, qx = 0, qy = 0, qz = 0, id = 0;
-  int max_ex = 0;   // largest x extent among the orientations od may name
-  int max_dim = 0;  // largest extent along any axis among them (0: unknown)
-};
 
 // The column test's tray masks after the placement pr: only the columns under
 // the placed box are rebuilt (k_tray_masks_inc), when the box side is known.
@@ -776,6 +791,7 @@ struct HostItem {
   int dims[3];
   int lo[3], hi[3];   // bounds of voxels > 0, hi = -1 when empty
   long long sum;
+  int vmax;           // largest voxel value
 };
 
 // Bounds and sum of the voxels > 0 of an item.  Row by row: the minimum and
@@ -785,12 +801,12 @@ struct HostItem {
 HostItem scan_item(const int* data, int ix, int iy, int iz, int* copy = nullptr) {
   if (!data && (long long)ix * iy * iz > 0) throw SpkError{SPK_EINVAL, "null item"};
   check_dims(ix, iy, iz);
-  HostItem h{data, {ix, iy, iz}, {INT_MAX, INT_MAX, INT_MAX}, {-1, -1, -1}, 0};
+  HostItem h{data, {ix, iy, iz}, {INT_MAX, INT_MAX, INT_MAX}, {-1, -1, -1}, 0, 0};
   for (int i = 0; i < ix; ++i)
     for (int j = 0; j < iy; ++j) {
       const size_t r0 = ((size_t)i * iy + j) * iz;
       const int* row = data + r0;
-      int mn = INT_MAX;
+      int mn = INT_MAX, mx = 0;
       long long s = 0;
       if (copy) {
         int* dst = copy + r0;
@@ -798,14 +814,17 @@ HostItem scan_item(const int* data, int ix, int iy, int iz, int* copy = nullptr)
           const int v = row[k];
           dst[k] = v;
           mn = std::min(mn, v);
+          mx = std::max(mx, v);
           s += v;
         }
       } else {
         for (int k = 0; k < iz; ++k) {
           mn = std::min(mn, row[k]);
+          mx = std::max(mx, row[k]);
           s += row[k];
         }
       }
+      h.vmax = std::max(h.vmax, mx);
       if (mn < 0) throw SpkError{SPK_EINVAL, "negative item voxel values are not supported"};
       if (s == 0) continue;
       int k0 = 0, k1 = iz - 1;
@@ -851,6 +870,7 @@ OrientDesc make_desc(const spk_ctx* c, const HostItem& it, int item_off, const s
   d.hx = hi[0]; d.hy = hi[1]; d.hz = hi[2];
   d.nvx = nv[0]; d.nvy = nv[1]; d.nvz = nv[2];
   d.isum = it.sum;
+  d.binary = it.vmax <= 1 ? 1 : 0;
   return d;
 }
 
@@ -1563,7 +1583,8 @@ void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e
     enqueue_search(c, od, m, max_ex, max_e, penalty, c->ctl.p, nullptr, nullptr, nullptr);
   }
   launch(c, "decide", k_decide, dim3(1), dim3(64), 0, (const SearchOut*)c->sout.p,
-         (const long long*)c->ver.p, c->pout.p, t, c->resid_limit, c->ctl.p, halt_not_found);
+         (const long long*)c->ver.p, c->pout.p, t, c->resid_limit, c->ctl.p, halt_not_found,
+         spec_max(c), od, (int)id_offset);
   if (rec_row)
     launch(c, "record", k_record_scores, dim3(1), dim3(64), 0, (const WgBest*)c->per.p, m, rec_map,
            rec_row, (const int*)c->ctl.p);
@@ -1788,6 +1809,7 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
   const bool rounds = c->prescreen > 0 && !rec_feat && !rec_scores;
   std::vector<char> skip(n_items, 0);
   std::vector<const int*> pre(n_items, nullptr);
+  int spec_n = 0;   // incremental spectrum updates in a row, carried across rounds
   while (t0 < n_items) {
     const int t_end = rounds ? std::min(n_items, t0 + c->prescreen) : n_items;
     if (rounds) {
@@ -1798,7 +1820,8 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
         pre[t] = pre_flags(c, plans, t, live[t - t0], od0);
       }
     }
-    const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
+    int ctl0[CTL_WORDS] = {0, -1, 0, placed};
+    ctl0[CTL_SPEC_N] = spec_n;
     upload(c, c->ctl.p, ctl0, sizeof(ctl0));
     LoopGraph lg(c, c->graph && !rec_feat && !rec_scores);
     for (int t = t0; t < t_end; ++t) {
@@ -1832,6 +1855,8 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
                 po[t - t0].collision, s.prox, s.resid, 0, &results[t]);
       if (s.found) ++placed;
     }
+    // (an item placed on the host's exact path is followed by a full transform)
+    spec_n = ctl[CTL_HALT] ? 0 : ctl[CTL_SPEC_N];
     if (!ctl[CTL_HALT]) {
       t0 = t_end;
       continue;
@@ -2323,6 +2348,7 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   std::vector<char> skip(n_items, 0), propose(n_items, 1);
   std::vector<int> slot(n_items, 0);
   std::vector<const int*> pre(n_items, nullptr);
+  int spec_n = 0;   // incremental spectrum updates in a row, carried across rounds
   while (t0 < n_items) {
     const int t_end = c->prescreen > 0 ? std::min(n_items, t0 + c->prescreen) : n_items;
     if (c->prescreen > 0) {
@@ -2343,7 +2369,8 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
         }
       learned_item_pool(c, nn, bases, off, order, orients, n_orient, &need);
     }
-    const int ctl0[CTL_WORDS] = {0, -1, 0, placed};
+    int ctl0[CTL_WORDS] = {0, -1, 0, placed};
+    ctl0[CTL_SPEC_N] = spec_n;
     upload(c, c->ctl.p, ctl0, sizeof(ctl0));
     int* ctl = c->ctl.p;
     LoopGraph lg(c, c->graph);
@@ -2374,7 +2401,8 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       enqueue_item_spectra(c, od, topk, pl.max_ex, ctl);
       enqueue_scored(c, od, topk, pl.max_ex, pl.max_e, penalty, ctl, nullptr, nullptr, nullptr);
       launch(c, "decide", k_decide, dim3(1), dim3(64), 0, (const SearchOut*)c->sout.p,
-             (const long long*)c->ver.p, c->pout.p, t, c->resid_limit, ctl, 1);
+             (const long long*)c->ver.p, c->pout.p, t, c->resid_limit, ctl, 1, spec_max(c), od,
+             (int)id_offset);
       const int nb = std::max(1, std::min(1024, (pl.max_e + 255) / 256));
       launch(c, "place", k_place, dim3(nb), dim3(256), 0, c->tray.p, (const int*)c->items.p, od,
              (const SearchOut*)c->sout.p, 0, 0, 0, 0, (int)id_offset, c->nx, c->ny, c->nz, 1,
@@ -2429,6 +2457,7 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
         for (int j = 0; j < topk; ++j)
           proposals[(size_t)t * topk + j] = pk[j] >= 0 ? pl.map[pk[j]] : -1;
     }
+    spec_n = cw[CTL_HALT] ? 0 : cw[CTL_SPEC_N];
     if (!cw[CTL_HALT]) {
       t0 = t_end;
       continue;
@@ -3385,8 +3414,8 @@ int spk_save_vox(const int32_t* grid, int nx, int ny, int nz, const char* path) 
 int spk_timing_enable(spk_ctx* c, int on) {
   return guarded(c, [&]() {
     if (on && !c->pairs.p) {
-      c->pairs.ensure(3);
-      HIP_OK(hipMemsetAsync(c->pairs.p, 0, 3 * sizeof(unsigned long long), c->stream));
+      c->pairs.ensure(4);
+      HIP_OK(hipMemsetAsync(c->pairs.p, 0, 4 * sizeof(unsigned long long), c->stream));
     }
     c->timing = on != 0;
     return SPK_OK;
@@ -3407,15 +3436,16 @@ int spk_timing_units(spk_ctx* c, const char* kernel, int64_t* units) {
   return guarded(c, [&]() {
     const std::string k = kernel ? kernel : "";
     long long v = 0;
-    if (k == "feas_x" || k == "feas_yz" || ((k == "score_yz" || k == "corr_x") && c->pairs.p)) {
+    if (k == "feas_x" || k == "feas_yz" ||
+        ((k == "score_yz" || k == "corr_x" || k == "spec_add") && c->pairs.p)) {
       // counted on the device: pairs and orientations the column test
-      // excluded never run
-      unsigned long long h[3] = {0, 0, 0};
+      // excluded never run; incremental spectrum updates that ran
+      unsigned long long h[4] = {0, 0, 0, 0};
       if (c->pairs.p) {
         download(c, h, c->pairs.p, sizeof(h));
         sync(c);
       }
-      v = (long long)(k == "score_yz" || k == "corr_x" ? h[2] : h[0]);
+      v = (long long)(k == "spec_add" ? h[3] : (k == "score_yz" || k == "corr_x") ? h[2] : h[0]);
     } else {
       auto it = c->units.find(k);
       v = it == c->units.end() ? 0 : it->second;
@@ -3430,7 +3460,7 @@ int spk_timing_reset(spk_ctx* c) {
     c->flush_timing();
     c->acc.clear();
     c->units.clear();
-    if (c->pairs.p) HIP_OK(hipMemsetAsync(c->pairs.p, 0, 3 * sizeof(unsigned long long), c->stream));
+    if (c->pairs.p) HIP_OK(hipMemsetAsync(c->pairs.p, 0, 4 * sizeof(unsigned long long), c->stream));
     return SPK_OK;
   });
 }
@@ -3447,6 +3477,11 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->plan_mode = v;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "spec_inc") {
+    if (value < 0 || value > 1e6) return fail(SPK_EINVAL, "spec_inc must be in [0, 1e6]");
+    c->spec_inc = (int)value;
     return SPK_OK;
   }
   if (std::string(name) == "score2") {

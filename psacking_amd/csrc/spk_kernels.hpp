@@ -75,6 +75,7 @@ struct OrientDesc {          // one searched orientation
   int hx, hy, hz;            // oriented max index of voxels > 0 (-1 if empty)
   int nvx, nvy, nvz;         // valid offsets per axis: N - 1 - h  (+1), clamped
   int line_off;              // the base item's line masks (k_item_lines), -1 if none
+  int binary;                // every voxel value is 0 or 1
   long long isum;            // sum of item values
 };
 
@@ -104,10 +105,14 @@ struct SearchOut {           // written by k_select, read by verify/place
 // item's scored search is skipped; CTL_FEAS: the feasibility pass runs for
 // the current item; CTL_PRUNE: the coarse pass ran and the full-resolution
 // pass may skip orientation pairs it proved infeasible; CTL_COARSE_DONE: the
-// coarse pass proved every orientation infeasible (no full-resolution pass).
+// coarse pass proved every orientation infeasible (no full-resolution pass);
+// CTL_SPEC: the placement left the distance field as it was, so the tray
+// spectrum G1 is updated by the placed voxels alone (k_spec_add) and the
+// tray's forward transforms are skipped; CTL_SPEC_N: such updates since the
+// last full transform.
 enum {
   CTL_HALT = 0, CTL_HALT_ITEM = 1, CTL_SKIP_REFRESH = 2, CTL_PLACED = 3, CTL_SKIP_SEARCH = 4,
-  CTL_FEAS = 5, CTL_PRUNE = 6, CTL_COARSE_DONE = 7, CTL_WORDS = 8
+  CTL_FEAS = 5, CTL_PRUNE = 6, CTL_COARSE_DONE = 7, CTL_SPEC = 8, CTL_SPEC_N = 9, CTL_WORDS = 10
 };
 // search kernels stop once a search has been handed back to the host, and
 // for an item the feasibility pass has already decided
@@ -121,6 +126,10 @@ __device__ __forceinline__ bool feas_off(const int* ctl) {
 // refresh kernels also stop when the last item placed nothing
 __device__ __forceinline__ bool refresh_off(const int* ctl) {
   return ctl && (ctl[CTL_HALT] | ctl[CTL_SKIP_REFRESH]);
+}
+// the tray's forward transforms also stop when k_spec_add updates G1
+__device__ __forceinline__ bool spectra_off(const int* ctl) {
+  return ctl && (ctl[CTL_HALT] | ctl[CTL_SKIP_REFRESH] | ctl[CTL_SPEC]);
 }
 // full-resolution work made unnecessary by the coarse pass
 __device__ __forceinline__ bool coarse_done(const int* ctl) {
@@ -824,7 +833,7 @@ k_tray_slab_fwd(const int* __restrict__ tray, const int* __restrict__ phi,
                 const float2* __restrict__ tw_y, const float2* __restrict__ tw_z,
                 const int* __restrict__ ctl) {
   constexpr int FY = SY::N, FZ = SZ::N, FH = FZ / 2 + 1;
-  if (refresh_off(ctl)) return;
+  if (spectra_off(ctl)) return;
   extern __shared__ float2 smem[];
   const int py = FY > 0 ? FY : py_.n, pz = FZ > 0 ? FZ : pz_.n, hz = pz / 2 + 1;
   float2* twy = smem;
@@ -899,7 +908,7 @@ __global__ void __launch_bounds__(kPencilThreads)
 k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
              float2* __restrict__ g1, float2* __restrict__ g2, int nx, AxisPlan px_, int py,
              int pz, int T_, const float2* __restrict__ tw_x, const int* __restrict__ ctl) {
-  if (refresh_off(ctl)) return;
+  if (spectra_off(ctl)) return;
   extern __shared__ float2 smem[];
   const int px = SX::N > 0 ? SX::N : px_.n, hz = pz / 2 + 1;
   const int T = TT > 0 ? TT : T_;
@@ -950,6 +959,49 @@ k_tray_x_fwd(const float2* __restrict__ t1, const float2* __restrict__ t2,
 // cf (nullable): column-test flags of the list; with CTL_PRUNE set, excluded
 // orientations need no spectra (neither pass reads them)
 // SY, SZ: compile-time plans of the padded y and z lengths, or DynPlan.
+// Incremental tray spectrum.  After a placement that leaves the distance
+// field (and so h, l and their statistics) as it was, B = (tray != 0) gains
+// the placed voxels, the oriented item I (values 0 or 1) shifted by the
+// offset q, and G1 = FFT(B + i h) gains
+//     FFT(I shifted by q)[k] = e^{-2 pi i (kx qx / px + ky qy / py + kz qz / pz)} FFT(I)[k].
+// FFT(I) comes from the winner orientation's yz spectra (ispec, written by
+// the item's search) transformed along x here.  Grid (z tiles of T, py),
+// kPencilThreads threads; runs only when k_decide set CTL_SPEC.
+__global__ void __launch_bounds__(kPencilThreads)
+k_spec_add(float2* __restrict__ g1, const float2* __restrict__ ispec,
+           const OrientDesc* __restrict__ od, const SearchOut* __restrict__ so, int max_ex,
+           int ny, int nz, AxisPlan px_, int py, int pz, int T, const float2* __restrict__ tw,
+           unsigned long long* __restrict__ applied, const int* __restrict__ ctl) {
+  if (!ctl || refresh_off(ctl) || !ctl[CTL_SPEC]) return;
+  if (applied && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) atomicAdd(applied, 1ULL);
+  extern __shared__ float2 smem[];
+  const int px = px_.n, tid = threadIdx.x;
+  float2* twx = smem;
+  float2* buf = twx + px;
+  const SearchOut s = *so;
+  const OrientDesc d = od[s.orient];
+  const int qz = s.idx % nz, qy = (s.idx / nz) % ny, qx = s.idx / (ny * nz);
+  const int ky = blockIdx.y, kz0 = blockIdx.x * T;
+  load_table(twx, tw, px, tid, kPencilThreads);
+  const float2* is = ispec + (size_t)s.orient * max_ex * py * pz;
+  for (int e = tid; e < px * T; e += kPencilThreads) {
+    const int x = e / T, kz = kz0 + (e - x * T);
+    buf[e] = (x < d.ex && kz < pz) ? is[((size_t)x * py + ky) * pz + kz] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  lds_fft<true, kPencilThreads, kPencilMaxEl, kPencilRadix>(buf, px_, T, T, 1, false, twx, tid);
+  // phases from the tables exp(-2 pi i q / n) of the three axes (k and q
+  // are below 1024 per axis: the products fit 32 bits)
+  const float2 wy = tw[px + (ky * qy) % py];
+  for (int e = tid; e < px * T; e += kPencilThreads) {
+    const int kx = e / T, kz = kz0 + (e - kx * T);
+    if (kz >= pz) continue;
+    const float2 w = cmul(cmul(twx[(kx * qx) % px], wy), tw[px + py + (kz * qz) % pz]);
+    float2* g = g1 + ((size_t)kx * py + ky) * pz + kz;
+    *g = cadd(*g, cmul(buf[e], w));
+  }
+}
+
 template <class SY, class SZ>
 __global__ void __launch_bounds__(kSlabThreads)
 k_item_slab_fwd(const int* __restrict__ items, const OrientDesc* __restrict__ od,
@@ -3226,7 +3278,8 @@ k_place(int* __restrict__ tray, const int* __restrict__ items, const OrientDesc*
 // CTL_HALT = 2, so that the host can search every orientation of the item.
 __global__ void k_decide(const SearchOut* __restrict__ so, const long long* __restrict__ ver,
                          PackOut* __restrict__ out, int t, float resid_limit,
-                         int* __restrict__ ctl, int halt_not_found) {
+                         int* __restrict__ ctl, int halt_not_found, int spec_max = 0,
+                         const OrientDesc* __restrict__ od = nullptr, int id_offset = 0) {
   // an item the feasibility pass decided (nothing feasible) keeps its record
   if (threadIdx.x != 0 || ctl[CTL_HALT] || ctl[CTL_SKIP_SEARCH]) return;
   const SearchOut s = *so;
@@ -3243,7 +3296,15 @@ __global__ void k_decide(const SearchOut* __restrict__ so, const long long* __re
     return;
   }
   ctl[CTL_SKIP_REFRESH] = s.found ? 0 : 1;
-  if (s.found) ctl[CTL_PLACED] += 1;
+  if (!s.found) return;
+  ctl[CTL_PLACED] += 1;
+  // spec_max > 0 (sources "eq1", the field this library's own): a placement
+  // with id != 1 adds no source, so the field stays and G1 is updated by the
+  // placed voxels (k_spec_add), at most spec_max times in a row
+  const int id = id_offset + ctl[CTL_PLACED];
+  const bool inc = spec_max > 0 && id != 1 && od[s.orient].binary && ctl[CTL_SPEC_N] < spec_max;
+  ctl[CTL_SPEC] = inc ? 1 : 0;
+  ctl[CTL_SPEC_N] = inc ? ctl[CTL_SPEC_N] + 1 : 0;
 }
 
 // Up to 24 orientation indices passed by value.

@@ -267,7 +267,7 @@ def test_orientation_count_case(P, kats):
 
 
 def _session_pack(P, w, n_orient, resid_limit=None, metric="l1", sources="eq1", n=None,
-                  interlocking_free=False, options=()):
+                  interlocking_free=False, options=(), timing=False):
     from psacking_amd.rotations import orientation_maps
     items = w.items if n is None else w.items[:n]
     vols = [int((it > 0).sum()) for it in items]
@@ -279,6 +279,8 @@ def _session_pack(P, w, n_orient, resid_limit=None, metric="l1", sources="eq1", 
     for name, value in options:
         s.set_option(name, value)
     s.set_tray(np.zeros(w.tray_size, np.int32), metric, sources, 0)
+    if timing:
+        s.timing(True)
     res = s.pack(items, np.array(order, np.int32), perms, flips,
                  "pow" if n_orient == 1 and not interlocking_free else "cube", interlocking_free, 0)
     exp = O.pack_voxels(items, w.tray_size, num_orientations=n_orient, order=order,
@@ -738,3 +740,37 @@ def test_incremental_distance_matches_oracle(P, metric, sources):
         t = O.place_in_tray(item, t, res.position, placed)
         np.testing.assert_array_equal(s.tray(), t)
         np.testing.assert_array_equal(s.distance(), O.distance_field(t, metric, sources))
+
+
+@pytest.mark.parametrize("spec_inc", [0, 1, 16, 1000])
+def test_incremental_spectrum_matches_oracle(P, spec_inc):
+    """Sources "eq1": a placement with id != 1 adds no distance source, so the
+    device loop updates the tray spectrum by the placed voxels alone (option
+    spec_inc: at most that many updates in a row, then a full transform).
+    Whole pack runs against the oracle with 1 and 24 orientations; every
+    search stays on the FFT path and the update ran when it is on."""
+    from psacking_amd import workloads as W
+    runs = [(W.blobs(), 1, 25), (W.blobs(seed=15, n=12, tray=(64, 48, 40)), 24, None)]
+    for w, n_orient, n in runs:
+        say("incremental spectrum", spec_inc, w.name, n_orient)
+        s, res, exp, order = _session_pack(P, w, n_orient, n=n, options=(("spec_inc", spec_inc),),
+                                           timing=True)
+        by_item = {p.item_index: p for p in exp.placements}
+        for t, r in enumerate(res):
+            e = by_item[order[t]]
+            assert r["path"] == 0 and r["found"] == e.success, (t, r)
+            if e.success:
+                assert (tuple(r["position"]), r["orientation"], r["score"]) == \
+                    (e.position, e.orientation_index, e.score)
+        np.testing.assert_array_equal(s.tray(), exp.tray)
+        placed = sum(1 for r in res if r["found"])
+        updates = s.timing_units("spec_add")
+        say("  placed", placed, "incremental updates", updates,
+            "max residual", max(float(r["residual"]) for r in res))
+        if spec_inc == 0:
+            assert updates == 0
+        else:
+            # every placement after the first (id 1, which adds the sources)
+            # updates incrementally except the full transform after each
+            # spec_inc updates in a row
+            assert updates == (placed - 1) - (placed - 1) // (spec_inc + 1)
