@@ -155,7 +155,7 @@ def bench_uneven(args) -> int:
         "uneven runs with and without kernel events differ"
     kt = {}
     for s in ev_sessions:
-        for n in KERNELS + ["ff_seed", "ff_tile", "ff_batch_init", "masked_argmin"]:
+        for n in KERNELS + ["ff_seed", "ff_tile", "ff_batch_init", "ff_unpack", "masked_argmin"]:
             ms, cnt = s.timing_get(n)
             a = kt.setdefault(n, [0.0, 0])
             a[0] += ms

@@ -271,6 +271,10 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  25; "yz_radix" the same for the other y/z passes (2..25),
  *                  "pad_y"/"pad_z" padded FFT sizes (0: smallest 2^a3^b5^c7^d).
  *   "corr_chunk"   orientations per pass-A block (0: automatic).
+ *   "ff_bits"      1 (default): the interlocking flood fill of trays with
+ *                  z <= 128 holds each (x, y) column of an offset grid as a
+ *                  128-bit mask (runs of free offsets fill along z in one
+ *                  step); 0: one byte per offset.
  *   "spec_inc"     after a placement in a device-driven loop that leaves the
  *                  distance field unchanged (sources "eq1", the library's own
  *                  field, a placed id other than 1, an item of 0/1 values,

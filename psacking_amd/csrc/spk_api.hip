@@ -180,6 +180,8 @@ struct spk_ctx {
   DevBuf<long long> pgrid;               // IF: proximity grids [m][V]
   DevBuf<int> exits;                     // IF: high seed faces [m][3]
   DevBuf<unsigned char> ff_act;          // IF: tiles changed in the last round (two buffers)
+  DevBuf<U128> fb_free, fb_reach;        // IF, nz <= 128: free / reached bit columns [m][nx][ny]
+  bool ff_bits = true;                   // option "ff_bits": bit-column flood fill
   DevBuf<int> ff_flags;                  // IF: go / changed flags of a batch of rounds
   // greedy loop
   DevBuf<OrientDesc> od_all;             // descriptors of every item's fitting orientations
@@ -946,11 +948,52 @@ WgBest exact_search_one(spk_ctx* c, const HostItem& it, const OrientDesc& d, int
 // to a change are worked); the host reads one flag per batch: the last
 // round's change flag.  Every round that reports a change reaches at least
 // one new cell, so the loop ends.
+// The same fixpoint on bit columns (nz <= 128, option "ff_bits"): pack the
+// free offsets and seeds into 128-bit column masks, rounds of k_ffb_tile over
+// 32 x 32-column tiles, then the reached bits back to one byte per offset.
+void flood_fill_bits(spk_ctx* c, int m, int nx, int ny, int nz) {
+  const size_t ncol = (size_t)nx * ny, V = ncol * nz;
+  c->freach.ensure((size_t)m * V);
+  c->fb_free.ensure((size_t)m * ncol);
+  c->fb_reach.ensure((size_t)m * ncol);
+  const dim3 cg((unsigned)((ncol + 3) / 4), m);
+  launch(c, "ff_seed", k_ffb_pack, cg, dim3(256), 0, (const unsigned char*)c->ffree.p,
+         c->fb_free.p, c->fb_reach.p, (const int*)c->exits.p, nx, ny, nz);
+  const int xt = (nx + FBT - 1) / FBT, yt = (ny + FBT - 1) / FBT;
+  const size_t tiles = (size_t)xt * yt * m;
+  c->ff_act.ensure(2 * tiles);
+  c->ff_flags.ensure(kFfBatchMax + 1);
+  int batch = 8;
+  size_t round = 0;
+  for (size_t it = 0; it <= (size_t)m * V; it += batch) {
+    launch(c, "ff_batch_init", k_ff_batch_init, dim3(round == 0 ? 256 : 1), dim3(256), 0,
+           c->ff_flags.p, batch, round == 0 ? c->ff_act.p + (round & 1) * tiles : nullptr,
+           (long long)tiles);
+    for (int r = 0; r < batch; ++r, ++round)
+      launch(c, "ff_tile", k_ffb_tile, dim3(xt * yt, m), dim3(FBT * FBT), 0,
+             (const U128*)c->fb_free.p, c->fb_reach.p, nx, ny, xt, yt,
+             (const int*)c->ff_flags.p + r, c->ff_flags.p + r + 1,
+             (const unsigned char*)c->ff_act.p + (round & 1) * tiles,
+             c->ff_act.p + ((round + 1) & 1) * tiles);
+    int ch = 0;
+    download(c, &ch, c->ff_flags.p + batch, sizeof(int));
+    sync(c);
+    if (!ch) break;
+    batch = std::min(2 * batch, kFfBatchMax);   // long fills: fewer host reads
+  }
+  launch(c, "ff_unpack", k_ffb_unpack, cg, dim3(256), 0, (const U128*)c->fb_reach.p, c->freach.p,
+         nx, ny, nz);
+}
+
 void flood_fill(spk_ctx* c, int m, const std::vector<int>& exits, int nx, int ny, int nz) {
   const size_t V = (size_t)nx * ny * nz;
   c->freach.ensure((size_t)m * V);
   c->exits.ensure((size_t)3 * m);
   upload(c, c->exits.p, exits.data(), exits.size() * sizeof(int));
+  if (c->ff_bits && nz <= 128) {
+    flood_fill_bits(c, m, nx, ny, nz);
+    return;
+  }
   const int nb = (int)std::max<size_t>(1, std::min<size_t>(2048, (V + 255) / 256));
   launch(c, "ff_seed", k_ff_seed, dim3(nb, m), dim3(256), 0, (const unsigned char*)c->ffree.p,
          c->freach.p, (const int*)c->exits.p, nx, ny, nz);
@@ -3477,6 +3520,10 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
     c->plan_mode = v;
     c->tw.release();   // re-plan at the next spk_tray_set
     c->valid = false;
+    return SPK_OK;
+  }
+  if (std::string(name) == "ff_bits") {
+    c->ff_bits = value != 0.0;
     return SPK_OK;
   }
   if (std::string(name) == "spec_inc") {

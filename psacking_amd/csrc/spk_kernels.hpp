@@ -3837,6 +3837,151 @@ k_ff_tile(const unsigned char* __restrict__ fr_all, unsigned char* __restrict__ 
 
 // start of a batch of flood-fill rounds: flags[0] = go (1), flags[1..k] = 0;
 // with init_act, every tile of act starts active
+// ---- flood fill on bit columns (trays with nz <= 128) ---------------------
+// The same fixpoint as k_ff_seed / k_ff_tile with each (x, y) column of an
+// offset grid held as a 128-bit mask: bit z of column (o, x, y) is offset
+// (x, y, z) of grid o.  Along z a column fills in one step by carry
+// arithmetic: for free bits m and seeds s (s within m),
+//     up(m, s) = (((s + m) ^ m) | s) & m
+// holds every bit of a run of m at or above the lowest seed of that run:
+// within a run [a, b] with seeds s1 < s2 < ..., m + s has bits a .. s1-1,
+// s2, s3, ... and b+1 (the carry of s1 runs to the end of the run and stops
+// at the clear bit b+1), so the xor with m leaves s1 .. b without s2, s3,
+// ..., which the or puts back; runs do not interact.  down is up on the
+// bit-reversed words; the union is every run of m that holds a seed.
+struct U128 {
+  unsigned long long lo, hi;
+};
+__device__ __forceinline__ U128 u_or(U128 a, U128 b) { return {a.lo | b.lo, a.hi | b.hi}; }
+__device__ __forceinline__ U128 u_and(U128 a, U128 b) { return {a.lo & b.lo, a.hi & b.hi}; }
+__device__ __forceinline__ bool u_eq(U128 a, U128 b) { return a.lo == b.lo && a.hi == b.hi; }
+__device__ __forceinline__ U128 u_rev(U128 a) { return {__brevll(a.hi), __brevll(a.lo)}; }
+__device__ __forceinline__ U128 fill_up(U128 m, U128 s) {
+  const unsigned long long lo = s.lo + m.lo;
+  const unsigned long long hi = s.hi + m.hi + (lo < s.lo ? 1ULL : 0ULL);
+  return {((lo ^ m.lo) | s.lo) & m.lo, ((hi ^ m.hi) | s.hi) & m.hi};
+}
+// every run of m that holds a bit of s (s need not lie within m)
+__device__ __forceinline__ U128 fill_runs(U128 m, U128 s) {
+  s = u_and(s, m);
+  return u_or(fill_up(m, s), u_rev(fill_up(u_rev(m), u_rev(s))));
+}
+
+// Free bits and seeds of every column of the m grids (one wavefront per
+// column: lanes read z = lane and z = 64 + lane, then ballot).  Seeds: the
+// free bits of the columns on an x or y face (x == 0 or the exit face x ==
+// exits[3o], the same for y) and bits z == 0 and z == exits[3o + 2] of the
+// others.  grid (ceil(nx ny / 4), m), 256 threads.
+__global__ void __launch_bounds__(256)
+k_ffb_pack(const unsigned char* __restrict__ fr, U128* __restrict__ fm, U128* __restrict__ rb,
+           const int* __restrict__ exits, int nx, int ny, int nz) {
+  const long long ncol = (long long)nx * ny;
+  const long long col = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, o = blockIdx.y;
+  if (col >= ncol) return;   // whole wavefronts
+  const unsigned char* f = fr + ((size_t)o * ncol + col) * nz;
+  const unsigned long long lo = __ballot(lane < nz && f[lane] != 0);
+  const unsigned long long hi = __ballot(64 + lane < nz && f[64 + lane] != 0);
+  if (lane != 0) return;
+  const int x = (int)(col / ny), y = (int)(col - (long long)x * ny);
+  const int ex = exits[3 * o], ey = exits[3 * o + 1], ez = exits[3 * o + 2];
+  const U128 m{lo, hi};
+  U128 sd{1ULL, 0ULL};
+  if (ez < 64) sd.lo |= 1ULL << ez;
+  else sd.hi |= 1ULL << (ez - 64);
+  const bool face = x == 0 || y == 0 || x == ex || y == ey;
+  fm[(size_t)o * ncol + col] = m;
+  rb[(size_t)o * ncol + col] = face ? m : u_and(m, sd);
+}
+
+// One round over 32 x 32-column tiles: load the tile's columns and a
+// one-column halo, then iterate to the tile's local fixpoint (reached =
+// runs of free bits holding a bit reached by the column or one of its four
+// x / y neighbours), write back.  prev / changed / act_prev / act_cur as in
+// k_ff_tile (a tile is worked when it or one of its four neighbouring tiles
+// changed in the previous round).  grid (xt * yt, m), 1024 threads.
+constexpr int FBT = 32;
+__global__ void __launch_bounds__(FBT * FBT)
+k_ffb_tile(const U128* __restrict__ fm_all, U128* __restrict__ rb_all, int nx, int ny, int xt,
+           int yt, const int* __restrict__ prev, int* __restrict__ changed,
+           const unsigned char* __restrict__ act_prev, unsigned char* __restrict__ act_cur) {
+  if (*prev == 0) return;
+  const int tid = threadIdx.x, o = blockIdx.y;
+  const int tx0 = blockIdx.x % xt, ty0 = blockIdx.x / xt;
+  const size_t tile = (size_t)o * xt * yt + blockIdx.x;
+  {
+    bool act = act_prev[tile] != 0;
+    act = act || (tx0 > 0 && act_prev[tile - 1]);
+    act = act || (tx0 + 1 < xt && act_prev[tile + 1]);
+    act = act || (ty0 > 0 && act_prev[tile - xt]);
+    act = act || (ty0 + 1 < yt && act_prev[tile + xt]);
+    if (!act) {
+      if (tid == 0) act_cur[tile] = 0;
+      return;
+    }
+  }
+  constexpr int H = FBT + 2;
+  __shared__ U128 sr[H * H];
+  __shared__ int gch;
+  const size_t ncol = (size_t)nx * ny;
+  const U128* fm = fm_all + (size_t)o * ncol;
+  U128* rb = rb_all + (size_t)o * ncol;
+  const int bx = tx0 * FBT, by = ty0 * FBT;
+  // lanes along y (consecutive y are consecutive columns in memory)
+  const int ly = tid % FBT, lx = tid / FBT;
+  const int X = bx + lx, Y = by + ly;
+  const bool inside = X < nx && Y < ny;
+  for (int e = tid; e < H * H; e += FBT * FBT) {
+    const int hy = e % H, hx = e / H;
+    const int gx = bx + hx - 1, gy = by + hy - 1;
+    U128 r{0ULL, 0ULL};
+    if (gx >= 0 && gy >= 0 && gx < nx && gy < ny) r = rb[(size_t)gx * ny + gy];
+    sr[e] = r;
+  }
+  const U128 m = inside ? fm[(size_t)X * ny + Y] : U128{0ULL, 0ULL};
+  if (tid == 0) gch = 0;
+  __syncthreads();
+  const int c = (lx + 1) * H + (ly + 1);
+  U128 r = sr[c];
+  const U128 r0 = r;
+  for (int it = 0; it < 4 * FBT * FBT; ++it) {
+    const U128 nb = u_or(u_or(sr[c - 1], sr[c + 1]), u_or(sr[c - H], sr[c + H]));
+    const U128 nr = fill_runs(m, u_or(r, nb));
+    const bool upd = !u_eq(nr, r);
+    __syncthreads();
+    if (upd) {
+      r = nr;
+      sr[c] = r;
+    }
+    if (!__syncthreads_or(upd)) break;
+  }
+  const bool mine = inside && !u_eq(r, r0);
+  if (mine) {
+    rb[(size_t)X * ny + Y] = r;
+    gch = 1;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    act_cur[tile] = gch ? 1 : 0;
+    if (gch) atomicOr(changed, 1);
+  }
+}
+
+// reached bits -> one byte per offset (the layout k_masked_argmin and the
+// interlocking mask read); grid (ceil(nx ny / 4), m), 256 threads
+__global__ void __launch_bounds__(256)
+k_ffb_unpack(const U128* __restrict__ rb, unsigned char* __restrict__ reached, int nx, int ny,
+             int nz) {
+  const long long ncol = (long long)nx * ny;
+  const long long col = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63, o = blockIdx.y;
+  if (col >= ncol) return;
+  const U128 r = rb[(size_t)o * ncol + col];
+  unsigned char* out = reached + ((size_t)o * ncol + col) * nz;
+  if (lane < nz) out[lane] = (unsigned char)((r.lo >> lane) & 1ULL);
+  if (64 + lane < nz) out[64 + lane] = (unsigned char)((r.hi >> lane) & 1ULL);
+}
+
 __global__ void k_ff_batch_init(int* __restrict__ flags, int k, unsigned char* __restrict__ act,
                                 long long n_act) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -82,7 +82,10 @@ def test_collision_grid(P):
 
 def test_interlocking_mask(P):
     rng = np.random.default_rng(4)
-    for N, I in [((12, 11, 10), (2, 2, 2)), ((20, 20, 20), (3, 1, 2)), ((9, 9, 9), (1, 1, 1))]:
+    # bit-column fill (z <= 128), z past 128 on the byte fill, and a tray
+    # wider than one 32 x 32-column tile of the bit fill
+    for N, I in [((12, 11, 10), (2, 2, 2)), ((20, 20, 20), (3, 1, 2)), ((9, 9, 9), (1, 1, 1)),
+                 ((7, 6, 140), (2, 2, 3)), ((70, 40, 9), (3, 2, 2))]:
         say("interlock", N, I)
         tray = (rng.random(N) < 0.3).astype(np.int32)
         obj = (rng.random(I) < 0.8).astype(np.int32)
@@ -388,10 +391,11 @@ def test_large_plane_tray_uses_split_plan(P):
     _pack_with_plan(P, w, 4, 0, "edt", "nonzero")
 
 
-def test_pack_many_uneven_trays(P):
+@pytest.mark.parametrize("ff_bits", [1, 0])
+def test_pack_many_uneven_trays(P, ff_bits):
     """Independent instances of different tray sizes packed concurrently,
-    interlocking-free check after each placement, each equal to its own
-    oracle run."""
+    interlocking-free check after each placement (flood fill on bit columns
+    or bytes), each equal to its own oracle run."""
     from psacking_amd import workloads as W
     from psacking_amd.rotations import orientation_maps
     ws = [W.blobs(seed=21, n=4, tray=(34, 30, 26)),
@@ -401,6 +405,7 @@ def test_pack_many_uneven_trays(P):
     sessions, lists, orders = [], [], []
     for w in ws:
         s = P.Session(0)
+        s.set_option("ff_bits", ff_bits)
         s.set_tray(np.zeros(w.tray_size, np.int32), "l1", "eq1", 0)
         vols = [int((it > 0).sum()) for it in w.items]
         sessions.append(s)
