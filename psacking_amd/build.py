@@ -25,6 +25,9 @@ HIPCC_FLAGS = [
     # packed FP32 instructions are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-value", "-Wno-unused-result", "-Wno-return-type-c-linkage",
+    # a non-void function that can end without a return is undefined
+    # behaviour the optimiser exploits: refused at build time
+    "-Werror=return-type", "-Wno-error=return-type-c-linkage",
 ]
 
 
@@ -67,7 +70,7 @@ def build_core() -> str:
     out = core_path()
     tmp = out + ".tmp"
     inc = sysconfig.get_paths()["include"]
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-Werror=return-type",
           f"-I{inc}", f"-I{pybind11.get_include()}", f"-I{os.path.join(ROOT, 'include')}",
           os.path.join(CSRC, "core_bindings.cpp"), "-o", tmp,
           f"-L{HERE}", "-lspk", "-Wl,-rpath,$ORIGIN"])

@@ -951,9 +951,9 @@ WgBest exact_search_one(spk_ctx* c, const HostItem& it, const OrientDesc& d, int
 // The same fixpoint on bit columns (nz <= 128, option "ff_bits"): pack the
 // free offsets and seeds into 128-bit column masks, rounds of k_ffb_tile over
 // 32 x 32-column tiles, then the reached bits back to one byte per offset.
-void flood_fill_bits(spk_ctx* c, int m, int nx, int ny, int nz) {
+void flood_fill_bits(spk_ctx* c, int m, int nx, int ny, int nz, bool bytes) {
   const size_t ncol = (size_t)nx * ny, V = ncol * nz;
-  c->freach.ensure((size_t)m * V);
+  if (bytes) c->freach.ensure((size_t)m * V);
   c->fb_free.ensure((size_t)m * ncol);
   c->fb_reach.ensure((size_t)m * ncol);
   const dim3 cg((unsigned)((ncol + 3) / 4), m);
@@ -981,18 +981,22 @@ void flood_fill_bits(spk_ctx* c, int m, int nx, int ny, int nz) {
     if (!ch) break;
     batch = std::min(2 * batch, kFfBatchMax);   // long fills: fewer host reads
   }
-  launch(c, "ff_unpack", k_ffb_unpack, cg, dim3(256), 0, (const U128*)c->fb_reach.p, c->freach.p,
-         nx, ny, nz);
+  if (bytes)
+    launch(c, "ff_unpack", k_ffb_unpack, cg, dim3(256), 0, (const U128*)c->fb_reach.p,
+           c->freach.p, nx, ny, nz);
 }
 
-void flood_fill(spk_ctx* c, int m, const std::vector<int>& exits, int nx, int ny, int nz) {
+// Returns true when the bit fill ran: then c->fb_reach holds the result, and
+// c->freach too only when bytes is set.
+bool flood_fill(spk_ctx* c, int m, const std::vector<int>& exits, int nx, int ny, int nz,
+                bool bytes = true) {
   const size_t V = (size_t)nx * ny * nz;
   c->freach.ensure((size_t)m * V);
   c->exits.ensure((size_t)3 * m);
   upload(c, c->exits.p, exits.data(), exits.size() * sizeof(int));
   if (c->ff_bits && nz <= 128) {
-    flood_fill_bits(c, m, nx, ny, nz);
-    return;
+    flood_fill_bits(c, m, nx, ny, nz, bytes);
+    return true;
   }
   const int nb = (int)std::max<size_t>(1, std::min<size_t>(2048, (V + 255) / 256));
   launch(c, "ff_seed", k_ff_seed, dim3(nb, m), dim3(256), 0, (const unsigned char*)c->ffree.p,
@@ -1020,6 +1024,7 @@ void flood_fill(spk_ctx* c, int m, const std::vector<int>& exits, int nx, int ny
     if (!ch) break;
     batch = std::min(2 * batch, kFfBatchMax);   // long fills: fewer host reads
   }
+  return false;
 }
 
 // IF mode: feasibility grids -> reachable set -> masked argmin records
@@ -1031,13 +1036,13 @@ void if_masked_select(spk_ctx* c, const std::vector<OrientDesc>& ods, const doub
     exits[3 * o + 1] = c->ny - 1 - ods[o].hy;
     exits[3 * o + 2] = c->nz - 1 - ods[o].hz;
   }
-  flood_fill(c, m, exits, c->nx, c->ny, c->nz);
+  const bool bits = flood_fill(c, m, exits, c->nx, c->ny, c->nz, false);
   // c->per holds each orientation's unmasked summary (residual, range flag)
   c->rec2.ensure((size_t)m * c->nx);
   launch(c, "masked_argmin", k_masked_argmin, dim3(c->nx, m), dim3(256), 0,
          (const unsigned char*)c->freach.p, (const long long*)c->pgrid.p,
          (const OrientDesc*)c->od.p, pen, (const WgBest*)c->per.p, c->rec2.p, c->nx, c->ny,
-         c->nz);
+         c->nz, bits ? (const U128*)c->fb_reach.p : (const U128*)nullptr);
 }
 
 // records per orientation of the score pass: slab plan one per x slice,

@@ -3997,7 +3997,9 @@ __global__ void __launch_bounds__(256)
 k_masked_argmin(const unsigned char* __restrict__ reached, const long long* __restrict__ pgrid,
                 const OrientDesc* __restrict__ od, const double* __restrict__ pen,
                 const WgBest* __restrict__ per_o, WgBest* __restrict__ rec, int nx, int ny,
-                int nz) {
+                int nz, const U128* __restrict__ rbits) {
+  // rbits (nullable): the reached offsets as the bit fill left them, read in
+  // place of the byte grid
   const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
   const OrientDesc d = od[o];
   if (x >= d.nvx) return;
@@ -4009,7 +4011,12 @@ k_masked_argmin(const unsigned char* __restrict__ reached, const long long* __re
   for (int e = tid; e < S; e += 256) {
     const int j = e / d.nvz, k = e - j * d.nvz;
     const int q = j * nz + k;
-    if (!reached[base + q]) continue;
+    if (rbits) {
+      const U128 r = rbits[((size_t)o * nx + x) * ny + j];
+      if (!(((k < 64 ? r.lo : r.hi) >> (k & 63)) & 1ULL)) continue;
+    } else if (!reached[base + q]) {
+      continue;
+    }
     const long long p = pgrid[base + q];
     const double sc = (double)p + pen[k];
     const int idx = x * ny * nz + q;
