@@ -1151,7 +1151,10 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
                     const int* ctl, PackOut* out = nullptr, int t = 0, int halt_not_found = 0,
                     bool clear_flags = false, bool decide = true, int* flags = nullptr,
                     const int* pre = nullptr, int* cnt = nullptr, const int* od_item = nullptr,
-                    int cap = 0) {
+                    int cap = 0, bool begin = false, int topk = 0, int* gate = nullptr) {
+  // begin: k_item_masks opens the item in place of k_feas_begin (control words
+  // and flags cleared, feasibility pass off), when it runs (coarse_begins);
+  // gate: k_coarse_decide also sets the learned loop's proposer gate
   spk_ctx::Coarse& q = c->co;
   if (!q.ready) return;
   int* fs = flags ? flags : q.fs.p;
@@ -1168,7 +1171,8 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
   if (max_exy <= kMaskCols) {
     launch(c, n_masks, k_item_masks, dim3(m), dim3(kMaskThreads), 0, (const int*)c->items.p,
            (const M128*)q.lines.p, od, m, q.cm.p, ccx, ccy, q.im.p, mx, my, q.rl.p, q.frl.p,
-           q.nrun.p, clear_flags ? fs : (int*)nullptr, ctl, pre);
+           q.nrun.p, clear_flags || begin ? fs : (int*)nullptr, (int*)ctl, pre,
+           begin ? c->fstate.p : (int*)nullptr, begin ? 1 : 0);
   } else {
     launch(c, "core_columns", k_core_columns, dim3((ccx * ccy + 3) / 4, m), dim3(256), 0,
            (const int*)c->items.p, od, m, q.cm.p, ccx, ccy, ctl);
@@ -1195,8 +1199,12 @@ void enqueue_coarse(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
   }
   if (decide)
     launch(c, "coarse_decide", k_coarse_decide, dim3(1), dim3(64), 0, (const int*)fs, m,
-         c->fstate.p, out, t, halt_not_found, (int*)ctl);
+         c->fstate.p, out, t, halt_not_found, (int*)ctl, topk, gate);
 }
+
+// whether enqueue_coarse's first kernel can open an item (begin): the
+// one-kernel item masks, used when every orientation's footprint fits its LDS
+bool coarse_begins(const spk_ctx* c, int max_exy) { return c->co.ready && max_exy <= kMaskCols; }
 
 
 using ScoreKernel = decltype(&k_score_yz<10>);
@@ -1612,13 +1620,17 @@ void enqueue_step(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e
   if (feas_mode < 0) feas_mode = c->feas_mode;
   const bool feas = feas_mode && !c->split;
   if (feas || item_bound || c->co.ready) {
-    launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p,
-           c->co.ready ? c->co.fs.p : (int*)nullptr, m, feas ? feas_mode : 0, c->ctl.p,
-           (const FieldStats*)c->stats.p, item_bound, c->pout.p, t);
+    // with the column test exact (no FFT feasibility pass after it) and no
+    // certificate, its first kernel opens the item in place of k_feas_begin
+    const bool begin = !known_feasible && !item_bound && coarse_begins(c, max_exy);
+    if (!begin)
+      launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p,
+             c->co.ready ? c->co.fs.p : (int*)nullptr, m, feas ? feas_mode : 0, c->ctl.p,
+             (const FieldStats*)c->stats.p, item_bound, c->pout.p, t);
     // orientations a column test already found feasible need no second one
     if (!known_feasible)
       enqueue_coarse(c, od, m, max_ex, max_exy, c->ctl.p, c->pout.p, t, halt_not_found, false,
-                     true, nullptr, pre);
+                     true, nullptr, pre, nullptr, nullptr, 0, begin);
     enqueue_item_spectra(c, od, m, max_ex, c->ctl.p);
     // the column test's flags are exact: no FFT feasibility pass after it
     if (feas && !c->co.ready) {
@@ -2426,13 +2438,16 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       const ItemPlan& pl = plans[t];
       if (pl.m == 0 || skip[t]) continue;   // nothing fits / fits nowhere: tray unchanged
       const OrientDesc* od_fit = c->od_all.p + pl.od_off;
-      launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, c->co.fs.p, pl.m, 0,
-             ctl, (const FieldStats*)nullptr, (const FieldStats*)nullptr, c->pout.p, t);
+      // the item opens in its first column-test kernel (else k_feas_begin),
+      // and the proposer gate is set with the column test's decision
+      const bool begin = coarse_begins(c, pl.max_exy);
+      if (!begin)
+        launch(c, "feas_begin", k_feas_begin, dim3(1), dim3(64), 0, c->fstate.p, c->co.fs.p, pl.m,
+               0, ctl, (const FieldStats*)nullptr, (const FieldStats*)nullptr, c->pout.p, t);
       enqueue_coarse(c, od_fit, pl.m, pl.max_ex, pl.max_exy, ctl, c->pout.p, t, 0, false, true,
-                     nullptr, pre[t]);
+                     nullptr, pre[t], nullptr, nullptr, 0, begin, topk,
+                     propose[t] ? c->learn_gate.p : (int*)nullptr);
       if (propose[t]) {
-        launch(c, "learn_gate", k_learn_gate, dim3(1), dim3(64), 0, (const int*)c->co.fs.p,
-               pl.m, topk, c->learn_gate.p, (const int*)ctl);
         nn->gate = c->learn_gate.p;
         timed_group(c, "proposer", [&]() {
           nn_tray_features(s, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, c->tfeat.p,

@@ -2719,9 +2719,23 @@ k_item_masks(const int* __restrict__ items, const M128* __restrict__ lines,
              const OrientDesc* __restrict__ od, int m,
              M128* __restrict__ cm, int ccx, int ccy, M128* __restrict__ im, int mx, int my,
              unsigned* __restrict__ runs, unsigned* __restrict__ fruns, int* __restrict__ nrun,
-             int* __restrict__ cfs, const int* __restrict__ ctl, const int* __restrict__ pre) {
-  if (search_off(ctl)) return;
+             int* __restrict__ cfs, int* __restrict__ ctl, const int* __restrict__ pre,
+             int* __restrict__ fs = nullptr, int begin = 0) {
+  // begin: the first kernel of a greedy-loop item (in place of k_feas_begin
+  // with the feasibility pass off): block 0 clears the item's control words
+  // and fs, and only a halt stops the kernel (the skip word still belongs to
+  // the previous item)
+  if (begin ? (ctl && ctl[CTL_HALT]) : search_off(ctl)) return;
   const int o = blockIdx.x, tid = threadIdx.x;
+  if (begin && o == 0) {
+    if (tid == 0) {
+      ctl[CTL_PRUNE] = 0;
+      ctl[CTL_COARSE_DONE] = 0;
+      ctl[CTL_SKIP_SEARCH] = 0;
+      ctl[CTL_FEAS] = 0;
+    }
+    for (int j = tid; j < kFeasHeader + m && fs; j += blockDim.x) fs[j] = 0;
+  }
   if (o >= m) return;
   if (cfs && tid == 0) {   // clears the column test's flags (cfs: null when already cleared)
     cfs[kFeasHeader + o] = 0;
@@ -3047,12 +3061,24 @@ k_column_test(const M128* __restrict__ cols, int cx, int cy, const M128* __restr
 // (halt_not_found), the loop stops with CTL_HALT = 2 as k_decide would.
 __global__ void k_coarse_decide(const int* __restrict__ cfs, int m, int* __restrict__ fs,
                                 PackOut* __restrict__ out, int t, int halt_not_found,
-                                int* __restrict__ ctl) {
-  if (search_off(ctl)) return;
+                                int* __restrict__ ctl, int topk = 0,
+                                int* __restrict__ gate = nullptr) {
+  // gate (nullable): the learned loop's proposer gate (k_learn_gate), set
+  // here: 1 (the proposer's kernels exit at once) unless the item is still to
+  // be searched and more than topk of its orientations are feasible
+  if (search_off(ctl)) {
+    if (gate && threadIdx.x == 0) *gate = 1;
+    return;
+  }
   // one wavefront: a lane per flag, then a ballot
   int v = 0;
   for (int o = threadIdx.x; o < m; o += 64) v |= cfs[kFeasHeader + o];
   const bool any = __any(v != 0);
+  if (gate) {   // m <= 64 here: one flag per lane
+    const int f = threadIdx.x < m ? cfs[kFeasHeader + threadIdx.x] : 0;
+    const int n = __popcll(__ballot(f != 0));
+    if (threadIdx.x == 0) *gate = n <= topk ? 1 : 0;
+  }
   if (threadIdx.x != 0) return;
   ctl[CTL_PRUNE] = any ? 1 : 0;
   ctl[CTL_COARSE_DONE] = any ? 0 : 1;
