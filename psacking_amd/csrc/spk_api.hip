@@ -191,6 +191,9 @@ struct spk_ctx {
   float resid_limit = 0.25f;             // largest accepted FFT rounding residual
   // learned proposals and training records
   DevBuf<int> heights;                   // column heights [nx][ny]
+  DevBuf<spknn::bf16_t> icat;            // learned loop: head rows [item | tray] per item slot
+  bool tray_nonneg = true;               // no tray voxel is negative (the tray features can
+                                         // then come from the column-test masks)
   DevBuf<int> map_all;                   // orientation index of each od_all entry
   DevBuf<unsigned short> tfeat;          // tray features [64][32][32][2] (bf16)
   DevBuf<unsigned short> ifeat, ipool;   // oriented item features, pooled rows [n][24][384]
@@ -1985,10 +1988,12 @@ void timed_group(spk_ctx* c, const char* name, F&& f) {
 // n_orient repeat the base item; their logits are unused).
 // which (optional): only these items (positions in processing order), whose
 // rows go to consecutive slots of c->ipool in the order listed.
+// (cat: the pooled rows go to c->icat at the head's row pitch kNnConcat, whose
+// tray parts nn_record_rows fills per evaluation; else to c->ipool)
 void learned_item_pool(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
                        const std::vector<int>& off, const int32_t* order,
                        const spk_orient* orients, int n_orient,
-                       const std::vector<int>* which = nullptr) {
+                       const std::vector<int>* which = nullptr, bool cat = false) {
   const int n_items = which ? (int)which->size() : (int)bases.size();
   if (n_items == 0) return;
   hipStream_t s = c->stream;
@@ -2014,13 +2019,14 @@ void learned_item_pool(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& base
     upload(c, c->imaps.p, maps.data(), maps.size() * sizeof(spknn::ItemMap));
     const int bmax = nn->max_batch;
     c->ifeat.ensure((size_t)std::min(n_items, bmax) * kNnLogits * kNnItemFeat);
-    c->ipool.ensure((size_t)n_items * kNnLogits * kNnItemPooled);
+    const int ld = cat ? kNnConcat : kNnItemPooled;
+    DevBuf<spknn::bf16_t>& pool = cat ? c->icat : c->ipool;
+    pool.ensure((size_t)n_items * kNnLogits * ld);
     for (int b0 = 0; b0 < n_items; b0 += bmax) {
       const int nb = std::min(bmax, n_items - b0);
       nn_item_features(s, c->items.p, c->imaps.p + (size_t)b0 * kNnLogits, nb * kNnLogits,
                        c->ifeat.p);
-      nn_item_branch(nn, s, c->ifeat.p, nb * kNnLogits,
-                     c->ipool.p + (size_t)b0 * kNnLogits * kNnItemPooled);
+      nn_item_branch(nn, s, c->ifeat.p, nb * kNnLogits, pool.p + (size_t)b0 * kNnLogits * ld, ld);
     }
   });
 }
@@ -2407,6 +2413,9 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   // slots of c->ipool (slot[t]).
   std::vector<char> skip(n_items, 0), propose(n_items, 1);
   std::vector<int> slot(n_items, 0);
+  // tray features from the column test's masks (they mark voxels != 0, the
+  // features count voxels > 0: equal while no voxel is negative)
+  const bool feat_masks = c->tray_nonneg && id_offset >= 0 && c->co.ready;
   std::vector<const int*> pre(n_items, nullptr);
   int spec_n = 0;   // incremental spectrum updates in a row, carried across rounds
   while (t0 < n_items) {
@@ -2427,7 +2436,7 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
           slot[t] = (int)need.size();
           need.push_back(t);
         }
-      learned_item_pool(c, nn, bases, off, order, orients, n_orient, &need);
+      learned_item_pool(c, nn, bases, off, order, orients, n_orient, &need, true);
     }
     int ctl0[CTL_WORDS] = {0, -1, 0, placed};
     ctl0[CTL_SPEC_N] = spec_n;
@@ -2450,10 +2459,13 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       if (propose[t]) {
         nn->gate = c->learn_gate.p;
         timed_group(c, "proposer", [&]() {
-          nn_tray_features(s, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, c->tfeat.p,
-                           c->learn_gate.p);
-          nn_tray_branch(nn, s, c->tfeat.p, 1);
-          nn_head(nn, s, c->ipool.p + (size_t)slot[t] * kNnLogits * kNnItemPooled, 1);
+          if (feat_masks)
+            nn_tray_features_masks(s, c->co.tf.p, c->nx, c->ny, c->nz, c->tfeat.p,
+                                   c->learn_gate.p);
+          else
+            nn_tray_features(s, c->tray.p, c->nx, c->ny, c->nz, c->heights.p, c->tfeat.p,
+                             c->learn_gate.p);
+          nn_record_rows(nn, s, c->tfeat.p, c->icat.p + (size_t)slot[t] * kNnLogits * kNnConcat);
         });
         nn->gate = nullptr;
       }
@@ -2963,6 +2975,11 @@ int spk_tray_set(spk_ctx* c, const int32_t* tray, int nx, int ny, int nz, const 
     c->tmpi.ensure(V);
     c->stats.ensure(1);
     upload(c, c->tray.p, tray, V * sizeof(int));
+    {
+      int mn = 0;
+      for (size_t i = 0; i < V; ++i) mn = std::min(mn, tray[i]);
+      c->tray_nonneg = mn >= 0;
+    }
     refresh_field_and_spectra(c, dist);
     if (c->tile_force) {
       if ((long long)c->px.n * c->tile_force > kPencilMax)
@@ -3108,6 +3125,7 @@ int spk_place(spk_ctx* c, const int32_t* item, int ix, int iy, int iz, const spk
     upload(c, c->items.p, item, sz * sizeof(int));
     OrientDesc d = make_desc(c, it, 0, orient);
     do_place(c, d, x, y, z, id, true);
+    if (id < 0) c->tray_nonneg = false;
     sync(c);
     if (c->timing) c->flush_timing();
     return SPK_OK;
@@ -3124,6 +3142,7 @@ int pack_impl(spk_ctx* c, const int32_t* items, const int32_t* const* ptrs, cons
     std::vector<HostItem> bases;
     std::vector<int> off;
     load_items(c, items, ptrs, dims, n_items, order, bases, off);
+    if (id_offset < 0) c->tray_nonneg = false;   // placed ids may be negative
     int placed = 0;
     if (interlocking_free) {
       // the flood fill iterates to its fixpoint under host control: one
@@ -3966,6 +3985,7 @@ int spk_pack_learned(spk_ctx* c, spk_nn* nn, const spk_pack_args* a, int topk, f
     std::vector<HostItem> bases;
     std::vector<int> off;
     load_items(c, a->items, a->item_ptrs, a->dims, a->n_items, a->order, bases, off);
+    if (a->id_offset < 0) c->tray_nonneg = false;   // placed ids may be negative
     // the device-driven loop covers the default configuration: the column
     // test screens every item, no confidence threshold, no certificate
     const bool dev = c->learn_dev && c->co.ready && c->screen_all && c->feas_mode != 0 &&

@@ -240,9 +240,11 @@ k_igemm(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wm, const float
 }
 
 // epilogue of a split-K forward: Y = relu(sum_z part[z] + bias)
+// (Yc, nullable: column 0 of every row also at Yc[row], the scorer's logits)
 __global__ void k_split_finalize(const float* __restrict__ part, int ks, int M, int N,
                                  const float* __restrict__ bias, int relu, bf16_t* __restrict__ Y,
-                                 float* __restrict__ Yf, const int* __restrict__ gate) {
+                                 float* __restrict__ Yf, const int* __restrict__ gate,
+                                 float* __restrict__ Yc = nullptr) {
   if (gate && *gate) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= M * N) return;
@@ -252,6 +254,7 @@ __global__ void k_split_finalize(const float* __restrict__ part, int ks, int M, 
   if (relu) v = fmaxf(v, 0.f);
   if (Yf) Yf[i] = v;
   if (Y) Y[i] = f2bf(v);
+  if (Yc && i % N == 0) Yc[i / N] = v;
 }
 
 // Weight gradient partial sums over row chunks.  Block (kb, chunk) computes
@@ -349,9 +352,11 @@ __global__ void k_reduce_parts(const float* __restrict__ part, int nchunk, int c
 }
 
 // 2x2x2 average pool of X [B][d0][d1][d2][C] (even sizes) into rows of a wider
-// buffer: out[b*ld + off + ((h0*h1n + h1)*h2n + h2)*C + c]
+// buffer: out[b*ld + off + ((h0*h1n + h1)*h2n + h2)*C + c], each row written to
+// rep consecutive rows (b*rep .. b*rep + rep-1) when rep > 1
 __global__ void k_pool_fwd(const bf16_t* __restrict__ X, int B, int d0, int d1, int d2, int C,
-                           bf16_t* __restrict__ out, int ld, int off, const int* __restrict__ gate) {
+                           bf16_t* __restrict__ out, int ld, int off, const int* __restrict__ gate,
+                           int rep = 1) {
   if (gate && *gate) return;
   const int h0n = d0 / 2, h1n = d1 / 2, h2n = d2 / 2;
   const int per = h0n * h1n * h2n * C;
@@ -368,7 +373,8 @@ __global__ void k_pool_fwd(const bf16_t* __restrict__ X, int B, int d0, int d1, 
     const int x0 = 2 * h0 + (e >> 2), x1 = 2 * h1 + ((e >> 1) & 1), x2 = 2 * h2 + (e & 1);
     s += bf2f(X[((((size_t)b * d0 + x0) * d1 + x1) * d2 + x2) * C + c]);
   }
-  out[(size_t)b * ld + off + r * C + c] = f2bf(s * 0.125f);
+  const bf16_t v = f2bf(s * 0.125f);
+  for (int q = 0; q < rep; ++q) out[((size_t)b * rep + q) * ld + off + r * C + c] = v;
 }
 
 // gradient of the pool, masked by the pooled activation (post-ReLU)
@@ -600,6 +606,49 @@ __global__ void k_tray_feat(const int* __restrict__ tray, const int* __restrict_
         occ += col[z] > 0;
         below += z < hh;
       }
+    }
+  const float cnt = (float)((x1 - x0) * (y1 - y0) * (z1 - z0));
+  out[2 * i] = f2bf((float)occ / cnt);
+  out[2 * i + 1] = f2bf((float)below / cnt);
+}
+
+// k_heightmap and k_tray_feat in one kernel, from the tray's 128-bit column
+// masks (bit z of tf[x ny + y] set when voxel (x, y, z) is nonzero; nz <= 128)
+// instead of the int32 grid: the height of a column is one past its highest
+// set bit and the occupied voxels of a cell are a popcount.  Equal to the two
+// kernels when no tray voxel is negative (they count voxels > 0).
+__device__ __forceinline__ int popc_range(unsigned long long lo, unsigned long long hi, int z0,
+                                          int z1) {
+  // set bits of (lo, hi) in [z0, z1), 0 <= z0 < z1 <= 128
+  auto mask64 = [](int a, int b) -> unsigned long long {   // bits [a, b) of one word, 0..64
+    if (b <= a) return 0ULL;
+    const unsigned long long hi_m = b >= 64 ? ~0ULL : ((1ULL << b) - 1ULL);
+    const unsigned long long lo_m = a >= 64 ? ~0ULL : ((1ULL << a) - 1ULL);
+    return hi_m & ~lo_m;
+  };
+  const unsigned long long ml = mask64(min(z0, 64), min(z1, 64));
+  const unsigned long long mh = mask64(max(z0 - 64, 0), max(z1 - 64, 0));
+  return __popcll(lo & ml) + __popcll(hi & mh);
+}
+__global__ void __launch_bounds__(256)
+k_tray_feat_m(const unsigned long long* __restrict__ tf, int nx, int ny, int nz,
+              bf16_t* __restrict__ out, const int* __restrict__ gate) {
+  if (gate && *gate) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kTray0 * kTray1 * kTray2) return;
+  const int c = i % kTray2, b = (i / kTray2) % kTray1, a = i / (kTray2 * kTray1);
+  int x0, x1, y0, y1, z0, z1;
+  cell_range(a, nx, kTray0, x0, x1);
+  cell_range(b, ny, kTray1, y0, y1);
+  cell_range(c, nz, kTray2, z0, z1);
+  int occ = 0, below = 0;
+  for (int x = x0; x < x1; ++x)
+    for (int y = y0; y < y1; ++y) {
+      const size_t q = (size_t)x * ny + y;
+      const unsigned long long lo = tf[2 * q], hi = tf[2 * q + 1];
+      const int hh = hi ? 128 - __clzll((long long)hi) : (lo ? 64 - __clzll((long long)lo) : 0);
+      occ += popc_range(lo, hi, z0, z1);
+      below += max(0, min(hh, z1) - z0);
     }
   const float cnt = (float)((x1 - x0) * (y1 - y0) * (z1 - z0));
   out[2 * i] = f2bf((float)occ / cnt);

@@ -159,9 +159,11 @@ void nn_cast(spk_nn* nn, hipStream_t s) {
 
 int nt_for(int n) { return n <= 16 ? 1 : n <= 32 ? 2 : n <= 48 ? 3 : 4; }
 
-// forward of one layer over a batch of B rows of inputs
-void nn_fwd(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* X, bf16_t* Y,
-            float* Yf) {
+// forward of one layer over a batch of B rows of inputs; Yc (nullable): column
+// 0 of every output row also written compactly to Yc[row] by the split-K
+// epilogue; returns false when there was none (no split: Yc is not written)
+bool nn_fwd(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* X, bf16_t* Y,
+            float* Yf, float* Yc = nullptr) {
   ConvDesc d = l.d;
   d.B = B;
   const int M = B * d.o0 * d.o1 * d.o2, N = d.cout, NT = nt_for(N);
@@ -202,7 +204,8 @@ void nn_fwd(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* X,
 #undef SPK_NN_FWD
   if (part)
     nn_launch(s, spknn::k_split_finalize, dim3((M * N + 255) / 256), dim3(256), 0,
-              (const float*)part, ks, M, N, bias, l.relu, Y, Yf, nn->gate);
+              (const float*)part, ks, M, N, bias, l.relu, Y, Yf, nn->gate, Yc);
+  return part != nullptr;
 }
 
 // input gradient of one layer: dX = (dY * W) masked by the layer input > 0
@@ -247,11 +250,11 @@ void nn_wgrad(spk_nn* nn, hipStream_t s, const NnLayer& l, int B, const bf16_t* 
 }
 
 void nn_pool_fwd(hipStream_t s, const NnLayer& l, int B, const bf16_t* X, bf16_t* out, int ld,
-                 int off, const int* gate) {
+                 int off, const int* gate, int rep = 1) {
   const ConvDesc& d = l.d;
   const int n = B * (d.o0 / 2) * (d.o1 / 2) * (d.o2 / 2) * d.cout;
   nn_launch(s, spknn::k_pool_fwd, dim3((n + 255) / 256), dim3(256), 0, X, B, d.o0, d.o1, d.o2,
-            d.cout, out, ld, off, gate);
+            d.cout, out, ld, off, gate, rep);
 }
 
 void nn_pool_bwd(hipStream_t s, const NnLayer& l, int B, const bf16_t* g, int ld, int off,
@@ -262,12 +265,13 @@ void nn_pool_bwd(hipStream_t s, const NnLayer& l, int B, const bf16_t* g, int ld
             B, d.o0, d.o1, d.o2, d.cout, dX);
 }
 
-// item branch: n oriented item grids -> pooled rows (pitch 384) in out
-void nn_item_branch(spk_nn* nn, hipStream_t s, const bf16_t* item, int n, bf16_t* out) {
+// item branch: n oriented item grids -> pooled rows (pitch ld, default 384) in out
+void nn_item_branch(spk_nn* nn, hipStream_t s, const bf16_t* item, int n, bf16_t* out,
+                    int ld = kNnItemPooled) {
   nn_fwd(nn, s, nn->L[NN_I1], n, item, nn->act[NN_I1].p, nullptr);
   nn_fwd(nn, s, nn->L[NN_I2], n, nn->act[NN_I1].p, nn->act[NN_I2].p, nullptr);
   nn_fwd(nn, s, nn->L[NN_I3], n, nn->act[NN_I2].p, nn->act[NN_I3].p, nullptr);
-  nn_pool_fwd(s, nn->L[NN_I3], n, nn->act[NN_I3].p, out, kNnItemPooled, 0, nn->gate);
+  nn_pool_fwd(s, nn->L[NN_I3], n, nn->act[NN_I3].p, out, ld, 0, nn->gate);
 }
 
 // tray branch: B tray grids -> pooled rows (pitch 768) in nn->tpool
@@ -276,6 +280,24 @@ void nn_tray_branch(spk_nn* nn, hipStream_t s, const bf16_t* tray, int B) {
   nn_fwd(nn, s, nn->L[NN_T2], B, nn->act[NN_T1].p, nn->act[NN_T2].p, nullptr);
   nn_fwd(nn, s, nn->L[NN_T3], B, nn->act[NN_T2].p, nn->act[NN_T3].p, nullptr);
   nn_pool_fwd(s, nn->L[NN_T3], B, nn->act[NN_T3].p, nn->tpool.p, kNnTrayPooled, 0, nn->gate);
+}
+
+// One record's head input built in place: rows [item row r | tray row] of a
+// buffer of kNnLogits rows of pitch kNnConcat whose item parts hold the
+// record's pooled item rows (nn_item_branch with that pitch): the tray branch
+// pools into the tray part of every row, then F1 and F2 read the rows and the
+// logits come out of F2's split-K epilogue.
+void nn_record_rows(spk_nn* nn, hipStream_t s, const bf16_t* tray, bf16_t* rows) {
+  nn_fwd(nn, s, nn->L[NN_T1], 1, tray, nn->act[NN_T1].p, nullptr);
+  nn_fwd(nn, s, nn->L[NN_T2], 1, nn->act[NN_T1].p, nn->act[NN_T2].p, nullptr);
+  nn_fwd(nn, s, nn->L[NN_T3], 1, nn->act[NN_T2].p, nn->act[NN_T3].p, nullptr);
+  nn_pool_fwd(s, nn->L[NN_T3], 1, nn->act[NN_T3].p, rows, kNnConcat, kNnItemPooled, nn->gate,
+              kNnLogits);
+  nn_fwd(nn, s, nn->L[NN_F1], kNnLogits, rows, nn->act[NN_F1].p, nullptr);
+  if (!nn_fwd(nn, s, nn->L[NN_F2], kNnLogits, nn->act[NN_F1].p, nn->act[NN_F2].p, nn->out.p,
+              nn->logits.p))
+    nn_launch(s, spknn::k_logits_compact, dim3(1), dim3(256), 0, (const float*)nn->out.p, kNnOut,
+              kNnLogits, nn->logits.p, nn->gate);
 }
 
 // head on B records: rows [item row r | tray row r/24] -> F1 -> F2 -> logits
@@ -351,6 +373,15 @@ void nn_tray_features(hipStream_t s, const int* tray, int nx, int ny, int nz, in
             heights, gate);   // a wavefront per column
   nn_launch(s, spknn::k_tray_feat, dim3(kNnTrayFeat / 2 / 256), dim3(256), 0, tray,
             (const int*)heights, nx, ny, nz, out, gate);
+}
+
+// the same features from the tray's 128-bit column masks (level 0 of the
+// column test's full-resolution masks), one kernel; equal when no tray voxel
+// is negative
+void nn_tray_features_masks(hipStream_t s, const void* masks, int nx, int ny, int nz, bf16_t* out,
+                            const int* gate = nullptr) {
+  nn_launch(s, spknn::k_tray_feat_m, dim3(kNnTrayFeat / 2 / 256), dim3(256), 0,
+            (const unsigned long long*)masks, nx, ny, nz, out, gate);
 }
 
 template <typename F>

@@ -318,3 +318,29 @@ def test_trained_proposer_beats_untrained(P, PR):
     say("top-4 recall", recall)
     assert recall["trained"] >= recall["untrained"] + 0.1
 
+
+
+def test_learned_device_and_host_loops_agree(P, PR):
+    """The device-driven learned loop (tray features from the column-test
+    masks, head rows built in place, the pick as a kernel) proposes and
+    places exactly as the host-driven loop (tray features from the int32
+    tray, the pick on the host), item for item."""
+    from psacking_amd.rotations import orientation_maps
+    w, order = _workload(53, 12, (64, 48, 40))
+    perms, flips = orientation_maps(24)
+    nn = PR.Proposer(0, 8, PR.init_params(54))
+    out = []
+    for dev in (1, 0):
+        s = P.Session(0)
+        s.set_option("learn_dev", dev)
+        s.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
+        say("pack learned, learn_dev", dev)
+        res, props, counts = PR.pack_learned(s, nn, w.items, order, perms, flips, "cube", topk=4)
+        out.append((res, props, s.tray()))
+    (r1, p1, t1), (r0, p0, t0) = out
+    assert (p1 >= 0).any()
+    np.testing.assert_array_equal(p1, p0)
+    for a, b in zip(r1, r0):
+        assert (a["found"], tuple(a["position"]), a["orientation"], a["score"]) == \
+            (b["found"], tuple(b["position"]), b["orientation"], b["score"])
+    np.testing.assert_array_equal(t1, t0)
