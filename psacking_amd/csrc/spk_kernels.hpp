@@ -689,14 +689,22 @@ k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict
     const int xl = b.qx, xh = b.qx + m - 1;
     const int share = (nx + kIncSplit - 1) / kIncSplit;
     const int xa = blockIdx.z * share, xb = min(nx, xa + share);
+    // the smallest candidate of the line: every candidate at distance >= d
+    // from x costs at least fmin + dcost(d)
+    int fmin = big;
+    for (int rr = 0; rr < m; ++rr) fmin = min(fmin, cand[rr * kIncLines + lane]);
 #pragma unroll 4
     for (int x = xa + wave; x < xb; x += 4) {
       int* pv = phi + ((size_t)x * ny + y) * nz + z;
       const int old = *pv;
       int v = old;
       const int dx = x < xl ? xl - x : (x > xh ? x - xh : 0);
-      // the new field at x is at least dx (its distance to the item's slabs)
-      if (dx < old) {
+      // the new value can only be below old when the cheapest conceivable
+      // candidate (fmin at the nearest slab) is: for EDT the rounded root of
+      // s is >= old exactly when s >= old^2 - old + 1 (old >= 1)
+      const long long lb = fmin >= big ? (long long)big : (long long)fmin + dcost(dx, EDT);
+      const bool may = EDT ? (old > 0 && lb < (long long)old * old - old + 1) : lb < old;
+      if (may) {
         int best = big;
         for (int rr = 0; rr < m; ++rr) {
           const int f = cand[rr * kIncLines + lane];
