@@ -273,6 +273,8 @@ struct spk_ctx {
   unsigned mail_seq = 0;                     // sequence number of the last post
   int* stage = nullptr;                      // pinned staging buffer of the items
   size_t stage_cap = 0;
+  int* pre_h = nullptr;                      // pinned: batched screen's aux upload and flags
+  size_t pre_h_cap = 0;
   hipEvent_t stage_ev = nullptr;             // recorded after its last copy
   // host-side copy of item buffer bookkeeping
   std::vector<int> h_items;
@@ -1778,12 +1780,24 @@ int place_exact(spk_ctx* c, const ExactBest& eb, const ItemPlan& pl,
 // per-item column tests of the round (pre_flags): 1 feasible, 0 infeasible,
 // 2 untested.  An item stops being tested once cap of its orientations are
 // proven feasible (the count returned is then at least cap).
-std::vector<int> prescreen(spk_ctx* c, const std::vector<ItemPlan>& plans, int t_begin,
-                           int t_end, int* od0_out, int cap) {
-  std::vector<int> live((size_t)std::max(0, t_end - t_begin), -1);
-  *od0_out = 0;
+// The batched column test of a round, in two halves so that it can be
+// enqueued behind the previous round's items and read with that round's
+// outcomes (one host synchronisation per round): screen_enqueue uploads the
+// per-item bookkeeping and enqueues the test and the copy of its flags into
+// pinned memory; screen_finish, after the caller has synchronised the
+// stream, turns the flags into live[t - t_begin] = feasible orientations of
+// item t proven by the test (-1 when it did not run).
+struct Screen {
+  int t_begin = 0, t_end = 0, od0 = 0, m = 0;
+  bool ran = false;
+};
+Screen screen_enqueue(spk_ctx* c, const std::vector<ItemPlan>& plans, int t_begin, int t_end,
+                      int cap) {
+  Screen sc;
+  sc.t_begin = t_begin;
+  sc.t_end = t_end;
   spk_ctx::Coarse& q = c->co;
-  if (!q.ready || !q.on || t_end <= t_begin) return live;
+  if (!q.ready || !q.on || t_end <= t_begin) return sc;
   int od0 = -1, od1 = 0, max_ex = 1, max_exy = 1;
   for (int t = t_begin; t < t_end; ++t) {
     const ItemPlan& pl = plans[t];
@@ -1793,35 +1807,94 @@ std::vector<int> prescreen(spk_ctx* c, const std::vector<ItemPlan>& plans, int t
     max_ex = std::max(max_ex, pl.max_ex);
     max_exy = std::max(max_exy, pl.max_exy);
   }
-  if (od0 < 0) return live;
-  *od0_out = od0;
+  if (od0 < 0) return sc;
+  sc.od0 = od0;
   const int m = od1 - od0;   // the fitting orientations of a range are contiguous
+  sc.m = m;
   const int ni = t_end - t_begin;
-  // per-item feasible counts (zero), then each orientation's item
-  std::vector<int> aux((size_t)ni + m, 0);
+  // pinned: per-item feasible counts (zero), each orientation's item, then
+  // the flags coming back
+  const size_t need = (size_t)ni + 2 * (size_t)m;
+  if (c->pre_h_cap < need) {
+    if (c->pre_h) HIP_OK(hipHostFree(c->pre_h));
+    c->pre_h = nullptr;
+    c->pre_h_cap = 0;
+    HIP_OK(hipHostMalloc((void**)&c->pre_h, need * sizeof(int)));
+    c->pre_h_cap = need;
+  }
+  int* aux = c->pre_h;
+  std::fill(aux, aux + ni + m, 0);
   for (int t = t_begin; t < t_end; ++t)
     for (int i = 0; i < plans[t].m; ++i) aux[(size_t)ni + plans[t].od_off - od0 + i] = t - t_begin;
-  c->pre_aux.ensure(aux.size());
-  upload(c, c->pre_aux.p, aux.data(), aux.size() * sizeof(int));
+  c->pre_aux.ensure((size_t)ni + m);
+  HIP_OK(hipMemcpyAsync(c->pre_aux.p, aux, ((size_t)ni + m) * sizeof(int), hipMemcpyHostToDevice,
+                        c->stream));
   c->pre_fs.ensure((size_t)kFeasHeader + m);
-  HIP_OK(hipMemsetAsync(c->pre_fs.p, 0, ((size_t)kFeasHeader + m) * sizeof(int), c->stream));
+  launch(c, "pre_clear", k_fill_int, dim3(1), dim3(256), 0, c->pre_fs.p, kFeasHeader + m, 0);
   enqueue_coarse(c, c->od_all.p + od0, m, max_ex, max_exy, nullptr, nullptr, 0, 0, false, false,
                  c->pre_fs.p, nullptr, c->pre_aux.p, c->pre_aux.p + ni, std::max(1, cap));
-  std::vector<int> f((size_t)m);
-  download(c, f.data(), c->pre_fs.p + kFeasHeader, (size_t)m * sizeof(int));
-  sync(c);
-  for (int t = t_begin; t < t_end; ++t) {
+  HIP_OK(hipMemcpyAsync(c->pre_h + ni + m, c->pre_fs.p + kFeasHeader, (size_t)m * sizeof(int),
+                        hipMemcpyDeviceToHost, c->stream));
+  sc.ran = true;
+  return sc;
+}
+std::vector<int> screen_finish(const spk_ctx* c, const std::vector<ItemPlan>& plans,
+                               const Screen& sc) {
+  std::vector<int> live((size_t)std::max(0, sc.t_end - sc.t_begin), -1);
+  if (!sc.ran) return live;
+  const int ni = sc.t_end - sc.t_begin;
+  const int* f = c->pre_h + ni + sc.m;
+  for (int t = sc.t_begin; t < sc.t_end; ++t) {
     const ItemPlan& pl = plans[t];
     if (pl.m == 0) continue;
     int n = 0;
-    for (int i = 0; i < pl.m; ++i) n += f[(size_t)(pl.od_off - od0 + i)] != 0 ? 1 : 0;
-    live[t - t_begin] = n;
+    for (int i = 0; i < pl.m; ++i) n += f[(size_t)(pl.od_off - sc.od0 + i)] != 0 ? 1 : 0;
+    live[t - sc.t_begin] = n;
   }
   return live;
 }
+// both halves with a synchronisation between them
+std::vector<int> prescreen(spk_ctx* c, const std::vector<ItemPlan>& plans, int t_begin,
+                           int t_end, int* od0_out, int cap) {
+  const Screen sc = screen_enqueue(c, plans, t_begin, t_end, cap);
+  *od0_out = sc.od0;
+  if (sc.ran) sync(c);
+  return screen_finish(c, plans, sc);
+}
+// Buffers of the batched screens of every round sized up front, so that a
+// screen enqueued behind a round's items never reallocates one in use.
+void presize_screens(spk_ctx* c, const std::vector<ItemPlan>& plans, int n_items, int round) {
+  spk_ctx::Coarse& q = c->co;
+  if (!q.ready || round <= 0) return;
+  int mmax = 1, exmax = 1, nimax = 1;
+  for (int t0 = 0; t0 < n_items; t0 += round) {
+    const int t1 = std::min(n_items, t0 + round);
+    int m = 0;
+    for (int t = t0; t < t1; ++t) {
+      m += plans[t].m;
+      exmax = std::max(exmax, plans[t].max_ex);
+    }
+    mmax = std::max(mmax, m);
+    nimax = std::max(nimax, t1 - t0);
+  }
+  const int ccx = (exmax + 1) / 2;
+  q.cm.ensure((size_t)mmax * ccx * q.cy);
+  q.im.ensure((size_t)mmax * exmax * c->ny);
+  q.rl.ensure((size_t)mmax * kColRuns);
+  q.frl.ensure((size_t)mmax * kFineRuns);
+  q.nrun.ensure(2 * (size_t)mmax);
+  c->pre_aux.ensure((size_t)nimax + mmax);
+  c->pre_fs.ensure((size_t)kFeasHeader + mmax);
+  const size_t need = (size_t)nimax + 2 * (size_t)mmax;
+  if (c->pre_h_cap < need) {
+    if (c->pre_h) HIP_OK(hipHostFree(c->pre_h));
+    c->pre_h = nullptr;
+    c->pre_h_cap = 0;
+    HIP_OK(hipHostMalloc((void**)&c->pre_h, need * sizeof(int)));
+    c->pre_h_cap = need;
+  }
+}
 
-// item t's flags from its round's batched screen, or nullptr when it was not
-// screened (the column test then tests every fitting orientation)
 const int* pre_flags(const spk_ctx* c, const std::vector<ItemPlan>& plans, int t, int live,
                      int od0) {
   return live > 0 ? c->pre_fs.p + kFeasHeader + (plans[t].od_off - od0) : nullptr;
@@ -1873,11 +1946,23 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
   std::vector<char> skip(n_items, 0);
   std::vector<const int*> pre(n_items, nullptr);
   int spec_n = 0;   // incremental spectrum updates in a row, carried across rounds
+  // the next round's screen, enqueued behind a round's items and read with
+  // its outcomes (void when the loop halts: the host then changes the tray)
+  Screen next;
+  bool have_next = false;
+  if (rounds) presize_screens(c, plans, n_items, c->prescreen);
   while (t0 < n_items) {
     const int t_end = rounds ? std::min(n_items, t0 + c->prescreen) : n_items;
     if (rounds) {
       int od0 = 0;
-      const std::vector<int> live = prescreen(c, plans, t0, t_end, &od0, 1);
+      std::vector<int> live;
+      if (have_next && next.t_begin == t0 && next.t_end == t_end) {
+        live = screen_finish(c, plans, next);
+        od0 = next.od0;
+      } else {
+        live = prescreen(c, plans, t0, t_end, &od0, 1);
+      }
+      have_next = false;
       for (int t = t0; t < t_end; ++t) {
         skip[t] = live[t - t0] == 0 ? 1 : 0;
         pre[t] = pre_flags(c, plans, t, live[t - t0], od0);
@@ -1900,6 +1985,10 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
                    pl.max_dim, pre[t]);
     }
     lg.run();
+    if (rounds && t_end < n_items) {
+      next = screen_enqueue(c, plans, t_end, std::min(n_items, t_end + c->prescreen), 1);
+      have_next = next.ran;
+    }
     int ctl[CTL_WORDS];
     std::vector<PackOut> po(t_end - t0);
     download(c, ctl, c->ctl.p, sizeof(ctl));
@@ -1924,6 +2013,7 @@ int pack_pipelined(spk_ctx* c, const std::vector<HostItem>& bases, const std::ve
       t0 = t_end;
       continue;
     }
+    have_next = false;
     // item `stop` goes through the exact path on the host's orders
     const int h = stop;
     const ItemPlan& pl = plans[h];
@@ -2418,11 +2508,23 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
   const bool feat_masks = c->tray_nonneg && id_offset >= 0 && c->co.ready;
   std::vector<const int*> pre(n_items, nullptr);
   int spec_n = 0;   // incremental spectrum updates in a row, carried across rounds
+  // the next round's screen, enqueued behind a round's items (as in
+  // pack_pipelined)
+  Screen next;
+  bool have_next = false;
+  if (c->prescreen > 0) presize_screens(c, plans, n_items, c->prescreen);
   while (t0 < n_items) {
     const int t_end = c->prescreen > 0 ? std::min(n_items, t0 + c->prescreen) : n_items;
     if (c->prescreen > 0) {
       int od0 = 0;
-      const std::vector<int> live = prescreen(c, plans, t0, t_end, &od0, topk + 1);
+      std::vector<int> live;
+      if (have_next && next.t_begin == t0 && next.t_end == t_end) {
+        live = screen_finish(c, plans, next);
+        od0 = next.od0;
+      } else {
+        live = prescreen(c, plans, t0, t_end, &od0, topk + 1);
+      }
+      have_next = false;
       for (int t = t0; t < t_end; ++t) {
         skip[t] = live[t - t0] == 0 ? 1 : 0;
         propose[t] = live[t - t0] < 0 || live[t - t0] > topk ? 1 : 0;
@@ -2491,6 +2593,10 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       refresh_field_and_spectra(c, nullptr, ctl, &pr);
     }
     lg.run();
+    if (c->prescreen > 0 && t_end < n_items) {
+      next = screen_enqueue(c, plans, t_end, std::min(n_items, t_end + c->prescreen), topk + 1);
+      have_next = next.ran;
+    }
     int cw[CTL_WORDS];
     const int nrem = t_end - t0;
     std::vector<PackOut> po(nrem);
@@ -2537,6 +2643,7 @@ int pack_learned_dev(spk_ctx* c, spk_nn* nn, const std::vector<HostItem>& bases,
       t0 = t_end;
       continue;
     }
+    have_next = false;
     // item `stop` on the exact path: over its pick, then (no proposal
     // feasible) over every fitting orientation
     const int h = stop, r = h - t0;
@@ -2943,6 +3050,7 @@ int spk_ctx_destroy(spk_ctx* c) {
     if (c->mail_h) (void)hipHostFree(c->mail_h);
     if (c->stage_ev) (void)hipEventSynchronize(c->stage_ev);
     if (c->stage) (void)hipHostFree(c->stage);
+    if (c->pre_h) (void)hipHostFree(c->pre_h);
     if (c->stage_ev) (void)hipEventDestroy(c->stage_ev);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;   // the device buffers free themselves, with the device current

@@ -4016,6 +4016,11 @@ k_ffb_unpack(const U128* __restrict__ rb, unsigned char* __restrict__ reached, i
   if (64 + lane < nz) out[64 + lane] = (unsigned char)((r.hi >> lane) & 1ULL);
 }
 
+// dst[0, n) = v (a clear that a captured launch sequence may hold: no memset)
+__global__ void k_fill_int(int* __restrict__ dst, int n, int v) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = v;
+}
+
 __global__ void k_ff_batch_init(int* __restrict__ flags, int k, unsigned char* __restrict__ act,
                                 long long n_act) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
