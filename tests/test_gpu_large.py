@@ -80,6 +80,24 @@ def test_readme_prefix_matches_oracle(P, fx, readme, mode):
     assert sha(r.tray) == f["tray_sha256"] and int((r.tray > 0).sum()) == f["tray_occupied"]
 
 
+@pytest.mark.parametrize("reg_x", [0, 1, 2])
+def test_readme_prefix_pass_a_variants(P, fx, readme, reg_x):
+    """Every pass-A kernel for the 240-point x length (0: LDS FFT, 1/2:
+    registers with three or two LDS buffers) on the readme prefix."""
+    from psacking_amd.rotations import orientation_maps
+    f = fx["readme"]["edt/nonzero"]
+    items = [readme.items[i] for i in f["order"]]
+    perms, flips = orientation_maps(24)
+    s = P.Session(0)
+    s.set_option("reg_x", reg_x)
+    s.set_tray(np.zeros(readme.tray_size, np.int32), "edt", "nonzero", 0)
+    say("readme prefix, reg_x", reg_x)
+    res = s.pack(items, np.arange(len(items), dtype=np.int32), perms, flips, "cube", False, 0)
+    _check_outcomes(f, [bool(r["found"]) for r in res], [r["position"] for r in res],
+                    [r["orientation"] for r in res], [r["score"] for r in res])
+    assert sha(s.tray()) == f["tray_sha256"]
+
+
 def test_uneven_instances_interlocking_free(P, fx):
     from psacking_amd import workloads as W
     from psacking_amd.rotations import orientation_maps
