@@ -116,6 +116,11 @@ enum {
 };
 // search kernels stop once a search has been handed back to the host, and
 // for an item the feasibility pass has already decided
+// A device-coherent read of a word that other workgroups of the running
+// kernel update with atomics (no stale copy from a cache of another XCD).
+__device__ __forceinline__ int load_agent(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ bool search_off(const int* ctl) {
   return ctl && (ctl[CTL_HALT] | ctl[CTL_SKIP_SEARCH]);
 }
@@ -2379,7 +2384,7 @@ k_tile_heights(const M128* __restrict__ tf, int nx, int ny, int* __restrict__ ht
   for (int s = 1; s < 64; s <<= 1) top = max(top, __shfl_xor(top, s, 64));
   if (lane == 0) {
     ht[t] = top;
-    atomicMax(&ht[tx * ty], top);
+    if (top > load_agent(&ht[tx * ty])) atomicMax(&ht[tx * ty], top);   // (it only grows)
   }
 }
 
@@ -2431,9 +2436,13 @@ k_tray_masks_inc(const int* __restrict__ tray, int nx, int ny, int nz, M128* __r
       tf[(size_t)lane * nx * ny + col] = wv;
     }
     if (lane == 0) {
+      // heights only grow, so a value read here (device-coherent) is at most
+      // the current one: a top not above it needs no atomic (the tray top is
+      // one word that every column of the box would otherwise update)
       const int top = m.hi ? 128 - __clzll((long long)m.hi) : (m.lo ? 64 - __clzll((long long)m.lo) : 0);
-      atomicMax(&ht[(x / kSkyTile) * ty + y / kSkyTile], top);
-      atomicMax(&ht[tx * ty], top);
+      int* tile = &ht[(x / kSkyTile) * ty + y / kSkyTile];
+      if (top > load_agent(tile)) atomicMax(tile, top);
+      if (top > load_agent(&ht[tx * ty])) atomicMax(&ht[tx * ty], top);
     }
     return;
   }
@@ -3865,7 +3874,8 @@ k_ff_tile(const unsigned char* __restrict__ fr_all, unsigned char* __restrict__ 
   __syncthreads();
   if (tid == 0) {
     act_cur[tile] = gch ? 1 : 0;
-    if (gch) atomicOr(changed, 1);
+    // (the flag only goes from 0 to 1 here: once it reads 1 no atomic is needed)
+    if (gch && !load_agent(changed)) atomicOr(changed, 1);
   }
 }
 
@@ -3997,7 +4007,8 @@ k_ffb_tile(const U128* __restrict__ fm_all, U128* __restrict__ rb_all, int nx, i
   __syncthreads();
   if (tid == 0) {
     act_cur[tile] = gch ? 1 : 0;
-    if (gch) atomicOr(changed, 1);
+    // (the flag only goes from 0 to 1 here: once it reads 1 no atomic is needed)
+    if (gch && !load_agent(changed)) atomicOr(changed, 1);
   }
 }
 
