@@ -89,6 +89,13 @@ int spk_ctx_destroy(spk_ctx* ctx);
 int spk_tray_set(spk_ctx* ctx, const int32_t* tray, int nx, int ny, int nz,
                  const int32_t* dist, int metric, int sources, uint64_t generation);
 int spk_tray_get(spk_ctx* ctx, int32_t* tray_out, int32_t* dist_out);
+/* Statistics of the tray's distance field as the scored search uses them
+ * (no counterpart in the reference; for tests and diagnostics): the sum,
+ * minimum and maximum of the field, and the largest min(field, distance to
+ * the outside of the grid) over its voxels, the distance to the outside of
+ * voxel (x, y, z) being min(x+1, nx-x, y+1, ny-y, z+1, nz-z).  Any pointer
+ * may be NULL. */
+int spk_tray_stats(spk_ctx* ctx, int64_t* sum, int32_t* mn, int32_t* mx, int32_t* mxw);
 int spk_tray_info(spk_ctx* ctx, int* nx, int* ny, int* nz, uint64_t* generation,
                   int* valid);
 int spk_tray_clear(spk_ctx* ctx);
@@ -282,6 +289,9 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  voxels alone, G1 += e^{-2 pi i k.q / P} FFT(item), at most
  *                  this many times in a row before a full transform (default
  *                  16; 0: always the full transform).
+ *   "inc_split"    blocks of the incremental distance x pass per group of
+ *                  64 (y, z) lines, each over one share of x (1 to 16,
+ *                  default 8).
  *   "score2"       1 (default): the score pass holds both digit slabs in LDS
  *                  at once and transforms them in shared stages (compile-time
  *                  plans, when they fit the device's LDS); 0: one after the

@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libspk.so")
 # every entry point include/spk.h declares
 EXPORTS = (
     "spk_last_error", "spk_version", "spk_ctx_create", "spk_ctx_destroy", "spk_tray_set",
-    "spk_tray_get", "spk_tray_info", "spk_tray_clear", "spk_search", "spk_search_oriented",
+    "spk_tray_get", "spk_tray_stats", "spk_tray_info", "spk_tray_clear", "spk_search", "spk_search_oriented",
     "spk_place", "spk_pack", "spk_pack_gather", "spk_pack_many", "spk_distance", "spk_correlate", "spk_collision_grid",
     "spk_interlock_mask", "spk_place_grid", "spk_bounds", "spk_save_vox", "spk_timing_enable",
     "spk_timing_get", "spk_timing_units", "spk_timing_reset", "spk_set_option", "spk_plan_info",

@@ -379,6 +379,12 @@ class Session {
     check(spk_tray_get(ctx_, nullptr, out.mutable_data()));
     return out;
   }
+  py::tuple field_stats() {
+    int64_t sum = 0;
+    int32_t mn = 0, mx = 0, mxw = 0;
+    check(spk_tray_stats(ctx_, &sum, &mn, &mx, &mxw));
+    return py::make_tuple(sum, mn, mx, mxw);
+  }
   py::list search(py::object item, py::object perms, py::object flips, const std::string& penalty,
                   bool interlocking_free) {
     Grid it = to_grid(item);
@@ -566,6 +572,7 @@ PYBIND11_MODULE(_core, m) {
            py::arg("sources") = "eq1", py::arg("generation") = 0)
       .def("tray", &Session::tray)
       .def("distance", &Session::distance)
+      .def("field_stats", &Session::field_stats)
       .def("search", &Session::search, py::arg("item"), py::arg("perms"), py::arg("flips"),
            py::arg("penalty") = "cube", py::arg("interlocking_free") = false)
       .def("place", &Session::place, py::arg("item"), py::arg("perm"), py::arg("flip"),

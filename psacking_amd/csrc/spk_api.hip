@@ -212,6 +212,7 @@ struct spk_ctx {
   bool graph = false;                    // option "graph": device loops captured as a hipGraph
   bool fixed_fft = true;                 // option "fixed_fft": compile-time-plan score kernels
   int spec_inc = 16;                     // option "spec_inc": incremental G1 updates in a row
+  int inc_split = 8;                     // option "inc_split": k_inc_x blocks per line group
   bool score2 = true;                    // option "score2": score pass with both slabs in LDS
   int lds_max = 0;                       // the device's LDS bytes per block
   int reg_x = 2;                         // option "reg_x": register pass A for px = 240
@@ -753,7 +754,7 @@ void run_distance_inc(spk_ctx* c, const PlaceRef& pr, const int* ctl) {
   launch(c, "inc_zy", zy, dim3(std::min(pr.max_ex, c->nx), kIncSplit), dim3(kDistThreads), sh,
          (const int*)c->tray.p, c->tmpi.p, pr.so, pr.od, pr.orient, pr.qx, pr.qy, pr.qz, pr.id,
          c->nx, c->ny, c->nz, c->sources, big, ctl);
-  const dim3 grid((c->nz + kIncLines - 1) / kIncLines, c->ny, kIncSplit);
+  const dim3 grid((c->nz + kIncLines - 1) / kIncLines, c->ny, c->inc_split);
   const int nb = (int)(grid.x * grid.y * grid.z);
   c->stats_part.ensure(nb);
   auto xk = edt ? k_inc_x<1> : k_inc_x<0>;
@@ -1720,7 +1721,7 @@ void presize_loop(spk_ctx* c, int m, int max_ex) {
     q.frl.ensure((size_t)m * kFineRuns);
     q.nrun.ensure(2 * (size_t)m);
   }
-  c->stats_part.ensure((size_t)((c->nz + kIncLines - 1) / kIncLines) * c->ny * kIncSplit);
+  c->stats_part.ensure((size_t)((c->nz + kIncLines - 1) / kIncLines) * c->ny * kIncXSplitMax);
 }
 
 struct ExactBest {
@@ -3117,6 +3118,20 @@ int spk_tray_get(spk_ctx* c, int32_t* tray_out, int32_t* dist_out) {
   });
 }
 
+int spk_tray_stats(spk_ctx* c, int64_t* sum, int32_t* mn, int32_t* mx, int32_t* mxw) {
+  return guarded(c, [&]() {
+    if (!c->valid) throw SpkError{SPK_ESTATE, "no tray set in this context"};
+    FieldStats f;
+    download(c, &f, c->stats.p, sizeof(f));
+    sync(c);
+    if (sum) *sum = (int64_t)f.sum;
+    if (mn) *mn = f.mn;
+    if (mx) *mx = f.mx;
+    if (mxw) *mxw = f.mxw;
+    return SPK_OK;
+  });
+}
+
 int spk_tray_info(spk_ctx* c, int* nx, int* ny, int* nz, uint64_t* generation, int* valid) {
   if (!c) return fail(SPK_EINVAL, "null context");
   if (nx) *nx = c->nx;
@@ -3680,6 +3695,12 @@ int spk_set_option(spk_ctx* c, const char* name, double value) {
   }
   if (std::string(name) == "score2") {
     c->score2 = value != 0.0;
+    return SPK_OK;
+  }
+  if (std::string(name) == "inc_split") {
+    const int v = (int)value;
+    if (v < 1 || v > kIncXSplitMax) return fail(SPK_EINVAL, "inc_split must be 1 to 16");
+    c->inc_split = v;
     return SPK_OK;
   }
   if (std::string(name) == "reg_x") {

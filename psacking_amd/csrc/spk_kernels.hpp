@@ -567,7 +567,8 @@ k_dist_x(const int* __restrict__ in, int* __restrict__ phi, FieldStats* __restri
 // ---------------------------------------------------------------------------
 constexpr int kIncMaxEx = 128;       // largest x extent of an item updated this way
 constexpr int kIncLines = 64;        // (y, z) lines per k_inc_x block (one per lane)
-constexpr int kIncSplit = 4;         // k_inc_x blocks per line group (shares of x)
+constexpr int kIncSplit = 4;         // k_inc_zy blocks per slab (shares of its y pass)
+constexpr int kIncXSplitMax = 16;    // k_inc_x blocks per line group (shares of x), at most
 
 struct PlacedBox {
   int qx, qy, qz;                    // offset of the oriented item
@@ -662,9 +663,9 @@ k_inc_zy(const int* __restrict__ tray, int* __restrict__ g, const SearchOut* __r
   }
 }
 
-// grid (ceil(nz / kIncLines), ny, kIncSplit), 256 threads: lane l of every
-// wave owns line (y = blockIdx.y, z = z0 + l); wave w evaluates x = x0 + w,
-// x0 + w + 4, ... in the block's share of x (blockIdx.z of kIncSplit).
+// grid (ceil(nz / kIncLines), ny, S), S <= kIncXSplitMax, 256 threads: lane l
+// of every wave owns line (y = blockIdx.y, z = z0 + l); wave w evaluates
+// x = x0 + w, x0 + w + 4, ... in the block's share of x (blockIdx.z of S).
 // Every output is the minimum over the hx+1 candidate slabs (staged in LDS,
 // dynamic: (hx+1) * kIncLines ints); an x whose distance to the candidate
 // range alone reaches the old value cannot lower it and is skipped.
@@ -687,17 +688,18 @@ k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict
     cand[e] = (z0 + l < nz) ? g[((size_t)rr * ny + y) * nz + z0 + l] : big;
   }
   __syncthreads();
+  const int xl = b.qx, xh = b.qx + m - 1;
+  const int share = (nx + (int)gridDim.z - 1) / (int)gridDim.z;
+  const int xa = blockIdx.z * share, xb = min(nx, xa + share);
+  const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  // the smallest candidate of the line (big past nz): every candidate at
+  // distance >= d from x costs at least fmin + dcost(d)
+  int fmin = big;
+  for (int rr = 0; rr < m; ++rr) fmin = min(fmin, cand[rr * kIncLines + lane]);
   long long lsum = 0;
   int lmn = INT_MAX, lmx = INT_MIN, lmw = INT_MIN;
   if (z < nz) {
     const int wy = min(y + 1, ny - y), wz = min(z + 1, nz - z);
-    const int xl = b.qx, xh = b.qx + m - 1;
-    const int share = (nx + kIncSplit - 1) / kIncSplit;
-    const int xa = blockIdx.z * share, xb = min(nx, xa + share);
-    // the smallest candidate of the line: every candidate at distance >= d
-    // from x costs at least fmin + dcost(d)
-    int fmin = big;
-    for (int rr = 0; rr < m; ++rr) fmin = min(fmin, cand[rr * kIncLines + lane]);
 #pragma unroll 4
     for (int x = xa + wave; x < xb; x += 4) {
       int* pv = phi + ((size_t)x * ny + y) * nz + z;
@@ -756,7 +758,7 @@ k_inc_x(const int* __restrict__ g, int* __restrict__ phi, FieldStats* __restrict
       f.mx = rmx[w] > f.mx ? rmx[w] : f.mx;
       f.mxw = rmw[w] > f.mxw ? rmw[w] : f.mxw;
     }
-    part[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = f;
+    part[bid] = f;
   }
 }
 

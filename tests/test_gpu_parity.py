@@ -732,6 +732,7 @@ def test_incremental_distance_matches_oracle(P, metric, sources):
         r = s.pack([w.items[i]], np.zeros(1, np.int32), perms, flips, "cube", False, placed)[0]
         placed += int(r["found"])
         np.testing.assert_array_equal(s.distance(), O.distance_field(s.tray(), metric, sources))
+        assert s.field_stats() == _field_stats(s.distance())
     assert placed >= 3
     say("incremental field, explicit placements", metric, sources)
     t = s.tray()
@@ -745,6 +746,19 @@ def test_incremental_distance_matches_oracle(P, metric, sources):
         t = O.place_in_tray(item, t, res.position, placed)
         np.testing.assert_array_equal(s.tray(), t)
         np.testing.assert_array_equal(s.distance(), O.distance_field(t, metric, sources))
+        assert s.field_stats() == _field_stats(s.distance())
+
+
+def _field_stats(phi):
+    """sum, min, max of a field and the largest min(field, distance to the
+    outside of the grid), the statistics the digit split is sized from."""
+    w = None
+    for a, n in enumerate(phi.shape):
+        i = np.arange(n)
+        d = np.minimum(i + 1, n - i).reshape([-1 if b == a else 1 for b in range(3)])
+        w = d if w is None else np.minimum(w, d)
+    return (int(phi.astype(np.int64).sum()), int(phi.min()), int(phi.max()),
+            int(np.minimum(phi, w).max()))
 
 
 @pytest.mark.parametrize("spec_inc", [0, 1, 16, 1000])
