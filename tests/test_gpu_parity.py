@@ -310,11 +310,13 @@ def test_pack_forced_exact_path(P):
 
 
 @pytest.mark.parametrize("options", [(), (("corr_chunk", 1), ("col_chunks", 3), ("score_radix", 25)),
-                                     (("graph", 1), ("inc_dist", 0))])
+                                     (("graph", 1), ("inc_dist", 0)),
+                                     (("prescreen", 0), ("inc_masks", 0))])
 def test_pack_pipelined_matches_oracle(P, options):
     """Default options; pass A one orientation per block, column-test
     blocks over several offset chunks, radix-25 score pass; the loop captured
-    as a hipGraph, with the full distance transform after every placement."""
+    as a hipGraph, with the full distance transform after every placement;
+    one round with the tray masks rebuilt whole after every placement."""
     from psacking_amd import workloads as W
     w = W.blobs(seed=15, n=10, tray=(64, 48, 40))
     say("pack pipelined", options)
