@@ -178,6 +178,7 @@ struct spk_ctx {
   DevBuf<long long> big;       // standalone full-grid outputs
   DevBuf<unsigned char> ffree, freach;   // IF: feasibility / reached grids [m][V]
   DevBuf<long long> pgrid;               // IF: proximity grids [m][V]
+  DevBuf<int> if_keep;                   // IF: 1 when the unmasked winner is the masked one
   DevBuf<int> exits;                     // IF: high seed faces [m][3]
   DevBuf<unsigned char> ff_act;          // IF: tiles changed in the last round (two buffers)
   DevBuf<U128> fb_free, fb_reach;        // IF, nz <= 128: free / reached bit columns [m][nx][ny]
@@ -957,14 +958,15 @@ WgBest exact_search_one(spk_ctx* c, const HostItem& it, const OrientDesc& d, int
 // The same fixpoint on bit columns (nz <= 128, option "ff_bits"): pack the
 // free offsets and seeds into 128-bit column masks, rounds of k_ffb_tile over
 // 32 x 32-column tiles, then the reached bits back to one byte per offset.
-void flood_fill_bits(spk_ctx* c, int m, int nx, int ny, int nz, bool bytes) {
+void flood_fill_bits(spk_ctx* c, int m, int nx, int ny, int nz, bool bytes,
+                     const int* skip = nullptr) {
   const size_t ncol = (size_t)nx * ny, V = ncol * nz;
   if (bytes) c->freach.ensure((size_t)m * V);
   c->fb_free.ensure((size_t)m * ncol);
   c->fb_reach.ensure((size_t)m * ncol);
   const dim3 cg((unsigned)((ncol + 3) / 4), m);
   launch(c, "ff_seed", k_ffb_pack, cg, dim3(256), 0, (const unsigned char*)c->ffree.p,
-         c->fb_free.p, c->fb_reach.p, (const int*)c->exits.p, nx, ny, nz);
+         c->fb_free.p, c->fb_reach.p, (const int*)c->exits.p, nx, ny, nz, skip);
   const int xt = (nx + FBT - 1) / FBT, yt = (ny + FBT - 1) / FBT;
   const size_t tiles = (size_t)xt * yt * m;
   c->ff_act.ensure(2 * tiles);
@@ -974,7 +976,7 @@ void flood_fill_bits(spk_ctx* c, int m, int nx, int ny, int nz, bool bytes) {
   for (size_t it = 0; it <= (size_t)m * V; it += batch) {
     launch(c, "ff_batch_init", k_ff_batch_init, dim3(round == 0 ? 256 : 1), dim3(256), 0,
            c->ff_flags.p, batch, round == 0 ? c->ff_act.p + (round & 1) * tiles : nullptr,
-           (long long)tiles);
+           (long long)tiles, skip);
     for (int r = 0; r < batch; ++r, ++round)
       launch(c, "ff_tile", k_ffb_tile, dim3(xt * yt, m), dim3(FBT * FBT), 0,
              (const U128*)c->fb_free.p, c->fb_reach.p, nx, ny, xt, yt,
@@ -994,19 +996,20 @@ void flood_fill_bits(spk_ctx* c, int m, int nx, int ny, int nz, bool bytes) {
 
 // Returns true when the bit fill ran: then c->fb_reach holds the result, and
 // c->freach too only when bytes is set.
+// skip (nullable, device): when set, the fill's kernels exit at once
 bool flood_fill(spk_ctx* c, int m, const std::vector<int>& exits, int nx, int ny, int nz,
-                bool bytes = true) {
+                bool bytes = true, const int* skip = nullptr) {
   const size_t V = (size_t)nx * ny * nz;
   c->freach.ensure((size_t)m * V);
   c->exits.ensure((size_t)3 * m);
   upload(c, c->exits.p, exits.data(), exits.size() * sizeof(int));
   if (c->ff_bits && nz <= 128) {
-    flood_fill_bits(c, m, nx, ny, nz, bytes);
+    flood_fill_bits(c, m, nx, ny, nz, bytes, skip);
     return true;
   }
   const int nb = (int)std::max<size_t>(1, std::min<size_t>(2048, (V + 255) / 256));
   launch(c, "ff_seed", k_ff_seed, dim3(nb, m), dim3(256), 0, (const unsigned char*)c->ffree.p,
-         c->freach.p, (const int*)c->exits.p, nx, ny, nz);
+         c->freach.p, (const int*)c->exits.p, nx, ny, nz, skip);
   const int xt = (nx + FFX - 1) / FFX;
   const dim3 grid((nz + FFZ - 1) / FFZ, (ny + FFY - 1) / FFY, xt * m);
   const size_t tiles = (size_t)grid.x * grid.y * grid.z;
@@ -1017,7 +1020,7 @@ bool flood_fill(spk_ctx* c, int m, const std::vector<int>& exits, int nx, int ny
   for (size_t it = 0; it <= (size_t)m * V; it += batch) {
     launch(c, "ff_batch_init", k_ff_batch_init, dim3(round == 0 ? 256 : 1), dim3(256), 0,
            c->ff_flags.p, batch, round == 0 ? c->ff_act.p + (round & 1) * tiles : nullptr,
-           (long long)tiles);
+           (long long)tiles, skip);
     for (int r = 0; r < batch; ++r, ++round)
       launch(c, "ff_tile", k_ff_tile, grid, dim3(FFX * FFY * FFZ), 0,
              (const unsigned char*)c->ffree.p, c->freach.p, nx, ny, nz, xt,
@@ -1034,7 +1037,11 @@ bool flood_fill(spk_ctx* c, int m, const std::vector<int>& exits, int nx, int ny
 }
 
 // IF mode: feasibility grids -> reachable set -> masked argmin records
-void if_masked_select(spk_ctx* c, const std::vector<OrientDesc>& ods, const double* pen) {
+// winner_only: only the winner is wanted (no per-orientation results), so
+// the masked argmin and the selection after it keep the unmasked winner when
+// the fill reached it (c->if_keep = 1)
+void if_masked_select(spk_ctx* c, const std::vector<OrientDesc>& ods, const double* pen,
+                      bool winner_only = false) {
   const int m = (int)ods.size();
   std::vector<int> exits(3 * (size_t)m);
   for (int o = 0; o < m; ++o) {
@@ -1042,13 +1049,20 @@ void if_masked_select(spk_ctx* c, const std::vector<OrientDesc>& ods, const doub
     exits[3 * o + 1] = c->ny - 1 - ods[o].hy;
     exits[3 * o + 2] = c->nz - 1 - ods[o].hz;
   }
-  const bool bits = flood_fill(c, m, exits, c->nx, c->ny, c->nz, false);
+  c->if_keep.ensure(1);
+  if (winner_only)   // a winner reachable on its own needs no fill
+    launch(c, "if_direct", k_if_direct, dim3(1), dim3(64), 0, (const SearchOut*)c->sout.p,
+           (const OrientDesc*)c->od.p, (const unsigned char*)c->ffree.p, c->nx, c->ny, c->nz,
+           c->if_keep.p);
+  const bool bits = flood_fill(c, m, exits, c->nx, c->ny, c->nz, false,
+                               winner_only ? (const int*)c->if_keep.p : (const int*)nullptr);
   // c->per holds each orientation's unmasked summary (residual, range flag)
   c->rec2.ensure((size_t)m * c->nx);
   launch(c, "masked_argmin", k_masked_argmin, dim3(c->nx, m), dim3(256), 0,
          (const unsigned char*)c->freach.p, (const long long*)c->pgrid.p,
          (const OrientDesc*)c->od.p, pen, (const WgBest*)c->per.p, c->rec2.p, c->nx, c->ny,
-         c->nz, bits ? (const U128*)c->fb_reach.p : (const U128*)nullptr);
+         c->nz, bits ? (const U128*)c->fb_reach.p : (const U128*)nullptr,
+         winner_only ? (const SearchOut*)c->sout.p : (const SearchOut*)nullptr, c->if_keep.p);
 }
 
 // records per orientation of the score pass: slab plan one per x slice,
@@ -1280,7 +1294,7 @@ CorrLaunch corr_launch(const spk_ctx* c) {
 // m descriptors whose item spectra are in c->ispec.
 void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
                     const int* ctl, unsigned char* fgrid, long long* pgrid,
-                    const std::vector<OrientDesc>* host_ods) {
+                    const std::vector<OrientDesc>* host_ods, bool winner_only = false) {
   const int py = c->py.n, pz = c->pz.n, hz = pz / 2 + 1;
   const float2* tw = c->tw.p;
   const size_t ysh = (size_t)(py + py * c->Ty) * sizeof(float2);
@@ -1323,13 +1337,14 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
            c->timing && c->pairs.p ? c->pairs.p + 2 : (unsigned long long*)nullptr, ctl);
   }
   launch(c, "select", k_select, dim3(1), dim3(kSelectThreads), 0, (const WgBest*)c->rec.p, od, m,
-         rs, c->split ? c->Rs : 0, c->per.p, c->sout.p, c->ver.p, ctl);
+         rs, c->split ? c->Rs : 0, c->per.p, c->sout.p, c->ver.p, ctl, (const int*)nullptr);
   if (fgrid) {
     // interlocking-free: the summary above feeds the masked argmin, whose
     // per-slice records are selected again
-    if_masked_select(c, *host_ods, pen);
+    if_masked_select(c, *host_ods, pen, winner_only);
     launch(c, "select", k_select, dim3(1), dim3(kSelectThreads), 0, (const WgBest*)c->rec2.p, od,
-           m, c->nx, 0, c->per.p, c->sout.p, c->ver.p, ctl);
+           m, c->nx, 0, c->per.p, c->sout.p, c->ver.p, ctl,
+           winner_only ? (const int*)c->if_keep.p : (const int*)nullptr);
   }
   launch(c, "verify", k_verify, dim3(std::min(512, (max_e + 255) / 256)), dim3(256), 0,
          (const int*)c->tray.p, (const int*)c->phi.p, (const int*)c->items.p, od,
@@ -1338,9 +1353,9 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
 
 void enqueue_search(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max_e, int penalty,
                     const int* ctl, unsigned char* fgrid, long long* pgrid,
-                    const std::vector<OrientDesc>* host_ods) {
+                    const std::vector<OrientDesc>* host_ods, bool winner_only = false) {
   enqueue_item_spectra(c, od, m, max_ex, ctl);
-  enqueue_scored(c, od, m, max_ex, max_e, penalty, ctl, fgrid, pgrid, host_ods);
+  enqueue_scored(c, od, m, max_ex, max_e, penalty, ctl, fgrid, pgrid, host_ods, winner_only);
 }
 
 // items: per entry base item + orientation; all on host.
@@ -1398,9 +1413,10 @@ void do_search(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector
     c->pgrid.ensure((size_t)m * V);
     HIP_OK(hipMemsetAsync(c->ffree.p, 0, (size_t)m * V, c->stream));
   }
+  // (without per-orientation results only the winner is wanted)
   enqueue_search(c, c->od.p, m, max_ex, max_e, penalty, nullptr,
                  interlocking_free ? c->ffree.p : nullptr, interlocking_free ? c->pgrid.p : nullptr,
-                 interlocking_free ? &ods : nullptr);
+                 interlocking_free ? &ods : nullptr, per_orient == nullptr);
   SearchOut so;
   std::vector<WgBest> per(m);
   long long ver[3];
@@ -1440,7 +1456,7 @@ void do_search(spk_ctx* c, const std::vector<HostItem>& bases, const std::vector
     if_masked_select(c, ods, pen);
     launch(c, "select", k_select, dim3(1), dim3(kSelectThreads), 0, (const WgBest*)c->rec2.p,
            (const OrientDesc*)c->od.p, m, c->nx, 0, c->per.p, c->sout.p, c->ver.p,
-           (const int*)nullptr);
+           (const int*)nullptr, (const int*)nullptr);
     SearchOut s2;
     download(c, &s2, c->sout.p, sizeof(s2));
     download(c, per.data(), c->per.p, m * sizeof(WgBest));

@@ -3219,11 +3219,14 @@ __device__ __forceinline__ void select_body(const WgBest* __restrict__ rec,
   }
 }
 
+// keep (nullable): 1 when k_masked_argmin found the unmasked winner to be the
+// masked one (the selection over its records then keeps out as it is)
 __global__ void __launch_bounds__(kSelectThreads)
 k_select(const WgBest* __restrict__ rec, const OrientDesc* __restrict__ od, int n_orient,
          int rec_stride, int rows_per_rec, WgBest* __restrict__ per_orient,
-         SearchOut* __restrict__ out, long long* __restrict__ ver, const int* __restrict__ ctl) {
-  if (search_off(ctl)) return;
+         SearchOut* __restrict__ out, long long* __restrict__ ver, const int* __restrict__ ctl,
+         const int* __restrict__ keep = nullptr) {
+  if (search_off(ctl) || (keep && *keep)) return;
   if (threadIdx.x < 3) ver[threadIdx.x] = 0;   // accumulator of the following k_verify
   select_body(rec, od, n_orient, rec_stride, rows_per_rec, per_orient, out, nullptr);
 }
@@ -3762,7 +3765,8 @@ k_direct_grid(const int* __restrict__ tray, const int* __restrict__ phi,
 // exit faces exits[3o..3o+2] (N-1-h, fft3.cu:276-287).
 __global__ void __launch_bounds__(256)
 k_ff_seed(const unsigned char* __restrict__ fr, unsigned char* __restrict__ reached,
-          const int* __restrict__ exits, int nx, int ny, int nz) {
+          const int* __restrict__ exits, int nx, int ny, int nz, const int* __restrict__ skip) {
+  if (skip && *skip) return;   // (k_if_direct: the fill is not needed)
   const long long V = (long long)nx * ny * nz;
   const int o = blockIdx.y;
   const int ex = exits[3 * o], ey = exits[3 * o + 1], ez = exits[3 * o + 2];
@@ -3920,7 +3924,8 @@ __device__ __forceinline__ U128 fill_runs(U128 m, U128 s) {
 // others.  grid (ceil(nx ny / 4), m), 256 threads.
 __global__ void __launch_bounds__(256)
 k_ffb_pack(const unsigned char* __restrict__ fr, U128* __restrict__ fm, U128* __restrict__ rb,
-           const int* __restrict__ exits, int nx, int ny, int nz) {
+           const int* __restrict__ exits, int nx, int ny, int nz, const int* __restrict__ skip) {
+  if (skip && *skip) return;   // (k_if_direct: the fill is not needed)
   const long long ncol = (long long)nx * ny;
   const long long col = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63, o = blockIdx.y;
@@ -4034,12 +4039,47 @@ __global__ void k_fill_int(int* __restrict__ dst, int n, int v) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = v;
 }
 
+// skip (nullable): when set, the batch's first round already reads "nothing
+// changed", so every round exits at once
 __global__ void k_ff_batch_init(int* __restrict__ flags, int k, unsigned char* __restrict__ act,
-                                long long n_act) {
+                                long long n_act, const int* __restrict__ skip) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i <= k) flags[i] = i == 0 ? 1 : 0;
+  if (i <= k) flags[i] = i == 0 ? ((skip && *skip) ? 0 : 1) : 0;
   if (act)
     for (long long t = i; t < n_act; t += (long long)gridDim.x * blockDim.x) act[t] = 1;
+}
+
+// Interlocking-free search, only the winner wanted: before any fill, *keep =
+// 1 when the unmasked winner is reachable on its own, so that the fill, the
+// masked argmin and the selection after it are skipped.  It is when nothing
+// is free (the masked search finds nothing either), when its offset is a
+// seed of the fill (a free offset in an exit column, x = 0, y = 0, x = ex or
+// y = ey), or when every offset straight below it to z = 0 or straight above
+// it to z = ez is free (a path to a seed; z = 0 and z = ez are exit faces).
+// One wavefront.
+__global__ void k_if_direct(const SearchOut* __restrict__ win, const OrientDesc* __restrict__ od,
+                            const unsigned char* __restrict__ ffree, int nx, int ny, int nz,
+                            int* __restrict__ keep) {
+  const int lane = threadIdx.x;
+  const SearchOut s = *win;
+  if (!s.found) {
+    if (lane == 0) *keep = 1;
+    return;
+  }
+  const OrientDesc d = od[s.orient];
+  const int k = s.idx % nz, j = (s.idx / nz) % ny, x = s.idx / (ny * nz);
+  const int ex = nx - 1 - d.hx, ey = ny - 1 - d.hy, ez = nz - 1 - d.hz;
+  const bool face = x == 0 || j == 0 || x == ex || j == ey;
+  const unsigned char* col = ffree + (size_t)s.orient * nx * ny * nz + ((size_t)x * ny + j) * nz;
+  bool below = true, above = true;
+  for (int z = lane; z < nz; z += 64) {
+    const bool f = col[z] != 0;
+    if (z <= k && !f) below = false;
+    if (z >= k && z <= ez && !f) above = false;
+  }
+  below = __all(below);
+  above = __all(above);
+  if (lane == 0) *keep = (face || below || above) ? 1 : 0;
 }
 
 // grid (nx, m): argmin of double(p) + penalty[k] over reached offsets of slice x
@@ -4049,10 +4089,32 @@ __global__ void __launch_bounds__(256)
 k_masked_argmin(const unsigned char* __restrict__ reached, const long long* __restrict__ pgrid,
                 const OrientDesc* __restrict__ od, const double* __restrict__ pen,
                 const WgBest* __restrict__ per_o, WgBest* __restrict__ rec, int nx, int ny,
-                int nz, const U128* __restrict__ rbits) {
+                int nz, const U128* __restrict__ rbits, const SearchOut* __restrict__ win,
+                int* __restrict__ keep) {
   // rbits (nullable): the reached offsets as the bit fill left them, read in
   // place of the byte grid
   const int x = blockIdx.x, o = blockIdx.y, tid = threadIdx.x;
+  // win (nullable): the unmasked search's winner.  When it is reachable (or
+  // nothing is free at all) it is also the masked winner: every other
+  // orientation's masked best is no better than its unmasked best, which lost
+  // to it, and the winner's own orientation keeps it.  Every block then
+  // returns, and *keep = 1 tells the following selection to keep it.
+  if (win) {
+    if (*keep) return;   // k_if_direct found the winner reachable (no fill ran)
+    const SearchOut s = *win;
+    bool same = !s.found;
+    if (s.found) {
+      const int k = s.idx % nz, j = (s.idx / nz) % ny, xw = s.idx / (ny * nz);
+      if (rbits) {
+        const U128 r = rbits[((size_t)s.orient * nx + xw) * ny + j];
+        same = (((k < 64 ? r.lo : r.hi) >> (k & 63)) & 1ULL) != 0;
+      } else {
+        same = reached[(size_t)s.orient * nx * ny * nz + s.idx] != 0;
+      }
+    }
+    if (x == 0 && o == 0 && tid == 0) *keep = same ? 1 : 0;
+    if (same) return;
+  }
   const OrientDesc d = od[o];
   if (x >= d.nvx) return;
   const size_t base = (size_t)o * nx * ny * nz + (size_t)x * ny * nz;

@@ -176,6 +176,32 @@ def test_search_batch_interlocking_free(P):
     assert got == exp
 
 
+def test_search_interlocking_free_enclosed_winner(P):
+    """Without the interlocking mask the best offset lies in a closed cavity
+    (unreachable): the masked search must rank the reachable offsets (the
+    shortcut that keeps a reachable unmasked winner does not apply); with the
+    cavity opened to the top the two winners coincide."""
+    N = (16, 16, 16)
+    tray = np.zeros(N, np.int32)
+    tray[:, :, 0] = 1                    # a floor (offsets at z = 0 are exits)
+    tray[4:12, 4:12, 1:9] = 1
+    tray[6:10, 6:10, 1:5] = 0            # a 4x4x4 cavity closed on every side
+    item = [np.ones((4, 4, 4), np.int32)]
+    phi = O.calculate_distance(tray)
+    free = O.search_batch(item, tray, phi, penalty="cube", interlocking_free=False)
+    masked = O.search_batch(item, tray, phi, penalty="cube", interlocking_free=True)
+    assert free[0] == (6, 6, 1) and masked[0] != free[0]
+    say("interlocking-free search, enclosed winner")
+    assert P.fft_search_batch_interlocking_free(item, tray, phi, 5) == masked
+    tray[6:10, 6:10, 5:9] = 0            # a shaft from the cavity to the top
+    phi = O.calculate_distance(tray)
+    free = O.search_batch(item, tray, phi, penalty="cube", interlocking_free=False)
+    masked = O.search_batch(item, tray, phi, penalty="cube", interlocking_free=True)
+    assert free == masked
+    say("interlocking-free search, reachable winner")
+    assert P.fft_search_batch_interlocking_free(item, tray, phi, 6) == masked
+
+
 def test_gpu_tray_context(P):
     say("gpu_tray context")
     rng = np.random.default_rng(9)
