@@ -296,11 +296,13 @@ int spk_timing_reset(spk_ctx* ctx);
  *                  at once and transforms them in shared stages (compile-time
  *                  plans, when they fit the device's LDS); 0: one after the
  *                  other.
- *   "reg_x"        pass A when the padded x length is 240 (= 16 * 15): the
- *                  x transforms in registers with one LDS exchange per
- *                  transform, 2 (default) two LDS buffers per block, 1 three
- *                  (two fewer barriers, a third more LDS); 0: the LDS FFT
- *                  kernel used for every other length.
+ *   "reg_x"        pass A for padded x lengths 16 R (R = 6, 8, 9, 10, 12,
+ *                  14, 15, 16; a tray's x length is padded to one when that
+ *                  adds at most an eighth): the x transforms in registers
+ *                  with one LDS exchange per transform, 2 (default) two LDS
+ *                  buffers per block, 1 three for the length 240 (two fewer
+ *                  barriers, a third more LDS); 0: the LDS FFT kernel for
+ *                  every length (set before spk_tray_set).
  *   "feas_group"   orientation pairs per feasibility launch pair (0: all).
  *   "graph"        1: the device-driven loops (spk_pack, spk_pack_gather,
  *                  spk_pack_learned) capture each enqueue of their items as

@@ -515,8 +515,22 @@ XFwdKernel tray_x_kernel(const spk_ctx* c) {
   return k_tray_x_fwd<DynPlan, 0>;
 }
 
+// x lengths of the register pass A (k_corr_xr): 16 R with a compiled R
+bool reg_len(const spk_ctx* c, int n) {
+  if (!c->fixed_fft || c->reg_x <= 0 || n % 16) return false;
+  const int R = n / 16;
+  return R == 6 || R == 8 || R == 9 || R == 10 || R == 12 || R == 14 || R == 15 || R == 16;
+}
+
 void plan_tray(spk_ctx* c) {
-  const int px = next_smooth(c->nx);
+  // the x length: the smallest 2^a 3^b 5^c 7^d >= nx, or the register pass A's
+  // next length when it pads by at most an eighth more
+  int px = next_smooth(c->nx);
+  for (int n = (c->nx + 15) / 16 * 16; n <= 256 && 8 * n <= 9 * px; n += 16)
+    if (reg_len(c, n)) {
+      px = n;
+      break;
+    }
   const int pz = c->pad_z >= c->nz && smooth(c->pad_z) ? c->pad_z : next_smooth(c->nz);
   int py = c->pad_y >= c->ny && smooth(c->pad_y) ? c->pad_y : next_smooth(c->ny);
   if (px > kMaxAxis || py > kMaxAxis || pz > kMaxAxis)
@@ -536,7 +550,7 @@ void plan_tray(spk_ctx* c) {
     return T;
   };
   c->T = tile(px);
-  c->Tc = c->reg_x > 0 && px == 240 ? 4 : c->T;
+  c->Tc = reg_len(c, px) ? 4 : c->T;
   c->Ty = tile(py);
   c->Rf = std::max(1, kRowMax / pz);        // forward rows per block
   c->Rs = std::max(1, kRowMax / (2 * pz));  // score rows per block (two row halves)
@@ -597,12 +611,12 @@ void plan_tray(spk_ctx* c) {
     allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 16>, dyn);
     allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 8>, dyn);
     allow_shmem(k_corr_x_f<FPlan<240, 16, 15>, 4>, dyn);
-    allow_shmem(k_corr_x240<4, 3>, corr_x240_shmem(4, 3));
-    allow_shmem(k_corr_x240<8, 3>, corr_x240_shmem(8, 3));
-    allow_shmem(k_corr_x240<16, 3>, corr_x240_shmem(16, 3));
-    allow_shmem(k_corr_x240<4, 2>, corr_x240_shmem(4, 2));
-    allow_shmem(k_corr_x240<8, 2>, corr_x240_shmem(8, 2));
-    allow_shmem(k_corr_x240<16, 2>, corr_x240_shmem(16, 2));
+    allow_shmem(k_corr_xr<15, 4, 3>, corr_xr_shmem(15, 4, 3));
+    allow_shmem(k_corr_xr<15, 8, 3>, corr_xr_shmem(15, 8, 3));
+    allow_shmem(k_corr_xr<15, 16, 3>, corr_xr_shmem(15, 16, 3));
+    allow_shmem(k_corr_xr<15, 4, 2>, corr_xr_shmem(15, 4, 2));
+    allow_shmem(k_corr_xr<15, 8, 2>, corr_xr_shmem(15, 8, 2));
+    allow_shmem(k_corr_xr<15, 16, 2>, corr_xr_shmem(15, 16, 2));
     allow_shmem(k_tray_x_fwd<DynPlan, 0>, dyn);
     allow_shmem(k_tray_x_fwd<X, 16>, dyn);
     allow_shmem(k_tray_x_fwd<X, 8>, dyn);
@@ -1278,9 +1292,18 @@ CorrLaunch corr_launch(const spk_ctx* c) {
   if (c->fixed_fft && c->reg_x > 0 && c->px.n == 240 && (T == 4 || T == 8 || T == 16)) {
     const int nb = c->reg_x == 2 ? 2 : 3;
     CorrKernel k;
-    if (nb == 3) k = T == 4 ? k_corr_x240<4, 3> : T == 8 ? k_corr_x240<8, 3> : k_corr_x240<16, 3>;
-    else k = T == 4 ? k_corr_x240<4, 2> : T == 8 ? k_corr_x240<8, 2> : k_corr_x240<16, 2>;
-    return {k, 16 * T, corr_x240_shmem(T, nb)};
+    if (nb == 3) k = T == 4 ? k_corr_xr<15, 4, 3> : T == 8 ? k_corr_xr<15, 8, 3> : k_corr_xr<15, 16, 3>;
+    else k = T == 4 ? k_corr_xr<15, 4, 2> : T == 8 ? k_corr_xr<15, 8, 2> : k_corr_xr<15, 16, 2>;
+    return {k, 16 * T, corr_xr_shmem(15, T, nb)};
+  }
+  // other x lengths 16 R: tile 4, two LDS buffers
+  if (reg_len(c, c->px.n) && T == 4) {
+    const int R = c->px.n / 16;
+    CorrKernel k = R == 6 ? k_corr_xr<6, 4, 2> : R == 8 ? k_corr_xr<8, 4, 2> :
+                   R == 9 ? k_corr_xr<9, 4, 2> : R == 10 ? k_corr_xr<10, 4, 2> :
+                   R == 12 ? k_corr_xr<12, 4, 2> : R == 14 ? k_corr_xr<14, 4, 2> :
+                   k_corr_xr<16, 4, 2>;
+    return {k, 64, corr_xr_shmem(R, 4, 2)};
   }
   CorrKernel k = k_corr_x;
   if (c->fixed_fft) {
@@ -2831,10 +2854,10 @@ void autotune_tile(spk_ctx* c) {
         consider(ch);
       }
       apply(best);
-      // 1b. pass A's own tile (the register kernel for px = 240 keeps tile 4,
-      // measured fastest at both 24 and 2-4 orientations per launch)
+      // 1b. pass A's own tile (the register kernel keeps tile 4, measured
+      // fastest for px = 240 at both 24 and 2-4 orientations per launch)
       const TuneChoice base2 = best;
-      const bool reg = c->fixed_fft && c->reg_x > 0 && c->px.n == 240;
+      const bool reg = reg_len(c, c->px.n);
       for (int T : {4, 8, 16}) {
         if (reg) break;
         if (T == base2.Tc || c->px.n * T > kPencilMax || !(T <= 2 * c->pz.n || T == 4)) continue;

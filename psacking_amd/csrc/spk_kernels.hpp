@@ -1242,39 +1242,39 @@ k_corr_x_f(const float2* __restrict__ g1, const float2* __restrict__ g2,
   }
 }
 
-// Pass A for px = 240 = 16 * 15 (the x length of the common trays) with the
-// x transforms in registers: the same products as k_corr_x.  Thread (b, c),
-// b = tid / T < 16, c = tid % T, owns column kz0 + c of the tile.
-//  forward, x = n + 15 r, k = m0 + 16 m1:
-//    X[m0 + 16 m1] = sum_n w15^(n m1) [w240^(n m0) sum_r x[n + 15 r] w16^(r m0)]
-//    step 1 (thread n = b < 15): DFT16 over r, times w240^(n m0), to LDS;
-//    step 2 (thread m0 = b): DFT15 over n, giving X[b + 16 m1], m1 < 15;
-//  the products P = G conj(X) scale at those 15 frequencies, then inverse,
-//  x = x0 + 15 x1:
-//    y[x0 + 15 x1] = sum_b w16^(-b x1) [w240^(-b x0) sum_m1 P[b + 16 m1] w15^(-m1 x0)]
-//    step 3 (thread b): inverse DFT15 over m1, times w240^(-b x0), to LDS;
-//    step 4 (thread x0 = b < 15): inverse DFT16 over b, giving y[x0 + 15 x1].
+// Pass A for px = N = 16 R, R <= 16 (240 = 16 * 15 for the common trays),
+// with the x transforms in registers: the same products as k_corr_x.  Thread
+// (b, c), b = tid / T < 16, c = tid % T, owns column kz0 + c of the tile.
+//  forward, x = n + R r, k = m0 + 16 m1:
+//    X[m0 + 16 m1] = sum_n wR^(n m1) [wN^(n m0) sum_r x[n + R r] w16^(r m0)]
+//    step 1 (thread n = b < R): DFT16 over r, times wN^(n m0), to LDS;
+//    step 2 (thread m0 = b): DFT-R over n, giving X[b + 16 m1], m1 < R;
+//  the products P = G conj(X) scale at those R frequencies, then inverse,
+//  x = x0 + R x1:
+//    y[x0 + R x1] = sum_b w16^(-b x1) [wN^(-b x0) sum_m1 P[b + 16 m1] wR^(-m1 x0)]
+//    step 3 (thread b): inverse DFT-R over m1, times wN^(-b x0), to LDS;
+//    step 4 (thread x0 = b < R): inverse DFT16 over b, giving y[x0 + R x1].
 // With NB = 3 LDS buffers (step 1's output and the two products) a block
 // takes two barriers per orientation; with NB = 2 the low-digit product
 // reuses step 1's buffer, for two more barriers and a third less LDS.  G1 and
-// G2 at the thread's 15 frequencies stay in registers across the chunk's
+// G2 at the thread's R frequencies stay in registers across the chunk's
 // orientations.  LDS rows are padded to 17 T values so that the strided reads
 // of steps 2 and 4 spread over the banks.
 constexpr int kX240Pitch = 17;
-__host__ __device__ constexpr size_t corr_x240_shmem(int T, int NB) {
-  return (size_t)(240 + 16 * kX240Pitch * T + (NB - 1) * 15 * kX240Pitch * T) * sizeof(float2);
+__host__ __device__ constexpr size_t corr_xr_shmem(int R, int T, int NB) {
+  return (size_t)(16 * R + 16 * kX240Pitch * T + (NB - 1) * R * kX240Pitch * T) * sizeof(float2);
 }
 
-// The orientation loop of k_corr_x240; BOTH: the tile has columns with a
+// The orientation loop of k_corr_xr; BOTH: the tile has columns with a
 // low-digit product (kz < hz), else only the first product is formed.  The two
 // cases are separate instantiations so that each keeps its whole loop body
 // free of branches (the scheduler interleaves the two products' work).
-template <int T, int NB, bool BOTH>
-__device__ __forceinline__ void x240_orients(
+template <int R, int T, int NB, bool BOTH>
+__device__ __forceinline__ void xr_orients(
     const float2* __restrict__ ispec, const OrientDesc* __restrict__ od, float2* __restrict__ y1,
     float2* __restrict__ y2, int o0, int o1, bool prune, const int* __restrict__ cf, int max_ex,
     int nx, int rz, int rh, int ky, int pz, int hz, int kz, bool colok, bool has2,
-    const float2 (&G1)[15], const float2 (&G2)[15], const float2* tw, float2* sa, float2* s1,
+    const float2 (&G1)[R], const float2 (&G2)[R], const float2* tw, float2* sa, float2* s1,
     float2* s2, int b, int c) {
   constexpr int PA = kX240Pitch * T;
   for (int o = o0; o < o1; ++o) {
@@ -1284,43 +1284,43 @@ __device__ __forceinline__ void x240_orients(
     // (hoisted out of the loop, 48 of them would hold registers throughout)
     int fo = b * rz + ky * pz + kz, ho = b * rh + ky * hz + kz;
     asm volatile("" : "+v"(fo), "+v"(ho));
-    {   // step 1 (threads b < 15; the barriers stay outside the branch)
+    {   // step 1 (threads b < R; the barriers stay outside the branch)
       const float2* is = ispec + (size_t)o * max_ex * rz;
       float2 v[16];
-      if (b < 15) {
+      if (b < R) {
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
-          const int x = b + 15 * r;
-          v[r] = (x < d.ex && colok) ? is[fo + 15 * r * rz] : make_float2(0.f, 0.f);
+          const int x = b + R * r;
+          v[r] = (x < d.ex && colok) ? is[fo + R * r * rz] : make_float2(0.f, 0.f);
         }
         Bfly<16, true>::run(v);
 #pragma unroll
         for (int m0 = 1; m0 < 16; ++m0) v[m0] = cmul(v[m0], tw[b * m0]);
       }
       if (NB == 2 && BOTH) __syncthreads();   // the previous orientation's step 4 has read sa
-      if (b < 15) {
+      if (b < R) {
 #pragma unroll
         for (int m0 = 0; m0 < 16; ++m0) sa[m0 * PA + b * T + c] = v[m0];
       }
     }
     __syncthreads();
-    float2 X[15];   // step 2
+    float2 X[R];   // step 2
 #pragma unroll
-    for (int n = 0; n < 15; ++n) X[n] = sa[b * PA + n * T + c];
-    Bfly<15, true>::run(X);
+    for (int n = 0; n < R; ++n) X[n] = sa[b * PA + n * T + c];
+    Bfly<R, true>::run(X);
     // step 3
     {
-      float2 P[15];
+      float2 P[R];
 #pragma unroll
-      for (int m1 = 0; m1 < 15; ++m1) P[m1] = cmulc(G1[m1], X[m1]);
-      Bfly<15, false>::run(P);
+      for (int m1 = 0; m1 < R; ++m1) P[m1] = cmulc(G1[m1], X[m1]);
+      Bfly<R, false>::run(P);
       if (BOTH && NB == 3) {
-        float2 Q[15];
+        float2 Q[R];
 #pragma unroll
-        for (int m1 = 0; m1 < 15; ++m1) Q[m1] = cmulc(G2[m1], X[m1]);
-        Bfly<15, false>::run(Q);
+        for (int m1 = 0; m1 < R; ++m1) Q[m1] = cmulc(G2[m1], X[m1]);
+        Bfly<R, false>::run(Q);
 #pragma unroll
-        for (int x0 = 0; x0 < 15; ++x0) {
+        for (int x0 = 0; x0 < R; ++x0) {
           float2 p = P[x0], q = Q[x0];
           if (x0 > 0) {
             float2 w = tw[b * x0];
@@ -1333,7 +1333,7 @@ __device__ __forceinline__ void x240_orients(
         }
       } else {
 #pragma unroll
-        for (int x0 = 0; x0 < 15; ++x0) {
+        for (int x0 = 0; x0 < R; ++x0) {
           float2 p = P[x0];
           if (x0 > 0) {
             float2 w = tw[b * x0];
@@ -1344,11 +1344,11 @@ __device__ __forceinline__ void x240_orients(
         }
         if (BOTH) {   // NB == 2: the second product goes to sa once step 2 has read it
 #pragma unroll
-          for (int m1 = 0; m1 < 15; ++m1) P[m1] = cmulc(G2[m1], X[m1]);
-          Bfly<15, false>::run(P);
+          for (int m1 = 0; m1 < R; ++m1) P[m1] = cmulc(G2[m1], X[m1]);
+          Bfly<R, false>::run(P);
           __syncthreads();
 #pragma unroll
-          for (int x0 = 0; x0 < 15; ++x0) {
+          for (int x0 = 0; x0 < R; ++x0) {
             float2 p = P[x0];
             if (x0 > 0) {
               float2 w = tw[b * x0];
@@ -1361,7 +1361,7 @@ __device__ __forceinline__ void x240_orients(
       }
     }
     __syncthreads();
-    if (b < 15 && colok) {   // step 4
+    if (b < R && colok) {   // step 4
       float2 v[16];
 #pragma unroll
       for (int bb = 0; bb < 16; ++bb) v[bb] = s1[b * PA + bb * T + c];
@@ -1369,8 +1369,8 @@ __device__ __forceinline__ void x240_orients(
       float2* d1 = y1 + (size_t)o * nx * rz;
 #pragma unroll
       for (int x1 = 0; x1 < 16; ++x1) {
-        const int x = b + 15 * x1;
-        if (x < d.nvx) d1[fo + 15 * x1 * rz] = v[x1];
+        const int x = b + R * x1;
+        if (x < d.nvx) d1[fo + R * x1 * rz] = v[x1];
       }
       if (BOTH && has2) {
 #pragma unroll
@@ -1379,17 +1379,17 @@ __device__ __forceinline__ void x240_orients(
         float2* d2 = y2 + (size_t)o * nx * rh;
 #pragma unroll
         for (int x1 = 0; x1 < 16; ++x1) {
-          const int x = b + 15 * x1;
-          if (x < d.nvx) d2[ho + 15 * x1 * rh] = v[x1];
+          const int x = b + R * x1;
+          if (x < d.nvx) d2[ho + R * x1 * rh] = v[x1];
         }
       }
     }
   }
 }
 
-template <int T, int NB>
+template <int R, int T, int NB>
 __global__ void __launch_bounds__(16 * T)
-k_corr_x240(const float2* __restrict__ g1, const float2* __restrict__ g2,
+k_corr_xr(const float2* __restrict__ g1, const float2* __restrict__ g2,
             const float2* __restrict__ ispec, const OrientDesc* __restrict__ od,
             float2* __restrict__ y1, float2* __restrict__ y2, int n_orient, int chunk,
             int max_ex, int nx, AxisPlan px_, int py, int pz, int T_, float scale,
@@ -1411,10 +1411,10 @@ k_corr_x240(const float2* __restrict__ g1, const float2* __restrict__ g2,
   if (!any) return;
   extern __shared__ float2 smem[];
   float2* tw = smem;
-  float2* sa = tw + 240;
+  float2* sa = tw + 16 * R;
   float2* s1 = sa + 16 * PA;
-  float2* s2 = NB == 3 ? s1 + 15 * PA : sa;
-  for (int q = tid; q < 240; q += NT) tw[q] = tw_x[q];
+  float2* s2 = NB == 3 ? s1 + R * PA : sa;
+  for (int q = tid; q < 16 * R; q += NT) tw[q] = tw_x[q];
   const int b = tid / T, c = tid - b * T, kz = kz0 + c;
   const int hz = pz / 2 + 1;
   const bool colok = kz < pz, has2 = kz < hz;
@@ -1422,9 +1422,9 @@ k_corr_x240(const float2* __restrict__ g1, const float2* __restrict__ g2,
   // full (pz) and half (hz) spectra are py * pz and py * hz elements apart
   const int rz = py * pz, rh = py * hz;
   // the tray spectra with the inverse transform's 1 / (px py pz) folded in
-  float2 G1[15], G2[15];
+  float2 G1[R], G2[R];
 #pragma unroll
-  for (int m1 = 0; m1 < 15; ++m1) {
+  for (int m1 = 0; m1 < R; ++m1) {
     const int k = b + 16 * m1;
     G1[m1] = colok ? cscale(g1[k * rz + ky * pz + kz], scale) : make_float2(0.f, 0.f);
     G2[m1] = has2 ? cscale(g2[k * rh + ky * hz + kz], scale) : make_float2(0.f, 0.f);
@@ -1432,10 +1432,10 @@ k_corr_x240(const float2* __restrict__ g1, const float2* __restrict__ g2,
   __syncthreads();   // twiddle table
   // every wave holds all T columns of the tile, so this branch is uniform
   if (kz0 < hz)
-    x240_orients<T, NB, true>(ispec, od, y1, y2, o0, o1, prune, cf, max_ex, nx, rz, rh, ky, pz,
+    xr_orients<R, T, NB, true>(ispec, od, y1, y2, o0, o1, prune, cf, max_ex, nx, rz, rh, ky, pz,
                               hz, kz, colok, has2, G1, G2, tw, sa, s1, s2, b, c);
   else
-    x240_orients<T, NB, false>(ispec, od, y1, y2, o0, o1, prune, cf, max_ex, nx, rz, rh, ky, pz,
+    xr_orients<R, T, NB, false>(ispec, od, y1, y2, o0, o1, prune, cf, max_ex, nx, rz, rh, ky, pz,
                                hz, kz, colok, has2, G1, G2, tw, sa, s1, s2, b, c);
 }
 

@@ -264,6 +264,30 @@ def _compare_pack(P, w, n_orient, interlocking_free=False, metric="l1", sources=
     assert got.density == exp.density
 
 
+@pytest.mark.parametrize("nx, px", [(96, 96), (123, 128), (140, 144), (160, 160), (180, 192),
+                                     (200, 224), (250, 256)])
+def test_register_pass_a_lengths(P, nx, px):
+    """Pass A in registers for x lengths 16 R (R = 6 .. 16): the tray's x
+    length is padded to px (at most an eighth past the smallest 2^a 3^b 5^c
+    7^d length), and the pipelined loop places as the oracle does."""
+    from psacking_amd import workloads as W
+    from psacking_amd.rotations import orientation_maps
+    w = W.blobs(seed=30 + nx % 17, n=5, tray=(nx, 22, 18))
+    s = P.Session(0)
+    s.set_tray(np.zeros(w.tray_size, np.int32), "edt", "nonzero", 0)
+    assert s.plan()[0] == px
+    say("register pass A, x length", px)
+    s, res, exp, order = _session_pack(P, w, 24, metric="edt", sources="nonzero")
+    by_item = {p.item_index: p for p in exp.placements}
+    for t, r in enumerate(res):
+        e = by_item[order[t]]
+        assert r["found"] == e.success
+        if e.success:
+            assert (tuple(r["position"]), r["orientation"], r["score"]) == \
+                (e.position, e.orientation_index, e.score)
+    np.testing.assert_array_equal(s.tray(), exp.tray)
+
+
 def test_pack_boxes_64(P):
     from psacking_amd import workloads as W
     _compare_pack(P, W.boxes(), 1)
