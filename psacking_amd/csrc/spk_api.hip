@@ -553,7 +553,8 @@ void plan_tray(spk_ctx* c) {
   c->Tc = reg_len(c, px) ? 4 : c->T;
   c->Ty = tile(py);
   c->Rf = std::max(1, kRowMax / pz);        // forward rows per block
-  c->Rs = std::max(1, kRowMax / (2 * pz));  // score rows per block (two row halves)
+  // score rows per block: R complex rows and ceil(R / 2) paired low-digit rows
+  c->Rs = std::max(1, 2 * (kRowMax / pz) / 3);
   // twiddle tables exp(-2 pi i q / n), double -> float
   std::vector<float2> tw(px + py + pz);
   auto fill = [&](int off, int n) {
@@ -1345,7 +1346,7 @@ void enqueue_scored(spk_ctx* c, const OrientDesc* od, int m, int max_ex, int max
            dim3(kPencilThreads), ysh, c->y2.p, c->py, hz, c->Ty, py, py, tw + c->px.n, od, c->nx,
            (int)PENCIL_SCORE, ctl, 0);
     launch(c, "rows_score", k_rows_score, dim3(rs, m), dim3(kRowThreads),
-           (size_t)(pz + 2 * c->Rs * pz) * sizeof(float2), (const float2*)c->y1.p,
+           (size_t)(pz + (c->Rs + (c->Rs + 1) / 2) * pz) * sizeof(float2), (const float2*)c->y1.p,
            (const float2*)c->y2.p, od, (const FieldStats*)c->stats.p, pen, c->rec.p, rs, c->nx,
            c->ny, c->nz, py, c->pz, c->Rs, tw + c->px.n + py, fgrid, pgrid, ctl);
   } else {

@@ -3627,30 +3627,37 @@ k_rows_score(const float2* __restrict__ y1, const float2* __restrict__ y2,
   const int r0 = blockIdx.x * R;
   if (r0 >= nrows) return;
   const int rows = min(R, nrows - r0);
+  // LDS rows [0, Rh): the low digit's rows paired, Z = X_2q + i X_2q+1 over
+  // the full z spectrum (X at pz - k is conj X at k), so that the inverse
+  // transform of row q holds row 2q's low-digit sums in its real part and row
+  // 2q+1's in its imaginary part; rows [Rh, Rh + R): (c, h)
+  const int Rh = (R + 1) / 2;
   float2* tw = smem;
-  float2* buf = tw + pz;              // rows [0, R): low digit, [R, 2R): (c, h)
+  float2* buf = tw + pz;
   load_table(tw, tw_z, pz, tid, kRowThreads);
   const size_t so = (size_t)o * nx;
-  for (int e = tid; e < rows * hz; e += kRowThreads) {
-    const int r = e / hz, k = e - r * hz;
-    const int rr = r0 + r, x = rr / d.nvy, j = rr - x * d.nvy;
-    buf[r * pz + k] = y2[((so + x) * py + j) * hz + k];
+  const int nq = (rows + 1) / 2;
+  for (int e = tid; e < nq * pz; e += kRowThreads) {
+    const int q = e / pz, k = e - q * pz;
+    const int kk = k < hz ? k : pz - k;
+    const int ra = r0 + 2 * q, xa = ra / d.nvy, ja = ra - xa * d.nvy;
+    const float2 a = y2[((so + xa) * py + ja) * hz + kk];
+    float2 b = make_float2(0.f, 0.f);
+    if (2 * q + 1 < rows) {
+      const int rb = ra + 1, xb = rb / d.nvy, jb = rb - xb * d.nvy;
+      b = y2[((so + xb) * py + jb) * hz + kk];
+    }
+    buf[q * pz + k] = k < hz ? make_float2(a.x - b.y, a.y + b.x)    // a + i b
+                             : make_float2(a.x + b.y, b.x - a.y);   // conj(a) + i conj(b)
   }
   for (int e = tid; e < rows * pz; e += kRowThreads) {
     const int r = e / pz, k = e - r * pz;
     const int rr = r0 + r, x = rr / d.nvy, j = rr - x * d.nvy;
-    buf[(R + r) * pz + k] = y1[((so + x) * py + j) * pz + k];
+    buf[(Rh + r) * pz + k] = y1[((so + x) * py + j) * pz + k];
   }
+  // rows past nq and past Rh + rows hold stale data and are not read
   __syncthreads();
-  for (int e = tid; e < rows * (pz - hz); e += kRowThreads) {
-    const int r = e / (pz - hz), k = hz + (e - r * (pz - hz));
-    const float2 v = buf[r * pz + (pz - k)];
-    buf[r * pz + k] = make_float2(v.x, -v.y);
-  }
-  // rows [rows, R) of the low-digit half stay unused; transform 2R rows so
-  // both halves line up (the unused ones hold stale data and are not read)
-  __syncthreads();
-  lds_fft<false, kRowThreads, kRowMaxEl>(buf, pz_, 1, 2 * R, pz, true, tw, tid);
+  lds_fft<false, kRowThreads, kRowMaxEl>(buf, pz_, 1, Rh + R, pz, true, tw, tid);
   const FieldParams fp = field_params(st, (long long)nx * ny * nz);
   const long long b = 1LL << fp.lb;
   const long long addback = (b / 2 + fp.mu) * d.isum;
@@ -3661,8 +3668,9 @@ k_rows_score(const float2* __restrict__ y1, const float2* __restrict__ y2,
   for (int e = tid; e < rows * d.nvz; e += kRowThreads) {
     const int r = e / d.nvz, k = e - r * d.nvz;
     const int rr = r0 + r, x = rr / d.nvy, j = rr - x * d.nvy;
-    const float l = buf[r * pz + k].x;
-    const float2 v = buf[(R + r) * pz + k];
+    const float2 lz = buf[(r >> 1) * pz + k];
+    const float l = (r & 1) ? lz.y : lz.x;
+    const float2 v = buf[(Rh + r) * pz + k];
     const float lr = rintf(l), c = rintf(v.x), h = rintf(v.y);
     res = fmaxf(res, fmaxf(fabsf(l - lr), fmaxf(fabsf(v.x - c), fabsf(v.y - h))));
     const long long p = b * (long long)h + (long long)lr + addback;
