@@ -1260,9 +1260,9 @@ k_corr_x_f(const float2* __restrict__ g1, const float2* __restrict__ g2,
 // G2 at the thread's R frequencies stay in registers across the chunk's
 // orientations.  LDS rows are padded to 17 T values so that the strided reads
 // of steps 2 and 4 spread over the banks.
-constexpr int kX240Pitch = 17;
+constexpr int kXrPitch = 17;
 __host__ __device__ constexpr size_t corr_xr_shmem(int R, int T, int NB) {
-  return (size_t)(16 * R + 16 * kX240Pitch * T + (NB - 1) * R * kX240Pitch * T) * sizeof(float2);
+  return (size_t)(16 * R + 16 * kXrPitch * T + (NB - 1) * R * kXrPitch * T) * sizeof(float2);
 }
 
 // The orientation loop of k_corr_xr; BOTH: the tile has columns with a
@@ -1276,7 +1276,7 @@ __device__ __forceinline__ void xr_orients(
     int nx, int rz, int rh, int ky, int pz, int hz, int kz, bool colok, bool has2,
     const float2 (&G1)[R], const float2 (&G2)[R], const float2* tw, float2* sa, float2* s1,
     float2* s2, int b, int c) {
-  constexpr int PA = kX240Pitch * T;
+  constexpr int PA = kXrPitch * T;
   for (int o = o0; o < o1; ++o) {
     if (prune && !cf[o]) continue;
     const OrientDesc d = od[o];
@@ -1395,7 +1395,7 @@ k_corr_xr(const float2* __restrict__ g1, const float2* __restrict__ g2,
             int max_ex, int nx, AxisPlan px_, int py, int pz, int T_, float scale,
             const float2* __restrict__ tw_x, const int* __restrict__ cf,
             const int* __restrict__ ctl) {
-  constexpr int NT = 16 * T, PA = kX240Pitch * T;
+  constexpr int NT = 16 * T, PA = kXrPitch * T;
   if (search_off(ctl)) return;
   const bool prune = cf && ctl && ctl[CTL_PRUNE];
   // rank = (ky * tiles + tile) * chunks + chunk, as in k_corr_x_f
